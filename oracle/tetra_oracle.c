@@ -1,0 +1,1074 @@
+/*
+ * ORACLE -- test infrastructure only.  Nothing in the product path may call,
+ * link or load this file; only tests/, the smoke check and the CPU-baseline
+ * leg of bench.py use it, as the checker.
+ *
+ * Plain-C restatement of PyGW's Matsubara-axis linear-tetrahedron
+ * polarization weights (the `bz_calcqdepw_par2` / `_par3` path), branch for
+ * branch, in Hartree units.  Every function cites the reference routine it
+ * follows.  Differences from the reference:
+ *   - Fortran `stop` becomes a negative status code (see ORC_E_* below).
+ *   - `sortnodes` nnod=5 "six triangles" case: the reference resets its
+ *     corner counters inside the node loop (for_tetrahedra2.f90:2422-2423),
+ *     which leaves pentahedron corners 3..5 uninitialised (undefined
+ *     behaviour).  Here the counters are initialised once, which gives the
+ *     geometrically correct bipyramid split (apexes at positions 1 and 5,
+ *     equator at 2..4); the case is reported through the ORC_F_F7A flag.
+ *   - `check_weight` warnings are counted instead of printed.
+ * Compiled with -ffp-contract=off so that every product and sum rounds as in
+ * a strict-IEEE build of the Fortran.
+ *
+ * Parity pins (tests/test_oracle_tetra.py): bitwise/near-bitwise against the
+ * reference Fortran compiled from its own sources into oracle/_ref (when
+ * present), the flat-band closed form, the corner-sum rule, an independent
+ * cubature on the unit tetrahedron, and an independent half-space clipping of
+ * the metal polyhedron.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Fortran default-real literals widened to real(8). */
+#define F32(x) ((double)(float)(x))
+
+enum {
+  ORC_OK = 0,
+  ORC_E_SORTEQ = -1,     /* sorteq: class not found       (for_tetrahedra2.f90:1779-1783) */
+  ORC_E_IMAGCLASS = -2,  /* stweight_imag: bad equiv_flag (for_tetrahedra3.f90:686-690)   */
+  ORC_E_NNOD = -3,       /* convw1t: nnod > 8              (for_tetrahedra3.f90:120-123)   */
+  ORC_E_NTYPE = -4,      /* relnodes: ntype out of range   (for_tetrahedra2.f90:2248-2250) */
+  ORC_E_SORTNODES = -5,  /* sortnodes: no decomposition    (for_tetrahedra2.f90:2515,2649,2683,2770) */
+  ORC_E_ARG = -10
+};
+/* flag bits reported through *flags */
+#define ORC_F_F7A 1      /* nnod=5 bipyramid: reference behaviour undefined */
+#define ORC_F_ZEROVOL 2  /* a sub-tetrahedron had |vol| < ztol_vol (info=1) */
+
+/* ------------------------------------------------------------------------- */
+/* sorteq  (for_tetrahedra2.f90:1688-1798)                                    */
+/* ------------------------------------------------------------------------- */
+static double f_rdist(double v1, double v2) {
+  if ((fabs(v1) + fabs(v2)) < F32(1.0e-10)) return 0.0;
+  return 2.0 * fabs(v1 - v2) / (fabs(v1) + fabs(v2));
+}
+
+/* v: in/out (reordered); index: 0-based original positions; returns status */
+int orc_sorteq(double v[4], int index[4], int *sigeq) {
+  const double ztol = F32(1.e-2);
+  int sig[5] = {0, 0, 0, 0, 0};
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      if (f_rdist(v[i], v[j]) < ztol) sig[i]++;
+  sig[4] = sig[0] + sig[1] + sig[2] + sig[3];
+  int mx = sig[0], mn = sig[0];
+  for (int i = 1; i < 4; i++) { if (sig[i] > mx) mx = sig[i]; if (sig[i] < mn) mn = sig[i]; }
+  if (sig[4] == 8 && mx == 3) {
+    for (int i = 0; i < 4; i++) if (sig[i] == 2) sig[i] += 1;
+    sig[4] = sig[0] + sig[1] + sig[2] + sig[3];
+  }
+  mn = sig[0];
+  for (int i = 1; i < 4; i++) if (sig[i] < mn) mn = sig[i];
+  if (sig[4] == 10 && mn == 2) {
+    for (int i = 0; i < 4; i++) sig[i] = 4;
+    sig[4] = 16;
+  }
+  if (sig[4] == 12) sig[4] = 16;
+  if (sig[4] == 14) sig[4] = 16;
+  for (int i = 0; i < 4; i++) index[i] = i;
+#define SWAP3(a, b) do { double vt = v[a]; v[a] = v[b]; v[b] = vt;          \
+                         int it = index[a]; index[a] = index[b]; index[b] = it; \
+                         int st = sig[a]; sig[a] = sig[b]; sig[b] = st; } while (0)
+  switch (sig[4]) {
+  case 16: break;
+  case 10:
+    for (int i = 0; i < 3; i++) if (sig[i] == 1) SWAP3(i, 3);
+    break;
+  case 8:
+    for (int i = 2; i < 4; i++)
+      if (f_rdist(v[i], v[0]) < f_rdist(v[1], v[0])) SWAP3(i, 1);
+    break;
+  case 6: {
+    int j = 0;
+    for (int i = 0; i < 4; i++)
+      if (sig[i] == 2) { SWAP3(j, i); j++; }
+    break;
+  }
+  case 4: break;
+  default: return ORC_E_SORTEQ;
+  }
+#undef SWAP3
+  *sigeq = sig[4];
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* stweight_itaylor_par  (for_tetrahedra3.f90:407-437)                        */
+/* ------------------------------------------------------------------------- */
+void orc_stweight_itaylor(const double d[4], double omeg, double w[4]) {
+  double omt2 = omeg * omeg;
+  double omt4 = omt2 * omt2;
+  double denom1 = 6.0e1 * omt2;
+  double denom3 = 4.2e2 * omt4;
+  for (int iv = 0; iv < 4; iv++) {
+    double ev[4];
+    for (int j = 0; j < 4; j++) ev[j] = d[(j + iv) % 4];
+    double w01 = -(2.0 * ev[0] + ev[1] + ev[2] + ev[3]) / denom1;
+    double n03 = 4.0 * (ev[0] * ev[0] * ev[0]) + 3.0 * (ev[0] * ev[0]) * (ev[1] + ev[2] + ev[3]);
+    n03 = n03 + 2.0 * ev[0] * (ev[1] * ev[1] + ev[2] * ev[2] + ev[3] * ev[3] + ev[1] * ev[2] + ev[1] * ev[3] + ev[2] * ev[3]);
+    n03 = n03 + (ev[1] * ev[1] * ev[1]) + (ev[2] * ev[2] * ev[2]) + (ev[3] * ev[3] * ev[3])
+              + (ev[1] * ev[1]) * ev[2] + (ev[1] * ev[1]) * ev[3] + (ev[2] * ev[2]) * ev[3]
+              + ev[1] * (ev[2] * ev[2]) + ev[1] * (ev[3] * ev[3]) + ev[2] * (ev[3] * ev[3])
+              + ev[1] * ev[2] * ev[3];
+    double w03 = n03 / denom3;
+    w[iv] = w01 + w03;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* stweight_imag_par  (for_tetrahedra3.f90:439-743)                           */
+/* ------------------------------------------------------------------------- */
+/* 1: reproduce the reference's case-4 vertex rotation (for_tetrahedra3.f90:482-486) */
+int orc_compat_f7b = 1;
+/* transcendental perturbation for sensitivity estimates (0 = off): atan and
+ * log results are scaled by (1 +/- k*2^-52), the sign and k in {0,1} taken
+ * from a hash of the argument bits and this seed. */
+unsigned long long orc_perturb_seed = 0;
+
+static double perturb(double x, double arg) {
+  if (!orc_perturb_seed) return x;
+  union { double d; unsigned long long u; } a = {arg};
+  unsigned long long z = a.u ^ (orc_perturb_seed * 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  int k = (int)(z & 3u);              /* 0,1,2,3 -> -1,0,0,+1 ulp-ish */
+  double s = (k == 0) ? -1.0 : (k == 3 ? 1.0 : 0.0);
+  return x * (1.0 + s * 2.220446049250313e-16);
+}
+
+static void tans_logs(double tans[4], double logs[4], const double ev[4], double omeg) {
+  for (int i = 0; i < 4; i++) {
+    if (omeg > F32(1.0e-20)) tans[i] = perturb(atan(ev[i] / omeg), ev[i] / omeg);
+    else tans[i] = copysign(1.0, ev[i]) * 2.0 * atan(1.0);
+    if ((omeg * omeg + ev[i] * ev[i]) > 1.0e-10) logs[i] = perturb(log(ev[i] * ev[i] + omeg * omeg), -(ev[i] * ev[i] + omeg * omeg));
+    else logs[i] = 0.0;
+  }
+}
+
+static void check_weight(double *w, double omeg, int *nwarn) {
+  if (fabs(*w) > 1.0e4) {
+    if (fabs(omeg) < F32(1.e-6)) *w = 0.0;
+    else (*nwarn)++;
+  }
+}
+
+static inline double p2(double x) { return x * x; }
+static inline double p3(double x) { return (x * x) * x; }
+static inline double p4(double x) { return ((x * x) * x) * x; }
+
+/* vdif(i,j) = ev(i)-ev(j), 1-based in the formulas below via V(i,j) */
+#define V(i, j) (ev[(i) - 1] - ev[(j) - 1])
+#define E(i) ev[(i) - 1]
+
+int orc_stweight_imag(double *wv /* [nom][4] */, const double d[4], const double *omega,
+                      int eqflag, const unsigned char *qtaylor, int nom, int *nwarn) {
+  double ev[4], tans[4], logs[4], evt[9], evw[9];
+  for (int iom = 0; iom < nom; iom++) for (int i = 0; i < 4; i++) wv[4 * iom + i] = 0.0;
+  switch (eqflag) {
+  case 4:
+    for (int iv = 0; iv < 4; iv++) {
+      /* Reference: ev(k) = deltae_vert(j), k = mod(j+ivert-2,4)+1.  That puts
+       * deltae_vert(6-ivert) (not deltae_vert(ivert)) in ev(1) for ivert = 2, 4,
+       * and the closed form is the weight of ev(1): vertex weights 2 and 4 come
+       * out exchanged.  orc_compat_f7b = 0 rotates as stweight_itaylor_par does
+       * (ev(j) = deltae_vert(k)), which gives the exact integral. */
+      if (orc_compat_f7b) for (int j = 0; j < 4; j++) ev[(j + iv) % 4] = d[j];
+      else for (int j = 0; j < 4; j++) ev[j] = d[(j + iv) % 4];
+      double ev3t4 = E(3) * E(4);
+      double ev1_2 = E(1) * E(1);
+      double ev_2[4] = {3.0 * ev1_2, 3.0 * p2(E(2)), 3.0 * p2(E(3)), 3.0 * p2(E(4))};
+      double vdf24 = V(2, 3) * V(2, 4) * V(3, 4);
+      evw[0] = 2.0 * V(1, 2) * (V(1, 3) * V(1, 4)) * vdf24;
+      evt[0] = -ev1_2 * evw[0];
+      evw[1] = vdf24 * 2.0 * ((2.0 * E(1) - E(2)) * (E(3) + E(4)) + E(1) * (2.0 * E(2) - 3.0 * E(1)) - E(3) * E(4));
+      evt[1] = vdf24 * 6.0 * E(1) * (2.0 * E(2) * ev3t4 + E(1) * (ev1_2 - ev3t4 - E(2) * (E(3) + E(4))));
+      evw[2] = 2.0 * p2(V(1, 3) * V(1, 4)) * V(3, 4);
+      evt[2] = -ev_2[1] * evw[2];
+      evw[3] = -2.0 * p2(V(1, 2) * V(1, 4)) * V(2, 4);
+      evt[3] = -ev_2[2] * evw[3];
+      evw[4] = 2.0 * p2(V(1, 2) * V(1, 3)) * V(2, 3);
+      evt[4] = -ev_2[3] * evw[4];
+      evt[5] = vdf24 * ev1_2 * (3.0 * E(2) * ev3t4 + E(1) * (E(2) * (E(1) - 2.0 * E(3)) + E(4) * (E(1) - 2.0 * E(2)) + E(3) * (E(1) - 2.0 * E(4))));
+      evw[5] = -3.0 * vdf24 * (E(2) * E(3) * E(4) + ev1_2 * (2.0 * E(1) - E(2) - E(3) - E(4)));
+      double evdf2 = E(2) * p2(V(1, 3) * V(1, 4)) * V(3, 4) / 3.0;
+      double evdf3 = E(3) * p2(V(1, 2) * V(1, 4)) * V(2, 4) / 3.0;
+      double evdf4 = E(4) * p2(V(1, 2) * V(1, 3)) * V(2, 3) / 3.0;
+      evt[6] = -ev_2[1] * evdf2;
+      evw[6] = 9.0 * evdf2;
+      evw[7] = -9.0 * evdf3;
+      evt[7] = ev_2[2] * evdf3;
+      evw[8] = 9.0 * evdf4;
+      evt[8] = -ev_2[3] * evdf4;
+      double cc = 6.0 * p2(V(1, 2) * V(1, 3) * V(1, 4)) * V(2, 3) * V(2, 4) * V(3, 4);
+      for (int iom = 0; iom < nom; iom++) {
+        if (qtaylor[iom]) continue;
+        double omeg = omega[iom];
+        tans_logs(tans, logs, ev, omeg);
+        double omeg2 = p2(omeg);
+        double aa = evt[0] + omeg2 * evw[0] + omeg * (tans[0] * evt[1] + tans[1] * evt[2] + tans[2] * evt[3] + tans[3] * evt[4]
+                    + omeg2 * (tans[0] * evw[1] + tans[1] * evw[2] + tans[2] * evw[3] + tans[3] * evw[4]));
+        double bb = logs[0] * evt[5] + logs[1] * evt[6] + logs[2] * evt[7] + logs[3] * evt[8]
+                    + omeg2 * (logs[0] * evw[5] + logs[1] * evw[6] + logs[2] * evw[7] + logs[3] * evw[8]);
+        double w = (aa + bb) / cc;
+        check_weight(&w, omeg, nwarn);
+        wv[4 * iom + iv] = w;
+      }
+    }
+    break;
+  case 6:
+    for (int iv = 0; iv < 2; iv++) {
+      ev[0] = d[iv]; ev[1] = ev[0]; ev[2] = d[2]; ev[3] = d[3];
+      double evf = V(3, 1) * V(1, 4) * V(3, 4);
+      evw[0] = (2.0 * E(1) - E(3) - E(4)) * evf * 2.0;
+      evt[0] = E(1) * (5.0 * E(3) * E(4) + E(1) * (E(1) - 3.0 * E(3) - 3.0 * E(4))) * evf;
+      evt[1] = -6.0 * (p2(E(1)) * (E(1) * E(3) + E(1) * E(4) - 3.0 * E(3) * E(4)) + p2(E(3) * E(4))) * V(3, 4);
+      evw[1] = 2.0 * (3.0 * E(1) * (E(1) - E(3) - E(4)) + p2(E(3)) + E(3) * E(4) + p2(E(4))) * V(3, 4);
+      evw[2] = -2.0 * p3(V(1, 4));
+      evt[2] = -3.0 * p2(E(3)) * evw[2];
+      evw[3] = 2.0 * p3(V(1, 3));
+      evt[3] = -3.0 * p2(E(4)) * evw[3];
+      evt[4] = -V(3, 4) * E(1) * (3.0 * E(3) * E(4) * (E(1) * (E(1) - E(3) - E(4)) + E(3) * E(4)) + p2(E(1)) * p2(E(3) - E(4)));
+      evw[4] = -3.0 * V(3, 4) * (E(3) * E(4) * (3.0 * E(1) - E(3) - E(4)) - p3(E(1)));
+      evf = E(3) * p3(V(1, 4));
+      evw[5] = -3.0 * evf;
+      evt[5] = p2(E(3)) * evf;
+      evf = E(4) * p3(V(1, 3));
+      evw[6] = 3.0 * evf;
+      evt[6] = -p2(E(4)) * evf;
+      double cc = 6.0 * p3(V(1, 3)) * p3(V(1, 4)) * V(3, 4);
+      for (int iom = 0; iom < nom; iom++) {
+        if (qtaylor[iom]) continue;
+        double omeg = omega[iom];
+        tans_logs(tans, logs, ev, omeg);
+        double omeg2 = p2(omeg);
+        double aa = evt[0] + omeg2 * evw[0] + omeg * (tans[0] * evt[1] + tans[2] * evt[2] + tans[3] * evt[3]
+                    + omeg2 * (tans[0] * evw[1] + tans[2] * evw[2] + tans[3] * evw[3]));
+        double bb = logs[0] * evt[4] + logs[2] * evt[5] + logs[3] * evt[6]
+                    + omeg2 * (logs[0] * evw[4] + logs[2] * evw[5] + logs[3] * evw[6]);
+        double w = (aa + bb) / cc;
+        check_weight(&w, omeg, nwarn);
+        wv[4 * iom + iv] = w;
+      }
+    }
+    for (int iv = 2; iv < 4; iv++) {
+      ev[0] = (d[0] + d[1]) * 0.5;
+      ev[1] = ev[0];
+      /* k = mod(j+ivert,2)+3 for j=3,4 */
+      for (int j = 3; j <= 4; j++) { int k = (j + iv + 1) % 2 + 3; ev[k - 1] = d[j - 1]; }
+      evt[0] = (p2(E(3)) * E(4) + p2(E(1)) * V(4, 3) - E(1) * p2(E(3))) * 2.0 * V(1, 3) * V(1, 4) * V(3, 4);
+      evw[0] = ((E(1) + E(3) - 2.0 * E(4))) * 2.0 * V(1, 3) * V(1, 4) * V(3, 4);
+      evt[1] = (3.0 * E(1) * (E(1) * (E(3) + E(1)) - 2.0 * E(3) * E(4))) * 2.0 * p2(V(3, 4));
+      evw[1] = (E(3) + 2.0 * E(4) - 3.0 * E(1)) * 2.0 * p2(V(3, 4));
+      evt[2] = -3.0 * E(3) * (p2(E(3)) + E(1) * (E(3) - 2.0 * E(4))) * 2.0 * p2(V(1, 4));
+      evw[2] = -(E(1) - 3.0 * E(3) + 2.0 * E(4)) * 2.0 * p2(V(1, 4));
+      evw[3] = 2.0 * p3(V(1, 3));
+      evt[3] = (-3.0 * p2(E(4))) * evw[3];
+      evt[4] = p2(E(1)) * (-3.0 * E(3) * E(4) + E(1) * (2.0 * E(3) + E(4))) * p2(V(3, 4));
+      evw[4] = (-6.0 * p2(E(1)) + 3.0 * E(1) * E(4) + 3.0 * E(3) * E(4)) * p2(V(3, 4));
+      evt[5] = -p2(E(3)) * (2.0 * E(1) * E(3) - 3.0 * E(1) * E(4) + E(3) * E(4)) * p2(V(1, 4));
+      evw[5] = -3.0 * (-2.0 * p2(E(3)) + (E(1) + E(3)) * E(4)) * p2(V(1, 4));
+      evt[6] = -E(4) * p3(V(1, 3)) * p2(E(4));
+      evw[6] = E(4) * p3(V(1, 3)) * 3.0;
+      double cc = 6.0 * p3(V(1, 3)) * p2(V(1, 4)) * p2(V(3, 4));
+      for (int iom = 0; iom < nom; iom++) {
+        if (qtaylor[iom]) continue;
+        double omeg = omega[iom];
+        double omeg2 = p2(omeg);
+        tans_logs(tans, logs, ev, omeg);
+        double aa = evt[0] + omeg2 * evw[0] + omeg * (tans[0] * evt[1] + tans[2] * evt[2] + tans[3] * evt[3]
+                    + omeg2 * (tans[0] * evw[1] + tans[2] * evw[2] + tans[3] * evw[3]));
+        double bb = logs[0] * evt[4] + logs[2] * evt[5] + logs[3] * evt[6]
+                    + omeg2 * (logs[0] * evw[4] + logs[2] * evw[5] + logs[3] * evw[6]);
+        double w = (aa + bb) / cc;
+        check_weight(&w, omeg, nwarn);
+        wv[4 * iom + iv] = w;
+      }
+    }
+    break;
+  case 8: {
+    ev[0] = (d[0] + d[1]) / 2.0; ev[1] = ev[0];
+    ev[2] = (d[2] + d[3]) / 2.0; ev[3] = ev[2];
+    evw[0] = 6.0 * V(3, 1);
+    evt[0] = (p2(E(1)) - 5.0 * E(1) * E(3) - 2.0 * p2(E(3))) * V(3, 1);
+    evt[1] = -6.0 * (p2(E(3)) + 2.0 * E(1) * E(3));
+    evw[2] = 3.0 * (E(1) + 2.0 * E(3));
+    evt[2] = -3.0 * E(1) * p2(E(3));
+    evw[3] = 6.0 * V(1, 3);
+    evt[3] = -(2.0 * p2(E(1)) + 5.0 * E(1) * E(3) - p2(E(3))) * V(1, 3);
+    evt[4] = 6.0 * (p2(E(1)) + 2.0 * E(1) * E(3));
+    evw[5] = -3.0 * (2.0 * E(1) + E(3));
+    evt[5] = 3.0 * p2(E(1)) * E(3);
+    double cc = 6.0 * p4(V(1, 3));
+    for (int iom = 0; iom < nom; iom++) {
+      if (qtaylor[iom]) continue;
+      double omeg = omega[iom];
+      double omeg2 = p2(omeg);
+      tans_logs(tans, logs, ev, omeg);
+      double aa = evt[0] + evw[0] * omeg2 + omeg * (tans[0] - tans[2]) * (6.0 * omeg2 + evt[1]);
+      double bb = (logs[0] - logs[2]) * (evt[2] + evw[2] * omeg2);
+      double w = (aa + bb) / cc;
+      check_weight(&w, omeg, nwarn);
+      wv[4 * iom + 0] = w;
+      wv[4 * iom + 1] = w;
+      aa = evt[3] + evw[3] * omeg2 + omeg * (tans[0] - tans[2]) * (-6.0 * omeg2 + evt[4]);
+      bb = (logs[0] - logs[2]) * (evt[5] + evw[5] * omeg2);
+      w = (aa + bb) / cc;
+      check_weight(&w, omeg, nwarn);
+      wv[4 * iom + 2] = w;
+      wv[4 * iom + 3] = w;
+    }
+    break;
+  }
+  case 10: {
+    double a = (d[0] + d[1] + d[2]) / 3.0;
+    ev[0] = a; ev[1] = a; ev[2] = a; ev[3] = d[3];
+    evw[0] = 6.0 * V(1, 4);
+    evt[0] = -V(1, 4) * (2.0 * p2(E(1)) - 7.0 * E(1) * E(4) + 11.0 * p2(E(4)));
+    evt[1] = 18.0 * p2(E(4));
+    evw[2] = -9.0 * E(4);
+    evt[2] = 3.0 * p3(E(4));
+    evw[3] = 6.0 * V(4, 1);
+    evt[3] = V(4, 1) * (p2(E(1)) - 5.0 * E(1) * E(4) - 2.0 * p2(E(4)));
+    evt[4] = -6.0 * E(4) * (E(4) + 2.0 * E(1));
+    evt[5] = -3.0 * E(1) * p2(E(4));
+    evw[5] = 3.0 * (E(1) + 2.0 * E(4));
+    double cc1 = 18.0 * p4(V(1, 4));
+    double cc2 = 6.0 * p4(V(1, 4));
+    for (int iom = 0; iom < nom; iom++) {
+      if (qtaylor[iom]) continue;
+      double omeg = omega[iom];
+      double omeg2 = p2(omeg);
+      tans_logs(tans, logs, ev, omeg);
+      double aa = evt[0] + omeg2 * evw[0] + omeg * (tans[0] - tans[3]) * (-6.0 * omeg2 + evt[1]);
+      double bb = (logs[0] - logs[3]) * (evt[2] + evw[2] * omeg2);
+      double w = (aa + bb) / cc1;
+      check_weight(&w, omeg, nwarn);
+      wv[4 * iom + 0] = w; wv[4 * iom + 1] = w; wv[4 * iom + 2] = w;
+      aa = evt[3] + evw[3] * omeg2 + omeg * (tans[0] - tans[3]) * (6.0 * omeg2 + evt[4]);
+      bb = (logs[0] - logs[3]) * (evt[5] + evw[5] * omeg2);
+      w = (aa + bb) / cc2;
+      check_weight(&w, omeg, nwarn);
+      wv[4 * iom + 3] = w;
+    }
+    break;
+  }
+  case 16: {
+    double a = (d[0] + d[1] + d[2] + d[3]) / 4.0;
+    for (int iom = 0; iom < nom; iom++) {
+      if (qtaylor[iom]) continue;
+      double omeg = omega[iom];
+      double w = -a / (12.0 * (p2(omeg) + p2(a)));
+      check_weight(&w, omeg, nwarn);
+      for (int i = 0; i < 4; i++) wv[4 * iom + i] = w;
+    }
+    break;
+  }
+  default:
+    return ORC_E_IMAGCLASS;
+  }
+  return ORC_OK;
+}
+#undef V
+#undef E
+
+/* ------------------------------------------------------------------------- */
+/* generictetra_par  (for_tetrahedra3.f90:220-405)                            */
+/* corners: [4][3] (corner j, coordinate i)                                   */
+/* out: [nom][4], overwritten                                                 */
+/* ------------------------------------------------------------------------- */
+int orc_generictetra(double *out, const double corners[4][3], const double e[4], const double f[4],
+                     const double *omega, int nom, int *nwarn, int *flags,
+                     double *work /* >= 8*nom */, unsigned char *qt /* >= nom */) {
+  double *wsmall = work;            /* [nom][4] */
+  double *wtmp = work + 4 * nom;    /* [nom][4] */
+  double dbig[4], dsmall[4], vec[3][3];
+  for (int i = 0; i < 4 * nom; i++) { out[i] = 0.0; wsmall[i] = 0.0; wtmp[i] = 0.0; }
+  for (int i = 0; i < 4; i++) dbig[i] = f[i] - e[i];
+  /* vec(i,j) = corners(j,i+1)-corners(j,1) */
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) vec[i][j] = corners[i + 1][j] - corners[0][j];
+  /* det(i,j) = vec(2,i)*vec(3,j) - vec(2,j)*vec(3,i); vol = sum_i vec(1,i)*det(j,k) */
+#define DET(i, j) (vec[1][i] * vec[2][j] - vec[1][j] * vec[2][i])
+  double vol = 0.0;
+  for (int i = 0; i < 3; i++) {
+    int j = (i + 1) % 3, k = (j + 1) % 3;
+    vol = vol + vec[0][i] * DET(j, k);
+  }
+#undef DET
+  vol = fabs(vol);
+  if (vol < F32(1.e-10)) {
+    *flags |= ORC_F_ZEROVOL;
+    return ORC_OK;
+  }
+  double maxd = 0.0;
+  for (int i = 0; i < 4; i++) {
+    dsmall[i] = (dbig[1] - dbig[0]) * corners[i][0] + (dbig[2] - dbig[0]) * corners[i][1]
+              + (dbig[3] - dbig[0]) * corners[i][2] + dbig[0];
+    if (fabs(dsmall[i]) > maxd) maxd = fabs(dsmall[i]);
+  }
+  double taylor_omega = 10.0 * maxd;
+  for (int iom = 0; iom < nom; iom++) qt[iom] = (omega[iom] > taylor_omega) ? 1 : 0;
+  for (int iom = 0; iom < nom; iom++)
+    if (qt[iom]) orc_stweight_itaylor(dsmall, omega[iom], &wsmall[4 * iom]);
+  int ind[4], eqf;
+  int st = orc_sorteq(dsmall, ind, &eqf);
+  if (st) return st;
+  st = orc_stweight_imag(wtmp, dsmall, omega, eqf, qt, nom, nwarn);
+  if (st) return st;
+  for (int iom = 0; iom < nom; iom++)
+    if (!qt[iom])
+      for (int i = 0; i < 4; i++) wsmall[4 * iom + ind[i]] = wtmp[4 * iom + i];
+  for (int iom = 0; iom < nom; iom++) {
+    double *w = &wsmall[4 * iom], *o = &out[4 * iom];
+    o[0] = w[0] + w[1] + w[2] + w[3];
+    for (int i = 0; i < 3; i++) {
+      for (int j = 0; j < 4; j++) o[i + 1] = o[i + 1] + w[j] * corners[j][i];
+      o[0] = o[0] - o[i + 1];
+    }
+    for (int i = 0; i < 4; i++) o[i] = o[i] * vol;
+  }
+  return ORC_OK;
+}
+
+/* genericfunf_par (for_tetrahedra3.f90:139-179): tets (1..4) and (2..5) */
+static int orc_genericfunf(double *out, const double corners[5][3], const double e[4], const double f[4],
+                           const double *omega, int nom, int *nwarn, int *flags, double *work,
+                           unsigned char *qt, double *twt) {
+  for (int i = 0; i < 4 * nom; i++) out[i] = 0.0;
+  for (int it = 0; it < 2; it++) {
+    int st = orc_generictetra(twt, (const double(*)[3])corners[it], e, f, omega, nom, nwarn, flags, work, qt);
+    if (st) return st;
+    for (int i = 0; i < 4 * nom; i++) out[i] = out[i] + twt[i];
+  }
+  return ORC_OK;
+}
+
+/* genericprism_par (for_tetrahedra3.f90:181-218): tets (1..4), (2..5), (3..6) */
+static int orc_genericprism(double *out, const double corners[6][3], const double e[4], const double f[4],
+                            const double *omega, int nom, int *nwarn, int *flags, double *work,
+                            unsigned char *qt, double *twt) {
+  for (int i = 0; i < 4 * nom; i++) out[i] = 0.0;
+  for (int it = 0; it < 3; it++) {
+    int st = orc_generictetra(twt, (const double(*)[3])corners[it], e, f, omega, nom, nwarn, flags, work, qt);
+    if (st) return st;
+    for (int i = 0; i < 4 * nom; i++) out[i] = out[i] + twt[i];
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* surfnodes (for_tetrahedra2.f90:1803-2114)                                  */
+/* ------------------------------------------------------------------------- */
+static void orc_surfnodes(signed char ntype[20], double nodes[20][3], int *nnod,
+                          const double e[4], const double f[4], double ef) {
+  static const signed char code[20] = {7, 11, 13, 14, 19, 21, 22, 25, 26, 28,
+                                       35, 37, 38, 41, 42, 44, 49, 50, 52, 56};
+  const double zerotol = 1.0e-12;
+#define EDIF(i, j) (e[(i) - 1] - e[(j) - 1])
+#define FDIF(i, j) (f[(i) - 1] - f[(j) - 1])
+#define EFDIF(i, j, k) (EDIF(i, j) * FDIF(k, j) - EDIF(k, j) * FDIF(i, j))
+#define EFEF(i, j) ((ef - e[(i) - 1]) * (f[(j) - 1] - f[(i) - 1]) - (ef - f[(i) - 1]) * (e[(j) - 1] - e[(i) - 1]))
+  double nt[3];
+  for (int i = 0; i < 20; i++) { nodes[i][0] = nodes[i][1] = nodes[i][2] = 0.0; ntype[i] = code[i]; }
+  /* corners: intnodes(5-i,i) = 1, i=2..4 */
+  for (int i = 2; i <= 4; i++) nodes[i - 1][5 - i - 1] = 1.0;
+  int inod = 4, iint = 4;
+  /* plane 5 with edges from corner 1 */
+  for (int j = 2; j <= 4; j++) {
+    int i = 6 - j;
+    iint++;
+    double denom = EDIF(i, 1);
+    nt[0] = nt[1] = nt[2] = 0.0;
+    if (fabs(denom) > zerotol) {
+      nt[i - 2] = (ef - e[0]) / denom;
+      if ((nt[i - 2] > 0.0) && (nt[i - 2] <= 1.0)) {
+        inod++;
+        for (int c = 0; c < 3; c++) nodes[inod - 1][c] = nt[c];
+        ntype[inod - 1] = ntype[iint - 1];
+      }
+    }
+  }
+  /* plane 5 with the edges on face 4 */
+  for (int k = 1; k <= 3; k++) {
+    int i = k % 3 + 1, j = i % 3 + 1;
+    iint++;
+    double denom = EDIF(i + 1, j + 1);
+    nt[0] = nt[1] = nt[2] = 0.0;
+    if (fabs(denom) > zerotol) {
+      nt[i - 1] = (ef - e[j]) / denom;
+      nt[j - 1] = (e[i] - ef) / denom;
+      if ((nt[i - 1] >= 0.0) && (nt[j - 1] >= 0.0)) {
+        inod++;
+        for (int c = 0; c < 3; c++) nodes[inod - 1][c] = nt[c];
+        ntype[inod - 1] = ntype[iint - 1];
+      }
+    }
+  }
+  /* plane 6 with edges from corner 1 */
+  for (int j = 2; j <= 4; j++) {
+    int i = 6 - j;
+    iint++;
+    double denom = FDIF(i, 1);
+    nt[0] = nt[1] = nt[2] = 0.0;
+    if (fabs(denom) > zerotol) {
+      nt[i - 2] = (ef - f[0]) / denom;
+      if ((nt[i - 2] >= 0.0) && (nt[i - 2] <= 1.0)) {
+        inod++;
+        for (int c = 0; c < 3; c++) nodes[inod - 1][c] = nt[c];
+        ntype[inod - 1] = ntype[iint - 1];
+      }
+    }
+  }
+  /* plane 6 with the edges on face 4 */
+  for (int k = 1; k <= 3; k++) {
+    int i = k % 3 + 1;
+    iint++;
+    int j = i % 3 + 1;
+    double denom = FDIF(i + 1, j + 1);
+    nt[0] = nt[1] = nt[2] = 0.0;
+    if (fabs(denom) > zerotol) {
+      nt[i - 1] = (ef - f[j]) / denom;
+      nt[j - 1] = (f[i] - ef) / denom;
+      if ((nt[i - 1] >= 0.0) && (nt[j - 1] >= 0.0)) {
+        inod++;
+        for (int c = 0; c < 3; c++) nodes[inod - 1][c] = nt[c];
+        ntype[inod - 1] = ntype[iint - 1];
+      }
+    }
+  }
+  /* planes 5 and 6 with faces 1..3 */
+  for (int k = 1; k <= 3; k++) {
+    int i = k % 3 + 1;
+    iint++;
+    int j = i % 3 + 1;
+    double denom1 = EFDIF(i + 1, 1, j + 1);
+    double denom2 = EFDIF(j + 1, 1, i + 1);
+    nt[0] = nt[1] = nt[2] = 0.0;
+    if ((fabs(denom1) > zerotol) && (fabs(denom2) > zerotol)) {
+      nt[i - 1] = EFEF(1, j + 1) / denom1;
+      nt[j - 1] = EFEF(1, i + 1) / denom2;
+      double sumnt = nt[i - 1] + nt[j - 1];
+      if ((nt[i - 1] > 0.0) && (nt[j - 1] > 0.0) && (sumnt <= 1.0)) {
+        inod++;
+        for (int c = 0; c < 3; c++) nodes[inod - 1][c] = nt[c];
+        ntype[inod - 1] = ntype[iint - 1];
+      }
+    }
+  }
+  /* planes 5, 6 and face 4 */
+  {
+    double denom1 = EFDIF(2, 4, 3);
+    double denom2 = EFDIF(3, 4, 2);
+    double denom3 = EFDIF(4, 2, 3);
+    iint++;
+    if ((fabs(denom1) > zerotol) && (fabs(denom2) > zerotol) && (fabs(denom3) > zerotol)) {
+      nt[0] = EFEF(4, 3) / denom1;
+      nt[1] = EFEF(4, 2) / denom2;
+      nt[2] = EFEF(2, 3) / denom3;
+      if ((nt[0] > 0.0) && (nt[1] > 0.0) && (nt[2] > 0.0)) {
+        inod++;
+        for (int c = 0; c < 3; c++) nodes[inod - 1][c] = nt[c];
+        ntype[inod - 1] = ntype[iint - 1];
+      }
+    }
+  }
+#undef EDIF
+#undef FDIF
+#undef EFDIF
+#undef EFEF
+  *nnod = inod;
+}
+
+/* unrepnodes (for_tetrahedra2.f90:2116-2175) */
+static void orc_unrepnodes(signed char ntype[20], double nodes[20][3], int *nnod) {
+  const double zerotol = 1.0e-8;
+  int n = *nnod;
+  int i = 1;
+  while (i < n) {
+    double ni[3] = {nodes[i - 1][0], nodes[i - 1][1], nodes[i - 1][2]};
+    int j = i + 1;
+    while (j <= n) {
+      double s = 0.0;
+      for (int k = 0; k < 3; k++) s = s + fabs(ni[k] - nodes[j - 1][k]);
+      if (s < zerotol) {
+        ntype[i - 1] = (signed char)(ntype[i - 1] | (ntype[j - 1] & 63));
+        for (int k = j + 1; k <= n; k++) {
+          nodes[k - 2][0] = nodes[k - 1][0]; nodes[k - 2][1] = nodes[k - 1][1]; nodes[k - 2][2] = nodes[k - 1][2];
+          ntype[k - 2] = ntype[k - 1];
+        }
+        nodes[n - 1][0] = nodes[n - 1][1] = nodes[n - 1][2] = 0.0;
+        ntype[n - 1] = 0;
+        n--;
+      } else {
+        j++;
+      }
+    }
+    i++;
+  }
+  *nnod = n;
+}
+
+/* tlinap (for_tetrahedra2.f90:2283-2304) */
+static double orc_tlinap(const double x[3], const double fv[4]) {
+  double fx = fv[0];
+  for (int i = 0; i < 3; i++) {
+    double df = fv[i + 1] - fv[0];
+    fx = fx + df * x[i];
+  }
+  return fx;
+}
+
+/* relnodes (for_tetrahedra2.f90:2177-2281) */
+static int orc_relnodes(signed char ntype[20], double nodes[20][3], int *nnod,
+                        const double e[4], const double f[4], double ef) {
+  const double zerotol = 1.00e-12;
+  int info = 0;
+  int inod = 1, maxnod = *nnod;
+  while (inod <= maxnod) {
+    double e1 = orc_tlinap(nodes[inod - 1], e) - ef;
+    double e2 = orc_tlinap(nodes[inod - 1], f) - ef;
+    int keep = 0;
+    int t = ntype[inod - 1];
+    if (t >= 7 && t <= 15) {
+      int occ = (e1 < 0.0) || (fabs(e1) <= zerotol);
+      int unocc = (e2 > 0.0) || (fabs(e2) <= zerotol);
+      keep = occ && unocc;
+    } else if (t >= 16 && t <= 31) {
+      keep = (e2 > 0.0) || (fabs(e2) <= zerotol);
+    } else if (t >= 32 && t <= 47) {
+      keep = (e1 < 0.0) || (fabs(e1) <= zerotol);
+    } else if (t >= 48 && t <= 63) {
+      keep = 1;
+    } else {
+      info = 1;  /* keep stays as it was: undefined in the reference; treat as drop */
+    }
+    if (keep) {
+      inod++;
+    } else {
+      for (int j = inod + 1; j <= maxnod; j++) {
+        nodes[j - 2][0] = nodes[j - 1][0]; nodes[j - 2][1] = nodes[j - 1][1]; nodes[j - 2][2] = nodes[j - 1][2];
+        ntype[j - 2] = ntype[j - 1];
+      }
+      maxnod--;
+    }
+  }
+  for (int i = maxnod + 1; i <= *nnod; i++) {
+    nodes[i - 1][0] = nodes[i - 1][1] = nodes[i - 1][2] = 0.0;
+    ntype[i - 1] = 0;
+  }
+  *nnod = maxnod;
+  return info ? ORC_E_NTYPE : ORC_OK;
+}
+
+/* sortnodes (for_tetrahedra2.f90:2306-2773).  index[inod][isub], 1-based
+ * positions as in the reference, 0 = not part of sub-polyhedron isub. */
+#define BT(x, b) ((((int)(x)) >> (b)) & 1)
+static int orc_sortnodes(const signed char ntype[20], int nnod, int index[20][2], int *flags) {
+  int spl[6] = {0, 0, 0, 0, 0, 0};
+  int it = 0, is = 0, ip = 0;
+  int triangle[6] = {0}, square[6] = {0}, penta[2] = {0};
+  for (int i = 0; i < 20; i++) index[i][0] = index[i][1] = 0;
+  if (nnod < 5 || nnod > 8) return ORC_OK;
+  for (int ibit = 0; ibit <= 5; ibit++) {
+    for (int inod = 0; inod < nnod; inod++) if (BT(ntype[inod], ibit)) spl[ibit]++;
+  }
+  switch (nnod) {
+  case 5: {
+    int signbit = 0;
+    for (int ibit = 0; ibit <= 5; ibit++) {
+      if (spl[ibit] == 3) { triangle[it++] = ibit; }
+      else if (spl[ibit] == 4) { square[is++] = ibit; signbit = ibit; }
+    }
+    if (is != 0) {
+      int is1 = 1, signnod = 0;
+      for (int inod = 0; inod < 5; inod++) {
+        if (BT(ntype[inod], triangle[0])) {
+          if (!BT(ntype[inod], signbit)) index[inod][0] = 2;
+          else { index[inod][0] = is1; is1 += 2; signnod = inod; }
+        }
+      }
+      for (int i = 1; i < 4; i++) if (BT(ntype[signnod], triangle[i])) signbit = triangle[i];
+      for (int inod = 0; inod < 5; inod++)
+        if (index[inod][0] == 0) index[inod][0] = BT(ntype[inod], signbit) ? 5 : 4;
+    } else {
+      /* bipyramid: see header (reference resets it1/it2 per node) */
+      int it1 = 1, it2 = 1;
+      *flags |= ORC_F_F7A;
+      for (int inod = 0; inod < 5; inod++) {
+        int nb = 0;
+        for (int ibit = 0; ibit <= 5; ibit++) if (BT(ntype[inod], ibit)) nb++;
+        if (nb == 3) { index[inod][0] = 1 + (it1 - 1) * 4; it1++; }
+        else { index[inod][0] = 1 + it2; it2++; }
+      }
+      if (it1 != 3 || it2 != 4) return ORC_E_SORTNODES;
+    }
+    break;
+  }
+  case 6: {
+    for (int ibit = 0; ibit <= 5; ibit++) {
+      if (spl[ibit] == 3) triangle[it++] = ibit;
+      else if (spl[ibit] == 4) square[is++] = ibit;
+    }
+    if (is == 2) {
+      if (it != 4) return ORC_E_SORTNODES;
+      int cnt = 1, signnod = -1, signnodb = -1;
+      for (int inod = 0; inod < 6; inod++) {
+        if (BT(ntype[inod], square[0]) && BT(ntype[inod], square[1])) {
+          index[inod][0] = 2 + (cnt - 1) * 3;
+          if (cnt == 1) signnod = inod;
+          if (cnt == 2) signnodb = inod;
+          cnt++;
+        }
+      }
+      if (signnod < 0 || signnodb < 0) return ORC_E_SORTNODES;
+      int sigtri[6] = {0};
+      for (int inod = 0; inod < 6; inod++)
+        for (int t = 0; t < 4; t++) if (BT(ntype[inod], triangle[t])) sigtri[inod]++;
+      int sa = -1, sb = -1;
+      for (int t = 0; t < 4; t++) {
+        if (BT(ntype[signnod], triangle[t])) sa = t;
+        if (BT(ntype[signnodb], triangle[t])) sb = t;
+      }
+      if (sa < 0 || sb < 0) return ORC_E_SORTNODES;
+      for (int inod = 0; inod < 6; inod++) {
+        if (index[inod][0] == 0) {
+          if (sigtri[inod] == 3 && BT(ntype[inod], triangle[sa])) index[inod][0] = 4;
+          if (sigtri[inod] == 2 && BT(ntype[inod], triangle[sa])) index[inod][0] = 1;
+          if (sigtri[inod] == 3 && BT(ntype[inod], triangle[sb])) index[inod][0] = 3;
+          if (sigtri[inod] == 2 && BT(ntype[inod], triangle[sb])) index[inod][0] = 6;
+        }
+      }
+    } else if (is == 3) {
+      for (int inod = 0; inod < 6; inod++) {
+        for (int t = 0; t < 2; t++) {
+          if (BT(ntype[inod], triangle[t])) {
+            if (BT(ntype[inod], square[0]) && BT(ntype[inod], square[1])) index[inod][0] = 1 + 3 * t;
+            else if (BT(ntype[inod], square[0])) index[inod][0] = 2 + 3 * t;
+            else if (BT(ntype[inod], square[1])) index[inod][0] = 3 + 3 * t;
+            else return ORC_E_SORTNODES;
+          }
+        }
+      }
+    } else {
+      return ORC_E_SORTNODES;
+    }
+    for (int inod = 0; inod < 6; inod++) if (index[inod][0] < 1 || index[inod][0] > 6) return ORC_E_SORTNODES;
+    break;
+  }
+  case 7: {
+    for (int ibit = 0; ibit <= 5; ibit++) {
+      if (spl[ibit] == 3) triangle[it++] = ibit;
+      else if (spl[ibit] == 4) square[is++] = ibit;
+      else if (spl[ibit] == 5) { if (ip < 2) penta[ip] = ibit; ip++; }
+    }
+    if (it == 2) {
+      if (is != 4) return ORC_E_SORTNODES;
+      int signnod = -1;
+      for (int inod = 0; inod < 7; inod++)
+        if (BT(ntype[inod], triangle[0]) && BT(ntype[inod], triangle[1])) {
+          index[inod][0] = index[inod][1] = 2; signnod = inod;
+        }
+      if (signnod < 0) return ORC_E_SORTNODES;
+      int is1 = 0, sa = -1, sb = -1;
+      for (int ibit = 0; ibit <= 5; ibit++) {
+        if (BT(ntype[signnod], ibit) && spl[ibit] == 4 && is1 == 0) { sa = ibit; is1++; }
+        else if (BT(ntype[signnod], ibit) && spl[ibit] == 4) sb = ibit;
+      }
+      if (sa < 0 || sb < 0) return ORC_E_SORTNODES;
+      for (int inod = 0; inod < 7; inod++) {
+        if (inod == signnod) continue;
+        if (BT(ntype[inod], triangle[0]) && BT(ntype[inod], sa)) index[inod][0] = 1;
+        else if (BT(ntype[inod], triangle[0]) && BT(ntype[inod], sb)) index[inod][0] = 3;
+        else if (BT(ntype[inod], triangle[1]) && BT(ntype[inod], sa)) index[inod][1] = 1;
+        else if (BT(ntype[inod], triangle[1]) && BT(ntype[inod], sb)) index[inod][1] = 3;
+        else if (BT(ntype[inod], sa)) { index[inod][0] = index[inod][1] = 4; }
+        else { index[inod][0] = index[inod][1] = 5; }
+      }
+    } else if (it == 3) {
+      if (ip < 1 || is < 2) return ORC_E_SORTNODES;
+      int iind = -1;
+      for (int inod = 0; inod < 7; inod++) {
+        int nb = 0;
+        for (int t = 0; t < 3; t++) if (BT(ntype[inod], triangle[t])) nb++;
+        if (BT(ntype[inod], penta[0])) {
+          if (nb == 2) index[inod][0] = 1;
+        } else {
+          if (nb == 2) { index[inod][0] = 2; index[inod][1] = 1; }
+          else { index[inod][1] = 4; iind = inod; }
+        }
+      }
+      if (iind < 0) return ORC_E_SORTNODES;
+      int sa = -1;
+      for (int t = 0; t < 3; t++) if (BT(ntype[iind], triangle[t])) sa = triangle[t];
+      if (sa < 0) return ORC_E_SORTNODES;
+      for (int inod = 0; inod < 7; inod++) {
+        if (index[inod][0] == 0 && index[inod][1] == 0) {
+          if (BT(ntype[inod], sa) && BT(ntype[inod], square[0])) index[inod][1] = 5;
+          else if (BT(ntype[inod], sa) && BT(ntype[inod], square[1])) { index[inod][0] = 5; index[inod][1] = 2; }
+          else if (BT(ntype[inod], square[0])) { index[inod][0] = 4; index[inod][1] = 3; }
+          else index[inod][0] = 3;
+        }
+      }
+    } else {
+      return ORC_E_SORTNODES;
+    }
+    break;
+  }
+  case 8: {
+    for (int ibit = 0; ibit <= 5; ibit++) {
+      if (spl[ibit] == 3) triangle[it++] = ibit;
+      else if (spl[ibit] == 4) square[is++] = ibit;
+      else if (spl[ibit] == 5) { if (ip < 2) penta[ip] = ibit; ip++; }
+    }
+    if (is == 2) {
+      if (it < 2 || ip < 2) return ORC_E_SORTNODES;
+      for (int inod = 0; inod < 8; inod++) {
+        for (int t = 0; t < 2; t++) {
+          if (BT(ntype[inod], triangle[t])) {
+            if (BT(ntype[inod], penta[0]) && BT(ntype[inod], penta[1])) index[inod][0] = 1 + t * 3;
+            else if (BT(ntype[inod], penta[0])) index[inod][0] = 2 + t * 3;
+            else if (BT(ntype[inod], penta[1])) index[inod][0] = 3 + t * 3;
+          }
+        }
+        for (int q = 0; q < 2; q++) {
+          if (BT(ntype[inod], penta[q])) {
+            if (BT(ntype[inod], square[0]) && BT(ntype[inod], square[1])) index[inod][1] = 1 + q * 3;
+            else if (BT(ntype[inod], square[0])) index[inod][1] = 2 + q * 3;
+            else if (BT(ntype[inod], square[1])) index[inod][1] = 3 + q * 3;
+          }
+        }
+      }
+    } else if (is == 6) {
+      int is1 = 0, ibt[3] = {0, 0, 0};
+      index[0][0] = 1; index[0][1] = 0;
+      for (int ibit = 0; ibit <= 5; ibit++)
+        if (BT(ntype[0], ibit)) { if (is1 < 3) ibt[is1] = ibit; is1++; }
+      if (is1 != 3) return ORC_E_SORTNODES;
+      for (int inod = 1; inod < 8; inod++) {
+        if (BT(ntype[inod], ibt[0])) {
+          if (BT(ntype[inod], ibt[1])) { index[inod][0] = index[inod][1] = 2; }
+          else if (BT(ntype[inod], ibt[2])) { index[inod][0] = index[inod][1] = 3; }
+          else { index[inod][0] = 0; index[inod][1] = 1; }
+        } else if (BT(ntype[inod], ibt[1])) {
+          if (BT(ntype[inod], ibt[2])) { index[inod][0] = 4; index[inod][1] = 0; }
+          else { index[inod][0] = index[inod][1] = 5; }
+        } else {
+          if (BT(ntype[inod], ibt[2])) { index[inod][0] = index[inod][1] = 6; }
+          else { index[inod][0] = 0; index[inod][1] = 4; }
+        }
+      }
+    } else {
+      return ORC_E_SORTNODES;
+    }
+    break;
+  }
+  }
+  return ORC_OK;
+}
+#undef BT
+
+/* scratch needed by orc_convw1t: doubles */
+static size_t conv_work_doubles(int nom) { return (size_t)nom * (8 + 4 + 4 + 4 + 4); }
+
+/* Sum of the two sub-polyhedra of nnod = 7, 8 (for_tetrahedra3.f90:108,118).
+ * The reference declares its scratch as wtemp(4,2,nom) but passes the
+ * sections wtemp(:,:,isub) as w(4,nom) (for_tetrahedra3.f90:33,105,115): the
+ * second call writes from element 8 on, over the first call's frequencies
+ * 3..nom, and the final sum reads across the declared extent.  Its result is
+ *   weight(:,j) = [j<=2 ? sub1(:,j) : sub2(:,j-2)] + sub2(:,j).
+ * orc_compat_nnod78 = 1 (default) reproduces that; 0 gives sub1 + sub2.     */
+int orc_compat_nnod78 = 1;
+
+static void combine_subs(double *weight, const double *wt1, const double *wt2, int nom) {
+  for (int j = 0; j < nom; j++)
+    for (int i = 0; i < 4; i++) {
+      double a;
+      if (!orc_compat_nnod78 || j < 2) a = wt1[4 * j + i];
+      else a = wt2[4 * (j - 2) + i];
+      weight[4 * j + i] = a + wt2[4 * j + i];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* convw1t_par (for_tetrahedra3.f90:3-136).  weight: [nom][4]                 */
+/* ------------------------------------------------------------------------- */
+int orc_convw1t(double *weight, const double e[4], const double f[4], double ef,
+                const double *omega, int nom, int *nwarn, int *flags, int *nnod_out,
+                double *work, unsigned char *qt) {
+  signed char ntype[20];
+  double nodes[20][3];
+  int nnod, index[20][2];
+  double *gwork = work;               /* 8*nom for generictetra */
+  double *twt = work + 8 * nom;       /* 4*nom */
+  double *wt1 = twt + 4 * nom;        /* 4*nom */
+  double *wt2 = wt1 + 4 * nom;        /* 4*nom */
+  orc_surfnodes(ntype, nodes, &nnod, e, f, ef);
+  orc_unrepnodes(ntype, nodes, &nnod);
+  int st = orc_relnodes(ntype, nodes, &nnod, e, f, ef);
+  if (st) return st;
+  st = orc_sortnodes(ntype, nnod, index, flags);
+  if (st) return st;
+  if (nnod_out) *nnod_out = nnod;
+  for (int i = 0; i < 4 * nom; i++) weight[i] = 0.0;
+  if (nnod >= 5 && nnod <= 8) {
+    /* every corner position of each sub-polyhedron must be filled once */
+    int npos = (nnod == 5 || nnod == 7) ? 5 : 6;
+    int nsub = (nnod > 6) ? 2 : 1;
+    for (int isub = 0; isub < nsub; isub++) {
+      int seen[7] = {0};
+      for (int j = 0; j < nnod; j++) {
+        int p = index[j][isub];
+        if (p < 0 || p > npos) return ORC_E_SORTNODES;
+        if (p) seen[p]++;
+      }
+      for (int p = 1; p <= npos; p++) if (seen[p] != 1) return ORC_E_SORTNODES;
+    }
+  }
+  switch (nnod) {
+  case 0: case 1: case 2: case 3:
+    break;
+  case 4: {
+    double c[4][3];
+    for (int j = 0; j < 4; j++) for (int i = 0; i < 3; i++) c[j][i] = nodes[j][i];
+    st = orc_generictetra(weight, (const double(*)[3])c, e, f, omega, nom, nwarn, flags, gwork, qt);
+    break;
+  }
+  case 5: {
+    double c[5][3];
+    for (int j = 0; j < 5; j++) for (int i = 0; i < 3; i++) c[index[j][0] - 1][i] = nodes[j][i];
+    st = orc_genericfunf(weight, (const double(*)[3])c, e, f, omega, nom, nwarn, flags, gwork, qt, twt);
+    break;
+  }
+  case 6: {
+    double c[6][3];
+    for (int j = 0; j < 6; j++) for (int i = 0; i < 3; i++) c[index[j][0] - 1][i] = nodes[j][i];
+    st = orc_genericprism(weight, (const double(*)[3])c, e, f, omega, nom, nwarn, flags, gwork, qt, twt);
+    break;
+  }
+  case 7: {
+    double c[5][3];
+    memset(c, 0, sizeof(c));
+    for (int isub = 0; isub < 2; isub++) {
+      for (int j = 0; j < 7; j++)
+        if (index[j][isub] != 0) for (int i = 0; i < 3; i++) c[index[j][isub] - 1][i] = nodes[j][i];
+      st = orc_genericfunf(isub ? wt2 : wt1, (const double(*)[3])c, e, f, omega, nom, nwarn, flags, gwork, qt, twt);
+      if (st) return st;
+    }
+    combine_subs(weight, wt1, wt2, nom);
+    break;
+  }
+  case 8: {
+    double c[6][3];
+    memset(c, 0, sizeof(c));
+    for (int isub = 0; isub < 2; isub++) {
+      for (int j = 0; j < 8; j++)
+        if (index[j][isub] != 0) for (int i = 0; i < 3; i++) c[index[j][isub] - 1][i] = nodes[j][i];
+      st = orc_genericprism(isub ? wt2 : wt1, (const double(*)[3])c, e, f, omega, nom, nwarn, flags, gwork, qt, twt);
+      if (st) return st;
+    }
+    combine_subs(weight, wt1, wt2, nom);
+    break;
+  }
+  default:
+    return ORC_E_NNOD;
+  }
+  return st;
+}
+
+/* Convenience entry used by the tests: one (e,f) pair, weight [nom][4]. */
+int orc_convw1t_simple(double *weight, const double *e, const double *f, double ef,
+                       const double *omega, int nom, int *nwarn, int *flags, int *nnod) {
+  double *work = (double *)malloc(conv_work_doubles(nom) * sizeof(double));
+  unsigned char *qt = (unsigned char *)malloc((size_t)nom);
+  if (!work || !qt) { free(work); free(qt); return ORC_E_ARG; }
+  int st = orc_convw1t(weight, e, f, ef, omega, nom, nwarn, flags, nnod, work, qt);
+  free(work); free(qt);
+  return st;
+}
+
+/* Exposed single-tetrahedron weight for the tests (unit tet corners given). */
+int orc_generictetra_simple(double *out, const double *corners12, const double *e, const double *f,
+                            const double *omega, int nom, int *nwarn) {
+  double c[4][3];
+  for (int j = 0; j < 4; j++) for (int i = 0; i < 3; i++) c[j][i] = corners12[3 * j + i];
+  double *work = (double *)malloc((size_t)nom * 8 * sizeof(double));
+  unsigned char *qt = (unsigned char *)malloc((size_t)nom);
+  int flags = 0;
+  int st = orc_generictetra(out, (const double(*)[3])c, e, f, omega, nom, nwarn, &flags, work, qt);
+  free(work); free(qt);
+  return st;
+}
+
+/* ------------------------------------------------------------------------- */
+/* bz_calcqdepw_par2 / par3 (for_tetrahedra.f90:390-542)                      */
+/* enk: F-order (nbnd, nkp); tetc: F-order (4, ntet) 0-based; kqid 0-based.   */
+/* kcw: F-order (ncg+nomx+1, nbnd-ncg-numin, nkp, nout), zero-filled here.    */
+/* dUl: F-order (nom, nl) or NULL (then nout = nom).                          */
+/* Status: <0 error; otherwise number of check_weight warnings.               */
+/* ------------------------------------------------------------------------- */
+int orc_bz_calcqdepw(double *kcw, const double *enk, double ef, const int *kqid, int ncg, int nomx,
+                     int numin, const double *omega, const int *tetc, int nkp, int nom, int nbnd,
+                     int ntet, const double *dUl, int nl, int nthreads, int *flags_out) {
+  int no = ncg + nomx + 1, ne = nbnd - ncg - numin;
+  int nout = dUl ? nl : nom;
+  if (no <= 0 || ne <= 0 || nkp <= 0 || nom <= 0 || ntet != 6 * nkp) return ORC_E_ARG;
+  size_t tot = (size_t)no * ne * nkp * nout;
+  for (size_t i = 0; i < tot; i++) kcw[i] = 0.0;
+  double wtet = 1.0 / ntet;
+  int status = 0, nwarn_total = 0, flags_total = 0;
+  (void)nthreads;
+  for (int itet = 0; itet < ntet; itet++) {
+    int tetln = 6 * kqid[itet / 6] + itet % 6;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) reduction(+:nwarn_total) reduction(|:flags_total) reduction(min:status)
+    for (int jb = 0; jb < ne; jb++) {
+      double ee2[4], ee1[4];
+      for (int i = 0; i < 4; i++) ee2[i] = enk[(jb + ncg + numin) + (size_t)nbnd * tetc[i + 4 * tetln]];
+      double mx = ee2[0];
+      for (int i = 1; i < 4; i++) if (ee2[i] > mx) mx = ee2[i];
+      if (mx <= ef) continue;
+      double *w1t = (double *)malloc((size_t)nom * 4 * sizeof(double));
+      double *work = (double *)malloc(conv_work_doubles(nom) * sizeof(double));
+      unsigned char *qt = (unsigned char *)malloc((size_t)nom);
+      for (int ib = 0; ib < no; ib++) {
+        for (int i = 0; i < 4; i++) ee1[i] = enk[ib + (size_t)nbnd * tetc[i + 4 * itet]];
+        double mn = ee1[0];
+        for (int i = 1; i < 4; i++) if (ee1[i] < mn) mn = ee1[i];
+        if (mn > ef) continue;
+        int nw = 0, fl = 0;
+        int st = orc_convw1t(w1t, ee1, ee2, ef, omega, nom, &nw, &fl, NULL, work, qt);
+        nwarn_total += nw;
+        flags_total |= fl;
+        if (st < 0) { if (st < status) status = st; continue; }
+        for (int i = 0; i < 4; i++) {
+          int kin = tetc[i + 4 * itet];
+          size_t base = (size_t)ib + (size_t)no * ((size_t)jb + (size_t)ne * (size_t)kin);
+          size_t stride = (size_t)no * ne * nkp;
+          if (!dUl) {
+            for (int iom = 0; iom < nom; iom++)
+              kcw[base + stride * iom] = kcw[base + stride * iom] + w1t[4 * iom + i] * 6 * wtet;
+          } else {
+            for (int il = 0; il < nl; il++) {
+              double s = 0.0;  /* w2t(i,il) = sum_iom w1t(i,iom)*dUl(iom,il) */
+              for (int iom = 0; iom < nom; iom++) s = s + w1t[4 * iom + i] * dUl[iom + (size_t)nom * il];
+              kcw[base + stride * il] = kcw[base + stride * il] + s * 6 * wtet;
+            }
+          }
+        }
+      }
+      free(w1t); free(work); free(qt);
+    }
+  }
+  if (flags_out) *flags_out = flags_total;
+  return status < 0 ? status : nwarn_total;
+}

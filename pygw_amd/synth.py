@@ -1,0 +1,204 @@
+"""Seeded synthetic inputs for the polarization path.
+
+Band model: e_n(k) = c_n + t_n * S(k), S(k) = (1/3) * sum_i cos(2 pi k_i) in
+[-1, 1].  Occupied c_n ~ U[-10,-1] eV, empty c_n ~ U[1,20] eV, t_n ~ U[0.2,1]
+eV; metals add bands with c_n ~ U[-2,2] eV that cross E_F = 0.  (The unnormalised
+sum has amplitude 3 t_n, which would push roughly half of all seeds through
+E_F and turn the stated "insulator" into a metal; the 1/3 keeps every stated
+range and keeps the gap open.)  Energies are sorted per k and converted to
+Hartree, E_F = 0 (pygw.py:785-794).
+
+Matrix elements: M[k, p, I] with p = i_occ * n_e + i_emp (the C-order ravel of
+pygw.py:3170), complex Gaussian with Re, Im ~ N(0, 1/2), scaled by
+exp(-|e_n(k) - e_m(k-q)| / 20 eV) / sqrt(PB) and by v_half[I].  The Gaussian
+is counter-based (SplitMix64 of (seed, linear index), Box-Muller), so the
+device generator and `gaussian_pairs` below produce the same draws up to the
+last-ulp differences of log/cos/sin.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import mesh
+
+BASE_SEED = 20261203
+MASK64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+@dataclass
+class PolConfig:
+    name: str
+    div: tuple
+    n_occ: int
+    n_emp: int
+    n_om: int
+    wmin_ev: float
+    wmax_ev: float
+    pb: int
+    q: tuple = (1, 0, 0)          # integer q on the k-mesh
+    n_metal: int = 0              # bands crossing E_F (c_n ~ U[-2,2] eV)
+    vhalf: str = "none"           # "none" | "coulomb" (4 pi / |q+G|^2, literal)
+    seed: int = BASE_SEED
+    mesh_kind: str = "log"        # "log" | "tan"
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def nk(self):
+        return int(np.prod(self.div))
+
+    @property
+    def ntet(self):
+        return 6 * self.nk
+
+
+# The reference's five workloads (stated in the benchmark description).
+CONFIGS = {
+    "c1": PolConfig("c1", (4, 4, 4), 8, 24, 32, 0.01, 100.0, 200, q=(1, 0, 0), seed=BASE_SEED + 1),
+    "c2": PolConfig("c2", (8, 8, 8), 16, 48, 64, 0.005, 200.0, 600, q=(1, 0, 0), vhalf="coulomb",
+                    seed=BASE_SEED + 2),
+    "c3": PolConfig("c3", (10, 10, 10), 32, 200, 64, 0.005, 200.0, 1200, q=(1, 0, 0), seed=BASE_SEED + 3),
+    "c4": PolConfig("c4", (8, 8, 8), 16, 48, 48, 0.005, 200.0, 700, n_metal=3, seed=BASE_SEED + 4),
+    "c5": PolConfig("c5", (8, 8, 8), 20, 120, 64, 0.005, 200.0, 900, q=(1, 0, 0), vhalf="coulomb",
+                    seed=BASE_SEED + 5, mesh_kind="tan"),
+}
+
+
+def small_config(name="tiny", div=(2, 2, 2), n_occ=2, n_emp=3, n_om=6, pb=8, n_metal=0, seed=7,
+                 wmin_ev=0.01, wmax_ev=100.0, vhalf="none", q=(1, 0, 0)):
+    return PolConfig(name, tuple(div), n_occ, n_emp, n_om, wmin_ev, wmax_ev, pb, q=tuple(q),
+                     n_metal=n_metal, vhalf=vhalf, seed=seed)
+
+
+@dataclass
+class Bands:
+    enk: np.ndarray        # Fortran (nbnd, N_k), Hartree, E_F = 0
+    ncg: int
+    nomx: int
+    numin: int
+    ef: float = 0.0
+
+    @property
+    def n_o(self):
+        return self.ncg + self.nomx + 1
+
+    @property
+    def n_e(self):
+        return self.enk.shape[0] - self.ncg - self.numin
+
+
+def band_energies(cfg: PolConfig) -> Bands:
+    rng = np.random.default_rng(cfg.seed)
+    c_occ = rng.uniform(-10.0, -1.0, cfg.n_occ)
+    c_emp = rng.uniform(1.0, 20.0, cfg.n_emp)
+    c_met = rng.uniform(-2.0, 2.0, cfg.n_metal)
+    c = np.concatenate([c_occ, c_met, c_emp])
+    t = rng.uniform(0.2, 1.0, c.size)
+    kf = mesh.kpoint_coords(cfg.div)
+    s = np.cos(2 * np.pi * kf).sum(axis=1) / 3.0
+    e = (c[:, None] + t[:, None] * s[None, :]) / mesh.H2EV   # [nbnd, nk]
+    e = np.sort(e, axis=0)
+    nocc_k = (e < 0.0).sum(axis=0)
+    nomx = int(nocc_k.max()) - 1      # pygw.py:1002-1009
+    numin = int(nocc_k.min())
+    return Bands(np.asfortranarray(e), 0, nomx, numin, 0.0)
+
+
+def omega_mesh(cfg: PolConfig):
+    """(omega, womeg) in Hartree.  Log mesh: womeg is the trapezoid rule."""
+    if cfg.mesh_kind == "tan":
+        fm = mesh.FrequencyMesh(cfg.n_om, cfg.wmin_ev / mesh.H2EV, cfg.wmax_ev / mesh.H2EV)
+        return fm.omega, fm.womeg
+    om = mesh.log_mesh(cfg.wmin_ev, cfg.wmax_ev, cfg.n_om)
+    w = np.zeros_like(om)
+    d = np.diff(om)
+    w[:-1] += 0.5 * d
+    w[1:] += 0.5 * d
+    return om, w
+
+
+def g_vectors(cfg: PolConfig) -> np.ndarray:
+    """PB distinct integer G with |G| <= 6 (more if PB exceeds their count),
+    seeded; dropping G = 0 at q = 0."""
+    r = 6
+    while True:
+        rng_ = np.arange(-r, r + 1)
+        g = np.array(np.meshgrid(rng_, rng_, rng_, indexing="ij")).reshape(3, -1).T
+        g = g[(g ** 2).sum(1) <= r * r]
+        if all(x == 0 for x in cfg.q):
+            g = g[(g ** 2).sum(1) > 0]
+        if len(g) >= cfg.pb:
+            break
+        r += 1
+    rng = np.random.default_rng(cfg.seed + 77)
+    sel = rng.choice(len(g), cfg.pb, replace=False)
+    return g[np.sort(sel)]
+
+
+def vhalf(cfg: PolConfig) -> np.ndarray:
+    """Diagonal v_half[I]: 4 pi / |q+G_I|^2 taken literally (q, G in units of
+    the reciprocal-lattice vector of a unit cube) or ones."""
+    if cfg.vhalf == "none":
+        return np.ones(cfg.pb)
+    g = g_vectors(cfg).astype(np.float64)
+    qf = np.array(cfg.q, dtype=np.float64) / np.array(cfg.div, dtype=np.float64)
+    qg2 = ((qf[None, :] + g) ** 2).sum(1)
+    return 4.0 * np.pi / qg2
+
+
+def q_index(cfg: PolConfig) -> int:
+    return int(mesh.k2indx(np.array(cfg.q) % np.array(cfg.div), np.array(cfg.div)))
+
+
+# ---------------------------------------------------------------------------
+# counter-based Gaussian (host twin of the device generator)
+# ---------------------------------------------------------------------------
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        x = (x + np.uint64(0x9E3779B97F4A7C15)) & MASK64
+        z = x
+        z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & MASK64
+        z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & MASK64
+        return z ^ (z >> np.uint64(31))
+
+
+def gaussian_pairs(seed: int, idx: np.ndarray):
+    """(re, im) with Re, Im ~ N(0, 1/2) for linear indices idx (uint64)."""
+    idx = np.asarray(idx, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        base = (np.uint64(seed) * np.uint64(0xD1B54A32D192ED03)) & MASK64
+        a = _splitmix64((base ^ (idx * np.uint64(2))) & MASK64)
+        b = _splitmix64((base ^ (idx * np.uint64(2) + np.uint64(1))) & MASK64)
+    u1 = ((a >> np.uint64(11)).astype(np.float64) + 1.0) * (1.0 / 9007199254740992.0)  # (0, 1]
+    u2 = (b >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)          # [0, 1)
+    r = np.sqrt(-np.log(u1))
+    return r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)
+
+
+def pair_scale(bands: Bands, kqid_col: np.ndarray, ik: int) -> np.ndarray:
+    """exp(-|e_n(k) - e_m(k-q)| / 20 eV) for p = i_o * n_e + i_e."""
+    no, ne = bands.n_o, bands.n_e
+    eo = bands.enk[:no, ik]
+    ee = bands.enk[bands.ncg + bands.numin:, kqid_col[ik]]
+    d = np.abs(eo[:, None] - ee[None, :]) * mesh.H2EV
+    return np.exp(-d / 20.0).reshape(no * ne)
+
+
+def matrix_elements(cfg: PolConfig, bands: Bands, kqid_col: np.ndarray, ks=None,
+                    dtype=np.complex128) -> np.ndarray:
+    """M[k, p, I] for the k in `ks` (default: all), host generator."""
+    ks = range(cfg.nk) if ks is None else ks
+    no, ne, pb = bands.n_o, bands.n_e, cfg.pb
+    np_ = no * ne
+    vh = vhalf(cfg)
+    out = np.empty((len(ks), np_, pb), dtype=dtype)
+    for i, ik in enumerate(ks):
+        idx = (np.uint64(ik) * np.uint64(np_ * pb)) + np.arange(np_ * pb, dtype=np.uint64)
+        re, im = gaussian_pairs(cfg.seed, idx)
+        m = (re + 1j * im).reshape(np_, pb)
+        m *= (pair_scale(bands, kqid_col, ik) / np.sqrt(pb))[:, None]
+        m *= vh[None, :]
+        out[i] = m
+    return out

@@ -1,0 +1,188 @@
+"""Pins of the tetrahedron-weight oracle (CPU only).
+
+The C restatement (oracle/tetra_oracle.c) is checked against
+  * PyGW's own Fortran (golden fixtures; live when oracle/_ref is built),
+  * the flat-band closed form and the Taylor-series moments,
+  * an independent cubature of the weight integral (exact mode), and
+  * an independent half-space clipping of the metal polyhedron.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oracle import cubature, ref, tetra
+from pygw_amd import mesh, synth
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture
+def exact_mode():
+    L = tetra.lib()
+    a = C.c_int.in_dll(L, "orc_compat_f7b")
+    b = C.c_int.in_dll(L, "orc_compat_nnod78")
+    old = (a.value, b.value)
+    a.value, b.value = 0, 0
+    yield
+    a.value, b.value = old
+
+
+def _gold(name):
+    return np.load(os.path.join(GOLD, name + ".npz"))
+
+
+@pytest.mark.parametrize("name", ["tetra_insulator", "tetra_metal", "tetra_metal_b"])
+def test_oracle_equals_reference_fixture(name):
+    g = _gold(name)
+    ncg, nomx, numin = (int(x) for x in g["win"])
+    kcw, st, fl = tetra.bz_calcqdepw(g["enk"], 0.0, g["kqid"], ncg, nomx, numin, g["omega"], g["tetc"])
+    assert st >= 0 and not (fl & tetra.ORC_F_F7A)
+    np.testing.assert_array_equal(kcw, g["kcw"])
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_oracle_equals_reference_live(seed):
+    cfg = synth.small_config(div=(4, 4, 4), n_occ=4, n_emp=6, n_metal=3 if seed else 0, n_om=8, seed=seed)
+    b = synth.band_energies(cfg)
+    om, _ = synth.omega_mesh(cfg)
+    tetc = mesh.tetra_corners(cfg.div)
+    kq = mesh.k_minus_q(cfg.div)[:, 1 + seed]
+    k1, st, fl = tetra.bz_calcqdepw(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc)
+    assert st >= 0
+    if fl & tetra.ORC_F_F7A:
+        pytest.skip("bipyramid case: reference undefined")
+    k2 = ref.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc)
+    np.testing.assert_array_equal(k1, k2)
+
+
+def test_flat_band_closed_form():
+    """Delta constant everywhere -> kcw = -2D/(N_k(w^2+D^2)) below the Taylor
+    switch, and the 1/w^4-truncated series above it."""
+    div = (3, 4, 2)
+    nk = int(np.prod(div))
+    d = 0.37
+    enk = np.asfortranarray(np.vstack([np.full(nk, -0.1), np.full(nk, -0.1 + d)]))
+    om = np.array([0.001, 0.05, 0.3, 1.0, 3.0, 3.69, 3.71, 10.0, 80.0])
+    tetc = mesh.tetra_corners(div)
+    kq = mesh.k_minus_q(div)[:, 5]
+    kcw, st, fl = tetra.bz_calcqdepw(enk, 0.0, kq, 0, 0, 1, om, tetc)
+    assert st == 0
+    got = kcw[0, 0]  # [nk, nom]
+    exact = cubature.flat_band_kcw(d, om, nk)
+    taylor = (-2 * d / om ** 2 + 2 * d ** 3 / om ** 4) / nk
+    want = np.where(om > 10 * d, taylor, exact)
+    np.testing.assert_allclose(got, np.broadcast_to(want, got.shape), rtol=1e-12, atol=0)
+
+
+def test_corner_sum_rule_matches_cubature():
+    """sum_i w_i = int_T g dV is unchanged by the reference's corner
+    exchange, so it pins the compat-mode weights directly."""
+    rng = np.random.default_rng(9)
+    om = np.array([0.02, 0.2, 0.9])
+    for _ in range(40):
+        e = rng.uniform(-0.6, -0.1, 4)
+        f = rng.uniform(0.1, 0.6, 4)
+        w, st, _, _, nn = tetra.convw1t(e, f, 0.0, om)
+        assert st == 0 and nn == 4
+        d = f - e
+        dd = np.abs(d[:, None] - d[None, :]) * 2 / (np.abs(d[:, None]) + np.abs(d[None, :]))
+        if (dd[np.triu_indices(4, 1)] < 0.05).any():
+            continue  # grouped classes average Delta: pinned separately
+        ex = cubature.unit_tet_weights(d, om)
+        np.testing.assert_allclose(w.sum(0), ex.sum(0), rtol=1e-7)
+
+
+def test_exact_mode_matches_cubature(exact_mode):
+    rng = np.random.default_rng(4)
+    om = np.array([0.01, 0.1, 0.5, 0.9])
+    n = 0
+    while n < 30:
+        e = rng.uniform(-0.6, -0.1, 4)
+        f = rng.uniform(0.1, 0.6, 4)
+        d = f - e
+        dd = np.abs(d[:, None] - d[None, :]) * 2 / (np.abs(d[:, None]) + np.abs(d[None, :]))
+        if (dd[np.triu_indices(4, 1)] < 0.05).any():
+            continue
+        n += 1
+        w, st, _, _, nn = tetra.convw1t(e, f, 0.0, om)
+        ex = cubature.unit_tet_weights(d, om)
+        mask = om <= 10 * np.abs(d).max()
+        np.testing.assert_allclose(w[:, mask], ex[:, mask], rtol=1e-7, atol=1e-13)
+
+
+@pytest.mark.parametrize("pattern", [(0, 0, 1, 2), (0, 0, 1, 1), (0, 0, 0, 1), (0, 0, 0, 0)])
+def test_degenerate_classes_exact(exact_mode, pattern):
+    """Exactly equal Delta's: classes 6/8/10/16 are the analytic limits."""
+    vals = np.array([0.31, 0.52, 0.77])
+    d = vals[list(pattern)]
+    e = np.array([-0.2, -0.25, -0.15, -0.3])
+    f = e + d
+    om = np.array([0.01, 0.3, 1.0, 2.0])
+    w, st, _, _, _ = tetra.convw1t(e, f, 0.0, om)
+    assert st == 0
+    ex = cubature.unit_tet_weights(d, om, n=64)
+    mask = om <= 10 * np.abs(d).max()
+    np.testing.assert_allclose(w[:, mask], ex[:, mask], rtol=1e-7, atol=1e-13)
+
+
+def test_reference_exchanges_corners_2_and_4():
+    """Default (reference) mode: class-4 weights of big-tet corners 2 and 4
+    come out exchanged (for_tetrahedra3.f90:482-486)."""
+    e = np.array([-0.21, -0.33, -0.17, -0.28])
+    d = np.array([0.61, 0.72, 0.52, 0.45])
+    om = np.array([0.05, 0.7])
+    w, st, _, _, _ = tetra.convw1t(e, e + d, 0.0, om)
+    ex = cubature.unit_tet_weights(d, om)
+    np.testing.assert_allclose(w[[0, 3, 2, 1]], ex, rtol=1e-7)
+    assert np.abs(w[1] - ex[1]).max() > 1e-4
+
+
+def test_taylor_branch_moments():
+    """Above w > 10 max|Delta| the weights are the exact moments of
+    -2D/w^2 + 2D^3/w^4 (Dirichlet moments; cubature is exact on them)."""
+    d = np.array([0.11, 0.13, 0.09, 0.12])
+    e = np.full(4, -0.05)
+    om = np.array([2.0, 7.5])
+    w, st, _, _, _ = tetra.convw1t(e, e + d, 0.0, om)
+    pts, wts = cubature.rule(16)
+    lam = np.stack([1 - pts.sum(1), pts[:, 0], pts[:, 1], pts[:, 2]], 1)
+    x = lam @ d
+    for j, o in enumerate(om):
+        want = lam.T @ ((-2 * x / o ** 2 + 2 * x ** 3 / o ** 4) * wts)
+        np.testing.assert_allclose(w[:, j], want, rtol=1e-12)
+
+
+def test_metal_region_matches_clipping(exact_mode):
+    """Polyhedra with 4 and 6 nodes: the node geometry (surfnodes..sortnodes)
+    against an independent half-space clipping, at frequencies where the
+    closed forms are well conditioned."""
+    rng = np.random.default_rng(12)
+    om = np.array([0.3, 1.0, 2.0])
+    seen = set()
+    for _ in range(400):
+        e = rng.uniform(-1, 0.5, 4)
+        f = rng.uniform(-0.5, 1, 4)
+        if f.max() <= 0 or e.min() > 0:
+            continue
+        w, st, _, fl, nn = tetra.convw1t(e, f, 0.0, om)
+        assert st == 0
+        if nn not in (4, 6):
+            continue
+        ex = cubature.region_weights(e, f, om, n=24)
+        np.testing.assert_allclose(w, ex, rtol=0, atol=3e-3 * np.abs(ex).max() + 1e-12)
+        seen.add(nn)
+    assert seen == {4, 6}
+
+
+def test_sorteq_classes():
+    cases = [([1.0, 2.0, 3.0, 4.0], 4), ([1.0, 1.001, 3.0, 4.0], 6), ([1.0, 3.0, 1.001, 3.002], 8),
+             ([1.0, 1.001, 1.002, 4.0], 10), ([2.0, 2.001, 2.0005, 2.0015], 16), ([3.0, 1.0, 1.001, 1.002], 10)]
+    for v, cls in cases:
+        out, idx, c, st = tetra.sorteq(v)
+        assert st == 0 and c == cls
+        np.testing.assert_array_equal(out, np.asarray(v)[idx])
+    out, idx, c, _ = tetra.sorteq([3.0, 1.0, 1.001, 1.002])
+    assert abs(out[3] - 3.0) < 1e-15  # the odd one goes last in class 10
