@@ -1,0 +1,106 @@
+/*
+ * pygw_hip.h -- C ABI of libpygw_hip.so, the MI355X (gfx950) backend of
+ * PyGW's polarization -> dielectric inversion -> Sigma_c path.
+ *
+ * Conventions for every entry point:
+ *   - array arguments are DEVICE pointers (HIP global memory), caller-owned;
+ *   - integer index arrays are int32 and 0-based, as PyGW passes them to
+ *     its Fortran (for_tetrahedra.f90:445-456 adds the 1);
+ *   - `stream` is a hipStream_t (NULL = default stream);
+ *   - the return value is 0 on success, < 0 on error:
+ *       -10 invalid argument, -11 HIP runtime error,
+ *       -12 geometry/classification error (where PyGW's Fortran would `stop`),
+ *       -13 matrix not Hermitian positive definite;
+ *   - where an `info` pointer is given (host memory), the call synchronises
+ *     `stream` and fills it; pass NULL to stay asynchronous;
+ *   - `workspace` may be NULL (the library then allocates on `stream`);
+ *     otherwise it must hold the number of bytes the *_workspace query
+ *     returns.
+ * All floating-point outputs are FP64 (complex outputs interleaved re,im).
+ */
+#ifndef PYGW_HIP_H
+#define PYGW_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library version (major*10000 + minor*100 + patch). */
+int pygw_version(void);
+
+/* ------------------------------------------------------------------------
+ * Tetrahedron polarization weights.
+ * Replaces ft.bz_calcqdepw_par2(enk, ef, kqid, ncg, nomx, numin, omega,
+ * tetc) (for_tetrahedra.f90:390-464; called at pygw.py:3443).
+ *   enk    [n_bnd, n_k]  Fortran order (band fastest), Hartree
+ *   kqid   [n_k]         index of k-q for each k (kqid[:, iq])
+ *   tetc   [4, n_tet]    Fortran order, n_tet = 6*n_k
+ *   omega  [n_om]        Matsubara frequencies (Hartree)
+ *   kcw    layout 0: [n_o, n_e, n_k, n_om] Fortran order (PyGW's array),
+ *          layout 1: [n_k][n_om][n_o*n_e] with pair p = i_o*n_e + i_e
+ *          (the pair order of pygw.py:3170,3182), where
+ *          n_o = ncg+nomx+1, n_e = n_bnd-ncg-numin.
+ *   compat bit 0: reproduce PyGW's class-4 vertex rotation
+ *                 (for_tetrahedra3.f90:482-486; weights of corners 2 and 4
+ *                 exchanged); bit 1: reproduce the wtemp(4,2,nom) aliasing
+ *                 of the nnod = 7, 8 polyhedra (for_tetrahedra3.f90:33,
+ *                 105-118).  3 = PyGW's results, 0 = the exact integrals.
+ *   info   (optional) [2]: status bits (1 geometry error, 2 nnod=5
+ *          bipyramid, 4 malformed tetc), number of check_weight warnings.
+ * ------------------------------------------------------------------------ */
+int pygw_tetra_weights_workspace(int n_k, size_t* bytes);
+int pygw_tetra_weights(const double* enk, double ef, const int* kqid, const int* tetc, int ncg, int nomx,
+                       int numin, const double* omega, int n_om, int n_k, int n_bnd, int n_tet, int layout,
+                       int compat, double* kcw, void* workspace, size_t ws_bytes, int* info, void* stream);
+
+/* ------------------------------------------------------------------------
+ * chi0 contraction (calc_eps, pygw.py:3161-3200):
+ *   eps[I,J,w] (+)= sum_k sum_p coef*kcw[k][w][p] * M[k][p][I] * conj(M[k][p][J])
+ * Replaces the per-(k, w) `la.matmul(tmat, cminm2)` ZGEMMs with
+ * tmat = minm2.T*coef*kcw (pygw.py:3182-3185).
+ *   M     [n_k][n_pairs][pb], k stride `stride_k` elements, holding the
+ *         k-points k0..k0+n_k-1 (PyGW's minm2 = reshape(minm,
+ *         (nb1*nb2, matsiz)) per k, v^1/2 folded in as at pygw.py:2666);
+ *         complex128 for precision 0, complex64 for precision 1
+ *   kcw   [N_k][n_om][n_pairs] (layout 1 of pygw_tetra_weights), all k
+ *   eps   complex128 Fortran [pb, pb, n_om]; beta = 0 overwrite, 1 add
+ *   precision 0: FP64 on v_mfma_f64_16x16x4f64
+ *   precision 1: FP16 hi+lo split with per-column and per-frequency
+ *                power-of-two scaling, 3 products on
+ *                v_mfma_f32_16x16x32_f16, FP32 partial sums folded into
+ *                FP64 across k
+ * ------------------------------------------------------------------------ */
+int pygw_chi0_workspace(int pb, int n_om, int precision, size_t* bytes);
+int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, const double* kcw, int n_om, int k0,
+                         int n_k, double coef, int precision, int beta, void* eps, void* workspace,
+                         size_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Dielectric inversion (calc_eps, pygw.py:3229-3271, q != 0):
+ *   eps[:,:,w] <- ((eps + eps^H)/2 + 1)^{-1} - 1   for every w, in place.
+ * Replaces `linalg.inv` (pygw.py:3244); block Gauss-Jordan in FP64 without
+ * pivoting (the matrix is Hermitian positive definite).
+ *   info (optional) [1]: non-zero if some pivot was not positive
+ *   (return -13).
+ * ------------------------------------------------------------------------ */
+int pygw_eps_inverse_workspace(int pb, int n_om, size_t* bytes);
+int pygw_eps_inverse(void* eps, int pb, int n_om, void* workspace, size_t ws_bytes, int* info, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Synthetic matrix elements for benchmarks and tests.  No counterpart in
+ * PyGW (its M comes from calc_minm, pygw.py:2547-2721); for callers that
+ * need seeded inputs of the benchmark shapes without a DFT calculation.
+ *   out[k-k0][p][I], k = k0..k0+n_kc-1, p = i_o*n_e + i_e:
+ *   complex Gaussian(seed, counter) * exp(-|e_o(k)-e_e(k-q)|/20 eV)
+ *   / sqrt(pb) * vhalf[I]; complex128 if complex128 != 0 else complex64.
+ * ------------------------------------------------------------------------ */
+int pygw_synth_minm(const double* enk, int n_bnd, const int* kqid, int ncg, int numin, int n_o, int n_e, int pb,
+                    const double* vhalf, unsigned long long seed, int k0, int n_kc, int complex128, void* out,
+                    void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYGW_HIP_H */

@@ -1,0 +1,65 @@
+"""ctypes loader of libpygw_hip.so (the gfx950 backend).
+
+There is no CPU fallback: if the library or a HIP device is missing, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpygw_hip.so")
+_lib = None
+
+ERRORS = {-10: "invalid argument", -11: "HIP runtime error", -12: "geometry/classification error",
+          -13: "matrix not Hermitian positive definite"}
+
+
+class PygwError(RuntimeError):
+    def __init__(self, fn, code):
+        super().__init__(f"{fn} failed: {ERRORS.get(code, 'error')} ({code})")
+        self.code = code
+
+
+_SIGS = {
+    "pygw_version": (C.c_int, []),
+    "pygw_tetra_weights_workspace": (C.c_int, [C.c_int, C.POINTER(C.c_size_t)]),
+    "pygw_tetra_weights": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                     C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                     C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
+    "pygw_chi0_workspace": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
+    "pygw_chi0_accumulate": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                       C.c_int, C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t,
+                                       C.c_void_p]),
+    "pygw_eps_inverse_workspace": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
+    "pygw_eps_inverse": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p,
+                                   C.c_void_p]),
+    "pygw_synth_minm": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_void_p, C.c_ulonglong, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def call(name, *args):
+    rc = getattr(lib(), name)(*args)
+    if rc < 0:
+        raise PygwError(name, rc)
+    return rc

@@ -1,0 +1,128 @@
+"""Host side of the gfx950 polarization path, mirroring PyGW's call sites.
+
+Each function takes the arguments of the reference routine it replaces and
+runs the HIP kernels of libpygw_hip.so on the current torch device/stream.
+torch only provides device memory and streams.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+
+COMPAT_REFERENCE = 3   # reproduce PyGW exactly (both switches)
+COMPAT_EXACT = 0       # exact corner weights (see include/pygw_hip.h)
+
+
+def _dev(x, dtype, device):
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device, dtype=dtype)
+    return torch.as_tensor(np.asarray(x), dtype=dtype, device=device)
+
+
+def _stream_ptr(device):
+    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _require_device(device):
+    if not torch.cuda.is_available():
+        raise RuntimeError("the gfx950 polarization path needs a HIP device")
+    return torch.device(device)
+
+
+def bz_calcqdepw_par2(enk, ef, kqid, ncg, nomx, numin, omega, tetc, *, layout="fortran",
+                      compat=COMPAT_REFERENCE, device="cuda", check=True):
+    """Polarization weights kcw (ft.bz_calcqdepw_par2, for_tetrahedra.f90:390).
+
+    enk [nbnd, nkp] (band index first, as PyGW's Fortran-ordered array),
+    kqid [nkp] (= kqm.kqid[:, iq]), tetc [4, ntet]; 0-based indices.
+    layout="fortran": returns a [n_o, n_e, nkp, nom] tensor with Fortran
+    strides (PyGW's kcw); layout="native": [nkp, nom, n_o*n_e] contiguous.
+    With check=True the call synchronises and raises on geometry errors;
+    the check_weight warning count is stored in `bz_calcqdepw_par2.nwarn`.
+    """
+    device = _require_device(device)
+    enk_t = _dev(np.asarray(enk).T if not isinstance(enk, torch.Tensor) else enk.T, torch.float64, device)
+    enk_t = enk_t.contiguous()  # [nkp, nbnd] row-major == Fortran (nbnd, nkp)
+    nkp, nbnd = enk_t.shape
+    tetc_np = np.asarray(tetc) if not isinstance(tetc, torch.Tensor) else tetc.cpu().numpy()
+    tetc_t = torch.as_tensor(np.ascontiguousarray(tetc_np.T), dtype=torch.int32, device=device)  # [ntet, 4]
+    ntet = tetc_t.shape[0]
+    kq_t = _dev(kqid, torch.int32, device).contiguous()
+    om_t = _dev(omega, torch.float64, device).contiguous()
+    nom = om_t.numel()
+    no, ne = ncg + nomx + 1, nbnd - ncg - numin
+    if no <= 0 or ne <= 0:
+        raise ValueError("empty band window")
+    buf = torch.empty(no * ne * nkp * nom, dtype=torch.float64, device=device)
+    lay = 0 if layout == "fortran" else 1
+    info = (C.c_int * 2)()
+    _lib.call("pygw_tetra_weights", C.c_void_p(enk_t.data_ptr()), float(ef), C.c_void_p(kq_t.data_ptr()),
+              C.c_void_p(tetc_t.data_ptr()), int(ncg), int(nomx), int(numin), C.c_void_p(om_t.data_ptr()), nom, nkp,
+              nbnd, ntet, lay, int(compat), C.c_void_p(buf.data_ptr()), None, 0,
+              C.cast(info, C.c_void_p) if check else None, _stream_ptr(device))
+    if check:
+        bz_calcqdepw_par2.nwarn = info[1]
+        bz_calcqdepw_par2.status = info[0]
+        if info[0] & 4:
+            raise _lib.PygwError("pygw_tetra_weights", -10)
+    if lay == 0:
+        return buf.as_strided((no, ne, nkp, nom), (1, no, no * ne, no * ne * nkp))
+    return buf.view(nkp, nom, no * ne)
+
+
+bz_calcqdepw_par2.nwarn = 0
+bz_calcqdepw_par2.status = 0
+
+
+def _fortran_view(buf, pb, nom):
+    return buf.as_strided((pb, pb, nom), (1, pb, pb * pb))
+
+
+def chi0_accumulate(M, kcw_native, coef=-2.0, *, eps=None, precision="fp64", k0=0, device="cuda"):
+    """chi0 contraction of calc_eps (pygw.py:3161-3200).
+
+    M: complex [n_kc, n_pairs, PB] (minm2 of the k-points k0..k0+n_kc-1,
+    pair p = i_o*n_e + i_e); kcw_native: [N_k, n_om, n_pairs] (weights in
+    layout "native", all k).  precision "fp64" (M complex128) or "mixed"
+    (M complex64, FP16 hi+lo split on MFMA).  Returns eps as a complex128
+    [PB, PB, n_om] tensor with Fortran strides; with eps given (same
+    layout), the contribution is added to it.
+    """
+    device = _require_device(device)
+    dt = {"fp64": torch.complex128, "mixed": torch.complex64}[precision]
+    M = M.to(device=device, dtype=dt).contiguous() if isinstance(M, torch.Tensor) else \
+        torch.as_tensor(np.ascontiguousarray(M), dtype=dt, device=device)
+    nkc, npair, pb = M.shape
+    kc = _dev(kcw_native, torch.float64, device).contiguous()
+    if kc.shape[2] != npair or kc.shape[0] < k0 + nkc:
+        raise ValueError("kcw_native must be [N_k, n_om, n_pairs] covering k0..k0+n_kc-1")
+    nom = kc.shape[1]
+    beta = 0
+    if eps is None:
+        buf = torch.empty(pb * pb * nom, dtype=torch.complex128, device=device)
+    else:
+        if eps.stride() != (1, pb, pb * pb) or eps.shape != (pb, pb, nom):
+            raise ValueError("eps must be a Fortran-strided [PB, PB, n_om] complex128 tensor")
+        buf = eps.as_strided((pb * pb * nom,), (1,))
+        beta = 1
+    prec = {"fp64": 0, "mixed": 1}[precision]
+    _lib.call("pygw_chi0_accumulate", C.c_void_p(M.data_ptr()), npair * pb, npair, pb,
+              C.c_void_p(kc.data_ptr()), nom, int(k0), nkc, float(coef), prec, beta, C.c_void_p(buf.data_ptr()),
+              None, 0, _stream_ptr(device))
+    return _fortran_view(buf, pb, nom)
+
+
+def eps_inverse(eps, *, device="cuda", check=True):
+    """In place: eps[:,:,w] <- ((eps+eps^H)/2 + 1)^-1 - 1 (pygw.py:3229-3271)."""
+    device = _require_device(device)
+    pb, pb2, nom = eps.shape
+    if pb != pb2 or eps.stride() != (1, pb, pb * pb) or eps.dtype != torch.complex128:
+        raise ValueError("eps must be a Fortran-strided [PB, PB, n_om] complex128 tensor")
+    info = (C.c_int * 1)()
+    _lib.call("pygw_eps_inverse", C.c_void_p(eps.data_ptr()), pb, nom, None, 0,
+              C.cast(info, C.c_void_p) if check else None, _stream_ptr(device))
+    return eps

@@ -46,9 +46,11 @@ def main():
         np.savez_compressed(os.path.join(HERE, name + ".npz"), enk=b.enk, kqid=kq, tetc=tetc, omega=om,
                             win=np.array([b.ncg, b.nomx, b.numin]), kcw=kcw)
         print(name, kcw.shape)
-    # Pade (for_pade.f90:3-51) on a two-pole function sampled at i*omega_n
-    z = 1j * np.geomspace(0.005, 5.0, 24)
-    y = 0.3 / (z - 0.2 + 0.05j) + 0.7 / (z + 0.5 - 0.02j)
+    # Pade (for_pade.f90:3-51) on a five-pole function sampled at i*omega_n
+    z = 1j * np.geomspace(0.02, 5.0, 10)
+    poles = np.array([0.2 - 0.05j, -0.5 + 0.02j, 0.9 - 0.3j, -1.3 + 0.4j, 0.05 - 0.6j])
+    res = np.array([0.3, 0.7, 0.25, 0.4, 0.15])
+    y = (res[None, :] / (z[:, None] - poles[None, :])).sum(1)
     apar = ref.padecof(y, z)
     x = np.linspace(-1, 1, 9) + 0.0j
     g = ref.padeg(x, z, apar)
