@@ -51,6 +51,14 @@ int pygw_version(void);
  *          bipyramid, 4 malformed tetc), number of check_weight warnings.
  * ------------------------------------------------------------------------ */
 int pygw_tetra_weights_workspace(int n_k, size_t* bytes);
+
+/* SVD-basis weights: replaces ft.bz_calcqdepw_par3 (for_tetrahedra.f90:466-542,
+ * called at pygw.py:3441) as kcw_l = kcw . dUl applied to the layout-1
+ * output of pygw_tetra_weights.  dUl Fortran (n_om, n_l) as built by
+ * svd_functions (pygw.py:3478-3501).  out layout 0: Fortran
+ * [n_o, n_e, n_k, n_l]; layout 1: [n_k][n_l][n_o*n_e]. */
+int pygw_tetra_weights_svd(const double* kcw_native, int n_k, int n_om, int n_o, int n_e, const double* dUl,
+                           int n_l, int layout, double* out, void* stream);
 int pygw_tetra_weights(const double* enk, double ef, const int* kqid, const int* tetc, int ncg, int nomx,
                        int numin, const double* omega, int n_om, int n_k, int n_bnd, int n_tet, int layout,
                        int compat, double* kcw, void* workspace, size_t ws_bytes, int* info, void* stream);
@@ -87,6 +95,51 @@ int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, cons
  * ------------------------------------------------------------------------ */
 int pygw_eps_inverse_workspace(int pb, int n_om, size_t* bytes);
 int pygw_eps_inverse(void* eps, int pb, int n_om, void* workspace, size_t ws_bytes, int* info, void* stream);
+
+/* ------------------------------------------------------------------------
+ * MWM contraction (calc_selfc, pygw.py:3353-3358):
+ *   mwm[k][w][r] = wkq * sum_{I,J} conj(M[k][r][I]) W[I,J,w] M[k][r][J]
+ * Replaces `la.matmul(cminm2, eps[:,:,iom])` and the row sum.
+ *   M    complex128 [n_k][rows_per_k][pb], rows r = ib1*nb2 + ib2 (minm2 of
+ *        the self-energy bands, pygw.py:3349-3351)
+ *   W    complex128 Fortran [pb, pb, n_om] (eps after pygw_eps_inverse)
+ *   out  complex128 [n_k][n_om][rows_per_k] (PyGW's mwm[iom, ib1, ib2] per k)
+ *   precision 0 = FP64 MFMA
+ * ------------------------------------------------------------------------ */
+int pygw_mwm(const void* M, int n_k, int rows_per_k, int pb, const void* W, int n_om, double wkq, int precision,
+             void* mwm_out, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Spline refinement of mwm onto the precise mesh (Compute_selfc_inside,
+ * mcommon.py:126-140): cubic spline of (w^2+1) mwm with y''(w_0) = 0 and
+ * y'(w_max) = 0, extrapolated at the ends, divided by (w_p^2+1).
+ *   mwm    complex128 [n_k][n_om][nb1][nb2]
+ *   mwm_p  complex128 [n_k][nb1][nb2][n_omp]
+ * ------------------------------------------------------------------------ */
+int pygw_selfc_spline(const void* mwm, int n_k, int nb1, int nb2, const double* omega, int n_om,
+                      const double* omega_p, int n_omp, void* mwm_p, void* stream);
+
+/* ------------------------------------------------------------------------
+ * G0 (x) W convolution on the imaginary axis.  Replaces
+ * fnc.all_fr_convolutions (mwm_p = NULL, coarse mesh; fnc.f90:139-159) and
+ * the loop over fnc.few_fr_convolutions (mcommon.py:141-144, fnc.f90:161-176).
+ *   enk    [n_k][nb2] energies of the k-q states (Hartree)
+ *   sc_out complex128 [n_k][nb1][n_om] (PyGW's sc_p[irk, ie1, iom])
+ * ------------------------------------------------------------------------ */
+int pygw_selfc_convolve(const double* enk, int n_k, int nb1, int nb2, const void* mwm, const double* omega,
+                        int n_om, const void* mwm_p, const double* omega_p, const double* womeg_p, int n_omp,
+                        void* sc_out, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Pade continuation and quasiparticle energies (AnalyticContinuation
+ * iop_ac = 0, mcommon.py:330-338, with fpade.padecof/padeg,
+ * for_pade.f90:3-51; Compute_quasiparticles G0W0 update, mcommon.py:182-203).
+ *   sigc  complex128 [n_state][n_om]; the first n_pade points are used
+ *   ene, sigx, vxc  [n_state] (sigx, vxc may be NULL = 0)
+ *   out   [n_state][6]: Re S(e), Im S(e), Re dS/de, Im dS/de, Z, E_qp
+ * ------------------------------------------------------------------------ */
+int pygw_pade_qp(const void* sigc, int n_state, int n_om, const double* omega, int n_pade, const double* ene,
+                 const double* sigx, const double* vxc, double* out, void* stream);
 
 /* ------------------------------------------------------------------------
  * Synthetic matrix elements for benchmarks and tests.  No counterpart in

@@ -1,0 +1,43 @@
+// C-ABI for the Sigma_c part of the path (include/pygw_hip.h).
+#include "common.h"
+
+int mwm_f64_launch(const void* M, int nrows, int pb, const void* W, int nom, double wkq, int rpk, void* out,
+                   hipStream_t stream);
+int selfc_spline_launch(const void* mwm, int nrows, int nb1, int nom, int nb2, const double* x, const double* xp,
+                        int nomp, void* out, hipStream_t stream);
+int selfc_conv_launch(const double* enk, int nrows, int nb1, int nb2, const void* mwm, int nom, const double* omega,
+                      const void* mwmp, int nomp, const double* omp, const double* womp, void* out,
+                      hipStream_t stream);
+int pade_qp_launch(const void* sig, int nstate, int nom, const double* omega, int npts, const double* ene,
+                   const double* sigx, const double* vxc, double* out, hipStream_t stream);
+
+extern "C" int pygw_mwm(const void* M, int n_k, int rows_per_k, int pb, const void* W, int n_om, double wkq,
+                        int precision, void* mwm_out, void* stream) {
+  if (!M || !W || !mwm_out || n_k <= 0 || rows_per_k <= 0 || pb <= 0 || n_om <= 0) return PYGW_E_ARG;
+  if ((long)n_k * rows_per_k > (1L << 31) - 1) return PYGW_E_ARG;
+  if (precision != 0) return PYGW_E_ARG;
+  return mwm_f64_launch(M, n_k * rows_per_k, pb, W, n_om, wkq, rows_per_k, mwm_out, (hipStream_t)stream);
+}
+
+extern "C" int pygw_selfc_spline(const void* mwm, int n_k, int nb1, int nb2, const double* omega, int n_om,
+                                 const double* omega_p, int n_omp, void* mwm_p, void* stream) {
+  if (!mwm || !omega || !omega_p || !mwm_p || n_k <= 0 || nb1 <= 0 || nb2 <= 0 || n_omp <= 0) return PYGW_E_ARG;
+  return selfc_spline_launch(mwm, n_k * nb1, nb1, n_om, nb2, omega, omega_p, n_omp, mwm_p, (hipStream_t)stream);
+}
+
+extern "C" int pygw_selfc_convolve(const double* enk, int n_k, int nb1, int nb2, const void* mwm, const double* omega,
+                                   int n_om, const void* mwm_p, const double* omega_p, const double* womeg_p,
+                                   int n_omp, void* sc_out, void* stream) {
+  if (!enk || !mwm || !omega || !omega_p || !womeg_p || !sc_out || n_k <= 0 || nb1 <= 0 || nb2 <= 0 || n_om <= 0)
+    return PYGW_E_ARG;
+  if (!mwm_p && n_omp != n_om) return PYGW_E_ARG;
+  if ((long)n_k * nb1 > 2147483647L || n_om > 65535) return PYGW_E_ARG;
+  return selfc_conv_launch(enk, n_k * nb1, nb1, nb2, mwm, n_om, omega, mwm_p, n_omp, omega_p, womeg_p, sc_out,
+                           (hipStream_t)stream);
+}
+
+extern "C" int pygw_pade_qp(const void* sigc, int n_state, int n_om, const double* omega, int n_pade,
+                            const double* ene, const double* sigx, const double* vxc, double* out, void* stream) {
+  if (!sigc || !omega || !ene || !out || n_state <= 0 || n_om <= 0) return PYGW_E_ARG;
+  return pade_qp_launch(sigc, n_state, n_om, omega, n_pade, ene, sigx, vxc, out, (hipStream_t)stream);
+}

@@ -7,6 +7,9 @@ __global__ void pygw_tw_gather(const double* enk, int nbnd, double ef, const int
                                int ne, const double* omega, int nom, const int* tetc, int nk, int ntet, const int* inc,
                                const int* cnt, int maxinc, int compat, int layout, double* kcw, int* dstat);
 
+__global__ void pygw_tw_project(const double* kcw, int np, int nom, int nk, const double* dUl, int nl, int layout,
+                                int no, int ne, double* out);
+
 static const int kMaxInc = 24;
 
 static size_t tw_ws_bytes(int n_k) {
@@ -64,4 +67,14 @@ extern "C" int pygw_tetra_weights(const double* enk, double ef, const int* kqid,
   }
   if (own) HIP_TRY(hipFreeAsync(ws, stream));
   return rc;
+}
+
+extern "C" int pygw_tetra_weights_svd(const double* kcw_native, int n_k, int n_om, int n_o, int n_e, const double* dUl,
+                                      int n_l, int layout, double* out, void* stream) {
+  if (!kcw_native || !dUl || !out || n_k <= 0 || n_om <= 0 || n_o <= 0 || n_e <= 0 || n_l <= 0) return PYGW_E_ARG;
+  if ((layout != 0 && layout != 1) || n_k > 65535) return PYGW_E_ARG;
+  const int np = n_o * n_e;
+  hipLaunchKernelGGL(pygw_tw_project, dim3(ceil_div(np, 256), n_k), dim3(256), 0, (hipStream_t)stream, kcw_native,
+                     np, n_om, n_k, dUl, n_l, layout, n_o, n_e, out);
+  return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }

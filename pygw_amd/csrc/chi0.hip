@@ -57,59 +57,73 @@ __global__ void __launch_bounds__(256) pygw_chi0_f64(const double2* __restrict__
         accI[w][a][b] = d4{0.0, 0.0, 0.0, 0.0};
       }
 
-  for (int k = 0; k < nk; k++) {
+  // chunks c = (k, p0) walked in order; the next chunk's global loads are
+  // issued before the MFMA work on the current one (register prefetch)
+  const int npc = (np + KC - 1) / KC;
+  const long nch = (long)nk * npc;
+  constexpr int NQ = (KC * TB) / 256;
+  double2 pa[NQ], pv[NQ];
+  double pw = 0.0;
+  auto fetch = [&](long ch) {
+    const int k = (int)(ch / npc), p0 = (int)(ch % npc) * KC;
     const double2* Mk = M + (size_t)k * strideK;
-    const double* Kk = kcw + (size_t)k * nom * np;
-    for (int p0 = 0; p0 < np; p0 += KC) {
 #pragma unroll
-      for (int q = 0; q < (KC * TB) / 256; q++) {
-        const int e = tid + 256 * q;
-        const int r = e / TB, c = e % TB;
-        const int p = p0 + r;
-        double2 va = {0.0, 0.0}, vb = {0.0, 0.0};
-        if (p < np) {
-          if (I0 + c < pb) va = Mk[(size_t)p * pb + I0 + c];
-          if (J0 + c < pb) vb = Mk[(size_t)p * pb + J0 + c];
-        }
-        As[r][c] = va;
-        Bs[r][c] = vb;
+    for (int q = 0; q < NQ; q++) {
+      const int e = tid + 256 * q;
+      const int r = e / TB, c = e % TB;
+      const int p = p0 + r;
+      pa[q] = double2{0.0, 0.0};
+      pv[q] = double2{0.0, 0.0};
+      if (p < np) {
+        if (I0 + c < pb) pa[q] = Mk[(size_t)p * pb + I0 + c];
+        if (J0 + c < pb) pv[q] = Mk[(size_t)p * pb + J0 + c];
       }
-      if (tid < NW * KC) {
-        const int w = tid / KC, r = tid % KC;
-        double v = 0.0;
-        if (p0 + r < np && w0 + w < nom) v = coef * Kk[(size_t)(w0 + w) * np + p0 + r];
-        Ws[w][r] = v;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int ks = 0; ks < KC / 4; ks++) {
-        const int kk = ks * 4 + lk;
-        double ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-        for (int t = 0; t < 2; t++) {
-          const double2 a = As[kk][32 * wr + 16 * t + li];
-          const double2 b = Bs[kk][32 * wc + 16 * t + li];
-          ar[t] = a.x; ai[t] = a.y; br[t] = b.x; bi[t] = b.y;
-        }
-#pragma unroll
-        for (int w = 0; w < NW; w++) {
-          const double wv = Ws[w][kk];
-          double bwr[2], bwi[2];
-#pragma unroll
-          for (int t = 0; t < 2; t++) { bwr[t] = br[t] * wv; bwi[t] = bi[t] * wv; }
-#pragma unroll
-          for (int a = 0; a < 2; a++)
-#pragma unroll
-            for (int b = 0; b < 2; b++) {
-              accR[w][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bwr[b], accR[w][a][b], 0, 0, 0);
-              accR[w][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bwi[b], accR[w][a][b], 0, 0, 0);
-              accI[w][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bwr[b], accI[w][a][b], 0, 0, 0);
-              accI[w][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[a], bwi[b], accI[w][a][b], 0, 0, 0);
-            }
-        }
-      }
-      __syncthreads();
     }
+    pw = 0.0;
+    if (tid < NW * KC) {
+      const int w = tid / KC, r = tid % KC;
+      if (p0 + r < np && w0 + w < nom) pw = coef * kcw[((size_t)k * nom + (w0 + w)) * np + p0 + r];
+    }
+  };
+  fetch(0);
+  for (long ch = 0; ch < nch; ch++) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const int e = tid + 256 * q;
+      As[e / TB][e % TB] = pa[q];
+      Bs[e / TB][e % TB] = pv[q];
+    }
+    if (tid < NW * KC) Ws[tid / KC][tid % KC] = pw;
+    __syncthreads();
+    if (ch + 1 < nch) fetch(ch + 1);
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ks++) {
+      const int kk = ks * 4 + lk;
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const double2 a = As[kk][32 * wr + 16 * t + li];
+        const double2 b = Bs[kk][32 * wc + 16 * t + li];
+        ar[t] = a.x; ai[t] = a.y; br[t] = b.x; bi[t] = b.y;
+      }
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        const double wv = Ws[w][kk];
+        double bwr[2], bwi[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) { bwr[t] = br[t] * wv; bwi[t] = bi[t] * wv; }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int b = 0; b < 2; b++) {
+            accR[w][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bwr[b], accR[w][a][b], 0, 0, 0);
+            accR[w][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bwi[b], accR[w][a][b], 0, 0, 0);
+            accI[w][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bwr[b], accI[w][a][b], 0, 0, 0);
+            accI[w][a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[a], bwi[b], accI[w][a][b], 0, 0, 0);
+          }
+      }
+    }
+    __syncthreads();
   }
   const size_t pb2 = (size_t)pb * pb;
 #pragma unroll

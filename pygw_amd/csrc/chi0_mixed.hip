@@ -97,87 +97,102 @@ __global__ void __launch_bounds__(256) pygw_chi0_mixed(const float2* __restrict_
     }
   };
   zero();
-  for (int kk = 0; kk < nk; kk++) {
-    const int k = k0 + kk;
+  // chunks (k, p0) in order, next chunk prefetched into registers
+  const int npc = (np + KC - 1) / KC;
+  const long nch = (long)nk * npc;
+  constexpr int NQ = (KC * TB) / 256;
+  float2 pa[NQ], pv[NQ];
+  float pw = 0.f;
+  auto fetch = [&](long ch) {
+    const int kk = (int)(ch / npc), p0 = (int)(ch % npc) * KC;
     const float2* Mk = M + (size_t)kk * strideK;
-    const double* Kk = kcw + (size_t)k * nom * np;
-    for (int p0 = 0; p0 < np; p0 += KC) {
 #pragma unroll
-      for (int q = 0; q < (KC * TB) / 256; q++) {
-        const int e = tid + 256 * q;
-        const int r = e / TB, c = e % TB;
-        const int p = p0 + r;
-        float2 va = {0.f, 0.f}, vb = {0.f, 0.f};
-        if (p < np) {
-          if (I0 + c < pb) { va = Mk[(size_t)p * pb + I0 + c]; va.x *= cs[I0 + c]; va.y *= cs[I0 + c]; }
-          if (J0 + c < pb) { vb = Mk[(size_t)p * pb + J0 + c]; vb.x *= cs[J0 + c]; vb.y *= cs[J0 + c]; }
-        }
-        Ar[r][c] = va.x; Ai[r][c] = va.y;
-        Br[r][c] = vb.x; Bi[r][c] = vb.y;
+    for (int q = 0; q < NQ; q++) {
+      const int e = tid + 256 * q;
+      const int r = e / TB, c = e % TB;
+      const int p = p0 + r;
+      pa[q] = float2{0.f, 0.f};
+      pv[q] = float2{0.f, 0.f};
+      if (p < np) {
+        if (I0 + c < pb) { pa[q] = Mk[(size_t)p * pb + I0 + c]; pa[q].x *= cs[I0 + c]; pa[q].y *= cs[I0 + c]; }
+        if (J0 + c < pb) { pv[q] = Mk[(size_t)p * pb + J0 + c]; pv[q].x *= cs[J0 + c]; pv[q].y *= cs[J0 + c]; }
       }
-      if (tid < NW * KC) {
-        const int w = tid / KC, r = tid % KC;
-        float v = 0.f;
-        if (p0 + r < np && w0 + w < nom) v = (float)(coef * Kk[(size_t)(w0 + w) * np + p0 + r] * ws[w0 + w]);
-        Wt[w][r] = v;
-      }
-      __syncthreads();
-      // A operands (I side): hi/lo of a_r, a_i and of -a_r
-      h8 arh[2], arl[2], aih[2], ail[2], anh[2], anl[2];
+    }
+    pw = 0.f;
+    if (tid < NW * KC) {
+      const int w = tid / KC, r = tid % KC;
+      if (p0 + r < np && w0 + w < nom)
+        pw = (float)(coef * kcw[((size_t)(k0 + kk) * nom + (w0 + w)) * np + p0 + r] * ws[w0 + w]);
+    }
+  };
+  fetch(0);
+  for (long ch = 0; ch < nch; ch++) {
 #pragma unroll
-      for (int t = 0; t < 2; t++) {
-        const int col = 32 * wr + 16 * t + li;
+    for (int q = 0; q < NQ; q++) {
+      const int e = tid + 256 * q;
+      const int r = e / TB, c = e % TB;
+      Ar[r][c] = pa[q].x; Ai[r][c] = pa[q].y;
+      Br[r][c] = pv[q].x; Bi[r][c] = pv[q].y;
+    }
+    if (tid < NW * KC) Wt[tid / KC][tid % KC] = pw;
+    __syncthreads();
+    if (ch + 1 < nch) fetch(ch + 1);
+    // A operands (I side): hi/lo of a_r, a_i and of -a_r
+    h8 arh[2], arl[2], aih[2], ail[2], anh[2], anl[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const int col = 32 * wr + 16 * t + li;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int r = 8 * lk + j;
+        _Float16 h, l;
+        cm::split(Ar[r][col], h, l); arh[t][j] = h; arl[t][j] = l; anh[t][j] = -h; anl[t][j] = -l;
+        cm::split(Ai[r][col], h, l); aih[t][j] = h; ail[t][j] = l;
+      }
+    }
+    float br[2][8], bi[2][8];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const int col = 32 * wc + 16 * t + li;
+#pragma unroll
+      for (int j = 0; j < 8; j++) { br[t][j] = Br[8 * lk + j][col]; bi[t][j] = Bi[8 * lk + j][col]; }
+    }
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+      h8 brh[2], brl[2], bih[2], bil[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++)
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-          const int r = 8 * lk + j;
+          const float wv = Wt[w][8 * lk + j];
           _Float16 h, l;
-          cm::split(Ar[r][col], h, l); arh[t][j] = h; arl[t][j] = l; anh[t][j] = -h; anl[t][j] = -l;
-          cm::split(Ai[r][col], h, l); aih[t][j] = h; ail[t][j] = l;
+          cm::split(br[t][j] * wv, h, l); brh[t][j] = h; brl[t][j] = l;
+          cm::split(bi[t][j] * wv, h, l); bih[t][j] = h; bil[t][j] = l;
         }
-      }
-      float br[2][8], bi[2][8];
 #pragma unroll
-      for (int t = 0; t < 2; t++) {
-        const int col = 32 * wc + 16 * t + li;
+      for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int j = 0; j < 8; j++) { br[t][j] = Br[8 * lk + j][col]; bi[t][j] = Bi[8 * lk + j][col]; }
-      }
-#pragma unroll
-      for (int w = 0; w < NW; w++) {
-        h8 brh[2], brl[2], bih[2], bil[2];
-#pragma unroll
-        for (int t = 0; t < 2; t++)
-#pragma unroll
-          for (int j = 0; j < 8; j++) {
-            const float wv = Wt[w][8 * lk + j];
-            _Float16 h, l;
-            cm::split(br[t][j] * wv, h, l); brh[t][j] = h; brl[t][j] = l;
-            cm::split(bi[t][j] * wv, h, l); bih[t][j] = h; bil[t][j] = l;
-          }
-#pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-          for (int b = 0; b < 2; b++) {
-            f4 r = aR[w][a][b], m = aI[w][a][b];
-            r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[a], brh[b], r, 0, 0, 0);
-            r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[a], brl[b], r, 0, 0, 0);
-            r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl[a], brh[b], r, 0, 0, 0);
-            r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[a], bih[b], r, 0, 0, 0);
-            r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[a], bil[b], r, 0, 0, 0);
-            r = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[a], bih[b], r, 0, 0, 0);
-            m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[a], brh[b], m, 0, 0, 0);
-            m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[a], brl[b], m, 0, 0, 0);
-            m = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[a], brh[b], m, 0, 0, 0);
-            m = __builtin_amdgcn_mfma_f32_16x16x32_f16(anh[a], bih[b], m, 0, 0, 0);
-            m = __builtin_amdgcn_mfma_f32_16x16x32_f16(anh[a], bil[b], m, 0, 0, 0);
-            m = __builtin_amdgcn_mfma_f32_16x16x32_f16(anl[a], bih[b], m, 0, 0, 0);
-            aR[w][a][b] = r;
-            aI[w][a][b] = m;
-          }
-      }
-      __syncthreads();
+        for (int b = 0; b < 2; b++) {
+          f4 r = aR[w][a][b], m = aI[w][a][b];
+          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[a], brh[b], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[a], brl[b], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl[a], brh[b], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[a], bih[b], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[a], bil[b], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[a], bih[b], r, 0, 0, 0);
+          m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[a], brh[b], m, 0, 0, 0);
+          m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[a], brl[b], m, 0, 0, 0);
+          m = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[a], brh[b], m, 0, 0, 0);
+          m = __builtin_amdgcn_mfma_f32_16x16x32_f16(anh[a], bih[b], m, 0, 0, 0);
+          m = __builtin_amdgcn_mfma_f32_16x16x32_f16(anh[a], bil[b], m, 0, 0, 0);
+          m = __builtin_amdgcn_mfma_f32_16x16x32_f16(anl[a], bih[b], m, 0, 0, 0);
+          aR[w][a][b] = r;
+          aI[w][a][b] = m;
+        }
     }
-    if ((kk + 1) % kflush == 0 || kk + 1 == nk) {
+    __syncthreads();
+    const int kk = (int)(ch / npc);
+    if ((ch % npc) == npc - 1 && ((kk + 1) % kflush == 0 || kk + 1 == nk)) {
       flush();
       zero();
     }

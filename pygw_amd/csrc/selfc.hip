@@ -1,0 +1,221 @@
+// Sigma_c tail on gfx950, FP64: the G0 (x) W frequency convolution
+// (Compute_selfc_inside, mcommon.py:80-165, with fnc.f90:86-176), its
+// cubic-spline refinement onto the precise mesh (mcommon.py:126-140), and
+// the Pade continuation plus the quasiparticle equation
+// (AnalyticContinuation iop_ac=0, mcommon.py:315-343; Compute_quasiparticles,
+// mcommon.py:167-210; Padecof/PadeG, for_pade.f90:3-51).
+#include "common.h"
+
+namespace sc {
+constexpr double PI_F = 3.14159265358979;  // pi as written in fnc.f90
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return double2{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return double2{a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+  if (fabs(b.x) >= fabs(b.y)) {
+    double r = b.y / b.x, d = b.x + b.y * r;
+    return double2{(a.x + a.y * r) / d, (a.y - a.x * r) / d};
+  }
+  double r = b.x / b.y, d = b.x * r + b.y;
+  return double2{(a.x * r + a.y) / d, (a.y * r - a.x) / d};
+}
+__device__ __forceinline__ double2 clog_(double2 z) { return double2{log(hypot(z.x, z.y)), atan2(z.y, z.x)}; }
+// principal atan(z) = (i/2) [log(1 - i z) - log(1 + i z)]
+__device__ __forceinline__ double2 catan_(double2 z) {
+  const double2 iz = double2{-z.y, z.x};
+  const double2 l1 = clog_(double2{1.0 - iz.x, -iz.y});
+  const double2 l2 = clog_(double2{1.0 + iz.x, iz.y});
+  const double2 d = csub(l1, l2);
+  return double2{-0.5 * d.y, 0.5 * d.x};
+}
+}  // namespace sc
+
+// Cubic spline of y(x) = (x^2+1) * mwm(x) on the coarse mesh, bc y''(x_0)=0,
+// y'(x_{n-1})=0 (scipy CubicSpline bc_type=((2,0),(1,0)), extrapolating with
+// the end polynomials), evaluated on the precise mesh and divided by
+// (xp^2+1).  The imaginary part is splined only where sum|Im|/n > 1e-10, as
+// at mcommon.py:134-138.  One thread per (row, ie2), row = (k, ie1): mwm is
+// PyGW's per-k mwm[iom, ie1, ie2] stacked over k, [N_k][nom][nb1][nb2];
+// out is [nrows][nb2][nomp].
+__global__ void __launch_bounds__(64) pygw_selfc_spline(const double2* __restrict__ mwm, int nrows, int nb1, int nom,
+                                                        int nb2,
+                                                        const double* __restrict__ x, const double* __restrict__ xp,
+                                                        int nomp, double2* __restrict__ out) {
+  using namespace sc;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nrows * nb2) return;
+  const int row = (int)(t / nb2), ie2 = (int)(t % nb2);
+  const int kq = row / nb1, ie1 = row % nb1;
+  constexpr int NMAX = 128;
+  double sr[NMAX], si[NMAX], cp[NMAX];
+  const int n = nom;
+  double imsum = 0.0;
+  for (int i = 0; i < n; i++) imsum += fabs(mwm[(((size_t)kq * nom + i) * nb1 + ie1) * nb2 + ie2].y);
+  const bool doim = imsum / n > 1e-10;
+  // slopes s: tridiagonal system (Thomas); rhs for both parts
+  auto yv = [&](int i, int part) {
+    const double2 v = mwm[(((size_t)kq * nom + i) * nb1 + ie1) * nb2 + ie2];
+    return (x[i] * x[i] + 1.0) * (part ? v.y : v.x);
+  };
+  for (int part = 0; part < (doim ? 2 : 1); part++) {
+    double* s = part ? si : sr;
+    // row 0: 2 s0 + s1 = 3 m0 ; rows 1..n-2 ; row n-1: s = 0
+    double b0 = 2.0, c0 = 1.0;
+    double d0 = 3.0 * (yv(1, part) - yv(0, part)) / (x[1] - x[0]);
+    cp[0] = c0 / b0;
+    s[0] = d0 / b0;
+    for (int i = 1; i < n - 1; i++) {
+      const double h0 = x[i] - x[i - 1], h1 = x[i + 1] - x[i];
+      const double m0 = (yv(i, part) - yv(i - 1, part)) / h0, m1 = (yv(i + 1, part) - yv(i, part)) / h1;
+      const double a = h1, b = 2.0 * (h0 + h1), c = h0, d = 3.0 * (h1 * m0 + h0 * m1);
+      const double den = b - a * cp[i - 1];
+      cp[i] = c / den;
+      s[i] = (d - a * s[i - 1]) / den;
+    }
+    s[n - 1] = 0.0;
+    for (int i = n - 2; i >= 0; i--) s[i] = s[i] - cp[i] * s[i + 1];
+  }
+  double2* o = out + ((size_t)row * nb2 + ie2) * nomp;
+  int seg = 0;
+  for (int j = 0; j < nomp; j++) {
+    const double xv = xp[j];
+    while (seg < n - 2 && xv > x[seg + 1]) seg++;
+    const double h = x[seg + 1] - x[seg], tt = xv - x[seg];
+    double res[2] = {0.0, 0.0};
+    for (int part = 0; part < (doim ? 2 : 1); part++) {
+      const double* s = part ? si : sr;
+      const double y0 = yv(seg, part), y1 = yv(seg + 1, part);
+      const double m = (y1 - y0) / h;
+      const double c2 = (3.0 * m - 2.0 * s[seg] - s[seg + 1]) / h;
+      const double c3 = (s[seg] + s[seg + 1] - 2.0 * m) / (h * h);
+      res[part] = y0 + tt * (s[seg] + tt * (c2 + tt * c3));
+    }
+    const double f = xv * xv + 1.0;
+    o[j] = double2{res[0] / f, res[1] / f};
+  }
+}
+
+// sc[row][iom] = sum_ie2 fr_convolution2(enk[k][ie2], mwmp[row][ie2][:], mwm[row][iom][ie2],
+//                                        omega_p, omega[iom], womeg_p)    (fnc.f90:86-119)
+// row = (k, ie1); mwm = [N_k][nom][nb1][nb2]; mwmp = [nrows][nb2][nomp]; for
+// the coarse form (all_fr_convolutions) pass mwmp = NULL and the coarse mesh.
+// One wavefront per (row, iom): lanes split ie2, then reduce.
+__global__ void __launch_bounds__(64) pygw_selfc_conv(const double* __restrict__ enk, int nb1, int nb2,
+                                                      const double2* __restrict__ mwm, int nom,
+                                                      const double* __restrict__ omega,
+                                                      const double2* __restrict__ mwmp, int nomp,
+                                                      const double* __restrict__ omp, const double* __restrict__ womp,
+                                                      double2* __restrict__ out) {
+  using namespace sc;
+  const int row = blockIdx.x, iom = blockIdx.y;
+  const int k = row / nb1, ie1 = row % nb1;
+  const int lane = threadIdx.x;
+  const double wn = omega[iom];
+  const double wmax = omp[nomp - 1];
+  double2 acc = {0.0, 0.0};
+  for (int ie2 = lane; ie2 < nb2; ie2 += 64) {
+    const double e = enk[(size_t)k * nb2 + ie2];
+    const double2 mn = mwm[(((size_t)k * nom + iom) * nb1 + ie1) * nb2 + ie2];
+    const double2 eo = double2{e, -wn};
+    const double2 eo2 = cmul(eo, eo);
+    double2 s0 = {0.0, 0.0};
+    for (int l = 0; l < nomp; l++) {
+      const double2 ml = mwmp ? mwmp[((size_t)row * nb2 + ie2) * nomp + l]
+                              : mwm[(((size_t)k * nom + l) * nb1 + ie1) * nb2 + ie2];
+      const double2 num = double2{(ml.x - mn.x) * womp[l], (ml.y - mn.y) * womp[l]};
+      const double2 den = double2{omp[l] * omp[l] + eo2.x, eo2.y};
+      s0 = cadd(s0, cdiv(num, den));
+    }
+    double2 res = cmul(eo, s0);
+    res.x /= PI_F;
+    res.y /= PI_F;
+    if (hypot(mn.x, mn.y) > 1e-16) {
+      const double2 at = catan_(cdiv(double2{wmax, 0.0}, eo));
+      const double2 t = cmul(double2{mn.x / PI_F, mn.y / PI_F}, at);
+      res = cadd(res, t);
+    }
+    acc = cadd(acc, res);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    acc.x += __shfl_xor(acc.x, off, 64);
+    acc.y += __shfl_xor(acc.y, off, 64);
+  }
+  if (lane == 0) out[(size_t)row * nom + iom] = acc;
+}
+
+// Pade (Thiele) through the first npts points of sigma(i w_n), evaluated at
+// e and e +- 1e-3 (mcommon.py:330-338), then the G0W0 quasiparticle update
+// (mcommon.py:182-203, iop_es = 0):
+//   Z = 1/(1 - Re dS), set to 1 (and dS to 0) when Z > 1 or Z < 0;
+//   E_qp = e + Z (Re S(e) + sigx - vxc).
+// out per state: [S(e) re, im, dS re, im, Z, E_qp]
+__global__ void pygw_pade_qp(const double2* __restrict__ sig, int nstate, int nom, const double* __restrict__ omega,
+                             int npts, const double* __restrict__ ene, const double* __restrict__ sigx,
+                             const double* __restrict__ vxc, double* __restrict__ out) {
+  using namespace sc;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nstate) return;
+  constexpr int NP = 64;
+  double2 g[NP];
+  const int n = npts;
+  for (int j = 0; j < n; j++) g[j] = sig[(size_t)s * nom + j];
+  // p(i,j) = (p(i-1,i-1) - p(i-1,j)) / (z_j - z_{i-1}) / p(i-1,j), row i in place
+  for (int i = 1; i < n; i++) {
+    const double2 pd = g[i - 1];
+    for (int j = i; j < n; j++) {
+      const double2 dz = double2{0.0, omega[j] - omega[i - 1]};
+      g[j] = cdiv(cdiv(csub(pd, g[j]), dz), g[j]);
+    }
+  }
+  const double e = ene[s];
+  double2 val[3];
+  const double xs[3] = {e, e + 1e-3, e - 1e-3};
+  for (int q = 0; q < 3; q++) {
+    const double2 xv = double2{xs[q], 0.0};
+    double2 a0 = {0.0, 0.0}, a1 = g[0], b0 = {1.0, 0.0}, b1 = {1.0, 0.0};
+    for (int i = 1; i < n; i++) {
+      const double2 f = cmul(csub(xv, double2{0.0, omega[i - 1]}), g[i]);
+      const double2 a2 = cadd(a1, cmul(f, a0));
+      const double2 b2 = cadd(b1, cmul(f, b0));
+      a0 = a1; a1 = a2; b0 = b1; b1 = b2;
+    }
+    val[q] = cdiv(a1, b1);
+  }
+  const double2 ds = double2{(val[1].x - val[2].x) / 2e-3, (val[1].y - val[2].y) / 2e-3};
+  double z = 1.0 / (1.0 - ds.x);
+  if (z > 1.0 || z < 0.0) z = 1.0;
+  const double snk = val[0].x + (sigx ? sigx[s] : 0.0);
+  const double vx = vxc ? vxc[s] : 0.0;
+  double* o = out + (size_t)s * 6;
+  o[0] = val[0].x; o[1] = val[0].y; o[2] = ds.x; o[3] = ds.y; o[4] = z;
+  o[5] = e + z * (snk - vx);
+}
+
+int selfc_spline_launch(const void* mwm, int nrows, int nb1, int nom, int nb2, const double* x, const double* xp,
+                        int nomp, void* out, hipStream_t stream) {
+  if (nom > 128 || nom < 3) return PYGW_E_ARG;
+  const long n = (long)nrows * nb2;
+  hipLaunchKernelGGL(pygw_selfc_spline, dim3(ceil_div(n, 64)), dim3(64), 0, stream, (const double2*)mwm, nrows, nb1,
+                     nom, nb2, x, xp, nomp, (double2*)out);
+  return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+}
+
+int selfc_conv_launch(const double* enk, int nrows, int nb1, int nb2, const void* mwm, int nom, const double* omega,
+                      const void* mwmp, int nomp, const double* omp, const double* womp, void* out,
+                      hipStream_t stream) {
+  dim3 grid(nrows, nom);
+  hipLaunchKernelGGL(pygw_selfc_conv, grid, dim3(64), 0, stream, enk, nb1, nb2, (const double2*)mwm, nom, omega,
+                     (const double2*)mwmp, nomp, omp, womp, (double2*)out);
+  return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+}
+
+int pade_qp_launch(const void* sig, int nstate, int nom, const double* omega, int npts, const double* ene,
+                   const double* sigx, const double* vxc, double* out, hipStream_t stream) {
+  if (npts > 64 || npts < 1 || npts > nom) return PYGW_E_ARG;
+  hipLaunchKernelGGL(pygw_pade_qp, dim3(ceil_div(nstate, 64)), dim3(64), 0, stream, (const double2*)sig, nstate, nom,
+                     omega, npts, ene, sigx, vxc, out);
+  return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+}

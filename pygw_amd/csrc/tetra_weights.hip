@@ -707,9 +707,9 @@ __device__ void small_tet(const double cr[4][3], const double* db, int corner, c
 
 // Corner weight of one (occupied e, empty f) tetrahedron pair for corner
 // `corner`, frequencies c0..c0+OMC-1 (convw1t_par); res[w] overwritten.
-__device__ void corner_weight(const double* e, const double* f, double ef, int corner,
-                              const double* __restrict__ omega, int nom, int c0, int compat,
-                              double* res, int* nwarn, int* status) {
+__device__ __noinline__ void corner_weight(const double* e, const double* f, double ef, int corner,
+                                           const double* __restrict__ omega, int nom, int c0, int compat,
+                                           double* res, int* nwarn, int* status) {
   for (int w = 0; w < OMC; w++) res[w] = 0.0;
   Nodes g;
   surf_nodes(g, e, f, ef);
@@ -757,6 +757,26 @@ __device__ void corner_weight(const double* e, const double* f, double ef, int c
     double first = (!n78 || jj < 2) ? sub1[w] : sub2s[w];
     res[w] = first + sub2[w];
   }
+}
+
+// Insulating pair (every e below and every f above E_F by more than 1e-9):
+// surfnodes..relnodes keep exactly the four corners (0,0,0), (0,0,1),
+// (0,1,0), (1,0,0) -- no plane crosses the tetrahedron, and a margin of
+// 1e-9 is far beyond the rounding of the intersection quotients -- so the
+// polyhedron is the whole tetrahedron (nnod = 4) and only small_tet runs.
+__device__ __forceinline__ bool insulating(const double* e, const double* f, double ef) {
+  const double m = 1.0e-9;
+  return fmax(fmax(e[0], e[1]), fmax(e[2], e[3])) < ef - m && fmin(fmin(f[0], f[1]), fmin(f[2], f[3])) > ef + m;
+}
+
+__device__ __forceinline__ void corner_weight_full(const double* e, const double* f, int corner,
+                                                   const double* __restrict__ omega, int nom, int c0, int compat,
+                                                   double* res, int* nwarn, int* status) {
+  const double cr[4][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}, {0.0, 1.0, 0.0}, {1.0, 0.0, 0.0}};
+  double db[4];
+  for (int i = 0; i < 4; i++) db[i] = f[i] - e[i];
+  for (int w = 0; w < OMC; w++) res[w] = 0.0;
+  small_tet(cr, db, corner, omega, c0, 0, nom, 0, compat, res, nwarn, status);
 }
 
 }  // namespace tw
@@ -823,7 +843,8 @@ __global__ void __launch_bounds__(256) pygw_tw_gather(
       }
       if (fmax <= ef || emin > ef) continue;
       double res[tw::OMC];
-      tw::corner_weight(e, f, ef, corner, omega, nom, c0, compat, res, &nwarn, &status);
+      if (tw::insulating(e, f, ef)) tw::corner_weight_full(e, f, corner, omega, nom, c0, compat, res, &nwarn, &status);
+      else tw::corner_weight(e, f, ef, corner, omega, nom, c0, compat, res, &nwarn, &status);
       for (int w = 0; w < tw::OMC; w++) acc[w] = acc[w] + res[w] * 6 * wtet;
     }
     for (int w = 0; w < tw::OMC; w++) {
@@ -837,4 +858,30 @@ __global__ void __launch_bounds__(256) pygw_tw_gather(
   }
   if (status) atomicOr(&dstat[0], status);
   if (nwarn) atomicAdd(&dstat[1], nwarn);
+}
+
+// SVD-basis weights (bz_calcqdepw_par3, for_tetrahedra.f90:466-542):
+//   kcw_l[k][l][p] = sum_w kcw[k][w][p] * dUl[w][l]      (dUl Fortran (nom, nl))
+// The reference projects each tetrahedron's w1t before the scatter
+// (w2t = matmul(w1t, dUl), :532); the projection is linear, so projecting
+// the gathered kcw gives the same sums up to rounding.
+// layout 0: out Fortran [n_o, n_e, N_k, nl]; layout 1: out [N_k][nl][n_o*n_e].
+__global__ void __launch_bounds__(256) pygw_tw_project(const double* __restrict__ kcw, int np, int nom, int nk,
+                                                       const double* __restrict__ dUl, int nl, int layout, int no,
+                                                       int ne, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (t >= np || k >= nk) return;
+  // thread t holds pair index in the output layout's order
+  int p;
+  if (layout == 0) { const int ib = t % no, jb = t / no; p = ib * ne + jb; }
+  else p = t;
+  const double* src = kcw + (size_t)k * nom * np + p;
+  for (int l = 0; l < nl; l++) {
+    double s = 0.0;
+    for (int w = 0; w < nom; w++) s = s + src[(size_t)w * np] * dUl[w + (size_t)nom * l];
+    size_t off = (layout == 0) ? (size_t)t + (size_t)np * ((size_t)k + (size_t)nk * l)
+                               : (size_t)t + (size_t)np * ((size_t)l + (size_t)nl * k);
+    out[off] = s;
+  }
 }

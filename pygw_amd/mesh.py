@@ -147,3 +147,20 @@ class FrequencyMesh:
         n = len(om1) // 2
         self.omega_precise, self.womeg_precise = om1[n:].copy(), dom1[n:].copy()
         self.iopfreq = 4
+
+
+def svd_functions(iomega, wiomeg, om_min, om_max, om_nom, svd_cutoff):
+    """SVD basis of the bosonic Matsubara kernel (pygw.py:3478-3501).
+    Returns (Ul [n_l, nom], dUl [nom, n_l])."""
+    om, dom = give2tan_mesh(om_min, om_max, om_nom)
+    iom = np.hstack((-iomega[::-1], iomega[:]))
+    diom = np.hstack((wiomeg[::-1], wiomeg[:]))
+    num = om * dom / np.pi
+    om2 = om ** 2
+    ker = num[None, :] * np.sqrt(diom)[:, None] / (om2[None, :] + iom[:, None] ** 2)
+    u, s, vh = np.linalg.svd(ker, full_matrices=True)
+    n = int(np.where(s < svd_cutoff)[0][0])
+    ul = np.transpose(u[:, :n]) / np.sqrt(diom)[None, :]
+    n2 = len(iom) // 2
+    dul = (ul[:, n2:] * diom[n2:] * 2.0).T
+    return ul[:, n2:], dul

@@ -126,3 +126,93 @@ def eps_inverse(eps, *, device="cuda", check=True):
     _lib.call("pygw_eps_inverse", C.c_void_p(eps.data_ptr()), pb, nom, None, 0,
               C.cast(info, C.c_void_p) if check else None, _stream_ptr(device))
     return eps
+
+
+def calc_mwm(M_selfe, W, wkq, *, device="cuda"):
+    """MWM of calc_selfc (pygw.py:3353-3358) for all k at once.
+
+    M_selfe: complex [n_k, nb1*nb2, PB]; W: Fortran [PB, PB, n_om] (eps after
+    eps_inverse).  Returns complex128 [n_k, n_om, nb1*nb2]."""
+    device = _require_device(device)
+    M = M_selfe.to(device=device, dtype=torch.complex128).contiguous() if isinstance(M_selfe, torch.Tensor) else \
+        torch.as_tensor(np.ascontiguousarray(M_selfe), dtype=torch.complex128, device=device)
+    nk, rpk, pb = M.shape
+    if W.shape[0] != pb or W.stride() != (1, pb, pb * pb) or W.dtype != torch.complex128:
+        raise ValueError("W must be a Fortran-strided [PB, PB, n_om] complex128 tensor")
+    nom = W.shape[2]
+    out = torch.empty((nk, nom, rpk), dtype=torch.complex128, device=device)
+    _lib.call("pygw_mwm", C.c_void_p(M.data_ptr()), nk, rpk, pb, C.c_void_p(W.data_ptr()), nom, float(wkq), 0,
+              C.c_void_p(out.data_ptr()), _stream_ptr(device))
+    return out
+
+
+def compute_selfc_inside(enk, mwm, omega, womeg, omega_p=None, womeg_p=None, *, device="cuda"):
+    """Sigma_c contribution of one q for all k (Compute_selfc_inside,
+    mcommon.py:80-165).  enk [n_k, nb2] (energies at k-q), mwm complex
+    [n_k, n_om, nb1, nb2].  With omega_p/womeg_p the spline refinement onto
+    the precise mesh is used (iopfreq = 4 branch), else the coarse
+    all_fr_convolutions.  Returns complex128 [n_k, nb1, n_om]."""
+    device = _require_device(device)
+    mw = mwm.to(device=device, dtype=torch.complex128).contiguous() if isinstance(mwm, torch.Tensor) else \
+        torch.as_tensor(np.ascontiguousarray(mwm), dtype=torch.complex128, device=device)
+    nk, nom, nb1, nb2 = mw.shape
+    e = _dev(enk, torch.float64, device).contiguous()
+    om = _dev(omega, torch.float64, device).contiguous()
+    wo = _dev(womeg, torch.float64, device).contiguous()
+    out = torch.empty((nk, nb1, nom), dtype=torch.complex128, device=device)
+    s = _stream_ptr(device)
+    if omega_p is None:
+        _lib.call("pygw_selfc_convolve", C.c_void_p(e.data_ptr()), nk, nb1, nb2, C.c_void_p(mw.data_ptr()),
+                  C.c_void_p(om.data_ptr()), nom, None, C.c_void_p(om.data_ptr()), C.c_void_p(wo.data_ptr()), nom,
+                  C.c_void_p(out.data_ptr()), s)
+        return out
+    op = _dev(omega_p, torch.float64, device).contiguous()
+    wp = _dev(womeg_p, torch.float64, device).contiguous()
+    nomp = op.numel()
+    mp = torch.empty((nk, nb1, nb2, nomp), dtype=torch.complex128, device=device)
+    _lib.call("pygw_selfc_spline", C.c_void_p(mw.data_ptr()), nk, nb1, nb2, C.c_void_p(om.data_ptr()), nom,
+              C.c_void_p(op.data_ptr()), nomp, C.c_void_p(mp.data_ptr()), s)
+    _lib.call("pygw_selfc_convolve", C.c_void_p(e.data_ptr()), nk, nb1, nb2, C.c_void_p(mw.data_ptr()),
+              C.c_void_p(om.data_ptr()), nom, C.c_void_p(mp.data_ptr()), C.c_void_p(op.data_ptr()),
+              C.c_void_p(wp.data_ptr()), nomp, C.c_void_p(out.data_ptr()), s)
+    return out
+
+
+def pade_quasiparticles(sigc, omega, ene, sigx=None, vxc=None, npade=32, *, device="cuda"):
+    """Pade continuation (iop_ac = 0) and G0W0 quasiparticle energies for
+    every state (mcommon.py:167-210, 315-343).  sigc complex [n_state, n_om].
+    Returns float64 [n_state, 6]: Re S(e), Im S(e), Re S', Im S', Z, E_qp."""
+    device = _require_device(device)
+    sg = sigc.to(device=device, dtype=torch.complex128).contiguous() if isinstance(sigc, torch.Tensor) else \
+        torch.as_tensor(np.ascontiguousarray(sigc), dtype=torch.complex128, device=device)
+    ns, nom = sg.shape
+    om = _dev(omega, torch.float64, device).contiguous()
+    en = _dev(ene, torch.float64, device).contiguous()
+    sx = _dev(sigx, torch.float64, device).contiguous() if sigx is not None else None
+    vx = _dev(vxc, torch.float64, device).contiguous() if vxc is not None else None
+    out = torch.empty((ns, 6), dtype=torch.float64, device=device)
+    _lib.call("pygw_pade_qp", C.c_void_p(sg.data_ptr()), ns, nom, C.c_void_p(om.data_ptr()), min(npade, nom),
+              C.c_void_p(en.data_ptr()), C.c_void_p(sx.data_ptr()) if sx is not None else None,
+              C.c_void_p(vx.data_ptr()) if vx is not None else None, C.c_void_p(out.data_ptr()), _stream_ptr(device))
+    return out
+
+
+def bz_calcqdepw_par3(enk, ef, kqid, ncg, nomx, numin, dUl, omega, tetc, *, layout="fortran",
+                      compat=COMPAT_REFERENCE, device="cuda", check=True):
+    """SVD-basis weights (ft.bz_calcqdepw_par3, for_tetrahedra.f90:466-542):
+    the weights of bz_calcqdepw_par2 projected on dUl [nom, n_l]."""
+    device = _require_device(device)
+    kn = bz_calcqdepw_par2(enk, ef, kqid, ncg, nomx, numin, omega, tetc, layout="native", compat=compat,
+                           device=device, check=check)
+    nkp, nom, npair = kn.shape
+    du = torch.as_tensor(np.asfortranarray(dUl, dtype=np.float64).reshape(-1, order="F"), device=device)
+    nl = np.asarray(dUl).shape[1]
+    nbnd = np.asarray(enk).shape[0]
+    no, ne = ncg + nomx + 1, nbnd - ncg - numin
+    out = torch.empty(no * ne * nkp * nl, dtype=torch.float64, device=device)
+    lay = 0 if layout == "fortran" else 1
+    _lib.call("pygw_tetra_weights_svd", C.c_void_p(kn.data_ptr()), nkp, nom, no, ne, C.c_void_p(du.data_ptr()), nl,
+              lay, C.c_void_p(out.data_ptr()), _stream_ptr(device))
+    if lay == 0:
+        return out.as_strided((no, ne, nkp, nl), (1, no, no * ne, no * ne * nkp))
+    return out.view(nkp, nl, no * ne)

@@ -1,0 +1,84 @@
+"""GPU parity of the Sigma_c part: MWM, spline, convolution, Pade/QP."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import chain
+from pygw_amd import mesh
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _herm_w(pb, nom, rng):
+    x = rng.normal(size=(pb, pb, nom)) + 1j * rng.normal(size=(pb, pb, nom))
+    w = (x + np.conj(x.transpose(1, 0, 2))) / 2
+    return np.asfortranarray(w)
+
+
+@pytest.mark.parametrize("nk,nb1,nb2,pb,nom", [(2, 3, 7, 70, 3), (3, 2, 33, 129, 2)])
+def test_mwm_vs_oracle(nk, nb1, nb2, pb, nom):
+    import torch
+    from pygw_amd import polarization as P
+    rng = np.random.default_rng(nk * 100 + pb)
+    M = rng.normal(size=(nk, nb1 * nb2, pb)) + 1j * rng.normal(size=(nk, nb1 * nb2, pb))
+    W = _herm_w(pb, nom, rng)
+    print(f"case mwm nk={nk} rows={nb1*nb2} pb={pb}", flush=True)
+    Wt = torch.as_tensor(W.reshape(-1, order="F"), device="cuda").as_strided((pb, pb, nom), (1, pb, pb * pb))
+    got = P.calc_mwm(M, Wt, 0.125).cpu().numpy()
+    for k in range(nk):
+        want = chain.calc_mwm(M[k], W, 0.125)
+        assert _rel(got[k], want) < 1e-13
+
+
+def test_convolution_vs_reference_fixture():
+    from pygw_amd import polarization as P
+    g = np.load(os.path.join(GOLD, "conv.npz"))
+    mwm = g["mwm"]  # [nom, nb1, nb2]
+    print("case conv fixture", flush=True)
+    sc = P.compute_selfc_inside(g["enk"][None, :], mwm[None], g["omega"], g["womeg"]).cpu().numpy()[0]
+    np.testing.assert_allclose(sc, g["sc"], rtol=1e-12, atol=1e-14)
+
+
+def test_spline_convolution_vs_oracle():
+    """Spline branch (precise mesh) against scipy CubicSpline + the
+    convolution formula (oracle.chain.selfc_spline)."""
+    from pygw_amd import polarization as P
+    rng = np.random.default_rng(8)
+    fm = mesh.FrequencyMesh(16, 0.02, 20.0, iop_multiple=5)
+    om, wo, omp, wop = fm.omega, fm.womeg, fm.omega_precise, fm.womeg_precise
+    nk, nb1, nb2 = 2, 2, 6
+    a = rng.uniform(0.3, 1.5, (nk, nb1, nb2))
+    mwm = np.empty((nk, om.size, nb1, nb2), dtype=complex)
+    for k in range(nk):
+        mwm[k] = (1.0 / (om[:, None, None] ** 2 + a[k][None] ** 2)) * (1 + 0.1j * rng.normal(size=(1, nb1, nb2)))
+    enk = rng.uniform(-0.6, 0.6, (nk, nb2))
+    print("case spline conv", flush=True)
+    got = P.compute_selfc_inside(enk, mwm, om, wo, omp, wop).cpu().numpy()
+    for k in range(nk):
+        want = chain.selfc_spline(enk[k], mwm[k], om, omp, wop)
+        np.testing.assert_allclose(got[k], want, rtol=1e-11, atol=1e-13)
+
+
+def test_pade_qp_vs_reference_fixture():
+    from pygw_amd import polarization as P
+    g = np.load(os.path.join(GOLD, "pade.npz"))
+    z, y = g["z"], g["y"]
+    om = z.imag
+    ene = np.array([0.1, -0.3])
+    print("case pade", flush=True)
+    out = P.pade_quasiparticles(np.vstack([y, y]), om, ene, sigx=np.array([0.2, 0.0]), vxc=np.array([0.05, 0.1]),
+                                npade=z.size).cpu().numpy()
+    for s, e in enumerate(ene):
+        yc, dyc = chain.analytic_continuation(om, y, e, npts=z.size)
+        np.testing.assert_allclose(out[s, 0] + 1j * out[s, 1], yc, rtol=1e-9)
+        np.testing.assert_allclose(out[s, 2] + 1j * out[s, 3], dyc, rtol=1e-7)
+        zz = 1 / (1 - dyc.real)
+        zz = 1.0 if (zz > 1 or zz < 0) else zz
+        want = e + zz * (yc.real + [0.2, 0.0][s] - [0.05, 0.1][s])
+        np.testing.assert_allclose(out[s, 5], want, rtol=1e-9)

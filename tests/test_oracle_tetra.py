@@ -186,3 +186,17 @@ def test_sorteq_classes():
         np.testing.assert_array_equal(out, np.asarray(v)[idx])
     out, idx, c, _ = tetra.sorteq([3.0, 1.0, 1.001, 1.002])
     assert abs(out[3] - 3.0) < 1e-15  # the odd one goes last in class 10
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built")
+def test_oracle_par3_equals_reference():
+    cfg = synth.small_config(div=(3, 3, 3), n_occ=3, n_emp=5, n_om=12, seed=41)
+    b = synth.band_energies(cfg)
+    fm = mesh.FrequencyMesh(12, 0.01, 5.0, iop_multiple=2)
+    _, dUl = mesh.svd_functions(fm.omega, fm.womeg, 0.005, 10.0, 40, 1e-6)
+    tetc = mesh.tetra_corners(cfg.div)
+    kq = mesh.k_minus_q(cfg.div)[:, 4]
+    win = (b.ncg, b.nomx, b.numin)
+    kl, st, _ = tetra.bz_calcqdepw(b.enk, 0.0, kq, *win, fm.omega, tetc, dUl=dUl)
+    assert st >= 0
+    np.testing.assert_array_equal(kl, ref.bz_calcqdepw_par3(b.enk, 0.0, kq, *win, dUl, fm.omega, tetc))
