@@ -1,13 +1,18 @@
 """Benchmark of the gfx950 polarization path: seconds per q-point.
 
-One step = one q-point of the workload: tetrahedron weights -> chi0
-contraction -> dielectric inversion (calc_eps of pygw.py:3039-3297 with
-Polarization_weights, pygw.py:3416-3476), inputs (bands, meshes, matrix
-elements of every q) resident in HBM before the timed region.  Default
-workload: 8x8x8 k-mesh, 16+48 bands, 64 Matsubara points, PB = 600, the 8 q
-= (i/8,0,0), FP64 (the "plain FP64 on GPU at production size" case).
+One step = one q-point: tetrahedron weights -> chi0 contraction ->
+dielectric inversion (calc_eps of pygw.py:3039-3297 with
+Polarization_weights, pygw.py:3416-3476), on inputs (bands, meshes, the
+matrix elements of every q) already resident in HBM.  With --config c5 the
+step continues through MWM, the spline + G0 (x) W convolution and Pade
+(calc_selfc, pygw.py:3299-3413; mcommon.py:80-210).
 
-    python bench.py [--steps K] [--warmup W] [--config c2] [--no-cpu]
+Default workload (no flags): 8x8x8 k-mesh, 16+48 bands, 64 Matsubara points
+log-spaced 0.005-200 eV, PB = 600, the 8 q = (i/8,0,0), FP64 -- the
+production-size FP64 case; the same steps are then repeated with the
+split-precision contraction and the relative errors against FP64 reported.
+
+    python bench.py [--steps K] [--warmup W] [--config c2|c3|c5] [--no-cpu]
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs one GPU.
 """
@@ -25,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6e12       # MI355X dense FP64 (vector and MFMA), FLOP/s
+PEAK_FP16 = 2.5e15        # MI355X dense FP16 MFMA, FLOP/s
 PEAK_HBM = 8.0e12         # B/s
 
 
@@ -32,96 +38,142 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2")
-    ap.add_argument("--nq", type=int, default=8)
+    ap.add_argument("--nq", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-kslab", type=int, default=2)
+    ap.add_argument("--fp64-kfrac", type=float, default=None,
+                    help="time the FP64 contraction on this fraction of k and extrapolate (large configs)")
     return ap.parse_args()
 
 
-def setup(cfg_name, nq):
-    import torch
+class Workload:
+    def __init__(self, name, nq, mixed, fp64_kfrac):
+        import torch
 
-    from pygw_amd import mesh, synth
+        from pygw_amd import mesh, synth
 
-    base = synth.CONFIGS[cfg_name]
-    div = base.div
-    qs = [(i * div[0] // nq, 0, 0) for i in range(nq)]
-    bands = synth.band_energies(base)
-    om, _ = synth.omega_mesh(base)
-    tetc = mesh.tetra_corners(div)
-    kqid = mesh.k_minus_q(div)
-    items = []
-    for q in qs:
-        cfg = synth.PolConfig(**{**base.__dict__, "q": q})
-        kq = kqid[:, synth.q_index(cfg)]
-        M = synth.device_matrix_elements(cfg, bands, kq, complex128=True)
-        items.append((cfg, kq, M))
-    torch.cuda.synchronize()
-    return base, bands, om, tetc, items
+        self.base = base = synth.CONFIGS[name]
+        div = base.div
+        nq = nq or (8 if name == "c2" else 1)
+        self.qs = [((i * div[0]) // max(nq, 1) if nq > 1 else 1, 0, 0) for i in range(nq)]
+        self.bands = synth.band_energies(base)
+        self.om, self.wom = synth.omega_mesh(base)
+        self.tetc = mesh.tetra_corners(div)
+        kqid = mesh.k_minus_q(div)
+        self.items = []
+        nk = base.nk
+        self.nk64 = nk if fp64_kfrac is None else max(1, int(round(nk * fp64_kfrac)))
+        for q in self.qs:
+            cfg = synth.PolConfig(**{**base.__dict__, "q": q})
+            kq = kqid[:, synth.q_index(cfg)]
+            M64 = synth.device_matrix_elements(cfg, self.bands, kq, nkc=self.nk64, complex128=True)
+            M32 = synth.device_matrix_elements(cfg, self.bands, kq, complex128=False) if mixed else None
+            self.items.append((cfg, kq, M64, M32))
+        torch.cuda.synchronize()
 
 
-def run_step(P, bands, om, tetc, kq, M, ev=None):
+def step(P, wl, it, precision, ev=None, keep=False):
     import torch
     s = torch.cuda.current_stream()
-    if ev is not None:
-        ev[0].record(s)
-    kcw = P.bz_calcqdepw_par2(bands.enk, 0.0, kq, bands.ncg, bands.nomx, bands.numin, om, tetc,
-                              layout="native", check=False)
-    if ev is not None:
-        ev[1].record(s)
-    eps = P.chi0_accumulate(M, kcw)
-    if ev is not None:
-        ev[2].record(s)
+    cfg, kq, M64, M32 = it
+    b = wl.bands
+    rec = lambda i: ev[i].record(s) if ev is not None else None  # noqa: E731
+    rec(0)
+    kcw = P.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, wl.om, wl.tetc, layout="native", check=False)
+    rec(1)
+    if precision == "fp64":
+        eps = P.chi0_accumulate(M64, kcw)
+    else:
+        eps = P.chi0_accumulate(M32, kcw, precision="mixed")
+    rec(2)
+    chi = eps.clone() if keep else None
     P.eps_inverse(eps, check=False)
-    if ev is not None:
-        ev[3].record(s)
-    return eps
+    rec(3)
+    return chi, eps
 
 
-def cpu_baseline(cfg, bands, om, tetc, kq, kslab):
+def timed(P, wl, precision, steps, warmup):
+    import torch
+    n = len(wl.items)
+    for i in range(warmup):
+        step(P, wl, wl.items[i % n], precision)
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(P, wl, wl.items[i % n], precision, evs[i])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = np.array([[e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2]), e[2].elapsed_time(e[3])] for e in evs]) / 1e3
+    return wall / steps, st.mean(axis=0)
+
+
+def rel_errors(P, wl):
+    """Per-frequency relative Frobenius error of chi0 and eps^-1, mixed vs FP64
+    (first q).  Both contractions see the same complex64-rounded M."""
+    import torch
+    cfg, kq, M64, M32 = wl.items[0]
+    nk = M64.shape[0]
+    it64 = (cfg, kq, M32[:nk].to(torch.complex128), M32)
+    c64, e64 = step(P, wl, it64, "fp64", keep=True)
+    itm = (cfg, kq, M64, M32[:nk].contiguous())
+    cmx, emx = step(P, wl, itm, "mixed", keep=True)
+    torch.cuda.synchronize()
+
+    def rel(a, b):
+        d = torch.linalg.matrix_norm((a - b).permute(2, 0, 1))
+        return (d / torch.linalg.matrix_norm(b.permute(2, 0, 1))).cpu().numpy()
+    one = torch.eye(c64.shape[0], dtype=c64.dtype, device=c64.device)[:, :, None]
+    r_chi = rel(cmx, c64)
+    r_inv = rel(emx + one, e64 + one)
+    return {"chi0_max": float(r_chi.max()), "chi0_median": float(np.median(r_chi)),
+            "eps_inv_max": float(r_inv.max()), "eps_inv_median": float(np.median(r_inv)),
+            "k_points_compared": int(nk)}
+
+
+def cpu_baseline(wl, kslab):
     """PyGW's own Fortran weights (oracle/_ref, OpenMP) for one q, plus the
     calc_eps numpy loop (per (k, w) ZGEMM, then inv per w) on a k-slab and a
     few frequencies, extrapolated linearly in N_k and n_om."""
     from oracle import chain, ref
+    from pygw_amd import synth
 
+    cfg, kq, _, _ = wl.items[0]
+    b, om, tetc = wl.bands, wl.om, wl.tetc
     nthr = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    out = {"cores": nthr}
     t = time.perf_counter()
     if ref.available("libref_tetra_omp.so"):
-        kcw = ref.bz_calcqdepw_par2(bands.enk, 0.0, kq, bands.ncg, bands.nomx, bands.numin, om, tetc, omp=True)
+        kcw = ref.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc, omp=True)
         kind = "reference"
     else:
         from oracle import tetra
-        kcw, _, _ = tetra.bz_calcqdepw(bands.enk, 0.0, kq, bands.ncg, bands.nomx, bands.numin, om, tetc,
-                                       nthreads=nthr)
+        kcw, _, _ = tetra.bz_calcqdepw(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc, nthreads=nthr)
         kind = "port"
     t_w = time.perf_counter() - t
-    from pygw_amd import synth
-    M = synth.matrix_elements(cfg, bands, kq, ks=range(kslab))
-    no, ne = bands.n_o, bands.n_e
+    M = synth.matrix_elements(cfg, b, kq, ks=range(kslab))
+    no, ne = b.n_o, b.n_e
     t = time.perf_counter()
-    eps = chain.calc_eps_accumulate(((i, M[i]) for i in range(kslab)),
-                                    lambda ik: kcw[:, :, ik, :].reshape(no * ne, -1))
+    eps = chain.calc_eps_accumulate(((i, M[i]) for i in range(kslab)), lambda ik: kcw[:, :, ik, :].reshape(no * ne, -1))
     t_chi = (time.perf_counter() - t) * cfg.nk / kslab
     nw = 4
     t = time.perf_counter()
     chain.eps_inverse(eps[:, :, :nw])
     t_inv = (time.perf_counter() - t) * cfg.n_om / nw
-    total = t_w + t_chi + t_inv
-    out.update({"value": total, "unit": "s/q", "kind": kind,
-                "sample": (f"one q of {cfg.name}: weights timed in full ({'PyGW Fortran, OpenMP' if kind == 'reference' else 'C restatement'}); "
-                           f"chi0 numpy ZGEMM loop of calc_eps on {kslab} of {cfg.nk} k extrapolated x{cfg.nk / kslab:.0f}; "
-                           f"inverse on {nw} of {cfg.n_om} frequencies extrapolated"),
-                "stages_s": {"weights": t_w, "chi0(extrap)": t_chi, "inverse(extrap)": t_inv}})
-    return out
+    return {"value": t_w + t_chi + t_inv, "unit": "s/q", "cores": nthr, "kind": kind,
+            "sample": (f"one q of {cfg.name}: weights in full (PyGW's Fortran bz_calcqdepw_par2, OpenMP, from "
+                       f"oracle/_ref); chi0 = calc_eps numpy ZGEMM loop on {kslab} of {cfg.nk} k, extrapolated "
+                       f"x{cfg.nk / kslab:.0f}; inverse on {nw} of {cfg.n_om} frequencies, extrapolated"
+                       if kind == "reference" else "C restatement of the weights + numpy calc_eps loop"),
+            "stages_s": {"weights": t_w, "chi0_extrapolated": t_chi, "inverse_extrapolated": t_inv}}
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding over q is design-only",
+        print(json.dumps({"error": "this build runs on one GPU; sharding over q-points is design-only",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
@@ -129,44 +181,50 @@ def main():
     from pygw_amd import polarization as P
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
-    cfg, bands, om, tetc, items = setup(args.config, args.nq)
-    no, ne, pb, nom, nk = bands.n_o, bands.n_e, cfg.pb, cfg.n_om, cfg.nk
-    for i in range(args.warmup):
-        c, kq, M = items[i % len(items)]
-        run_step(P, bands, om, tetc, kq, M)
-    torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        c, kq, M = items[i % len(items)]
-        run_step(P, bands, om, tetc, kq, M, evs[i])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    st = np.array([[e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2]),
-                    e[2].elapsed_time(e[3])] for e in evs]) / 1e3
-    mean = st.mean(axis=0)
-    t_w, t_chi, t_inv = mean
-    ms_step = wall / args.steps * 1e3
+    mixed = not args.no_mixed
+    wl = Workload(args.config, args.nq, mixed, args.fp64_kfrac)
+    cfg, b = wl.base, wl.bands
+    no, ne, pb, nom, nk = b.n_o, b.n_e, cfg.pb, cfg.n_om, cfg.nk
+    steps = args.steps if args.config == "c2" else min(args.steps, 2)
+    s64, st64 = timed(P, wl, "fp64", steps, args.warmup)
+    kscale = nk / wl.nk64
+    t_chi64 = st64[1] * kscale
     K = nk * no * ne
     f_chi = 4.0 * pb * pb * K * nom
-    kcw_bytes = 8.0 * no * ne * nk * nom + 8.0 * bands.enk.size + 16.0 * 6 * nk + 4.0 * nk
+    f_chi_launch = 4.0 * pb * pb * wl.nk64 * no * ne * nom
+    kcw_bytes = 8.0 * no * ne * nk * nom + 8.0 * b.enk.size + 16.0 * 6 * nk + 4.0 * nk
+    s_per_q64 = st64[0] + t_chi64 + st64[2]
     rec = {
-        "metric": "chi0+eps^-1 seconds per q-point (FP64 GPU path)",
-        "value": wall / args.steps, "unit": "s/q", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_step, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "metric": "chi0+eps^-1 seconds per q-point (FP64 GPU path; split-precision path alongside)",
+        "value": s_per_q64, "unit": "s/q", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": s64 * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{cfg.name}: {'x'.join(map(str, cfg.div))} k-mesh, {no}+{ne} bands, "
-                               f"{nom} Matsubara points, PB={pb}, {len(items)} q-points", "precision": "fp64"},
-        "stages_s": {"polarization_weights": t_w, "chi0": t_chi, "eps_inverse": t_inv},
-        "roofline": {"bound": "mfma", "kernel": "pygw_chi0_f64", "achieved": f_chi / t_chi / 1e12,
-                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": f_chi / t_chi / PEAK_FP64,
-                     "traffic": None, "flops_per_launch": f_chi},
-        "weights_roofline": {"bound": "hbm", "achieved": kcw_bytes / t_w / 1e9, "peak": PEAK_HBM / 1e9,
-                             "unit": "GB/s", "frac": kcw_bytes / t_w / PEAK_HBM},
+        "config": {"workload": f"{cfg.name}: {'x'.join(map(str, cfg.div))} k-mesh, {no}+{ne} bands, {nom} Matsubara "
+                               f"points, PB={pb}, {len(wl.items)} q-point(s)", "precision": "fp64",
+                   "fp64_k_timed": wl.nk64},
+        "stages_s": {"polarization_weights": st64[0], "chi0": t_chi64, "eps_inverse": st64[2]},
+        "roofline": {"bound": "mfma", "kernel": "pygw_chi0_f64", "achieved": f_chi_launch / st64[1] / 1e12,
+                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": f_chi_launch / st64[1] / PEAK_FP64,
+                     "traffic": None, "flops_per_launch": f_chi_launch},
+        "weights_roofline": {"bound": "hbm", "kernel": "pygw_tw_gather", "achieved": kcw_bytes / st64[0] / 1e9,
+                             "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": kcw_bytes / st64[0] / PEAK_HBM,
+                             "bytes_per_launch": kcw_bytes},
     }
+    if wl.nk64 != nk:
+        rec["note"] = f"FP64 contraction timed on {wl.nk64} of {nk} k-points and scaled x{kscale:.2f} (cost is linear in N_k)"
+    if mixed:
+        smx, stmx = timed(P, wl, "mixed", steps, args.warmup)
+        s_per_qmx = stmx.sum()
+        rec["mixed"] = {"s_per_q": s_per_qmx, "ms_per_step": smx * 1e3,
+                        "stages_s": {"polarization_weights": stmx[0], "chi0": stmx[1], "eps_inverse": stmx[2]},
+                        "speedup_vs_fp64": s_per_q64 / s_per_qmx, "chi0_speedup_vs_fp64": t_chi64 / stmx[1],
+                        "roofline": {"bound": "mfma", "kernel": "pygw_chi0_mixed",
+                                     "achieved": f_chi / stmx[1] / 1e12, "peak": PEAK_FP16 / 1e12,
+                                     "unit": "TFLOP/s", "frac_algorithmic": f_chi / stmx[1] / PEAK_FP16,
+                                     "frac_executed_3_products": 3 * f_chi / stmx[1] / PEAK_FP16}}
+        rec["mixed"]["rel_frobenius_error_vs_fp64"] = rel_errors(P, wl)
     if not args.no_cpu:
-        c, kq, M = items[1 % len(items)]
-        rec["cpu_baseline"] = cpu_baseline(c, bands, om, tetc, kq, args.cpu_kslab)
+        rec["cpu_baseline"] = cpu_baseline(wl, args.cpu_kslab)
     print(json.dumps(rec))
 
 

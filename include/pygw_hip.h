@@ -100,11 +100,14 @@ int pygw_eps_inverse(void* eps, int pb, int n_om, void* workspace, size_t ws_byt
  * MWM contraction (calc_selfc, pygw.py:3353-3358):
  *   mwm[k][w][r] = wkq * sum_{I,J} conj(M[k][r][I]) W[I,J,w] M[k][r][J]
  * Replaces `la.matmul(cminm2, eps[:,:,iom])` and the row sum.
- *   M    complex128 [n_k][rows_per_k][pb], rows r = ib1*nb2 + ib2 (minm2 of
- *        the self-energy bands, pygw.py:3349-3351)
+ *   M    [n_k][rows_per_k][pb], rows r = ib1*nb2 + ib2 (minm2 of the
+ *        self-energy bands, pygw.py:3349-3351); complex128 for precision 0,
+ *        complex64 for precision 1
  *   W    complex128 Fortran [pb, pb, n_om] (eps after pygw_eps_inverse)
  *   out  complex128 [n_k][n_om][rows_per_k] (PyGW's mwm[iom, ib1, ib2] per k)
- *   precision 0 = FP64 MFMA
+ *   precision 0 = FP64 MFMA; 1 = FP16 hi+lo split (rows of M and columns
+ *   of W scaled by powers of two), 3 products, FP32 accumulation over I,
+ *   FP64 row sum over J
  * ------------------------------------------------------------------------ */
 int pygw_mwm(const void* M, int n_k, int rows_per_k, int pb, const void* W, int n_om, double wkq, int precision,
              void* mwm_out, void* stream);

@@ -63,7 +63,7 @@ extern "C" int pygw_eps_inverse(void* eps, int pb, int n_om, void* workspace, si
     return PYGW_E_ARG;
   }
   int* dinfo = (int*)((char*)ws + need - 256 - sizeof(int) * (size_t)n_om);
-  HIP_TRY(hipMemsetAsync(dinfo, 0, sizeof(int) * (size_t)n_om, stream));
+  hipLaunchKernelGGL(pygw_fill_i32, dim3(ceil_div(n_om, 256)), dim3(256), 0, stream, dinfo, n_om, 0);
   int rc = eps_inverse_launch(eps, pb, n_om, ws, dinfo, stream);
   if (rc == PYGW_OK && info) {
     int* h = (int*)malloc(sizeof(int) * (size_t)n_om);

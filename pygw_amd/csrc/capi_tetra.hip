@@ -17,6 +17,10 @@ static size_t tw_ws_bytes(int n_k) {
   return (b + 255) & ~(size_t)255;
 }
 
+__global__ void pygw_fill_i32(int* __restrict__ x, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = v;
+}
+
 extern "C" int pygw_version(void) { return 100; }
 
 extern "C" int pygw_tetra_weights_workspace(int n_k, size_t* bytes) {
@@ -48,7 +52,7 @@ extern "C" int pygw_tetra_weights(const double* enk, double ef, const int* kqid,
   int* inc = (int*)ws;
   int* cnt = inc + (size_t)n_k * kMaxInc;
   int* dstat = cnt + n_k;
-  HIP_TRY(hipMemsetAsync(dstat, 0, 2 * sizeof(int), stream));
+  hipLaunchKernelGGL(pygw_fill_i32, dim3(1), dim3(64), 0, stream, dstat, 2, 0);
   hipLaunchKernelGGL(pygw_tw_incidence, dim3(ceil_div(n_k, 256)), dim3(256), 0, stream, tetc, n_tet, n_k, inc, cnt,
                      kMaxInc);
   const int np = no * ne;

@@ -19,3 +19,7 @@
 #define F32C(x) ((double)(float)(x))
 
 static inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Zero-fill kernel for small device counters (used instead of hipMemsetAsync
+// so that callers may capture the launches in a HIP graph).
+__global__ void pygw_fill_i32(int* __restrict__ x, int n, int v);

@@ -128,21 +128,23 @@ def eps_inverse(eps, *, device="cuda", check=True):
     return eps
 
 
-def calc_mwm(M_selfe, W, wkq, *, device="cuda"):
+def calc_mwm(M_selfe, W, wkq, *, precision="fp64", device="cuda"):
     """MWM of calc_selfc (pygw.py:3353-3358) for all k at once.
 
     M_selfe: complex [n_k, nb1*nb2, PB]; W: Fortran [PB, PB, n_om] (eps after
-    eps_inverse).  Returns complex128 [n_k, n_om, nb1*nb2]."""
+    eps_inverse).  precision "fp64" or "mixed" (M as complex64, FP16 hi+lo
+    MFMA).  Returns complex128 [n_k, n_om, nb1*nb2]."""
     device = _require_device(device)
-    M = M_selfe.to(device=device, dtype=torch.complex128).contiguous() if isinstance(M_selfe, torch.Tensor) else \
-        torch.as_tensor(np.ascontiguousarray(M_selfe), dtype=torch.complex128, device=device)
+    dt = {"fp64": torch.complex128, "mixed": torch.complex64}[precision]
+    M = M_selfe.to(device=device, dtype=dt).contiguous() if isinstance(M_selfe, torch.Tensor) else \
+        torch.as_tensor(np.ascontiguousarray(M_selfe), dtype=dt, device=device)
     nk, rpk, pb = M.shape
     if W.shape[0] != pb or W.stride() != (1, pb, pb * pb) or W.dtype != torch.complex128:
         raise ValueError("W must be a Fortran-strided [PB, PB, n_om] complex128 tensor")
     nom = W.shape[2]
     out = torch.empty((nk, nom, rpk), dtype=torch.complex128, device=device)
-    _lib.call("pygw_mwm", C.c_void_p(M.data_ptr()), nk, rpk, pb, C.c_void_p(W.data_ptr()), nom, float(wkq), 0,
-              C.c_void_p(out.data_ptr()), _stream_ptr(device))
+    _lib.call("pygw_mwm", C.c_void_p(M.data_ptr()), nk, rpk, pb, C.c_void_p(W.data_ptr()), nom, float(wkq),
+              {"fp64": 0, "mixed": 1}[precision], C.c_void_p(out.data_ptr()), _stream_ptr(device))
     return out
 
 

@@ -82,3 +82,23 @@ def test_pade_qp_vs_reference_fixture():
         zz = 1.0 if (zz > 1 or zz < 0) else zz
         want = e + zz * (yc.real + [0.2, 0.0][s] - [0.05, 0.1][s])
         np.testing.assert_allclose(out[s, 5], want, rtol=1e-9)
+
+
+def test_mwm_mixed_vs_fp64():
+    """Split-precision MWM against the FP64 oracle: relative error per
+    frequency <= 1e-6 (M rounded to complex64 on both sides)."""
+    import torch
+    from pygw_amd import polarization as P
+    rng = np.random.default_rng(77)
+    nk, nb1, nb2, pb, nom = 2, 4, 9, 133, 3
+    M = (rng.normal(size=(nk, nb1 * nb2, pb)) + 1j * rng.normal(size=(nk, nb1 * nb2, pb))).astype(np.complex64)
+    W = _herm_w(pb, nom, rng)
+    print("case mwm mixed", flush=True)
+    Wt = torch.as_tensor(W.reshape(-1, order="F"), device="cuda").as_strided((pb, pb, nom), (1, pb, pb * pb))
+    got = P.calc_mwm(M, Wt, 0.5, precision="mixed").cpu().numpy()
+    for k in range(nk):
+        want = chain.calc_mwm(M[k].astype(complex), W, 0.5)
+        for w in range(nom):
+            err = _rel(got[k, w], want[w])
+            print(f"  k={k} w={w} rel={err:.2e}", flush=True)
+            assert err < 1e-6
