@@ -170,6 +170,94 @@ def cpu_baseline(wl, kslab):
             "stages_s": {"weights": t_w, "chi0_extrapolated": t_chi, "inverse_extrapolated": t_inv}}
 
 
+def run_chain(args):
+    """Full chain at C5: weights -> chi0 -> eps^-1 -> MWM (8 GW bands x all
+    bands, 64 output k = the 4x4x4 sub-lattice) -> spline onto the 20x
+    precise tan mesh -> G0 (x) W convolution; Pade + QP after the q-loop.
+    FP64 and split precision (chi0 and MWM) side by side."""
+    import torch
+
+    from pygw_amd import mesh, synth
+    from pygw_amd import polarization as P
+
+    base = synth.CONFIGS["c5"]
+    b = synth.band_energies(base)
+    fm = mesh.FrequencyMesh(base.n_om, base.wmin_ev / mesh.H2EV, base.wmax_ev / mesh.H2EV, iop_multiple=20)
+    om, wo, omp, wop = fm.omega, fm.womeg, fm.omega_precise, fm.womeg_precise
+    tetc = mesh.tetra_corners(base.div)
+    kqid = mesh.k_minus_q(base.div)
+    div = np.array(base.div)
+    pts = np.array([[i, j, k] for i in range(div[0]) for j in range(div[1]) for k in range(div[2])])
+    klist = np.where((pts % 2 == 0).all(1))[0]          # 4x4x4 sub-lattice, 64 k
+    nbgw = 8
+    ibgw = b.n_o - nbgw // 2                               # 4 highest occupied + 4 lowest empty
+    nb2 = b.enk.shape[0]
+    nq = args.nq or 1
+    qs = [((i + 1) % div[0], 0, 0) for i in range(nq)]
+    wkq = 1.0 / base.nk
+    res = {}
+    for prec in ("fp64", "mixed"):
+        sig = torch.zeros((len(klist), nbgw, base.n_om), dtype=torch.complex128, device="cuda")
+        times = []
+        for q in qs:
+            cfg = synth.PolConfig(**{**base.__dict__, "q": q})
+            kq = kqid[:, synth.q_index(cfg)]
+            M = synth.device_matrix_elements(cfg, b, kq, complex128=(prec == "fp64"))
+            e2 = np.asfortranarray(np.hstack([b.enk[:, klist], b.enk[:, kq[klist]]]))
+            b2 = synth.Bands(e2, b.ncg, b.nomx, b.numin)
+            kq2 = np.arange(len(klist), 2 * len(klist), dtype=np.int32)
+            kq2 = np.concatenate([kq2, kq2])
+            Ms = synth.device_matrix_elements(cfg, b2, kq2, nkc=len(klist), complex128=(prec == "fp64"),
+                                              rows=(ibgw, nbgw, 0, nb2), seed=cfg.seed + 99)
+            enk_q = torch.as_tensor(np.ascontiguousarray(b.enk[:, kq[klist]].T), device="cuda")
+            torch.cuda.synchronize()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+            s = torch.cuda.current_stream()
+            ev[0].record(s)
+            kcw = P.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc, layout="native", check=False)
+            ev[1].record(s)
+            eps = P.chi0_accumulate(M, kcw, precision=prec)
+            ev[2].record(s)
+            P.eps_inverse(eps, check=False)
+            ev[3].record(s)
+            mwm = P.calc_mwm(Ms, eps, wkq, precision=prec)
+            ev[4].record(s)
+            sc = P.compute_selfc_inside(enk_q, mwm.view(len(klist), base.n_om, nbgw, nb2), om, wo, omp, wop)
+            ev[5].record(s)
+            sig += sc
+            torch.cuda.synchronize()
+            times.append([ev[i].elapsed_time(ev[i + 1]) / 1e3 for i in range(5)])
+            del M, Ms, eps, kcw, mwm
+        ene = np.ascontiguousarray(b.enk[ibgw:ibgw + nbgw][:, klist].T).reshape(-1)
+        t0 = time.perf_counter()
+        qp = P.pade_quasiparticles(sig.view(-1, base.n_om), om, ene).cpu().numpy()
+        t_pade = time.perf_counter() - t0
+        tm = np.array(times).mean(0)
+        res[prec] = {"s_per_q": float(tm.sum()), "sig": sig.cpu().numpy(), "qp": qp,
+                     "stages_s": dict(zip(["polarization_weights", "chi0", "eps_inverse", "mwm", "selfc_convolution"],
+                                          map(float, tm))), "pade_qp_s": t_pade}
+    s64, smx = res["fp64"], res["mixed"]
+    rel_sig = np.linalg.norm(smx["sig"] - s64["sig"], axis=(0, 1)) / np.linalg.norm(s64["sig"], axis=(0, 1))
+    dqp = np.abs(smx["qp"][:, 5] - s64["qp"][:, 5]) * mesh.H2EV * 1e3
+    nk, no, ne, pb, nom = base.nk, b.n_o, b.n_e, base.pb, base.n_om
+    f_chi = 4.0 * pb * pb * nk * no * ne * nom
+    rec = {"metric": "chi0+eps^-1+Sigma_c seconds per q-point (full chain)", "value": s64["s_per_q"], "unit": "s/q",
+           "n_gpus": 1, "steps": len(qs), "warmup": 0, "ms_per_step": s64["s_per_q"] * 1e3, "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"c5: 8x8x8 k-mesh, {no}+{ne} bands, PB={pb}, {nom} tan-mesh Matsubara points, "
+                                  f"Sigma_c of {nbgw} bands at {len(klist)} k, precise mesh x20 ({omp.size} points)",
+                      "precision": "fp64"},
+           "stages_s": s64["stages_s"],
+           "roofline": {"bound": "mfma", "kernel": "pygw_chi0_f64", "achieved": f_chi / s64["stages_s"]["chi0"] / 1e12,
+                        "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                        "frac": f_chi / s64["stages_s"]["chi0"] / PEAK_FP64, "traffic": None},
+           "mixed": {"s_per_q": smx["s_per_q"], "stages_s": smx["stages_s"],
+                     "speedup_vs_fp64": s64["s_per_q"] / smx["s_per_q"],
+                     "sigma_c_rel_frobenius_vs_fp64_max": float(rel_sig.max()),
+                     "qp_shift_abs_diff_meV_max": float(dqp.max()), "qp_shift_abs_diff_meV_mean": float(dqp.mean())}}
+    print(json.dumps(rec))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -181,6 +269,9 @@ def main():
     from pygw_amd import polarization as P
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
+    if args.config == "c5":
+        run_chain(args)
+        return
     mixed = not args.no_mixed
     wl = Workload(args.config, args.nq, mixed, args.fp64_kfrac)
     cfg, b = wl.base, wl.bands

@@ -149,12 +149,13 @@ int pygw_pade_qp(const void* sigc, int n_state, int n_om, const double* omega, i
  * PyGW (its M comes from calc_minm, pygw.py:2547-2721); for callers that
  * need seeded inputs of the benchmark shapes without a DFT calculation.
  *   out[k-k0][p][I], k = k0..k0+n_kc-1, p = i_o*n_e + i_e:
- *   complex Gaussian(seed, counter) * exp(-|e_o(k)-e_e(k-q)|/20 eV)
- *   / sqrt(pb) * vhalf[I]; complex128 if complex128 != 0 else complex64.
+ *   complex Gaussian(seed, counter) * exp(-|e_n(k)-e_m(k-q)|/20 eV)
+ *   / sqrt(pb) * vhalf[I], n = row_band0 + i_o, m = col_band0 + i_e;
+ *   complex128 if complex128 != 0 else complex64.
  * ------------------------------------------------------------------------ */
-int pygw_synth_minm(const double* enk, int n_bnd, const int* kqid, int ncg, int numin, int n_o, int n_e, int pb,
-                    const double* vhalf, unsigned long long seed, int k0, int n_kc, int complex128, void* out,
-                    void* stream);
+int pygw_synth_minm(const double* enk, int n_bnd, const int* kqid, int row_band0, int col_band0, int n_o, int n_e,
+                    int pb, const double* vhalf, unsigned long long seed, int k0, int n_kc, int complex128,
+                    void* out, void* stream);
 
 #ifdef __cplusplus
 }

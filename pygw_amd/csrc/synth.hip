@@ -1,8 +1,11 @@
 // Seeded synthetic matrix elements on the device (benchmark input generator;
 // no counterpart in PyGW, whose M comes from calc_minm, pygw.py:2547-2721).
 //
-//   M[k][p][I] = g(seed, (k*n_pairs + p)*pb + I) * exp(-|e_o(k) - e_e(k-q)|/20 eV)
-//                / sqrt(pb) * vhalf[I],     p = i_o*n_e + i_e
+//   M[k][p][I] = g(seed, (k*n_pairs + p)*pb + I) * exp(-|e_n(k) - e_m(k-q)|/20 eV)
+//                / sqrt(pb) * vhalf[I],     p = i_o*n_e + i_e,
+// n = row_band0 + i_o, m = col_band0 + i_e (0-based band indices): the
+// polarization pairs use row_band0 = 0, col_band0 = ncg + numin; the
+// self-energy rows use the GW bands against all bands.
 // g = complex Gaussian with Re, Im ~ N(0, 1/2): SplitMix64 of the counter,
 // Box-Muller.  pygw_amd/synth.py holds the host twin of the same draws.
 #include "common.h"
@@ -17,7 +20,7 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
 
 // enk: Fortran (nbnd, nk); out complex128 (cplx=1) or complex64 (cplx=0)
 __global__ void __launch_bounds__(256) pygw_synth_minm_k(const double* __restrict__ enk, int nbnd, const int* __restrict__ kqid,
-                                                         int ncg, int numin, int no, int ne, int pb,
+                                                         int row0, int col0, int no, int ne, int pb,
                                                          const double* __restrict__ vhalf, uint64_t seed, int k0, int nkc,
                                                          int cplx, void* __restrict__ out) {
   const long np = (long)no * ne;
@@ -32,8 +35,8 @@ __global__ void __launch_bounds__(256) pygw_synth_minm_k(const double* __restric
     const int p = (int)(r / pb), I = (int)(r % pb);
     const int k = k0 + kl;
     const int io = p / ne, ie = p % ne;
-    const double eo = enk[io + (size_t)nbnd * k];
-    const double ee = enk[(ncg + numin + ie) + (size_t)nbnd * kqid[k]];
+    const double eo = enk[(row0 + io) + (size_t)nbnd * k];
+    const double ee = enk[(col0 + ie) + (size_t)nbnd * kqid[k]];
     const double sc = exp(-fabs(eo - ee) * h2ev / 20.0) * rs * vhalf[I];
     const uint64_t idx = (uint64_t)k * (uint64_t)per_k + (uint64_t)r;
     const uint64_t a = splitmix64(base ^ (idx * 2ull));
@@ -49,16 +52,16 @@ __global__ void __launch_bounds__(256) pygw_synth_minm_k(const double* __restric
   }
 }
 
-extern "C" int pygw_synth_minm(const double* enk, int n_bnd, const int* kqid, int ncg, int numin, int n_o, int n_e,
-                               int pb, const double* vhalf, unsigned long long seed, int k0, int n_kc, int complex128,
-                               void* out, void* stream_) {
+extern "C" int pygw_synth_minm(const double* enk, int n_bnd, const int* kqid, int row_band0, int col_band0, int n_o,
+                               int n_e, int pb, const double* vhalf, unsigned long long seed, int k0, int n_kc,
+                               int complex128, void* out, void* stream_) {
   hipStream_t stream = (hipStream_t)stream_;
   if (!enk || !kqid || !vhalf || !out || n_o <= 0 || n_e <= 0 || pb <= 0 || n_kc <= 0 || k0 < 0) return PYGW_E_ARG;
-  if (ncg + numin + n_e > n_bnd || n_o > n_bnd) return PYGW_E_ARG;
+  if (row_band0 < 0 || col_band0 < 0 || col_band0 + n_e > n_bnd || row_band0 + n_o > n_bnd) return PYGW_E_ARG;
   const long n = (long)n_o * n_e * pb * n_kc;
   int blocks = (int)((n + 255) / 256);
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(pygw_synth_minm_k, dim3(blocks), dim3(256), 0, stream, enk, n_bnd, kqid, ncg, numin, n_o, n_e,
+  hipLaunchKernelGGL(pygw_synth_minm_k, dim3(blocks), dim3(256), 0, stream, enk, n_bnd, kqid, row_band0, col_band0, n_o, n_e,
                      pb, vhalf, (uint64_t)seed, k0, n_kc, complex128, out);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }

@@ -676,7 +676,9 @@ __device__ void small_tet(const double cr[4][3], const double* db, int corner, c
   int ind[4], cls;
   if (!sort_eq(dsort, ind, &cls)) { *status |= ST_GEOM; return; }
   double tl[OMC];
+#pragma unroll
   for (int w = 0; w < OMC; w++) tl[w] = 0.0;
+#pragma unroll 1
   for (int j = 0; j < 4; j++) {  // original small-vertex order (transfer sum order)
     double beta = (corner == 0) ? (1.0 - cr[j][0] - cr[j][1] - cr[j][2]) : cr[j][corner - 1];
     if (beta == 0.0) continue;
@@ -686,19 +688,26 @@ __device__ void small_tet(const double cr[4][3], const double* db, int corner, c
     vertex_coef(c, cls, dsort, v, compat);
     double s1, n03;
     taylor_coef(ds, j, s1, n03);
-#pragma unroll
+    // one frequency per iteration (not unrolled: the closed form is long);
+    // the accumulators rotate so that slot 0 always belongs to frequency w
+#pragma unroll 1
     for (int w = 0; w < OMC; w++) {
-      int jj = c0 + w;
-      if (jj < jlo || jj >= jhi) continue;
-      double om = omega[jj - shift];
-      double wv;
-      if (om > thr) {
-        double o2 = om * om;
-        wv = s1 / (6.0e1 * o2) + n03 / (4.2e2 * (o2 * o2));
-      } else {
-        wv = vertex_eval(c, om, nwarn);
+      const int jj = c0 + w;
+      double t0 = tl[0];
+      if (jj >= jlo && jj < jhi) {
+        const double om = omega[jj - shift];
+        double wv;
+        if (om > thr) {
+          const double o2 = om * om;
+          wv = s1 / (6.0e1 * o2) + n03 / (4.2e2 * (o2 * o2));
+        } else {
+          wv = vertex_eval(c, om, nwarn);
+        }
+        t0 = t0 + wv * beta;
       }
-      tl[w] = tl[w] + wv * beta;
+#pragma unroll
+      for (int i = 0; i < OMC - 1; i++) tl[i] = tl[i + 1];
+      tl[OMC - 1] = t0;
     }
   }
 #pragma unroll

@@ -205,10 +205,11 @@ def matrix_elements(cfg: PolConfig, bands: Bands, kqid_col: np.ndarray, ks=None,
 
 
 def device_matrix_elements(cfg: PolConfig, bands: Bands, kqid_col, k0=0, nkc=None, complex128=True,
-                           device="cuda", out=None):
+                           device="cuda", out=None, rows=None, seed=None):
     """M[k0:k0+nkc] generated on the device (same draws as matrix_elements
     up to last-ulp differences of log/sincos).  Returns a torch tensor
-    [nkc, n_o*n_e, PB]."""
+    [nkc, n_rows*n_cols, PB].  rows = (row_band0, n_rows, col_band0, n_cols)
+    selects other band windows (default: the polarization pairs)."""
     import ctypes as C
 
     import torch
@@ -216,7 +217,10 @@ def device_matrix_elements(cfg: PolConfig, bands: Bands, kqid_col, k0=0, nkc=Non
     from . import _lib
 
     nkc = cfg.nk - k0 if nkc is None else nkc
-    no, ne, pb = bands.n_o, bands.n_e, cfg.pb
+    pb = cfg.pb
+    if rows is None:
+        rows = (0, bands.n_o, bands.ncg + bands.numin, bands.n_e)
+    r0, no, c0, ne = rows
     dt = torch.complex128 if complex128 else torch.complex64
     enk = torch.as_tensor(np.ascontiguousarray(bands.enk.T), dtype=torch.float64, device=device)
     kq = torch.as_tensor(np.asarray(kqid_col, dtype=np.int32), device=device)
@@ -224,7 +228,7 @@ def device_matrix_elements(cfg: PolConfig, bands: Bands, kqid_col, k0=0, nkc=Non
     if out is None:
         out = torch.empty((nkc, no * ne, pb), dtype=dt, device=device)
     _lib.call("pygw_synth_minm", C.c_void_p(enk.data_ptr()), bands.enk.shape[0], C.c_void_p(kq.data_ptr()),
-              bands.ncg, bands.numin, no, ne, pb, C.c_void_p(vh.data_ptr()), C.c_ulonglong(cfg.seed), k0, nkc,
+              r0, c0, no, ne, pb, C.c_void_p(vh.data_ptr()), C.c_ulonglong(cfg.seed if seed is None else seed), k0, nkc,
               1 if complex128 else 0, C.c_void_p(out.data_ptr()),
               C.c_void_p(torch.cuda.current_stream().cuda_stream))
     torch.cuda.current_stream().synchronize()
