@@ -85,6 +85,17 @@ int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, cons
                          int n_k, double coef, int precision, int beta, void* eps, void* workspace,
                          size_t ws_bytes, void* stream);
 
+/* Autotuner (no counterpart in PyGW): times the contraction's tile schedules
+ * (1, 2 or 4 Matsubara frequencies per 64x64 tile block) on the given
+ * inputs and records the fastest for (device, precision, pb, n_pairs, n_om);
+ * later pygw_chi0_accumulate calls with that shape use it.  eps is
+ * overwritten (scratch).  best: chosen frequencies per block; times[3]
+ * (optional): milliseconds of each candidate. */
+int pygw_chi0_autotune(const void* M, long stride_k, int n_pairs, int pb, const double* kcw, int n_om, int k0,
+                       int n_k, double coef, int precision, void* eps, int* best, float* times, void* stream);
+/* Sets (nw = 1, 2, 4) or clears (nw = 0) the cached schedule of one shape. */
+int pygw_chi0_set_schedule(int precision, int pb, int n_pairs, int n_om, int nw);
+
 /* ------------------------------------------------------------------------
  * Dielectric inversion (calc_eps, pygw.py:3229-3271, q != 0):
  *   eps[:,:,w] <- ((eps + eps^H)/2 + 1)^{-1} - 1   for every w, in place.

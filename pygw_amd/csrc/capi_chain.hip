@@ -1,10 +1,30 @@
 // C-ABI for the contractions and the dielectric inversion (include/pygw_hip.h).
 #include "common.h"
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 int chi0_f64_launch(const void* M, long strideK, int np, int pb, const double* kcw, int nom, int k0, int nk,
-                    double coef, void* eps, int beta, hipStream_t stream);
+                    double coef, void* eps, int beta, int nw, hipStream_t stream);
 int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double* kcw, int nom, int k0, int nkc,
-                      double coef, int kflush, int beta, int have_scales, void* ws, void* eps, hipStream_t stream);
+                      double coef, int kflush, int beta, int have_scales, int nw, void* ws, void* eps,
+                      hipStream_t stream);
+
+// Autotune cache: (device, precision, pb, n_pairs, n_om) -> frequencies per
+// block.  The only global state of the library; guarded by a mutex.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int>, int> g_tune;
+
+int tuned_nw(int precision, int pb, int np, int nom) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(std::make_tuple(dev, precision, pb, np, nom));
+  return it == g_tune.end() ? 2 : it->second;
+}
+}  // namespace
 size_t chi0_mixed_ws_bytes(int pb, int nom);
 int eps_inverse_launch(void* eps, int n, int nom, void* ws, int* dinfo, hipStream_t stream);
 size_t eps_inverse_ws_bytes(int n, int nom);
@@ -21,7 +41,9 @@ extern "C" int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, i
   hipStream_t stream = (hipStream_t)stream_;
   if (!M || !kcw || !eps || n_pairs <= 0 || pb <= 0 || n_om <= 0 || n_k <= 0 || k0 < 0) return PYGW_E_ARG;
   if (stride_k < (long)n_pairs * pb || (beta != 0 && beta != 1)) return PYGW_E_ARG;
-  if (precision == 0) return chi0_f64_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, eps, beta, stream);
+  const int nw = tuned_nw(precision, pb, n_pairs, n_om);
+  if (precision == 0)
+    return chi0_f64_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, eps, beta, nw, stream);
   if (precision != 1) return PYGW_E_ARG;
   const size_t need = chi0_mixed_ws_bytes(pb, n_om);
   void* ws = workspace;
@@ -35,7 +57,7 @@ extern "C" int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, i
   const int steps_per_k = (n_pairs + 31) / 32;
   int kflush = 256 / steps_per_k;
   if (kflush < 1) kflush = 1;
-  int rc = chi0_mixed_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, kflush, beta, 0, ws, eps, stream);
+  int rc = chi0_mixed_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, kflush, beta, 0, nw, ws, eps, stream);
   if (own) {
     hipError_t e = hipFreeAsync(ws, stream);
     if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
@@ -84,4 +106,63 @@ extern "C" int pygw_eps_inverse(void* eps, int pb, int n_om, void* workspace, si
     if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
   }
   return rc;
+}
+
+// Times the chi0 tile schedules (1, 2, 4 frequencies per block) on the given
+// inputs, records the fastest for (device, precision, pb, n_pairs, n_om) and
+// returns it in *best (times in ms in times[3] if given).  eps is scratch.
+extern "C" int pygw_chi0_autotune(const void* M, long stride_k, int n_pairs, int pb, const double* kcw, int n_om,
+                                  int k0, int n_k, double coef, int precision, void* eps, int* best, float* times,
+                                  void* stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  if (!M || !kcw || !eps || !best || n_pairs <= 0 || pb <= 0 || n_om <= 0 || n_k <= 0 || k0 < 0) return PYGW_E_ARG;
+  if (precision != 0 && precision != 1) return PYGW_E_ARG;
+  void* ws = nullptr;
+  if (precision == 1) HIP_TRY(hipMallocAsync(&ws, chi0_mixed_ws_bytes(pb, n_om), stream));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const int cand[3] = {1, 2, 4};
+  float tbest = 1e30f;
+  int nbest = 2, rc = PYGW_OK;
+  const int steps_per_k = (n_pairs + 31) / 32;
+  int kflush = 256 / steps_per_k;
+  if (kflush < 1) kflush = 1;
+  for (int i = 0; i < 3 && rc == PYGW_OK; i++) {
+    const int nw = cand[i];
+    for (int rep = 0; rep < 2 && rc == PYGW_OK; rep++) {  // first run warms up
+      if (hipEventRecord(e0, stream) != hipSuccess) { rc = PYGW_E_HIP; break; }
+      if (precision == 0) rc = chi0_f64_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, eps, 0, nw, stream);
+      else rc = chi0_mixed_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, kflush, 0, 0, nw, ws, eps, stream);
+      if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = PYGW_E_HIP; break; }
+    }
+    float ms = 0.f;
+    if (rc == PYGW_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PYGW_E_HIP;
+    if (times) times[i] = ms;
+    if (rc == PYGW_OK && ms < tbest) { tbest = ms; nbest = nw; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ws) (void)hipFreeAsync(ws, stream);
+  if (rc != PYGW_OK) return rc;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[std::make_tuple(dev, precision, pb, n_pairs, n_om)] = nbest;
+  }
+  *best = nbest;
+  return PYGW_OK;
+}
+
+// Sets the cached schedule for one shape (nw = 1, 2 or 4; 0 removes it).
+extern "C" int pygw_chi0_set_schedule(int precision, int pb, int n_pairs, int n_om, int nw) {
+  if ((precision != 0 && precision != 1) || (nw != 0 && nw != 1 && nw != 2 && nw != 4)) return PYGW_E_ARG;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto key = std::make_tuple(dev, precision, pb, n_pairs, n_om);
+  if (nw == 0) g_tune.erase(key);
+  else g_tune[key] = nw;
+  return PYGW_OK;
 }

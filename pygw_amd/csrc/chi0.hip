@@ -153,13 +153,21 @@ __global__ void __launch_bounds__(256) pygw_chi0_f64(const double2* __restrict__
   }
 }
 
-int chi0_f64_launch(const void* M, long strideK, int np, int pb, const double* kcw, int nom, int k0, int nk,
-                    double coef, void* eps, int beta, hipStream_t stream) {
-  kcw += (size_t)k0 * nom * np;
-  const int NW = 2;
+template <int NW>
+static void chi0_f64_go(const void* M, long strideK, int np, int pb, const double* kcw, int nom, int nk, double coef,
+                        void* eps, int beta, hipStream_t stream) {
   const int T = ceil_div(pb, c0::TB);
   dim3 grid(T * (T + 1) / 2, ceil_div(nom, NW));
   hipLaunchKernelGGL(pygw_chi0_f64<NW>, grid, dim3(256), 0, stream, (const double2*)M, strideK, np, pb, kcw, nom,
                      nk, coef, T, (double2*)eps, beta);
+}
+
+// nw: frequencies per block (1, 2 or 4) -- the tile schedule the autotuner picks
+int chi0_f64_launch(const void* M, long strideK, int np, int pb, const double* kcw, int nom, int k0, int nk,
+                    double coef, void* eps, int beta, int nw, hipStream_t stream) {
+  kcw += (size_t)k0 * nom * np;
+  if (nw == 1) chi0_f64_go<1>(M, strideK, np, pb, kcw, nom, nk, coef, eps, beta, stream);
+  else if (nw == 4) chi0_f64_go<4>(M, strideK, np, pb, kcw, nom, nk, coef, eps, beta, stream);
+  else chi0_f64_go<2>(M, strideK, np, pb, kcw, nom, nk, coef, eps, beta, stream);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }

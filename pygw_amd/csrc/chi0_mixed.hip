@@ -272,7 +272,8 @@ size_t chi0_mixed_ws_bytes(int pb, int nom) {
 // M: complex64 [n_kc][np][pb] holding k = k0..k0+n_kc-1; kcw covers all k.
 // scales: computed from this M chunk unless `have_scales` (then taken from ws).
 int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double* kcw, int nom, int k0, int nkc,
-                      double coef, int kflush, int beta, int have_scales, void* ws, void* eps, hipStream_t stream) {
+                      double coef, int kflush, int beta, int have_scales, int nw, void* ws, void* eps,
+                      hipStream_t stream) {
   float* cs = (float*)ws;
   unsigned int* cmax = (unsigned int*)(cs + pb);
   double* wsc = (double*)(((uintptr_t)(cmax + pb) + 15) & ~(uintptr_t)15);
@@ -289,11 +290,20 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
   }
   const long ntot = (long)pb * pb * nom;
   if (!beta) hipLaunchKernelGGL(pygw_zero_c128, dim3(2048), dim3(256), 0, stream, (double2*)eps, ntot);
-  const int NW = 2;
   const int T = ceil_div(pb, cm::TB);
-  dim3 grid(T * (T + 1) / 2, ceil_div(nom, NW));
-  hipLaunchKernelGGL(pygw_chi0_mixed<NW>, grid, dim3(256), 0, stream, (const float2*)M, strideK, np, pb, kcw, nom, k0,
-                     nkc, coef, cs, wsc, kflush, T, (double2*)eps);
+  if (nw == 1) {
+    dim3 grid(T * (T + 1) / 2, nom);
+    hipLaunchKernelGGL(pygw_chi0_mixed<1>, grid, dim3(256), 0, stream, (const float2*)M, strideK, np, pb, kcw, nom,
+                       k0, nkc, coef, cs, wsc, kflush, T, (double2*)eps);
+  } else if (nw == 4) {
+    dim3 grid(T * (T + 1) / 2, ceil_div(nom, 4));
+    hipLaunchKernelGGL(pygw_chi0_mixed<4>, grid, dim3(256), 0, stream, (const float2*)M, strideK, np, pb, kcw, nom,
+                       k0, nkc, coef, cs, wsc, kflush, T, (double2*)eps);
+  } else {
+    dim3 grid(T * (T + 1) / 2, ceil_div(nom, 2));
+    hipLaunchKernelGGL(pygw_chi0_mixed<2>, grid, dim3(256), 0, stream, (const float2*)M, strideK, np, pb, kcw, nom,
+                       k0, nkc, coef, cs, wsc, kflush, T, (double2*)eps);
+  }
   hipLaunchKernelGGL(pygw_chi0_mirror, dim3(ceil_div((long)pb * pb, 256), nom), dim3(256), 0, stream,
                      (double2*)eps, pb, nom);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;

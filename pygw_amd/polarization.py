@@ -218,3 +218,29 @@ def bz_calcqdepw_par3(enk, ef, kqid, ncg, nomx, numin, dUl, omega, tetc, *, layo
     if lay == 0:
         return out.as_strided((no, ne, nkp, nl), (1, no, no * ne, no * ne * nkp))
     return out.view(nkp, nl, no * ne)
+
+
+def autotune_chi0(M, kcw_native, coef=-2.0, *, precision="fp64", k0=0, device="cuda"):
+    """Pick the chi0 tile schedule (Matsubara frequencies per tile block) for
+    this (precision, PB, n_pairs, n_om) by timing the candidates on the given
+    inputs; the choice is cached in the library for later calls.
+    Returns (best, {candidate: ms})."""
+    device = _require_device(device)
+    dt = {"fp64": torch.complex128, "mixed": torch.complex64}[precision]
+    M = M.to(device=device, dtype=dt).contiguous() if isinstance(M, torch.Tensor) else \
+        torch.as_tensor(np.ascontiguousarray(M), dtype=dt, device=device)
+    nkc, npair, pb = M.shape
+    kc = _dev(kcw_native, torch.float64, device).contiguous()
+    nom = kc.shape[1]
+    scratch = torch.empty(pb * pb * nom, dtype=torch.complex128, device=device)
+    best = C.c_int(0)
+    times = (C.c_float * 3)()
+    _lib.call("pygw_chi0_autotune", C.c_void_p(M.data_ptr()), npair * pb, npair, pb, C.c_void_p(kc.data_ptr()),
+              nom, int(k0), nkc, float(coef), {"fp64": 0, "mixed": 1}[precision], C.c_void_p(scratch.data_ptr()),
+              C.byref(best), C.cast(times, C.c_void_p), _stream_ptr(device))
+    return best.value, dict(zip((1, 2, 4), list(times)))
+
+
+def set_chi0_schedule(precision, pb, n_pairs, n_om, nw):
+    """Force (nw = 1, 2, 4) or clear (0) the chi0 tile schedule of a shape."""
+    _lib.call("pygw_chi0_set_schedule", {"fp64": 0, "mixed": 1}[precision], pb, n_pairs, n_om, nw)

@@ -111,3 +111,31 @@ def test_chi0_mixed_vs_fp64(shape):
     P.chi0_accumulate(m64[h:], kcw, precision="mixed", eps=e2, k0=h)
     assert _rel(e2.cpu().numpy(), want) < 1e-6
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_chi0_schedules_and_autotune(precision):
+    """Every tile schedule (1, 2, 4 frequencies per block) gives the same
+    contraction; the autotuner picks one of them and caches it."""
+    import torch
+    from pygw_amd import polarization as P
+    cfg = synth.small_config(div=(2, 3, 2), n_occ=3, n_emp=5, n_om=7, pb=80, seed=5)
+    b, om, tetc, kq, M = _inputs(cfg)
+    print(f"case schedules {precision}", flush=True)
+    kcw = P.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc, layout="native")
+    kh = kcw.cpu().numpy()
+    Mp = M.astype(np.complex64) if precision == "mixed" else M
+    want = chain.calc_eps_accumulate(((ik, Mp[ik].astype(complex)) for ik in range(cfg.nk)), lambda ik: kh[ik].T)
+    tol = 1e-13 if precision == "fp64" else 1e-6
+    for nw in (1, 2, 4):
+        P.set_chi0_schedule(precision, cfg.pb, b.n_o * b.n_e, cfg.n_om, nw)
+        got = P.chi0_accumulate(Mp, kcw, precision=precision).cpu().numpy()
+        print(f"  nw={nw} rel={_rel(got, want):.2e}", flush=True)
+        assert _rel(got, want) < tol
+    best, times = P.autotune_chi0(Mp, kcw, precision=precision)
+    print(f"  autotune best={best} times={times}", flush=True)
+    assert best in (1, 2, 4)
+    got = P.chi0_accumulate(Mp, kcw, precision=precision).cpu().numpy()
+    assert _rel(got, want) < tol
+    P.set_chi0_schedule(precision, cfg.pb, b.n_o * b.n_e, cfg.n_om, 0)
+    torch.cuda.synchronize()
