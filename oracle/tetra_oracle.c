@@ -131,8 +131,8 @@ void orc_stweight_itaylor(const double d[4], double omeg, double w[4]) {
 /* 1: reproduce the reference's case-4 vertex rotation (for_tetrahedra3.f90:482-486) */
 int orc_compat_f7b = 1;
 /* transcendental perturbation for sensitivity estimates (0 = off): atan and
- * log results are scaled by (1 +/- k*2^-52), the sign and k in {0,1} taken
- * from a hash of the argument bits and this seed. */
+ * log results are scaled by (1 +/- 2^-52), the sign taken from a hash of the
+ * argument bits and this seed. */
 unsigned long long orc_perturb_seed = 0;
 
 static double perturb(double x, double arg) {
@@ -142,8 +142,7 @@ static double perturb(double x, double arg) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   z ^= z >> 31;
-  int k = (int)(z & 3u);              /* 0,1,2,3 -> -1,0,0,+1 ulp-ish */
-  double s = (k == 0) ? -1.0 : (k == 3 ? 1.0 : 0.0);
+  double s = (z & 1u) ? 1.0 : -1.0;  /* every value moves by one unit of 2^-52 */
   return x * (1.0 + s * 2.220446049250313e-16);
 }
 

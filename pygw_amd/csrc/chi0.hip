@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(256) pygw_chi0_f64(const double2* __restrict__
       for (int b = 0; b < 2; b++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const int I = I0 + 32 * wr + 16 * a + 4 * lk + v;
+          const int I = I0 + 32 * wr + 16 * a + 4 * v + lk;
           const int J = J0 + 32 * wc + 16 * b + li;
           if (I < pb && J < pb) {
             const double re = accR[w][a][b][v], im = accI[w][a][b][v];

@@ -3,7 +3,9 @@
 //   eps[I,J,w] += sum_k sum_p coef*kcw[k,w,p] * M[k,p,I] * conj(M[k,p,J])
 //
 // M (complex64) is scaled per column by an exact power of two s_I (so every
-// |Re|,|Im| <= 1) and the weights per frequency by a power of two t_w; each
+// |Re|,|Im| < 2^14, high in FP16's normal range so that the low halves of
+// small entries stay normal too) and the weights per frequency by a power of
+// two t_w (|w t_w| < 1); each
 // real operand x is split into FP16 x_hi = fp16(x), x_lo = fp16(x - x_hi)
 // (22 significant bits), and each real product uses three MFMA terms
 //   a_hi b_hi + a_hi b_lo + a_lo b_hi        (FP32 accumulation).
@@ -212,7 +214,7 @@ __global__ void pygw_chi0_mirror(double2* __restrict__ eps, int pb, int nom) {
   E[(size_t)I + (size_t)pb * J] = double2{v.x, -v.y};
 }
 
-// per-column scale: cs[I] = 2^-ceil(log2(max_{k,p} max(|Re|,|Im|)))
+// per-column scale: cs[I] = 2^(14 - ceil(log2(max_{k,p} max(|Re|,|Im|))))
 __global__ void pygw_colmax(const float2* __restrict__ M, long strideK, int np, int pb, int nk,
                             unsigned int* __restrict__ cmax) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
@@ -232,7 +234,7 @@ __global__ void pygw_colscale(const unsigned int* __restrict__ cmax, int pb, flo
   const float m = __uint_as_float(cmax[I]);
   int e = 0;
   if (m > 0.f) { frexpf(m, &e); }  // m = f * 2^e, f in [0.5, 1)
-  cs[I] = ldexpf(1.0f, -e);         // |x * cs| < 1
+  cs[I] = ldexpf(1.0f, 14 - e);     // |x * cs| < 2^14
 }
 
 // per-frequency weight scale: ws[w] = 2^-e, e = exponent of max_{k,p} |coef*kcw|

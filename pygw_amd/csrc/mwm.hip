@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(256) pygw_mwm_f64(const double2* __restrict__ 
   const int wr = wave >> 1, wc = wave & 1;
   const int li = lane & 15, lk = lane >> 4;
   for (int e = tid; e < 2 * TR; e += 256) racc[e / TR][e % TR] = double2{0.0, 0.0};
-  // per-lane partial row sums: rows 32*wr + 16*a + 4*lk + v
+  // per-lane partial row sums: rows 32*wr + 16*a + 4*v + lk
   double2 rs[2][4];
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(256) pygw_mwm_f64(const double2* __restrict__ 
     for (int a = 0; a < 2; a++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int row = r0 + 32 * wr + 16 * a + 4 * lk + v;
+        const int row = r0 + 32 * wr + 16 * a + 4 * v + lk;
         double2 s = rs[a][v];
 #pragma unroll
         for (int b = 0; b < 2; b++) {
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(256) pygw_mwm_f64(const double2* __restrict__ 
         x += __shfl_xor(x, off, 64);
         y += __shfl_xor(y, off, 64);
       }
-      if (li == 0) racc[wc][32 * wr + 16 * a + 4 * lk + v] = double2{x, y};
+      if (li == 0) racc[wc][32 * wr + 16 * a + 4 * v + lk] = double2{x, y};
     }
   __syncthreads();
   for (int r = tid; r < TR; r += 256) {
@@ -135,8 +135,9 @@ int mwm_f64_launch(const void* M, int nrows, int pb, const void* W, int nom, dou
 
 // ---------------------------------------------------------------------------
 // Split-precision MWM (v_mfma_f32_16x16x32_f16): conj(M) W with M rows
-// scaled by alpha_r = 2^-e_r and W columns by beta_J = 2^-e_J (exact powers
-// of two, so every operand is <= 1), each real operand split into FP16
+// scaled by alpha_r = 2^(14-e_r) and W columns by beta_J = 2^(14-e_J) (exact
+// powers of two, so every operand is < 2^14, high in FP16's normal range and
+// the low halves of small entries stay normal), each real operand split into FP16
 // hi + lo, three products per real product, FP32 accumulation over I
 // (PB/32 MFMA steps), then unscaled, multiplied by M[r,J] and summed over J
 // in FP64.
@@ -262,7 +263,7 @@ __global__ void __launch_bounds__(256) pygw_mwm_mixed(const float2* __restrict__
   }
 }
 
-// alpha_r = 2^-e, e = exponent of max_I max(|Re|,|Im|) of row r (complex64 M)
+// alpha_r = 2^(14-e), e = exponent of max_I max(|Re|,|Im|) of row r (complex64 M)
 __global__ void pygw_mwm_rowscale(const float2* __restrict__ M, int nrows, int pb, float* __restrict__ rs) {
   const int row = blockIdx.x;
   float m = 0.f;
@@ -279,11 +280,11 @@ __global__ void pygw_mwm_rowscale(const float2* __restrict__ M, int nrows, int p
     m = fmaxf(m, red[0]);
     int e = 0;
     if (m > 0.f) frexpf(m, &e);
-    rs[row] = ldexpf(1.0f, -e);
+    rs[row] = ldexpf(1.0f, 14 - e);
   }
 }
 
-// beta_{J,w} = 2^-e, e = exponent of max_I max(|Re|,|Im|) of column J of W_w
+// beta_{J,w} = 2^(14-e), e = exponent of max_I max(|Re|,|Im|) of column J of W_w
 __global__ void pygw_mwm_colscale(const double2* __restrict__ W, int pb, float* __restrict__ cs) {
   const int J = blockIdx.x, w = blockIdx.y;
   const double2* col = W + (size_t)w * pb * pb + (size_t)pb * J;
@@ -293,7 +294,7 @@ __global__ void pygw_mwm_colscale(const double2* __restrict__ W, int pb, float* 
   if (threadIdx.x == 0) {
     int e = 0;
     if (m > 0.0) frexp(m, &e);
-    cs[(size_t)w * pb + J] = (float)ldexp(1.0, -e);
+    cs[(size_t)w * pb + J] = (float)ldexp(1.0, 14 - e);
   }
 }
 

@@ -244,3 +244,66 @@ def autotune_chi0(M, kcw_native, coef=-2.0, *, precision="fp64", k0=0, device="c
 def set_chi0_schedule(precision, pb, n_pairs, n_om, nw):
     """Force (nw = 1, 2, 4) or clear (0) the chi0 tile schedule of a shape."""
     _lib.call("pygw_chi0_set_schedule", {"fp64": 0, "mixed": 1}[precision], pb, n_pairs, n_om, nw)
+
+
+class QPlan:
+    """Device buffers for the per-q steps of one workload (weights -> chi0 ->
+    eps^-1), allocated once.  run() launches with explicit workspaces and
+    no allocation, host synchronisation or memset, so it can be captured in
+    a HIP graph (torch.cuda.CUDAGraph) and replayed for every q.
+    Per-q inputs: set_q(kqid column) copies into a fixed buffer; M is taken
+    by pointer (one graph per M buffer)."""
+
+    def __init__(self, enk, ef, ncg, nomx, numin, omega, tetc, pb, precision="fp64", compat=COMPAT_REFERENCE,
+                 device="cuda"):
+        self.device = d = _require_device(device)
+        enk = np.asarray(enk)
+        self.nbnd, self.nk = enk.shape
+        self.ef, self.ncg, self.nomx, self.numin = float(ef), int(ncg), int(nomx), int(numin)
+        self.no, self.ne = ncg + nomx + 1, self.nbnd - ncg - numin
+        self.np = self.no * self.ne
+        self.pb, self.precision, self.compat = int(pb), precision, int(compat)
+        self.enk = torch.as_tensor(np.ascontiguousarray(enk.T), dtype=torch.float64, device=d)
+        self.tetc = torch.as_tensor(np.ascontiguousarray(np.asarray(tetc).T), dtype=torch.int32, device=d)
+        self.ntet = self.tetc.shape[0]
+        self.om = torch.as_tensor(np.asarray(omega, dtype=np.float64), device=d)
+        self.nom = self.om.numel()
+        self.kq = torch.zeros(self.nk, dtype=torch.int32, device=d)
+        self.kcw = torch.empty((self.nk, self.nom, self.np), dtype=torch.float64, device=d)
+        self.eps_buf = torch.empty(self.pb * self.pb * self.nom, dtype=torch.complex128, device=d)
+        self.eps = _fortran_view(self.eps_buf, self.pb, self.nom)
+        sz = C.c_size_t(0)
+        _lib.call("pygw_tetra_weights_workspace", self.nk, C.byref(sz))
+        self.ws_tw = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
+        _lib.call("pygw_chi0_workspace", self.pb, self.nom, {"fp64": 0, "mixed": 1}[precision], C.byref(sz))
+        self.ws_c0 = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
+        _lib.call("pygw_eps_inverse_workspace", self.pb, self.nom, C.byref(sz))
+        self.ws_ei = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
+
+    def set_q(self, kqid_col):
+        self.kq.copy_(torch.as_tensor(np.asarray(kqid_col, dtype=np.int32)))
+
+    def run(self, M):
+        s = _stream_ptr(self.device)
+        if M.shape != (self.nk, self.np, self.pb) or not M.is_contiguous():
+            raise ValueError("M must be a contiguous [N_k, n_pairs, PB] device tensor")
+        _lib.call("pygw_tetra_weights", C.c_void_p(self.enk.data_ptr()), self.ef, C.c_void_p(self.kq.data_ptr()),
+                  C.c_void_p(self.tetc.data_ptr()), self.ncg, self.nomx, self.numin, C.c_void_p(self.om.data_ptr()),
+                  self.nom, self.nk, self.nbnd, self.ntet, 1, self.compat, C.c_void_p(self.kcw.data_ptr()),
+                  C.c_void_p(self.ws_tw.data_ptr()), self.ws_tw.numel(), None, s)
+        prec = {"fp64": 0, "mixed": 1}[self.precision]
+        _lib.call("pygw_chi0_accumulate", C.c_void_p(M.data_ptr()), self.np * self.pb, self.np, self.pb,
+                  C.c_void_p(self.kcw.data_ptr()), self.nom, 0, self.nk, -2.0, prec, 0,
+                  C.c_void_p(self.eps_buf.data_ptr()), C.c_void_p(self.ws_c0.data_ptr()), self.ws_c0.numel(), s)
+        _lib.call("pygw_eps_inverse", C.c_void_p(self.eps_buf.data_ptr()), self.pb, self.nom,
+                  C.c_void_p(self.ws_ei.data_ptr()), self.ws_ei.numel(), None, s)
+        return self.eps
+
+    def capture(self, M):
+        """Capture run(M) in a HIP graph; returns the torch.cuda.CUDAGraph."""
+        g = torch.cuda.CUDAGraph()
+        self.run(M)                     # warm-up outside the graph (module load)
+        torch.cuda.synchronize(self.device)
+        with torch.cuda.graph(g):
+            self.run(M)
+        return g

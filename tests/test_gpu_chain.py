@@ -139,3 +139,28 @@ def test_chi0_schedules_and_autotune(precision):
     assert _rel(got, want) < tol
     P.set_chi0_schedule(precision, cfg.pb, b.n_o * b.n_e, cfg.n_om, 0)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_qplan_graph_replays(precision):
+    """The per-q steps captured in a HIP graph: every replay (with eager work
+    and synchronisation between replays) equals the eager result."""
+    import torch
+    from pygw_amd import polarization as P
+    cfg = synth.small_config(div=(3, 2, 2), n_occ=3, n_emp=4, n_om=6, pb=72, seed=13, vhalf="coulomb")
+    b, om, tetc, kq, M = _inputs(cfg)
+    print(f"case graph {precision}", flush=True)
+    plan = P.QPlan(b.enk, 0.0, b.ncg, b.nomx, b.numin, om, tetc, cfg.pb, precision=precision)
+    dt = torch.complex128 if precision == "fp64" else torch.complex64
+    Md = torch.as_tensor(M, device="cuda").to(dt).contiguous()
+    plan.set_q(kq)
+    eager = plan.run(Md).clone()
+    torch.cuda.synchronize()
+    g = plan.capture(Md)
+    for rep in range(3):
+        plan.eps.zero_()
+        _ = torch.randn(1000, device="cuda").sum()   # unrelated eager work
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(plan.eps, eager), f"replay {rep} differs"
