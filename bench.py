@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-kslab", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one plain 1 GiB device copy (scale for HBM byte counters)")
     ap.add_argument("--fp64-kfrac", type=float, default=None,
                     help="time the FP64 contraction on this fraction of k and extrapolate (large configs)")
     return ap.parse_args()
@@ -273,6 +275,15 @@ def main():
         run_chain(args)
         return
     mixed = not args.no_mixed
+    if args.copy_calibration:
+        # one copy kernel that reads and writes exactly 1 GiB: the ratio of its
+        # FETCH_SIZE / WRITE_SIZE counters to 1 GiB scales the kernels' counters
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     wl = Workload(args.config, args.nq, mixed, args.fp64_kfrac)
     cfg, b = wl.base, wl.bands
     no, ne, pb, nom, nk = b.n_o, b.n_e, cfg.pb, cfg.n_om, cfg.nk

@@ -98,3 +98,53 @@ def bz_calcqdepw(enk, ef, kqid_col, ncg, nomx, numin, omega, tetc, dUl=None, nth
                                 om.ctypes.data, tetc.ctypes.data, nkp, om.size, nbnd, ntet, dptr, nl,
                                 int(nthreads), C.byref(fl))
     return kcw, st, fl.value
+
+
+PERTURB_SEEDS = tuple(range(1, 9))
+
+
+def with_spread(enk, ef, kqid_col, ncg, nomx, numin, omega, tetc, compat=(1, 1), seeds=PERTURB_SEEDS,
+                nthreads=8):
+    """Oracle weights plus their sensitivity to one-ulp changes of atan/log.
+
+    The closed forms of the Matsubara weights (for_tetrahedra3.f90:1-137) are
+    differences of atan/log values divided by high powers of corner-Delta
+    differences; near degenerate classes a one-ulp change of one atan moves
+    the weight by many orders of magnitude more than one ulp of the weight.
+    Each seed re-runs the oracle with every atan/log result scaled by
+    (1 +- 2^-52), the sign a hash of (argument, seed); the spread is the
+    largest deviation over the seeds.  Several seeds are needed: one seed can
+    move two nearly equal atan values the same way and leave their
+    difference unchanged.  `compat` = (orc_compat_f7b, orc_compat_nnod78).
+    Returns (kcw, spread, flags)."""
+    L = lib()
+    a = C.c_int.in_dll(L, "orc_compat_f7b")
+    b = C.c_int.in_dll(L, "orc_compat_nnod78")
+    p = C.c_ulonglong.in_dll(L, "orc_perturb_seed")
+    old = (a.value, b.value, p.value)
+    a.value, b.value = compat
+    try:
+        p.value = 0
+        k0, st, fl = bz_calcqdepw(enk, ef, kqid_col, ncg, nomx, numin, omega, tetc, nthreads=nthreads)
+        if st < 0:
+            raise RuntimeError(f"oracle bz_calcqdepw failed: {st}")
+        spread = np.zeros_like(k0)
+        for s in seeds:
+            p.value = int(s)
+            k1, _, _ = bz_calcqdepw(enk, ef, kqid_col, ncg, nomx, numin, omega, tetc, nthreads=nthreads)
+            np.maximum(spread, np.abs(k1 - k0), out=spread)
+    finally:
+        a.value, b.value, p.value = old
+    return k0, spread, fl
+
+
+def within_spread(got, kcw, spread, factor=8.0, floor=1e-13):
+    """max over elements of |got - kcw| / (factor*spread + floor*max_w|kcw|);
+    <= 1 means the device weights agree with the oracle to within what one-ulp
+    differences of atan/log explain."""
+    scale = np.abs(kcw).max(axis=tuple(range(kcw.ndim - 1)), keepdims=True)
+    tol = factor * spread + floor * scale
+    dev = np.abs(got - kcw)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.where(tol > 0, dev / np.where(tol > 0, tol, 1.0), np.where(dev == 0, 0.0, np.inf))
+    return float(r.max()) if r.size else 0.0

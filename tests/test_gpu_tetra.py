@@ -5,8 +5,9 @@ corner Delta's and for w close to the Taylor switch (10 max|Delta|): there
 the reference itself loses many digits, and a device atan/log differing by
 one ulp from the host libm moves the result by the same amount.  The
 tolerance is therefore measured, not guessed: the oracle is re-run with
-atan/log perturbed by +-1 ulp (two seeds), and the device result must stay
-within 16x that spread plus 1e-13 of the largest |kcw| of its frequency.
+atan/log perturbed by +-1 ulp (eight seeds, oracle.tetra.with_spread), and
+the device result must stay within 8x that spread plus 1e-13 of the largest
+|kcw| of its frequency.
 """
 import ctypes as C
 import os
@@ -21,31 +22,10 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 pytestmark = pytest.mark.gpu
 
 
-def _oracle(enk, kq, win, om, tetc, compat=(1, 1), perturb=0):
-    L = tetra.lib()
-    a = C.c_int.in_dll(L, "orc_compat_f7b")
-    b = C.c_int.in_dll(L, "orc_compat_nnod78")
-    p = C.c_ulonglong.in_dll(L, "orc_perturb_seed")
-    old = (a.value, b.value, p.value)
-    a.value, b.value, p.value = compat[0], compat[1], perturb
-    try:
-        kcw, st, fl = tetra.bz_calcqdepw(enk, 0.0, kq, *win, om, tetc, nthreads=8)
-    finally:
-        a.value, b.value, p.value = old
-    assert st >= 0
-    return kcw, fl
-
-
 def _check(gpu, enk, kq, win, om, tetc, compat=(1, 1), name=""):
-    o0, fl = _oracle(enk, kq, win, om, tetc, compat)
-    spread = np.zeros_like(o0)
-    for s in (11, 23):
-        o1, _ = _oracle(enk, kq, win, om, tetc, compat, perturb=s)
-        spread = np.maximum(spread, np.abs(o1 - o0))
-    scale = np.abs(o0).max(axis=(0, 1, 2), keepdims=True)
-    tol = 16 * spread + 1e-13 * scale
+    o0, spread, _ = tetra.with_spread(enk, 0.0, kq, *win, om, tetc, compat=compat)
+    ratio = tetra.within_spread(gpu, o0, spread)
     dev = np.abs(gpu - o0)
-    ratio = (dev / tol).max()
     print(f"[{name}] max|gpu-oracle|={dev.max():.3e} max|oracle|={np.abs(o0).max():.3e} "
           f"max dev/tol={ratio:.3f} exact-equal={np.mean(gpu == o0):.3f}", flush=True)
     assert ratio <= 1.0
@@ -132,13 +112,9 @@ def test_gpu_weights_svd_basis():
     from pygw_amd import polarization as P
     print(f"case svd n_l={dUl.shape[1]}", flush=True)
     got = P.bz_calcqdepw_par3(b.enk, 0.0, kq, *win, dUl, om, tetc).cpu().numpy()
-    o0, _ = _oracle(b.enk, kq, win, om, tetc)
-    spread = np.zeros_like(o0)
-    for s in (11, 23):
-        o1, _ = _oracle(b.enk, kq, win, om, tetc, perturb=s)
-        spread = np.maximum(spread, np.abs(o1 - o0))
+    o0, spread, _ = tetra.with_spread(b.enk, 0.0, kq, *win, om, tetc)
     want = o0 @ dUl
-    tol = 16 * (spread @ np.abs(dUl)) + 1e-12 * np.abs(want).max()
+    tol = 8 * (spread @ np.abs(dUl)) + 1e-12 * np.abs(want).max()
     assert (np.abs(got - want) <= tol).all()
     L = tetra.lib()
     kl, st, _ = tetra.bz_calcqdepw(b.enk, 0.0, kq, *win, om, tetc, dUl=dUl)

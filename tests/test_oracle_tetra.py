@@ -200,3 +200,39 @@ def test_oracle_par3_equals_reference():
     kl, st, _ = tetra.bz_calcqdepw(b.enk, 0.0, kq, *win, fm.omega, tetc, dUl=dUl)
     assert st >= 0
     np.testing.assert_array_equal(kl, ref.bz_calcqdepw_par3(b.enk, 0.0, kq, *win, dUl, fm.omega, tetc))
+
+
+def test_ulp_spread_tolerance_discriminates():
+    """The device-parity bar (8x the one-ulp atan/log spread) must accept a
+    perturbed oracle run and reject plausible mistakes: the class-4 corner
+    exchange toggled, or one tetrahedron's contribution dropped."""
+    cfg = synth.small_config(div=(3, 3, 3), n_occ=3, n_emp=5, n_om=8, seed=1)
+    b = synth.band_energies(cfg)
+    om, _ = synth.omega_mesh(cfg)
+    tetc = mesh.tetra_corners(cfg.div)
+    kq = mesh.k_minus_q(cfg.div)[:, 1]
+    win = (b.ncg, b.nomx, b.numin)
+    k0, spread, _ = tetra.with_spread(b.enk, 0.0, kq, *win, om, tetc)
+    # a run with other perturbation seeds stays inside
+    k1, _, _ = tetra.with_spread(b.enk, 0.0, kq, *win, om, tetc, seeds=())
+    assert tetra.within_spread(k1, k0, spread) == 0.0
+    L = tetra.lib()
+    p = C.c_ulonglong.in_dll(L, "orc_perturb_seed")
+    p.value = 99
+    try:
+        k2, _, _ = tetra.bz_calcqdepw(b.enk, 0.0, kq, *win, om, tetc, nthreads=8)
+    finally:
+        p.value = 0
+    assert tetra.within_spread(k2, k0, spread) <= 1.0
+    # the exact class-4 rotation (not PyGW's) is rejected
+    k3, _, _ = tetra.with_spread(b.enk, 0.0, kq, *win, om, tetc, compat=(0, 1), seeds=())
+    assert tetra.within_spread(k3, k0, spread) > 1.0
+    # dropping the first tetrahedron is rejected
+    tet2 = tetc.copy()
+    tet2[:, 0] = tet2[:, 1]
+    k4, _, _ = tetra.bz_calcqdepw(b.enk, 0.0, kq, *win, om, tet2, nthreads=8)
+    assert tetra.within_spread(k4, k0, spread) > 1.0
+    # PyGW's closed forms lose digits (a one-ulp atan moves a typical weight
+    # by ~1e-9 of the largest); the bar is that loss, not a blanket tolerance
+    rel = 8 * spread / np.abs(k0).max(axis=(0, 1, 2), keepdims=True)
+    assert np.median(rel) < 1e-7
