@@ -77,16 +77,21 @@ int pygw_tetra_weights(const double* enk, double ef, const int* kqid, const int*
  *   precision 0: FP64 on v_mfma_f64_16x16x4f64
  *   precision 1: FP16 hi+lo split with per-column and per-frequency
  *                power-of-two scaling, 3 products on
- *                v_mfma_f32_16x16x32_f16, FP32 partial sums folded into
- *                FP64 across k
+ *                v_mfma_f32_16x16x32_f16, one fresh MFMA chain per 32
+ *                pairs, FP32 tile sums folded into the FP64 eps
+ *   workspace: pygw_chi0_workspace bytes for (pb, n_pairs, n_om, n_k) (0 for
+ *         precision 0); NULL lets the call allocate it (stream-ordered).  A
+ *         smaller caller workspace (at least one k-point's fragment planes)
+ *         makes the call walk k in smaller slabs.
  * ------------------------------------------------------------------------ */
-int pygw_chi0_workspace(int pb, int n_om, int precision, size_t* bytes);
+int pygw_chi0_workspace(int pb, int n_pairs, int n_om, int n_k, int precision, size_t* bytes);
 int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, const double* kcw, int n_om, int k0,
                          int n_k, double coef, int precision, int beta, void* eps, void* workspace,
                          size_t ws_bytes, void* stream);
 
-/* Autotuner (no counterpart in PyGW): times the contraction's tile schedules
- * (1, 2 or 4 Matsubara frequencies per 64x64 tile block) on the given
+/* Autotuner (no counterpart in PyGW): times the contraction's schedules
+ * s = 1, 2, 4 (FP64: s Matsubara frequencies per 64x64 tile block; split
+ * precision: FP32 tile sums folded into FP64 every 16*s chunks) on the given
  * inputs and records the fastest for (device, precision, pb, n_pairs, n_om);
  * later pygw_chi0_accumulate calls with that shape use it.  eps is
  * overwritten (scratch).  best: chosen frequencies per block; times[3]

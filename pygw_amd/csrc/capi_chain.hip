@@ -8,11 +8,12 @@
 int chi0_f64_launch(const void* M, long strideK, int np, int pb, const double* kcw, int nom, int k0, int nk,
                     double coef, void* eps, int beta, int nw, hipStream_t stream);
 int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double* kcw, int nom, int k0, int nkc,
-                      double coef, int kflush, int beta, int have_scales, int nw, void* ws, void* eps,
-                      hipStream_t stream);
+                      double coef, int G, int beta, void* ws, size_t ws_bytes, void* eps, hipStream_t stream);
 
-// Autotune cache: (device, precision, pb, n_pairs, n_om) -> frequencies per
-// block.  The only global state of the library; guarded by a mutex.
+// Autotune cache: (device, precision, pb, n_pairs, n_om) -> schedule s in
+// {1, 2, 4}: FP64, s frequencies per 64x64 tile block; split precision, the
+// FP32 tile accumulator folded into FP64 every 16*s chunks of 32 pairs.  The
+// only global state of the library; guarded by a mutex.
 namespace {
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int, int>, int> g_tune;
@@ -25,13 +26,14 @@ int tuned_nw(int precision, int pb, int np, int nom) {
   return it == g_tune.end() ? 2 : it->second;
 }
 }  // namespace
-size_t chi0_mixed_ws_bytes(int pb, int nom);
+size_t chi0_mixed_ws_bytes(int pb, int np, int nom, int nk);
 int eps_inverse_launch(void* eps, int n, int nom, void* ws, int* dinfo, hipStream_t stream);
 size_t eps_inverse_ws_bytes(int n, int nom);
 
-extern "C" int pygw_chi0_workspace(int pb, int n_om, int precision, size_t* bytes) {
-  if (pb <= 0 || n_om <= 0 || !bytes || (precision != 0 && precision != 1)) return PYGW_E_ARG;
-  *bytes = precision == 0 ? 0 : chi0_mixed_ws_bytes(pb, n_om);
+extern "C" int pygw_chi0_workspace(int pb, int n_pairs, int n_om, int n_k, int precision, size_t* bytes) {
+  if (pb <= 0 || n_pairs <= 0 || n_om <= 0 || n_k <= 0 || !bytes || (precision != 0 && precision != 1))
+    return PYGW_E_ARG;
+  *bytes = precision == 0 ? 0 : chi0_mixed_ws_bytes(pb, n_pairs, n_om, n_k);
   return PYGW_OK;
 }
 
@@ -45,19 +47,16 @@ extern "C" int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, i
   if (precision == 0)
     return chi0_f64_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, eps, beta, nw, stream);
   if (precision != 1) return PYGW_E_ARG;
-  const size_t need = chi0_mixed_ws_bytes(pb, n_om);
   void* ws = workspace;
   bool own = false;
   if (!ws) {
-    HIP_TRY(hipMallocAsync(&ws, need, stream));
+    ws_bytes = chi0_mixed_ws_bytes(pb, n_pairs, n_om, n_k);
+    HIP_TRY(hipMallocAsync(&ws, ws_bytes, stream));
     own = true;
-  } else if (ws_bytes < need) {
-    return PYGW_E_ARG;
   }
-  const int steps_per_k = (n_pairs + 31) / 32;
-  int kflush = 256 / steps_per_k;
-  if (kflush < 1) kflush = 1;
-  int rc = chi0_mixed_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, kflush, beta, 0, nw, ws, eps, stream);
+  // a smaller caller workspace walks k in smaller slabs (rejected below one k)
+  int rc = chi0_mixed_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, 16 * nw, beta, ws, ws_bytes, eps,
+                             stream);
   if (own) {
     hipError_t e = hipFreeAsync(ws, stream);
     if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
@@ -118,22 +117,20 @@ extern "C" int pygw_chi0_autotune(const void* M, long stride_k, int n_pairs, int
   if (!M || !kcw || !eps || !best || n_pairs <= 0 || pb <= 0 || n_om <= 0 || n_k <= 0 || k0 < 0) return PYGW_E_ARG;
   if (precision != 0 && precision != 1) return PYGW_E_ARG;
   void* ws = nullptr;
-  if (precision == 1) HIP_TRY(hipMallocAsync(&ws, chi0_mixed_ws_bytes(pb, n_om), stream));
+  const size_t wsb = precision == 1 ? chi0_mixed_ws_bytes(pb, n_pairs, n_om, n_k) : 0;
+  if (precision == 1) HIP_TRY(hipMallocAsync(&ws, wsb, stream));
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   const int cand[3] = {1, 2, 4};
   float tbest = 1e30f;
   int nbest = 2, rc = PYGW_OK;
-  const int steps_per_k = (n_pairs + 31) / 32;
-  int kflush = 256 / steps_per_k;
-  if (kflush < 1) kflush = 1;
   for (int i = 0; i < 3 && rc == PYGW_OK; i++) {
     const int nw = cand[i];
     for (int rep = 0; rep < 2 && rc == PYGW_OK; rep++) {  // first run warms up
       if (hipEventRecord(e0, stream) != hipSuccess) { rc = PYGW_E_HIP; break; }
       if (precision == 0) rc = chi0_f64_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, eps, 0, nw, stream);
-      else rc = chi0_mixed_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, kflush, 0, 0, nw, ws, eps, stream);
+      else rc = chi0_mixed_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, 16 * nw, 0, ws, wsb, eps, stream);
       if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = PYGW_E_HIP; break; }
     }
     float ms = 0.f;

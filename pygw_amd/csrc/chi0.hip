@@ -29,7 +29,7 @@ __device__ __forceinline__ void tile_pair(int t, int T, int& It, int& Jt) {
 
 // MFMA 16x16x4 f64 operand layouts (lane l):
 //   A[i][k]: i = l%16, k = l/16;  B[k][j]: j = l%16, k = l/16;
-//   D[i][j] (4 values v): j = l%16, i = 4*(l/16) + v.
+//   D[i][j] (4 values v): j = l%16, i = 4*v + l/16  (not the FP16 order).
 template <int NW>
 __global__ void __launch_bounds__(256) pygw_chi0_f64(const double2* __restrict__ M, long strideK, int np, int pb,
                                                      const double* __restrict__ kcw, int nom, int nk, double coef,

@@ -2,31 +2,53 @@
 //
 //   eps[I,J,w] += sum_k sum_p coef*kcw[k,w,p] * M[k,p,I] * conj(M[k,p,J])
 //
-// M (complex64) is scaled per column by an exact power of two s_I (so every
-// |Re|,|Im| < 2^14, high in FP16's normal range so that the low halves of
-// small entries stay normal too) and the weights per frequency by a power of
-// two t_w (|w t_w| < 1); each
-// real operand x is split into FP16 x_hi = fp16(x), x_lo = fp16(x - x_hi)
-// (22 significant bits), and each real product uses three MFMA terms
-//   a_hi b_hi + a_hi b_lo + a_lo b_hi        (FP32 accumulation).
-// The FP32 partial sums are folded into the FP64 eps every `kflush` k-points
-// (multiplied back by the exact factor 1/(s_I s_J t_w)), so no FP32
-// accumulator sums more than kflush*N_p/32 MFMA steps: the compensated
-// FP64 sum across k keeps the long K = N_k*N_p reduction in FP64.
+// Operands.  M (complex64) is scaled per column by an exact power of two s_I
+// (|Re|,|Im| < 2^14, high in FP16's normal range so that the low halves of
+// small entries stay normal) and the weights per frequency by a power of two
+// t_w (|coef*kcw*t_w| < 1).  A real operand x is split into FP16
+// x_hi = fp16(x), x_lo = fp16(x - x_hi) (22 significant bits) and a real
+// product uses three MFMA terms a_hi b_hi + a_hi b_lo + a_lo b_hi.
+//   I side:  a = s_I M[p,I], split once per call by pygw_chi0_presplit into
+//            four FP16 planes in MFMA fragment order (frequency independent).
+//   J side:  b = (s_J M[p,J]) (w_p t_w), weighted and split per frequency
+//            while it is staged into LDS (one block = one frequency).
+//   Re += a_r b_r + a_i b_i,   Im += a_i b_r - a_r b_i   (12 MFMAs / 32 pairs;
+//   the two halves of Im run as separate chains and are subtracted in FP32).
 //
-// The weight enters on the J operand, re-split per frequency:
-//   b = (w_p t_w) (s_J M[p,J]),  Re += a_r b_r + a_i b_i,  Im += a_i b_r - a_r b_i.
-// Only tiles I_t <= J_t are accumulated (into eps, which must hold the
-// running sum, zero for a fresh contraction); pygw_chi0_mirror fills the
-// rest from the Hermitian symmetry.
+// Accumulation in three levels.  The MFMA accumulator's own rounding on
+// gfx950 is not round-to-nearest over long chains (measured: chains of a few
+// hundred steps lose ~1e-6 relative), so each 32-pair chunk starts a fresh
+// 6-MFMA chain from zero; the chain is added (VALU, round-to-nearest) into an
+// FP32 tile accumulator; every `G` chunks the tile accumulator is unscaled by
+// 1/(s_I s_J t_w) (exact) and added into the FP64 eps in memory.  The long
+// K = N_k*N_p reduction is therefore carried in FP64.
+//
+// Tiling.  512 threads = 8 waves per block, block tile 128 (I) x 128 (J),
+// wave tile 64 x 32 (2 x 4 waves), two waves per SIMD.  LDS holds the staged
+// I and J slabs of the current and the next chunk (2 x 2 x 32 KB); the next
+// chunk's global loads are issued before the current chunk's MFMAs.  Only
+// tiles I_t <= J_t are accumulated; pygw_chi0_mirror fills the rest.  The
+// block -> (tile pair, frequency) map keeps all frequencies of one tile pair
+// on one XCD (blocks are dealt to the 8 XCDs round robin), so the slabs they
+// share are read from HBM once per XCD and then served from its L2.
 #include "common.h"
+
+// No FMA contraction in this file, and every split goes through cm::split,
+// which pins x to a rounded f32 register first: the hi/lo split of x = a*b
+// must use the same f32 product in x_hi = fp16(x) and x_lo = fp16(x - x_hi).
+// Otherwise the compiler may form x_hi = fp16(a*b) in one rounding
+// (v_fma_mixlo_f16) and x_lo from the f32-rounded product; for products on an
+// f16 rounding tie the pair then misses x by a full ulp of x_lo (measured on
+// gfx950: 1e-5 relative errors in whole chi0 columns).
+#pragma clang fp contract(off)
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 
 namespace cm {
-constexpr int TB = 64;   // block tile edge
-constexpr int KC = 32;   // pairs per MFMA step
+constexpr int TB = 128;  // block tile edge (I and J)
+constexpr int KC = 32;   // pairs per chunk = MFMA K
+constexpr int NT = 512;  // threads per block
 
 __device__ __forceinline__ void tile_pair(int t, int T, int& It, int& Jt) {
   int i = 0;
@@ -35,169 +57,223 @@ __device__ __forceinline__ void tile_pair(int t, int T, int& It, int& Jt) {
   Jt = i + t;
 }
 
+// x = hi + lo to 22 bits; the empty asm makes x an opaque f32 value so both
+// halves are taken from the same rounding of it
 __device__ __forceinline__ void split(float x, _Float16& hi, _Float16& lo) {
+  asm volatile("" : "+v"(x));
   hi = (_Float16)x;
   lo = (_Float16)(x - (float)hi);
 }
 }  // namespace cm
 
+// Fragment planes.  For k-point k (of the slab), chunk c, group g = 0..3 and
+// plane q (0 re_hi, 1 re_lo, 2 im_hi, 3 im_lo), column I holds the 8 FP16
+// values of pairs p = 32c + 8g + j, j = 0..7 (16 bytes):
+//   S[(((k*npc + c)*4 + g)*4 + q)*pbp + I]     (h8 units; pbp = PB padded to 128)
+// Columns I >= PB and pairs p >= N_p are zero.
+__global__ void __launch_bounds__(256) pygw_chi0_presplit(const float2* __restrict__ M, long strideK, int np, int pb,
+                                                          int pbp, int npc, int nkc, const float* __restrict__ cs,
+                                                          h8* __restrict__ S) {
+  const long n = (long)nkc * npc * 4 * pbp;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const int I = (int)(e % pbp);
+    long r = e / pbp;
+    const int g = (int)(r & 3);
+    r >>= 2;
+    const int c = (int)(r % npc);
+    const int k = (int)(r / npc);
+    const float s = I < pb ? cs[I] : 0.f;
+    h8 rh, rl, ih, il;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int p = 32 * c + 8 * g + j;
+      float2 v = float2{0.f, 0.f};
+      if (p < np && I < pb) v = M[(size_t)k * strideK + (size_t)p * pb + I];
+      _Float16 h, l;
+      cm::split(v.x * s, h, l);
+      rh[j] = h;
+      rl[j] = l;
+      cm::split(v.y * s, h, l);
+      ih[j] = h;
+      il[j] = l;
+    }
+    h8* o = S + (size_t)(e / pbp) * 4 * pbp + I;  // ((k*npc + c)*4 + g)*4*pbp + I
+    o[0] = rh;
+    o[(size_t)pbp] = rl;
+    o[2 * (size_t)pbp] = ih;
+    o[3 * (size_t)pbp] = il;
+  }
+}
+
 // MFMA 16x16x32 f16 operand layouts (lane l, element j = 0..7 of the h8):
 //   A[i][k]: i = l%16, k = 8*(l/16) + j;  B[k][n]: n = l%16, k = 8*(l/16) + j;
 //   D[i][n] (4 values v): n = l%16, i = 4*(l/16) + v.
-template <int NW>
-__global__ void __launch_bounds__(256) pygw_chi0_mixed(const float2* __restrict__ M, long strideK, int np, int pb,
-                                                       const double* __restrict__ kcw, int nom, int k0, int nk,
+// S: fragment planes of this slab; M: the same slab as complex64 (J side);
+// kcw: native [k][w][p] over all k, the slab starting at k-point kbase.
+__global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S, const float2* __restrict__ M,
+                                                       long strideK, int np, int pb, int pbp, int npc, int nkc,
+                                                       const double* __restrict__ kcw, int nom, int kbase,
                                                        double coef, const float* __restrict__ cs,
-                                                       const double* __restrict__ ws, int kflush, int T,
+                                                       const double* __restrict__ ws, int G, int T, int nwork,
                                                        double2* __restrict__ eps) {
   using namespace cm;
-  // staged tiles: [pair][column] (real, imag), pre-scaled by the column scale
-  __shared__ float Ar[KC][TB + 1], Ai[KC][TB + 1], Br[KC][TB + 1], Bi[KC][TB + 1];
-  __shared__ float Wt[NW][KC];
+  __shared__ h8 As[2][16][TB];  // [buf][g*4 + plane][I]
+  __shared__ h8 Bs[2][16][TB];  // [buf][g*4 + plane][J], weighted
+  // XCD-aware work map: blocks b, b+8, b+16, ... run on one XCD and take
+  // consecutive work items (frequency fastest).
+  const int per = gridDim.x >> 3;
+  const int work = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (work >= nwork) return;
+  const int om = work % nom;
   int It, Jt;
-  tile_pair(blockIdx.x, T, It, Jt);
-  const int w0 = blockIdx.y * NW;
+  tile_pair(work / nom, T, It, Jt);
   const int I0 = It * TB, J0 = Jt * TB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
+  const int wr = wave >> 2, wc = wave & 3;
   const int li = lane & 15, lk = lane >> 4;
+  const double tw = ws[om];
 
-  f4 aR[NW][2][2], aI[NW][2][2];
+  // staging roles: A: 4 x 16 B per thread (contiguous copy of 16 planes x 128
+  // columns); B: column Jb = tid % 128, pair group gb = tid / 128 (8 pairs)
+  const int Jb = tid & 127, gb = tid >> 7;
+  const bool jin = (J0 + Jb) < pb;
+  const float sJ = jin ? cs[J0 + Jb] : 0.f;
+  h8 pa[4];
+  float2 pm[8];
+  float pw[8];
+  const long nch = (long)nkc * npc;
+  // per-lane 32-bit offsets against chunk-uniform bases (the bases live in
+  // scalar registers; only the offsets occupy vector registers)
+  const int aoff = (tid >> 7) * pbp + (tid & 127);   // + q*4*pbp for q = 0..3
+  auto fetch = [&](long ch) {
+    const int k = (int)(ch / npc), c = (int)(ch % npc);
+    const h8* Sk = S + ((size_t)k * npc + c) * 16 * (size_t)pbp + I0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) pa[q] = Sk[aoff + q * 4 * pbp];
+    const float2* Mk = M + (size_t)k * strideK;
+    const double* wk = kcw + ((size_t)(kbase + k) * nom + om) * np;
+    const int p0 = 32 * c + 8 * gb;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int p = p0 + j;
+      pm[j] = float2{0.f, 0.f};
+      pw[j] = 0.f;
+      if (p < np) {
+        if (jin) pm[j] = Mk[p * pb + J0 + Jb];
+        pw[j] = (float)(coef * wk[p] * tw);
+      }
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int idx = tid + NT * q;
+      As[buf][idx >> 7][idx & 127] = pa[q];
+    }
+    h8 rh, rl, ih, il;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      _Float16 h, l;
+      split((pm[j].x * sJ) * pw[j], h, l);
+      rh[j] = h;
+      rl[j] = l;
+      split((pm[j].y * sJ) * pw[j], h, l);
+      ih[j] = h;
+      il[j] = l;
+    }
+    Bs[buf][4 * gb + 0][Jb] = rh;
+    Bs[buf][4 * gb + 1][Jb] = rl;
+    Bs[buf][4 * gb + 2][Jb] = ih;
+    Bs[buf][4 * gb + 3][Jb] = il;
+  };
+
+  f4 accR[4][2], accI[4][2];
   auto zero = [&]() {
 #pragma unroll
-    for (int w = 0; w < NW; w++)
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-          aR[w][a][b] = f4{0.f, 0.f, 0.f, 0.f};
-          aI[w][a][b] = f4{0.f, 0.f, 0.f, 0.f};
-        }
+      for (int b = 0; b < 2; b++) {
+        accR[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+        accI[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+      }
   };
   auto flush = [&]() {
-    const size_t pb2 = (size_t)pb * pb;
+    // opaque copies keep the compiler from hoisting the 32 element addresses
+    // (and their scales) out of the chunk loop into live registers
+    int ib = I0 + 64 * wr + 4 * lk, jb = J0 + 32 * wc + li;
+    asm volatile("" : "+v"(ib), "+v"(jb));
+    double2* E = eps + (size_t)pb * pb * om;
 #pragma unroll
-    for (int w = 0; w < NW; w++) {
-      if (w0 + w >= nom) break;
-      const double tw = ws[w0 + w];
-      double2* E = eps + pb2 * (w0 + w);
+    for (int b = 0; b < 2; b++) {
+      const int J = jb + 16 * b;
+      if (J >= pb) continue;
+      const double fj = 1.0 / ((double)cs[J] * tw);
 #pragma unroll
-      for (int a = 0; a < 2; a++)
+      for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++)
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const int I = I0 + 32 * wr + 16 * a + 4 * lk + v;
-            const int J = J0 + 32 * wc + 16 * b + li;
-            if (I < pb && J < pb) {
-              const double f = 1.0 / ((double)cs[I] * (double)cs[J] * tw);  // exact: powers of two
-              double2* e = E + (size_t)I + (size_t)pb * J;
-              double2 o = *e;
-              *e = double2{o.x + (double)aR[w][a][b][v] * f, o.y + (double)aI[w][a][b][v] * f};
-            }
+        for (int v = 0; v < 4; v++) {
+          const int I = ib + 16 * a + v;
+          if (I < pb) {
+            const double f = fj / (double)cs[I];  // exact: powers of two
+            double2* e = E + (size_t)I + (size_t)pb * J;
+            const double2 o = *e;
+            *e = double2{o.x + (double)accR[a][b][v] * f, o.y + (double)accI[a][b][v] * f};
           }
+        }
     }
   };
+
   zero();
-  // chunks (k, p0) in order, next chunk prefetched into registers
-  const int npc = (np + KC - 1) / KC;
-  const long nch = (long)nk * npc;
-  constexpr int NQ = (KC * TB) / 256;
-  float2 pa[NQ], pv[NQ];
-  float pw = 0.f;
-  auto fetch = [&](long ch) {
-    const int kk = (int)(ch / npc), p0 = (int)(ch % npc) * KC;
-    const float2* Mk = M + (size_t)kk * strideK;
-#pragma unroll
-    for (int q = 0; q < NQ; q++) {
-      const int e = tid + 256 * q;
-      const int r = e / TB, c = e % TB;
-      const int p = p0 + r;
-      pa[q] = float2{0.f, 0.f};
-      pv[q] = float2{0.f, 0.f};
-      if (p < np) {
-        if (I0 + c < pb) { pa[q] = Mk[(size_t)p * pb + I0 + c]; pa[q].x *= cs[I0 + c]; pa[q].y *= cs[I0 + c]; }
-        if (J0 + c < pb) { pv[q] = Mk[(size_t)p * pb + J0 + c]; pv[q].x *= cs[J0 + c]; pv[q].y *= cs[J0 + c]; }
-      }
-    }
-    pw = 0.f;
-    if (tid < NW * KC) {
-      const int w = tid / KC, r = tid % KC;
-      if (p0 + r < np && w0 + w < nom)
-        pw = (float)(coef * kcw[((size_t)(k0 + kk) * nom + (w0 + w)) * np + p0 + r] * ws[w0 + w]);
-    }
-  };
   fetch(0);
+  stage(0);
+  __syncthreads();
+  const f4 z = f4{0.f, 0.f, 0.f, 0.f};
   for (long ch = 0; ch < nch; ch++) {
-#pragma unroll
-    for (int q = 0; q < NQ; q++) {
-      const int e = tid + 256 * q;
-      const int r = e / TB, c = e % TB;
-      Ar[r][c] = pa[q].x; Ai[r][c] = pa[q].y;
-      Br[r][c] = pv[q].x; Bi[r][c] = pv[q].y;
-    }
-    if (tid < NW * KC) Wt[tid / KC][tid % KC] = pw;
-    __syncthreads();
+    const int buf = (int)(ch & 1);
     if (ch + 1 < nch) fetch(ch + 1);
-    // A operands (I side): hi/lo of a_r, a_i and of -a_r
-    h8 arh[2], arl[2], aih[2], ail[2], anh[2], anl[2];
+    // J-side fragments (both 16-column sub-tiles of this wave)
+    h8 brh[2], brl[2], bih[2], bil[2];
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-      const int col = 32 * wr + 16 * t + li;
+    for (int b = 0; b < 2; b++) {
+      const int J = 32 * wc + 16 * b + li;
+      brh[b] = Bs[buf][4 * lk + 0][J];
+      brl[b] = Bs[buf][4 * lk + 1][J];
+      bih[b] = Bs[buf][4 * lk + 2][J];
+      bil[b] = Bs[buf][4 * lk + 3][J];
+    }
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int r = 8 * lk + j;
-        _Float16 h, l;
-        cm::split(Ar[r][col], h, l); arh[t][j] = h; arl[t][j] = l; anh[t][j] = -h; anl[t][j] = -l;
-        cm::split(Ai[r][col], h, l); aih[t][j] = h; ail[t][j] = l;
+    for (int a = 0; a < 4; a++) {
+      const int I = 64 * wr + 16 * a + li;
+      const h8 arh = As[buf][4 * lk + 0][I];
+      const h8 arl = As[buf][4 * lk + 1][I];
+      const h8 aih = As[buf][4 * lk + 2][I];
+      const h8 ail = As[buf][4 * lk + 3][I];
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        f4 r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brh[b], z, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brl[b], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, brh[b], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bih[b], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bil[b], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bih[b], r, 0, 0, 0);
+        f4 m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brh[b], z, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brl[b], m, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, brh[b], m, 0, 0, 0);
+        f4 n = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bih[b], z, 0, 0, 0);
+        n = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bil[b], n, 0, 0, 0);
+        n = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bih[b], n, 0, 0, 0);
+        accR[a][b] += r;
+        accI[a][b] += m - n;
       }
+      // keep one 16-row sub-tile's operands and chains live at a time
+      __builtin_amdgcn_sched_barrier(0);
     }
-    float br[2][8], bi[2][8];
-#pragma unroll
-    for (int t = 0; t < 2; t++) {
-      const int col = 32 * wc + 16 * t + li;
-#pragma unroll
-      for (int j = 0; j < 8; j++) { br[t][j] = Br[8 * lk + j][col]; bi[t][j] = Bi[8 * lk + j][col]; }
-    }
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-      h8 brh[2], brl[2], bih[2], bil[2];
-#pragma unroll
-      for (int t = 0; t < 2; t++)
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-          const float wv = Wt[w][8 * lk + j];
-          _Float16 h, l;
-          cm::split(br[t][j] * wv, h, l); brh[t][j] = h; brl[t][j] = l;
-          cm::split(bi[t][j] * wv, h, l); bih[t][j] = h; bil[t][j] = l;
-        }
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-          f4 r = aR[w][a][b], m = aI[w][a][b];
-          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[a], brh[b], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh[a], brl[b], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl[a], brh[b], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[a], bih[b], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[a], bil[b], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[a], bih[b], r, 0, 0, 0);
-          m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[a], brh[b], m, 0, 0, 0);
-          m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih[a], brl[b], m, 0, 0, 0);
-          m = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail[a], brh[b], m, 0, 0, 0);
-          m = __builtin_amdgcn_mfma_f32_16x16x32_f16(anh[a], bih[b], m, 0, 0, 0);
-          m = __builtin_amdgcn_mfma_f32_16x16x32_f16(anh[a], bil[b], m, 0, 0, 0);
-          m = __builtin_amdgcn_mfma_f32_16x16x32_f16(anl[a], bih[b], m, 0, 0, 0);
-          aR[w][a][b] = r;
-          aI[w][a][b] = m;
-        }
-    }
-    __syncthreads();
-    const int kk = (int)(ch / npc);
-    if ((ch % npc) == npc - 1 && ((kk + 1) % kflush == 0 || kk + 1 == nk)) {
+    if (ch + 1 < nch) stage(buf ^ 1);
+    if ((ch + 1) % G == 0 || ch + 1 == nch) {
       flush();
       zero();
     }
+    __syncthreads();
   }
 }
 
@@ -214,41 +290,59 @@ __global__ void pygw_chi0_mirror(double2* __restrict__ eps, int pb, int nom) {
   E[(size_t)I + (size_t)pb * J] = double2{v.x, -v.y};
 }
 
-// per-column scale: cs[I] = 2^(14 - ceil(log2(max_{k,p} max(|Re|,|Im|))))
-__global__ void pygw_colmax(const float2* __restrict__ M, long strideK, int np, int pb, int nk,
-                            unsigned int* __restrict__ cmax) {
-  const int I = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= pb) return;
+// Per-column maximum of max(|Re|,|Im|) over all (k, p): one block per
+// 64-column strip and k-range, a block reduction then one atomic per column.
+__global__ void __launch_bounds__(256) pygw_colmax(const float2* __restrict__ M, long strideK, int np, int pb, int nk,
+                                                   unsigned int* __restrict__ cmax) {
+  __shared__ float red[4][64];
+  const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
+  const int I = blockIdx.x * 64 + c;
   float m = 0.f;
-  for (int k = blockIdx.y; k < nk; k += gridDim.y)
-    for (int p = 0; p < np; p++) {
-      const float2 v = M[(size_t)k * strideK + (size_t)p * pb + I];
-      m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
-    }
-  atomicMax(&cmax[I], __float_as_uint(m));  // non-negative floats order as uints
+  if (I < pb)
+    for (int k = blockIdx.y; k < nk; k += gridDim.y)
+      for (int p = r; p < np; p += 4) {
+        const float2 v = M[(size_t)k * strideK + (size_t)p * pb + I];
+        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+      }
+  red[r][c] = m;
+  __syncthreads();
+  if (r == 0 && I < pb) {
+    m = fmaxf(fmaxf(red[0][c], red[1][c]), fmaxf(red[2][c], red[3][c]));
+    atomicMax(&cmax[I], __float_as_uint(m));  // non-negative floats order as uints
+  }
 }
 
+// cs[I] = 2^(14 - e), e the exponent of the column maximum (|x cs| < 2^14)
 __global__ void pygw_colscale(const unsigned int* __restrict__ cmax, int pb, float* __restrict__ cs) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= pb) return;
   const float m = __uint_as_float(cmax[I]);
   int e = 0;
-  if (m > 0.f) { frexpf(m, &e); }  // m = f * 2^e, f in [0.5, 1)
-  cs[I] = ldexpf(1.0f, 14 - e);     // |x * cs| < 2^14
+  if (m > 0.f) frexpf(m, &e);  // m = f * 2^e, f in [0.5, 1)
+  cs[I] = ldexpf(1.0f, 14 - e);
 }
 
-// per-frequency weight scale: ws[w] = 2^-e, e = exponent of max_{k,p} |coef*kcw|
-__global__ void pygw_wmax(const double* __restrict__ kcw, int np, int nom, int nk, double coef,
-                          unsigned long long* __restrict__ wmax) {
+// per-frequency maximum of |coef*kcw| over (k, p): block reduction, one
+// atomic per block
+__global__ void __launch_bounds__(256) pygw_wmax(const double* __restrict__ kcw, int np, int nom, int nk, double coef,
+                                                 unsigned long long* __restrict__ wmax) {
+  __shared__ double red[4];
   const int w = blockIdx.y;
   double m = 0.0;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)nk * np; e += (long)gridDim.x * blockDim.x) {
     const int k = (int)(e / np), p = (int)(e % np);
     m = fmax(m, fabs(coef * kcw[((size_t)k * nom + w) * np + p]));
   }
-  atomicMax(&wmax[w], (unsigned long long)__double_as_longlong(m));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    atomicMax(&wmax[w], (unsigned long long)__double_as_longlong(m));
+  }
 }
 
+// ws[w] = 2^-e, e the exponent of the frequency maximum (|coef kcw ws| < 1)
 __global__ void pygw_wscale(const unsigned long long* __restrict__ wmax, int nom, double* __restrict__ ws) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nom) return;
@@ -267,44 +361,73 @@ __global__ void pygw_zero_c128(double2* __restrict__ x, long n) {
     x[e] = double2{0.0, 0.0};
 }
 
-size_t chi0_mixed_ws_bytes(int pb, int nom) {
-  return (size_t)pb * 8 + (size_t)nom * 16 + 512;
+namespace {
+constexpr size_t kHead = 1 << 16;                // scales region granule, bytes
+constexpr size_t kSlabBudget = (size_t)1 << 30;  // default fragment-plane budget, bytes
+
+size_t head_bytes(int pb, int nom) {
+  const size_t h = (size_t)pb * 8 + (size_t)nom * 16 + 512;
+  return (h + kHead - 1) / kHead * kHead;
+}
+size_t plane_bytes_per_k(int np, int pb) {
+  const size_t npc = (size_t)(np + cm::KC - 1) / cm::KC;
+  const size_t pbp = (size_t)(pb + cm::TB - 1) / cm::TB * cm::TB;
+  return npc * 16 * pbp * sizeof(h8);
+}
+}  // namespace
+
+// Workspace for a call over n_k k-points: scales + fragment planes of up to
+// min(n_k, 1 GiB / per-k bytes) k-points (the call walks k in slabs).
+size_t chi0_mixed_ws_bytes(int pb, int np, int nom, int nk) {
+  const size_t per_k = plane_bytes_per_k(np, pb);
+  size_t ks = kSlabBudget / per_k;
+  if (ks < 1) ks = 1;
+  if (ks > (size_t)nk) ks = (size_t)nk;
+  return head_bytes(pb, nom) + ks * per_k;
 }
 
 // M: complex64 [n_kc][np][pb] holding k = k0..k0+n_kc-1; kcw covers all k.
-// scales: computed from this M chunk unless `have_scales` (then taken from ws).
+// eps accumulates (beta = 1) or is overwritten (beta = 0).
 int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double* kcw, int nom, int k0, int nkc,
-                      double coef, int kflush, int beta, int have_scales, int nw, void* ws, void* eps,
-                      hipStream_t stream) {
+                      double coef, int G, int beta, void* ws, size_t ws_bytes, void* eps, hipStream_t stream) {
+  using namespace cm;
+  const size_t head = head_bytes(pb, nom);
+  const size_t per_k = plane_bytes_per_k(np, pb);
+  if (ws_bytes < head + per_k || G < 1) return PYGW_E_ARG;
+  size_t ks = (ws_bytes - head) / per_k;
+  if (ks > (size_t)nkc) ks = (size_t)nkc;
   float* cs = (float*)ws;
   unsigned int* cmax = (unsigned int*)(cs + pb);
   double* wsc = (double*)(((uintptr_t)(cmax + pb) + 15) & ~(uintptr_t)15);
   unsigned long long* wmax = (unsigned long long*)(wsc + nom);
-  if (!have_scales) {
-    hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, cmax, (long)pb);
-    hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, (unsigned int*)wmax, 2L * nom);
-    hipLaunchKernelGGL(pygw_colmax, dim3(ceil_div(pb, 64), nkc < 64 ? nkc : 64), dim3(64), 0, stream,
-                       (const float2*)M, strideK, np, pb, nkc, cmax);
-    hipLaunchKernelGGL(pygw_colscale, dim3(ceil_div(pb, 256)), dim3(256), 0, stream, cmax, pb, cs);
-    hipLaunchKernelGGL(pygw_wmax, dim3(64, nom), dim3(256), 0, stream, kcw + (size_t)k0 * nom * np, np, nom, nkc,
-                       coef, wmax);
-    hipLaunchKernelGGL(pygw_wscale, dim3(ceil_div(nom, 256)), dim3(256), 0, stream, wmax, nom, wsc);
-  }
+  h8* S = (h8*)((char*)ws + head);
+  const float2* Mf = (const float2*)M;
+
+  hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, cmax, (long)pb);
+  hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, (unsigned int*)wmax, 2L * nom);
+  hipLaunchKernelGGL(pygw_colmax, dim3(ceil_div(pb, 64), nkc < 256 ? nkc : 256), dim3(256), 0, stream, Mf, strideK,
+                     np, pb, nkc, cmax);
+  hipLaunchKernelGGL(pygw_colscale, dim3(ceil_div(pb, 256)), dim3(256), 0, stream, cmax, pb, cs);
+  hipLaunchKernelGGL(pygw_wmax, dim3(64, nom), dim3(256), 0, stream, kcw + (size_t)k0 * nom * np, np, nom, nkc, coef,
+                     wmax);
+  hipLaunchKernelGGL(pygw_wscale, dim3(ceil_div(nom, 256)), dim3(256), 0, stream, wmax, nom, wsc);
   const long ntot = (long)pb * pb * nom;
   if (!beta) hipLaunchKernelGGL(pygw_zero_c128, dim3(2048), dim3(256), 0, stream, (double2*)eps, ntot);
-  const int T = ceil_div(pb, cm::TB);
-  if (nw == 1) {
-    dim3 grid(T * (T + 1) / 2, nom);
-    hipLaunchKernelGGL(pygw_chi0_mixed<1>, grid, dim3(256), 0, stream, (const float2*)M, strideK, np, pb, kcw, nom,
-                       k0, nkc, coef, cs, wsc, kflush, T, (double2*)eps);
-  } else if (nw == 4) {
-    dim3 grid(T * (T + 1) / 2, ceil_div(nom, 4));
-    hipLaunchKernelGGL(pygw_chi0_mixed<4>, grid, dim3(256), 0, stream, (const float2*)M, strideK, np, pb, kcw, nom,
-                       k0, nkc, coef, cs, wsc, kflush, T, (double2*)eps);
-  } else {
-    dim3 grid(T * (T + 1) / 2, ceil_div(nom, 2));
-    hipLaunchKernelGGL(pygw_chi0_mixed<2>, grid, dim3(256), 0, stream, (const float2*)M, strideK, np, pb, kcw, nom,
-                       k0, nkc, coef, cs, wsc, kflush, T, (double2*)eps);
+
+  const int T = ceil_div(pb, TB);
+  const int pbp = T * TB;
+  const int npc = ceil_div(np, KC);
+  const int nwork = T * (T + 1) / 2 * nom;
+  const int grid = ceil_div(nwork, 8) * 8;
+  for (long kk = 0; kk < nkc; kk += (long)ks) {
+    const int n = (int)((nkc - kk) < (long)ks ? (nkc - kk) : (long)ks);
+    const float2* Ms = Mf + (size_t)kk * strideK;
+    const long nel = (long)n * npc * 4 * pbp;
+    int blocks = (int)((nel + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(pygw_chi0_presplit, dim3(blocks), dim3(256), 0, stream, Ms, strideK, np, pb, pbp, npc, n, cs, S);
+    hipLaunchKernelGGL(pygw_chi0_mixed, dim3(grid), dim3(NT), 0, stream, (const h8*)S, Ms, strideK, np, pb, pbp, npc,
+                       n, kcw, nom, k0 + (int)kk, coef, cs, wsc, G, T, nwork, (double2*)eps);
   }
   hipLaunchKernelGGL(pygw_chi0_mirror, dim3(ceil_div((long)pb * pb, 256), nom), dim3(256), 0, stream,
                      (double2*)eps, pb, nom);

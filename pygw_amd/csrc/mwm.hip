@@ -10,6 +10,10 @@
 // in registers and LDS, so mw never leaves the chip and no atomics are used.
 #include "common.h"
 
+// No FMA contraction: the FP16 hi/lo splits below must see one rounded f32
+// value in both halves.
+#pragma clang fp contract(off)
+
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 namespace mw {
@@ -146,6 +150,7 @@ typedef float f4m __attribute__((ext_vector_type(4)));
 typedef _Float16 h8m __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ void mw_split(float x, _Float16& hi, _Float16& lo) {
+  asm volatile("" : "+v"(x));  // one f32 rounding of x feeds both halves
   hi = (_Float16)x;
   lo = (_Float16)(x - (float)hi);
 }

@@ -110,9 +110,15 @@ def chi0_accumulate(M, kcw_native, coef=-2.0, *, eps=None, precision="fp64", k0=
         buf = eps.as_strided((pb * pb * nom,), (1,))
         beta = 1
     prec = {"fp64": 0, "mixed": 1}[precision]
+    ws, wsb = None, 0
+    if prec == 1:  # fragment planes + scales, from torch's caching allocator
+        sz = C.c_size_t(0)
+        _lib.call("pygw_chi0_workspace", pb, npair, nom, nkc, prec, C.byref(sz))
+        wbuf = torch.empty(sz.value, dtype=torch.uint8, device=device)
+        ws, wsb = C.c_void_p(wbuf.data_ptr()), sz.value
     _lib.call("pygw_chi0_accumulate", C.c_void_p(M.data_ptr()), npair * pb, npair, pb,
               C.c_void_p(kc.data_ptr()), nom, int(k0), nkc, float(coef), prec, beta, C.c_void_p(buf.data_ptr()),
-              None, 0, _stream_ptr(device))
+              ws, wsb, _stream_ptr(device))
     return _fortran_view(buf, pb, nom)
 
 
@@ -221,9 +227,10 @@ def bz_calcqdepw_par3(enk, ef, kqid, ncg, nomx, numin, dUl, omega, tetc, *, layo
 
 
 def autotune_chi0(M, kcw_native, coef=-2.0, *, precision="fp64", k0=0, device="cuda"):
-    """Pick the chi0 tile schedule (Matsubara frequencies per tile block) for
-    this (precision, PB, n_pairs, n_om) by timing the candidates on the given
-    inputs; the choice is cached in the library for later calls.
+    """Pick the chi0 schedule s in {1, 2, 4} for this (precision, PB, n_pairs,
+    n_om) by timing the candidates on the given inputs (FP64: s Matsubara
+    frequencies per tile block; mixed: FP32 tile sums folded into FP64 every
+    16*s chunks); the choice is cached in the library for later calls.
     Returns (best, {candidate: ms})."""
     device = _require_device(device)
     dt = {"fp64": torch.complex128, "mixed": torch.complex64}[precision]
@@ -242,7 +249,7 @@ def autotune_chi0(M, kcw_native, coef=-2.0, *, precision="fp64", k0=0, device="c
 
 
 def set_chi0_schedule(precision, pb, n_pairs, n_om, nw):
-    """Force (nw = 1, 2, 4) or clear (0) the chi0 tile schedule of a shape."""
+    """Force (nw = 1, 2, 4) or clear (0) the chi0 schedule of a shape."""
     _lib.call("pygw_chi0_set_schedule", {"fp64": 0, "mixed": 1}[precision], pb, n_pairs, n_om, nw)
 
 
@@ -275,7 +282,8 @@ class QPlan:
         sz = C.c_size_t(0)
         _lib.call("pygw_tetra_weights_workspace", self.nk, C.byref(sz))
         self.ws_tw = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
-        _lib.call("pygw_chi0_workspace", self.pb, self.nom, {"fp64": 0, "mixed": 1}[precision], C.byref(sz))
+        _lib.call("pygw_chi0_workspace", self.pb, self.np, self.nom, self.nk, {"fp64": 0, "mixed": 1}[precision],
+                  C.byref(sz))
         self.ws_c0 = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
         _lib.call("pygw_eps_inverse_workspace", self.pb, self.nom, C.byref(sz))
         self.ws_ei = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)

@@ -102,7 +102,7 @@ def test_chi0_mixed_vs_fp64(shape):
         print(f"  w={w} rel_frob={err:.2e}", flush=True)
         assert err < 1e-6
     lower = np.tril(np.ones((pb, pb), bool), -1)
-    tile = (np.arange(pb)[:, None] // 64) > (np.arange(pb)[None, :] // 64)
+    tile = (np.arange(pb)[:, None] // 128) > (np.arange(pb)[None, :] // 128)
     mir = np.conj(got.transpose(1, 0, 2))
     assert np.array_equal(got[tile & lower], mir[tile & lower])
     # chunked accumulation over k equals one call to the same precision bar
@@ -110,6 +110,24 @@ def test_chi0_mixed_vs_fp64(shape):
     e2 = P.chi0_accumulate(m64[:h], kcw, precision="mixed")
     P.chi0_accumulate(m64[h:], kcw, precision="mixed", eps=e2, k0=h)
     assert _rel(e2.cpu().numpy(), want) < 1e-6
+    # the smallest workspace (one k-point of fragment planes) walks k in
+    # slabs of one k-point: same result to the same bar
+    import ctypes as C
+    from pygw_amd import _lib
+    sz = C.c_size_t(0)
+    _lib.call("pygw_chi0_workspace", pb, no * ne, nom, 1, 1, C.byref(sz))
+    ws = torch.empty(sz.value, dtype=torch.uint8, device="cuda")
+    Md = torch.as_tensor(m64, device="cuda")
+    e3d = torch.empty(pb * pb * nom, dtype=torch.complex128, device="cuda")
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    args = (C.c_void_p(Md.data_ptr()), no * ne * pb, no * ne, pb, C.c_void_p(kcw.data_ptr()), nom, 0, cfg.nk,
+            -2.0, 1, 0, C.c_void_p(e3d.data_ptr()), C.c_void_p(ws.data_ptr()))
+    _lib.call("pygw_chi0_accumulate", *args, sz.value, stream)
+    e3 = e3d.view(nom, pb, pb).permute(2, 1, 0).cpu().numpy()
+    assert _rel(e3, want) < 1e-6
+    # below one k-point of planes the call is rejected before any launch
+    with pytest.raises(_lib.PygwError):
+        _lib.call("pygw_chi0_accumulate", *args, sz.value // 2, stream)
     torch.cuda.synchronize()
 
 
