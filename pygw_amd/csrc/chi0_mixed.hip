@@ -12,16 +12,13 @@
 //            four FP16 planes in MFMA fragment order (frequency independent).
 //   J side:  b = (s_J M[p,J]) (w_p t_w), weighted and split per frequency
 //            while it is staged into LDS (one block = one frequency).
-//   Re += a_r b_r + a_i b_i,   Im += a_i b_r - a_r b_i   (12 MFMAs / 32 pairs;
-//   the two halves of Im run as separate chains and are subtracted in FP32).
+//   Re += a_r b_r + a_i b_i,   Im += a_i b_r + a_r (-b_i)   (12 MFMAs / 32
+//   pairs; -b_i by a sign flip of the FP16 planes).
 //
-// Accumulation in three levels.  The MFMA accumulator's own rounding on
-// gfx950 is not round-to-nearest over long chains (measured: chains of a few
-// hundred steps lose ~1e-6 relative), so each 32-pair chunk starts a fresh
-// 6-MFMA chain from zero; the chain is added (VALU, round-to-nearest) into an
-// FP32 tile accumulator; every `G` chunks the tile accumulator is unscaled by
-// 1/(s_I s_J t_w) (exact) and added into the FP64 eps in memory.  The long
-// K = N_k*N_p reduction is therefore carried in FP64.
+// Accumulation.  The MFMA accumulators (FP32; the accumulate rounds to
+// nearest, measured on gfx950) carry G chunks of 32 pairs; then the tile is
+// unscaled by 1/(s_I s_J t_w) (exact) and added into the FP64 eps in memory,
+// so the long K = N_k*N_p reduction is carried in FP64.
 //
 // Tiling.  512 threads = 8 waves per block, block tile 128 (I) x 128 (J),
 // wave tile 64 x 32 (2 x 4 waves), two waves per SIMD.  LDS holds the staged
@@ -44,6 +41,7 @@
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
 namespace cm {
 constexpr int TB = 128;  // block tile edge (I and J)
@@ -55,6 +53,12 @@ __device__ __forceinline__ void tile_pair(int t, int T, int& It, int& Jt) {
   while (t >= T - i) { t -= T - i; i++; }
   It = i;
   Jt = i + t;
+}
+
+__device__ __forceinline__ h8 neg(h8 x) {
+  u4 u = __builtin_bit_cast(u4, x);
+  u ^= u4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+  return __builtin_bit_cast(h8, u);
 }
 
 // x = hi + lo to 22 bits; the empty asm makes x an opaque f32 value so both
@@ -109,11 +113,12 @@ __global__ void __launch_bounds__(256) pygw_chi0_presplit(const float2* __restri
 //   A[i][k]: i = l%16, k = 8*(l/16) + j;  B[k][n]: n = l%16, k = 8*(l/16) + j;
 //   D[i][n] (4 values v): n = l%16, i = 4*(l/16) + v.
 // S: fragment planes of this slab; M: the same slab as complex64 (J side);
-// kcw: native [k][w][p] over all k, the slab starting at k-point kbase.
+// wf: the slab's scaled weights coef*kcw*t_w in FP32, [k][w][npc*32] (zero
+// past N_p), from pygw_chi0_wprep.
 __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S, const float2* __restrict__ M,
                                                        long strideK, int np, int pb, int pbp, int npc, int nkc,
-                                                       const double* __restrict__ kcw, int nom, int kbase,
-                                                       double coef, const float* __restrict__ cs,
+                                                       const float* __restrict__ wf, int nom,
+                                                       const float* __restrict__ cs,
                                                        const double* __restrict__ ws, int G, int T, int nwork,
                                                        double2* __restrict__ eps) {
   using namespace cm;
@@ -142,27 +147,36 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
   float2 pm[8];
   float pw[8];
   const long nch = (long)nkc * npc;
-  // per-lane 32-bit offsets against chunk-uniform bases (the bases live in
-  // scalar registers; only the offsets occupy vector registers)
-  const int aoff = (tid >> 7) * pbp + (tid & 127);   // + q*4*pbp for q = 0..3
-  auto fetch = [&](long ch) {
-    const int k = (int)(ch / npc), c = (int)(ch % npc);
+  // Loads go through buffer resources built from chunk-uniform bases (scalar
+  // registers) with per-lane byte offsets that stay fixed across chunks, and
+  // out-of-range offsets read as zero: J-side pairs p >= N_p fall past the
+  // end of the k-point's M block, columns J >= PB get an offset past it.
+  constexpr int RSRC_FLAGS = 0x00020000;
+  const unsigned aoffB = (unsigned)((tid >> 7) * pbp + (tid & 127)) * 16u;  // + q * 4*pbp*16
+  const unsigned aqB = 4u * (unsigned)pbp * 16u;
+  const unsigned pb8 = (unsigned)pb * 8u;
+  const unsigned moff0 = jin ? ((unsigned)(8 * gb) * pb8 + (unsigned)(J0 + Jb) * 8u) : 0xF0000000u;
+  const int npad = npc * KC;
+  auto fetch = [&](int k, int c) {
     const h8* Sk = S + ((size_t)k * npc + c) * 16 * (size_t)pbp + I0;
+    const __amdgpu_buffer_rsrc_t ra =
+        __builtin_amdgcn_make_buffer_rsrc((void*)Sk, (short)0, (int)(16u * (unsigned)pbp * 16u), RSRC_FLAGS);
 #pragma unroll
-    for (int q = 0; q < 4; q++) pa[q] = Sk[aoff + q * 4 * pbp];
-    const float2* Mk = M + (size_t)k * strideK;
-    const double* wk = kcw + ((size_t)(kbase + k) * nom + om) * np;
-    const int p0 = 32 * c + 8 * gb;
+    for (int q = 0; q < 4; q++)
+      pa[q] = __builtin_bit_cast(h8, __builtin_amdgcn_raw_buffer_load_b128(ra, aoffB + q * aqB, 0, 0));
+    const __amdgpu_buffer_rsrc_t rm = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(M + (size_t)k * strideK), (short)0, (int)((unsigned)np * pb8), RSRC_FLAGS);
+    const unsigned mo = moff0 + (unsigned)(32 * c) * pb8;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int p = p0 + j;
-      pm[j] = float2{0.f, 0.f};
-      pw[j] = 0.f;
-      if (p < np) {
-        if (jin) pm[j] = Mk[p * pb + J0 + Jb];
-        pw[j] = (float)(coef * wk[p] * tw);
-      }
-    }
+    for (int j = 0; j < 8; j++)
+      pm[j] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rm, mo + j * pb8, 0, 0));
+    const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(wf + ((size_t)k * nom + om) * (size_t)npad), (short)0, npad * 4, RSRC_FLAGS);
+    const unsigned wo = (unsigned)(32 * c + 8 * gb) * 4u;
+    const u4 w0 = __builtin_amdgcn_raw_buffer_load_b128(rw, wo, 0, 0);
+    const u4 w1 = __builtin_amdgcn_raw_buffer_load_b128(rw, wo + 16, 0, 0);
+    pw[0] = __uint_as_float(w0.x); pw[1] = __uint_as_float(w0.y); pw[2] = __uint_as_float(w0.z); pw[3] = __uint_as_float(w0.w);
+    pw[4] = __uint_as_float(w1.x); pw[5] = __uint_as_float(w1.y); pw[6] = __uint_as_float(w1.z); pw[7] = __uint_as_float(w1.w);
   };
   auto stage = [&](int buf) {
 #pragma unroll
@@ -224,15 +238,17 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
   };
 
   zero();
-  fetch(0);
+  fetch(0, 0);
   stage(0);
   __syncthreads();
-  const f4 z = f4{0.f, 0.f, 0.f, 0.f};
+  // (kn, cn): the chunk after the current one; g: chunks since the last flush
+  int kn = npc > 1 ? 0 : 1, cn = npc > 1 ? 1 : 0, g = 0;
   for (long ch = 0; ch < nch; ch++) {
     const int buf = (int)(ch & 1);
-    if (ch + 1 < nch) fetch(ch + 1);
-    // J-side fragments (both 16-column sub-tiles of this wave)
-    h8 brh[2], brl[2], bih[2], bil[2];
+    const bool more = ch + 1 < nch;
+    if (more) fetch(kn, cn);
+    // J-side fragments (both 16-column sub-tiles of this wave); -b_i for Im
+    h8 brh[2], brl[2], bih[2], bil[2], nbih[2], nbil[2];
 #pragma unroll
     for (int b = 0; b < 2; b++) {
       const int J = 32 * wc + 16 * b + li;
@@ -240,6 +256,8 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
       brl[b] = Bs[buf][4 * lk + 1][J];
       bih[b] = Bs[buf][4 * lk + 2][J];
       bil[b] = Bs[buf][4 * lk + 3][J];
+      nbih[b] = neg(bih[b]);
+      nbil[b] = neg(bil[b]);
     }
 #pragma unroll
     for (int a = 0; a < 4; a++) {
@@ -250,28 +268,31 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
       const h8 ail = As[buf][4 * lk + 3][I];
 #pragma unroll
       for (int b = 0; b < 2; b++) {
-        f4 r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brh[b], z, 0, 0, 0);
+        f4 r = accR[a][b], m = accI[a][b];
+        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brh[b], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brh[b], m, 0, 0, 0);
         r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brl[b], r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, brh[b], r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bih[b], r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bil[b], r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bih[b], r, 0, 0, 0);
-        f4 m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brh[b], z, 0, 0, 0);
         m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brl[b], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, brh[b], r, 0, 0, 0);
         m = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, brh[b], m, 0, 0, 0);
-        f4 n = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bih[b], z, 0, 0, 0);
-        n = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bil[b], n, 0, 0, 0);
-        n = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bih[b], n, 0, 0, 0);
-        accR[a][b] += r;
-        accI[a][b] += m - n;
+        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bih[b], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, nbih[b], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bil[b], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, nbil[b], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bih[b], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, nbih[b], m, 0, 0, 0);
+        accR[a][b] = r;
+        accI[a][b] = m;
       }
-      // keep one 16-row sub-tile's operands and chains live at a time
-      __builtin_amdgcn_sched_barrier(0);
     }
-    if (ch + 1 < nch) stage(buf ^ 1);
-    if ((ch + 1) % G == 0 || ch + 1 == nch) {
+    if (more) {
+      stage(buf ^ 1);
+      if (++cn == npc) { cn = 0; kn++; }
+    }
+    if (++g == G || !more) {
       flush();
       zero();
+      g = 0;
     }
     __syncthreads();
   }
@@ -361,25 +382,41 @@ __global__ void pygw_zero_c128(double2* __restrict__ x, long n) {
     x[e] = double2{0.0, 0.0};
 }
 
+// wf[k][w][p] = (float)(coef * kcw[k0+k][w][p] * ws[w]) for p < N_p, 0 up to
+// npc*32: the J-side weights of one slab, scaled and rounded once.
+__global__ void pygw_chi0_wprep(const double* __restrict__ kcw, int np, int nom, int k0, int nkc, int npad, double coef,
+                                const double* __restrict__ ws, float* __restrict__ wf) {
+  const long n = (long)nkc * nom * npad;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const int p = (int)(e % npad);
+    const long kw = e / npad;  // k*nom + w
+    const int w = (int)(kw % nom);
+    wf[e] = p < np ? (float)(coef * kcw[((size_t)k0 * nom + kw) * np + p] * ws[w]) : 0.f;
+  }
+}
+
 namespace {
 constexpr size_t kHead = 1 << 16;                // scales region granule, bytes
-constexpr size_t kSlabBudget = (size_t)1 << 30;  // default fragment-plane budget, bytes
+constexpr size_t kSlabBudget = (size_t)1 << 30;  // default slab budget, bytes
 
 size_t head_bytes(int pb, int nom) {
   const size_t h = (size_t)pb * 8 + (size_t)nom * 16 + 512;
   return (h + kHead - 1) / kHead * kHead;
 }
-size_t plane_bytes_per_k(int np, int pb) {
+// fragment planes + FP32 weights of one k-point (both multiples of 256 B)
+size_t slab_bytes_per_k(int np, int pb, int nom) {
   const size_t npc = (size_t)(np + cm::KC - 1) / cm::KC;
   const size_t pbp = (size_t)(pb + cm::TB - 1) / cm::TB * cm::TB;
-  return npc * 16 * pbp * sizeof(h8);
+  const size_t wbytes = ((size_t)nom * npc * cm::KC * sizeof(float) + 255) / 256 * 256;
+  return npc * 16 * pbp * sizeof(h8) + wbytes;
 }
 }  // namespace
 
-// Workspace for a call over n_k k-points: scales + fragment planes of up to
-// min(n_k, 1 GiB / per-k bytes) k-points (the call walks k in slabs).
+// Workspace for a call over n_k k-points: scales + fragment planes and
+// weights of up to min(n_k, 1 GiB / per-k bytes) k-points (the call walks k
+// in slabs).
 size_t chi0_mixed_ws_bytes(int pb, int np, int nom, int nk) {
-  const size_t per_k = plane_bytes_per_k(np, pb);
+  const size_t per_k = slab_bytes_per_k(np, pb, nom);
   size_t ks = kSlabBudget / per_k;
   if (ks < 1) ks = 1;
   if (ks > (size_t)nk) ks = (size_t)nk;
@@ -392,7 +429,7 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
                       double coef, int G, int beta, void* ws, size_t ws_bytes, void* eps, hipStream_t stream) {
   using namespace cm;
   const size_t head = head_bytes(pb, nom);
-  const size_t per_k = plane_bytes_per_k(np, pb);
+  const size_t per_k = slab_bytes_per_k(np, pb, nom);
   if (ws_bytes < head + per_k || G < 1) return PYGW_E_ARG;
   size_t ks = (ws_bytes - head) / per_k;
   if (ks > (size_t)nkc) ks = (size_t)nkc;
@@ -400,7 +437,11 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
   unsigned int* cmax = (unsigned int*)(cs + pb);
   double* wsc = (double*)(((uintptr_t)(cmax + pb) + 15) & ~(uintptr_t)15);
   unsigned long long* wmax = (unsigned long long*)(wsc + nom);
+  const int T = ceil_div(pb, TB);
+  const int pbp = T * TB;
+  const int npc = ceil_div(np, KC);
   h8* S = (h8*)((char*)ws + head);
+  float* wf = (float*)((char*)ws + head + ks * (size_t)npc * 16 * pbp * sizeof(h8));
   const float2* Mf = (const float2*)M;
 
   hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, cmax, (long)pb);
@@ -414,9 +455,6 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
   const long ntot = (long)pb * pb * nom;
   if (!beta) hipLaunchKernelGGL(pygw_zero_c128, dim3(2048), dim3(256), 0, stream, (double2*)eps, ntot);
 
-  const int T = ceil_div(pb, TB);
-  const int pbp = T * TB;
-  const int npc = ceil_div(np, KC);
   const int nwork = T * (T + 1) / 2 * nom;
   const int grid = ceil_div(nwork, 8) * 8;
   for (long kk = 0; kk < nkc; kk += (long)ks) {
@@ -426,8 +464,13 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
     int blocks = (int)((nel + 255) / 256);
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(pygw_chi0_presplit, dim3(blocks), dim3(256), 0, stream, Ms, strideK, np, pb, pbp, npc, n, cs, S);
+    const long nw_el = (long)n * nom * npc * KC;
+    int wblocks = (int)((nw_el + 255) / 256);
+    if (wblocks > 65536) wblocks = 65536;
+    hipLaunchKernelGGL(pygw_chi0_wprep, dim3(wblocks), dim3(256), 0, stream, kcw, np, nom, k0 + (int)kk, n, npc * KC,
+                       coef, (const double*)wsc, wf);
     hipLaunchKernelGGL(pygw_chi0_mixed, dim3(grid), dim3(NT), 0, stream, (const h8*)S, Ms, strideK, np, pb, pbp, npc,
-                       n, kcw, nom, k0 + (int)kk, coef, cs, wsc, G, T, nwork, (double2*)eps);
+                       n, (const float*)wf, nom, cs, wsc, G, T, nwork, (double2*)eps);
   }
   hipLaunchKernelGGL(pygw_chi0_mirror, dim3(ceil_div((long)pb * pb, 256), nom), dim3(256), 0, stream,
                      (double2*)eps, pb, nom);
