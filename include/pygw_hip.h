@@ -124,9 +124,13 @@ int pygw_eps_inverse(void* eps, int pb, int n_om, void* workspace, size_t ws_byt
  *   precision 0 = FP64 MFMA; 1 = FP16 hi+lo split (rows of M and columns
  *   of W scaled by powers of two), 3 products, FP32 accumulation over I,
  *   FP64 row sum over J
+ *   workspace: pygw_mwm_workspace bytes (0 for precision 0); NULL lets the
+ *   call allocate it (stream-ordered); a smaller one (at least the row
+ *   planes and one frequency's W planes) walks the frequencies in slabs.
  * ------------------------------------------------------------------------ */
+int pygw_mwm_workspace(int n_k, int rows_per_k, int pb, int n_om, int precision, size_t* bytes);
 int pygw_mwm(const void* M, int n_k, int rows_per_k, int pb, const void* W, int n_om, double wkq, int precision,
-             void* mwm_out, void* stream);
+             void* mwm_out, void* workspace, size_t ws_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Spline refinement of mwm onto the precise mesh (Compute_selfc_inside,

@@ -4,7 +4,7 @@
 int mwm_f64_launch(const void* M, int nrows, int pb, const void* W, int nom, double wkq, int rpk, void* out,
                    hipStream_t stream);
 int mwm_mixed_launch(const void* M, int nrows, int pb, const void* W, int nom, double wkq, int rpk, void* ws,
-                     void* out, hipStream_t stream);
+                     size_t ws_bytes, void* out, hipStream_t stream);
 size_t mwm_mixed_ws_bytes(int nrows, int pb, int nom);
 int selfc_spline_launch(const void* mwm, int nrows, int nb1, int nom, int nb2, const double* x, const double* xp,
                         int nomp, void* out, hipStream_t stream);
@@ -14,19 +14,35 @@ int selfc_conv_launch(const double* enk, int nrows, int nb1, int nb2, const void
 int pade_qp_launch(const void* sig, int nstate, int nom, const double* omega, int npts, const double* ene,
                    const double* sigx, const double* vxc, double* out, hipStream_t stream);
 
+extern "C" int pygw_mwm_workspace(int n_k, int rows_per_k, int pb, int n_om, int precision, size_t* bytes) {
+  if (n_k <= 0 || rows_per_k <= 0 || pb <= 0 || n_om <= 0 || !bytes || (precision != 0 && precision != 1))
+    return PYGW_E_ARG;
+  if ((long)n_k * rows_per_k > (1L << 31) - 1) return PYGW_E_ARG;
+  *bytes = precision == 0 ? 0 : mwm_mixed_ws_bytes(n_k * rows_per_k, pb, n_om);
+  return PYGW_OK;
+}
+
 extern "C" int pygw_mwm(const void* M, int n_k, int rows_per_k, int pb, const void* W, int n_om, double wkq,
-                        int precision, void* mwm_out, void* stream_) {
+                        int precision, void* mwm_out, void* workspace, size_t ws_bytes, void* stream_) {
   hipStream_t stream = (hipStream_t)stream_;
   if (!M || !W || !mwm_out || n_k <= 0 || rows_per_k <= 0 || pb <= 0 || n_om <= 0) return PYGW_E_ARG;
   if ((long)n_k * rows_per_k > (1L << 31) - 1 || n_om > 65535) return PYGW_E_ARG;
   const int nrows = n_k * rows_per_k;
   if (precision == 0) return mwm_f64_launch(M, nrows, pb, W, n_om, wkq, rows_per_k, mwm_out, stream);
   if (precision != 1) return PYGW_E_ARG;
-  void* ws = nullptr;
-  HIP_TRY(hipMallocAsync(&ws, mwm_mixed_ws_bytes(nrows, pb, n_om), stream));
-  int rc = mwm_mixed_launch(M, nrows, pb, W, n_om, wkq, rows_per_k, ws, mwm_out, stream);
-  hipError_t e = hipFreeAsync(ws, stream);
-  if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
+  void* ws = workspace;
+  bool own = false;
+  if (!ws) {
+    ws_bytes = mwm_mixed_ws_bytes(nrows, pb, n_om);
+    HIP_TRY(hipMallocAsync(&ws, ws_bytes, stream));
+    own = true;
+  }
+  // a smaller caller workspace walks the frequencies in smaller slabs
+  int rc = mwm_mixed_launch(M, nrows, pb, W, n_om, wkq, rows_per_k, ws, ws_bytes, mwm_out, stream);
+  if (own) {
+    hipError_t e = hipFreeAsync(ws, stream);
+    if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
+  }
   return rc;
 }
 

@@ -149,8 +149,12 @@ def calc_mwm(M_selfe, W, wkq, *, precision="fp64", device="cuda"):
         raise ValueError("W must be a Fortran-strided [PB, PB, n_om] complex128 tensor")
     nom = W.shape[2]
     out = torch.empty((nk, nom, rpk), dtype=torch.complex128, device=device)
+    prec = {"fp64": 0, "mixed": 1}[precision]
+    sz = C.c_size_t(0)
+    _lib.call("pygw_mwm_workspace", nk, rpk, pb, nom, prec, C.byref(sz))
+    wbuf = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=device)
     _lib.call("pygw_mwm", C.c_void_p(M.data_ptr()), nk, rpk, pb, C.c_void_p(W.data_ptr()), nom, float(wkq),
-              {"fp64": 0, "mixed": 1}[precision], C.c_void_p(out.data_ptr()), _stream_ptr(device))
+              prec, C.c_void_p(out.data_ptr()), C.c_void_p(wbuf.data_ptr()), sz.value, _stream_ptr(device))
     return out
 
 

@@ -102,3 +102,34 @@ def test_mwm_mixed_vs_fp64():
             err = _rel(got[k, w], want[w])
             print(f"  k={k} w={w} rel={err:.2e}", flush=True)
             assert err < 1e-6
+
+
+@pytest.mark.parametrize("shape", [(3, 5, 30, 129, 3), (1, 8, 17, 260, 2)])
+def test_mwm_mixed_slabs_and_ragged(shape):
+    """Split-precision MWM with rows and PB off the 128 tile and the
+    frequencies walked in slabs (a workspace one frequency short): same
+    result as one slab, both within 1e-6 of the FP64 oracle."""
+    import ctypes as C
+    import torch
+    from pygw_amd import _lib
+    from pygw_amd import polarization as P
+    nk, nb1, nb2, pb, nom = shape
+    rng = np.random.default_rng(91 + pb)
+    M = (rng.normal(size=(nk, nb1 * nb2, pb)) + 1j * rng.normal(size=(nk, nb1 * nb2, pb))).astype(np.complex64)
+    W = _herm_w(pb, nom, rng)
+    print(f"case mwm mixed slabs {shape}", flush=True)
+    Wt = torch.as_tensor(W.reshape(-1, order="F"), device="cuda").as_strided((pb, pb, nom), (1, pb, pb * pb))
+    full = P.calc_mwm(M, Wt, 0.25, precision="mixed").cpu().numpy()
+    sz = C.c_size_t(0)
+    _lib.call("pygw_mwm_workspace", nk, nb1 * nb2, pb, nom, 1, C.byref(sz))
+    ws = torch.empty(sz.value, dtype=torch.uint8, device="cuda")
+    Md = torch.as_tensor(M, device="cuda")
+    out = torch.empty((nk, nom, nb1 * nb2), dtype=torch.complex128, device="cuda")
+    _lib.call("pygw_mwm", C.c_void_p(Md.data_ptr()), nk, nb1 * nb2, pb, C.c_void_p(Wt.data_ptr()), nom, 0.25, 1,
+              C.c_void_p(out.data_ptr()), C.c_void_p(ws.data_ptr()), sz.value - 1,
+              C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    np.testing.assert_array_equal(out.cpu().numpy(), full)
+    for k in range(nk):
+        want = chain.calc_mwm(M[k].astype(complex), W, 0.25)
+        for w in range(nom):
+            assert _rel(full[k, w], want[w]) < 1e-6
