@@ -59,7 +59,7 @@ def calc_mwm(minm2, w, wkq):
     out = np.empty((nom, minm2.shape[0]), dtype=complex)
     cm = np.conj(minm2)
     for iom in range(nom):
-        mw = cm @ w[:, :, iom]
+        mw = cm @ np.ascontiguousarray(w[:, :, iom])  # a strided slice would miss BLAS
         out[iom] = wkq * np.sum(mw * minm2, axis=-1)
     return out
 
