@@ -12,8 +12,11 @@
 //            four FP16 planes in MFMA fragment order (frequency independent).
 //   J side:  b = (s_J M[p,J]) (w_p t_w), weighted and split per frequency
 //            while it is staged into LDS (one block = one frequency).
-//   Re += a_r b_r + a_i b_i,   Im += a_i b_r + a_r (-b_i)   (12 MFMAs / 32
-//   pairs; -b_i by a sign flip of the FP16 planes).
+//   Re += a_r b_r + a_i b_i,   Im += a_i b_r + (-a_r) b_i   (12 MFMAs per
+//   32x32 tile and 16 pairs on v_mfma_f32_32x32x16_f16; -a_r by a sign flip
+//   of the FP16 planes).  The 32x32 shape keeps the MFMA pipe full with the
+//   four independent accumulator chains a wave holds (2 tiles x Re/Im);
+//   16x16x32 needs ~16 chains for the same rate (measured).
 //
 // Accumulation.  The MFMA accumulators (FP32; the accumulate rounds to
 // nearest, measured on gfx950) carry G chunks of 32 pairs; then the tile is
@@ -21,7 +24,7 @@
 // so the long K = N_k*N_p reduction is carried in FP64.
 //
 // Tiling.  512 threads = 8 waves per block, block tile 128 (I) x 128 (J),
-// wave tile 64 x 32 (2 x 4 waves), two waves per SIMD.  LDS holds the staged
+// wave tile 64 x 32 (2 x 4 waves; two 32x32 MFMA tiles), two waves per SIMD.  LDS holds the staged
 // I and J slabs of the current and the next chunk (2 x 2 x 32 KB); the next
 // chunk's global loads are issued before the current chunk's MFMAs.  Only
 // tiles I_t <= J_t are accumulated; pygw_chi0_mirror fills the rest.  The
@@ -40,6 +43,7 @@
 #pragma clang fp contract(off)
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
@@ -109,9 +113,10 @@ __global__ void __launch_bounds__(256) pygw_chi0_presplit(const float2* __restri
   }
 }
 
-// MFMA 16x16x32 f16 operand layouts (lane l, element j = 0..7 of the h8):
-//   A[i][k]: i = l%16, k = 8*(l/16) + j;  B[k][n]: n = l%16, k = 8*(l/16) + j;
-//   D[i][n] (4 values v): n = l%16, i = 4*(l/16) + v.
+// MFMA 32x32x16 f16 operand layouts (lane l, element j = 0..7 of the h8;
+// probed on gfx950):
+//   A[i][k]: i = l%32, k = 8*(l/32) + j;  B[k][n]: n = l%32, k = 8*(l/32) + j;
+//   D[i][n] (16 values v): n = l%32, i = 8*(v/4) + 4*(l/32) + v%4.
 // S: fragment planes of this slab; M: the same slab as complex64 (J side);
 // wf: the slab's scaled weights coef*kcw*t_w in FP32, [k][w][npc*32] (zero
 // past N_p), from pygw_chi0_wprep.
@@ -135,7 +140,6 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
   const int I0 = It * TB, J0 = Jt * TB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 2, wc = wave & 3;
-  const int li = lane & 15, lk = lane >> 4;
   const double tw = ws[om];
 
   // staging roles: A: 4 x 16 B per thread (contiguous copy of 16 planes x 128
@@ -143,9 +147,14 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
   const int Jb = tid & 127, gb = tid >> 7;
   const bool jin = (J0 + Jb) < pb;
   const float sJ = jin ? cs[J0 + Jb] : 0.f;
-  h8 pa[4];
-  float2 pm[8];
-  float pw[8];
+  // one chunk's staged inputs, held in registers between its load and its
+  // LDS store (two sets: the next chunk is stored while the one after it
+  // loads)
+  struct Pre {
+    h8 a[4];
+    float2 m[8];
+    float w[8];
+  };
   const long nch = (long)nkc * npc;
   // Loads go through buffer resources built from chunk-uniform bases (scalar
   // registers) with per-lane byte offsets that stay fixed across chunks, and
@@ -157,41 +166,41 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
   const unsigned pb8 = (unsigned)pb * 8u;
   const unsigned moff0 = jin ? ((unsigned)(8 * gb) * pb8 + (unsigned)(J0 + Jb) * 8u) : 0xF0000000u;
   const int npad = npc * KC;
-  auto fetch = [&](int k, int c) {
+  auto fetch = [&](Pre& P, int k, int c) {
     const h8* Sk = S + ((size_t)k * npc + c) * 16 * (size_t)pbp + I0;
     const __amdgpu_buffer_rsrc_t ra =
         __builtin_amdgcn_make_buffer_rsrc((void*)Sk, (short)0, (int)(16u * (unsigned)pbp * 16u), RSRC_FLAGS);
 #pragma unroll
     for (int q = 0; q < 4; q++)
-      pa[q] = __builtin_bit_cast(h8, __builtin_amdgcn_raw_buffer_load_b128(ra, aoffB + q * aqB, 0, 0));
+      P.a[q] = __builtin_bit_cast(h8, __builtin_amdgcn_raw_buffer_load_b128(ra, aoffB + q * aqB, 0, 0));
     const __amdgpu_buffer_rsrc_t rm = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(M + (size_t)k * strideK), (short)0, (int)((unsigned)np * pb8), RSRC_FLAGS);
     const unsigned mo = moff0 + (unsigned)(32 * c) * pb8;
 #pragma unroll
     for (int j = 0; j < 8; j++)
-      pm[j] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rm, mo + j * pb8, 0, 0));
+      P.m[j] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rm, mo + j * pb8, 0, 0));
     const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(wf + ((size_t)k * nom + om) * (size_t)npad), (short)0, npad * 4, RSRC_FLAGS);
     const unsigned wo = (unsigned)(32 * c + 8 * gb) * 4u;
     const u4 w0 = __builtin_amdgcn_raw_buffer_load_b128(rw, wo, 0, 0);
     const u4 w1 = __builtin_amdgcn_raw_buffer_load_b128(rw, wo + 16, 0, 0);
-    pw[0] = __uint_as_float(w0.x); pw[1] = __uint_as_float(w0.y); pw[2] = __uint_as_float(w0.z); pw[3] = __uint_as_float(w0.w);
-    pw[4] = __uint_as_float(w1.x); pw[5] = __uint_as_float(w1.y); pw[6] = __uint_as_float(w1.z); pw[7] = __uint_as_float(w1.w);
+    P.w[0] = __uint_as_float(w0.x); P.w[1] = __uint_as_float(w0.y); P.w[2] = __uint_as_float(w0.z); P.w[3] = __uint_as_float(w0.w);
+    P.w[4] = __uint_as_float(w1.x); P.w[5] = __uint_as_float(w1.y); P.w[6] = __uint_as_float(w1.z); P.w[7] = __uint_as_float(w1.w);
   };
-  auto stage = [&](int buf) {
+  auto stage = [&](const Pre& P, int buf) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int idx = tid + NT * q;
-      As[buf][idx >> 7][idx & 127] = pa[q];
+      As[buf][idx >> 7][idx & 127] = P.a[q];
     }
     h8 rh, rl, ih, il;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       _Float16 h, l;
-      split((pm[j].x * sJ) * pw[j], h, l);
+      split((P.m[j].x * sJ) * P.w[j], h, l);
       rh[j] = h;
       rl[j] = l;
-      split((pm[j].y * sJ) * pw[j], h, l);
+      split((P.m[j].y * sJ) * P.w[j], h, l);
       ih[j] = h;
       il[j] = l;
     }
@@ -201,95 +210,118 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
     Bs[buf][4 * gb + 3][Jb] = il;
   };
 
-  f4 accR[4][2], accI[4][2];
+  // accumulators: two 32x32 tiles (rows 64*wr + 32*a) of the wave's 64 x 32
+  f16v accR[2], accI[2];
   auto zero = [&]() {
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int b = 0; b < 2; b++) {
-        accR[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-        accI[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+      for (int v = 0; v < 16; v++) {
+        accR[a][v] = 0.f;
+        accI[a][v] = 0.f;
       }
   };
   auto flush = [&]() {
     // opaque copies keep the compiler from hoisting the 32 element addresses
     // (and their scales) out of the chunk loop into live registers
-    int ib = I0 + 64 * wr + 4 * lk, jb = J0 + 32 * wc + li;
-    asm volatile("" : "+v"(ib), "+v"(jb));
-    double2* E = eps + (size_t)pb * pb * om;
+    int ib = I0 + 64 * wr + 4 * (lane >> 5), J = J0 + 32 * wc + (lane & 31);
+    asm volatile("" : "+v"(ib), "+v"(J));
+    if (J >= pb) return;
+    double2* E = eps + (size_t)pb * pb * om + (size_t)pb * J;
+    const double fj = 1.0 / ((double)cs[J] * tw);
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const int J = jb + 16 * b;
-      if (J >= pb) continue;
-      const double fj = 1.0 / ((double)cs[J] * tw);
+    for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int I = ib + 16 * a + v;
-          if (I < pb) {
-            const double f = fj / (double)cs[I];  // exact: powers of two
-            double2* e = E + (size_t)I + (size_t)pb * J;
-            const double2 o = *e;
-            *e = double2{o.x + (double)accR[a][b][v] * f, o.y + (double)accI[a][b][v] * f};
-          }
+      for (int v = 0; v < 16; v++) {
+        const int I = ib + 32 * a + 8 * (v >> 2) + (v & 3);
+        if (I < pb) {
+          const double f = fj / (double)cs[I];  // exact: powers of two
+          const double2 o = E[I];
+          E[I] = double2{o.x + (double)accR[a][v] * f, o.y + (double)accI[a][v] * f};
         }
+      }
+  };
+
+  // chunk `buf`'s products (LDS buffer buf), with the next chunk's inputs
+  // from S stored into the other buffer in the same basic block so that the
+  // compiler can interleave that VALU/LDS work with the MFMAs
+  auto body = [&](int buf, const Pre& S) {
+    // J-side fragments of the wave's 32 columns for the two K steps of 16
+    const int Jw = 32 * wc + (lane & 31), hk = lane >> 5;
+    h8 brh[2], brl[2], bih[2], bil[2];
+#pragma unroll
+    for (int st = 0; st < 2; st++) {
+      const int g = 2 * st + hk;
+      brh[st] = Bs[buf][4 * g + 0][Jw];
+      brl[st] = Bs[buf][4 * g + 1][Jw];
+      bih[st] = Bs[buf][4 * g + 2][Jw];
+      bil[st] = Bs[buf][4 * g + 3][Jw];
+    }
+    stage(S, buf ^ 1);
+#pragma unroll
+    for (int st = 0; st < 2; st++) {
+      const int g = 2 * st + hk;
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int I = 64 * wr + 32 * a + (lane & 31);
+        const h8 arh = As[buf][4 * g + 0][I];
+        const h8 arl = As[buf][4 * g + 1][I];
+        const h8 aih = As[buf][4 * g + 2][I];
+        const h8 ail = As[buf][4 * g + 3][I];
+        const h8 narh = neg(arh), narl = neg(arl);  // Im = a_i b_r + (-a_r) b_i
+        f16v r = accR[a], m = accI[a];
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh, brh[st], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, brh[st], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh, brl[st], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, brl[st], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(arl, brh[st], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(ail, brh[st], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, bih[st], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(narh, bih[st], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, bil[st], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(narh, bil[st], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(ail, bih[st], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(narl, bih[st], m, 0, 0, 0);
+        accR[a] = r;
+        accI[a] = m;
+      }
     }
   };
 
+  // Pipeline: chunk ch computes from LDS buffer ch&1 while chunk ch+1 (in
+  // registers since the previous step) is stored into the other buffer and
+  // chunk ch+2 loads.  Past the end the last chunk is loaded and stored
+  // again (harmless: that buffer is not read any more).
+  const long last = nch - 1;
+  int kf = 0, cf = 0;  // next chunk to fetch
+  long nf = 0;         // its index
+  auto fetch_next = [&](Pre& P) {
+    fetch(P, kf, cf);
+    if (nf < last) {
+      nf++;
+      if (++cf == npc) { cf = 0; kf++; }
+    }
+  };
   zero();
-  fetch(0, 0);
-  stage(0);
+  Pre P0, P1;
+  fetch_next(P0);
+  stage(P0, 0);
+  fetch_next(P1);
   __syncthreads();
-  // (kn, cn): the chunk after the current one; g: chunks since the last flush
-  int kn = npc > 1 ? 0 : 1, cn = npc > 1 ? 1 : 0, g = 0;
-  for (long ch = 0; ch < nch; ch++) {
-    const int buf = (int)(ch & 1);
-    const bool more = ch + 1 < nch;
-    if (more) fetch(kn, cn);
-    // J-side fragments (both 16-column sub-tiles of this wave); -b_i for Im
-    h8 brh[2], brl[2], bih[2], bil[2], nbih[2], nbil[2];
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const int J = 32 * wc + 16 * b + li;
-      brh[b] = Bs[buf][4 * lk + 0][J];
-      brl[b] = Bs[buf][4 * lk + 1][J];
-      bih[b] = Bs[buf][4 * lk + 2][J];
-      bil[b] = Bs[buf][4 * lk + 3][J];
-      nbih[b] = neg(bih[b]);
-      nbil[b] = neg(bil[b]);
+  int g = 0;
+  for (long ch = 0; ch < nch; ch += 2) {
+    fetch_next(P0);  // chunk ch+2
+    body(0, P1);     // computes ch, stores ch+1
+    if (++g == G || ch == last) {
+      flush();
+      zero();
+      g = 0;
     }
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-      const int I = 64 * wr + 16 * a + li;
-      const h8 arh = As[buf][4 * lk + 0][I];
-      const h8 arl = As[buf][4 * lk + 1][I];
-      const h8 aih = As[buf][4 * lk + 2][I];
-      const h8 ail = As[buf][4 * lk + 3][I];
-#pragma unroll
-      for (int b = 0; b < 2; b++) {
-        f4 r = accR[a][b], m = accI[a][b];
-        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brh[b], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brh[b], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, brl[b], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, brl[b], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, brh[b], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, brh[b], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bih[b], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, nbih[b], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bil[b], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, nbil[b], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bih[b], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, nbih[b], m, 0, 0, 0);
-        accR[a][b] = r;
-        accI[a][b] = m;
-      }
-    }
-    if (more) {
-      stage(buf ^ 1);
-      if (++cn == npc) { cn = 0; kn++; }
-    }
-    if (++g == G || !more) {
+    __syncthreads();
+    if (ch == last) break;
+    fetch_next(P1);  // chunk ch+3
+    body(1, P0);     // computes ch+1, stores ch+2
+    if (++g == G || ch + 1 == last) {
       flush();
       zero();
       g = 0;
