@@ -334,6 +334,8 @@ def run_chain(args):
                 sig += sc
                 times.append([ev[i].elapsed_time(ev[i + 1]) / 1e3 for i in range(5)])
                 walls.append(wall)
+            print(f"[bench] {prec} q={q} {wall:.3f} s{' (warm-up)' if iq < args.warmup else ''}", file=sys.stderr,
+                  flush=True)
             del M, Ms, sc
         ene = np.ascontiguousarray(b.enk[wl.ibgw:wl.ibgw + wl.nbgw][:, wl.klist].T).reshape(-1)
         qp = P.pade_quasiparticles(sig.view(-1, base.n_om), wl.om, ene).cpu().numpy()
@@ -400,8 +402,10 @@ def run_chain(args):
                                    "fp64_with_1e-7_noise_max": float(dqp_n.max()),
                                    "fp64_with_1e-7_noise_median": float(np.median(dqp_n))},
     }
+    print("[bench] precision report done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         rec["cpu_baseline"] = chain_cpu_baseline(wl, args.cpu_kslab)
+        print("[bench] cpu baseline done", file=sys.stderr, flush=True)
     print(json.dumps(rec))
 
 
@@ -418,10 +422,6 @@ def main():
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.steps is None:
         args.steps = {"c5": 2, "c2": 8}.get(args.config, 1)
-    if args.config == "c5":
-        run_chain(args)
-        return
-    mixed = not args.no_mixed
     if args.copy_calibration:
         # one copy kernel that reads and writes exactly 1 GiB: the ratio of its
         # FETCH_SIZE / WRITE_SIZE counters to 1 GiB scales the kernels' counters
@@ -431,11 +431,16 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.config == "c5":
+        run_chain(args)
+        return
+    mixed = not args.no_mixed
     wl = Workload(args.config, args.nq, mixed, args.fp64_kfrac)
     cfg, b = wl.base, wl.bands
     no, ne, pb, nom, nk = b.n_o, b.n_e, cfg.pb, cfg.n_om, cfg.nk
     steps = args.steps if args.config == "c2" else min(args.steps, 2)
     s64, st64 = timed(P, wl, "fp64", steps, args.warmup)
+    print(f"[bench] fp64 {s64:.3f} s/step", file=sys.stderr, flush=True)
     kscale = nk / wl.nk64
     t_chi64 = st64[1] * kscale
     K = nk * no * ne
