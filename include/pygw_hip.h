@@ -91,7 +91,7 @@ int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, cons
 
 /* Autotuner (no counterpart in PyGW): times the contraction's schedules
  * s = 1, 2, 4 (FP64: s Matsubara frequencies per 64x64 tile block; split
- * precision: FP32 tile sums folded into FP64 every 16*s chunks) on the given
+ * precision: FP32 tile sums folded into FP64 every 64*s chunks) on the given
  * inputs and records the fastest for (device, precision, pb, n_pairs, n_om);
  * later pygw_chi0_accumulate calls with that shape use it.  eps is
  * overwritten (scratch).  best: chosen frequencies per block; times[3]

@@ -12,7 +12,7 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
 
 // Autotune cache: (device, precision, pb, n_pairs, n_om) -> schedule s in
 // {1, 2, 4}: FP64, s frequencies per 64x64 tile block; split precision, the
-// FP32 tile accumulator folded into FP64 every 16*s chunks of 32 pairs.  The
+// FP32 tile accumulator folded into FP64 every 64*s chunks of 32 pairs.  The
 // only global state of the library; guarded by a mutex.
 namespace {
 std::mutex g_tune_mu;
@@ -55,7 +55,7 @@ extern "C" int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, i
     own = true;
   }
   // a smaller caller workspace walks k in smaller slabs (rejected below one k)
-  int rc = chi0_mixed_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, 16 * nw, beta, ws, ws_bytes, eps,
+  int rc = chi0_mixed_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, 64 * nw, beta, ws, ws_bytes, eps,
                              stream);
   if (own) {
     hipError_t e = hipFreeAsync(ws, stream);
@@ -130,7 +130,7 @@ extern "C" int pygw_chi0_autotune(const void* M, long stride_k, int n_pairs, int
     for (int rep = 0; rep < 2 && rc == PYGW_OK; rep++) {  // first run warms up
       if (hipEventRecord(e0, stream) != hipSuccess) { rc = PYGW_E_HIP; break; }
       if (precision == 0) rc = chi0_f64_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, eps, 0, nw, stream);
-      else rc = chi0_mixed_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, 16 * nw, 0, ws, wsb, eps, stream);
+      else rc = chi0_mixed_launch(M, stride_k, n_pairs, pb, kcw, n_om, k0, n_k, coef, 64 * nw, 0, ws, wsb, eps, stream);
       if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = PYGW_E_HIP; break; }
     }
     float ms = 0.f;

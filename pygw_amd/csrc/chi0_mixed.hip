@@ -27,7 +27,8 @@
 // wave tile 64 x 32 (2 x 4 waves; two 32x32 MFMA tiles), two waves per SIMD.  LDS holds the staged
 // I and J slabs of the current and the next chunk (2 x 2 x 32 KB); the next
 // chunk's global loads are issued before the current chunk's MFMAs.  Only
-// tiles I_t <= J_t are accumulated; pygw_chi0_mirror fills the rest.  The
+// tile pairs I_t <= J_t are accumulated, stored conjugated at the transposed
+// place (coalesced); pygw_chi0_mirror fills the rest.  The
 // block -> (tile pair, frequency) map keeps all frequencies of one tile pair
 // on one XCD (blocks are dealt to the 8 XCDs round robin), so the slabs they
 // share are read from HBM once per XCD and then served from its L2.
@@ -140,7 +141,8 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
   const int I0 = It * TB, J0 = Jt * TB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 2, wc = wave & 3;
-  const double tw = ws[om];
+  const double tw = ws[om], itw = 1.0 / tw;  // exact: a power of two
+  const float* ics = cs + pb;                  // 1/cs, exact
 
   // staging roles: A: 4 x 16 B per thread (contiguous copy of 16 planes x 128
   // columns); B: column Jb = tid % 128, pair group gb = tid / 128 (8 pairs)
@@ -221,23 +223,28 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
         accI[a][v] = 0.f;
       }
   };
+  // The tile is added, conjugated, at its transposed place eps[J, I]: there
+  // the 32 lanes of a half-wave (consecutive J, one I) touch 512 contiguous
+  // bytes of the column-major eps.  pygw_chi0_mirror then fills the tiles
+  // above the diagonal from the ones below.
   auto flush = [&]() {
     // opaque copies keep the compiler from hoisting the 32 element addresses
     // (and their scales) out of the chunk loop into live registers
     int ib = I0 + 64 * wr + 4 * (lane >> 5), J = J0 + 32 * wc + (lane & 31);
     asm volatile("" : "+v"(ib), "+v"(J));
     if (J >= pb) return;
-    double2* E = eps + (size_t)pb * pb * om + (size_t)pb * J;
-    const double fj = 1.0 / ((double)cs[J] * tw);
+    double2* E = eps + (size_t)pb * pb * om + J;
+    const double fj = (double)ics[J] * itw;
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
       for (int v = 0; v < 16; v++) {
         const int I = ib + 32 * a + 8 * (v >> 2) + (v & 3);
         if (I < pb) {
-          const double f = fj / (double)cs[I];  // exact: powers of two
-          const double2 o = E[I];
-          E[I] = double2{o.x + (double)accR[a][v] * f, o.y + (double)accI[a][v] * f};
+          const double f = fj * (double)ics[I];  // exact: powers of two
+          double2* e = E + (size_t)pb * I;
+          const double2 o = *e;
+          *e = double2{o.x + (double)accR[a][v] * f, o.y - (double)accI[a][v] * f};
         }
       }
   };
@@ -330,14 +337,15 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
   }
 }
 
-// Lower-triangle tiles from the upper ones: eps[J,I] = conj(eps[I,J]) for
-// I in tile It < Jt.  One thread per element of the strict lower part.
+// Tiles above the diagonal from the ones below: eps[I,J] = conj(eps[J,I])
+// for I in tile It < Jt (the contraction writes the tiles It >= Jt).  One
+// thread per element.
 __global__ void pygw_chi0_mirror(double2* __restrict__ eps, int pb, int nom) {
   const long n = (long)pb * pb;
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int I = (int)(e % pb), J = (int)(e / pb);  // element (I, J), column-major
-  if ((I / cm::TB) <= (J / cm::TB)) return;         // computed tile
+  if ((I / cm::TB) >= (J / cm::TB)) return;         // computed tile
   double2* E = eps + (size_t)blockIdx.y * n;
   const double2 v = E[(size_t)J + (size_t)pb * I];
   E[(size_t)I + (size_t)pb * J] = double2{v.x, -v.y};
@@ -365,7 +373,8 @@ __global__ void __launch_bounds__(256) pygw_colmax(const float2* __restrict__ M,
   }
 }
 
-// cs[I] = 2^(14 - e), e the exponent of the column maximum (|x cs| < 2^14)
+// cs[I] = 2^(14 - e), e the exponent of the column maximum (|x cs| < 2^14);
+// cs[pb + I] = 1 / cs[I]
 __global__ void pygw_colscale(const unsigned int* __restrict__ cmax, int pb, float* __restrict__ cs) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= pb) return;
@@ -373,6 +382,7 @@ __global__ void pygw_colscale(const unsigned int* __restrict__ cmax, int pb, flo
   int e = 0;
   if (m > 0.f) frexpf(m, &e);  // m = f * 2^e, f in [0.5, 1)
   cs[I] = ldexpf(1.0f, 14 - e);
+  cs[pb + I] = ldexpf(1.0f, e - 14);  // its exact inverse
 }
 
 // per-frequency maximum of |coef*kcw| over (k, p): block reduction, one
@@ -432,7 +442,7 @@ constexpr size_t kHead = 1 << 16;                // scales region granule, bytes
 constexpr size_t kSlabBudget = (size_t)1 << 30;  // default slab budget, bytes
 
 size_t head_bytes(int pb, int nom) {
-  const size_t h = (size_t)pb * 8 + (size_t)nom * 16 + 512;
+  const size_t h = (size_t)pb * 12 + (size_t)nom * 16 + 512;
   return (h + kHead - 1) / kHead * kHead;
 }
 // fragment planes + FP32 weights of one k-point (both multiples of 256 B)
@@ -465,8 +475,8 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
   if (ws_bytes < head + per_k || G < 1) return PYGW_E_ARG;
   size_t ks = (ws_bytes - head) / per_k;
   if (ks > (size_t)nkc) ks = (size_t)nkc;
-  float* cs = (float*)ws;
-  unsigned int* cmax = (unsigned int*)(cs + pb);
+  float* cs = (float*)ws;  // [pb] scales, then [pb] their inverses
+  unsigned int* cmax = (unsigned int*)(cs + 2 * pb);
   double* wsc = (double*)(((uintptr_t)(cmax + pb) + 15) & ~(uintptr_t)15);
   unsigned long long* wmax = (unsigned long long*)(wsc + nom);
   const int T = ceil_div(pb, TB);

@@ -102,48 +102,90 @@ __global__ void __launch_bounds__(64) pygw_selfc_spline(const double2* __restric
 //                                        omega_p, omega[iom], womeg_p)    (fnc.f90:86-119)
 // row = (k, ie1); mwm = [N_k][nom][nb1][nb2]; mwmp = [nrows][nb2][nomp]; for
 // the coarse form (all_fr_convolutions) pass mwmp = NULL and the coarse mesh.
-// One wavefront per (row, iom): lanes split ie2, then reduce.
-__global__ void __launch_bounds__(64) pygw_selfc_conv(const double* __restrict__ enk, int nb1, int nb2,
-                                                      const double2* __restrict__ mwm, int nom,
-                                                      const double* __restrict__ omega,
-                                                      const double2* __restrict__ mwmp, int nomp,
-                                                      const double* __restrict__ omp, const double* __restrict__ womp,
-                                                      double2* __restrict__ out) {
+// One block per (row, group of CG frequencies): the threads split the
+// precise-mesh points l (coalesced reads of mwmp, each read once per group),
+// sum over l per (ie2, iom) in registers, scale by eps_om/pi per ie2 as the
+// reference does, then reduce over the threads.
+namespace sc {
+constexpr int CG = 16;   // Matsubara points per block
+constexpr int CT = 256;  // threads per block
+}  // namespace sc
+
+__global__ void __launch_bounds__(256) pygw_selfc_conv(const double* __restrict__ enk, int nb1, int nb2,
+                                                       const double2* __restrict__ mwm, int nom,
+                                                       const double* __restrict__ omega,
+                                                       const double2* __restrict__ mwmp, int nomp,
+                                                       const double* __restrict__ omp, const double* __restrict__ womp,
+                                                       double2* __restrict__ out) {
   using namespace sc;
-  const int row = blockIdx.x, iom = blockIdx.y;
+  __shared__ double2 red[CT / 64][CG];
+  const int row = blockIdx.x, i0 = blockIdx.y * CG;
   const int k = row / nb1, ie1 = row % nb1;
-  const int lane = threadIdx.x;
-  const double wn = omega[iom];
+  const int t = threadIdx.x;
+  const int ng = nom - i0 < CG ? nom - i0 : CG;
   const double wmax = omp[nomp - 1];
-  double2 acc = {0.0, 0.0};
-  for (int ie2 = lane; ie2 < nb2; ie2 += 64) {
+  double2 acc[CG];
+#pragma unroll
+  for (int g = 0; g < CG; g++) acc[g] = double2{0.0, 0.0};
+  for (int ie2 = 0; ie2 < nb2; ie2++) {
     const double e = enk[(size_t)k * nb2 + ie2];
-    const double2 mn = mwm[(((size_t)k * nom + iom) * nb1 + ie1) * nb2 + ie2];
-    const double2 eo = double2{e, -wn};
-    const double2 eo2 = cmul(eo, eo);
-    double2 s0 = {0.0, 0.0};
-    for (int l = 0; l < nomp; l++) {
+    double2 s0[CG];
+#pragma unroll
+    for (int g = 0; g < CG; g++) s0[g] = double2{0.0, 0.0};
+    for (int l = t; l < nomp; l += CT) {
       const double2 ml = mwmp ? mwmp[((size_t)row * nb2 + ie2) * nomp + l]
                               : mwm[(((size_t)k * nom + l) * nb1 + ie1) * nb2 + ie2];
-      const double2 num = double2{(ml.x - mn.x) * womp[l], (ml.y - mn.y) * womp[l]};
-      const double2 den = double2{omp[l] * omp[l] + eo2.x, eo2.y};
-      s0 = cadd(s0, cdiv(num, den));
+      const double wl = womp[l], o2 = omp[l] * omp[l];
+#pragma unroll
+      for (int g = 0; g < CG; g++) {
+        if (g < ng) {
+          const double wn = omega[i0 + g];
+          const double2 mn = mwm[(((size_t)k * nom + i0 + g) * nb1 + ie1) * nb2 + ie2];
+          const double2 num = double2{(ml.x - mn.x) * wl, (ml.y - mn.y) * wl};
+          const double2 den = double2{o2 + (e * e - wn * wn), -2.0 * e * wn};  // omega_l^2 + (e - i w_n)^2
+          s0[g] = cadd(s0[g], cdiv(num, den));
+        }
+      }
     }
-    double2 res = cmul(eo, s0);
-    res.x /= PI_F;
-    res.y /= PI_F;
-    if (hypot(mn.x, mn.y) > 1e-16) {
-      const double2 at = catan_(cdiv(double2{wmax, 0.0}, eo));
-      const double2 t = cmul(double2{mn.x / PI_F, mn.y / PI_F}, at);
-      res = cadd(res, t);
+#pragma unroll
+    for (int g = 0; g < CG; g++) {
+      if (g < ng) {
+        const double2 eo = double2{e, -omega[i0 + g]};
+        const double2 r = cmul(eo, s0[g]);
+        acc[g] = double2{acc[g].x + r.x / PI_F, acc[g].y + r.y / PI_F};
+      }
     }
-    acc = cadd(acc, res);
+    // the atan tail once per (ie2, iom), by one thread
+    if ((ie2 % CT) == t) {
+#pragma unroll
+      for (int g = 0; g < CG; g++) {
+        if (g < ng) {
+          const double2 mn = mwm[(((size_t)k * nom + i0 + g) * nb1 + ie1) * nb2 + ie2];
+          if (hypot(mn.x, mn.y) > 1e-16) {
+            const double2 eo = double2{e, -omega[i0 + g]};
+            const double2 at = catan_(cdiv(double2{wmax, 0.0}, eo));
+            acc[g] = cadd(acc[g], cmul(double2{mn.x / PI_F, mn.y / PI_F}, at));
+          }
+        }
+      }
+    }
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    acc.x += __shfl_xor(acc.x, off, 64);
-    acc.y += __shfl_xor(acc.y, off, 64);
+  const int lane = t & 63, wv = t >> 6;
+#pragma unroll
+  for (int g = 0; g < CG; g++) {
+    double x = acc[g].x, y = acc[g].y;
+    for (int off = 32; off > 0; off >>= 1) {
+      x += __shfl_xor(x, off, 64);
+      y += __shfl_xor(y, off, 64);
+    }
+    if (lane == 0) red[wv][g] = double2{x, y};
   }
-  if (lane == 0) out[(size_t)row * nom + iom] = acc;
+  __syncthreads();
+  if (t < ng) {
+    double2 r = red[0][t];
+    for (int w = 1; w < CT / 64; w++) r = cadd(r, red[w][t]);
+    out[(size_t)row * nom + i0 + t] = r;
+  }
 }
 
 // Pade (Thiele) through the first npts points of sigma(i w_n), evaluated at
@@ -206,8 +248,8 @@ int selfc_spline_launch(const void* mwm, int nrows, int nb1, int nom, int nb2, c
 int selfc_conv_launch(const double* enk, int nrows, int nb1, int nb2, const void* mwm, int nom, const double* omega,
                       const void* mwmp, int nomp, const double* omp, const double* womp, void* out,
                       hipStream_t stream) {
-  dim3 grid(nrows, nom);
-  hipLaunchKernelGGL(pygw_selfc_conv, grid, dim3(64), 0, stream, enk, nb1, nb2, (const double2*)mwm, nom, omega,
+  dim3 grid(nrows, ceil_div(nom, sc::CG));
+  hipLaunchKernelGGL(pygw_selfc_conv, grid, dim3(sc::CT), 0, stream, enk, nb1, nb2, (const double2*)mwm, nom, omega,
                      (const double2*)mwmp, nomp, omp, womp, (double2*)out);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }

@@ -234,7 +234,7 @@ def autotune_chi0(M, kcw_native, coef=-2.0, *, precision="fp64", k0=0, device="c
     """Pick the chi0 schedule s in {1, 2, 4} for this (precision, PB, n_pairs,
     n_om) by timing the candidates on the given inputs (FP64: s Matsubara
     frequencies per tile block; mixed: FP32 tile sums folded into FP64 every
-    16*s chunks); the choice is cached in the library for later calls.
+    64*s chunks); the choice is cached in the library for later calls.
     Returns (best, {candidate: ms})."""
     device = _require_device(device)
     dt = {"fp64": torch.complex128, "mixed": torch.complex64}[precision]
