@@ -102,6 +102,27 @@ int pygw_chi0_autotune(const void* M, long stride_k, int n_pairs, int pb, const 
 int pygw_chi0_set_schedule(int precision, int pb, int n_pairs, int n_om, int nw);
 
 /* ------------------------------------------------------------------------
+ * Batched split-precision chi0 over many q-points (no single counterpart in
+ * PyGW: it is the q loop of G0W0.Compute, pygw.py:3654-3718, around the
+ * contraction of calc_eps, pygw.py:3161-3200, for uneven PB per q and
+ * ragged per-k pair windows).  Per q: M[q] complex64 [K_q][PB_q] (the
+ * window pairs of all k concatenated), kcw[q] FP64 [n_om][K_q], eps[q]
+ * complex128 Fortran [PB_q, PB_q, n_om] (overwritten).  plan uploads the
+ * descriptor tables into the workspace (synchronous on `stream`) and returns
+ * an opaque plan; run only launches kernels (capturable in a HIP graph):
+ * per-q scales and fragment planes, one persistent contraction over all
+ * (q, tile pair, frequency) work items, per-q mirror.  schedule 0 = the
+ * autotuned one (default 2); 1, 2, 4 = fold every 64*s chunks.
+ * ------------------------------------------------------------------------ */
+int pygw_chi0_batch_workspace(int n_q, const int* pb, const int* n_pairs, int n_om, size_t* bytes);
+int pygw_chi0_batch_plan(int n_q, const void* const* M, const int* pb, const int* n_pairs, const double* const* kcw,
+                         int n_om, double coef, void* const* eps, void* workspace, size_t ws_bytes, void* stream,
+                         void** plan);
+int pygw_chi0_batch_run(void* plan, int schedule, void* stream);
+int pygw_chi0_batch_autotune(void* plan, int* best, float* times, void* stream);
+int pygw_chi0_batch_free(void* plan);
+
+/* ------------------------------------------------------------------------
  * Dielectric inversion (calc_eps, pygw.py:3229-3271, q != 0):
  *   eps[:,:,w] <- ((eps + eps^H)/2 + 1)^{-1} - 1   for every w, in place.
  * Replaces `linalg.inv` (pygw.py:3244); block Gauss-Jordan in FP64 without
@@ -176,6 +197,11 @@ int pygw_pade_qp(const void* sigc, int n_state, int n_om, const double* omega, i
 int pygw_synth_minm(const double* enk, int n_bnd, const int* kqid, int row_band0, int col_band0, int n_o, int n_e,
                     int pb, const double* vhalf, unsigned long long seed, int k0, int n_kc, int complex128,
                     void* out, void* stream);
+/* The same draws for a list of (k, p) pairs (int32 pairs[2*j] = k, pairs[2*j+1] = p):
+ * out[j][I] = M[k_j][p_j][I]; for ragged per-k pair windows. */
+int pygw_synth_minm_pairs(const double* enk, int n_bnd, const int* kqid, int row_band0, int col_band0, int n_o,
+                          int n_e, int pb, const double* vhalf, unsigned long long seed, const int* pairs,
+                          long n_pairs, int complex128, void* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,12 @@ _SIGS = {
                                      C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p]),
     "pygw_chi0_set_schedule": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "pygw_chi0_batch_workspace": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_size_t)]),
+    "pygw_chi0_batch_plan": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                       C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "pygw_chi0_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "pygw_chi0_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_void_p, C.c_void_p]),
+    "pygw_chi0_batch_free": (C.c_int, [C.c_void_p]),
     "pygw_eps_inverse_workspace": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
     "pygw_eps_inverse": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p,
                                    C.c_void_p]),
@@ -50,6 +56,9 @@ _SIGS = {
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "pygw_pade_qp": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pygw_synth_minm_pairs": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_void_p, C.c_ulonglong, C.c_void_p, C.c_long, C.c_int,
+                                        C.c_void_p, C.c_void_p]),
     "pygw_synth_minm": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.c_void_p, C.c_ulonglong, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
 }

@@ -27,6 +27,13 @@ int tuned_nw(int precision, int pb, int np, int nom) {
 }
 }  // namespace
 size_t chi0_mixed_ws_bytes(int pb, int np, int nom, int nk);
+struct Chi0BatchPlan;
+size_t chi0_batch_ws_bytes(int nq, const int* pb, const int* np, int nom);
+int chi0_batch_plan(int nq, const void* const* M, const int* pb, const int* np, const double* const* kcw, int nom,
+                    double coef, void* const* eps, void* ws, size_t ws_bytes, hipStream_t stream,
+                    Chi0BatchPlan** out);
+int chi0_batch_run(const Chi0BatchPlan* P, int G, hipStream_t stream);
+void chi0_batch_free(Chi0BatchPlan* P);
 int eps_inverse_launch(void* eps, int n, int nom, void* ws, int* dinfo, hipStream_t stream);
 size_t eps_inverse_ws_bytes(int n, int nom);
 
@@ -161,5 +168,92 @@ extern "C" int pygw_chi0_set_schedule(int precision, int pb, int n_pairs, int n_
   auto key = std::make_tuple(dev, precision, pb, n_pairs, n_om);
   if (nw == 0) g_tune.erase(key);
   else g_tune[key] = nw;
+  return PYGW_OK;
+}
+
+// --- batched split-precision contraction over many q (one persistent launch)
+namespace {
+bool batch_args_ok(int n_q, const int* pb, const int* n_pairs, int n_om) {
+  if (n_q <= 0 || !pb || !n_pairs || n_om <= 0 || n_om > 65535) return false;
+  for (int q = 0; q < n_q; q++) {
+    if (pb[q] <= 0 || n_pairs[q] <= 0) return false;
+    if ((size_t)(n_pairs[q] + 32) * pb[q] * 8 >= ((size_t)1 << 31)) return false;  // 32-bit buffer offsets
+  }
+  return true;
+}
+std::mutex g_batch_mu;
+std::map<const void*, int> g_batch_sched;  // plan -> schedule chosen by pygw_chi0_batch_autotune
+}  // namespace
+
+extern "C" int pygw_chi0_batch_workspace(int n_q, const int* pb, const int* n_pairs, int n_om, size_t* bytes) {
+  if (!bytes || !batch_args_ok(n_q, pb, n_pairs, n_om)) return PYGW_E_ARG;
+  *bytes = chi0_batch_ws_bytes(n_q, pb, n_pairs, n_om);
+  return PYGW_OK;
+}
+
+extern "C" int pygw_chi0_batch_plan(int n_q, const void* const* M, const int* pb, const int* n_pairs,
+                                    const double* const* kcw, int n_om, double coef, void* const* eps,
+                                    void* workspace, size_t ws_bytes, void* stream, void** plan) {
+  if (!M || !kcw || !eps || !workspace || !plan || !batch_args_ok(n_q, pb, n_pairs, n_om)) return PYGW_E_ARG;
+  for (int q = 0; q < n_q; q++)
+    if (!M[q] || !kcw[q] || !eps[q]) return PYGW_E_ARG;
+  Chi0BatchPlan* P = nullptr;
+  const int rc = chi0_batch_plan(n_q, M, pb, n_pairs, kcw, n_om, coef, eps, workspace, ws_bytes,
+                                 (hipStream_t)stream, &P);
+  if (rc == PYGW_OK) *plan = P;
+  return rc;
+}
+
+extern "C" int pygw_chi0_batch_run(void* plan, int schedule, void* stream) {
+  if (!plan || (schedule != 0 && schedule != 1 && schedule != 2 && schedule != 4)) return PYGW_E_ARG;
+  if (schedule == 0) {
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    auto it = g_batch_sched.find(plan);
+    schedule = it == g_batch_sched.end() ? 2 : it->second;
+  }
+  return chi0_batch_run((const Chi0BatchPlan*)plan, 64 * schedule, (hipStream_t)stream);
+}
+
+// Times the batch with each schedule s = 1, 2, 4 (FP32 tiles folded into
+// FP64 every 64*s chunks) and keeps the fastest for later runs of this plan.
+extern "C" int pygw_chi0_batch_autotune(void* plan, int* best, float* times, void* stream_) {
+  if (!plan || !best) return PYGW_E_ARG;
+  hipStream_t stream = (hipStream_t)stream_;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const int cand[3] = {1, 2, 4};
+  float tbest = 1e30f;
+  int nbest = 2, rc = PYGW_OK;
+  for (int i = 0; i < 3 && rc == PYGW_OK; i++) {
+    for (int rep = 0; rep < 2 && rc == PYGW_OK; rep++) {
+      if (hipEventRecord(e0, stream) != hipSuccess) { rc = PYGW_E_HIP; break; }
+      rc = chi0_batch_run((const Chi0BatchPlan*)plan, 64 * cand[i], stream);
+      if (rc == PYGW_OK && (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+        rc = PYGW_E_HIP;
+    }
+    float ms = 0.f;
+    if (rc == PYGW_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PYGW_E_HIP;
+    if (times) times[i] = ms;
+    if (rc == PYGW_OK && ms < tbest) { tbest = ms; nbest = cand[i]; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != PYGW_OK) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    g_batch_sched[plan] = nbest;
+  }
+  *best = nbest;
+  return PYGW_OK;
+}
+
+extern "C" int pygw_chi0_batch_free(void* plan) {
+  if (!plan) return PYGW_E_ARG;
+  {
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    g_batch_sched.erase(plan);
+  }
+  chi0_batch_free((Chi0BatchPlan*)plan);
   return PYGW_OK;
 }

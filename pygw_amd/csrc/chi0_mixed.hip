@@ -34,6 +34,8 @@
 // share are read from HBM once per XCD and then served from its L2.
 #include "common.h"
 
+#include <vector>
+
 // No FMA contraction in this file, and every split goes through cm::split,
 // which pins x to a rounded f32 register first: the hi/lo split of x = a*b
 // must use the same f32 product in x_hi = fp16(x) and x_lo = fp16(x - x_hi).
@@ -121,23 +123,16 @@ __global__ void __launch_bounds__(256) pygw_chi0_presplit(const float2* __restri
 // S: fragment planes of this slab; M: the same slab as complex64 (J side);
 // wf: the slab's scaled weights coef*kcw*t_w in FP32, [k][w][npc*32] (zero
 // past N_p), from pygw_chi0_wprep.
-__global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S, const float2* __restrict__ M,
-                                                       long strideK, int np, int pb, int pbp, int npc, int nkc,
-                                                       const float* __restrict__ wf, int nom,
-                                                       const float* __restrict__ cs,
-                                                       const double* __restrict__ ws, int G, int T, int nwork,
-                                                       double2* __restrict__ eps) {
+// One (tile pair, frequency) work item: eps[.., om] += the 128 x 128 tile
+// (It, Jt) of the contraction over all chunks of S / M / wf.  As and Bs are
+// the block's LDS buffers; every thread of the 512-thread block calls this
+// with the same arguments, and it ends on a block barrier.
+__device__ __forceinline__ void chi0_tile(const h8* __restrict__ S, const float2* __restrict__ M, long strideK,
+                                          int np, int pb, int pbp, int npc, int nkc, const float* __restrict__ wf,
+                                          int nom, const float* __restrict__ cs, const double* __restrict__ ws,
+                                          int G, int om, int It, int Jt, double2* __restrict__ eps,
+                                          h8 (*As)[16][cm::TB], h8 (*Bs)[16][cm::TB]) {
   using namespace cm;
-  __shared__ h8 As[2][16][TB];  // [buf][g*4 + plane][I]
-  __shared__ h8 Bs[2][16][TB];  // [buf][g*4 + plane][J], weighted
-  // XCD-aware work map: blocks b, b+8, b+16, ... run on one XCD and take
-  // consecutive work items (frequency fastest).
-  const int per = gridDim.x >> 3;
-  const int work = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  if (work >= nwork) return;
-  const int om = work % nom;
-  int It, Jt;
-  tile_pair(work / nom, T, It, Jt);
   const int I0 = It * TB, J0 = Jt * TB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 2, wc = wave & 3;
@@ -161,12 +156,13 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
   // Loads go through buffer resources built from chunk-uniform bases (scalar
   // registers) with per-lane byte offsets that stay fixed across chunks, and
   // out-of-range offsets read as zero: J-side pairs p >= N_p fall past the
-  // end of the k-point's M block, columns J >= PB get an offset past it.
+  // end of the k-point's M block.
   constexpr int RSRC_FLAGS = 0x00020000;
   const unsigned aoffB = (unsigned)((tid >> 7) * pbp + (tid & 127)) * 16u;  // + q * 4*pbp*16
   const unsigned aqB = 4u * (unsigned)pbp * 16u;
   const unsigned pb8 = (unsigned)pb * 8u;
-  const unsigned moff0 = jin ? ((unsigned)(8 * gb) * pb8 + (unsigned)(J0 + Jb) * 8u) : 0xF0000000u;
+  // columns past PB read column PB-1 (finite) and are zeroed by sJ = 0
+  const unsigned moff0 = (unsigned)(8 * gb) * pb8 + (unsigned)(jin ? J0 + Jb : pb - 1) * 8u;
   const int npad = npc * KC;
   auto fetch = [&](Pre& P, int k, int c) {
     const h8* Sk = S + ((size_t)k * npc + c) * 16 * (size_t)pbp + I0;
@@ -337,6 +333,25 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
   }
 }
 
+__global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S, const float2* __restrict__ M,
+                                                       long strideK, int np, int pb, int pbp, int npc, int nkc,
+                                                       const float* __restrict__ wf, int nom,
+                                                       const float* __restrict__ cs,
+                                                       const double* __restrict__ ws, int G, int T, int nwork,
+                                                       double2* __restrict__ eps) {
+  using namespace cm;
+  __shared__ h8 As[2][16][TB];  // [buf][g*4 + plane][I]
+  __shared__ h8 Bs[2][16][TB];  // [buf][g*4 + plane][J], weighted
+  // XCD-aware work map: blocks b, b+8, b+16, ... run on one XCD and take
+  // consecutive work items (frequency fastest).
+  const int per = gridDim.x >> 3;
+  const int work = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (work >= nwork) return;
+  int It, Jt;
+  tile_pair(work / nom, T, It, Jt);
+  chi0_tile(S, M, strideK, np, pb, pbp, npc, nkc, wf, nom, cs, ws, G, work % nom, It, Jt, eps, As, Bs);
+}
+
 // Tiles above the diagonal from the ones below: eps[I,J] = conj(eps[J,I])
 // for I in tile It < Jt (the contraction writes the tiles It >= Jt).  One
 // thread per element.
@@ -473,6 +488,7 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
   const size_t head = head_bytes(pb, nom);
   const size_t per_k = slab_bytes_per_k(np, pb, nom);
   if (ws_bytes < head + per_k || G < 1) return PYGW_E_ARG;
+  if ((size_t)(np + KC) * pb * 8 >= ((size_t)1 << 31)) return PYGW_E_ARG;  // 32-bit buffer offsets per k
   size_t ks = (ws_bytes - head) / per_k;
   if (ks > (size_t)nkc) ks = (size_t)nkc;
   float* cs = (float*)ws;  // [pb] scales, then [pb] their inverses
@@ -518,3 +534,205 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
                      (double2*)eps, pb, nom);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }
+
+// ---------------------------------------------------------------------------
+// Batched contraction over many q-points in one persistent launch.  Each q
+// has its own PB and pair count (all pairs of all k concatenated: only the
+// (k, p) list matters to the sum), its own operands, scales and eps.  Work
+// item = (q, tile pair, frequency).  The (q, tile pair) groups are dealt to
+// the 8 XCDs round robin; within an XCD the resident blocks take the XCD's
+// items from an atomic counter, frequency fastest, so blocks that run
+// together on an XCD share a tile pair's operand slabs in its L2.
+// ---------------------------------------------------------------------------
+struct Chi0Q {
+  const h8* S;        // fragment planes [npc][16][pbp]
+  const float2* M;    // complex64 [K][PB]
+  const float* wf;    // FP32 scaled weights [nom][npc*32]
+  const float* cs;    // [2*PB] column scales and inverses
+  const double* ws;   // [nom] weight scales
+  double2* eps;       // Fortran [PB, PB, nom]
+  int pb, pbp, np, npc;
+};
+
+__global__ void __launch_bounds__(512) pygw_chi0_mixed_batch(const Chi0Q* __restrict__ qs,
+                                                             const int4* __restrict__ groups,
+                                                             const int* __restrict__ glen, int nom, int G,
+                                                             unsigned int* __restrict__ counters) {
+  using namespace cm;
+  __shared__ h8 As[2][16][TB];
+  __shared__ h8 Bs[2][16][TB];
+  __shared__ int item;
+  const int x = blockIdx.x & 7;
+  const int n = glen[x];
+  for (;;) {
+    if (threadIdx.x == 0) item = (int)atomicAdd(&counters[x], 1u);
+    __syncthreads();
+    const int i = item;
+    __syncthreads();  // all threads hold i before thread 0 takes the next one
+    if (i >= n) break;
+    const int4 gr = groups[x + 8 * (i / nom)];  // (q, It, Jt)
+    const Chi0Q d = qs[gr.x];
+    chi0_tile(d.S, d.M, 0, d.np, d.pb, d.pbp, d.npc, 1, d.wf, nom, d.cs, d.ws, G, i % nom, gr.y, gr.z, d.eps,
+              As, Bs);
+  }
+}
+
+__global__ void pygw_zero_u32_small(unsigned int* __restrict__ x, int n) {
+  if ((int)threadIdx.x < n) x[threadIdx.x] = 0u;
+}
+
+// Host-side plan of a batch (opaque to callers): per-q sizes and pointers,
+// device tables in the caller's workspace.
+struct Chi0BatchPlan {
+  int nq, nom;
+  double coef;
+  std::vector<Chi0Q> q;
+  std::vector<const double*> kcw;
+  std::vector<size_t> off_head;  // per q, into ws
+  Chi0Q* d_qs;
+  int4* d_groups;
+  int* d_glen;
+  unsigned int* d_counters;
+  int ngroups;
+  char* ws;
+};
+
+namespace {
+size_t batch_q_bytes(int pb, int np, int nom) {
+  const size_t npc = (size_t)(np + cm::KC - 1) / cm::KC;
+  const size_t pbp = (size_t)(pb + cm::TB - 1) / cm::TB * cm::TB;
+  return head_bytes(pb, nom) + npc * 16 * pbp * sizeof(h8) +
+         ((size_t)nom * npc * cm::KC * sizeof(float) + 255) / 256 * 256;
+}
+size_t batch_tables_bytes(int nq, const int* pb) {
+  size_t ng = 0;
+  for (int q = 0; q < nq; q++) {
+    const size_t T = (size_t)(pb[q] + cm::TB - 1) / cm::TB;
+    ng += T * (T + 1) / 2;
+  }
+  const size_t a = ((size_t)nq * sizeof(Chi0Q) + 255) / 256 * 256;
+  const size_t b = ((ng + 8) * sizeof(int4) + 255) / 256 * 256;
+  return a + b + 512;
+}
+}  // namespace
+
+size_t chi0_batch_ws_bytes(int nq, const int* pb, const int* np, int nom) {
+  size_t t = batch_tables_bytes(nq, pb);
+  for (int q = 0; q < nq; q++) t += batch_q_bytes(pb[q], np[q], nom);
+  return t;
+}
+
+// Builds the plan and uploads its tables (synchronously on `stream`).
+int chi0_batch_plan(int nq, const void* const* M, const int* pb, const int* np, const double* const* kcw, int nom,
+                    double coef, void* const* eps, void* ws, size_t ws_bytes, hipStream_t stream,
+                    Chi0BatchPlan** out) {
+  using namespace cm;
+  if (ws_bytes < chi0_batch_ws_bytes(nq, pb, np, nom)) return PYGW_E_ARG;
+  Chi0BatchPlan* P = new Chi0BatchPlan();
+  P->nq = nq;
+  P->nom = nom;
+  P->coef = coef;
+  P->ws = (char*)ws;
+  size_t off = batch_tables_bytes(nq, pb);
+  std::vector<int4> groups;
+  for (int q = 0; q < nq; q++) {
+    Chi0Q d;
+    d.pb = pb[q];
+    d.np = np[q];
+    d.npc = ceil_div(np[q], KC);
+    const int T = ceil_div(pb[q], TB);
+    d.pbp = T * TB;
+    char* base = (char*)ws + off;
+    const size_t head = head_bytes(pb[q], nom);
+    d.cs = (const float*)base;
+    d.ws = (const double*)(((uintptr_t)((const unsigned int*)(d.cs + 2 * pb[q]) + pb[q]) + 15) & ~(uintptr_t)15);
+    d.S = (const h8*)(base + head);
+    d.wf = (const float*)(base + head + (size_t)d.npc * 16 * d.pbp * sizeof(h8));
+    d.M = (const float2*)M[q];
+    d.eps = (double2*)eps[q];
+    P->q.push_back(d);
+    P->kcw.push_back(kcw[q]);
+    P->off_head.push_back(off);
+    off += batch_q_bytes(pb[q], np[q], nom);
+    for (int t = 0; t < T * (T + 1) / 2; t++) {
+      int It, Jt;
+      int tt = t, i = 0;
+      while (tt >= T - i) { tt -= T - i; i++; }
+      It = i;
+      Jt = i + tt;
+      groups.push_back(int4{q, It, Jt, 0});
+    }
+  }
+  P->ngroups = (int)groups.size();
+  const size_t a = ((size_t)nq * sizeof(Chi0Q) + 255) / 256 * 256;
+  const size_t b = (((size_t)P->ngroups + 8) * sizeof(int4) + 255) / 256 * 256;
+  P->d_qs = (Chi0Q*)ws;
+  P->d_groups = (int4*)((char*)ws + a);
+  P->d_glen = (int*)((char*)ws + a + b);
+  P->d_counters = (unsigned int*)((char*)ws + a + b + 256);
+  int glen[8];
+  for (int x = 0; x < 8; x++) {
+    const int ng = P->ngroups > x ? (P->ngroups - x + 7) / 8 : 0;
+    glen[x] = ng * nom;
+  }
+  hipError_t e = hipMemcpyAsync(P->d_qs, P->q.data(), sizeof(Chi0Q) * nq, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(P->d_groups, groups.data(), sizeof(int4) * groups.size(), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(P->d_glen, glen, sizeof(glen), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    delete P;
+    return PYGW_E_HIP;
+  }
+  *out = P;
+  return PYGW_OK;
+}
+
+// Launch-only (no allocation, copy or memset: capturable in a HIP graph).
+int chi0_batch_run(const Chi0BatchPlan* P, int G, hipStream_t stream) {
+  using namespace cm;
+  if (!P || G < 1) return PYGW_E_ARG;
+  const int nom = P->nom;
+  for (int q = 0; q < P->nq; q++) {
+    const Chi0Q& d = P->q[q];
+    float* cs = (float*)d.cs;
+    unsigned int* cmax = (unsigned int*)(cs + 2 * d.pb);
+    double* wsc = (double*)d.ws;
+    unsigned long long* wmax = (unsigned long long*)(wsc + nom);
+    const double* kc = P->kcw[q];
+    hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, cmax, (long)d.pb);
+    hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, (unsigned int*)wmax, 2L * nom);
+    hipLaunchKernelGGL(pygw_colmax, dim3(ceil_div(d.pb, 64), 1), dim3(256), 0, stream, d.M, 0L, d.np, d.pb, 1, cmax);
+    hipLaunchKernelGGL(pygw_colscale, dim3(ceil_div(d.pb, 256)), dim3(256), 0, stream, cmax, d.pb, cs);
+    hipLaunchKernelGGL(pygw_wmax, dim3(64, nom), dim3(256), 0, stream, kc, d.np, nom, 1, P->coef, wmax);
+    hipLaunchKernelGGL(pygw_wscale, dim3(ceil_div(nom, 256)), dim3(256), 0, stream, wmax, nom, wsc);
+    hipLaunchKernelGGL(pygw_zero_c128, dim3(2048), dim3(256), 0, stream, d.eps, (long)d.pb * d.pb * nom);
+    const long nel = (long)d.npc * 4 * d.pbp;
+    int blocks = (int)((nel + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(pygw_chi0_presplit, dim3(blocks), dim3(256), 0, stream, d.M, 0L, d.np, d.pb, d.pbp, d.npc, 1,
+                       (const float*)cs, (h8*)d.S);
+    const long nw_el = (long)nom * d.npc * KC;
+    int wblocks = (int)((nw_el + 255) / 256);
+    if (wblocks > 65536) wblocks = 65536;
+    hipLaunchKernelGGL(pygw_chi0_wprep, dim3(wblocks), dim3(256), 0, stream, kc, d.np, nom, 0, 1, d.npc * KC, P->coef,
+                       (const double*)wsc, (float*)d.wf);
+  }
+  hipLaunchKernelGGL(pygw_zero_u32_small, dim3(1), dim3(64), 0, stream, P->d_counters, 8);
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ncu = v;
+  }
+  const int grid = (ncu + 7) / 8 * 8;
+  hipLaunchKernelGGL(pygw_chi0_mixed_batch, dim3(grid), dim3(NT), 0, stream, (const Chi0Q*)P->d_qs,
+                     (const int4*)P->d_groups, (const int*)P->d_glen, nom, G, P->d_counters);
+  for (int q = 0; q < P->nq; q++) {
+    const Chi0Q& d = P->q[q];
+    hipLaunchKernelGGL(pygw_chi0_mirror, dim3(ceil_div((long)d.pb * d.pb, 256), nom), dim3(256), 0, stream, d.eps,
+                       d.pb, nom);
+  }
+  return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+}
+
+void chi0_batch_free(Chi0BatchPlan* P) { delete P; }

@@ -319,3 +319,75 @@ class QPlan:
         with torch.cuda.graph(g):
             self.run(M)
         return g
+
+
+class Chi0Batch:
+    """Split-precision chi0 for a batch of q-points with their own PB and
+    pair counts, in one persistent launch (the q loop of G0W0.Compute,
+    pygw.py:3654-3718, around the contraction of calc_eps,
+    pygw.py:3161-3200).
+
+    Ms[q]: complex64 [K_q, PB_q] device tensors (the window pairs of all k
+    of q concatenated); kcws[q]: float64 [n_om, K_q] device tensors (their
+    weights, layout of the kernel-native kcw with the pairs gathered).  The
+    eps of every q (complex128, Fortran [PB_q, PB_q, n_om]) and the
+    workspace are allocated once; run() only launches kernels and can be
+    captured in a HIP graph (capture())."""
+
+    def __init__(self, Ms, kcws, coef=-2.0, device="cuda"):
+        self.device = d = _require_device(device)
+        if len(Ms) != len(kcws) or not Ms:
+            raise ValueError("need one kcw per M")
+        self.Ms = [m.to(device=d, dtype=torch.complex64).contiguous() for m in Ms]
+        self.kcws = [w.to(device=d, dtype=torch.float64).contiguous() for w in kcws]
+        self.nq = len(Ms)
+        self.nom = self.kcws[0].shape[0]
+        self.pb = np.array([m.shape[1] for m in self.Ms], dtype=np.int32)
+        self.np = np.array([m.shape[0] for m in self.Ms], dtype=np.int32)
+        for m, w in zip(self.Ms, self.kcws):
+            if w.shape != (self.nom, m.shape[0]):
+                raise ValueError("kcw[q] must be [n_om, K_q]")
+        self.eps_buf = [torch.empty(int(p) * int(p) * self.nom, dtype=torch.complex128, device=d) for p in self.pb]
+        self.eps = [_fortran_view(b, int(p), self.nom) for b, p in zip(self.eps_buf, self.pb)]
+        sz = C.c_size_t(0)
+        _lib.call("pygw_chi0_batch_workspace", self.nq, C.c_void_p(self.pb.ctypes.data),
+                  C.c_void_p(self.np.ctypes.data), self.nom, C.byref(sz))
+        self.ws = torch.empty(sz.value, dtype=torch.uint8, device=d)
+        ptr = lambda ts: (C.c_void_p * len(ts))(*[t.data_ptr() for t in ts])  # noqa: E731
+        self._mp, self._kp, self._ep = ptr(self.Ms), ptr(self.kcws), ptr(self.eps_buf)
+        self.plan = C.c_void_p(0)
+        _lib.call("pygw_chi0_batch_plan", self.nq, C.cast(self._mp, C.c_void_p), C.c_void_p(self.pb.ctypes.data),
+                  C.c_void_p(self.np.ctypes.data), C.cast(self._kp, C.c_void_p), self.nom, float(coef),
+                  C.cast(self._ep, C.c_void_p), C.c_void_p(self.ws.data_ptr()), sz.value, _stream_ptr(d),
+                  C.byref(self.plan))
+
+    def run(self, schedule=0):
+        _lib.call("pygw_chi0_batch_run", self.plan, int(schedule), _stream_ptr(self.device))
+        return self.eps
+
+    def autotune(self):
+        best = C.c_int(0)
+        times = (C.c_float * 3)()
+        _lib.call("pygw_chi0_batch_autotune", self.plan, C.byref(best), C.cast(times, C.c_void_p),
+                  _stream_ptr(self.device))
+        return best.value, dict(zip((1, 2, 4), list(times)))
+
+    def capture(self):
+        """Capture run() in a HIP graph; returns the torch.cuda.CUDAGraph."""
+        g = torch.cuda.CUDAGraph()
+        self.run()
+        torch.cuda.synchronize(self.device)
+        with torch.cuda.graph(g):
+            self.run()
+        return g
+
+    def close(self):
+        if self.plan:
+            _lib.call("pygw_chi0_batch_free", self.plan)
+            self.plan = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -233,3 +233,82 @@ def device_matrix_elements(cfg: PolConfig, bands: Bands, kqid_col, k0=0, nkc=Non
               C.c_void_p(torch.cuda.current_stream().cuda_stream))
     torch.cuda.current_stream().synchronize()
     return out
+
+
+# ---------------------------------------------------------------------------
+# Batched uneven metal workload: many q with their own PB and ragged per-k
+# pair windows
+# ---------------------------------------------------------------------------
+
+@dataclass
+class QBatch:
+    cfg: PolConfig              # band model, mesh, frequencies (pb unused)
+    bands: Bands
+    q_index: np.ndarray         # [n_q] linear k-index of each q
+    pb: np.ndarray              # [n_q] product-basis size per q
+    windows: list               # per q: int32 [K_q, 2] (k, p) pairs, p = i_o*n_e + i_e
+
+    @property
+    def n_q(self):
+        return len(self.pb)
+
+    def q_vector(self, i):
+        d = np.array(self.cfg.div)
+        j = int(self.q_index[i])
+        return (int(j // (d[1] * d[2])), int((j // d[2]) % d[1]), int(j % d[2]))
+
+    def q_config(self, i):
+        return PolConfig(**{**self.cfg.__dict__, "q": self.q_vector(i), "pb": int(self.pb[i])})
+
+
+def metal_batch(cfg: PolConfig, n_q=32, pb_range=(300, 1100), window_range=(40, 260)) -> QBatch:
+    """Seeded batch: n_q distinct q != 0, PB_q ~ U{pb_range}, and for every
+    (q, k) a window of n_k ~ U{window_range} pairs -- the n_k pairs with the
+    smallest transition energy |e_m(k-q) - e_n(k)| (lowest-energy
+    transitions first)."""
+    bands = band_energies(cfg)
+    rng = np.random.default_rng(cfg.seed + 404)
+    nk = cfg.nk
+    qidx = rng.permutation(np.arange(1, nk))[:n_q]
+    pbs = rng.integers(pb_range[0], pb_range[1] + 1, size=n_q)
+    kqid = mesh.k_minus_q(cfg.div)
+    no, ne = bands.n_o, bands.n_e
+    c0 = bands.ncg + bands.numin
+    wins = []
+    for iq in range(n_q):
+        kq = kqid[:, qidx[iq]]
+        nwin = rng.integers(window_range[0], min(window_range[1], no * ne) + 1, size=nk)
+        rows = []
+        for k in range(nk):
+            d = np.abs(bands.enk[c0:, kq[k]][None, :] - bands.enk[:no, k][:, None]).reshape(-1)
+            sel = np.sort(np.argsort(d, kind="stable")[:nwin[k]])
+            rows.append(np.stack([np.full(sel.size, k), sel], axis=1))
+        wins.append(np.ascontiguousarray(np.concatenate(rows).astype(np.int32)))
+    return QBatch(cfg, bands, qidx.astype(np.int64), pbs.astype(np.int64), wins)
+
+
+def device_window_elements(batch: QBatch, i, complex128=False, device="cuda"):
+    """M rows of q i for its window pairs, [K_q, PB_q] on the device: the same
+    draws as matrix_elements / device_matrix_elements at (k, p)."""
+    import ctypes as C
+
+    import torch
+
+    from . import _lib
+
+    cfg = batch.q_config(i)
+    b = batch.bands
+    kq = mesh.k_minus_q(cfg.div)[:, batch.q_index[i]]
+    win = batch.windows[i]
+    dt = torch.complex128 if complex128 else torch.complex64
+    enk = torch.as_tensor(np.ascontiguousarray(b.enk.T), dtype=torch.float64, device=device)
+    kqd = torch.as_tensor(np.asarray(kq, dtype=np.int32), device=device)
+    vh = torch.as_tensor(vhalf(cfg), dtype=torch.float64, device=device)
+    pr = torch.as_tensor(win, device=device)
+    out = torch.empty((win.shape[0], cfg.pb), dtype=dt, device=device)
+    _lib.call("pygw_synth_minm_pairs", C.c_void_p(enk.data_ptr()), b.enk.shape[0], C.c_void_p(kqd.data_ptr()),
+              0, b.ncg + b.numin, b.n_o, b.n_e, cfg.pb, C.c_void_p(vh.data_ptr()), C.c_ulonglong(cfg.seed),
+              C.c_void_p(pr.data_ptr()), win.shape[0], 1 if complex128 else 0, C.c_void_p(out.data_ptr()),
+              C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    torch.cuda.current_stream().synchronize()
+    return out
