@@ -14,10 +14,12 @@ contractions; value = split-precision seconds per q when its chi0 is within
 1e-6 (relative Frobenius, every frequency) of FP64 on the same
 complex64-rounded M, else the FP64 seconds per q.
 
-    python bench.py [--steps K] [--warmup W] [--config c5|c2|c3] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--config c5|c2|c3|c4] [--no-cpu]
 
 --config c2 / c3: chi0 + eps^-1 only (FP64 and split precision) at
 8x8x8 k, 16+48 bands, PB = 600 / 10x10x10 k, 32+200 bands, PB = 1200.
+--config c4: a metal batch of 32 q (PB_q 300..1100, ragged per-k pair
+windows) through the one-launch batched chi0, then eps^-1 per q.
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs one GPU.
 """
 from __future__ import annotations
@@ -409,6 +411,198 @@ def run_chain(args):
     print(json.dumps(rec))
 
 
+def batch_cpu_baseline(batch, Ks, om, tetc, kqid, kslab=16, nw=2):
+    """One q of the batch (PB closest to the batch mean): PyGW's Fortran
+    weights (oracle/_ref, OpenMP) in full, the numpy calc_eps restatement on
+    the windows of `kslab` k-points and the inversion on `nw` frequencies;
+    extrapolated linearly in k / frequency, then to the batch mean by the
+    flop ratio sum_q PB_q^2 K_q / (n_q PB_i^2 K_i) (chi0) and PB^3 (inverse)."""
+    from oracle import chain, ref
+    from pygw_amd import synth
+
+    b, cfg = batch.bands, batch.cfg
+    pbs = np.asarray(batch.pb, dtype=float)
+    i = int(np.argmin(np.abs(pbs - pbs.mean())))
+    qc = batch.q_config(i)
+    kq = kqid[:, batch.q_index[i]]
+    nthr = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t = time.perf_counter()
+    if ref.available("libref_tetra_omp.so"):
+        kcw = ref.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc, omp=True)
+        kind = "reference"
+    else:
+        from oracle import tetra
+        kcw, _, _ = tetra.bz_calcqdepw(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc, nthreads=nthr)
+        kind = "port"
+    t_w = time.perf_counter() - t
+    win = batch.windows[i]
+    ks = list(range(kslab))
+    M = synth.matrix_elements(qc, b, kq, ks=ks)
+    ne = b.n_e
+    t = time.perf_counter()
+    rows = [win[win[:, 0] == k, 1] for k in ks]     # pair p = i_o*n_e + i_e
+    eps = chain.calc_eps_accumulate(((k, M[k][rows[k]]) for k in ks),
+                                    lambda k: kcw[rows[k] // ne, rows[k] % ne, k, :])
+    k_in = sum(r.size for r in rows)
+    t_chi = (time.perf_counter() - t) * Ks[i] / k_in
+    t = time.perf_counter()
+    chain.eps_inverse(eps[:, :, :nw])
+    t_inv = (time.perf_counter() - t) * cfg.n_om / nw
+    f_ratio = float(np.mean(pbs ** 2 * np.asarray(Ks))) / (pbs[i] ** 2 * Ks[i])
+    i_ratio = float(np.mean(pbs ** 3)) / pbs[i] ** 3
+    val = t_w + t_chi * f_ratio + t_inv * i_ratio
+    return {"value": val, "unit": "s/q", "cores": nthr, "kind": kind,
+            "sample": (f"q {i} of the batch (PB={int(pbs[i])}, K={Ks[i]} window pairs): weights in full "
+                       f"({'PyGW Fortran bz_calcqdepw_par2, OpenMP' if kind == 'reference' else 'C restatement'}); "
+                       f"calc_eps numpy ZGEMM loop on the windows of {kslab} of {cfg.nk} k ({k_in} pairs, "
+                       f"x{Ks[i] / k_in:.1f}); inverse on {nw} of {cfg.n_om} frequencies; scaled to the batch "
+                       f"mean by PB^2 K (x{f_ratio:.3f}) and PB^3 (x{i_ratio:.3f})"),
+            "stages_s": {"weights": t_w, "chi0_extrapolated": t_chi * f_ratio, "inverse_extrapolated": t_inv * i_ratio}}
+
+
+def run_batch(args):
+    """C4: n_q (default 32) q-points of a metal (3 bands crossing E_F) on
+    the 8x8x8 k-mesh, PB_q ~ U{300..1100}, per-(q, k) windows of 40-260
+    pairs, 48 frequencies.  One step = the whole batch: the weights of every
+    q gathered to its window pairs, then one persistent split-precision chi0
+    launch over all q and eps^-1 of every q (both captured in HIP graphs,
+    fold interval autotuned).  The FP64 path runs the same q one after
+    another (chi0_accumulate on the complex64-rounded M, then eps^-1).
+    value = mixed seconds per q (batch time / n_q) when every chi0 is within
+    1e-6 of FP64 (relative Frobenius, every q and frequency)."""
+    import torch
+
+    from pygw_amd import mesh, synth
+    from pygw_amd import polarization as P
+
+    cfg = synth.CONFIGS["c4"]
+    nq = args.nq or 32
+    batch = synth.metal_batch(cfg, n_q=nq)
+    b = batch.bands
+    om, _ = synth.omega_mesh(cfg)
+    tetc = mesh.tetra_corners(cfg.div)
+    kqid = mesh.k_minus_q(cfg.div)
+    nom, npair = cfg.n_om, b.n_o * b.n_e
+    Ks = [int(w.shape[0]) for w in batch.windows]
+    pbs = [int(p) for p in batch.pb]
+    kqs = [kqid[:, batch.q_index[i]] for i in range(nq)]
+    dev = "cuda"
+    ar = torch.arange(nom, dtype=torch.int64, device=dev)[:, None]
+    gidx = []  # flat index of (k, w, p) in the native weights [nk, nom, n_o*n_e], shaped [nom, K_q]
+    for w in batch.windows:
+        k = torch.as_tensor(w[:, 0], dtype=torch.int64, device=dev)[None, :]
+        p = torch.as_tensor(w[:, 1], dtype=torch.int64, device=dev)[None, :]
+        gidx.append(((k * nom + ar) * npair + p).reshape(-1))
+    kcws = [torch.empty((nom, K), dtype=torch.float64, device=dev) for K in Ks]
+    Ms = [synth.device_window_elements(batch, i) for i in range(nq)]
+    print(f"[bench] c4 batch: {nq} q, PB {min(pbs)}..{max(pbs)}, K mean {np.mean(Ks):.0f}", file=sys.stderr,
+          flush=True)
+
+    def weights(i):
+        kn = P.bz_calcqdepw_par2(b.enk, 0.0, kqs[i], b.ncg, b.nomx, b.numin, om, tetc, layout="native",
+                                 check=False)
+        torch.index_select(kn.view(-1), 0, gidx[i], out=kcws[i].view(-1))
+
+    for i in range(nq):
+        weights(i)
+    B = P.Chi0Batch(Ms, kcws, inverse=True)
+    best, tune = B.autotune()
+    g_chi = B.capture(best, inverse=False)
+    g_inv = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g_inv):
+        B.invert()
+    torch.cuda.synchronize()
+    print(f"[bench] autotune best={best} {tune}", file=sys.stderr, flush=True)
+    s = torch.cuda.current_stream()
+    times = []
+    for it in range(args.warmup + args.steps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        torch.cuda.synchronize()
+        ev[0].record(s)
+        for i in range(nq):
+            weights(i)
+        ev[1].record(s)
+        g_chi.replay()
+        ev[2].record(s)
+        g_inv.replay()
+        ev[3].record(s)
+        torch.cuda.synchronize()
+        st = [ev[j].elapsed_time(ev[j + 1]) / 1e3 for j in range(3)]
+        print(f"[bench] mixed batch step {it}: {sum(st):.3f} s{' (warm-up)' if it < args.warmup else ''}",
+              file=sys.stderr, flush=True)
+        if it >= args.warmup:
+            times.append(st)
+    tm = np.array(times).mean(0)
+    # FP64: the same q one after another on the complex64-rounded M
+    t64 = np.zeros(3)
+    for i in range(nq):
+        M128 = Ms[i][None].to(torch.complex128)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        torch.cuda.synchronize()
+        ev[0].record(s)
+        weights(i)
+        ev[1].record(s)
+        e64 = P.chi0_accumulate(M128, kcws[i][None])
+        ev[2].record(s)
+        P.eps_inverse(e64, check=False)
+        ev[3].record(s)
+        torch.cuda.synchronize()
+        t64 += [ev[j].elapsed_time(ev[j + 1]) / 1e3 for j in range(3)]
+        del M128, e64
+    print(f"[bench] fp64 per q {t64.sum() / nq:.4f} s", file=sys.stderr, flush=True)
+    # precision: chi0 and eps^-1 of every q against FP64 on the same inputs
+    g_chi.replay()
+    torch.cuda.synchronize()
+    r_chi, r_inv = [], []
+    one = None
+    for i in range(nq):
+        ref64 = P.chi0_accumulate(Ms[i][None].to(torch.complex128), kcws[i][None])
+        mx = B.eps[i].clone()
+        d = torch.linalg.matrix_norm((mx - ref64).permute(2, 0, 1))
+        r_chi.append((d / torch.linalg.matrix_norm(ref64.permute(2, 0, 1))).cpu().numpy())
+        P.eps_inverse(ref64, check=False)
+        P.eps_inverse(mx, check=False)
+        one = torch.eye(pbs[i], dtype=torch.complex128, device=dev)[:, :, None]
+        d = torch.linalg.matrix_norm((mx - ref64).permute(2, 0, 1))
+        r_inv.append((d / torch.linalg.matrix_norm((ref64 + one).permute(2, 0, 1))).cpu().numpy())
+        del ref64, mx, one
+    r_chi, r_inv = np.concatenate(r_chi), np.concatenate(r_inv)
+    ok = bool(r_chi.max() <= 1e-6)
+    f_q = [4.0 * p * p * K * nom for p, K in zip(pbs, Ks)]
+    f_tot = float(sum(f_q))
+    s_mx, s_64 = float(tm.sum()) / nq, float(t64.sum()) / nq
+    kern = "pygw_chi0_mixed_batch"
+    rec = {
+        "metric": "chi0+eps^-1 seconds per q-point, mixed vs FP64 GPU at <=1e-6 rel err; % roofline",
+        "value": s_mx if ok else s_64, "unit": "s/q", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": (tm.sum() if ok else t64.sum()) * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16x3-split/fp32-acc chi0, f64 weights/inverse" if ok else "f64", "data": "synthetic",
+        "config": {"workload": f"c4: metal ({b.n_o}+{b.n_e} bands, 3 crossing E_F), 8x8x8 k-mesh, {nq} q in one "
+                               f"batch, PB_q {min(pbs)}..{max(pbs)} (U{{300..1100}}), per-k windows of 40-260 "
+                               f"pairs (K_q mean {np.mean(Ks):.0f}), {nom} frequencies; one step = the batch",
+                   "precision": "mixed" if ok else "fp64", "fold_schedule": best, "autotune_s": tune},
+        "stages_s_per_q": {"polarization_weights": tm[0] / nq, "chi0": tm[1] / nq, "eps_inverse": tm[2] / nq},
+        "roofline": {"bound": "mfma", "kernel": kern, "achieved": f_tot / tm[1] / 1e12, "peak": PEAK_FP16 / 1e12,
+                     "unit": "TFLOP/s", "frac": f_tot / tm[1] / PEAK_FP16, "traffic": load_traffic("c4", kern),
+                     "flops_per_launch": f_tot, "executed_flops_per_launch": 3 * f_tot,
+                     "timed": "HIP events around the captured batch (prep + persistent contraction + mirror)"},
+        "fp64": {"s_per_q": s_64, "stages_s_per_q": dict(zip(("polarization_weights", "chi0", "eps_inverse"),
+                                                              map(float, t64 / nq))),
+                 "chi0_tflops": f_tot / t64[1] / 1e12, "chi0_frac_fp64_peak": f_tot / t64[1] / PEAK_FP64},
+        "mixed_speedup_vs_fp64": s_64 / s_mx,
+        "rel_frobenius_error_mixed_vs_fp64": {"chi0_max": float(r_chi.max()), "chi0_median": float(np.median(r_chi)),
+                                              "eps_inv_max": float(r_inv.max()),
+                                              "eps_inv_median": float(np.median(r_inv))},
+    }
+    B.close()
+    if not args.no_cpu:
+        rec["cpu_baseline"] = batch_cpu_baseline(batch, Ks, om, tetc, kqid)
+        print("[bench] cpu baseline done", file=sys.stderr, flush=True)
+    print(json.dumps(rec))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -421,7 +615,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.steps is None:
-        args.steps = {"c5": 2, "c2": 8}.get(args.config, 1)
+        args.steps = {"c5": 2, "c2": 8, "c4": 3}.get(args.config, 1)
     if args.copy_calibration:
         # one copy kernel that reads and writes exactly 1 GiB: the ratio of its
         # FETCH_SIZE / WRITE_SIZE counters to 1 GiB scales the kernels' counters
@@ -433,6 +627,9 @@ def main():
         del src, dst
     if args.config == "c5":
         run_chain(args)
+        return
+    if args.config == "c4":
+        run_batch(args)
         return
     mixed = not args.no_mixed
     wl = Workload(args.config, args.nq, mixed, args.fp64_kfrac)

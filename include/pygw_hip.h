@@ -50,7 +50,12 @@ int pygw_version(void);
  *   info   (optional) [2]: status bits (1 geometry error, 2 nnod=5
  *          bipyramid, 4 malformed tetc), number of check_weight warnings.
  * ------------------------------------------------------------------------ */
-int pygw_tetra_weights_workspace(int n_k, size_t* bytes);
+/* Workspace of pygw_tetra_weights: the k incidence lists plus a buffer of
+ * corner weights (n_tet*4*n_om doubles per band pair) for a chunk of pairs;
+ * the returned size holds up to 2 GiB of it.  A smaller workspace (at least
+ * one chunk of 64 pairs, or of all n_pairs = n_o*n_e when fewer) runs more
+ * chunks; NULL workspace: the call allocates the returned size itself. */
+int pygw_tetra_weights_workspace(int n_k, int n_pairs, int n_om, size_t* bytes);
 
 /* SVD-basis weights: replaces ft.bz_calcqdepw_par3 (for_tetrahedra.f90:466-542,
  * called at pygw.py:3441) as kcw_l = kcw . dUl applied to the layout-1

@@ -24,7 +24,7 @@ class PygwError(RuntimeError):
 
 _SIGS = {
     "pygw_version": (C.c_int, []),
-    "pygw_tetra_weights_workspace": (C.c_int, [C.c_int, C.POINTER(C.c_size_t)]),
+    "pygw_tetra_weights_workspace": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
     "pygw_tetra_weights": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                      C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                      C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),

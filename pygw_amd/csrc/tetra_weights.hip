@@ -5,19 +5,27 @@
 //                  6/N_tet * w_corner(t,k; e_n(t), e_m(t linked by q), w)
 // with the corner weights of convw1t_par (for_tetrahedra3.f90:3-136).
 //
-// Design (gather, no atomics, deterministic):
-//   * one lane per (k, band pair); a wavefront holds 64 pairs of the same k,
-//     so every lane walks the same list of the 24 (tetrahedron, corner)
-//     incidences of k, in increasing tetrahedron order -- the order in which
-//     the reference scatters into kcw(:,:,k,:), so the FP64 sums associate
-//     identically;
-//   * Matsubara frequencies are innermost, OMC at a time, accumulated in
-//     registers; each lane writes its pair's row once, coalesced across the
-//     wavefront (pair index fastest in both output layouts);
-//   * for one incidence only the weight of corner k is needed, so only the
-//     small-tetrahedron vertices with a non-zero barycentric coordinate at
-//     that corner are evaluated (one vertex for the insulating full
-//     tetrahedron instead of four).
+// Design (deterministic, no floating-point atomics):
+//   * pass 0: the atan / log of Delta = e_f(k-q) - e_e(k) for every
+//     (k-point, frequency, pair) of the chunk, once -- a whole tetrahedron's
+//     vertex energies are these per-k values, so each is shared by the 24
+//     tetrahedra around k and by their four corners;
+//   * pass 1, one lane per (tetrahedron, band pair), a wavefront holding 64
+//     pairs of one tetrahedron: the four corner weights over all Matsubara
+//     frequencies (the reference's w1t(4, nom) of one convw1t_par call).  For
+//     the whole tetrahedron (no E_F plane inside) each corner weight is one
+//     vertex's closed form on the tabulated atan / log, used only where the
+//     vertex energy equals the table's bit for bit; polyhedra cut by E_F,
+//     degenerate classes and table misses are compacted into a list and
+//     evaluated by a separate pass, one lane per (pair, corner), on the
+//     general path, so they do not stall the wavefronts of whole tetrahedra;
+//   * pass 2, one lane per (k, band pair): the 24 (tetrahedron, corner)
+//     incidences of k summed in increasing tetrahedron order -- the order in
+//     which the reference scatters into kcw(:,:,k,:), so the FP64 sums
+//     associate identically; writes coalesced across the wavefront (pair
+//     index fastest in both output layouts);
+//   * the pass-1 buffer holds ntet*4*nom doubles per pair; pairs are taken in
+//     chunks sized to the workspace.
 // Arithmetic follows the reference expression by expression (no FMA
 // contraction in this file); atan/log come from the device math library, so
 // results agree with the CPU reference to the conditioning of the closed
@@ -426,40 +434,46 @@ struct VCoef {
   int na, nb;        // two-node form: slots of ev used
 };
 
+#define E(i) ev[(i) - 1]
+#define V(i, j) (ev[(i) - 1] - ev[(j) - 1])
+// class 4 (four distinct energies): coefficients of the vertex whose
+// rotated energies are ev
+__device__ __forceinline__ void coef_cls4(const double* ev, double* t, double* w, double& cc) {
+  double ev3t4 = E(3) * E(4);
+  double ev1_2 = E(1) * E(1);
+  double e22 = 3.0 * sq(E(2)), e32 = 3.0 * sq(E(3)), e42 = 3.0 * sq(E(4));
+  double vdf24 = V(2, 3) * V(2, 4) * V(3, 4);
+  w[0] = 2.0 * V(1, 2) * (V(1, 3) * V(1, 4)) * vdf24;
+  t[0] = -ev1_2 * w[0];
+  w[1] = vdf24 * 2.0 * ((2.0 * E(1) - E(2)) * (E(3) + E(4)) + E(1) * (2.0 * E(2) - 3.0 * E(1)) - E(3) * E(4));
+  t[1] = vdf24 * 6.0 * E(1) * (2.0 * E(2) * ev3t4 + E(1) * (ev1_2 - ev3t4 - E(2) * (E(3) + E(4))));
+  w[2] = 2.0 * sq(V(1, 3) * V(1, 4)) * V(3, 4);
+  t[2] = -e22 * w[2];
+  w[3] = -2.0 * sq(V(1, 2) * V(1, 4)) * V(2, 4);
+  t[3] = -e32 * w[3];
+  w[4] = 2.0 * sq(V(1, 2) * V(1, 3)) * V(2, 3);
+  t[4] = -e42 * w[4];
+  t[5] = vdf24 * ev1_2 * (3.0 * E(2) * ev3t4 + E(1) * (E(2) * (E(1) - 2.0 * E(3)) + E(4) * (E(1) - 2.0 * E(2)) + E(3) * (E(1) - 2.0 * E(4))));
+  w[5] = -3.0 * vdf24 * (E(2) * E(3) * E(4) + ev1_2 * (2.0 * E(1) - E(2) - E(3) - E(4)));
+  double d2 = E(2) * sq(V(1, 3) * V(1, 4)) * V(3, 4) / 3.0;
+  double d3 = E(3) * sq(V(1, 2) * V(1, 4)) * V(2, 4) / 3.0;
+  double d4 = E(4) * sq(V(1, 2) * V(1, 3)) * V(2, 3) / 3.0;
+  t[6] = -e22 * d2;
+  w[6] = 9.0 * d2;
+  w[7] = -9.0 * d3;
+  t[7] = e32 * d3;
+  w[8] = 9.0 * d4;
+  t[8] = -e42 * d4;
+  cc = 6.0 * sq(V(1, 2) * V(1, 3) * V(1, 4)) * V(2, 3) * V(2, 4) * V(3, 4);
+}
+
 // sorted-vertex v (0..3) of class cls with sorted Delta d
 __device__ void vertex_coef(VCoef& c, int cls, const double* d, int v, int compat) {
   double ev[4];
-#define E(i) ev[(i) - 1]
-#define V(i, j) (ev[(i) - 1] - ev[(j) - 1])
   if (cls == 4) {
     if (compat & COMPAT_F7B) { for (int j = 0; j < 4; j++) ev[(j + v) & 3] = d[j]; }
     else { for (int j = 0; j < 4; j++) ev[j] = d[(j + v) & 3]; }
-    double ev3t4 = E(3) * E(4);
-    double ev1_2 = E(1) * E(1);
-    double e22 = 3.0 * sq(E(2)), e32 = 3.0 * sq(E(3)), e42 = 3.0 * sq(E(4));
-    double vdf24 = V(2, 3) * V(2, 4) * V(3, 4);
-    c.w[0] = 2.0 * V(1, 2) * (V(1, 3) * V(1, 4)) * vdf24;
-    c.t[0] = -ev1_2 * c.w[0];
-    c.w[1] = vdf24 * 2.0 * ((2.0 * E(1) - E(2)) * (E(3) + E(4)) + E(1) * (2.0 * E(2) - 3.0 * E(1)) - E(3) * E(4));
-    c.t[1] = vdf24 * 6.0 * E(1) * (2.0 * E(2) * ev3t4 + E(1) * (ev1_2 - ev3t4 - E(2) * (E(3) + E(4))));
-    c.w[2] = 2.0 * sq(V(1, 3) * V(1, 4)) * V(3, 4);
-    c.t[2] = -e22 * c.w[2];
-    c.w[3] = -2.0 * sq(V(1, 2) * V(1, 4)) * V(2, 4);
-    c.t[3] = -e32 * c.w[3];
-    c.w[4] = 2.0 * sq(V(1, 2) * V(1, 3)) * V(2, 3);
-    c.t[4] = -e42 * c.w[4];
-    c.t[5] = vdf24 * ev1_2 * (3.0 * E(2) * ev3t4 + E(1) * (E(2) * (E(1) - 2.0 * E(3)) + E(4) * (E(1) - 2.0 * E(2)) + E(3) * (E(1) - 2.0 * E(4))));
-    c.w[5] = -3.0 * vdf24 * (E(2) * E(3) * E(4) + ev1_2 * (2.0 * E(1) - E(2) - E(3) - E(4)));
-    double d2 = E(2) * sq(V(1, 3) * V(1, 4)) * V(3, 4) / 3.0;
-    double d3 = E(3) * sq(V(1, 2) * V(1, 4)) * V(2, 4) / 3.0;
-    double d4 = E(4) * sq(V(1, 2) * V(1, 3)) * V(2, 3) / 3.0;
-    c.t[6] = -e22 * d2;
-    c.w[6] = 9.0 * d2;
-    c.w[7] = -9.0 * d3;
-    c.t[7] = e32 * d3;
-    c.w[8] = 9.0 * d4;
-    c.t[8] = -e42 * d4;
-    c.cc = 6.0 * sq(V(1, 2) * V(1, 3) * V(1, 4)) * V(2, 3) * V(2, 4) * V(3, 4);
+    coef_cls4(ev, c.t, c.w, c.cc);
     c.kind = 4;
   } else if (cls == 6) {
     if (v < 2) {
@@ -561,42 +575,69 @@ __device__ __forceinline__ void tan_log(double x, double om, double& t, double& 
   else l = 0.0;
 }
 
+// class-4 closed form at one frequency from the atan / log of the vertex
+// energies (T[i], L[i] of ev[i])
+__device__ __forceinline__ double eval_cls4(const double* t, const double* w, double cc, const double* T,
+                                            const double* L, double om) {
+  const double om2 = om * om;
+  double aa = t[0] + om2 * w[0] + om * (T[0] * t[1] + T[1] * t[2] + T[2] * t[3] + T[3] * t[4]
+             + om2 * (T[0] * w[1] + T[1] * w[2] + T[2] * w[3] + T[3] * w[4]));
+  double bb = L[0] * t[5] + L[1] * t[6] + L[2] * t[7] + L[3] * t[8]
+             + om2 * (L[0] * w[5] + L[1] * w[6] + L[2] * w[7] + L[3] * w[8]);
+  return (aa + bb) / cc;
+}
+
+// check_weight (for_tetrahedra3.f90:694-710): |w| > 1e4 is zeroed at omega ~ 0, else counted
+__device__ __forceinline__ double check_weight(double w, double om, int* nwarn) {
+  if (fabs(w) > 1.0e4) {
+    if (fabs(om) < F32C(1.e-6)) w = 0.0;
+    else (*nwarn)++;
+  }
+  return w;
+}
+
+// class-6 closed form from T/L of ev[0], ev[2], ev[3]
+__device__ __forceinline__ double eval_cls6(const VCoef& c, double om, double T0, double L0, double T2, double L2,
+                                            double T3, double L3) {
+  const double om2 = om * om;
+  double aa = c.t[0] + om2 * c.w[0] + om * (T0 * c.t[1] + T2 * c.t[2] + T3 * c.t[3]
+             + om2 * (T0 * c.w[1] + T2 * c.w[2] + T3 * c.w[3]));
+  double bb = L0 * c.t[4] + L2 * c.t[5] + L3 * c.t[6] + om2 * (L0 * c.w[4] + L2 * c.w[5] + L3 * c.w[6]);
+  return (aa + bb) / c.cc;
+}
+
+// two-node closed form (classes 8, 10) from T/L of ev[na], ev[nb]
+__device__ __forceinline__ double eval_kind2(const VCoef& c, double om, double Ta, double La, double Tb, double Lb) {
+  const double om2 = om * om;
+  double aa = c.t[0] + c.w[0] * om2 + om * (Ta - Tb) * (c.sgn6 * om2 + c.t[1]);
+  double bb = (La - Lb) * (c.t[2] + c.w[2] * om2);
+  return (aa + bb) / c.cc;
+}
+
+// class 16 (all four equal to a)
+__device__ __forceinline__ double eval_cls16(double a, double om) { return -a / (12.0 * (om * om + a * a)); }
+
 __device__ __forceinline__ double vertex_eval(const VCoef& c, double om, int* nwarn) {
   double w;
-  double om2 = om * om;
   if (c.kind == 4) {
     double T[4], L[4];
     for (int i = 0; i < 4; i++) tan_log(c.ev[i], om, T[i], L[i]);
-    double aa = c.t[0] + om2 * c.w[0] + om * (T[0] * c.t[1] + T[1] * c.t[2] + T[2] * c.t[3] + T[3] * c.t[4]
-               + om2 * (T[0] * c.w[1] + T[1] * c.w[2] + T[2] * c.w[3] + T[3] * c.w[4]));
-    double bb = L[0] * c.t[5] + L[1] * c.t[6] + L[2] * c.t[7] + L[3] * c.t[8]
-               + om2 * (L[0] * c.w[5] + L[1] * c.w[6] + L[2] * c.w[7] + L[3] * c.w[8]);
-    w = (aa + bb) / c.cc;
+    w = eval_cls4(c.t, c.w, c.cc, T, L, om);
   } else if (c.kind == 6) {
     double T0, L0, T2, L2, T3, L3;
     tan_log(c.ev[0], om, T0, L0);
     tan_log(c.ev[2], om, T2, L2);
     tan_log(c.ev[3], om, T3, L3);
-    double aa = c.t[0] + om2 * c.w[0] + om * (T0 * c.t[1] + T2 * c.t[2] + T3 * c.t[3]
-               + om2 * (T0 * c.w[1] + T2 * c.w[2] + T3 * c.w[3]));
-    double bb = L0 * c.t[4] + L2 * c.t[5] + L3 * c.t[6] + om2 * (L0 * c.w[4] + L2 * c.w[5] + L3 * c.w[6]);
-    w = (aa + bb) / c.cc;
+    w = eval_cls6(c, om, T0, L0, T2, L2, T3, L3);
   } else if (c.kind == 2) {
     double Ta, La, Tb, Lb;
     tan_log(c.ev[c.na], om, Ta, La);
     tan_log(c.ev[c.nb], om, Tb, Lb);
-    double aa = c.t[0] + c.w[0] * om2 + om * (Ta - Tb) * (c.sgn6 * om2 + c.t[1]);
-    double bb = (La - Lb) * (c.t[2] + c.w[2] * om2);
-    w = (aa + bb) / c.cc;
+    w = eval_kind2(c, om, Ta, La, Tb, Lb);
   } else {
-    double a = c.ev[0];
-    w = -a / (12.0 * (om2 + a * a));
+    w = eval_cls16(c.ev[0], om);
   }
-  if (fabs(w) > 1.0e4) {  // check_weight
-    if (fabs(om) < F32C(1.e-6)) w = 0.0;
-    else (*nwarn)++;
-  }
-  return w;
+  return check_weight(w, om, nwarn);
 }
 
 // stweight_itaylor_par for original vertex j (for_tetrahedra3.f90:407-437)
@@ -650,6 +691,17 @@ __device__ bool sort_eq(double* v, int* ind, int* cls) {
   return true;
 }
 
+// Delta = e_f - e_e at the small-tetrahedron vertices cr (linear in the
+// big-tetrahedron corner values db); returns the Taylor threshold 10 max|Delta|
+__device__ __forceinline__ double vertex_delta(const double cr[4][3], const double* db, double* ds) {
+  double mx = 0.0;
+  for (int i = 0; i < 4; i++) {
+    ds[i] = (db[1] - db[0]) * cr[i][0] + (db[2] - db[0]) * cr[i][1] + (db[3] - db[0]) * cr[i][2] + db[0];
+    if (fabs(ds[i]) > mx) mx = fabs(ds[i]);
+  }
+  return 10.0 * mx;
+}
+
 // One small tetrahedron (generictetra_par, for_tetrahedra3.f90:220-405):
 // adds vol * sum_j w_j beta_j for big-tetrahedron corner `corner` to
 // out[w] for frequencies j = c0+w in [jlo, jhi), evaluated at omega[j-shift].
@@ -666,13 +718,8 @@ __device__ void small_tet(const double cr[4][3], const double* db, int corner, c
   vol = fabs(vol);
   if (vol < F32C(1.e-10)) return;
   double ds[4], dsort[4];
-  double mx = 0.0;
-  for (int i = 0; i < 4; i++) {
-    ds[i] = (db[1] - db[0]) * cr[i][0] + (db[2] - db[0]) * cr[i][1] + (db[3] - db[0]) * cr[i][2] + db[0];
-    if (fabs(ds[i]) > mx) mx = fabs(ds[i]);
-    dsort[i] = ds[i];
-  }
-  double thr = 10.0 * mx;
+  const double thr = vertex_delta(cr, db, ds);
+  for (int i = 0; i < 4; i++) dsort[i] = ds[i];
   int ind[4], cls;
   if (!sort_eq(dsort, ind, &cls)) { *status |= ST_GEOM; return; }
   double tl[OMC];
@@ -772,20 +819,79 @@ __device__ __noinline__ void corner_weight(const double* e, const double* f, dou
 // surfnodes..relnodes keep exactly the four corners (0,0,0), (0,0,1),
 // (0,1,0), (1,0,0) -- no plane crosses the tetrahedron, and a margin of
 // 1e-9 is far beyond the rounding of the intersection quotients -- so the
-// polyhedron is the whole tetrahedron (nnod = 4) and only small_tet runs.
+// polyhedron is the whole tetrahedron (nnod = 4): corner_weight reduces to
+// small_tet on the unit corners, which full_tet_sym and pygw_tw_whole
+// evaluate directly.
 __device__ __forceinline__ bool insulating(const double* e, const double* f, double ef) {
   const double m = 1.0e-9;
   return fmax(fmax(e[0], e[1]), fmax(e[2], e[3])) < ef - m && fmin(fmin(f[0], f[1]), fmin(f[2], f[3])) > ef + m;
 }
 
-__device__ __forceinline__ void corner_weight_full(const double* e, const double* f, int corner,
-                                                   const double* __restrict__ omega, int nom, int c0, int compat,
-                                                   double* res, int* nwarn, int* status) {
-  const double cr[4][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}, {0.0, 1.0, 0.0}, {1.0, 0.0, 0.0}};
-  double db[4];
-  for (int i = 0; i < 4; i++) db[i] = f[i] - e[i];
-  for (int w = 0; w < OMC; w++) res[w] = 0.0;
-  small_tet(cr, db, corner, omega, c0, 0, nom, 0, compat, res, nwarn, status);
+// Whole tetrahedron of class 16, 8 or 10 (all four, two pairs of, or three
+// of the corner Delta's equal within sorteq's tolerance), all four corners at
+// once.  The small tetrahedron is the big one (volume 1, barycentric weight 1
+// at exactly one vertex per corner), so corner c's weight at a frequency is
+// the closed form of its vertex alone.  In these classes the vertices share
+// their energies (averages of the sorted Delta's) and fall into at most two
+// coefficient sets -- sorted vertices below nA use set a, the others set b --
+// so one frequency costs at most two atan / log evaluations for all four
+// corners.  out + c*sc + w*sw receives corner c, frequency w.
+__device__ __forceinline__ void full_tet_sym(int cls, const double* ds, const double* dsort, const int* ind,
+                                             double thr, const double* __restrict__ omega, int nom, int compat,
+                                             double* out, size_t sc, size_t sw, int* nwarn) {
+  VCoef ca, cb;
+  int nA;
+  if (cls == 8) {
+    vertex_coef(ca, 8, dsort, 0, compat);
+    vertex_coef(cb, 8, dsort, 2, compat);
+    nA = 2;
+  } else if (cls == 10) {
+    vertex_coef(ca, 10, dsort, 0, compat);
+    vertex_coef(cb, 10, dsort, 3, compat);
+    nA = 3;
+  } else {
+    vertex_coef(ca, 16, dsort, 0, compat);
+    cb = ca;
+    nA = 4;
+  }
+  // the two node energies of the two-node form (na = 0; nb = 2 for class 8, 3 for class 10)
+  const double xa = ca.ev[0];
+  const double xb = cls == 8 ? ca.ev[2] : ca.ev[3];
+  double s1j[4], n03j[4], s1[4], n03[4];
+  size_t oc[4];
+  bool inA[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) taylor_coef(ds, j, s1j[j], n03j[j]);
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int j = ind[v];
+    s1[v] = j == 0 ? s1j[0] : j == 1 ? s1j[1] : j == 2 ? s1j[2] : s1j[3];
+    n03[v] = j == 0 ? n03j[0] : j == 1 ? n03j[1] : j == 2 ? n03j[2] : n03j[3];
+    oc[v] = (size_t)(j == 0 ? 0 : 4 - j) * sc;   // corner of original vertex j
+    inA[v] = v < nA;
+  }
+#pragma unroll 1
+  for (int iw = 0; iw < nom; iw++) {
+    const double om = omega[iw];
+    double* o = out + (size_t)iw * sw;
+    if (om > thr) {
+      const double o2 = om * om;
+#pragma unroll
+      for (int v = 0; v < 4; v++) o[oc[v]] = s1[v] / (6.0e1 * o2) + n03[v] / (4.2e2 * (o2 * o2));
+    } else if (cls == 16) {
+      const double r = eval_cls16(xa, om);
+#pragma unroll
+      for (int v = 0; v < 4; v++) o[oc[v]] = check_weight(r, om, nwarn);
+    } else {
+      double Ta, La, Tb, Lb;
+      tan_log(xa, om, Ta, La);
+      tan_log(xb, om, Tb, Lb);
+      const double ra = eval_kind2(ca, om, Ta, La, Tb, Lb);
+      const double rb = eval_kind2(cb, om, Ta, La, Tb, Lb);
+#pragma unroll
+      for (int v = 0; v < 4; v++) o[oc[v]] = check_weight(inA[v] ? ra : rb, om, nwarn);
+    }
+  }
 }
 
 }  // namespace tw
@@ -795,78 +901,316 @@ __device__ __forceinline__ void corner_weight_full(const double* e, const double
 // ---------------------------------------------------------------------------
 
 // incidence list: inc[k*24 + r] = 4*tet + corner, increasing in (tet, corner);
-// cnt[k] = number found (24 on any mesh built by KQmesh.tetra).
-__global__ void __launch_bounds__(256) pygw_tw_incidence(const int* __restrict__ tetc, int ntet, int nk,
-                                                         int* __restrict__ inc, int* __restrict__ cnt, int maxinc) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+// cnt[k] = number found (24 on any mesh built by KQmesh.tetra).  One
+// wavefront per k scans 64 tetrahedra at a time; a ballot keeps the order.
+__global__ void __launch_bounds__(64) pygw_tw_incidence(const int* __restrict__ tetc, int ntet, int nk,
+                                                        int* __restrict__ inc, int* __restrict__ cnt, int maxinc) {
+  const int k = blockIdx.x;
+  const int lane = threadIdx.x;
   if (k >= nk) return;
   int c = 0;
   bool bad = false;
-  for (int t = 0; t < ntet; t++)
-    for (int i = 0; i < 4; i++) {
-      const int v = tetc[4 * t + i];
-      bad |= (v < 0 || v >= nk);
-      if (v == k) {
-        if (c < maxinc) inc[k * maxinc + c] = 4 * t + i;
-        c++;
+  for (int t0 = 0; t0 < ntet; t0 += 64) {
+    const int t = t0 + lane;
+    int hit = 0;  // corners of tetrahedron t equal to k (bit i)
+    if (t < ntet)
+      for (int i = 0; i < 4; i++) {
+        const int v = tetc[4 * t + i];
+        bad |= (v < 0 || v >= nk);
+        hit |= (v == k) << i;
       }
+    // exclusive prefix of the hit counts over the wavefront
+    const int nh = __popc(hit);
+    int x = nh;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
     }
-  cnt[k] = bad ? maxinc + 1 : c;  // > maxinc: malformed tetc, nothing is read
+    int slot = c + x - nh;
+    for (int i = 0; i < 4; i++)
+      if ((hit >> i) & 1) {
+        if (slot < maxinc) inc[k * maxinc + slot] = 4 * t + i;
+        slot++;
+      }
+    c += __shfl(x, 63, 64);
+  }
+  const bool anybad = __ballot(bad) != 0;
+  if (lane == 0) cnt[k] = anybad ? maxinc + 1 : c;  // > maxinc: malformed tetc, nothing is read
 }
 
-// layout 0: reference F-order kcw[n_o, n_e, N_k, nom]
-// layout 1: kernel-native kcw[N_k][nom][n_o*n_e], pair p = i_o*n_e + i_e
-__global__ void __launch_bounds__(256) pygw_tw_gather(
-    const double* __restrict__ enk, int nbnd, double ef, const int* __restrict__ kqid, int ncg, int numin, int no,
-    int ne, const double* __restrict__ omega, int nom, const int* __restrict__ tetc, int nk, int ntet,
-    const int* __restrict__ inc, const int* __restrict__ cnt, int maxinc, int compat, int layout,
-    double* __restrict__ kcw, int* __restrict__ dstat) {
-  const int np = no * ne;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// Two passes per chunk of band pairs (pair index q in the output layout's
+// order: layout 0 q = i_o + n_o*i_e, layout 1 q = i_o*n_e + i_e):
+//   pygw_tw_tet      one lane per (tetrahedron, pair): the four corner
+//                    weights of the pair over all frequencies into
+//                    w1t[T][corner][w][q - q0] (convw1t_par's w1t(4, nom)),
+//                    zeros where no band crosses E_F in T; pairs whose
+//                    polyhedron is cut by E_F go to a list for
+//   pygw_tw_general  one lane per (listed pair, corner): the general
+//                    polyhedron decomposition;
+//   pygw_tw_sum      one lane per (k, pair): kcw = sum over the incidences
+//                    of k in increasing tetrahedron order of w1t * 6/N_tet,
+//                    the scatter order of for_tetrahedra.f90:445-456.
+__device__ __forceinline__ void tw_pair(int q, int layout, int no, int ne, int& ib, int& jb) {
+  if (layout == 0) { ib = q % no; jb = q / no; }
+  else { ib = q / ne; jb = q % ne; }
+}
+
+// band energies at the corners of T (occupied band ib) and of its k-q
+// partner tl (empty band jba); false (nothing read) for a corner index
+// outside the k-mesh
+__device__ __forceinline__ bool tw_corners(const double* __restrict__ enk, int nbnd, int nk,
+                                           const int* __restrict__ tetc, int T, int tl, int ib, int jba, double* e,
+                                           double* f) {
+  int a[4], b[4];
+  bool ok = true;
+  for (int i = 0; i < 4; i++) {
+    a[i] = tetc[4 * T + i];
+    b[i] = tetc[4 * tl + i];
+    ok &= a[i] >= 0 && a[i] < nk && b[i] >= 0 && b[i] < nk;
+  }
+  if (!ok) return false;
+  for (int i = 0; i < 4; i++) {
+    f[i] = enk[jba + (size_t)nbnd * b[i]];
+    e[i] = enk[ib + (size_t)nbnd * a[i]];
+  }
+  return true;
+}
+
+// atan / log table of one chunk: for k-point k and pair q, Delta =
+// e_f(kqid[k]) - e_e(k) (a corner value of every whole tetrahedron with
+// corner k whose k-q partner corner is kqid[k]), dtab[k][q] = Delta and
+// tab[k][w][q] = (atan(Delta/w), log(Delta^2 + w^2)) as in the closed forms
+__global__ void __launch_bounds__(64) pygw_tw_table(const double* __restrict__ enk, int nbnd,
+                                                    const int* __restrict__ kqid, int ncg, int numin, int no, int ne,
+                                                    const double* __restrict__ omega, int nom, int nk, int q0, int pcn,
+                                                    int pp, int layout, double2* __restrict__ tab,
+                                                    double* __restrict__ dtab) {
+  const int ql = blockIdx.x * 64 + threadIdx.x;
   const int k = blockIdx.y;
-  if (t >= np || k >= nk) return;
+  if (ql >= pcn || k >= nk) return;
   int ib, jb;
-  if (layout == 0) { ib = t % no; jb = t / no; }
-  else { ib = t / ne; jb = t % ne; }
-  const int jba = jb + ncg + numin;
-  const double wtet = 1.0 / ntet;
-  int ninc = cnt[k];
+  tw_pair(q0 + ql, layout, no, ne, ib, jb);
+  const int kk = kqid[k];
+  if (kk < 0 || kk >= nk) {
+    dtab[(size_t)k * pp + ql] = __longlong_as_double(0x7ff8000000000001LL);  // matches no corner value
+    return;
+  }
+  const double d = enk[jb + ncg + numin + (size_t)nbnd * kk] - enk[ib + (size_t)nbnd * k];
+  dtab[(size_t)k * pp + ql] = d;
+  double2* o = tab + (size_t)k * nom * pp + ql;
+  for (int w = 0; w < nom; w++) {
+    double t, l;
+    tw::tan_log(d, omega[w], t, l);
+    o[(size_t)w * pp] = double2{t, l};
+  }
+}
+
+// append (T, ql) of the flagged lanes to a list (one atomic per wavefront)
+__device__ __forceinline__ void tw_append(bool flag, int2* __restrict__ list, int* __restrict__ count, int T, int ql) {
+  const unsigned long long m = __ballot(flag);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (flag) list[base + __popcll(m & ((1ull << lane) - 1ull))] = make_int2(T, ql);
+}
+
+__global__ void __launch_bounds__(64) pygw_tw_tet(
+    const double* __restrict__ enk, int nbnd, double ef, const int* __restrict__ kqid, int ncg, int numin, int no,
+    int ne, const double* __restrict__ omega, int nom, const int* __restrict__ tetc, int nk, int ntet, int q0, int pcn,
+    int pp, int compat, int layout, const double* __restrict__ dtab, double* __restrict__ w1t, int2* __restrict__ wlist,
+    int* __restrict__ wcount, int2* __restrict__ glist, int* __restrict__ gcount, int* __restrict__ dstat) {
+  const int nbp = (pcn + 63) >> 6;
+  const int T = blockIdx.x / nbp;
+  const int ql = (blockIdx.x - T * nbp) * 64 + threadIdx.x;
+  const bool act = ql < pcn && T < ntet;
   int nwarn = 0, status = 0;
-  if (ninc > maxinc) { ninc = 0; status |= 4; }
-  for (int c0 = 0; c0 < nom; c0 += tw::OMC) {
-    double acc[tw::OMC];
-    for (int w = 0; w < tw::OMC; w++) acc[w] = 0.0;
-    for (int r = 0; r < ninc; r++) {
-      const int code = inc[k * maxinc + r];
-      const int T = code >> 2, corner = code & 3;
-      const int kl = kqid[T / 6];
-      if (kl < 0 || kl >= nk) { status |= 4; continue; }
-      const int tl = 6 * kl + T % 6;
-      double e[4], f[4];
-      double fmax = -1e300, emin = 1e300;
-      for (int i = 0; i < 4; i++) {
-        f[i] = enk[jba + (size_t)nbnd * tetc[4 * tl + i]];
-        e[i] = enk[ib + (size_t)nbnd * tetc[4 * T + i]];
-        fmax = fmax > f[i] ? fmax : f[i];
-        emin = emin < e[i] ? emin : e[i];
-      }
-      if (fmax <= ef || emin > ef) continue;
-      double res[tw::OMC];
-      if (tw::insulating(e, f, ef)) tw::corner_weight_full(e, f, corner, omega, nom, c0, compat, res, &nwarn, &status);
-      else tw::corner_weight(e, f, ef, corner, omega, nom, c0, compat, res, &nwarn, &status);
-      for (int w = 0; w < tw::OMC; w++) acc[w] = acc[w] + res[w] * 6 * wtet;
+  bool general = false, whole = false;
+  if (act) {
+    int ib, jb;
+    tw_pair(q0 + ql, layout, no, ne, ib, jb);
+    const size_t sc = (size_t)nom * pp, sw = (size_t)pp;
+    double* out = w1t + (size_t)T * 4 * sc + ql;
+    const int kl = kqid[T / 6];
+    int cat = 0;  // 0: no weight, 1: whole tetrahedron, 2: polyhedron cut by E_F
+    double e[4], f[4];
+    if (kl < 0 || kl >= nk || !tw_corners(enk, nbnd, nk, tetc, T, 6 * kl + T % 6, ib, jb + ncg + numin, e, f)) {
+      status |= 4;
+    } else {
+      const double fmx = fmax(fmax(f[0], f[1]), fmax(f[2], f[3]));
+      const double emn = fmin(fmin(e[0], e[1]), fmin(e[2], e[3]));
+      if (!(fmx <= ef || emn > ef)) cat = tw::insulating(e, f, ef) ? 1 : 2;
     }
-    for (int w = 0; w < tw::OMC; w++) {
-      int j = c0 + w;
-      if (j >= nom) break;
-      size_t off;
-      if (layout == 0) off = (size_t)t + (size_t)np * ((size_t)k + (size_t)nk * j);
-      else off = (size_t)t + (size_t)np * ((size_t)j + (size_t)nom * k);
-      kcw[off] = acc[w];
+    if (cat == 0) {
+      for (int c = 0; c < 4; c++)
+        for (int w = 0; w < nom; w++) out[c * sc + w * sw] = 0.0;
+    } else if (cat == 1) {
+      // the whole tetrahedron: vertex energies by the small-tetrahedron formula
+      const double cr[4][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}, {0.0, 1.0, 0.0}, {1.0, 0.0, 0.0}};
+      double db[4], ds[4], dsort[4];
+      for (int i = 0; i < 4; i++) db[i] = f[i] - e[i];
+      const double thr = tw::vertex_delta(cr, db, ds);
+      for (int i = 0; i < 4; i++) dsort[i] = ds[i];
+      int ind[4], cls;
+      if (!tw::sort_eq(dsort, ind, &cls)) {
+        general = true;  // the general path reports the classification error
+      } else if (cls == 16 || cls == 8 || cls == 10) {
+        tw::full_tet_sym(cls, ds, dsort, ind, thr, omega, nom, compat, out, sc, sw, &nwarn);
+      } else {
+        // classes 4 and 6: per corner with the tabulated atan / log when every
+        // vertex energy is its corner's table entry bit for bit
+        bool hit = true;
+        for (int j = 0; j < 4; j++) {  // vertex j sits at big corner 0, 3, 2, 1
+          const int kc = tetc[4 * T + (j == 0 ? 0 : 4 - j)];
+          hit &= __double_as_longlong(ds[j]) == __double_as_longlong(dtab[(size_t)kc * pp + ql]);
+        }
+        if (hit) whole = true;
+        else general = true;
+      }
+    } else {
+      general = true;
+    }
+  }
+  tw_append(whole, wlist, wcount, T, ql);
+  tw_append(general, glist, gcount, T, ql);
+  if (status) atomicOr(&dstat[0], status);
+  if (nwarn) atomicAdd(&dstat[1], nwarn);
+}
+
+// Whole tetrahedra of class 4 or 6 listed by pygw_tw_tet, one lane per
+// (listed pair, corner): the closed form of the corner's vertex with the
+// atan / log of the sorted vertex energies read from the table (in class 6
+// the vertices 3, 4 use the average of the equal pair, evaluated here).
+__global__ void __launch_bounds__(64) pygw_tw_whole(
+    const double* __restrict__ enk, int nbnd, const int* __restrict__ kqid, int ncg, int numin, int no, int ne,
+    const double* __restrict__ omega, int nom, const int* __restrict__ tetc, int nk, int q0, int pp, int compat,
+    int layout, const double2* __restrict__ tab, const int2* __restrict__ wlist, const int* __restrict__ wcount,
+    double* __restrict__ w1t, int* __restrict__ dstat) {
+  const int n = 4 * wcount[0];
+  int nwarn = 0;
+  const size_t sc = (size_t)nom * pp, sw = (size_t)pp;
+  for (int u = blockIdx.x * 64 + threadIdx.x; u < n; u += gridDim.x * 64) {
+    const int2 it = wlist[u >> 2];
+    const int corner = u & 3, T = it.x, ql = it.y;
+    int ib, jb;
+    tw_pair(q0 + ql, layout, no, ne, ib, jb);
+    double e[4], f[4];
+    tw_corners(enk, nbnd, nk, tetc, T, 6 * kqid[T / 6] + T % 6, ib, jb + ncg + numin, e, f);  // checked in pass 1
+    const double cr[4][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}, {0.0, 1.0, 0.0}, {1.0, 0.0, 0.0}};
+    double db[4], ds[4], dsort[4];
+    for (int i = 0; i < 4; i++) db[i] = f[i] - e[i];
+    const double thr = tw::vertex_delta(cr, db, ds);
+    for (int i = 0; i < 4; i++) dsort[i] = ds[i];
+    int ind[4], cls;
+    tw::sort_eq(dsort, ind, &cls);  // class 4 or 6, as in pass 1
+    const int j = corner == 0 ? 0 : 4 - corner;  // original vertex of this corner
+    int v = 0;
+    for (int i = 0; i < 4; i++) if (ind[i] == j) v = i;
+    tw::VCoef c;
+    tw::vertex_coef(c, cls, dsort, v, compat);
+    double s1, n03;
+    tw::taylor_coef(ds, j, s1, n03);
+    // table rows of the sorted energies, then of the vertex energies ev[i]
+    const double2* row[4];
+    for (int i = 0; i < 4; i++) {
+      const int ji = ind[i];
+      row[i] = tab + (size_t)tetc[4 * T + (ji == 0 ? 0 : 4 - ji)] * nom * pp + ql;
+    }
+    const double2* pe[4];
+    if (cls == 4) {
+      for (int i = 0; i < 4; i++) pe[i] = row[(compat & tw::COMPAT_F7B) ? ((i - v + 4) & 3) : ((i + v) & 3)];
+    } else {
+      pe[0] = row[v < 2 ? v : 0];   // v >= 2: ev[0] is the pair average (computed below)
+      pe[1] = pe[0];
+      pe[2] = row[v == 3 ? 3 : 2];
+      pe[3] = row[v == 3 ? 2 : 3];
+    }
+    double* out = w1t + ((size_t)T * 4 + corner) * sc + ql;
+    for (int iw = 0; iw < nom; iw++) {
+      const double om = omega[iw];
+      double r;
+      if (om > thr) {
+        const double o2 = om * om;
+        r = s1 / (6.0e1 * o2) + n03 / (4.2e2 * (o2 * o2));
+      } else if (cls == 4) {
+        double Tt[4], Lt[4];
+        for (int i = 0; i < 4; i++) {
+          const double2 x = pe[i][(size_t)iw * sw];
+          Tt[i] = x.x;
+          Lt[i] = x.y;
+        }
+        r = tw::check_weight(tw::eval_cls4(c.t, c.w, c.cc, Tt, Lt, om), om, &nwarn);
+      } else {
+        double T0, L0;
+        if (v < 2) {
+          const double2 x = pe[0][(size_t)iw * sw];
+          T0 = x.x;
+          L0 = x.y;
+        } else {
+          tw::tan_log(c.ev[0], om, T0, L0);
+        }
+        const double2 x2 = pe[2][(size_t)iw * sw], x3 = pe[3][(size_t)iw * sw];
+        r = tw::check_weight(tw::eval_cls6(c, om, T0, L0, x2.x, x2.y, x3.x, x3.y), om, &nwarn);
+      }
+      out[(size_t)iw * sw] = r;
+    }
+  }
+  if (nwarn) atomicAdd(&dstat[1], nwarn);
+}
+
+__global__ void __launch_bounds__(64) pygw_tw_general(
+    const double* __restrict__ enk, int nbnd, double ef, const int* __restrict__ kqid, int ncg, int numin, int no,
+    int ne, const double* __restrict__ omega, int nom, const int* __restrict__ tetc, int nk, int q0, int pp,
+    int compat, int layout, const int2* __restrict__ glist, const int* __restrict__ gcount, double* __restrict__ w1t,
+    int* __restrict__ dstat) {
+  const int n = 4 * gcount[0];
+  int nwarn = 0, status = 0;
+  const size_t sc = (size_t)nom * pp, sw = (size_t)pp;
+  for (int u = blockIdx.x * 64 + threadIdx.x; u < n; u += gridDim.x * 64) {
+    const int2 it = glist[u >> 2];
+    const int corner = u & 3, T = it.x, ql = it.y;
+    int ib, jb;
+    tw_pair(q0 + ql, layout, no, ne, ib, jb);
+    double e[4], f[4];
+    tw_corners(enk, nbnd, nk, tetc, T, 6 * kqid[T / 6] + T % 6, ib, jb + ncg + numin, e, f);  // checked in pass 1
+    double* out = w1t + ((size_t)T * 4 + corner) * sc + ql;
+    for (int c0 = 0; c0 < nom; c0 += tw::OMC) {
+      double res[tw::OMC];
+      tw::corner_weight(e, f, ef, corner, omega, nom, c0, compat, res, &nwarn, &status);
+      for (int i = 0; i < tw::OMC && c0 + i < nom; i++) out[(size_t)(c0 + i) * sw] = res[i];
     }
   }
   if (status) atomicOr(&dstat[0], status);
   if (nwarn) atomicAdd(&dstat[1], nwarn);
+}
+
+// layout 0: reference F-order kcw[n_o, n_e, N_k, nom]
+// layout 1: kernel-native kcw[N_k][nom][n_o*n_e], pair p = i_o*n_e + i_e
+// one lane per (k, frequency, pair): the 24 incidences summed in order
+__global__ void __launch_bounds__(64) pygw_tw_sum(const double* __restrict__ w1t, const int* __restrict__ inc,
+                                                  const int* __restrict__ cnt, int maxinc, int nk, int ntet, int nom,
+                                                  int q0, int pcn, int pp, int np, int layout,
+                                                  double* __restrict__ kcw,
+                                                  int* __restrict__ dstat) {
+  const int ql = blockIdx.x * 64 + threadIdx.x;
+  const int w = blockIdx.y, k = blockIdx.z;
+  if (ql >= pcn || k >= nk || w >= nom) return;
+  const double wtet = 1.0 / ntet;
+  int ninc = cnt[k];
+  if (ninc > maxinc) {
+    ninc = 0;
+    if (ql == 0 && w == 0) atomicOr(&dstat[0], 4);
+  }
+  const size_t sc = (size_t)nom * pp;
+  const double* src = w1t + (size_t)w * pp + ql;
+  double acc = 0.0;
+  for (int r = 0; r < ninc; r++) acc = acc + src[(size_t)inc[k * maxinc + r] * sc] * 6 * wtet;
+  const size_t q = (size_t)(q0 + ql);
+  const size_t off = layout == 0 ? q + (size_t)np * ((size_t)k + (size_t)nk * w)
+                                 : q + (size_t)np * ((size_t)w + (size_t)nom * k);
+  kcw[off] = acc;
 }
 
 // SVD-basis weights (bz_calcqdepw_par3, for_tetrahedra.f90:466-542):

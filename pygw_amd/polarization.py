@@ -34,7 +34,7 @@ def _require_device(device):
 
 
 def bz_calcqdepw_par2(enk, ef, kqid, ncg, nomx, numin, omega, tetc, *, layout="fortran",
-                      compat=COMPAT_REFERENCE, device="cuda", check=True):
+                      compat=COMPAT_REFERENCE, device="cuda", check=True, workspace=None):
     """Polarization weights kcw (ft.bz_calcqdepw_par2, for_tetrahedra.f90:390).
 
     enk [nbnd, nkp] (band index first, as PyGW's Fortran-ordered array),
@@ -43,6 +43,8 @@ def bz_calcqdepw_par2(enk, ef, kqid, ncg, nomx, numin, omega, tetc, *, layout="f
     strides (PyGW's kcw); layout="native": [nkp, nom, n_o*n_e] contiguous.
     With check=True the call synchronises and raises on geometry errors;
     the check_weight warning count is stored in `bz_calcqdepw_par2.nwarn`.
+    workspace: optional uint8 device tensor (pygw_tetra_weights_workspace
+    gives the full size; a smaller one runs the band pairs in more chunks).
     """
     device = _require_device(device)
     enk_t = _dev(np.asarray(enk).T if not isinstance(enk, torch.Tensor) else enk.T, torch.float64, device)
@@ -62,7 +64,9 @@ def bz_calcqdepw_par2(enk, ef, kqid, ncg, nomx, numin, omega, tetc, *, layout="f
     info = (C.c_int * 2)()
     _lib.call("pygw_tetra_weights", C.c_void_p(enk_t.data_ptr()), float(ef), C.c_void_p(kq_t.data_ptr()),
               C.c_void_p(tetc_t.data_ptr()), int(ncg), int(nomx), int(numin), C.c_void_p(om_t.data_ptr()), nom, nkp,
-              nbnd, ntet, lay, int(compat), C.c_void_p(buf.data_ptr()), None, 0,
+              nbnd, ntet, lay, int(compat), C.c_void_p(buf.data_ptr()),
+              None if workspace is None else C.c_void_p(workspace.data_ptr()),
+              0 if workspace is None else workspace.numel(),
               C.cast(info, C.c_void_p) if check else None, _stream_ptr(device))
     if check:
         bz_calcqdepw_par2.nwarn = info[1]
@@ -284,7 +288,7 @@ class QPlan:
         self.eps_buf = torch.empty(self.pb * self.pb * self.nom, dtype=torch.complex128, device=d)
         self.eps = _fortran_view(self.eps_buf, self.pb, self.nom)
         sz = C.c_size_t(0)
-        _lib.call("pygw_tetra_weights_workspace", self.nk, C.byref(sz))
+        _lib.call("pygw_tetra_weights_workspace", self.nk, self.np, self.nom, C.byref(sz))
         self.ws_tw = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
         _lib.call("pygw_chi0_workspace", self.pb, self.np, self.nom, self.nk, {"fp64": 0, "mixed": 1}[precision],
                   C.byref(sz))
@@ -332,9 +336,10 @@ class Chi0Batch:
     weights, layout of the kernel-native kcw with the pairs gathered).  The
     eps of every q (complex128, Fortran [PB_q, PB_q, n_om]) and the
     workspace are allocated once; run() only launches kernels and can be
-    captured in a HIP graph (capture())."""
+    captured in a HIP graph (capture()).  With inverse=True run() continues
+    with eps^-1 of every q (pygw_eps_inverse, explicit workspaces)."""
 
-    def __init__(self, Ms, kcws, coef=-2.0, device="cuda"):
+    def __init__(self, Ms, kcws, coef=-2.0, inverse=False, device="cuda"):
         self.device = d = _require_device(device)
         if len(Ms) != len(kcws) or not Ms:
             raise ValueError("need one kcw per M")
@@ -355,14 +360,36 @@ class Chi0Batch:
         self.ws = torch.empty(sz.value, dtype=torch.uint8, device=d)
         ptr = lambda ts: (C.c_void_p * len(ts))(*[t.data_ptr() for t in ts])  # noqa: E731
         self._mp, self._kp, self._ep = ptr(self.Ms), ptr(self.kcws), ptr(self.eps_buf)
+        self.inverse = bool(inverse)
+        self.ws_inv = []
+        if self.inverse:  # eps^-1 of every q after the contraction (explicit workspaces)
+            si = C.c_size_t(0)
+            for p in self.pb:
+                _lib.call("pygw_eps_inverse_workspace", int(p), self.nom, C.byref(si))
+                self.ws_inv.append(torch.empty(si.value, dtype=torch.uint8, device=d))
         self.plan = C.c_void_p(0)
         _lib.call("pygw_chi0_batch_plan", self.nq, C.cast(self._mp, C.c_void_p), C.c_void_p(self.pb.ctypes.data),
                   C.c_void_p(self.np.ctypes.data), C.cast(self._kp, C.c_void_p), self.nom, float(coef),
                   C.cast(self._ep, C.c_void_p), C.c_void_p(self.ws.data_ptr()), sz.value, _stream_ptr(d),
                   C.byref(self.plan))
 
-    def run(self, schedule=0):
-        _lib.call("pygw_chi0_batch_run", self.plan, int(schedule), _stream_ptr(self.device))
+    def run(self, schedule=0, inverse=None):
+        """Launch the batch (schedule 0 = autotuned / default fold interval);
+        inverse (default: the constructor's) continues with eps^-1 per q."""
+        s = _stream_ptr(self.device)
+        _lib.call("pygw_chi0_batch_run", self.plan, int(schedule), s)
+        if not (self.inverse if inverse is None else inverse):
+            return self.eps
+        return self.invert()
+
+    def invert(self):
+        """eps^-1 - 1 of every q in place (the eps of the last run())."""
+        if not self.ws_inv:
+            raise ValueError("Chi0Batch built without inverse=True has no inversion workspace")
+        s = _stream_ptr(self.device)
+        for b, p, w in zip(self.eps_buf, self.pb, self.ws_inv):
+            _lib.call("pygw_eps_inverse", C.c_void_p(b.data_ptr()), int(p), self.nom, C.c_void_p(w.data_ptr()),
+                      w.numel(), None, s)
         return self.eps
 
     def autotune(self):
@@ -372,13 +399,13 @@ class Chi0Batch:
                   _stream_ptr(self.device))
         return best.value, dict(zip((1, 2, 4), list(times)))
 
-    def capture(self):
+    def capture(self, schedule=0, inverse=None):
         """Capture run() in a HIP graph; returns the torch.cuda.CUDAGraph."""
         g = torch.cuda.CUDAGraph()
-        self.run()
+        self.run(schedule, inverse)
         torch.cuda.synchronize(self.device)
         with torch.cuda.graph(g):
-            self.run()
+            self.run(schedule, inverse)
         return g
 
     def close(self):

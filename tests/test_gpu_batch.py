@@ -80,3 +80,21 @@ def test_batch_graph_schedules_autotune():
     assert best in (1, 2, 4)
     B.close()
     torch.cuda.synchronize()
+
+
+def test_batch_inverse_matches_eps_inverse():
+    import torch
+    from pygw_amd import polarization as P
+    batch, Ms, kcws = _batch_inputs(n_q=2)
+    print("case batch inverse", flush=True)
+    B = P.Chi0Batch(Ms, kcws, inverse=True)
+    chi = [e.clone() for e in B.run(inverse=False)]
+    got = B.invert()
+    for a, c in zip(got, chi):
+        want = P.eps_inverse(c.clone())
+        assert (torch.linalg.norm(a - want) / torch.linalg.norm(want)).item() < 1e-12
+    again = B.run()              # contraction + inversion in one call
+    for a, c in zip(again, got):
+        assert (torch.linalg.norm(a - c) / torch.linalg.norm(c)).item() < 1e-12
+    B.close()
+    torch.cuda.synchronize()

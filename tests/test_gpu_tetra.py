@@ -121,3 +121,30 @@ def test_gpu_weights_svd_basis():
     assert st >= 0
     np.testing.assert_allclose(kl, want, rtol=0, atol=1e-12 * np.abs(want).max())
     del L
+
+
+def test_gpu_weights_pair_chunks():
+    """A workspace of one 64-pair chunk (three chunks here) gives the same
+    bits as the default single chunk; both match the oracle."""
+    import torch
+    from pygw_amd import _lib
+    from pygw_amd import polarization as P
+    cfg = synth.small_config(div=(4, 4, 4), n_occ=8, n_emp=20, n_metal=3, n_om=7, seed=17)
+    b = synth.band_energies(cfg)
+    om, _ = synth.omega_mesh(cfg)
+    tetc = mesh.tetra_corners(cfg.div)
+    kq = mesh.k_minus_q(cfg.div)[:, 5]
+    win = (b.ncg, b.nomx, b.numin)
+    assert b.n_o * b.n_e > 128
+    sz = C.c_size_t(0)
+    _lib.call("pygw_tetra_weights_workspace", cfg.nk, 64, om.size, C.byref(sz))
+    ws = torch.empty(sz.value, dtype=torch.uint8, device="cuda")
+    print(f"case pair chunks np={b.n_o * b.n_e} ws={sz.value}", flush=True)
+    full = P.bz_calcqdepw_par2(b.enk, 0.0, kq, *win, om, tetc).cpu().numpy()
+    for layout in ("fortran", "native"):
+        a = P.bz_calcqdepw_par2(b.enk, 0.0, kq, *win, om, tetc, layout=layout).cpu().numpy()
+        c = P.bz_calcqdepw_par2(b.enk, 0.0, kq, *win, om, tetc, layout=layout, workspace=ws).cpu().numpy()
+        np.testing.assert_array_equal(a, c)
+    _check(full, b.enk, kq, win, om, tetc, name="pair chunks")
+    with pytest.raises(_lib.PygwError):
+        P.bz_calcqdepw_par2(b.enk, 0.0, kq, *win, om, tetc, workspace=ws[: sz.value // 2])
