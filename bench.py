@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--nq", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="c4: time the split-precision batch only (no FP64 path, no precision check; profiling runs)")
     ap.add_argument("--cpu-kslab", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain 1 GiB device copy (scale for HBM byte counters)")
@@ -534,6 +536,11 @@ def run_batch(args):
         if it >= args.warmup:
             times.append(st)
     tm = np.array(times).mean(0)
+    if args.no_fp64:
+        B.close()
+        print(json.dumps({"metric": "c4 split-precision batch only (profiling run, precision unchecked)",
+                          "s_per_q": float(tm.sum()) / nq, "stages_s_per_q": (tm / nq).tolist()}))
+        return
     # FP64: the same q one after another on the complex64-rounded M
     t64 = np.zeros(3)
     for i in range(nq):

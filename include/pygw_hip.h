@@ -100,7 +100,8 @@ int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, cons
  * inputs and records the fastest for (device, precision, pb, n_pairs, n_om);
  * later pygw_chi0_accumulate calls with that shape use it.  eps is
  * overwritten (scratch).  best: chosen frequencies per block; times[3]
- * (optional): milliseconds of each candidate. */
+ * (optional): milliseconds of each candidate (-1: not tried; split
+ * precision tries s = 1, 2 only, see the batch API below). */
 int pygw_chi0_autotune(const void* M, long stride_k, int n_pairs, int pb, const double* kcw, int n_om, int k0,
                        int n_k, double coef, int precision, void* eps, int* best, float* times, void* stream);
 /* Sets (nw = 1, 2, 4) or clears (nw = 0) the cached schedule of one shape. */
@@ -117,7 +118,9 @@ int pygw_chi0_set_schedule(int precision, int pb, int n_pairs, int n_om, int nw)
  * an opaque plan; run only launches kernels (capturable in a HIP graph):
  * per-q scales and fragment planes, one persistent contraction over all
  * (q, tile pair, frequency) work items, per-q mirror.  schedule 0 = the
- * autotuned one (default 2); 1, 2, 4 = fold every 64*s chunks.
+ * autotuned one (default 2); 1, 2, 4 = fold every 64*s chunks.  The
+ * autotuners (split precision) choose among s = 1, 2 (times[2] = -1): s = 4
+ * doubles the FP32 partial sums and lands near the 1e-6 accuracy bar.
  * ------------------------------------------------------------------------ */
 int pygw_chi0_batch_workspace(int n_q, const int* pb, const int* n_pairs, int n_om, size_t* bytes);
 int pygw_chi0_batch_plan(int n_q, const void* const* M, const int* pb, const int* n_pairs, const double* const* kcw,

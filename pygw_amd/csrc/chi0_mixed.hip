@@ -34,6 +34,7 @@
 // share are read from HBM once per XCD and then served from its L2.
 #include "common.h"
 
+#include <algorithm>
 #include <vector>
 
 // No FMA contraction in this file, and every split goes through cm::split,
@@ -374,18 +375,29 @@ __global__ void __launch_bounds__(256) pygw_colmax(const float2* __restrict__ M,
   const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
   const int I = blockIdx.x * 64 + c;
   float m = 0.f;
+  // rows (k, p) flattened, dealt 4 per block step over gridDim.y blocks
+  const int step = 4 * gridDim.y;
+  int k = 0, p = r + 4 * blockIdx.y;
+  while (k < nk && p >= np) { p -= np; k++; }
   if (I < pb)
-    for (int k = blockIdx.y; k < nk; k += gridDim.y)
-      for (int p = r; p < np; p += 4) {
-        const float2 v = M[(size_t)k * strideK + (size_t)p * pb + I];
-        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
-      }
+    while (k < nk) {
+      const float2 v = M[(size_t)k * strideK + (size_t)p * pb + I];
+      m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+      p += step;
+      while (k < nk && p >= np) { p -= np; k++; }
+    }
   red[r][c] = m;
   __syncthreads();
   if (r == 0 && I < pb) {
     m = fmaxf(fmaxf(red[0][c], red[1][c]), fmaxf(red[2][c], red[3][c]));
     atomicMax(&cmax[I], __float_as_uint(m));  // non-negative floats order as uints
   }
+}
+
+// blocks along the rows of pygw_colmax: about 64 rows per block, at most 1024
+static int colmax_rows_grid(long rows) {
+  const long g = (rows + 63) / 64;
+  return (int)(g < 1 ? 1 : g > 1024 ? 1024 : g);
 }
 
 // cs[I] = 2^(14 - e), e the exponent of the column maximum (|x cs| < 2^14);
@@ -504,7 +516,8 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
 
   hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, cmax, (long)pb);
   hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, (unsigned int*)wmax, 2L * nom);
-  hipLaunchKernelGGL(pygw_colmax, dim3(ceil_div(pb, 64), nkc < 256 ? nkc : 256), dim3(256), 0, stream, Mf, strideK,
+  hipLaunchKernelGGL(pygw_colmax, dim3(ceil_div(pb, 64), colmax_rows_grid((long)nkc * np)), dim3(256), 0, stream, Mf,
+                     strideK,
                      np, pb, nkc, cmax);
   hipLaunchKernelGGL(pygw_colscale, dim3(ceil_div(pb, 256)), dim3(256), 0, stream, cmax, pb, cs);
   hipLaunchKernelGGL(pygw_wmax, dim3(64, nom), dim3(256), 0, stream, kcw + (size_t)k0 * nom * np, np, nom, nkc, coef,
@@ -539,10 +552,11 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
 // Batched contraction over many q-points in one persistent launch.  Each q
 // has its own PB and pair count (all pairs of all k concatenated: only the
 // (k, p) list matters to the sum), its own operands, scales and eps.  Work
-// item = (q, tile pair, frequency).  The (q, tile pair) groups are dealt to
-// the 8 XCDs round robin; within an XCD the resident blocks take the XCD's
-// items from an atomic counter, frequency fastest, so blocks that run
-// together on an XCD share a tile pair's operand slabs in its L2.
+// item = (q, tile pair, frequency).  The (q, tile pair) groups, in order of
+// decreasing cost (the q's chunk count), are dealt to the 8 XCDs round robin;
+// within an XCD the resident blocks take the XCD's items from an atomic
+// counter, frequency fastest, so blocks that run together on an XCD share a
+// tile pair's operand slabs in its L2.
 // ---------------------------------------------------------------------------
 struct Chi0Q {
   const h8* S;        // fragment planes [npc][16][pbp]
@@ -663,6 +677,10 @@ int chi0_batch_plan(int nq, const void* const* M, const int* pb, const int* np, 
       groups.push_back(int4{q, It, Jt, 0});
     }
   }
+  // longest first: a group's items cost ~ its q's chunk count, so dealing
+  // them in decreasing cost leaves the short ones for the tail of each XCD
+  std::stable_sort(groups.begin(), groups.end(),
+                   [&](const int4& a, const int4& b) { return P->q[a.x].npc > P->q[b.x].npc; });
   P->ngroups = (int)groups.size();
   const size_t a = ((size_t)nq * sizeof(Chi0Q) + 255) / 256 * 256;
   const size_t b = (((size_t)P->ngroups + 8) * sizeof(int4) + 255) / 256 * 256;
@@ -702,7 +720,8 @@ int chi0_batch_run(const Chi0BatchPlan* P, int G, hipStream_t stream) {
     const double* kc = P->kcw[q];
     hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, cmax, (long)d.pb);
     hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, (unsigned int*)wmax, 2L * nom);
-    hipLaunchKernelGGL(pygw_colmax, dim3(ceil_div(d.pb, 64), 1), dim3(256), 0, stream, d.M, 0L, d.np, d.pb, 1, cmax);
+    hipLaunchKernelGGL(pygw_colmax, dim3(ceil_div(d.pb, 64), colmax_rows_grid(d.np)), dim3(256), 0, stream, d.M, 0L,
+                       d.np, d.pb, 1, cmax);
     hipLaunchKernelGGL(pygw_colscale, dim3(ceil_div(d.pb, 256)), dim3(256), 0, stream, cmax, d.pb, cs);
     hipLaunchKernelGGL(pygw_wmax, dim3(64, nom), dim3(256), 0, stream, kc, d.np, nom, 1, P->coef, wmax);
     hipLaunchKernelGGL(pygw_wscale, dim3(ceil_div(nom, 256)), dim3(256), 0, stream, wmax, nom, wsc);

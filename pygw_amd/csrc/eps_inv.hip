@@ -14,8 +14,8 @@
 #include "common.h"
 
 namespace ei {
-constexpr int NB = 32;   // pivot block
-constexpr int TT = 32;   // update tile edge
+constexpr int NB = 64;   // pivot block: each elimination step reads and writes A once
+constexpr int KH = 32;   // k half staged at a time by the update
 }
 
 // (A + A^H)/2 + I, in place; one thread per (i <= j).
@@ -72,13 +72,14 @@ __global__ void __launch_bounds__(256) pygw_ei_pivot(const double2* __restrict__
   __syncthreads();
   for (int k = 0; k < nb; k++) {
     const double2 piv = S[k][k];
+    const double2 rp = cdiv(double2{1.0, 0.0}, piv);
     if (threadIdx.x == 0 && !(piv.x > 0.0)) atomicOr(&info[w], 1);
     __syncthreads();
     // update all (i, j) with i, j != k first, using the pre-step row/col k
     for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
       const int i = e % nb, j = e / nb;
       if (i != k && j != k) {
-        const double2 t = cdiv(cmul(S[i][k], S[k][j]), piv);
+        const double2 t = cmul(S[i][k], cmul(S[k][j], rp));
         S[i][j].x -= t.x;
         S[i][j].y -= t.y;
       }
@@ -86,14 +87,14 @@ __global__ void __launch_bounds__(256) pygw_ei_pivot(const double2* __restrict__
     __syncthreads();
     for (int e = threadIdx.x; e < nb; e += blockDim.x) {
       if (e != k) {
-        const double2 c = cdiv(S[e][k], piv);
-        const double2 r = cdiv(S[k][e], piv);
+        const double2 c = cmul(S[e][k], rp);
+        const double2 r = cmul(S[k][e], rp);
         S[e][k] = double2{-c.x, -c.y};
         S[k][e] = r;
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) S[k][k] = cdiv(double2{1.0, 0.0}, piv);
+    if (threadIdx.x == 0) S[k][k] = rp;
     __syncthreads();
   }
   double2* b = B + w * NB * NB;
@@ -105,84 +106,137 @@ __global__ void __launch_bounds__(256) pygw_ei_pivot(const double2* __restrict__
 
 // Step 2: panels.  U[w] (nb x n, row panel) = B * A[P, :]  and
 // L[w] (n x nb, column panel) = A[:, P] (copy before the update).
-// grid (ceil(n/64), nom), 64 threads: thread = one column j (for U) and one
-// row i (for L).
-__global__ void __launch_bounds__(64) pygw_ei_panels(const double2* __restrict__ A, int n, int p0, int nb,
-                                                     const double2* __restrict__ B, double2* __restrict__ U,
-                                                     double2* __restrict__ L) {
+// grid (ceil(n/64), nom), 256 threads: 64 columns j per block, A[P, j] and B
+// staged in LDS, each thread forms 16 rows of one column of U and copies
+// 16 entries of L.
+__global__ void __launch_bounds__(256) pygw_ei_panels(const double2* __restrict__ A, int n, int p0, int nb,
+                                                      const double2* __restrict__ B, double2* __restrict__ U,
+                                                      double2* __restrict__ L) {
   using namespace ei;
-  __shared__ double2 Bs[NB][NB];
+  __shared__ double2 Bs[NB][NB + 1];
+  __shared__ double2 Cs[NB][64 + 1];  // A[p0 + c][j0 + jj]
   const size_t w = blockIdx.y;
   const double2* a = A + w * (size_t)n * n;
   const double2* b = B + w * NB * NB;
+  const int j0 = blockIdx.x * 64;
   for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) Bs[e % nb][e / nb] = b[(e % nb) + NB * (e / nb)];
-  __syncthreads();
-  const int j = blockIdx.x * 64 + threadIdx.x;
-  if (j >= n) return;
-  double2 col[NB];
-  for (int r = 0; r < nb; r++) col[r] = a[(p0 + r) + (size_t)n * j];
-  double2* u = U + w * (size_t)NB * n;
-  for (int r = 0; r < nb; r++) {
-    double2 s = {0.0, 0.0};
-    for (int c = 0; c < nb; c++) {
-      const double2 t = cmul(Bs[r][c], col[c]);
-      s.x += t.x;
-      s.y += t.y;
-    }
-    u[r + (size_t)NB * j] = s;
+  for (int e = threadIdx.x; e < NB * 64; e += blockDim.x) {
+    const int c = e % NB, jj = e / NB;  // consecutive threads: consecutive rows of one column
+    const int j = j0 + jj;
+    Cs[c][jj] = (c < nb && j < n) ? a[(p0 + c) + (size_t)n * j] : double2{0.0, 0.0};
   }
-  double2* l = L + w * (size_t)n * NB;
-  for (int c = 0; c < nb; c++) l[j + (size_t)n * c] = a[j + (size_t)n * (p0 + c)];
+  __syncthreads();
+  const int jj = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  const int j = j0 + jj;
+  if (j < n) {
+    double2* u = U + w * (size_t)NB * n;
+    for (int r = 16 * rg; r < 16 * rg + 16 && r < nb; r++) {
+      double2 s = {0.0, 0.0};
+      for (int c = 0; c < nb; c++) {
+        const double2 t = cmul(Bs[r][c], Cs[c][jj]);
+        s.x += t.x;
+        s.y += t.y;
+      }
+      u[r + (size_t)NB * j] = s;
+    }
+    double2* l = L + w * (size_t)n * NB;
+    for (int c = 16 * rg; c < 16 * rg + 16 && c < nb; c++) l[j + (size_t)n * c] = a[j + (size_t)n * (p0 + c)];
+  }
 }
 
-// Step 3: rank-nb update of all TT x TT tiles.
+// Step 3: rank-nb update of every 64 x 64 tile.
 //   rows i in P, cols j in P : A_PP = B
 //   rows i in P              : A_Pj = U_j
 //   cols j in P              : A_iP = -L_i B
 //   otherwise                : A_ij -= L_i U_j
-__global__ void __launch_bounds__(256) pygw_ei_update(double2* __restrict__ A, int n, int p0, int nb,
-                                                      const double2* __restrict__ B, const double2* __restrict__ U,
-                                                      const double2* __restrict__ L) {
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Step 3 on FP64 MFMA: 64 x 64 tiles, 4 waves of 32 x 32 (2 x 2 tiles of
+// v_mfma_f64_16x16x4f64).  The products are formed transposed, D = U^T L^T
+// (D row = 4v + lane/16 indexes j, D column = lane%16 indexes i), so each
+// group of 16 lanes reads and writes 16 consecutive rows i of the
+// column-major matrix.  S = L U in complex: Re = Lr Ur - Li Ui,
+// Im = Lr Ui + Li Ur (four real products per k step).  Same cases as
+// pygw_ei_update.
+__global__ void __launch_bounds__(256) pygw_ei_update_mfma(double2* __restrict__ A, int n, int p0, int nb,
+                                                           const double2* __restrict__ B,
+                                                           const double2* __restrict__ U,
+                                                           const double2* __restrict__ L) {
   using namespace ei;
-  __shared__ double2 Ls[TT][NB + 1];
-  __shared__ double2 Us[NB][TT + 1];
+  __shared__ double Lr[KH][64 + 1], Li[KH][64 + 1];  // [k - k0][i]
+  __shared__ double Ur[KH][64 + 1], Ui[KH][64 + 1];  // [k - k0][j]
   const size_t w = blockIdx.z;
   double2* a = A + w * (size_t)n * n;
   const double2* u = U + w * (size_t)NB * n;
   const double2* l = L + w * (size_t)n * NB;
   const double2* b = B + w * NB * NB;
-  const int i0 = blockIdx.x * TT, j0 = blockIdx.y * TT;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  // stage L rows i0.., U cols j0.. (for P-columns, U_j is replaced by B)
-  for (int e = threadIdx.x; e < TT * NB; e += 256) {
-    const int r = e % TT, c = e / TT;
-    const int i = i0 + r;
-    Ls[r][c] = (i < n && c < nb) ? l[i + (size_t)n * c] : double2{0.0, 0.0};
-  }
-  for (int e = threadIdx.x; e < NB * TT; e += 256) {
-    const int r = e % NB, c = e / NB;
-    const int j = j0 + c;
-    double2 v = {0.0, 0.0};
-    if (j < n && r < nb) v = (j >= p0 && j < p0 + nb) ? b[r + NB * (j - p0)] : u[r + (size_t)NB * j];
-    Us[r][c] = v;
-  }
-  __syncthreads();
-  for (int q = 0; q < TT / 8; q++) {
-    const int i = i0 + tx, j = j0 + ty + 8 * q;
-    if (i >= n || j >= n) continue;
-    const bool ri = (i >= p0 && i < p0 + nb), cj = (j >= p0 && j < p0 + nb);
-    double2 s = {0.0, 0.0};
-    for (int c = 0; c < nb; c++) {
-      const double2 t = cmul(Ls[tx][c], Us[c][ty + 8 * q]);
-      s.x += t.x;
-      s.y += t.y;
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wi = wave & 1, wj = wave >> 1;  // wave's 32 x 32: rows i0+32wi.., cols j0+32wj..
+  const int li = lane & 15, lk = lane >> 4;
+  d4 sr[2][2], si[2][2];  // [j tile][i tile]
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) { sr[x][y] = d4{0, 0, 0, 0}; si[x][y] = d4{0, 0, 0, 0}; }
+  for (int k0 = 0; k0 < nb; k0 += KH) {
+    if (k0) __syncthreads();  // the previous half is consumed
+    for (int e = tid; e < 64 * KH; e += 256) {
+      const int r = e & 63, c = e >> 6;  // consecutive threads: consecutive rows (column-major L)
+      const int i = i0 + r;
+      const double2 v = (i < n && k0 + c < nb) ? l[i + (size_t)n * (k0 + c)] : double2{0.0, 0.0};
+      Lr[c][r] = v.x;
+      Li[c][r] = v.y;
     }
-    double2* aij = a + i + (size_t)n * j;
-    if (ri && cj) *aij = Us[i - p0][ty + 8 * q];                     // B
-    else if (ri) *aij = Us[i - p0][ty + 8 * q];                      // U_j
-    else if (cj) *aij = double2{-s.x, -s.y};                         // -L_i B
-    else { double2 o = *aij; *aij = double2{o.x - s.x, o.y - s.y}; } // A - L U
+    for (int e = tid; e < KH * 64; e += 256) {
+      const int r = e % KH, c = e / KH;  // consecutive threads: consecutive k of one column
+      const int j = j0 + c, kr = k0 + r;
+      double2 v = {0.0, 0.0};
+      if (j < n && kr < nb) v = (j >= p0 && j < p0 + nb) ? b[kr + NB * (j - p0)] : u[kr + (size_t)NB * j];
+      Ur[r][c] = v.x;
+      Ui[r][c] = v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KH / 4; ks++) {
+      const int k = 4 * ks + lk;
+      double ur[2], ui[2], lr[2], lii[2], nli[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        ur[t] = Ur[k][32 * wj + 16 * t + li];
+        ui[t] = Ui[k][32 * wj + 16 * t + li];
+        lr[t] = Lr[k][32 * wi + 16 * t + li];
+        lii[t] = Li[k][32 * wi + 16 * t + li];
+        nli[t] = -lii[t];
+      }
+#pragma unroll
+      for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) {
+          sr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[x], lr[y], sr[x][y], 0, 0, 0);
+          sr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[x], nli[y], sr[x][y], 0, 0, 0);
+          si[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[x], lr[y], si[x][y], 0, 0, 0);
+          si[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[x], lii[y], si[x][y], 0, 0, 0);
+        }
+    }
   }
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int jl = 32 * wj + 16 * x + 4 * v + lk;  // D row
+        const int il = 32 * wi + 16 * y + li;          // D column
+        const int i = i0 + il, j = j0 + jl;
+        if (i >= n || j >= n) continue;
+        const bool ri = (i >= p0 && i < p0 + nb), cj = (j >= p0 && j < p0 + nb);
+        double2* aij = a + i + (size_t)n * j;
+        const double xr = sr[x][y][v], xi = si[x][y][v];
+        if (ri) *aij = cj ? b[(i - p0) + NB * (j - p0)] : u[(i - p0) + (size_t)NB * j];  // B or U_j
+        else if (cj) *aij = double2{-xr, -xi};                     // -L_i B
+        else { const double2 o = *aij; *aij = double2{o.x - xr, o.y - xi}; }  // A - L U
+      }
 }
 
 int eps_inverse_launch(void* eps, int n, int nom, void* ws, int* dinfo, hipStream_t stream) {
@@ -195,9 +249,9 @@ int eps_inverse_launch(void* eps, int n, int nom, void* ws, int* dinfo, hipStrea
   for (int p0 = 0; p0 < n; p0 += NB) {
     const int nb = (n - p0) < NB ? (n - p0) : NB;
     hipLaunchKernelGGL(pygw_ei_pivot, dim3(nom), dim3(256), 0, stream, A, n, p0, nb, B, dinfo);
-    hipLaunchKernelGGL(pygw_ei_panels, dim3(ceil_div(n, 64), nom), dim3(64), 0, stream, A, n, p0, nb, B, U, L);
-    hipLaunchKernelGGL(pygw_ei_update, dim3(ceil_div(n, TT), ceil_div(n, TT), nom), dim3(256), 0, stream, A, n, p0,
-                       nb, B, U, L);
+    hipLaunchKernelGGL(pygw_ei_panels, dim3(ceil_div(n, 64), nom), dim3(256), 0, stream, A, n, p0, nb, B, U, L);
+    hipLaunchKernelGGL(pygw_ei_update_mfma, dim3(ceil_div(n, 64), ceil_div(n, 64), nom), dim3(256), 0, stream, A, n,
+                       p0, nb, B, U, L);
   }
   hipLaunchKernelGGL(pygw_ei_minus_identity, dim3(ceil_div((long)n * nom, 256)), dim3(256), 0, stream, A, n, nom);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;

@@ -77,7 +77,7 @@ def test_batch_graph_schedules_autotune():
         torch.cuda.synchronize()
     best, times = B.autotune()
     print(f"  autotune best={best} times={times}", flush=True)
-    assert best in (1, 2, 4)
+    assert best in (1, 2) and times[4] == -1.0   # s = 4 is never auto-selected (accuracy)
     B.close()
     torch.cuda.synchronize()
 
