@@ -702,9 +702,15 @@ __device__ __forceinline__ double vertex_delta(const double cr[4][3], const doub
   return 10.0 * mx;
 }
 
-// One small tetrahedron (generictetra_par, for_tetrahedra3.f90:220-405):
-// adds vol * sum_j w_j beta_j for big-tetrahedron corner `corner` to
-// out[w] for frequencies j = c0+w in [jlo, jhi), evaluated at omega[j-shift].
+// One small tetrahedron (generictetra_par, for_tetrahedra3.f90:220-405),
+// evaluated by the four lanes of an aligned lane group together, lane c
+// handling big-tetrahedron corner c: lane c evaluates the closed form of
+// small vertex j = c (each vertex once, as the reference does) with the
+// atan / log of the vertex energies shared through the group (lane i
+// evaluates the i-th sorted energy, or in classes 8/10 lanes 0 and 1 the
+// two node energies), then adds sum_j w_j beta_c(j) (j ascending, beta = 0
+// skipped) times vol to out[w] for frequencies j = c0+w in [jlo, jhi),
+// evaluated at omega[j-shift].  The four lanes must run it together.
 __device__ void small_tet(const double cr[4][3], const double* db, int corner, const double* __restrict__ omega,
                           int c0, int jlo, int jhi, int shift, int compat, double* out, int* nwarn, int* status) {
   double vec[3][3];
@@ -722,40 +728,76 @@ __device__ void small_tet(const double cr[4][3], const double* db, int corner, c
   for (int i = 0; i < 4; i++) dsort[i] = ds[i];
   int ind[4], cls;
   if (!sort_eq(dsort, ind, &cls)) { *status |= ST_GEOM; return; }
+  double beta[4];  // barycentric weight of this lane's corner at each small vertex
+  for (int j = 0; j < 4; j++) beta[j] = (corner == 0) ? (1.0 - cr[j][0] - cr[j][1] - cr[j][2]) : cr[j][corner - 1];
+  const int jv = corner;  // the small vertex this lane evaluates
+  int v = 0;
+  for (int i = 0; i < 4; i++) if (ind[i] == jv) v = i;
+  VCoef c;
+  vertex_coef(c, cls, dsort, v, compat);
+  double s1, n03;
+  taylor_coef(ds, jv, s1, n03);
+  // this lane's share of the atan / log evaluations
+  const double xsh = (cls == 8 || cls == 10) ? ((corner & 1) ? (cls == 8 ? c.ev[2] : c.ev[3]) : c.ev[0])
+                                              : (corner == 0 ? dsort[0] : corner == 1 ? dsort[1]
+                                                 : corner == 2 ? dsort[2] : dsort[3]);
+  const bool f7b = compat & COMPAT_F7B;
   double tl[OMC];
 #pragma unroll
   for (int w = 0; w < OMC; w++) tl[w] = 0.0;
 #pragma unroll 1
-  for (int j = 0; j < 4; j++) {  // original small-vertex order (transfer sum order)
-    double beta = (corner == 0) ? (1.0 - cr[j][0] - cr[j][1] - cr[j][2]) : cr[j][corner - 1];
-    if (beta == 0.0) continue;
-    int v = 0;
-    for (int i = 0; i < 4; i++) if (ind[i] == j) v = i;
-    VCoef c;
-    vertex_coef(c, cls, dsort, v, compat);
-    double s1, n03;
-    taylor_coef(ds, j, s1, n03);
-    // one frequency per iteration (not unrolled: the closed form is long);
-    // the accumulators rotate so that slot 0 always belongs to frequency w
-#pragma unroll 1
-    for (int w = 0; w < OMC; w++) {
-      const int jj = c0 + w;
-      double t0 = tl[0];
-      if (jj >= jlo && jj < jhi) {
-        const double om = omega[jj - shift];
-        double wv;
-        if (om > thr) {
-          const double o2 = om * om;
-          wv = s1 / (6.0e1 * o2) + n03 / (4.2e2 * (o2 * o2));
-        } else {
-          wv = vertex_eval(c, om, nwarn);
+  for (int w = 0; w < OMC; w++) {
+    const int jj = c0 + w;
+    double t0 = tl[0];
+    if (jj >= jlo && jj < jhi) {
+      const double om = omega[jj - shift];
+      double wv;
+      if (om > thr) {
+        const double o2 = om * om;
+        wv = s1 / (6.0e1 * o2) + n03 / (4.2e2 * (o2 * o2));
+      } else if (cls == 16) {
+        wv = check_weight(eval_cls16(c.ev[0], om), om, nwarn);
+      } else {
+        double tm, lm, T[4], L[4];
+        tan_log(xsh, om, tm, lm);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          T[i] = __shfl(tm, i, 4);
+          L[i] = __shfl(lm, i, 4);
         }
-        t0 = t0 + wv * beta;
+        if (cls == 4) {
+          double Tv[4], Lv[4];
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const int r = f7b ? ((i - v + 4) & 3) : ((i + v) & 3);
+            Tv[i] = r == 0 ? T[0] : r == 1 ? T[1] : r == 2 ? T[2] : T[3];
+            Lv[i] = r == 0 ? L[0] : r == 1 ? L[1] : r == 2 ? L[2] : L[3];
+          }
+          wv = eval_cls4(c.t, c.w, c.cc, Tv, Lv, om);
+        } else if (cls == 6) {
+          double T0, L0;
+          if (v < 2) {
+            T0 = v == 0 ? T[0] : T[1];
+            L0 = v == 0 ? L[0] : L[1];
+          } else {
+            tan_log(c.ev[0], om, T0, L0);  // the pair average
+          }
+          const bool sw = v == 3;  // vertex 4 takes d4 as its third energy
+          wv = eval_cls6(c, om, T0, L0, sw ? T[3] : T[2], sw ? L[3] : L[2], sw ? T[2] : T[3], sw ? L[2] : L[3]);
+        } else {
+          wv = eval_kind2(c, om, T[0], L[0], T[1], L[1]);
+        }
+        wv = check_weight(wv, om, nwarn);
       }
 #pragma unroll
-      for (int i = 0; i < OMC - 1; i++) tl[i] = tl[i + 1];
-      tl[OMC - 1] = t0;
+      for (int j = 0; j < 4; j++) {
+        const double wj = __shfl(wv, j, 4);
+        if (beta[j] != 0.0) t0 = t0 + wj * beta[j];
+      }
     }
+#pragma unroll
+    for (int i = 0; i < OMC - 1; i++) tl[i] = tl[i + 1];
+    tl[OMC - 1] = t0;
   }
 #pragma unroll
   for (int w = 0; w < OMC; w++) out[w] = out[w] + tl[w] * vol;
@@ -945,8 +987,8 @@ __global__ void __launch_bounds__(64) pygw_tw_incidence(const int* __restrict__ 
 //                    w1t[T][corner][w][q - q0] (convw1t_par's w1t(4, nom)),
 //                    zeros where no band crosses E_F in T; pairs whose
 //                    polyhedron is cut by E_F go to a list for
-//   pygw_tw_general  one lane per (listed pair, corner): the general
-//                    polyhedron decomposition;
+//   pygw_tw_general  four lanes per listed pair (one per corner, running the
+//                    general polyhedron decomposition together);
 //   pygw_tw_sum      one lane per (k, pair): kcw = sum over the incidences
 //                    of k in increasing tetrahedron order of w1t * 6/N_tet,
 //                    the scatter order of for_tetrahedra.f90:445-456.
