@@ -81,7 +81,10 @@ def test_eps_inverse_rejects_indefinite():
         P.eps_inverse(t)
 
 
-@pytest.mark.parametrize("shape", [(3, 4, 7, 96, 5), (2, 5, 3, 129, 3)])
+# PB 96 / 30 / 60: one partial 128 tile; PB 129 / 170: the last tile
+# column is 1 / 42 columns wide
+@pytest.mark.parametrize("shape", [(3, 4, 7, 96, 5), (2, 5, 3, 129, 3), (2, 4, 5, 30, 3), (2, 3, 5, 60, 2),
+                                   (2, 3, 5, 170, 2)])
 def test_chi0_mixed_vs_fp64(shape):
     """Split-precision contraction: relative Frobenius error per frequency
     <= 1e-6 against the FP64 oracle (the bar the precision scheme is built
