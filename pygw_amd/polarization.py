@@ -362,6 +362,7 @@ class Chi0Batch:
         self._mp, self._kp, self._ep = ptr(self.Ms), ptr(self.kcws), ptr(self.eps_buf)
         self.inverse = bool(inverse)
         self.ws_inv = []
+        self._side = [torch.cuda.Stream(device=d) for _ in range(4)] if self.inverse else []  # default priority
         if self.inverse:  # eps^-1 of every q after the contraction (explicit workspaces)
             si = C.c_size_t(0)
             for p in self.pb:
@@ -382,14 +383,32 @@ class Chi0Batch:
             return self.eps
         return self.invert()
 
-    def invert(self):
-        """eps^-1 - 1 of every q in place (the eps of the last run())."""
+    def invert(self, streams=4):
+        """eps^-1 - 1 of every q in place (the eps of the last run()).  The
+        q's are independent: they are dealt (largest PB first) to `streams`
+        side streams that fork from and join the current stream, so small
+        inversions overlap (also when captured in a HIP graph)."""
         if not self.ws_inv:
             raise ValueError("Chi0Batch built without inverse=True has no inversion workspace")
-        s = _stream_ptr(self.device)
-        for b, p, w in zip(self.eps_buf, self.pb, self.ws_inv):
-            _lib.call("pygw_eps_inverse", C.c_void_p(b.data_ptr()), int(p), self.nom, C.c_void_p(w.data_ptr()),
-                      w.numel(), None, s)
+        main = torch.cuda.current_stream(self.device)
+        ns = max(1, min(int(streams), self.nq))
+        if ns == 1:
+            side = [main]
+        else:
+            if len(self._side) < ns:
+                self._side += [torch.cuda.Stream(device=self.device) for _ in range(ns - len(self._side))]
+            side = self._side[:ns]
+            for st in side:
+                st.wait_stream(main)
+        order = np.argsort(-self.pb, kind="stable")
+        for n, q in enumerate(order):
+            st = side[n % ns]
+            b, p, w = self.eps_buf[q], int(self.pb[q]), self.ws_inv[q]
+            _lib.call("pygw_eps_inverse", C.c_void_p(b.data_ptr()), p, self.nom, C.c_void_p(w.data_ptr()),
+                      w.numel(), None, C.c_void_p(st.cuda_stream))
+        if ns > 1:
+            for st in side:
+                main.wait_stream(st)
         return self.eps
 
     def autotune(self):

@@ -98,3 +98,36 @@ def test_batch_inverse_matches_eps_inverse():
         assert (torch.linalg.norm(a - c) / torch.linalg.norm(c)).item() < 1e-12
     B.close()
     torch.cuda.synchronize()
+
+
+def test_batch_inverse_streams_graph():
+    """eps^-1 of every q dealt to side streams: eager and graph replays (with
+    eager work between them) equal the one-stream inversion bit for bit."""
+    import torch
+    from pygw_amd import polarization as P
+    batch, Ms, kcws = _batch_inputs(n_q=4)
+    print("case batch inverse streams graph", flush=True)
+    B = P.Chi0Batch(Ms, kcws, inverse=True)
+    chi = [e.clone() for e in B.run(inverse=False)]
+    one = [e.clone() for e in B.invert(streams=1)]
+    for e, c in zip(B.eps, chi):
+        e.copy_(c)
+    four = [e.clone() for e in B.invert(streams=4)]
+    for a, b in zip(four, one):
+        assert torch.equal(a, b)
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        B.invert(streams=4)
+    for rep in range(3):
+        for e, c in zip(B.eps, chi):
+            e.copy_(c)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(B.eps, one):
+            assert torch.equal(a, b), rep
+        B.run(inverse=False)   # eager work between the replays
+        torch.cuda.synchronize()
+    B.close()
+    torch.cuda.synchronize()
