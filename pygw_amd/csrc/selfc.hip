@@ -143,7 +143,10 @@ __global__ void __launch_bounds__(256) pygw_selfc_conv(const double* __restrict_
           const double2 mn = mwm[(((size_t)k * nom + i0 + g) * nb1 + ie1) * nb2 + ie2];
           const double2 num = double2{(ml.x - mn.x) * wl, (ml.y - mn.y) * wl};
           const double2 den = double2{o2 + (e * e - wn * wn), -2.0 * e * wn};  // omega_l^2 + (e - i w_n)^2
-          s0[g] = cadd(s0[g], cdiv(num, den));
+          // num / den as num * conj(den) / |den|^2: one real division instead
+          // of Smith's two (|den|^2 vanishes only where den does)
+          const double r = 1.0 / (den.x * den.x + den.y * den.y);
+          s0[g] = cadd(s0[g], double2{(num.x * den.x + num.y * den.y) * r, (num.y * den.x - num.x * den.y) * r});
         }
       }
     }
