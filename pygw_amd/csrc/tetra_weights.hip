@@ -14,12 +14,15 @@
 //     pairs of one tetrahedron: the four corner weights over all Matsubara
 //     frequencies (the reference's w1t(4, nom) of one convw1t_par call).  For
 //     the whole tetrahedron (no E_F plane inside) each corner weight is one
-//     vertex's closed form on the tabulated atan / log, used only where the
-//     vertex energy equals the table's bit for bit; polyhedra cut by E_F,
-//     degenerate classes and table misses are compacted into a list and
-//     evaluated by a separate pass, one lane per (pair, corner), on the
-//     general path, so they do not stall the wavefronts of whole tetrahedra;
-//   * pass 2, one lane per (k, band pair): the 24 (tetrahedron, corner)
+//     vertex's closed form.  sorteq classes 16, 8 and 10 (nearly equal
+//     corner Delta's: most tetrahedra on a fine mesh) share their averaged
+//     vertex energies, so one frequency costs at most two atan / log for all
+//     four corners; classes 4 and 6 are compacted into a list evaluated per
+//     (pair, corner) on the tabulated atan / log (used only where the vertex
+//     energy equals the table's bit for bit), and polyhedra cut by E_F into
+//     another list evaluated by four cooperating lanes per pair on the
+//     general path, so neither stalls the wavefronts of whole tetrahedra;
+//   * pass 2, one lane per (k, frequency, band pair): the 24 (tetrahedron, corner)
 //     incidences of k summed in increasing tetrahedron order -- the order in
 //     which the reference scatters into kcw(:,:,k,:), so the FP64 sums
 //     associate identically; writes coalesced across the wavefront (pair
@@ -980,18 +983,26 @@ __global__ void __launch_bounds__(64) pygw_tw_incidence(const int* __restrict__ 
   if (lane == 0) cnt[k] = anybad ? maxinc + 1 : c;  // > maxinc: malformed tetc, nothing is read
 }
 
-// Two passes per chunk of band pairs (pair index q in the output layout's
-// order: layout 0 q = i_o + n_o*i_e, layout 1 q = i_o*n_e + i_e):
+// Per chunk of band pairs (pair index q in the output layout's order:
+// layout 0 q = i_o + n_o*i_e, layout 1 q = i_o*n_e + i_e; buffers at pitch
+// pp >= the chunk's pcn pairs):
+//   pygw_tw_table    one lane per (k, pair): Delta and its atan / log over
+//                    all frequencies;
 //   pygw_tw_tet      one lane per (tetrahedron, pair): the four corner
-//                    weights of the pair over all frequencies into
-//                    w1t[T][corner][w][q - q0] (convw1t_par's w1t(4, nom)),
-//                    zeros where no band crosses E_F in T; pairs whose
-//                    polyhedron is cut by E_F go to a list for
+//                    weights over all frequencies into
+//                    w1t[T][corner][w][q - q0] (convw1t_par's w1t(4, nom))
+//                    for whole tetrahedra of classes 16, 8, 10, zeros where
+//                    no band crosses E_F in T; whole tetrahedra of classes
+//                    4 and 6 go to one list, polyhedra cut by E_F (and
+//                    classification errors) to another;
+//   pygw_tw_whole    one lane per (listed pair, corner), classes 4 and 6
+//                    from the table;
 //   pygw_tw_general  four lanes per listed pair (one per corner, running the
 //                    general polyhedron decomposition together);
-//   pygw_tw_sum      one lane per (k, pair): kcw = sum over the incidences
-//                    of k in increasing tetrahedron order of w1t * 6/N_tet,
-//                    the scatter order of for_tetrahedra.f90:445-456.
+//   pygw_tw_sum      one lane per (k, frequency, pair): kcw = sum over the
+//                    incidences of k in increasing tetrahedron order of
+//                    w1t * 6/N_tet, the scatter order of
+//                    for_tetrahedra.f90:445-456.
 __device__ __forceinline__ void tw_pair(int q, int layout, int no, int ne, int& ib, int& jb) {
   if (layout == 0) { ib = q % no; jb = q / no; }
   else { ib = q / ne; jb = q % ne; }
