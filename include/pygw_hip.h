@@ -211,6 +211,44 @@ int pygw_synth_minm_pairs(const double* enk, int n_bnd, const int* kqid, int row
                           int n_e, int pb, const double* vhalf, unsigned long long seed, const int* pairs,
                           long n_pairs, int complex128, void* out, void* stream);
 
+/* ------------------------------------------------------------------------
+ * q = 0 (Gamma point) head and wings.
+ *
+ * Wings: the iq == 0 branch of calc_eps, wtmp = dot(tmat, pm) added to
+ * epsw1[:, iom] (pygw.py:3177-3193):
+ *   epsw1[I, w] (+)= sum_k sum_p coef*kcw[k][w][p] * M[k][p][I] * pm[k][p]
+ *   M as in pygw_chi0_accumulate (complex64 = 1 for complex64 M, else
+ *   complex128), kcw kernel-native [N_k][n_om][n_pairs] read from k-point k0,
+ *   pm complex128 [n_k][n_pairs], epsw1 complex128 Fortran [pb, n_om]
+ *   (beta = 0 overwrites, 1 accumulates).  NULL workspace: allocated.
+ * ------------------------------------------------------------------------ */
+int pygw_chi0_wings_workspace(int pb, int n_om, int n_k, size_t* bytes);
+int pygw_chi0_wings(const void* M, long stride_k, int n_pairs, int pb, int complex64, const double* kcw, int n_om,
+                    int k0, int n_k, double coef, const void* pm, void* epsw1, int beta, void* workspace,
+                    size_t ws_bytes, void* stream);
+
+/* Inversion with the head/wings update of iq == 0 (pygw.py:3202-3263):
+ * as pygw_eps_inverse, and between the inverse A and the final -1:
+ * epsw2 = conj(epsw1); bw1 = A epsw1; w2b = conj(A epsw2);
+ * ww = dot(epsw2, bw1); emac[w] = (1 + head - ww, 1 + head);
+ * h = 1/(1 + head - ww); epsw1 = -h bw1; epsw2 = -h w2b;
+ * A += h bw1 w2b^T; head = h - 1.
+ *   head complex128 [n_om] (in/out), epsw1 Fortran [pb, n_om] (in/out),
+ *   epsw2 (out, same shape), emac complex128 [n_om][2] (out). */
+int pygw_eps_inverse_head_workspace(int pb, int n_om, size_t* bytes);
+int pygw_eps_inverse_head(void* eps, int pb, int n_om, void* head, void* epsw1, void* epsw2, void* emac,
+                          void* workspace, size_t ws_bytes, int* info, void* stream);
+
+/* q = 0 terms of calc_selfc (pygw.py:3359-3364), added to the MWM output of
+ * pygw_mwm (mwm [n_k][n_om][nb1][nb2]) at (b, b + ncg_c) for b < nb1:
+ *   coefs2*head[w] + coefs1*sum_I M[b, b+ncg_c, I] epsw2[I, w]
+ *                  + coefs1*sum_I conj(M[b, b+ncg_c, I]) epsw1[I, w]
+ * M_selfe as in pygw_mwm ([n_k][nb1*nb2][pb], complex64 or complex128);
+ * coefs1 = singc1*sqrt(4 pi/Vol), coefs2 = singc2*4 pi/Vol (pygw.py:3310-3311). */
+int pygw_mwm_head(const void* M_selfe, int n_k, int nb1, int nb2, int pb, int ncg_c, int complex64, const void* head,
+                  const void* epsw1, const void* epsw2, int n_om, double coefs1, double coefs2, void* mwm,
+                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,8 +2,12 @@
 contractions and the self-energy tail of PyGW's polarization path.
 
   calc_eps_accumulate  chi0 contraction of calc_eps      (pygw.py:3161-3200)
+  calc_eps_wings       q = 0 wing of calc_eps            (pygw.py:3177-3193)
   eps_inverse          hermitize, +1, inverse, -1        (pygw.py:3229-3271)
+  eps_inverse_head     the same with the q = 0 head/wings update
+                                                         (pygw.py:3202-3263)
   calc_mwm             MWM of calc_selfc                 (pygw.py:3329-3366)
+  calc_mwm_head        q = 0 head/wing terms of MWM      (pygw.py:3310-3311,3359-3364)
   fr_convolution2 / all_fr_convolutions / few_fr_convolutions
                                                          (fnc.f90:86-176)
   selfc_spline         spline branch of Compute_selfc_inside
@@ -40,6 +44,54 @@ def calc_eps_accumulate(minm_k, kcw_k, coef=-2.0):
     return eps
 
 
+def calc_eps_wings(minm_k, kcw_k, pm_k, coef=-2.0):
+    """q = 0 wing: epsw1[I, iom] = sum_k dot(tmat, pm) with
+    tmat = minm2.T*coef*kcw (pygw.py:3177-3193).  pm_k(ik) -> [N_p]
+    momentum matrix elements in the pair order of minm2."""
+    epsw1 = None
+    for ik, m in minm_k:
+        mTc = m.T * coef
+        w = kcw_k(ik)
+        pm = pm_k(ik)
+        if epsw1 is None:
+            epsw1 = np.zeros((m.shape[1], w.shape[1]), dtype=complex)
+        for iom in range(w.shape[1]):
+            tmat = mTc * w[:, iom]
+            epsw1[:, iom] += np.dot(tmat, pm)
+    return epsw1
+
+
+def eps_inverse_head(eps, head, epsw1):
+    """q = 0 inversion (pygw.py:3202-3263): epsw2 = conj(epsw1); per
+    frequency hermitize, +1, inverse A, then
+      bw1 = A epsw1, w2b = conj(A epsw2), ww = dot(epsw2, bw1),
+      emac = (1 + head - ww, 1 + head), h = 1/(1 + head - ww),
+      epsw1 = -h bw1, epsw2 = -h w2b, A += h outer(bw1, w2b), head = h - 1,
+    and -1.  Returns (eps, head, epsw1, epsw2, emac)."""
+    out = np.empty_like(eps)
+    head = np.array(head, dtype=complex)
+    epsw1 = np.array(epsw1, dtype=complex)
+    epsw2 = np.conj(epsw1)
+    nom = eps.shape[2]
+    emac = np.zeros((nom, 2), dtype=complex)
+    ident = np.eye(eps.shape[0])
+    for iom in range(nom):
+        a = (eps[:, :, iom] + np.conj(eps[:, :, iom].T)) / 2.0 + ident
+        a = np.linalg.inv(a)
+        bw1 = np.dot(a, epsw1[:, iom])
+        w2b = np.conj(np.dot(a, epsw2[:, iom]))
+        ww = np.dot(epsw2[:, iom], bw1)
+        emac[iom, 0] = 1.0 + head[iom] - ww
+        emac[iom, 1] = 1.0 + head[iom]
+        h = 1.0 / (1.0 + head[iom] - ww)
+        epsw1[:, iom] = -h * bw1
+        epsw2[:, iom] = -h * w2b
+        a = a + h * np.multiply.outer(bw1, w2b)
+        head[iom] = h - 1.0
+        out[:, :, iom] = a - ident
+    return out, head, epsw1, epsw2, emac
+
+
 def eps_inverse(eps):
     """(1 + eps)^-1 - 1 per frequency after hermitizing (pygw.py:3229-3271,
     q != 0: no head/wings)."""
@@ -62,6 +114,23 @@ def calc_mwm(minm2, w, wkq):
         mw = cm @ np.ascontiguousarray(w[:, :, iom])  # a strided slice would miss BLAS
         out[iom] = wkq * np.sum(mw * minm2, axis=-1)
     return out
+
+
+def calc_mwm_head(minm, mwm, head, epsw1, epsw2, coefs1, coefs2, ncg_c):
+    """q = 0 terms of calc_selfc (pygw.py:3359-3364), added in place to
+    mwm[iom, ib1, ib1 + ncg_c] for every ib1 (ie1 - ibgw):
+      coefs2 head[iom] + coefs1 dot(minm[ib1, ib1+ncg_c, :], epsw2[:, iom])
+                       + coefs1 vdot(minm[ib1, ib1+ncg_c, :], epsw1[:, iom]).
+    minm: [nb1, nb2, PB]; coefs1 = singc1 sqrt(4 pi/Vol), coefs2 =
+    singc2 4 pi/Vol (pygw.py:3310-3311)."""
+    nb1 = minm.shape[0]
+    for iom in range(mwm.shape[0]):
+        for ib1 in range(nb1):
+            ib2 = ib1 + ncg_c
+            m = minm[ib1, ib2, :]
+            mwm[iom, ib1, ib2] += (coefs2 * head[iom] + coefs1 * np.dot(m, epsw2[:, iom])
+                                   + coefs1 * np.vdot(m, epsw1[:, iom]))
+    return mwm
 
 
 def fr_convolution2(enk, mwm, mwm_iom, omega, omega_iom, womeg):

@@ -239,7 +239,8 @@ __global__ void __launch_bounds__(256) pygw_ei_update_mfma(double2* __restrict__
       }
 }
 
-int eps_inverse_launch(void* eps, int n, int nom, void* ws, int* dinfo, hipStream_t stream) {
+// (eps + eps^H)/2 + 1, inverted in place (the -1 is left to the caller)
+int eps_inverse_gj(void* eps, int n, int nom, void* ws, int* dinfo, hipStream_t stream) {
   using namespace ei;
   double2* A = (double2*)eps;
   double2* B = (double2*)ws;
@@ -253,8 +254,18 @@ int eps_inverse_launch(void* eps, int n, int nom, void* ws, int* dinfo, hipStrea
     hipLaunchKernelGGL(pygw_ei_update_mfma, dim3(ceil_div(n, 64), ceil_div(n, 64), nom), dim3(256), 0, stream, A, n,
                        p0, nb, B, U, L);
   }
-  hipLaunchKernelGGL(pygw_ei_minus_identity, dim3(ceil_div((long)n * nom, 256)), dim3(256), 0, stream, A, n, nom);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+}
+
+int eps_minus_identity(void* eps, int n, int nom, hipStream_t stream) {
+  hipLaunchKernelGGL(pygw_ei_minus_identity, dim3(ceil_div((long)n * nom, 256)), dim3(256), 0, stream, (double2*)eps, n,
+                     nom);
+  return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+}
+
+int eps_inverse_launch(void* eps, int n, int nom, void* ws, int* dinfo, hipStream_t stream) {
+  const int rc = eps_inverse_gj(eps, n, nom, ws, dinfo, stream);
+  return rc == PYGW_OK ? eps_minus_identity(eps, n, nom, stream) : rc;
 }
 
 size_t eps_inverse_ws_bytes(int n, int nom) {

@@ -162,6 +162,102 @@ def calc_mwm(M_selfe, W, wkq, *, precision="fp64", device="cuda"):
     return out
 
 
+def _fortran_vec(t, pb, nom):
+    """A Fortran-strided [PB, n_om] complex128 view of t (any layout accepted
+    as input; returns a new tensor when a copy is needed)."""
+    if t.shape == (pb, nom) and t.dtype == torch.complex128 and t.stride() == (1, pb):
+        return t
+    buf = torch.empty(pb * nom, dtype=torch.complex128, device=t.device)
+    v = buf.as_strided((pb, nom), (1, pb))
+    v.copy_(t)
+    return v
+
+
+def chi0_wings(M, kcw_native, pm, coef=-2.0, *, epsw1=None, k0=0, device="cuda"):
+    """q = 0 wing of calc_eps (pygw.py:3177-3193): epsw1[I, iom] (+)=
+    sum_k sum_p coef*kcw[k,iom,p]*M[k,p,I]*pm[k,p], i.e. PyGW's
+    dot(tmat, pm) summed over k.  M [n_k, n_pairs, PB] complex128 or
+    complex64, kcw_native [N_k, n_om, n_pairs] read from k-point k0, pm
+    [n_k, n_pairs] complex.  Returns (or accumulates into) epsw1, a
+    Fortran-strided [PB, n_om] complex128 tensor."""
+    device = _require_device(device)
+    if not isinstance(M, torch.Tensor):
+        M = torch.as_tensor(np.ascontiguousarray(M), device=device)
+    M = M.to(device)
+    if M.dtype not in (torch.complex64, torch.complex128):
+        M = M.to(torch.complex128)
+    M = M.contiguous()
+    nk, npair, pb = M.shape
+    kc = _dev(kcw_native, torch.float64, device).contiguous()
+    nom = kc.shape[1]
+    if kc.shape[2] != npair or kc.shape[0] < k0 + nk:
+        raise ValueError("kcw_native must be [N_k >= k0 + n_k, n_om, n_pairs]")
+    pmt = _dev(pm, torch.complex128, device).contiguous()
+    if pmt.shape != (nk, npair):
+        raise ValueError("pm must be [n_k, n_pairs]")
+    beta = 1
+    if epsw1 is None:
+        epsw1 = torch.empty(pb * nom, dtype=torch.complex128, device=device).as_strided((pb, nom), (1, pb))
+        beta = 0
+    elif epsw1.shape != (pb, nom) or epsw1.stride() != (1, pb) or epsw1.dtype != torch.complex128:
+        raise ValueError("epsw1 must be a Fortran-strided [PB, n_om] complex128 tensor")
+    sz = C.c_size_t(0)
+    _lib.call("pygw_chi0_wings_workspace", pb, nom, nk, C.byref(sz))
+    ws = torch.empty(sz.value, dtype=torch.uint8, device=device)
+    _lib.call("pygw_chi0_wings", C.c_void_p(M.data_ptr()), npair * pb, npair, pb,
+              1 if M.dtype == torch.complex64 else 0, C.c_void_p(kc.data_ptr()), nom, int(k0), nk, float(coef),
+              C.c_void_p(pmt.data_ptr()), C.c_void_p(epsw1.data_ptr()), beta, C.c_void_p(ws.data_ptr()), sz.value,
+              _stream_ptr(device))
+    return epsw1
+
+
+def eps_inverse_head(eps, head, epsw1, *, device="cuda", check=True):
+    """q = 0 inversion (pygw.py:3202-3263): eps <- ((eps+eps^H)/2 + 1)^-1 - 1
+    with the head/wings update between the inverse and the -1.  eps is
+    Fortran [PB, PB, n_om] complex128 (in place); head [n_om] and epsw1
+    [PB, n_om] are updated.  Returns (eps, head, epsw1, epsw2, emac) with
+    emac [n_om, 2] = (1 + head - ww, 1 + head) of the incoming head."""
+    device = _require_device(device)
+    pb, pb2, nom = eps.shape
+    if pb != pb2 or eps.stride() != (1, pb, pb * pb) or eps.dtype != torch.complex128:
+        raise ValueError("eps must be a Fortran-strided [PB, PB, n_om] complex128 tensor")
+    h = _dev(head, torch.complex128, device).contiguous().clone()
+    if h.shape != (nom,):
+        raise ValueError("head must be [n_om]")
+    w1 = _fortran_vec(_dev(epsw1, torch.complex128, device), pb, nom)
+    w2 = torch.empty(pb * nom, dtype=torch.complex128, device=device).as_strided((pb, nom), (1, pb))
+    emac = torch.empty((nom, 2), dtype=torch.complex128, device=device)
+    info = (C.c_int * 1)()
+    _lib.call("pygw_eps_inverse_head", C.c_void_p(eps.data_ptr()), pb, nom, C.c_void_p(h.data_ptr()),
+              C.c_void_p(w1.data_ptr()), C.c_void_p(w2.data_ptr()), C.c_void_p(emac.data_ptr()), None, 0,
+              C.cast(info, C.c_void_p) if check else None, _stream_ptr(device))
+    return eps, h, w1, w2, emac
+
+
+def calc_mwm_head(M_selfe, mwm, head, epsw1, epsw2, coefs1, coefs2, ncg_c, nb1, nb2, *, device="cuda"):
+    """q = 0 terms of calc_selfc (pygw.py:3359-3364) added in place to mwm
+    (the [n_k, n_om, nb1*nb2] output of calc_mwm) at the rows
+    (b, b + ncg_c): coefs2*head + coefs1*M.epsw2 + coefs1*conj(M).epsw1."""
+    device = _require_device(device)
+    M = M_selfe if isinstance(M_selfe, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(M_selfe))
+    M = M.to(device)
+    if M.dtype not in (torch.complex64, torch.complex128):
+        M = M.to(torch.complex128)
+    M = M.contiguous()
+    nk, rpk, pb = M.shape
+    nom = mwm.shape[1]
+    if rpk != nb1 * nb2 or mwm.shape != (nk, nom, rpk) or not mwm.is_contiguous() or mwm.dtype != torch.complex128:
+        raise ValueError("mwm must be the contiguous [n_k, n_om, nb1*nb2] complex128 output of calc_mwm")
+    hd = _dev(head, torch.complex128, device).contiguous()
+    w1 = _fortran_vec(_dev(epsw1, torch.complex128, device), pb, nom)
+    w2 = _fortran_vec(_dev(epsw2, torch.complex128, device), pb, nom)
+    _lib.call("pygw_mwm_head", C.c_void_p(M.data_ptr()), nk, nb1, nb2, pb, int(ncg_c),
+              1 if M.dtype == torch.complex64 else 0, C.c_void_p(hd.data_ptr()), C.c_void_p(w1.data_ptr()),
+              C.c_void_p(w2.data_ptr()), nom, float(coefs1), float(coefs2), C.c_void_p(mwm.data_ptr()),
+              _stream_ptr(device))
+    return mwm
+
+
 def compute_selfc_inside(enk, mwm, omega, womeg, omega_p=None, womeg_p=None, *, device="cuda"):
     """Sigma_c contribution of one q for all k (Compute_selfc_inside,
     mcommon.py:80-165).  enk [n_k, nb2] (energies at k-q), mwm complex

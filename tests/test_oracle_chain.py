@@ -102,3 +102,83 @@ def test_tan_mesh_solves_its_equations():
     xt = np.linspace(0.0, 1.0, 2 * nw) * (np.pi - 2 * d) - np.pi / 2 + d
     np.testing.assert_allclose(om, w * np.tan(xt), rtol=1e-10, atol=1e-14)
     assert abs(om[-1] - L) < 1e-9 * L
+
+
+def _herm_pd(rng, n, nom, scale=0.3):
+    x = rng.normal(size=(nom, n, 2 * n)) + 1j * rng.normal(size=(nom, n, 2 * n))
+    e = np.einsum("wik,wjk->ijw", x, np.conj(x)) * scale / n
+    return e
+
+
+def test_wings_are_the_extra_column_of_chi0():
+    """The q = 0 wing equals the extra column of the chi0 contraction whose
+    M carries conj(pm) as one more product-basis function."""
+    rng = np.random.default_rng(11)
+    nk, npair, pb, nom = 3, 7, 9, 4
+    M = rng.normal(size=(nk, npair, pb)) + 1j * rng.normal(size=(nk, npair, pb))
+    pm = rng.normal(size=(nk, npair)) + 1j * rng.normal(size=(nk, npair))
+    kcw = -np.abs(rng.normal(size=(nk, npair, nom)))
+    w1 = chain.calc_eps_wings(((k, M[k]) for k in range(nk)), lambda k: kcw[k], lambda k: pm[k])
+    Mx = np.concatenate([M, np.conj(pm)[:, :, None]], axis=2)
+    ex = chain.calc_eps_accumulate(((k, Mx[k]) for k in range(nk)), lambda k: kcw[k])
+    np.testing.assert_allclose(w1, ex[:pb, pb, :], rtol=0, atol=1e-12 * np.abs(w1).max())
+
+
+def test_head_update_is_the_bordered_inverse_for_real_wings():
+    """With real wings the q = 0 update of pygw.py:3246-3263 is the block
+    inverse of the bordered matrix [[1 + head, epsw2^T], [epsw1, E]]
+    (E = hermitized eps + 1): head + 1, both wings, the body and emac."""
+    rng = np.random.default_rng(12)
+    pb, nom = 10, 3
+    eps = _herm_pd(rng, pb, nom)
+    head = 0.5 + rng.normal(size=nom) + 0.2j * rng.normal(size=nom)
+    w1 = 0.3 * rng.normal(size=(pb, nom)) + 0j
+    out, h, e1, e2, emac = chain.eps_inverse_head(eps, head, w1)
+    for iom in range(nom):
+        E = (eps[:, :, iom] + np.conj(eps[:, :, iom].T)) / 2 + np.eye(pb)
+        B = np.zeros((pb + 1, pb + 1), dtype=complex)
+        B[0, 0] = 1 + head[iom]
+        B[0, 1:] = np.conj(w1[:, iom])
+        B[1:, 0] = w1[:, iom]
+        B[1:, 1:] = E
+        Bi = np.linalg.inv(B)
+        np.testing.assert_allclose(h[iom] + 1, Bi[0, 0], rtol=1e-12)
+        np.testing.assert_allclose(e1[:, iom], Bi[1:, 0], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(e2[:, iom], Bi[0, 1:], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(out[:, :, iom] + np.eye(pb), Bi[1:, 1:], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(emac[iom, 0], 1 / Bi[0, 0], rtol=1e-12)
+    # complex wings: the reference's row wing conj(A epsw2) is not the block
+    # inverse's (A^T epsw2), so the update departs from it (kept as PyGW has it)
+    w1c = w1 + 0.3j * rng.normal(size=(pb, nom))
+    _, _, _, e2c, _ = chain.eps_inverse_head(eps, head, w1c)
+    E = (eps[:, :, 0] + np.conj(eps[:, :, 0].T)) / 2 + np.eye(pb)
+    B = np.block([[np.array([[1 + head[0]]]), np.conj(w1c[:, :1]).T], [w1c[:, :1], E]])
+    assert np.abs(e2c[:, 0] - np.linalg.inv(B)[0, 1:]).max() > 1e-3
+
+
+def test_mwm_head_terms_are_the_bordered_product():
+    """With coefs2 = coefs1^2 and wkq = 1 the q = 0 MWM terms complete
+    conj(Mf) Wf Mf over the bordered basis: Mf = [coefs1, M],
+    Wf = [[head, epsw2^T], [epsw1, W]] (pygw.py:3359-3364)."""
+    rng = np.random.default_rng(13)
+    nb1, ncg, nb2v, pb, nom = 3, 2, 4, 6, 2
+    nb2 = ncg + nb2v
+    minm = rng.normal(size=(nb1, nb2, pb)) + 1j * rng.normal(size=(nb1, nb2, pb))
+    W = rng.normal(size=(pb, pb, nom)) + 1j * rng.normal(size=(pb, pb, nom))
+    head = rng.normal(size=nom) + 1j * rng.normal(size=nom)
+    e1 = rng.normal(size=(pb, nom)) + 1j * rng.normal(size=(pb, nom))
+    e2 = rng.normal(size=(pb, nom)) + 1j * rng.normal(size=(pb, nom))
+    c1 = 0.7
+    mwm = chain.calc_mwm(minm.reshape(nb1 * nb2, pb), W, 1.0).reshape(nom, nb1, nb2)
+    chain.calc_mwm_head(minm, mwm, head, e1, e2, c1, c1 * c1, ncg)
+    for iom in range(nom):
+        Wf = np.zeros((pb + 1, pb + 1), dtype=complex)
+        Wf[0, 0] = head[iom]
+        Wf[0, 1:] = e2[:, iom]
+        Wf[1:, 0] = e1[:, iom]
+        Wf[1:, 1:] = W[:, :, iom]
+        for ib1 in range(nb1):
+            for ib2 in range(nb2):
+                mf = np.concatenate([[c1 if ib2 == ib1 + ncg else 0.0], minm[ib1, ib2]])
+                want = np.conj(mf) @ Wf @ mf
+                np.testing.assert_allclose(mwm[iom, ib1, ib2], want, rtol=1e-12)
