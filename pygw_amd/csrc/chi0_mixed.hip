@@ -1,4 +1,4 @@
-// Split-precision chi0 contraction on gfx950 MFMA (v_mfma_f32_16x16x32_f16).
+// Split-precision chi0 contraction on gfx950 MFMA (v_mfma_f32_32x32x16_f16).
 //
 //   eps[I,J,w] += sum_k sum_p coef*kcw[k,w,p] * M[k,p,I] * conj(M[k,p,J])
 //
@@ -10,8 +10,9 @@
 // product uses three MFMA terms a_hi b_hi + a_hi b_lo + a_lo b_hi.
 //   I side:  a = s_I M[p,I], split once per call by pygw_chi0_presplit into
 //            four FP16 planes in MFMA fragment order (frequency independent).
-//   J side:  b = (s_J M[p,J]) (w_p t_w), weighted and split per frequency
-//            while it is staged into LDS (one block = one frequency).
+//   column side: b = (s_J M[p,J]) (w_p t_w) for a column n = (J, w) of the
+//            merged column-frequency index, weighted and split while it is
+//            staged into LDS.
 //   Re += a_r b_r + a_i b_i,   Im += a_i b_r + (-a_r) b_i   (12 MFMAs per
 //   32x32 tile and 16 pairs on v_mfma_f32_32x32x16_f16; -a_r by a sign flip
 //   of the FP16 planes).  The 32x32 shape keeps the MFMA pipe full with the
@@ -23,15 +24,31 @@
 // unscaled by 1/(s_I s_J t_w) (exact) and added into the FP64 eps in memory,
 // so the long K = N_k*N_p reduction is carried in FP64.
 //
-// Tiling.  512 threads = 8 waves per block, block tile 128 (I) x 128 (J),
-// wave tile 64 x 32 (2 x 4 waves; two 32x32 MFMA tiles), two waves per SIMD.  LDS holds the staged
-// I and J slabs of the current and the next chunk (2 x 2 x 32 KB); the next
-// chunk's global loads are issued before the current chunk's MFMAs.  Only
-// tile pairs I_t <= J_t are accumulated, stored conjugated at the transposed
-// place (coalesced); pygw_chi0_mirror fills the rest.  The
-// block -> (tile pair, frequency) map keeps all frequencies of one tile pair
-// on one XCD (blocks are dealt to the 8 XCDs round robin), so the slabs they
-// share are read from HBM once per XCD and then served from its L2.
+// Work items.  Rows I come in tiles of 128.  For I tile It the columns are
+// the merged list m = 0 .. (PB - 128 It) nom - 1 of (J, w) = (128 It + m/nom,
+// m % nom): every frequency of every column J on or right of the tile's
+// diagonal block.  One work item = one I tile x 128 consecutive merged
+// columns, so the padding is confined to the last I tile and the last column
+// chunk of each I tile (at PB 900, 64 frequencies: 1.14x the Hermitian area,
+// against 1.46x for 128 x 128 (I, J) tile pairs one frequency at a time).
+// The result is eps[I, J, w] for J >= 128 It; pygw_chi0_mirror fills the
+// tiles below the block diagonal.
+//
+// Tiling.  512 threads = 8 waves per block, block tile 128 (merged columns,
+// the MFMA rows) x 128 (I, the MFMA columns: the FP64 folds of a half-wave
+// touch 512 contiguous bytes of the column-major eps), wave tile 64 x 32
+// (2 x 4 waves; two 32x32 MFMA tiles), two waves per SIMD.  LDS holds the
+// staged I and column slabs of the current and the next chunk (2 x 2 x
+// 32 KB); the next chunk's global loads are issued before the current
+// chunk's MFMAs.  Blocks are dealt to the 8 XCDs round robin and each XCD
+// takes consecutive work items (same I tile), so the I slab they share is
+// read from HBM once per XCD and then served from its L2.
+//
+// Rounds.  All items of one call cost the same, so a launch of n items on B
+// CUs (one block per CU) runs floor(n/B) full rounds and a partial one.  The
+// r items of the partial round are each cut into S = B/r parts along the
+// pairs; a part sums into its own FP64 slot and pygw_chi0_tailfix adds the
+// slots of an item in part order (deterministic).
 #include "common.h"
 
 #include <algorithm>
@@ -52,15 +69,20 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
 namespace cm {
-constexpr int TB = 128;  // block tile edge (I and J)
+constexpr int TB = 128;  // tile edge: I rows, merged columns
 constexpr int KC = 32;   // pairs per chunk = MFMA K
 constexpr int NT = 512;  // threads per block
+constexpr int kSlots = 256;  // FP64 part slots for the split items of a launch
 
-__device__ __forceinline__ void tile_pair(int t, int T, int& It, int& Jt) {
+// merged columns of I tile It and their 128-wide chunks
+__host__ __device__ inline int ncols(int pb, int nom, int It) { return (pb - It * TB) * nom; }
+__host__ __device__ inline int ncchunks(int pb, int nom, int It) { return (ncols(pb, nom, It) + TB - 1) / TB; }
+// work item t of the It-major list -> (It, Nc)
+__device__ inline void item_of(int t, int pb, int nom, int& It, int& Nc) {
   int i = 0;
-  while (t >= T - i) { t -= T - i; i++; }
+  for (int n = ncchunks(pb, nom, 0); t >= n; n = ncchunks(pb, nom, ++i)) t -= n;
   It = i;
-  Jt = i + t;
+  Nc = t;
 }
 
 __device__ __forceinline__ h8 neg(h8 x) {
@@ -77,7 +99,6 @@ __device__ __forceinline__ void split(float x, _Float16& hi, _Float16& lo) {
   lo = (_Float16)(x - (float)hi);
 }
 }  // namespace cm
-
 // Fragment planes.  For k-point k (of the slab), chunk c, group g = 0..3 and
 // plane q (0 re_hi, 1 re_lo, 2 im_hi, 3 im_lo), column I holds the 8 FP16
 // values of pairs p = 32c + 8g + j, j = 0..7 (16 bytes):
@@ -121,30 +142,39 @@ __global__ void __launch_bounds__(256) pygw_chi0_presplit(const float2* __restri
 // probed on gfx950):
 //   A[i][k]: i = l%32, k = 8*(l/32) + j;  B[k][n]: n = l%32, k = 8*(l/32) + j;
 //   D[i][n] (16 values v): n = l%32, i = 8*(v/4) + 4*(l/32) + v%4.
-// S: fragment planes of this slab; M: the same slab as complex64 (J side);
-// wf: the slab's scaled weights coef*kcw*t_w in FP32, [k][w][npc*32] (zero
-// past N_p), from pygw_chi0_wprep.
-// One (tile pair, frequency) work item: eps[.., om] += the 128 x 128 tile
-// (It, Jt) of the contraction over all chunks of S / M / wf.  As and Bs are
-// the block's LDS buffers; every thread of the 512-thread block calls this
-// with the same arguments, and it ends on a block barrier.
-__device__ __forceinline__ void chi0_tile(const h8* __restrict__ S, const float2* __restrict__ M, long strideK,
-                                          int np, int pb, int pbp, int npc, int nkc, const float* __restrict__ wf,
-                                          int nom, const float* __restrict__ cs, const double* __restrict__ ws,
-                                          int G, int om, int It, int Jt, double2* __restrict__ eps,
+// A = the merged columns (rows of D), B = the I side (columns of D).
+// S: fragment planes of this slab; M: the same slab as complex64 (column
+// side); wt: the slab's scaled weights coef*kcw*t_w in FP32, [k][p][w]
+// (zero past N_p), from pygw_chi0_wprep.
+// One work item: rows I of tile It against the 128 merged columns
+// m = 128 Nc .. 128 Nc + 127 of tile It (see the file header), summed over
+// the chunks [ch0, ch1) of the slab (chunk ch = k*npc + c).  With part ==
+// nullptr the sum is added into eps[I, J, w]; otherwise it is written into
+// the part slot part[row][I - 128 It] (FP64, the first fold stores).  As and
+// Bs are the block's LDS buffers; every thread of the 512-thread block calls
+// this with the same arguments, and it ends on a block barrier.
+__device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2* __restrict__ M, long strideK,
+                                          int np, int pb, int pbp, int npc, long ch0, long ch1,
+                                          const float* __restrict__ wt, int nom, const float* __restrict__ cs,
+                                          const double* __restrict__ ws, int G, int It, int Nc,
+                                          double2* __restrict__ eps, double2* __restrict__ part,
                                           h8 (*As)[16][cm::TB], h8 (*Bs)[16][cm::TB]) {
   using namespace cm;
-  const int I0 = It * TB, J0 = Jt * TB;
+  const int I0 = It * TB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 2, wc = wave & 3;
-  const double tw = ws[om], itw = 1.0 / tw;  // exact: a power of two
-  const float* ics = cs + pb;                  // 1/cs, exact
+  const float* ics = cs + pb;  // 1/cs, exact
+  const int mcount = ncols(pb, nom, It);
 
-  // staging roles: A: 4 x 16 B per thread (contiguous copy of 16 planes x 128
-  // columns); B: column Jb = tid % 128, pair group gb = tid / 128 (8 pairs)
-  const int Jb = tid & 127, gb = tid >> 7;
-  const bool jin = (J0 + Jb) < pb;
-  const float sJ = jin ? cs[J0 + Jb] : 0.f;
+  // staging roles: A (I side): 4 x 16 B per thread (contiguous copy of 16
+  // planes x 128 columns); column side: merged column nb = tid % 128, pair
+  // group gb = tid / 128 (8 pairs)
+  const int nb = tid & 127, gb = tid >> 7;
+  const int mb = Nc * TB + nb;
+  const bool nin = mb < mcount;
+  const int jn = nin ? I0 + mb / nom : pb - 1;  // past the end: column PB-1 (finite), zeroed by sJ = 0
+  const int wn = mb % nom;
+  const float sJ = nin ? cs[jn] : 0.f;
   // one chunk's staged inputs, held in registers between its load and its
   // LDS store (two sets: the next chunk is stored while the one after it
   // loads)
@@ -153,18 +183,19 @@ __device__ __forceinline__ void chi0_tile(const h8* __restrict__ S, const float2
     float2 m[8];
     float w[8];
   };
-  const long nch = (long)nkc * npc;
+  const long nch = ch1 - ch0;
   // Loads go through buffer resources built from chunk-uniform bases (scalar
   // registers) with per-lane byte offsets that stay fixed across chunks, and
-  // out-of-range offsets read as zero: J-side pairs p >= N_p fall past the
-  // end of the k-point's M block.
+  // out-of-range offsets read as zero: pairs p >= N_p fall past the end of
+  // the k-point's M block.
   constexpr int RSRC_FLAGS = 0x00020000;
   const unsigned aoffB = (unsigned)((tid >> 7) * pbp + (tid & 127)) * 16u;  // + q * 4*pbp*16
   const unsigned aqB = 4u * (unsigned)pbp * 16u;
   const unsigned pb8 = (unsigned)pb * 8u;
-  // columns past PB read column PB-1 (finite) and are zeroed by sJ = 0
-  const unsigned moff0 = (unsigned)(8 * gb) * pb8 + (unsigned)(jin ? J0 + Jb : pb - 1) * 8u;
+  const unsigned moff0 = (unsigned)(8 * gb) * pb8 + (unsigned)jn * 8u;
   const int npad = npc * KC;
+  const unsigned nom4 = (unsigned)nom * 4u;
+  const unsigned woff0 = (unsigned)(8 * gb) * nom4 + (unsigned)wn * 4u;
   auto fetch = [&](Pre& P, int k, int c) {
     const h8* Sk = S + ((size_t)k * npc + c) * 16 * (size_t)pbp + I0;
     const __amdgpu_buffer_rsrc_t ra =
@@ -179,12 +210,10 @@ __device__ __forceinline__ void chi0_tile(const h8* __restrict__ S, const float2
     for (int j = 0; j < 8; j++)
       P.m[j] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rm, mo + j * pb8, 0, 0));
     const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(wf + ((size_t)k * nom + om) * (size_t)npad), (short)0, npad * 4, RSRC_FLAGS);
-    const unsigned wo = (unsigned)(32 * c + 8 * gb) * 4u;
-    const u4 w0 = __builtin_amdgcn_raw_buffer_load_b128(rw, wo, 0, 0);
-    const u4 w1 = __builtin_amdgcn_raw_buffer_load_b128(rw, wo + 16, 0, 0);
-    P.w[0] = __uint_as_float(w0.x); P.w[1] = __uint_as_float(w0.y); P.w[2] = __uint_as_float(w0.z); P.w[3] = __uint_as_float(w0.w);
-    P.w[4] = __uint_as_float(w1.x); P.w[5] = __uint_as_float(w1.y); P.w[6] = __uint_as_float(w1.z); P.w[7] = __uint_as_float(w1.w);
+        (void*)(wt + (size_t)k * npad * nom), (short)0, (int)((unsigned)npad * nom4), RSRC_FLAGS);
+    const unsigned wo = woff0 + (unsigned)(32 * c) * nom4;
+#pragma unroll
+    for (int j = 0; j < 8; j++) P.w[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rw, wo + j * nom4, 0, 0));
   };
   auto stage = [&](const Pre& P, int buf) {
 #pragma unroll
@@ -203,13 +232,14 @@ __device__ __forceinline__ void chi0_tile(const h8* __restrict__ S, const float2
       ih[j] = h;
       il[j] = l;
     }
-    Bs[buf][4 * gb + 0][Jb] = rh;
-    Bs[buf][4 * gb + 1][Jb] = rl;
-    Bs[buf][4 * gb + 2][Jb] = ih;
-    Bs[buf][4 * gb + 3][Jb] = il;
+    Bs[buf][4 * gb + 0][nb] = rh;
+    Bs[buf][4 * gb + 1][nb] = rl;
+    Bs[buf][4 * gb + 2][nb] = ih;
+    Bs[buf][4 * gb + 3][nb] = il;
   };
 
-  // accumulators: two 32x32 tiles (rows 64*wr + 32*a) of the wave's 64 x 32
+  // accumulators: two 32x32 tiles (merged rows 64*wr + 32*a) of the wave's
+  // 64 x 32
   f16v accR[2], accI[2];
   auto zero = [&]() {
 #pragma unroll
@@ -220,72 +250,83 @@ __device__ __forceinline__ void chi0_tile(const h8* __restrict__ S, const float2
         accI[a][v] = 0.f;
       }
   };
-  // The tile is added, conjugated, at its transposed place eps[J, I]: there
-  // the 32 lanes of a half-wave (consecutive J, one I) touch 512 contiguous
-  // bytes of the column-major eps.  pygw_chi0_mirror then fills the tiles
-  // above the diagonal from the ones below.
+  bool first = true;
   auto flush = [&]() {
     // opaque copies keep the compiler from hoisting the 32 element addresses
     // (and their scales) out of the chunk loop into live registers
-    int ib = I0 + 64 * wr + 4 * (lane >> 5), J = J0 + 32 * wc + (lane & 31);
-    asm volatile("" : "+v"(ib), "+v"(J));
-    if (J >= pb) return;
-    double2* E = eps + (size_t)pb * pb * om + J;
-    const double fj = (double)ics[J] * itw;
+    int I = I0 + 32 * wc + (lane & 31), rb = 64 * wr + 4 * (lane >> 5);
+    asm volatile("" : "+v"(I), "+v"(rb));
+    if (I < pb) {
+      const double fi = (double)ics[I];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+      for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int v = 0; v < 16; v++) {
-        const int I = ib + 32 * a + 8 * (v >> 2) + (v & 3);
-        if (I < pb) {
-          const double f = fj * (double)ics[I];  // exact: powers of two
-          double2* e = E + (size_t)pb * I;
-          const double2 o = *e;
-          *e = double2{o.x + (double)accR[a][v] * f, o.y - (double)accI[a][v] * f};
+        for (int v = 0; v < 16; v++) {
+          const int row = rb + 32 * a + 8 * (v >> 2) + (v & 3);
+          const int m = Nc * TB + row;
+          if (m < mcount) {
+            const int J = I0 + m / nom, w = m % nom;
+            const double f = fi * (double)ics[J] / ws[w];  // exact: powers of two
+            const double re = (double)accR[a][v] * f, im = (double)accI[a][v] * f;
+            if (part) {
+              double2* e = part + (size_t)row * TB + (I - I0);
+              if (first) {
+                *e = double2{re, im};
+              } else {
+                const double2 o = *e;
+                *e = double2{o.x + re, o.y + im};
+              }
+            } else {
+              double2* e = eps + ((size_t)pb * w + J) * (size_t)pb + I;
+              const double2 o = *e;
+              *e = double2{o.x + re, o.y + im};
+            }
+          }
         }
-      }
+    }
+    first = false;
   };
 
   // chunk `buf`'s products (LDS buffer buf), with the next chunk's inputs
-  // from S stored into the other buffer in the same basic block so that the
+  // stored into the other buffer in the same basic block so that the
   // compiler can interleave that VALU/LDS work with the MFMAs
-  auto body = [&](int buf, const Pre& S) {
-    // J-side fragments of the wave's 32 columns for the two K steps of 16
-    const int Jw = 32 * wc + (lane & 31), hk = lane >> 5;
-    h8 brh[2], brl[2], bih[2], bil[2];
+  auto body = [&](int buf, const Pre& P) {
+    // I-side fragments of the wave's 32 columns for the two K steps of 16
+    const int Iw = 32 * wc + (lane & 31), hk = lane >> 5;
+    h8 arh[2], arl[2], aih[2], ail[2];
 #pragma unroll
     for (int st = 0; st < 2; st++) {
       const int g = 2 * st + hk;
-      brh[st] = Bs[buf][4 * g + 0][Jw];
-      brl[st] = Bs[buf][4 * g + 1][Jw];
-      bih[st] = Bs[buf][4 * g + 2][Jw];
-      bil[st] = Bs[buf][4 * g + 3][Jw];
+      arh[st] = As[buf][4 * g + 0][Iw];
+      arl[st] = As[buf][4 * g + 1][Iw];
+      aih[st] = As[buf][4 * g + 2][Iw];
+      ail[st] = As[buf][4 * g + 3][Iw];
     }
-    stage(S, buf ^ 1);
+    stage(P, buf ^ 1);
 #pragma unroll
     for (int st = 0; st < 2; st++) {
       const int g = 2 * st + hk;
+      const h8 narh = neg(arh[st]), narl = neg(arl[st]);  // Im = a_i b_r + (-a_r) b_i
 #pragma unroll
       for (int a = 0; a < 2; a++) {
-        const int I = 64 * wr + 32 * a + (lane & 31);
-        const h8 arh = As[buf][4 * g + 0][I];
-        const h8 arl = As[buf][4 * g + 1][I];
-        const h8 aih = As[buf][4 * g + 2][I];
-        const h8 ail = As[buf][4 * g + 3][I];
-        const h8 narh = neg(arh), narl = neg(arl);  // Im = a_i b_r + (-a_r) b_i
+        const int R = 64 * wr + 32 * a + (lane & 31);
+        const h8 brh = Bs[buf][4 * g + 0][R];
+        const h8 brl = Bs[buf][4 * g + 1][R];
+        const h8 bih = Bs[buf][4 * g + 2][R];
+        const h8 bil = Bs[buf][4 * g + 3][R];
         f16v r = accR[a], m = accI[a];
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh, brh[st], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, brh[st], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh, brl[st], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, brl[st], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(arl, brh[st], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(ail, brh[st], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, bih[st], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(narh, bih[st], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, bil[st], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(narh, bil[st], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(ail, bih[st], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(narl, bih[st], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arh[st], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, aih[st], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, arh[st], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, aih[st], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arl[st], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, ail[st], m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, aih[st], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, narh, m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, aih[st], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, narh, m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, ail[st], r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, narl, m, 0, 0, 0);
         accR[a] = r;
         accI[a] = m;
       }
@@ -297,8 +338,8 @@ __device__ __forceinline__ void chi0_tile(const h8* __restrict__ S, const float2
   // chunk ch+2 loads.  Past the end the last chunk is loaded and stored
   // again (harmless: that buffer is not read any more).
   const long last = nch - 1;
-  int kf = 0, cf = 0;  // next chunk to fetch
-  long nf = 0;         // its index
+  int kf = (int)(ch0 / npc), cf = (int)(ch0 % npc);  // next chunk to fetch
+  long nf = 0;                                        // its index, relative to ch0
   auto fetch_next = [&](Pre& P) {
     fetch(P, kf, cf);
     if (nf < last) {
@@ -334,39 +375,80 @@ __device__ __forceinline__ void chi0_tile(const h8* __restrict__ S, const float2
   }
 }
 
+// One launch over a slab: blocks [0, nfull) take the items of the full
+// rounds (XCD-aware: blocks b, b+8, b+16, ... run on one XCD and take
+// consecutive items); with sp > 1 each of the remaining items is cut into sp
+// parts along the chunks (block nfull + t: item nfull + t/sp, part t%sp,
+// slot t), else block b takes item b.
 __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S, const float2* __restrict__ M,
                                                        long strideK, int np, int pb, int pbp, int npc, int nkc,
-                                                       const float* __restrict__ wf, int nom,
+                                                       const float* __restrict__ wt, int nom,
                                                        const float* __restrict__ cs,
-                                                       const double* __restrict__ ws, int G, int T, int nwork,
-                                                       double2* __restrict__ eps) {
+                                                       const double* __restrict__ ws, int G, int nfull, int sp,
+                                                       double2* __restrict__ eps, double2* __restrict__ part) {
   using namespace cm;
   __shared__ h8 As[2][16][TB];  // [buf][g*4 + plane][I]
-  __shared__ h8 Bs[2][16][TB];  // [buf][g*4 + plane][J], weighted
-  // XCD-aware work map: blocks b, b+8, b+16, ... run on one XCD and take
-  // consecutive work items (frequency fastest).
-  const int per = gridDim.x >> 3;
-  const int work = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  if (work >= nwork) return;
-  int It, Jt;
-  tile_pair(work / nom, T, It, Jt);
-  chi0_tile(S, M, strideK, np, pb, pbp, npc, nkc, wf, nom, cs, ws, G, work % nom, It, Jt, eps, As, Bs);
+  __shared__ h8 Bs[2][16][TB];  // [buf][g*4 + plane][merged column], weighted
+  const int b = blockIdx.x;
+  const long nch = (long)nkc * npc;
+  int item;
+  long c0 = 0, c1 = nch;
+  double2* slot = nullptr;
+  if (b < nfull) {
+    const int per = nfull >> 3;
+    item = (b & 7) * per + (b >> 3);
+  } else if (sp > 1) {
+    const int t = b - nfull, s = t % sp;
+    item = nfull + t / sp;
+    c0 = nch * s / sp;
+    c1 = nch * (s + 1) / sp;
+    slot = part + (size_t)t * TB * TB;
+  } else {
+    item = b;
+  }
+  int It, Nc;
+  item_of(item, pb, nom, It, Nc);
+  chi0_cols(S, M, strideK, np, pb, pbp, npc, c0, c1, wt, nom, cs, ws, G, It, Nc, eps, slot, As, Bs);
 }
 
-// Tiles above the diagonal from the ones below: eps[I,J] = conj(eps[J,I])
-// for I in tile It < Jt (the contraction writes the tiles It >= Jt).  One
+// eps[I, J, w] += sum over s of the part slots (t*sp + s) of split item
+// nfull + t, in part order.  blockIdx.y = t; one thread per (row, I).
+__global__ void __launch_bounds__(256) pygw_chi0_tailfix(const double2* __restrict__ part, int sp, int nfull, int pb,
+                                                         int nom, double2* __restrict__ eps) {
+  using namespace cm;
+  const int t = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = e / TB, i = e % TB;
+  int It, Nc;
+  item_of(nfull + t, pb, nom, It, Nc);
+  const int I = It * TB + i, m = Nc * TB + row;
+  if (I >= pb || m >= ncols(pb, nom, It)) return;
+  const int J = It * TB + m / nom, w = m % nom;
+  const double2* p = part + ((size_t)t * sp * TB + row) * TB + i;
+  double2 s = p[0];
+  for (int q = 1; q < sp; q++) {
+    const double2 v = p[(size_t)q * TB * TB];
+    s.x += v.x;
+    s.y += v.y;
+  }
+  double2* d = eps + ((size_t)pb * w + J) * (size_t)pb + I;
+  const double2 o = *d;
+  *d = double2{o.x + s.x, o.y + s.y};
+}
+
+// Tiles below the block diagonal from the ones above: eps[I,J] = conj(eps[J,I])
+// for I in tile It > Jt (the contraction writes the tiles It <= Jt).  One
 // thread per element.
 __global__ void pygw_chi0_mirror(double2* __restrict__ eps, int pb, int nom) {
   const long n = (long)pb * pb;
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int I = (int)(e % pb), J = (int)(e / pb);  // element (I, J), column-major
-  if ((I / cm::TB) >= (J / cm::TB)) return;         // computed tile
+  if ((I / cm::TB) <= (J / cm::TB)) return;         // computed tile
   double2* E = eps + (size_t)blockIdx.y * n;
   const double2 v = E[(size_t)J + (size_t)pb * I];
   E[(size_t)I + (size_t)pb * J] = double2{v.x, -v.y};
 }
-
 // Per-column maximum of max(|Re|,|Im|) over all (k, p): one block per
 // 64-column strip and k-range, a block reduction then one atomic per column.
 __global__ void __launch_bounds__(256) pygw_colmax(const float2* __restrict__ M, long strideK, int np, int pb, int nk,
@@ -451,22 +533,24 @@ __global__ void pygw_zero_c128(double2* __restrict__ x, long n) {
     x[e] = double2{0.0, 0.0};
 }
 
-// wf[k][w][p] = (float)(coef * kcw[k0+k][w][p] * ws[w]) for p < N_p, 0 up to
-// npc*32: the J-side weights of one slab, scaled and rounded once.
+// wt[k][p][w] = (float)(coef * kcw[k0+k][w][p] * ws[w]) for p < N_p, 0 up to
+// npad: the column-side weights of one slab, scaled and rounded once, with
+// the frequency fastest (a wave's merged columns are consecutive frequencies).
 __global__ void pygw_chi0_wprep(const double* __restrict__ kcw, int np, int nom, int k0, int nkc, int npad, double coef,
-                                const double* __restrict__ ws, float* __restrict__ wf) {
-  const long n = (long)nkc * nom * npad;
+                                const double* __restrict__ ws, float* __restrict__ wt) {
+  const long n = (long)nkc * npad * nom;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    const int p = (int)(e % npad);
-    const long kw = e / npad;  // k*nom + w
-    const int w = (int)(kw % nom);
-    wf[e] = p < np ? (float)(coef * kcw[((size_t)k0 * nom + kw) * np + p] * ws[w]) : 0.f;
+    const int w = (int)(e % nom);
+    const long kp = e / nom;  // k*npad + p
+    const int p = (int)(kp % npad), k = (int)(kp / npad);
+    wt[e] = p < np ? (float)(coef * kcw[((size_t)(k0 + k) * nom + w) * np + p] * ws[w]) : 0.f;
   }
 }
 
 namespace {
 constexpr size_t kHead = 1 << 16;                // scales region granule, bytes
 constexpr size_t kSlabBudget = (size_t)1 << 30;  // default slab budget, bytes
+constexpr size_t kPartBytes = (size_t)cm::kSlots * cm::TB * cm::TB * sizeof(double2);
 
 size_t head_bytes(int pb, int nom) {
   const size_t h = (size_t)pb * 12 + (size_t)nom * 16 + 512;
@@ -479,21 +563,40 @@ size_t slab_bytes_per_k(int np, int pb, int nom) {
   const size_t wbytes = ((size_t)nom * npc * cm::KC * sizeof(float) + 255) / 256 * 256;
   return npc * 16 * pbp * sizeof(h8) + wbytes;
 }
+int items_total(int pb, int nom) {
+  int n = 0;
+  for (int It = 0; It * cm::TB < pb; It++) n += cm::ncchunks(pb, nom, It);
+  return n;
+}
+// blocks per round: one per CU, a multiple of 8 (the XCD deal), at most the
+// number of part slots
+int round_blocks() {
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ncu = v;
+  }
+  int b = ncu / 8 * 8;
+  if (b < 8) b = 8;
+  return b > cm::kSlots ? cm::kSlots : b;
+}
 }  // namespace
 
-// Workspace for a call over n_k k-points: scales + fragment planes and
-// weights of up to min(n_k, 1 GiB / per-k bytes) k-points (the call walks k
-// in slabs).
+// Workspace for a call over n_k k-points: scales, the part slots of the
+// split items, and the fragment planes and weights of up to
+// min(n_k, 1 GiB / per-k bytes) k-points (the call walks k in slabs).
 size_t chi0_mixed_ws_bytes(int pb, int np, int nom, int nk) {
   const size_t per_k = slab_bytes_per_k(np, pb, nom);
   size_t ks = kSlabBudget / per_k;
   if (ks < 1) ks = 1;
   if (ks > (size_t)nk) ks = (size_t)nk;
-  return head_bytes(pb, nom) + ks * per_k;
+  return head_bytes(pb, nom) + kPartBytes + ks * per_k;
 }
 
 // M: complex64 [n_kc][np][pb] holding k = k0..k0+n_kc-1; kcw covers all k.
-// eps accumulates (beta = 1) or is overwritten (beta = 0).
+// eps accumulates (beta = 1) or is overwritten (beta = 0).  A workspace
+// without room for the part slots (at least head + one k-point) runs the
+// last round unsplit.
 int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double* kcw, int nom, int k0, int nkc,
                       double coef, int G, int beta, void* ws, size_t ws_bytes, void* eps, hipStream_t stream) {
   using namespace cm;
@@ -501,24 +604,27 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
   const size_t per_k = slab_bytes_per_k(np, pb, nom);
   if (ws_bytes < head + per_k || G < 1) return PYGW_E_ARG;
   if ((size_t)(np + KC) * pb * 8 >= ((size_t)1 << 31)) return PYGW_E_ARG;  // 32-bit buffer offsets per k
-  size_t ks = (ws_bytes - head) / per_k;
+  if ((long)pb * nom >= (1L << 30)) return PYGW_E_ARG;                     // 32-bit merged column index
+  const bool parts = ws_bytes >= head + kPartBytes + per_k;
+  const size_t body = ws_bytes - head - (parts ? kPartBytes : 0);
+  size_t ks = body / per_k;
   if (ks > (size_t)nkc) ks = (size_t)nkc;
   float* cs = (float*)ws;  // [pb] scales, then [pb] their inverses
   unsigned int* cmax = (unsigned int*)(cs + 2 * pb);
   double* wsc = (double*)(((uintptr_t)(cmax + pb) + 15) & ~(uintptr_t)15);
   unsigned long long* wmax = (unsigned long long*)(wsc + nom);
+  double2* part = parts ? (double2*)((char*)ws + head) : nullptr;
   const int T = ceil_div(pb, TB);
   const int pbp = T * TB;
   const int npc = ceil_div(np, KC);
-  h8* S = (h8*)((char*)ws + head);
-  float* wf = (float*)((char*)ws + head + ks * (size_t)npc * 16 * pbp * sizeof(h8));
+  h8* S = (h8*)((char*)ws + head + (parts ? kPartBytes : 0));
+  float* wt = (float*)((char*)S + ks * (size_t)npc * 16 * pbp * sizeof(h8));
   const float2* Mf = (const float2*)M;
 
   hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, cmax, (long)pb);
   hipLaunchKernelGGL(pygw_zero_u32, dim3(1), dim3(256), 0, stream, (unsigned int*)wmax, 2L * nom);
   hipLaunchKernelGGL(pygw_colmax, dim3(ceil_div(pb, 64), colmax_rows_grid((long)nkc * np)), dim3(256), 0, stream, Mf,
-                     strideK,
-                     np, pb, nkc, cmax);
+                     strideK, np, pb, nkc, cmax);
   hipLaunchKernelGGL(pygw_colscale, dim3(ceil_div(pb, 256)), dim3(256), 0, stream, cmax, pb, cs);
   hipLaunchKernelGGL(pygw_wmax, dim3(64, nom), dim3(256), 0, stream, kcw + (size_t)k0 * nom * np, np, nom, nkc, coef,
                      wmax);
@@ -526,8 +632,10 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
   const long ntot = (long)pb * pb * nom;
   if (!beta) hipLaunchKernelGGL(pygw_zero_c128, dim3(2048), dim3(256), 0, stream, (double2*)eps, ntot);
 
-  const int nwork = T * (T + 1) / 2 * nom;
-  const int grid = ceil_div(nwork, 8) * 8;
+  const int nitems = items_total(pb, nom);
+  const int B = round_blocks();
+  const int nfull = nitems / B * B;
+  const int rest = nitems - nfull;
   for (long kk = 0; kk < nkc; kk += (long)ks) {
     const int n = (int)((nkc - kk) < (long)ks ? (nkc - kk) : (long)ks);
     const float2* Ms = Mf + (size_t)kk * strideK;
@@ -539,29 +647,36 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
     int wblocks = (int)((nw_el + 255) / 256);
     if (wblocks > 65536) wblocks = 65536;
     hipLaunchKernelGGL(pygw_chi0_wprep, dim3(wblocks), dim3(256), 0, stream, kcw, np, nom, k0 + (int)kk, n, npc * KC,
-                       coef, (const double*)wsc, wf);
+                       coef, (const double*)wsc, wt);
+    // parts per split item: fill the round, at most one per chunk
+    int sp = (parts && rest > 0) ? B / rest : 1;
+    if ((long)sp > (long)n * npc) sp = (int)((long)n * npc);
+    if (sp < 2) sp = 1;
+    const int grid = sp > 1 ? nfull + rest * sp : nitems;
     hipLaunchKernelGGL(pygw_chi0_mixed, dim3(grid), dim3(NT), 0, stream, (const h8*)S, Ms, strideK, np, pb, pbp, npc,
-                       n, (const float*)wf, nom, cs, wsc, G, T, nwork, (double2*)eps);
+                       n, (const float*)wt, nom, cs, wsc, G, nfull, sp, (double2*)eps, part);
+    if (sp > 1)
+      hipLaunchKernelGGL(pygw_chi0_tailfix, dim3(TB * TB / 256, rest), dim3(256), 0, stream, (const double2*)part, sp,
+                         nfull, pb, nom, (double2*)eps);
   }
   hipLaunchKernelGGL(pygw_chi0_mirror, dim3(ceil_div((long)pb * pb, 256), nom), dim3(256), 0, stream,
                      (double2*)eps, pb, nom);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }
-
 // ---------------------------------------------------------------------------
 // Batched contraction over many q-points in one persistent launch.  Each q
 // has its own PB and pair count (all pairs of all k concatenated: only the
 // (k, p) list matters to the sum), its own operands, scales and eps.  Work
-// item = (q, tile pair, frequency).  The (q, tile pair) groups, in order of
-// decreasing cost (the q's chunk count), are dealt to the 8 XCDs round robin;
+// item = (q, I tile, merged-column chunk).  The items, in order of
+// decreasing cost (the q's chunk count) and then I tile, are cut into
+// segments of 32 (one per CU of an XCD) dealt to the 8 XCDs round robin;
 // within an XCD the resident blocks take the XCD's items from an atomic
-// counter, frequency fastest, so blocks that run together on an XCD share a
-// tile pair's operand slabs in its L2.
+// counter, so blocks that run together on an XCD share an I slab in its L2.
 // ---------------------------------------------------------------------------
 struct Chi0Q {
   const h8* S;        // fragment planes [npc][16][pbp]
   const float2* M;    // complex64 [K][PB]
-  const float* wf;    // FP32 scaled weights [nom][npc*32]
+  const float* wt;    // FP32 scaled weights [npc*32][nom]
   const float* cs;    // [2*PB] column scales and inverses
   const double* ws;   // [nom] weight scales
   double2* eps;       // Fortran [PB, PB, nom]
@@ -569,25 +684,25 @@ struct Chi0Q {
 };
 
 __global__ void __launch_bounds__(512) pygw_chi0_mixed_batch(const Chi0Q* __restrict__ qs,
-                                                             const int4* __restrict__ groups,
-                                                             const int* __restrict__ glen, int nom, int G,
+                                                             const int4* __restrict__ items,
+                                                             const int* __restrict__ xoff, int nom, int G,
                                                              unsigned int* __restrict__ counters) {
   using namespace cm;
   __shared__ h8 As[2][16][TB];
   __shared__ h8 Bs[2][16][TB];
   __shared__ int item;
   const int x = blockIdx.x & 7;
-  const int n = glen[x];
+  const int o = xoff[x], n = xoff[x + 1] - o;
   for (;;) {
     if (threadIdx.x == 0) item = (int)atomicAdd(&counters[x], 1u);
     __syncthreads();
     const int i = item;
     __syncthreads();  // all threads hold i before thread 0 takes the next one
     if (i >= n) break;
-    const int4 gr = groups[x + 8 * (i / nom)];  // (q, It, Jt)
-    const Chi0Q d = qs[gr.x];
-    chi0_tile(d.S, d.M, 0, d.np, d.pb, d.pbp, d.npc, 1, d.wf, nom, d.cs, d.ws, G, i % nom, gr.y, gr.z, d.eps,
-              As, Bs);
+    const int4 it = items[o + i];  // (q, It, Nc)
+    const Chi0Q d = qs[it.x];
+    chi0_cols(d.S, d.M, 0, d.np, d.pb, d.pbp, d.npc, 0, d.npc, d.wt, nom, d.cs, d.ws, G, it.y, it.z, d.eps,
+              nullptr, As, Bs);
   }
 }
 
@@ -602,36 +717,34 @@ struct Chi0BatchPlan {
   double coef;
   std::vector<Chi0Q> q;
   std::vector<const double*> kcw;
-  std::vector<size_t> off_head;  // per q, into ws
   Chi0Q* d_qs;
-  int4* d_groups;
-  int* d_glen;
+  int4* d_items;
+  int* d_xoff;
   unsigned int* d_counters;
-  int ngroups;
+  int nitems;
   char* ws;
 };
 
 namespace {
+constexpr int kSegment = 32;  // items per segment dealt to one XCD
+
 size_t batch_q_bytes(int pb, int np, int nom) {
   const size_t npc = (size_t)(np + cm::KC - 1) / cm::KC;
   const size_t pbp = (size_t)(pb + cm::TB - 1) / cm::TB * cm::TB;
   return head_bytes(pb, nom) + npc * 16 * pbp * sizeof(h8) +
          ((size_t)nom * npc * cm::KC * sizeof(float) + 255) / 256 * 256;
 }
-size_t batch_tables_bytes(int nq, const int* pb) {
-  size_t ng = 0;
-  for (int q = 0; q < nq; q++) {
-    const size_t T = (size_t)(pb[q] + cm::TB - 1) / cm::TB;
-    ng += T * (T + 1) / 2;
-  }
+size_t batch_tables_bytes(int nq, const int* pb, int nom) {
+  size_t ni = 0;
+  for (int q = 0; q < nq; q++) ni += (size_t)items_total(pb[q], nom);
   const size_t a = ((size_t)nq * sizeof(Chi0Q) + 255) / 256 * 256;
-  const size_t b = ((ng + 8) * sizeof(int4) + 255) / 256 * 256;
+  const size_t b = (ni * sizeof(int4) + 255) / 256 * 256;
   return a + b + 512;
 }
 }  // namespace
 
 size_t chi0_batch_ws_bytes(int nq, const int* pb, const int* np, int nom) {
-  size_t t = batch_tables_bytes(nq, pb);
+  size_t t = batch_tables_bytes(nq, pb, nom);
   for (int q = 0; q < nq; q++) t += batch_q_bytes(pb[q], np[q], nom);
   return t;
 }
@@ -642,13 +755,15 @@ int chi0_batch_plan(int nq, const void* const* M, const int* pb, const int* np, 
                     Chi0BatchPlan** out) {
   using namespace cm;
   if (ws_bytes < chi0_batch_ws_bytes(nq, pb, np, nom)) return PYGW_E_ARG;
+  for (int q = 0; q < nq; q++)
+    if ((long)pb[q] * nom >= (1L << 30)) return PYGW_E_ARG;  // 32-bit merged column index
   Chi0BatchPlan* P = new Chi0BatchPlan();
   P->nq = nq;
   P->nom = nom;
   P->coef = coef;
   P->ws = (char*)ws;
-  size_t off = batch_tables_bytes(nq, pb);
-  std::vector<int4> groups;
+  size_t off = batch_tables_bytes(nq, pb, nom);
+  std::vector<int4> all;
   for (int q = 0; q < nq; q++) {
     Chi0Q d;
     d.pb = pb[q];
@@ -661,42 +776,39 @@ int chi0_batch_plan(int nq, const void* const* M, const int* pb, const int* np, 
     d.cs = (const float*)base;
     d.ws = (const double*)(((uintptr_t)((const unsigned int*)(d.cs + 2 * pb[q]) + pb[q]) + 15) & ~(uintptr_t)15);
     d.S = (const h8*)(base + head);
-    d.wf = (const float*)(base + head + (size_t)d.npc * 16 * d.pbp * sizeof(h8));
+    d.wt = (const float*)(base + head + (size_t)d.npc * 16 * d.pbp * sizeof(h8));
     d.M = (const float2*)M[q];
     d.eps = (double2*)eps[q];
     P->q.push_back(d);
     P->kcw.push_back(kcw[q]);
-    P->off_head.push_back(off);
     off += batch_q_bytes(pb[q], np[q], nom);
-    for (int t = 0; t < T * (T + 1) / 2; t++) {
-      int It, Jt;
-      int tt = t, i = 0;
-      while (tt >= T - i) { tt -= T - i; i++; }
-      It = i;
-      Jt = i + tt;
-      groups.push_back(int4{q, It, Jt, 0});
-    }
+    for (int It = 0; It < T; It++)
+      for (int c = 0; c < ncchunks(pb[q], nom, It); c++) all.push_back(int4{q, It, c, 0});
   }
-  // longest first: a group's items cost ~ its q's chunk count, so dealing
-  // them in decreasing cost leaves the short ones for the tail of each XCD
-  std::stable_sort(groups.begin(), groups.end(),
+  // longest first: an item costs ~ its q's chunk count, so dealing them in
+  // decreasing cost leaves the short ones for the tail of each XCD
+  std::stable_sort(all.begin(), all.end(),
                    [&](const int4& a, const int4& b) { return P->q[a.x].npc > P->q[b.x].npc; });
-  P->ngroups = (int)groups.size();
-  const size_t a = ((size_t)nq * sizeof(Chi0Q) + 255) / 256 * 256;
-  const size_t b = (((size_t)P->ngroups + 8) * sizeof(int4) + 255) / 256 * 256;
-  P->d_qs = (Chi0Q*)ws;
-  P->d_groups = (int4*)((char*)ws + a);
-  P->d_glen = (int*)((char*)ws + a + b);
-  P->d_counters = (unsigned int*)((char*)ws + a + b + 256);
-  int glen[8];
+  P->nitems = (int)all.size();
+  std::vector<int4> items;
+  items.reserve(all.size());
+  int xoff[9];
   for (int x = 0; x < 8; x++) {
-    const int ng = P->ngroups > x ? (P->ngroups - x + 7) / 8 : 0;
-    glen[x] = ng * nom;
+    xoff[x] = (int)items.size();
+    for (size_t s = (size_t)x * kSegment; s < all.size(); s += 8 * kSegment)
+      for (size_t i = s; i < s + kSegment && i < all.size(); i++) items.push_back(all[i]);
   }
+  xoff[8] = (int)items.size();
+  const size_t a = ((size_t)nq * sizeof(Chi0Q) + 255) / 256 * 256;
+  const size_t b = ((size_t)P->nitems * sizeof(int4) + 255) / 256 * 256;
+  P->d_qs = (Chi0Q*)ws;
+  P->d_items = (int4*)((char*)ws + a);
+  P->d_xoff = (int*)((char*)ws + a + b);
+  P->d_counters = (unsigned int*)((char*)ws + a + b + 256);
   hipError_t e = hipMemcpyAsync(P->d_qs, P->q.data(), sizeof(Chi0Q) * nq, hipMemcpyHostToDevice, stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(P->d_groups, groups.data(), sizeof(int4) * groups.size(), hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(P->d_glen, glen, sizeof(glen), hipMemcpyHostToDevice, stream);
+    e = hipMemcpyAsync(P->d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(P->d_xoff, xoff, sizeof(xoff), hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) {
     delete P;
@@ -735,7 +847,7 @@ int chi0_batch_run(const Chi0BatchPlan* P, int G, hipStream_t stream) {
     int wblocks = (int)((nw_el + 255) / 256);
     if (wblocks > 65536) wblocks = 65536;
     hipLaunchKernelGGL(pygw_chi0_wprep, dim3(wblocks), dim3(256), 0, stream, kc, d.np, nom, 0, 1, d.npc * KC, P->coef,
-                       (const double*)wsc, (float*)d.wf);
+                       (const double*)wsc, (float*)d.wt);
   }
   hipLaunchKernelGGL(pygw_zero_u32_small, dim3(1), dim3(64), 0, stream, P->d_counters, 8);
   int dev = 0, ncu = 256;
@@ -745,7 +857,7 @@ int chi0_batch_run(const Chi0BatchPlan* P, int G, hipStream_t stream) {
   }
   const int grid = (ncu + 7) / 8 * 8;
   hipLaunchKernelGGL(pygw_chi0_mixed_batch, dim3(grid), dim3(NT), 0, stream, (const Chi0Q*)P->d_qs,
-                     (const int4*)P->d_groups, (const int*)P->d_glen, nom, G, P->d_counters);
+                     (const int4*)P->d_items, (const int*)P->d_xoff, nom, G, P->d_counters);
   for (int q = 0; q < P->nq; q++) {
     const Chi0Q& d = P->q[q];
     hipLaunchKernelGGL(pygw_chi0_mirror, dim3(ceil_div((long)d.pb * d.pb, 256), nom), dim3(256), 0, stream, d.eps,
