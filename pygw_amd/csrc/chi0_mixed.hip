@@ -31,8 +31,9 @@
 // columns, so the padding is confined to the last I tile and the last column
 // chunk of each I tile (at PB 900, 64 frequencies: 1.14x the Hermitian area,
 // against 1.46x for 128 x 128 (I, J) tile pairs one frequency at a time).
-// The result is eps[I, J, w] for J >= 128 It; pygw_chi0_mirror fills the
-// tiles below the block diagonal.
+// The result is eps[I, J, w] for J >= I; pygw_chi0_mirror fills the lower
+// triangle.  In the diagonal tiles a wave skips its 32 x 32 tiles that lie
+// wholly below the diagonal.
 //
 // Tiling.  512 threads = 8 waves per block, block tile 128 (merged columns,
 // the MFMA rows) x 128 (I, the MFMA columns: the FP64 folds of a half-wave
@@ -52,6 +53,7 @@
 #include "common.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 // No FMA contraction in this file, and every split goes through cm::split,
@@ -97,6 +99,19 @@ __device__ __forceinline__ void split(float x, _Float16& hi, _Float16& lo) {
   asm volatile("" : "+v"(x));
   hi = (_Float16)x;
   lo = (_Float16)(x - (float)hi);
+}
+
+// The same split for two values at once, as packed FP16 pairs: hi by one
+// packed conversion, x - hi (exact in FP32) by v_fma_mix_f32 with the FP16
+// half of hi as an operand, lo by one packed conversion.
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split2(float x0, float x1, unsigned& hi, unsigned& lo) {
+  asm volatile("" : "+v"(x0), "+v"(x1));
+  hi = __builtin_bit_cast(unsigned, h2{(_Float16)x0, (_Float16)x1});
+  float d0, d1;
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(d0) : "v"(hi), "v"(x0));
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(d1) : "v"(hi), "v"(x1));
+  lo = __builtin_bit_cast(unsigned, h2{(_Float16)d0, (_Float16)d1});
 }
 }  // namespace cm
 // Fragment planes.  For k-point k (of the slab), chunk c, group g = 0..3 and
@@ -144,8 +159,8 @@ __global__ void __launch_bounds__(256) pygw_chi0_presplit(const float2* __restri
 //   D[i][n] (16 values v): n = l%32, i = 8*(v/4) + 4*(l/32) + v%4.
 // A = the merged columns (rows of D), B = the I side (columns of D).
 // S: fragment planes of this slab; M: the same slab as complex64 (column
-// side); wt: the slab's scaled weights coef*kcw*t_w in FP32, [k][p][w]
-// (zero past N_p), from pygw_chi0_wprep.
+// side); wt: the slab's scaled weights coef*kcw*t_w in FP32,
+// [k][p/8][w][p%8] (zero past N_p), from pygw_chi0_wprep.
 // One work item: rows I of tile It against the 128 merged columns
 // m = 128 Nc .. 128 Nc + 127 of tile It (see the file header), summed over
 // the chunks [ch0, ch1) of the slab (chunk ch = k*npc + c).  With part ==
@@ -162,9 +177,19 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
   using namespace cm;
   const int I0 = It * TB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 2, wc = wave & 3;
+  const int wr = wave >> 2, wc = wave & 3;  // merged-row half, I-column quarter
   const float* ics = cs + pb;  // 1/cs, exact
   const int mcount = ncols(pb, nom, It);
+  // A 32-row tile a of the wave (merged rows 64 wr + 32 a ..) whose columns
+  // J all lie below the wave's first I (J < I for every element) is not
+  // needed: pygw_chi0_mirror fills eps[I, J] for I > J from eps[J, I].
+  int skip[2];
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+    const int m0 = Nc * TB + 64 * wr + 32 * a;
+    const int m1 = m0 + 31 < mcount ? m0 + 31 : mcount - 1;
+    skip[a] = __builtin_amdgcn_readfirstlane((m0 >= mcount || m1 / nom < 32 * wc) ? 1 : 0);
+  }
 
   // staging roles: A (I side): 4 x 16 B per thread (contiguous copy of 16
   // planes x 128 columns); column side: merged column nb = tid % 128, pair
@@ -195,7 +220,7 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
   const unsigned moff0 = (unsigned)(8 * gb) * pb8 + (unsigned)jn * 8u;
   const int npad = npc * KC;
   const unsigned nom4 = (unsigned)nom * 4u;
-  const unsigned woff0 = (unsigned)(8 * gb) * nom4 + (unsigned)wn * 4u;
+  const unsigned woff0 = ((unsigned)gb * (unsigned)nom + (unsigned)wn) * 32u;  // 8 pairs of one frequency
   auto fetch = [&](Pre& P, int k, int c) {
     const h8* Sk = S + ((size_t)k * npc + c) * 16 * (size_t)pbp + I0;
     const __amdgpu_buffer_rsrc_t ra =
@@ -210,10 +235,11 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
     for (int j = 0; j < 8; j++)
       P.m[j] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rm, mo + j * pb8, 0, 0));
     const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(wt + (size_t)k * npad * nom), (short)0, (int)((unsigned)npad * nom4), RSRC_FLAGS);
-    const unsigned wo = woff0 + (unsigned)(32 * c) * nom4;
-#pragma unroll
-    for (int j = 0; j < 8; j++) P.w[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rw, wo + j * nom4, 0, 0));
+        (void*)(wt + ((size_t)k * npad + 32 * c) * nom), (short)0, (int)(32u * nom4), RSRC_FLAGS);
+    const u4 w0 = __builtin_amdgcn_raw_buffer_load_b128(rw, woff0, 0, 0);
+    const u4 w1 = __builtin_amdgcn_raw_buffer_load_b128(rw, woff0 + 16u, 0, 0);
+    P.w[0] = __uint_as_float(w0.x); P.w[1] = __uint_as_float(w0.y); P.w[2] = __uint_as_float(w0.z); P.w[3] = __uint_as_float(w0.w);
+    P.w[4] = __uint_as_float(w1.x); P.w[5] = __uint_as_float(w1.y); P.w[6] = __uint_as_float(w1.z); P.w[7] = __uint_as_float(w1.w);
   };
   auto stage = [&](const Pre& P, int buf) {
 #pragma unroll
@@ -221,21 +247,21 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
       const int idx = tid + NT * q;
       As[buf][idx >> 7][idx & 127] = P.a[q];
     }
-    h8 rh, rl, ih, il;
+    u4 rh, rl, ih, il;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      _Float16 h, l;
-      split((P.m[j].x * sJ) * P.w[j], h, l);
-      rh[j] = h;
-      rl[j] = l;
-      split((P.m[j].y * sJ) * P.w[j], h, l);
-      ih[j] = h;
-      il[j] = l;
+    for (int j = 0; j < 4; j++) {
+      unsigned a, b, c, d;
+      split2((P.m[2 * j].x * sJ) * P.w[2 * j], (P.m[2 * j + 1].x * sJ) * P.w[2 * j + 1], a, b);
+      split2((P.m[2 * j].y * sJ) * P.w[2 * j], (P.m[2 * j + 1].y * sJ) * P.w[2 * j + 1], c, d);
+      rh[j] = a;
+      rl[j] = b;
+      ih[j] = c;
+      il[j] = d;
     }
-    Bs[buf][4 * gb + 0][nb] = rh;
-    Bs[buf][4 * gb + 1][nb] = rl;
-    Bs[buf][4 * gb + 2][nb] = ih;
-    Bs[buf][4 * gb + 3][nb] = il;
+    Bs[buf][4 * gb + 0][nb] = __builtin_bit_cast(h8, rh);
+    Bs[buf][4 * gb + 1][nb] = __builtin_bit_cast(h8, rl);
+    Bs[buf][4 * gb + 2][nb] = __builtin_bit_cast(h8, ih);
+    Bs[buf][4 * gb + 3][nb] = __builtin_bit_cast(h8, il);
   };
 
   // accumulators: two 32x32 tiles (merged rows 64*wr + 32*a) of the wave's
@@ -251,7 +277,7 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
       }
   };
   bool first = true;
-  auto flush = [&]() {
+  auto flush = [&](auto sk) {
     // opaque copies keep the compiler from hoisting the 32 element addresses
     // (and their scales) out of the chunk loop into live registers
     int I = I0 + 32 * wc + (lane & 31), rb = 64 * wr + 4 * (lane >> 5);
@@ -259,7 +285,8 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
     if (I < pb) {
       const double fi = (double)ics[I];
 #pragma unroll
-      for (int a = 0; a < 2; a++)
+      for (int a = 0; a < 2; a++) {
+        if (decltype(sk)::value && skip[a]) continue;
 #pragma unroll
         for (int v = 0; v < 16; v++) {
           const int row = rb + 32 * a + 8 * (v >> 2) + (v & 3);
@@ -283,6 +310,7 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
             }
           }
         }
+      }
     }
     first = false;
   };
@@ -290,7 +318,7 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
   // chunk `buf`'s products (LDS buffer buf), with the next chunk's inputs
   // stored into the other buffer in the same basic block so that the
   // compiler can interleave that VALU/LDS work with the MFMAs
-  auto body = [&](int buf, const Pre& P) {
+  auto body = [&](int buf, const Pre& P, auto sk) {
     // I-side fragments of the wave's 32 columns for the two K steps of 16
     const int Iw = 32 * wc + (lane & 31), hk = lane >> 5;
     h8 arh[2], arl[2], aih[2], ail[2];
@@ -309,6 +337,7 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
       const h8 narh = neg(arh[st]), narl = neg(arl[st]);  // Im = a_i b_r + (-a_r) b_i
 #pragma unroll
       for (int a = 0; a < 2; a++) {
+        if (decltype(sk)::value && skip[a]) continue;
         const int R = 64 * wr + 32 * a + (lane & 31);
         const h8 brh = Bs[buf][4 * g + 0][R];
         const h8 brl = Bs[buf][4 * g + 1][R];
@@ -347,32 +376,40 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
       if (++cf == npc) { cf = 0; kf++; }
     }
   };
-  zero();
-  Pre P0, P1;
-  fetch_next(P0);
-  stage(P0, 0);
-  fetch_next(P1);
-  __syncthreads();
-  int g = 0;
-  for (long ch = 0; ch < nch; ch += 2) {
-    fetch_next(P0);  // chunk ch+2
-    body(0, P1);     // computes ch, stores ch+1
-    if (++g == G || ch == last) {
-      flush();
-      zero();
-      g = 0;
-    }
+  // the chunk loop, with the tile skips compiled in only where an item has
+  // one (so that the products of the others stay one basic block)
+  auto run = [&](auto sk) {
+    zero();
+    Pre P0, P1;
+    fetch_next(P0);
+    stage(P0, 0);
+    fetch_next(P1);
     __syncthreads();
-    if (ch == last) break;
-    fetch_next(P1);  // chunk ch+3
-    body(1, P0);     // computes ch+1, stores ch+2
-    if (++g == G || ch + 1 == last) {
-      flush();
-      zero();
-      g = 0;
+    int g = 0;
+    for (long ch = 0; ch < nch; ch += 2) {
+      fetch_next(P0);  // chunk ch+2
+      body(0, P1, sk);     // computes ch, stores ch+1
+      if (++g == G || ch == last) {
+        flush(sk);
+        zero();
+        g = 0;
+      }
+      __syncthreads();
+      if (ch == last) break;
+      fetch_next(P1);  // chunk ch+3
+      body(1, P0, sk);     // computes ch+1, stores ch+2
+      if (++g == G || ch + 1 == last) {
+        flush(sk);
+        zero();
+        g = 0;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-  }
+  };
+  if (skip[0] | skip[1])
+    run(std::true_type{});
+  else
+    run(std::false_type{});
 }
 
 // One launch over a slab: blocks [0, nfull) take the items of the full
@@ -424,6 +461,7 @@ __global__ void __launch_bounds__(256) pygw_chi0_tailfix(const double2* __restri
   const int I = It * TB + i, m = Nc * TB + row;
   if (I >= pb || m >= ncols(pb, nom, It)) return;
   const int J = It * TB + m / nom, w = m % nom;
+  if (J < I) return;  // filled by pygw_chi0_mirror (its part slot may be unwritten)
   const double2* p = part + ((size_t)t * sp * TB + row) * TB + i;
   double2 s = p[0];
   for (int q = 1; q < sp; q++) {
@@ -436,15 +474,15 @@ __global__ void __launch_bounds__(256) pygw_chi0_tailfix(const double2* __restri
   *d = double2{o.x + s.x, o.y + s.y};
 }
 
-// Tiles below the block diagonal from the ones above: eps[I,J] = conj(eps[J,I])
-// for I in tile It > Jt (the contraction writes the tiles It <= Jt).  One
-// thread per element.
+// The lower triangle from the upper: eps[I,J] = conj(eps[J,I]) for I > J
+// (the contraction computes every eps[I, J] with J >= I, and some with
+// J < I inside the diagonal tiles).  One thread per element.
 __global__ void pygw_chi0_mirror(double2* __restrict__ eps, int pb, int nom) {
   const long n = (long)pb * pb;
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int I = (int)(e % pb), J = (int)(e / pb);  // element (I, J), column-major
-  if ((I / cm::TB) <= (J / cm::TB)) return;         // computed tile
+  if (I <= J) return;
   double2* E = eps + (size_t)blockIdx.y * n;
   const double2 v = E[(size_t)J + (size_t)pb * I];
   E[(size_t)I + (size_t)pb * J] = double2{v.x, -v.y};
@@ -533,16 +571,20 @@ __global__ void pygw_zero_c128(double2* __restrict__ x, long n) {
     x[e] = double2{0.0, 0.0};
 }
 
-// wt[k][p][w] = (float)(coef * kcw[k0+k][w][p] * ws[w]) for p < N_p, 0 up to
-// npad: the column-side weights of one slab, scaled and rounded once, with
-// the frequency fastest (a wave's merged columns are consecutive frequencies).
+// wt[k][p/8][w][p%8] = (float)(coef * kcw[k0+k][w][p] * ws[w]) for p < N_p,
+// 0 up to npad: the column-side weights of one slab, scaled and rounded
+// once.  A lane stages 8 consecutive pairs of one frequency (32 contiguous
+// bytes) and the lanes of a wave take consecutive frequencies.
 __global__ void pygw_chi0_wprep(const double* __restrict__ kcw, int np, int nom, int k0, int nkc, int npad, double coef,
                                 const double* __restrict__ ws, float* __restrict__ wt) {
   const long n = (long)nkc * npad * nom;
+  const int ng = npad / 8;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    const int w = (int)(e % nom);
-    const long kp = e / nom;  // k*npad + p
-    const int p = (int)(kp % npad), k = (int)(kp / npad);
+    const int j = (int)(e & 7);
+    const long r = e >> 3;  // (k*ng + pg)*nom + w
+    const int w = (int)(r % nom);
+    const long kg = r / nom;
+    const int p = 8 * (int)(kg % ng) + j, k = (int)(kg / ng);
     wt[e] = p < np ? (float)(coef * kcw[((size_t)(k0 + k) * nom + w) * np + p] * ws[w]) : 0.f;
   }
 }
@@ -676,7 +718,7 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
 struct Chi0Q {
   const h8* S;        // fragment planes [npc][16][pbp]
   const float2* M;    // complex64 [K][PB]
-  const float* wt;    // FP32 scaled weights [npc*32][nom]
+  const float* wt;    // FP32 scaled weights [npc*4][nom][8]
   const float* cs;    // [2*PB] column scales and inverses
   const double* ws;   // [nom] weight scales
   double2* eps;       // Fortran [PB, PB, nom]
