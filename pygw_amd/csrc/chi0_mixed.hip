@@ -173,7 +173,7 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
                                           const float* __restrict__ wt, int nom, const float* __restrict__ cs,
                                           const double* __restrict__ ws, int G, int It, int Nc,
                                           double2* __restrict__ eps, double2* __restrict__ part,
-                                          h8 (*As)[16][cm::TB], h8 (*Bs)[16][cm::TB]) {
+                                          h8 (*Bs)[16][cm::TB]) {
   using namespace cm;
   const int I0 = It * TB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -191,8 +191,9 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
     skip[a] = __builtin_amdgcn_readfirstlane((m0 >= mcount || m1 / nom < 32 * wc) ? 1 : 0);
   }
 
-  // staging roles: A (I side): 4 x 16 B per thread (contiguous copy of 16
-  // planes x 128 columns); column side: merged column nb = tid % 128, pair
+  // The I side is read by each wave straight from the fragment planes into
+  // MFMA operand registers (the planes are stored in fragment order); the
+  // column side is staged through LDS: merged column nb = tid % 128, pair
   // group gb = tid / 128 (8 pairs)
   const int nb = tid & 127, gb = tid >> 7;
   const int mb = Nc * TB + nb;
@@ -204,7 +205,6 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
   // LDS store (two sets: the next chunk is stored while the one after it
   // loads)
   struct Pre {
-    h8 a[4];
     float2 m[8];
     float w[8];
   };
@@ -214,20 +214,27 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
   // out-of-range offsets read as zero: pairs p >= N_p fall past the end of
   // the k-point's M block.
   constexpr int RSRC_FLAGS = 0x00020000;
-  const unsigned aoffB = (unsigned)((tid >> 7) * pbp + (tid & 127)) * 16u;  // + q * 4*pbp*16
-  const unsigned aqB = 4u * (unsigned)pbp * 16u;
+  // lane's I fragment: plane row 4 g + q with g = 2 st + lane / 32, column
+  // 32 wc + lane % 32; (st, q) go into the scalar offset
+  const unsigned ioff = (unsigned)(4 * (lane >> 5) * pbp + 32 * wc + (lane & 31)) * 16u;
+  const unsigned iq = (unsigned)pbp * 16u;
   const unsigned pb8 = (unsigned)pb * 8u;
   const unsigned moff0 = (unsigned)(8 * gb) * pb8 + (unsigned)jn * 8u;
   const int npad = npc * KC;
   const unsigned nom4 = (unsigned)nom * 4u;
   const unsigned woff0 = ((unsigned)gb * (unsigned)nom + (unsigned)wn) * 32u;  // 8 pairs of one frequency
-  auto fetch = [&](Pre& P, int k, int c) {
+  auto fetch_i = [&](h8* F, int k, int c) {
     const h8* Sk = S + ((size_t)k * npc + c) * 16 * (size_t)pbp + I0;
     const __amdgpu_buffer_rsrc_t ra =
         __builtin_amdgcn_make_buffer_rsrc((void*)Sk, (short)0, (int)(16u * (unsigned)pbp * 16u), RSRC_FLAGS);
 #pragma unroll
-    for (int q = 0; q < 4; q++)
-      P.a[q] = __builtin_bit_cast(h8, __builtin_amdgcn_raw_buffer_load_b128(ra, aoffB + q * aqB, 0, 0));
+    for (int st = 0; st < 2; st++)
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        F[4 * st + q] = __builtin_bit_cast(
+            h8, __builtin_amdgcn_raw_buffer_load_b128(ra, ioff, (int)((unsigned)(8 * st + q) * iq), 0));
+  };
+  auto fetch = [&](Pre& P, int k, int c) {
     const __amdgpu_buffer_rsrc_t rm = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(M + (size_t)k * strideK), (short)0, (int)((unsigned)np * pb8), RSRC_FLAGS);
     const unsigned mo = moff0 + (unsigned)(32 * c) * pb8;
@@ -242,11 +249,6 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
     P.w[4] = __uint_as_float(w1.x); P.w[5] = __uint_as_float(w1.y); P.w[6] = __uint_as_float(w1.z); P.w[7] = __uint_as_float(w1.w);
   };
   auto stage = [&](const Pre& P, int buf) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int idx = tid + NT * q;
-      As[buf][idx >> 7][idx & 127] = P.a[q];
-    }
     u4 rh, rl, ih, il;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -318,23 +320,14 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
   // chunk `buf`'s products (LDS buffer buf), with the next chunk's inputs
   // stored into the other buffer in the same basic block so that the
   // compiler can interleave that VALU/LDS work with the MFMAs
-  auto body = [&](int buf, const Pre& P, auto sk) {
-    // I-side fragments of the wave's 32 columns for the two K steps of 16
-    const int Iw = 32 * wc + (lane & 31), hk = lane >> 5;
-    h8 arh[2], arl[2], aih[2], ail[2];
-#pragma unroll
-    for (int st = 0; st < 2; st++) {
-      const int g = 2 * st + hk;
-      arh[st] = As[buf][4 * g + 0][Iw];
-      arl[st] = As[buf][4 * g + 1][Iw];
-      aih[st] = As[buf][4 * g + 2][Iw];
-      ail[st] = As[buf][4 * g + 3][Iw];
-    }
+  auto body = [&](int buf, const Pre& P, const h8* F, auto sk) {
+    const int hk = lane >> 5;
     stage(P, buf ^ 1);
 #pragma unroll
     for (int st = 0; st < 2; st++) {
       const int g = 2 * st + hk;
-      const h8 narh = neg(arh[st]), narl = neg(arl[st]);  // Im = a_i b_r + (-a_r) b_i
+      const h8 arh = F[4 * st + 0], arl = F[4 * st + 1], aih = F[4 * st + 2], ail = F[4 * st + 3];
+      const h8 narh = neg(arh), narl = neg(arl);  // Im = a_i b_r + (-a_r) b_i
 #pragma unroll
       for (int a = 0; a < 2; a++) {
         if (decltype(sk)::value && skip[a]) continue;
@@ -344,17 +337,17 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
         const h8 bih = Bs[buf][4 * g + 2][R];
         const h8 bil = Bs[buf][4 * g + 3][R];
         f16v r = accR[a], m = accI[a];
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arh[st], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, aih[st], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, arh[st], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, aih[st], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arl[st], r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, ail[st], m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, aih[st], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arh, r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, aih, m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, arh, r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, aih, m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arl, r, 0, 0, 0);
+        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, ail, m, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, aih, r, 0, 0, 0);
         m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, narh, m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, aih[st], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, aih, r, 0, 0, 0);
         m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, narh, m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, ail[st], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, ail, r, 0, 0, 0);
         m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, narl, m, 0, 0, 0);
         accR[a] = r;
         accI[a] = m;
@@ -379,16 +372,28 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
   // the chunk loop, with the tile skips compiled in only where an item has
   // one (so that the products of the others stay one basic block)
   auto run = [&](auto sk) {
+    int ki = (int)(ch0 / npc), ci = (int)(ch0 % npc);  // next I-fragment chunk
+    long ni = 0;
+    auto fetch_i_next = [&](h8* F) {
+      fetch_i(F, ki, ci);
+      if (ni < last) {
+        ni++;
+        if (++ci == npc) { ci = 0; ki++; }
+      }
+    };
     zero();
     Pre P0, P1;
+    h8 F0[8], F1[8];
+    fetch_i_next(F0);
     fetch_next(P0);
     stage(P0, 0);
     fetch_next(P1);
     __syncthreads();
     int g = 0;
     for (long ch = 0; ch < nch; ch += 2) {
-      fetch_next(P0);  // chunk ch+2
-      body(0, P1, sk);     // computes ch, stores ch+1
+      fetch_next(P0);    // column side of chunk ch+2
+      fetch_i_next(F1);  // I side of chunk ch+1
+      body(0, P1, F0, sk);  // computes ch, stores ch+1
       if (++g == G || ch == last) {
         flush(sk);
         zero();
@@ -396,8 +401,9 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
       }
       __syncthreads();
       if (ch == last) break;
-      fetch_next(P1);  // chunk ch+3
-      body(1, P0, sk);     // computes ch+1, stores ch+2
+      fetch_next(P1);    // chunk ch+3
+      fetch_i_next(F0);  // chunk ch+2
+      body(1, P0, F1, sk);  // computes ch+1, stores ch+2
       if (++g == G || ch + 1 == last) {
         flush(sk);
         zero();
@@ -424,7 +430,6 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
                                                        const double* __restrict__ ws, int G, int nfull, int sp,
                                                        double2* __restrict__ eps, double2* __restrict__ part) {
   using namespace cm;
-  __shared__ h8 As[2][16][TB];  // [buf][g*4 + plane][I]
   __shared__ h8 Bs[2][16][TB];  // [buf][g*4 + plane][merged column], weighted
   const int b = blockIdx.x;
   const long nch = (long)nkc * npc;
@@ -445,7 +450,7 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
   }
   int It, Nc;
   item_of(item, pb, nom, It, Nc);
-  chi0_cols(S, M, strideK, np, pb, pbp, npc, c0, c1, wt, nom, cs, ws, G, It, Nc, eps, slot, As, Bs);
+  chi0_cols(S, M, strideK, np, pb, pbp, npc, c0, c1, wt, nom, cs, ws, G, It, Nc, eps, slot, Bs);
 }
 
 // eps[I, J, w] += sum over s of the part slots (t*sp + s) of split item
@@ -712,8 +717,8 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
 // item = (q, I tile, merged-column chunk).  The items, in order of
 // decreasing cost (the q's chunk count) and then I tile, are cut into
 // segments of 32 (one per CU of an XCD) dealt to the 8 XCDs round robin;
-// within an XCD the resident blocks take the XCD's items from an atomic
-// counter, so blocks that run together on an XCD share an I slab in its L2.
+// each XCD starts its items in list order, so blocks that run together on
+// an XCD share an I slab in its L2.
 // ---------------------------------------------------------------------------
 struct Chi0Q {
   const h8* S;        // fragment planes [npc][16][pbp]
@@ -727,29 +732,19 @@ struct Chi0Q {
 
 __global__ void __launch_bounds__(512) pygw_chi0_mixed_batch(const Chi0Q* __restrict__ qs,
                                                              const int4* __restrict__ items,
-                                                             const int* __restrict__ xoff, int nom, int G,
-                                                             unsigned int* __restrict__ counters) {
+                                                             const int* __restrict__ xoff, int nom, int G) {
   using namespace cm;
-  __shared__ h8 As[2][16][TB];
   __shared__ h8 Bs[2][16][TB];
-  __shared__ int item;
-  const int x = blockIdx.x & 7;
-  const int o = xoff[x], n = xoff[x + 1] - o;
-  for (;;) {
-    if (threadIdx.x == 0) item = (int)atomicAdd(&counters[x], 1u);
-    __syncthreads();
-    const int i = item;
-    __syncthreads();  // all threads hold i before thread 0 takes the next one
-    if (i >= n) break;
-    const int4 it = items[o + i];  // (q, It, Nc)
-    const Chi0Q d = qs[it.x];
-    chi0_cols(d.S, d.M, 0, d.np, d.pb, d.pbp, d.npc, 0, d.npc, d.wt, nom, d.cs, d.ws, G, it.y, it.z, d.eps,
-              nullptr, As, Bs);
-  }
-}
-
-__global__ void pygw_zero_u32_small(unsigned int* __restrict__ x, int n) {
-  if ((int)threadIdx.x < n) x[threadIdx.x] = 0u;
+  // block b runs on XCD b % 8 (workgroups are dealt to the XCDs round
+  // robin, and each XCD starts its workgroups in order) and takes item
+  // b / 8 of that XCD's list
+  const int x = blockIdx.x & 7, i = blockIdx.x >> 3;
+  const int o = xoff[x];
+  if (i >= xoff[x + 1] - o) return;
+  const int4 it = items[o + i];  // (q, It, Nc)
+  const Chi0Q d = qs[it.x];
+  chi0_cols(d.S, d.M, 0, d.np, d.pb, d.pbp, d.npc, 0, d.npc, d.wt, nom, d.cs, d.ws, G, it.y, it.z, d.eps,
+            nullptr, Bs);
 }
 
 // Host-side plan of a batch (opaque to callers): per-q sizes and pointers,
@@ -762,8 +757,7 @@ struct Chi0BatchPlan {
   Chi0Q* d_qs;
   int4* d_items;
   int* d_xoff;
-  unsigned int* d_counters;
-  int nitems;
+  int nitems, maxlen;
   char* ws;
 };
 
@@ -841,12 +835,13 @@ int chi0_batch_plan(int nq, const void* const* M, const int* pb, const int* np, 
       for (size_t i = s; i < s + kSegment && i < all.size(); i++) items.push_back(all[i]);
   }
   xoff[8] = (int)items.size();
+  P->maxlen = 0;
+  for (int x = 0; x < 8; x++) P->maxlen = std::max(P->maxlen, xoff[x + 1] - xoff[x]);
   const size_t a = ((size_t)nq * sizeof(Chi0Q) + 255) / 256 * 256;
   const size_t b = ((size_t)P->nitems * sizeof(int4) + 255) / 256 * 256;
   P->d_qs = (Chi0Q*)ws;
   P->d_items = (int4*)((char*)ws + a);
   P->d_xoff = (int*)((char*)ws + a + b);
-  P->d_counters = (unsigned int*)((char*)ws + a + b + 256);
   hipError_t e = hipMemcpyAsync(P->d_qs, P->q.data(), sizeof(Chi0Q) * nq, hipMemcpyHostToDevice, stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(P->d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, stream);
@@ -891,15 +886,8 @@ int chi0_batch_run(const Chi0BatchPlan* P, int G, hipStream_t stream) {
     hipLaunchKernelGGL(pygw_chi0_wprep, dim3(wblocks), dim3(256), 0, stream, kc, d.np, nom, 0, 1, d.npc * KC, P->coef,
                        (const double*)wsc, (float*)d.wt);
   }
-  hipLaunchKernelGGL(pygw_zero_u32_small, dim3(1), dim3(64), 0, stream, P->d_counters, 8);
-  int dev = 0, ncu = 256;
-  if (hipGetDevice(&dev) == hipSuccess) {
-    int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ncu = v;
-  }
-  const int grid = (ncu + 7) / 8 * 8;
-  hipLaunchKernelGGL(pygw_chi0_mixed_batch, dim3(grid), dim3(NT), 0, stream, (const Chi0Q*)P->d_qs,
-                     (const int4*)P->d_items, (const int*)P->d_xoff, nom, G, P->d_counters);
+  hipLaunchKernelGGL(pygw_chi0_mixed_batch, dim3(8 * P->maxlen), dim3(NT), 0, stream, (const Chi0Q*)P->d_qs,
+                     (const int4*)P->d_items, (const int*)P->d_xoff, nom, G);
   for (int q = 0; q < P->nq; q++) {
     const Chi0Q& d = P->q[q];
     hipLaunchKernelGGL(pygw_chi0_mirror, dim3(ceil_div((long)d.pb * d.pb, 256), nom), dim3(256), 0, stream, d.eps,
