@@ -308,6 +308,19 @@ def load_traffic(workload, kernel):
         return None
 
 
+def chi0_launches(pb, npair, nom, nk):
+    """Split-precision contraction launches per call: the library walks k in
+    slabs of fragment planes of at most 1 GiB (its default workspace)."""
+    import ctypes as C
+    from pygw_amd import _lib
+    s1, s2 = C.c_size_t(0), C.c_size_t(0)
+    _lib.call("pygw_chi0_workspace", pb, npair, nom, 1, 1, C.byref(s1))
+    _lib.call("pygw_chi0_workspace", pb, npair, nom, 2, 1, C.byref(s2))
+    per_k = s2.value - s1.value
+    ks = max(1, min(nk, (1 << 30) // per_k))
+    return -(-nk // ks)
+
+
 def run_chain(args):
     """The default workload: chi0 -> eps^-1 -> Sigma_c at C5, mixed and FP64
     over the same q-points; value = mixed seconds per q when the mixed chi0
@@ -378,6 +391,7 @@ def run_chain(args):
     head = rmx if ok else r64
     t_chi = head["stages_s"]["chi0"]
     kern = "pygw_chi0_mixed" if ok else "pygw_chi0_f64"
+    nl = chi0_launches(pb, no * ne, nom, nk) if ok else 1
     peak = PEAK_FP16 if ok else PEAK_FP64
     rec = {
         "metric": "chi0+eps^-1+Sigma_c seconds per q-point, mixed vs FP64 GPU at <=1e-6 rel err; % roofline",
@@ -392,7 +406,8 @@ def run_chain(args):
         "stages_s": head["stages_s"],
         "roofline": {"bound": "mfma", "kernel": kern, "achieved": f_chi / t_chi / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": f_chi / t_chi / peak, "traffic": load_traffic("c5", kern),
-                     "flops_per_launch": f_chi, "executed_flops_per_launch": 3 * f_chi if ok else f_chi},
+                     "launches_per_q": nl, "flops_per_launch": f_chi / nl, "time_per_launch_s": t_chi / nl,
+                     "executed_flops_per_launch": (3 * f_chi if ok else f_chi) / nl},
         "fp64": {"s_per_q": r64["s_per_q"], "stages_s": r64["stages_s"],
                  "chi0_tflops": f_chi / r64["stages_s"]["chi0"] / 1e12,
                  "chi0_frac_fp64_peak": f_chi / r64["stages_s"]["chi0"] / PEAK_FP64},

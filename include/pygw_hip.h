@@ -82,12 +82,15 @@ int pygw_tetra_weights(const double* enk, double ef, const int* kqid, const int*
  *   precision 0: FP64 on v_mfma_f64_16x16x4f64
  *   precision 1: FP16 hi+lo split with per-column and per-frequency
  *                power-of-two scaling, 3 products on
- *                v_mfma_f32_16x16x32_f16, one fresh MFMA chain per 32
- *                pairs, FP32 tile sums folded into the FP64 eps
+ *                v_mfma_f32_32x32x16_f16, FP32 tile sums folded into the
+ *                FP64 eps every 64*s chunks of 32 pairs
  *   workspace: pygw_chi0_workspace bytes for (pb, n_pairs, n_om, n_k) (0 for
- *         precision 0); NULL lets the call allocate it (stream-ordered).  A
- *         smaller caller workspace (at least one k-point's fragment planes)
- *         makes the call walk k in smaller slabs.
+ *         precision 0): scales, 64 MiB of part slots for the split last
+ *         round, and the fragment planes of up to 1 GiB of k-points.  NULL
+ *         lets the call allocate it (stream-ordered).  A smaller caller
+ *         workspace (at least the scales and one k-point's fragment planes)
+ *         makes the call walk k in smaller slabs, and without room for the
+ *         part slots it runs the last round unsplit.
  * ------------------------------------------------------------------------ */
 int pygw_chi0_workspace(int pb, int n_pairs, int n_om, int n_k, int precision, size_t* bytes);
 int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, const double* kcw, int n_om, int k0,
@@ -116,8 +119,8 @@ int pygw_chi0_set_schedule(int precision, int pb, int n_pairs, int n_om, int nw)
  * complex128 Fortran [PB_q, PB_q, n_om] (overwritten).  plan uploads the
  * descriptor tables into the workspace (synchronous on `stream`) and returns
  * an opaque plan; run only launches kernels (capturable in a HIP graph):
- * per-q scales and fragment planes, one persistent contraction over all
- * (q, tile pair, frequency) work items, per-q mirror.  schedule 0 = the
+ * per-q scales and fragment planes, one contraction launch over all
+ * (q, I tile, merged column-frequency chunk) work items, per-q mirror.  schedule 0 = the
  * autotuned one (default 2); 1, 2, 4 = fold every 64*s chunks.  The
  * autotuners (split precision) choose among s = 1, 2 (times[2] = -1): s = 4
  * doubles the FP32 partial sums and lands near the 1e-6 accuracy bar.
