@@ -197,6 +197,39 @@ int pygw_pade_qp(const void* sigc, int n_state, int n_om, const double* omega, i
                  const double* sigx, const double* vxc, double* out, void* stream);
 
 /* ------------------------------------------------------------------------
+ * GW0 on stored mwm (the calceqp iteration, postprocess.py:206-321: Sigma_c
+ * of every (q, irk) recomputed with the quasiparticle bands, mixed, then the
+ * quasiparticle energies and the Fermi level again).
+ *
+ * pygw_pade_qp_gw0 replaces Compute_quasiparticles (mcommon.py:167-210) for
+ *   both of its modes: iop_es = 0 is pygw_pade_qp (G0W0, ene0 unused);
+ *   iop_es = -1 is the GW0 update with ene0 the Kohn-Sham energies and
+ *   ene the current bands: iop_gw0 = 1: E = e0 + s - vxc; 2: E = e + Z(s -
+ *   vxc) + e0 - e; otherwise E = e + Z(s - vxc + e0 - e) (s = Re S(e) +
+ *   sigx).  out as pygw_pade_qp.
+ * pygw_selfc_qsum: out = beta*out + alpha*sum_q parts[q] over n_q blocks of
+ *   n complex128 values, summed in q order (the q loop of Compute_selfc,
+ *   postprocess.py:176-186, and the mixing sigc*(1-mix) + mix*sigc_new,
+ *   postprocess.py:297).  beta = 0 does not read out.
+ * pygw_gw0_enk: enk [n_items][nb2] of Compute_selfc_inside (mcommon.py:
+ *   94-101): core energies, then bands[jrk[item]][0 .. nb2 - n_core - 1];
+ *   bands [n_irk][n_band] row-major, jrk the irreducible index of k - q of
+ *   each (q, irk) item (0-based).
+ * pygw_tetra_idos replaces ft.idos / ft.dostet (for_tetrahedra.f90:84-141)
+ *   at n_e energies at once: idos[e] = sum wtet*intdos1t, dos[e] = sum
+ *   wtet*dos1t over bands and tetrahedra; ebnd [n_k][n_band] row-major,
+ *   atet [n_tet][4] (0-based), wtet [n_tet].
+ * ------------------------------------------------------------------------ */
+int pygw_pade_qp_gw0(const void* sigc, int n_state, int n_om, const double* omega, int n_pade, const double* ene,
+                     const double* ene0, const double* sigx, const double* vxc, int iop_es, int iop_gw0, double* out,
+                     void* stream);
+int pygw_selfc_qsum(const void* parts, int n_q, long n, double alpha, double beta, void* out, void* stream);
+int pygw_gw0_enk(const double* bands, int n_irk, int n_band, const double* core, int n_core, const int* jrk,
+                 int n_items, int nb2, double* enk, void* stream);
+int pygw_tetra_idos(const double* ebnd, int n_k, int n_band, const int* atet, const double* wtet, int n_tet,
+                    const double* energies, int n_e, double* idos, double* dos, void* stream);
+
+/* ------------------------------------------------------------------------
  * Synthetic matrix elements for benchmarks and tests.  No counterpart in
  * PyGW (its M comes from calc_minm, pygw.py:2547-2721); for callers that
  * need seeded inputs of the benchmark shapes without a DFT calculation.

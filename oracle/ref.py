@@ -138,3 +138,28 @@ def padeg(x, z, apar):
     lib.padeg_(C.c_void_p(g.ctypes.data), C.c_void_p(x.ctypes.data), C.c_void_p(z.ctypes.data),
                C.c_void_p(a.ctypes.data), _i(z.size), _i(x.size))
     return g
+
+
+def _tet_args(ebnd, atet, wtet):
+    e = np.asfortranarray(ebnd, dtype=np.float64)           # Ebnd(nk, nbnd)
+    a = np.ascontiguousarray(atet, dtype=np.int32)          # [ntet][4] = atet(4, ntet)
+    w = np.ascontiguousarray(wtet, dtype=np.float64)
+    return e, a, w
+
+
+def idos(energy, ebnd, atet, wtet):
+    """for_tetrahedra.f90:84-116 (ebnd [nk, nbnd], atet [ntet, 4] 0-based)."""
+    f = _load("libref_tetra.so").idos_
+    f.restype = C.c_double
+    e, a, w = _tet_args(ebnd, atet, wtet)
+    return f(_d(energy), e.ctypes.data_as(C.c_void_p), a.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p),
+             _i(e.shape[0]), _i(e.shape[1]), _i(a.shape[0]))
+
+
+def dostet(energy, ebnd, atet, wtet):
+    """for_tetrahedra.f90:118-141."""
+    f = _load("libref_tetra.so").dostet_
+    f.restype = C.c_double
+    e, a, w = _tet_args(ebnd, atet, wtet)
+    return f(_d(energy), e.ctypes.data_as(C.c_void_p), a.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p),
+             _i(e.shape[0]), _i(e.shape[1]), _i(a.shape[0]))

@@ -12,7 +12,13 @@ int selfc_conv_launch(const double* enk, int nrows, int nb1, int nb2, const void
                       const void* mwmp, int nomp, const double* omp, const double* womp, void* out,
                       hipStream_t stream);
 int pade_qp_launch(const void* sig, int nstate, int nom, const double* omega, int npts, const double* ene,
-                   const double* sigx, const double* vxc, double* out, hipStream_t stream);
+                   const double* ene0, const double* sigx, const double* vxc, int iop_es, int iop_gw0, double* out,
+                   hipStream_t stream);
+int selfc_qsum_launch(const void* parts, int nq, long n, double alpha, double beta, void* out, hipStream_t stream);
+int gw0_enk_launch(const double* bands, int nband, const double* core, int ncore, const int* jrk, int nitems, int nb2,
+                   double* enk, hipStream_t stream);
+int tetra_idos_launch(const double* ebnd, int nband, const int* atet, const double* wtet, int ntet,
+                      const double* energies, int ne, double* idos, double* dos_out, hipStream_t stream);
 
 extern "C" int pygw_mwm_workspace(int n_k, int rows_per_k, int pb, int n_om, int precision, size_t* bytes) {
   if (n_k <= 0 || rows_per_k <= 0 || pb <= 0 || n_om <= 0 || !bytes || (precision != 0 && precision != 1))
@@ -66,5 +72,35 @@ extern "C" int pygw_selfc_convolve(const double* enk, int n_k, int nb1, int nb2,
 extern "C" int pygw_pade_qp(const void* sigc, int n_state, int n_om, const double* omega, int n_pade,
                             const double* ene, const double* sigx, const double* vxc, double* out, void* stream) {
   if (!sigc || !omega || !ene || !out || n_state <= 0 || n_om <= 0) return PYGW_E_ARG;
-  return pade_qp_launch(sigc, n_state, n_om, omega, n_pade, ene, sigx, vxc, out, (hipStream_t)stream);
+  return pade_qp_launch(sigc, n_state, n_om, omega, n_pade, ene, nullptr, sigx, vxc, 0, 0, out, (hipStream_t)stream);
+}
+
+extern "C" int pygw_pade_qp_gw0(const void* sigc, int n_state, int n_om, const double* omega, int n_pade,
+                                const double* ene, const double* ene0, const double* sigx, const double* vxc,
+                                int iop_es, int iop_gw0, double* out, void* stream) {
+  if (!sigc || !omega || !ene || !out || n_state <= 0 || n_om <= 0) return PYGW_E_ARG;
+  return pade_qp_launch(sigc, n_state, n_om, omega, n_pade, ene, ene0, sigx, vxc, iop_es, iop_gw0, out,
+                        (hipStream_t)stream);
+}
+
+extern "C" int pygw_selfc_qsum(const void* parts, int n_q, long n, double alpha, double beta, void* out,
+                               void* stream) {
+  if (!parts || !out || n_q <= 0 || n <= 0) return PYGW_E_ARG;
+  return selfc_qsum_launch(parts, n_q, n, alpha, beta, out, (hipStream_t)stream);
+}
+
+extern "C" int pygw_gw0_enk(const double* bands, int n_irk, int n_band, const double* core, int n_core,
+                            const int* jrk, int n_items, int nb2, double* enk, void* stream) {
+  if (!bands || !jrk || !enk || n_irk <= 0 || n_band <= 0 || n_core < 0 || n_items <= 0 || nb2 <= n_core ||
+      nb2 - n_core > n_band || (n_core > 0 && !core))
+    return PYGW_E_ARG;
+  return gw0_enk_launch(bands, n_band, core, n_core, jrk, n_items, nb2, enk, (hipStream_t)stream);
+}
+
+extern "C" int pygw_tetra_idos(const double* ebnd, int n_k, int n_band, const int* atet, const double* wtet,
+                               int n_tet, const double* energies, int n_e, double* idos, double* dos, void* stream) {
+  if (!ebnd || !atet || !wtet || !energies || !idos || !dos || n_k <= 0 || n_band <= 0 || n_tet <= 0 || n_e <= 0 ||
+      n_e > 65535)
+    return PYGW_E_ARG;
+  return tetra_idos_launch(ebnd, n_band, atet, wtet, n_tet, energies, n_e, idos, dos, (hipStream_t)stream);
 }

@@ -192,14 +192,20 @@ __global__ void __launch_bounds__(256) pygw_selfc_conv(const double* __restrict_
 }
 
 // Pade (Thiele) through the first npts points of sigma(i w_n), evaluated at
-// e and e +- 1e-3 (mcommon.py:330-338), then the G0W0 quasiparticle update
-// (mcommon.py:182-203, iop_es = 0):
+// e and e +- 1e-3 (mcommon.py:330-338), then the quasiparticle update of
+// Compute_quasiparticles (mcommon.py:182-203):
 //   Z = 1/(1 - Re dS), set to 1 (and dS to 0) when Z > 1 or Z < 0;
-//   E_qp = e + Z (Re S(e) + sigx - vxc).
+//   s = Re S(e) + sigx;
+//   iop_es = 0 (G0W0):        E_qp = e + Z (s - vxc)
+//   iop_es = -1 (GW0), e0 the Kohn-Sham energy:
+//     iop_gw0 = 1: E_qp = e + s - vxc + e0 - e
+//     iop_gw0 = 2: E_qp = e + Z (s - vxc) + e0 - e
+//     otherwise:   E_qp = e + Z (s - vxc + e0 - e)
 // out per state: [S(e) re, im, dS re, im, Z, E_qp]
 __global__ void pygw_pade_qp(const double2* __restrict__ sig, int nstate, int nom, const double* __restrict__ omega,
-                             int npts, const double* __restrict__ ene, const double* __restrict__ sigx,
-                             const double* __restrict__ vxc, double* __restrict__ out) {
+                             int npts, const double* __restrict__ ene, const double* __restrict__ ene0,
+                             const double* __restrict__ sigx, const double* __restrict__ vxc, int iop_es, int iop_gw0,
+                             double* __restrict__ out) {
   using namespace sc;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nstate) return;
@@ -234,9 +240,50 @@ __global__ void pygw_pade_qp(const double2* __restrict__ sig, int nstate, int no
   if (z > 1.0 || z < 0.0) z = 1.0;
   const double snk = val[0].x + (sigx ? sigx[s] : 0.0);
   const double vx = vxc ? vxc[s] : 0.0;
+  double delta;
+  if (iop_es == 0) {
+    delta = z * (snk - vx);
+  } else {
+    const double e0 = ene0[s];
+    if (iop_gw0 == 1)
+      delta = snk - vx + e0 - e;
+    else if (iop_gw0 == 2)
+      delta = z * (snk - vx) + e0 - e;
+    else
+      delta = z * (snk - vx + e0 - e);
+  }
   double* o = out + (size_t)s * 6;
   o[0] = val[0].x; o[1] = val[0].y; o[2] = ds.x; o[3] = ds.y; o[4] = z;
-  o[5] = e + z * (snk - vx);
+  o[5] = e + delta;
+}
+
+// out = beta * out + alpha * sum_q parts[q] (q in order): the sum over q of
+// the per-q Sigma_c contributions of Compute_selfc (postprocess.py:176-186)
+// and the self-energy mixing of the GW0 loop (postprocess.py:297).
+__global__ void pygw_selfc_qsum(const double2* __restrict__ parts, int nq, long n, double alpha, double beta,
+                                double2* __restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double2 acc = parts[i];
+    for (int q = 1; q < nq; q++) {
+      const double2 v = parts[(size_t)q * n + i];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    double2 o = double2{0.0, 0.0};
+    if (beta != 0.0) o = out[i];
+    out[i] = double2{beta * o.x + alpha * acc.x, beta * o.y + alpha * acc.y};
+  }
+}
+
+// enk of each (q, irk) item of Compute_selfc_inside (mcommon.py:94-101): the
+// core energies, then bands[jrk[item], 0 .. nb2 - ncore - 1]; bands is
+// [n_irk][n_band] (row-major), enk [n_items][nb2].
+__global__ void pygw_gw0_enk(const double* __restrict__ bands, int nband, const double* __restrict__ core, int ncore,
+                             const int* __restrict__ jrk, int nitems, int nb2, double* __restrict__ enk) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nitems * nb2) return;
+  const int it = (int)(t / nb2), ie = (int)(t % nb2);
+  enk[t] = ie < ncore ? core[ie] : bands[(size_t)jrk[it] * nband + (ie - ncore)];
 }
 
 int selfc_spline_launch(const void* mwm, int nrows, int nb1, int nom, int nb2, const double* x, const double* xp,
@@ -258,9 +305,27 @@ int selfc_conv_launch(const double* enk, int nrows, int nb1, int nb2, const void
 }
 
 int pade_qp_launch(const void* sig, int nstate, int nom, const double* omega, int npts, const double* ene,
-                   const double* sigx, const double* vxc, double* out, hipStream_t stream) {
+                   const double* ene0, const double* sigx, const double* vxc, int iop_es, int iop_gw0, double* out,
+                   hipStream_t stream) {
   if (npts > 64 || npts < 1 || npts > nom) return PYGW_E_ARG;
+  if (iop_es != 0 && (iop_es != -1 || !ene0)) return PYGW_E_ARG;
   hipLaunchKernelGGL(pygw_pade_qp, dim3(ceil_div(nstate, 64)), dim3(64), 0, stream, (const double2*)sig, nstate, nom,
-                     omega, npts, ene, sigx, vxc, out);
+                     omega, npts, ene, ene0, sigx, vxc, iop_es, iop_gw0, out);
+  return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+}
+
+int selfc_qsum_launch(const void* parts, int nq, long n, double alpha, double beta, void* out, hipStream_t stream) {
+  long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(pygw_selfc_qsum, dim3((unsigned)blocks), dim3(256), 0, stream, (const double2*)parts, nq, n,
+                     alpha, beta, (double2*)out);
+  return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+}
+
+int gw0_enk_launch(const double* bands, int nband, const double* core, int ncore, const int* jrk, int nitems, int nb2,
+                   double* enk, hipStream_t stream) {
+  const long n = (long)nitems * nb2;
+  hipLaunchKernelGGL(pygw_gw0_enk, dim3(ceil_div(n, 256)), dim3(256), 0, stream, bands, nband, core, ncore, jrk,
+                     nitems, nb2, enk);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }

@@ -312,3 +312,76 @@ def device_window_elements(batch: QBatch, i, complex128=False, device="cuda"):
               C.c_void_p(torch.cuda.current_stream().cuda_stream))
     torch.cuda.current_stream().synchronize()
     return out
+
+
+@dataclass
+class GW0System:
+    """Inputs of the GW0 iteration on stored mwm (postprocess.py:206-321) for
+    a mesh without symmetry (every k irreducible).  ebnd [n_k, n_band] Kohn-
+    Sham energies (Hartree, E_F = 0); the GW bands are ibgw .. ibgw+nb1-1;
+    core [n_core]; jrk [n_q, n_k] the index of k - q; mwm [n_q, n_k, n_om,
+    nb1, nb2] (the mwm.{iq}.{irk}.npy stack); sigc0 [n_k, nb1, n_om] the
+    G0W0 Sigma_c (the sum over q at the Kohn-Sham bands); sigx, vxc
+    [n_k, nb1]; atet [n_tet, 4], wtet [n_tet]; nval the valence electrons
+    (spin 2) counted from band 0."""
+    div: tuple
+    ebnd: np.ndarray
+    core: np.ndarray
+    jrk: np.ndarray
+    mwm: np.ndarray
+    sigx: np.ndarray
+    vxc: np.ndarray
+    omega: np.ndarray
+    womeg: np.ndarray
+    omega_p: np.ndarray
+    womeg_p: np.ndarray
+    atet: np.ndarray
+    wtet: np.ndarray
+    nval: int
+    ibgw: int
+
+
+def gw0_system(div=(2, 2, 2), n_occ=4, n_emp=6, nbgw=4, n_core=1, n_om=16, iop_multiple=5, q_list=None, seed=11,
+               mwm_array=True):
+    """A seeded insulating GW0 system.  Bands as band_energies with a gap of
+    at least 3 eV (occupied c_n ~ U[-10,-1.5] eV, empty ~ U[1.5,20] eV);
+    nb2 = n_core + n_occ + n_emp.  mwm(i w) of each (q, k, ie1, ie2) is a
+    pole form -a 2W/(w^2 + W^2) with a small imaginary part ib 2W/(w^2 +
+    W^2), W ~ U[0.3, 1.5] Ha, a ~ U[0, 0.15]/(nb2 n_q) (Sigma_c of order
+    0.1 Ha), b = a/10 for ie1 != ie2.  sigx = -U[0.3,0.6] Ha (occupied)
+    / -U[0.05,0.15] (empty); vxc = sigx + U[0, 0.03] (occupied) / - U[0, 0.03]
+    (empty), so that sigx - vxc opens the gap by up to 1.6 eV."""
+    rng = np.random.default_rng(seed)
+    nk = int(np.prod(div))
+    c = np.concatenate([rng.uniform(-10.0, -1.5, n_occ), rng.uniform(1.5, 20.0, n_emp)])
+    t = rng.uniform(0.2, 1.0, c.size)
+    s = np.cos(2 * np.pi * mesh.kpoint_coords(div)).sum(axis=1) / 3.0
+    e = np.sort((c[None, :] + t[None, :] * s[:, None]) / mesh.H2EV, axis=1)   # [nk, nband]
+    core = np.sort(rng.uniform(-8.0, -4.0, n_core))
+    ibgw = n_occ - nbgw // 2
+    nb1, nb2 = nbgw, n_core + n_occ + n_emp
+    kqid = mesh.k_minus_q(div)
+    q_list = np.arange(nk) if q_list is None else np.asarray(q_list)
+    jrk = np.ascontiguousarray(kqid[:, q_list].T.astype(np.int32))      # [n_q, n_k]
+    fm = mesh.FrequencyMesh(n_om, 0.02, 4.0, iop_multiple=iop_multiple)
+    nq = len(q_list)
+    mwm = None
+    if mwm_array:
+        a = rng.uniform(0.0, 0.15, (nq, nk, nb1, nb2)) / (nb2 * nq)
+        W = rng.uniform(0.3, 1.5, (nq, nk, nb1, nb2))
+        b = a / 10.0 * rng.uniform(-1.0, 1.0, a.shape)
+        diag = np.zeros((nb1, nb2), bool)   # (ie1, ie2) of the same state: real
+        for i in range(nb1):
+            diag[i, n_core + ibgw + i] = True
+        b[:, :, diag] = 0.0
+        om = fm.omega[None, None, :, None, None]
+        f = 2 * W[:, :, None] / (om ** 2 + W[:, :, None] ** 2)          # [nq, nk, nom, nb1, nb2]
+        mwm = (-a[:, :, None] * f + 1j * b[:, :, None] * f).astype(np.complex128)
+    occ = np.arange(ibgw, ibgw + nb1)[None, :] < n_occ
+    sigx = np.where(occ, -rng.uniform(0.3, 0.6, (nk, nb1)), -rng.uniform(0.05, 0.15, (nk, nb1)))
+    vxc = sigx + np.where(occ, rng.uniform(0.0, 0.03, (nk, nb1)), -rng.uniform(0.0, 0.03, (nk, nb1)))
+    tetc = mesh.tetra_corners(div)
+    atet = np.ascontiguousarray(tetc.T.astype(np.int32))
+    wtet = np.full(atet.shape[0], 1.0 / atet.shape[0])
+    return GW0System(tuple(div), e, core, jrk, mwm, sigx, vxc, fm.omega, fm.womeg, fm.omega_precise,
+                     fm.womeg_precise, atet, wtet, 2 * n_occ, ibgw)
