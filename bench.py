@@ -36,6 +36,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6e12       # MI355X dense FP64 (vector and MFMA), FLOP/s
+BASE_SEED_GW0 = 20261209  # synthetic mwm of the GW0 workload
 PEAK_FP16 = 2.5e15        # MI355X dense FP16 MFMA, FLOP/s
 PEAK_HBM = 8.0e12         # B/s
 
@@ -625,6 +626,126 @@ def run_batch(args):
     print(json.dumps(rec))
 
 
+def gw0_cpu_baseline(g, mwm_item, nb2):
+    """PyGW's own per-(q, k) Sigma_c of the GW0 loop (Compute_selfc_inside,
+    spline branch, mcommon.py:126-144): scipy CubicSpline per (ie1, ie2)
+    and the compiled fnc.few_fr_convolutions (oracle/_ref) per (ie1, iom),
+    on one (q, k) item, extrapolated to all items of an iteration."""
+    from scipy import interpolate
+
+    from oracle import ref
+    enk = g.ebnd[g.jrk[0, 0], :nb2]
+    nom, nb1, _ = mwm_item.shape
+    fa, fp = g.omega ** 2 + 1.0, g.omega_p ** 2 + 1.0
+    kind = "reference" if ref.available("libref_fnc.so") else "port"
+    t = time.perf_counter()
+    for ie1 in range(nb1):
+        mp = np.zeros((nb2, g.omega_p.size), dtype=complex)
+        for ie2 in range(nb2):
+            v = mwm_item[:, ie1, ie2]
+            sr = interpolate.CubicSpline(g.omega, fa * v.real, bc_type=((2, 0.0), (1, 0.0)), extrapolate=True)
+            si = interpolate.CubicSpline(g.omega, fa * v.imag, bc_type=((2, 0.0), (1, 0.0)), extrapolate=True)
+            mp[ie2] = (sr(g.omega_p) + 1j * si(g.omega_p)) / fp
+        for iom in range(nom):
+            if kind == "reference":
+                ref.few_fr_convolutions(enk, mp, mwm_item[iom, ie1], g.omega_p, g.omega[iom], g.womeg_p)
+            else:
+                from oracle import chain
+                sum(chain.fr_convolution2(enk[j], mp[j], mwm_item[iom, ie1, j], g.omega_p, g.omega[iom], g.womeg_p)
+                    for j in range(nb2))
+    t1 = time.perf_counter() - t
+    nitems = g.jrk.size
+    return {"value": t1 * nitems, "unit": "s/iteration", "cores": 1, "kind": kind,
+            "sample": (f"one (q, k) item of the {nitems} of an iteration: scipy CubicSpline per (ie1, ie2) and "
+                       f"{'the compiled fnc.few_fr_convolutions' if kind == 'reference' else 'the numpy convolution'} "
+                       f"per (ie1, iom), x{nitems}"),
+            "item_s": t1}
+
+
+def run_gw0(args):
+    """One GW0 iteration on stored mwm (postprocess.py:281-301): Sigma_c of
+    every (q, k) at the current bands, mixing, quasiparticles, Fermi level.
+    4x4x4 mesh without symmetry (64 irreducible k), 64 q, 8 GW bands, nb2 =
+    140, 64 Matsubara points, precise mesh x20; mwm resident in HBM."""
+    import torch
+
+    from pygw_amd import gw0, mesh, synth
+    g = synth.gw0_system(div=(4, 4, 4), n_occ=20, n_emp=120, nbgw=8, n_core=0, n_om=64, iop_multiple=20,
+                         mwm_array=False, seed=BASE_SEED_GW0)
+    nq, nk = g.jrk.shape
+    nom, nb1, nb2 = g.omega.size, 8, 140
+    # synthetic mwm on the device (the pole form of synth.gw0_system)
+    gen = torch.Generator(device="cuda").manual_seed(BASE_SEED_GW0)
+    shp = (nq, nk, 1, nb1, nb2)
+    a = torch.rand(shp, generator=gen, device="cuda", dtype=torch.float64) * (0.15 / (nb2 * nq))
+    W = 0.3 + 1.2 * torch.rand(shp, generator=gen, device="cuda", dtype=torch.float64)
+    b = a / 10.0 * (2 * torch.rand(shp, generator=gen, device="cuda", dtype=torch.float64) - 1)
+    om = torch.as_tensor(g.omega, device="cuda").view(1, 1, nom, 1, 1)
+    f = 2 * W / (om * om + W * W)
+    mwm = torch.complex(-a * f, b * f).contiguous()
+    del a, W, b, f
+    torch.cuda.synchronize()
+    print(f"[bench] gw0 mwm stack {tuple(mwm.shape)} {mwm.numel() * 16 / 2**30:.2f} GiB", file=sys.stderr, flush=True)
+    selfe = gw0.SelfEnergy(mwm, g.jrk, g.core, g.omega, g.omega_p, g.womeg_p)
+    dos = gw0.TetraDos(g.atet, g.wtet)
+    ibgw = g.ibgw
+    bands = g.ebnd.copy()
+    sig = selfe(bands)            # the G0W0 Sigma_c
+    snew = torch.empty_like(sig)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+
+    def iteration(step_bands):
+        ev[0].record()
+        selfe(step_bands, out=snew)
+        ev[1].record()
+        gw0.mix(sig, snew, 0.5)
+        eqp, _ = gw0.quasiparticles(step_bands[:, ibgw:ibgw + nb1], g.ebnd[:, ibgw:ibgw + nb1], sig, g.sigx, g.vxc,
+                                    g.omega, -1, 1)
+        ev[2].record()
+        ef = gw0.calc_fermi(eqp, dos, g.nval - 2 * ibgw)[0]
+        ev[3].record()
+        torch.cuda.synchronize()
+        return eqp, ef, [ev[0].elapsed_time(ev[1]) / 1e3, ev[1].elapsed_time(ev[2]) / 1e3,
+                         ev[2].elapsed_time(ev[3]) / 1e3]
+
+    for _ in range(max(1, args.warmup)):   # the first call also splines mwm onto the precise mesh
+        eqp, ef, _ = iteration(bands)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stages = np.zeros(3)
+    for _ in range(args.steps):
+        eqp, ef, st = iteration(bands)
+        stages += st
+        bands[:, ibgw:ibgw + nb1] = eqp - ef
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    stages /= args.steps
+    print(f"[bench] gw0 iteration {t:.3f} s", file=sys.stderr, flush=True)
+    # convolution kernel: per (item, ie1, iom, ie2, precise point) one complex
+    # term of fr_convolution2 (fnc.f90:86-119), counted as 20 FP64 flops
+    nomp = g.omega_p.size
+    terms = nq * nk * nb1 * nom * nb2 * nomp
+    rec = {"metric": "seconds per GW0 iteration (Sigma_c of every (q, k) on stored mwm + Pade QP + Fermi level)",
+           "value": t, "unit": "s/iteration", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": t * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"gw0: 4x4x4 k-mesh, every k irreducible (64), {nq} q, mwm [{nom}, {nb1}, {nb2}] "
+                                  f"per (q, k) resident in HBM ({mwm.numel() * 16 / 1e9:.1f} GB), precise mesh "
+                                  f"x20 = {nomp}, "
+                                  + (f"precise-mesh splines of all q kept in HBM ({selfe.mwmp.numel() * 16 / 1e9:.0f} GB,"
+                                     f" computed in the warm-up)" if selfe.cached else
+                                     f"q chunks of {selfe.qc} (splines recomputed per chunk)")},
+           "stages_s": {"selfc_all_q": stages[0], "mix_and_quasiparticles": stages[1], "fermi_level": stages[2]},
+           "roofline": {"bound": "valu", "kernel": "pygw_selfc_conv", "achieved": terms * 20 / stages[0] / 1e12,
+                        "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": terms * 20 / stages[0] / PEAK_FP64,
+                        "traffic": None, "flops_per_term": 20, "terms_per_iteration": terms,
+                        "note": "FP64 vector work; time of the whole all-q stage (enk, spline, convolution, q sum)"}}
+    if not args.no_cpu:
+        item = mwm[0, 0].cpu().numpy()
+        rec["cpu_baseline"] = gw0_cpu_baseline(g, item, nb2)
+    print(json.dumps(rec))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -637,7 +758,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.steps is None:
-        args.steps = {"c5": 2, "c2": 8, "c4": 3}.get(args.config, 1)
+        args.steps = {"c5": 2, "c2": 8, "c4": 3, "gw0": 2}.get(args.config, 1)
     if args.copy_calibration:
         # one copy kernel that reads and writes exactly 1 GiB: the ratio of its
         # FETCH_SIZE / WRITE_SIZE counters to 1 GiB scales the kernels' counters
@@ -652,6 +773,9 @@ def main():
         return
     if args.config == "c4":
         run_batch(args)
+        return
+    if args.config == "gw0":
+        run_gw0(args)
         return
     mixed = not args.no_mixed
     wl = Workload(args.config, args.nq, mixed, args.fp64_kfrac)

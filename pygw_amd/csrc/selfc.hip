@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(64) pygw_selfc_spline(const double2* __restric
 // sum over l per (ie2, iom) in registers, scale by eps_om/pi per ie2 as the
 // reference does, then reduce over the threads.
 namespace sc {
-constexpr int CG = 16;   // Matsubara points per block
+constexpr int CG = 8;   // Matsubara points per block
 constexpr int CT = 256;  // threads per block
 }  // namespace sc
 
@@ -143,9 +143,13 @@ __global__ void __launch_bounds__(256) pygw_selfc_conv(const double* __restrict_
           const double2 mn = mwm[(((size_t)k * nom + i0 + g) * nb1 + ie1) * nb2 + ie2];
           const double2 num = double2{(ml.x - mn.x) * wl, (ml.y - mn.y) * wl};
           const double2 den = double2{o2 + (e * e - wn * wn), -2.0 * e * wn};  // omega_l^2 + (e - i w_n)^2
-          // num / den as num * conj(den) / |den|^2: one real division instead
-          // of Smith's two (|den|^2 vanishes only where den does)
-          const double r = 1.0 / (den.x * den.x + den.y * den.y);
+          // num / den as num * conj(den) / |den|^2 with 1/|den|^2 from the
+          // hardware reciprocal and two Newton-Raphson steps (relative error
+          // a few ulp; |den|^2 vanishes only where den does)
+          const double d2 = den.x * den.x + den.y * den.y;
+          double r = __builtin_amdgcn_rcp(d2);
+          r = __builtin_fma(r, __builtin_fma(-d2, r, 1.0), r);
+          r = __builtin_fma(r, __builtin_fma(-d2, r, 1.0), r);
           s0[g] = cadd(s0[g], double2{(num.x * den.x + num.y * den.y) * r, (num.y * den.x - num.x * den.y) * r});
         }
       }
@@ -158,17 +162,18 @@ __global__ void __launch_bounds__(256) pygw_selfc_conv(const double* __restrict_
         acc[g] = double2{acc[g].x + r.x / PI_F, acc[g].y + r.y / PI_F};
       }
     }
-    // the atan tail once per (ie2, iom), by one thread
-    if ((ie2 % CT) == t) {
+  }
+  // the atan tail once per (ie2, iom), the ie2 dealt over the threads
+  for (int ie2 = t; ie2 < nb2; ie2 += CT) {
+    const double e = enk[(size_t)k * nb2 + ie2];
 #pragma unroll
-      for (int g = 0; g < CG; g++) {
-        if (g < ng) {
-          const double2 mn = mwm[(((size_t)k * nom + i0 + g) * nb1 + ie1) * nb2 + ie2];
-          if (hypot(mn.x, mn.y) > 1e-16) {
-            const double2 eo = double2{e, -omega[i0 + g]};
-            const double2 at = catan_(cdiv(double2{wmax, 0.0}, eo));
-            acc[g] = cadd(acc[g], cmul(double2{mn.x / PI_F, mn.y / PI_F}, at));
-          }
+    for (int g = 0; g < CG; g++) {
+      if (g < ng) {
+        const double2 mn = mwm[(((size_t)k * nom + i0 + g) * nb1 + ie1) * nb2 + ie2];
+        if (hypot(mn.x, mn.y) > 1e-16) {
+          const double2 eo = double2{e, -omega[i0 + g]};
+          const double2 at = catan_(cdiv(double2{wmax, 0.0}, eo));
+          acc[g] = cadd(acc[g], cmul(double2{mn.x / PI_F, mn.y / PI_F}, at));
         }
       }
     }

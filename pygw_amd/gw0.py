@@ -61,9 +61,12 @@ class SelfEnergy:
     """Sigma_c of all (q, k) for given bands: sigc[irk] = sum over q of
     Compute_selfc_inside (postprocess.py:176-186, mcommon.py:80-165, spline
     branch), with mwm [n_q, n_irk, n_om, nb1, nb2] resident on the device.
-    jrk [n_q, n_irk]: irreducible index of k - q; core [n_core] energies."""
+    jrk [n_q, n_irk]: irreducible index of k - q; core [n_core] energies.
+    cache: keep the precise-mesh splines of all q (None: when they fit in
+    half of the free device memory); else walk q in chunks of chunk_bytes
+    of splines, recomputing them on every call."""
 
-    def __init__(self, mwm, jrk, core, omega, omega_p, womeg_p, *, chunk_bytes=8 << 30, device="cuda"):
+    def __init__(self, mwm, jrk, core, omega, omega_p, womeg_p, *, chunk_bytes=8 << 30, cache=None, device="cuda"):
         import torch
         self.device = device = _require_device(device)
         self.mwm = mwm if isinstance(mwm, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(mwm), device=device)
@@ -80,7 +83,15 @@ class SelfEnergy:
         self.wop = _dev(womeg_p, torch.float64, device).contiguous()
         self.nomp = self.omp.numel()
         per_q = self.nirk * self.nb1 * self.nb2 * self.nomp * 16
-        self.qc = max(1, min(self.nq, chunk_bytes // per_q))
+        # The splines of mwm on the precise mesh do not depend on the bands:
+        # when those of every q fit (by default: in half of the free device
+        # memory) they are computed once, on the first call, and kept.
+        if cache is None:
+            free, _ = torch.cuda.mem_get_info(device)
+            cache = self.nq * per_q <= free // 2
+        self.cached = bool(cache)
+        self.splined = False
+        self.qc = self.nq if self.cached else max(1, min(self.nq, chunk_bytes // per_q))
         items = self.qc * self.nirk
         self.mwmp = torch.empty((items, self.nb1, self.nb2, self.nomp), dtype=torch.complex128, device=device)
         self.sc = torch.empty((items, self.nb1, self.nom), dtype=torch.complex128, device=device)
@@ -106,9 +117,11 @@ class SelfEnergy:
                       C.c_void_p(self.core.data_ptr()) if self.ncore else None, self.ncore,
                       C.c_void_p(jr.data_ptr()), items, self.nb2, C.c_void_p(self.enk.data_ptr()), s)
             mw = self.mwm[q0:q0 + nq]
-            _lib.call("pygw_selfc_spline", C.c_void_p(mw.data_ptr()), items, self.nb1, self.nb2,
-                      C.c_void_p(self.om.data_ptr()), self.nom, C.c_void_p(self.omp.data_ptr()), self.nomp,
-                      C.c_void_p(self.mwmp.data_ptr()), s)
+            if not self.splined:
+                _lib.call("pygw_selfc_spline", C.c_void_p(mw.data_ptr()), items, self.nb1, self.nb2,
+                          C.c_void_p(self.om.data_ptr()), self.nom, C.c_void_p(self.omp.data_ptr()), self.nomp,
+                          C.c_void_p(self.mwmp.data_ptr()), s)
+                self.splined = self.cached
             _lib.call("pygw_selfc_convolve", C.c_void_p(self.enk.data_ptr()), items, self.nb1, self.nb2,
                       C.c_void_p(mw.data_ptr()), C.c_void_p(self.om.data_ptr()), self.nom,
                       C.c_void_p(self.mwmp.data_ptr()), C.c_void_p(self.omp.data_ptr()),

@@ -83,12 +83,16 @@ def test_selfenergy_all_q_vs_oracle():
     want = G.compute_selfc(bands, g.core, g.jrk, g.mwm, g.omega, g.womeg, g.omega_p, g.womeg_p)
     per_q = nk * nb1 * nb2 * g.omega_p.size * 16
     print(f"case selfe nq={nq} nk={nk}", flush=True)
-    for chunk in (3 * per_q, 1 << 30):
-        se = gw0.SelfEnergy(g.mwm, g.jrk, g.core, g.omega, g.omega_p, g.womeg_p, chunk_bytes=chunk)
+    for chunk, cache in ((3 * per_q, False), (1 << 30, False), (3 * per_q, True)):
+        se = gw0.SelfEnergy(g.mwm, g.jrk, g.core, g.omega, g.omega_p, g.womeg_p, chunk_bytes=chunk, cache=cache)
         got = se(bands).cpu().numpy()
         err = np.abs(got - want).max() / np.abs(want).max()
-        print(f"  chunk q={se.qc} rel={err:.2e}", flush=True)
+        print(f"  chunk q={se.qc} cache={cache} rel={err:.2e}", flush=True)
         assert err < 1e-12
+        # a second call (splines reused when cached) at other bands
+        b2 = bands + 0.003
+        want2 = G.compute_selfc(b2, g.core, g.jrk, g.mwm, g.omega, g.womeg, g.omega_p, g.womeg_p)
+        assert np.abs(se(b2).cpu().numpy() - want2).max() < 1e-12 * np.abs(want2).max()
     # accumulate form: out = 0.25 out + 0.5 sum_q
     acc = torch.as_tensor(want, device="cuda").clone()
     se(bands, out=acc, alpha=0.5, beta=0.25)
