@@ -37,13 +37,18 @@
 //
 // Tiling.  512 threads = 8 waves per block, block tile 128 (merged columns,
 // the MFMA rows) x 128 (I, the MFMA columns: the FP64 folds of a half-wave
-// touch 512 contiguous bytes of the column-major eps), wave tile 64 x 32
-// (2 x 4 waves; two 32x32 MFMA tiles), two waves per SIMD.  LDS holds the
-// staged I and column slabs of the current and the next chunk (2 x 2 x
-// 32 KB); the next chunk's global loads are issued before the current
-// chunk's MFMAs.  Blocks are dealt to the 8 XCDs round robin and each XCD
-// takes consecutive work items (same I tile), so the I slab they share is
-// read from HBM once per XCD and then served from its L2.
+// touch 512 contiguous bytes of the column-major eps).  The waves split the
+// work: waves 0-3 (one per SIMD: a block's waves are dealt to the four SIMDs
+// round robin) each compute a 64 x 64 tile, four 32x32 MFMA tiles with
+// eight accumulator chains, from the LDS slabs of the current chunk; waves
+// 4-7 stage, loading the chunk after next into registers and storing the
+// next one into the other LDS buffer (I planes copied, column side weighted
+// and split; 2 x 2 x 32 KB).  Each SIMD thus runs one MFMA wave beside one
+// staging wave, whose VALU and memory work overlap the MFMAs instead of
+// alternating with them at every chunk barrier.  Blocks are dealt to the 8
+// XCDs round robin and each XCD takes consecutive work items (same I tile),
+// so the I slab they share is read from HBM once per XCD and then served
+// from its L2.
 //
 // Rounds.  All items of one call cost the same, so a launch of n items on B
 // CUs (one block per CU) runs floor(n/B) full rounds and a partial one.  The
@@ -165,130 +170,180 @@ __global__ void __launch_bounds__(256) pygw_chi0_presplit(const float2* __restri
 // m = 128 Nc .. 128 Nc + 127 of tile It (see the file header), summed over
 // the chunks [ch0, ch1) of the slab (chunk ch = k*npc + c).  With part ==
 // nullptr the sum is added into eps[I, J, w]; otherwise it is written into
-// the part slot part[row][I - 128 It] (FP64, the first fold stores).  As and
-// Bs are the block's LDS buffers; every thread of the 512-thread block calls
-// this with the same arguments, and it ends on a block barrier.
+// the part slot part[row][I - 128 It] (FP64, the first fold stores).
+//
+// The block's 8 waves split the work: waves 0-3 (one per SIMD) only compute,
+// each a 64 x 64 tile (four 32x32 MFMA tiles, eight accumulator chains)
+// from the LDS slabs of the current chunk; waves 4-7 only stage, loading the
+// chunk after next into registers and storing the next one (I planes copied,
+// column side weighted and split) into the other LDS buffer.  A SIMD then
+// holds one MFMA wave and one staging wave, whose VALU and memory work run
+// beside the MFMAs.  As and Bs are the block's LDS buffers; every thread of
+// the 512-thread block calls this with the same arguments, and it ends on a
+// block barrier.
 __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2* __restrict__ M, long strideK,
                                           int np, int pb, int pbp, int npc, long ch0, long ch1,
                                           const float* __restrict__ wt, int nom, const float* __restrict__ cs,
                                           const double* __restrict__ ws, int G, int It, int Nc,
                                           double2* __restrict__ eps, double2* __restrict__ part,
-                                          h8 (*Bs)[16][cm::TB]) {
+                                          h8 (*As)[16][cm::TB], h8 (*Bs)[16][cm::TB]) {
   using namespace cm;
   const int I0 = It * TB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 2, wc = wave & 3;  // merged-row half, I-column quarter
   const float* ics = cs + pb;  // 1/cs, exact
   const int mcount = ncols(pb, nom, It);
-  // A 32-row tile a of the wave (merged rows 64 wr + 32 a ..) whose columns
-  // J all lie below the wave's first I (J < I for every element) is not
-  // needed: pygw_chi0_mirror fills eps[I, J] for I > J from eps[J, I].
-  int skip[2];
+  const long nch = ch1 - ch0;
+  const long last = nch - 1;
+
+  if (wave >= 4) {
+    // ---------------- staging waves ----------------
+    const int pt = tid - 256;
+    // column side: merged column nb, pair groups 2 hf and 2 hf + 1 (8 pairs each)
+    const int nb = pt & 127, hf = pt >> 7;
+    const int mb = Nc * TB + nb;
+    const bool nin = mb < mcount;
+    const int jn = nin ? I0 + mb / nom : pb - 1;  // past the end: column PB-1 (finite), zeroed by sJ = 0
+    const int wn = mb % nom;
+    const float sJ = nin ? cs[jn] : 0.f;
+    struct Pre {
+      h8 a[8];
+      float2 m[16];
+      float w[16];
+    };
+    // Loads go through buffer resources built from chunk-dependent bases
+    // (scalar registers) with per-lane byte offsets; out-of-range offsets
+    // read as zero: pairs p >= N_p fall past the end of the k-point's M block.
+    constexpr int RSRC_FLAGS = 0x00020000;
+    const unsigned pb8 = (unsigned)pb * 8u;
+    const int npad = npc * KC;
+    const unsigned nom4 = (unsigned)nom * 4u;
+    const unsigned aoff = (unsigned)((pt >> 7) * pbp + (pt & 127)) * 16u, aq = 2u * (unsigned)pbp * 16u;
+    const unsigned moff = (unsigned)(16 * hf) * pb8 + (unsigned)jn * 8u;
+    const unsigned woff = ((unsigned)(2 * hf) * (unsigned)nom + (unsigned)wn) * 32u;
+    auto fetch = [&](Pre& P, int k, int c) {
+      const h8* Sk = S + ((size_t)k * npc + c) * 16 * (size_t)pbp + I0;
+      const __amdgpu_buffer_rsrc_t ra =
+          __builtin_amdgcn_make_buffer_rsrc((void*)Sk, (short)0, (int)(16u * (unsigned)pbp * 16u), RSRC_FLAGS);
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        P.a[q] = __builtin_bit_cast(h8, __builtin_amdgcn_raw_buffer_load_b128(ra, aoff + q * aq, 0, 0));
+      const __amdgpu_buffer_rsrc_t rm = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(M + (size_t)k * strideK), (short)0, (int)((unsigned)np * pb8), RSRC_FLAGS);
+      const unsigned mo = moff + (unsigned)(32 * c) * pb8;
+#pragma unroll
+      for (int j = 0; j < 16; j++)
+        P.m[j] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rm, mo + j * pb8, 0, 0));
+      const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(wt + ((size_t)k * npad + 32 * c) * nom), (short)0, (int)(32u * nom4), RSRC_FLAGS);
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const unsigned o = woff + (unsigned)h * nom4 * 8u;
+        const u4 w0 = __builtin_amdgcn_raw_buffer_load_b128(rw, o, 0, 0);
+        const u4 w1 = __builtin_amdgcn_raw_buffer_load_b128(rw, o + 16u, 0, 0);
+        P.w[8 * h + 0] = __uint_as_float(w0.x); P.w[8 * h + 1] = __uint_as_float(w0.y);
+        P.w[8 * h + 2] = __uint_as_float(w0.z); P.w[8 * h + 3] = __uint_as_float(w0.w);
+        P.w[8 * h + 4] = __uint_as_float(w1.x); P.w[8 * h + 5] = __uint_as_float(w1.y);
+        P.w[8 * h + 6] = __uint_as_float(w1.z); P.w[8 * h + 7] = __uint_as_float(w1.w);
+      }
+    };
+    auto stage = [&](const Pre& P, int buf) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int idx = pt + 256 * q;
+        As[buf][idx >> 7][idx & 127] = P.a[q];
+      }
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int gb = 2 * hf + h;
+        u4 rh, rl, ih, il;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          unsigned a, b, c, d;
+          const int p0 = 8 * h + 2 * j, p1 = p0 + 1;
+          split2((P.m[p0].x * sJ) * P.w[p0], (P.m[p1].x * sJ) * P.w[p1], a, b);
+          split2((P.m[p0].y * sJ) * P.w[p0], (P.m[p1].y * sJ) * P.w[p1], c, d);
+          rh[j] = a;
+          rl[j] = b;
+          ih[j] = c;
+          il[j] = d;
+        }
+        Bs[buf][4 * gb + 0][nb] = __builtin_bit_cast(h8, rh);
+        Bs[buf][4 * gb + 1][nb] = __builtin_bit_cast(h8, rl);
+        Bs[buf][4 * gb + 2][nb] = __builtin_bit_cast(h8, ih);
+        Bs[buf][4 * gb + 3][nb] = __builtin_bit_cast(h8, il);
+      }
+    };
+    int kf = (int)(ch0 / npc), cf = (int)(ch0 % npc);  // next chunk to fetch
+    long nf = 0;                                        // its index, relative to ch0
+    auto fetch_next = [&](Pre& P) {
+      fetch(P, kf, cf);
+      if (nf < last) {
+        nf++;
+        if (++cf == npc) { cf = 0; kf++; }
+      }
+    };
+    // Pipeline: while the MFMA waves compute chunk ch from buffer ch&1, chunk
+    // ch+1 (in registers since the previous step) is stored into the other
+    // buffer and chunk ch+2 loads.  Past the end the last chunk is loaded and
+    // stored again (harmless: that buffer is not read any more).
+    Pre P0, P1;
+    fetch_next(P0);
+    stage(P0, 0);
+    fetch_next(P1);
+    __syncthreads();
+    for (long ch = 0; ch < nch; ch += 2) {
+      fetch_next(P0);  // chunk ch+2
+      stage(P1, 1);    // chunk ch+1
+      __syncthreads();
+      if (ch == last) break;
+      fetch_next(P1);  // chunk ch+3
+      stage(P0, 0);    // chunk ch+2
+      __syncthreads();
+    }
+    return;
+  }
+
+  // ---------------- MFMA waves ----------------
+  const int wr = wave >> 1, wc = wave & 1;  // merged-row half, I-column half
+  // A 32x32 tile (a, i) of the wave (merged rows 64 wr + 32 a .., columns
+  // I0 + 64 wc + 32 i ..) whose columns J all lie below its first I (J < I
+  // for every element) is not needed: pygw_chi0_mirror fills eps[I, J] for
+  // I > J from eps[J, I].
+  int skip[2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++) {
     const int m0 = Nc * TB + 64 * wr + 32 * a;
     const int m1 = m0 + 31 < mcount ? m0 + 31 : mcount - 1;
-    skip[a] = __builtin_amdgcn_readfirstlane((m0 >= mcount || m1 / nom < 32 * wc) ? 1 : 0);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+      skip[a][i] = __builtin_amdgcn_readfirstlane((m0 >= mcount || m1 / nom < 64 * wc + 32 * i) ? 1 : 0);
   }
-
-  // The I side is read by each wave straight from the fragment planes into
-  // MFMA operand registers (the planes are stored in fragment order); the
-  // column side is staged through LDS: merged column nb = tid % 128, pair
-  // group gb = tid / 128 (8 pairs)
-  const int nb = tid & 127, gb = tid >> 7;
-  const int mb = Nc * TB + nb;
-  const bool nin = mb < mcount;
-  const int jn = nin ? I0 + mb / nom : pb - 1;  // past the end: column PB-1 (finite), zeroed by sJ = 0
-  const int wn = mb % nom;
-  const float sJ = nin ? cs[jn] : 0.f;
-  // one chunk's staged inputs, held in registers between its load and its
-  // LDS store (two sets: the next chunk is stored while the one after it
-  // loads)
-  struct Pre {
-    float2 m[8];
-    float w[8];
-  };
-  const long nch = ch1 - ch0;
-  // Loads go through buffer resources built from chunk-uniform bases (scalar
-  // registers) with per-lane byte offsets that stay fixed across chunks, and
-  // out-of-range offsets read as zero: pairs p >= N_p fall past the end of
-  // the k-point's M block.
-  constexpr int RSRC_FLAGS = 0x00020000;
-  // lane's I fragment: plane row 4 g + q with g = 2 st + lane / 32, column
-  // 32 wc + lane % 32; (st, q) go into the scalar offset
-  const unsigned ioff = (unsigned)(4 * (lane >> 5) * pbp + 32 * wc + (lane & 31)) * 16u;
-  const unsigned iq = (unsigned)pbp * 16u;
-  const unsigned pb8 = (unsigned)pb * 8u;
-  const unsigned moff0 = (unsigned)(8 * gb) * pb8 + (unsigned)jn * 8u;
-  const int npad = npc * KC;
-  const unsigned nom4 = (unsigned)nom * 4u;
-  const unsigned woff0 = ((unsigned)gb * (unsigned)nom + (unsigned)wn) * 32u;  // 8 pairs of one frequency
-  auto fetch_i = [&](h8* F, int k, int c) {
-    const h8* Sk = S + ((size_t)k * npc + c) * 16 * (size_t)pbp + I0;
-    const __amdgpu_buffer_rsrc_t ra =
-        __builtin_amdgcn_make_buffer_rsrc((void*)Sk, (short)0, (int)(16u * (unsigned)pbp * 16u), RSRC_FLAGS);
-#pragma unroll
-    for (int st = 0; st < 2; st++)
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-        F[4 * st + q] = __builtin_bit_cast(
-            h8, __builtin_amdgcn_raw_buffer_load_b128(ra, ioff, (int)((unsigned)(8 * st + q) * iq), 0));
-  };
-  auto fetch = [&](Pre& P, int k, int c) {
-    const __amdgpu_buffer_rsrc_t rm = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(M + (size_t)k * strideK), (short)0, (int)((unsigned)np * pb8), RSRC_FLAGS);
-    const unsigned mo = moff0 + (unsigned)(32 * c) * pb8;
-#pragma unroll
-    for (int j = 0; j < 8; j++)
-      P.m[j] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rm, mo + j * pb8, 0, 0));
-    const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(wt + ((size_t)k * npad + 32 * c) * nom), (short)0, (int)(32u * nom4), RSRC_FLAGS);
-    const u4 w0 = __builtin_amdgcn_raw_buffer_load_b128(rw, woff0, 0, 0);
-    const u4 w1 = __builtin_amdgcn_raw_buffer_load_b128(rw, woff0 + 16u, 0, 0);
-    P.w[0] = __uint_as_float(w0.x); P.w[1] = __uint_as_float(w0.y); P.w[2] = __uint_as_float(w0.z); P.w[3] = __uint_as_float(w0.w);
-    P.w[4] = __uint_as_float(w1.x); P.w[5] = __uint_as_float(w1.y); P.w[6] = __uint_as_float(w1.z); P.w[7] = __uint_as_float(w1.w);
-  };
-  auto stage = [&](const Pre& P, int buf) {
-    u4 rh, rl, ih, il;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      unsigned a, b, c, d;
-      split2((P.m[2 * j].x * sJ) * P.w[2 * j], (P.m[2 * j + 1].x * sJ) * P.w[2 * j + 1], a, b);
-      split2((P.m[2 * j].y * sJ) * P.w[2 * j], (P.m[2 * j + 1].y * sJ) * P.w[2 * j + 1], c, d);
-      rh[j] = a;
-      rl[j] = b;
-      ih[j] = c;
-      il[j] = d;
-    }
-    Bs[buf][4 * gb + 0][nb] = __builtin_bit_cast(h8, rh);
-    Bs[buf][4 * gb + 1][nb] = __builtin_bit_cast(h8, rl);
-    Bs[buf][4 * gb + 2][nb] = __builtin_bit_cast(h8, ih);
-    Bs[buf][4 * gb + 3][nb] = __builtin_bit_cast(h8, il);
-  };
-
-  // accumulators: two 32x32 tiles (merged rows 64*wr + 32*a) of the wave's
-  // 64 x 32
-  f16v accR[2], accI[2];
+  f16v accR[2][2], accI[2][2];
   auto zero = [&]() {
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int v = 0; v < 16; v++) {
-        accR[a][v] = 0.f;
-        accI[a][v] = 0.f;
-      }
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+          accR[a][i][v] = 0.f;
+          accI[a][i][v] = 0.f;
+        }
   };
   bool first = true;
   auto flush = [&](auto sk) {
-    // opaque copies keep the compiler from hoisting the 32 element addresses
+    // opaque copies keep the compiler from hoisting the element addresses
     // (and their scales) out of the chunk loop into live registers
-    int I = I0 + 32 * wc + (lane & 31), rb = 64 * wr + 4 * (lane >> 5);
-    asm volatile("" : "+v"(I), "+v"(rb));
-    if (I < pb) {
+    int Ib = I0 + 64 * wc + (lane & 31), rb = 64 * wr + 4 * (lane >> 5);
+    asm volatile("" : "+v"(Ib), "+v"(rb));
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const int I = Ib + 32 * i;
+      if (I >= pb) continue;
       const double fi = (double)ics[I];
 #pragma unroll
       for (int a = 0; a < 2; a++) {
-        if (decltype(sk)::value && skip[a]) continue;
+        if (decltype(sk)::value && skip[a][i]) continue;
 #pragma unroll
         for (int v = 0; v < 16; v++) {
           const int row = rb + 32 * a + 8 * (v >> 2) + (v & 3);
@@ -296,7 +351,7 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
           if (m < mcount) {
             const int J = I0 + m / nom, w = m % nom;
             const double f = fi * (double)ics[J] / ws[w];  // exact: powers of two
-            const double re = (double)accR[a][v] * f, im = (double)accI[a][v] * f;
+            const double re = (double)accR[a][i][v] * f, im = (double)accI[a][i][v] * f;
             if (part) {
               double2* e = part + (size_t)row * TB + (I - I0);
               if (first) {
@@ -316,84 +371,61 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
     }
     first = false;
   };
-
-  // chunk `buf`'s products (LDS buffer buf), with the next chunk's inputs
-  // stored into the other buffer in the same basic block so that the
-  // compiler can interleave that VALU/LDS work with the MFMAs
-  auto body = [&](int buf, const Pre& P, const h8* F, auto sk) {
+  auto products = [&](int buf, auto sk) {
     const int hk = lane >> 5;
-    stage(P, buf ^ 1);
 #pragma unroll
     for (int st = 0; st < 2; st++) {
       const int g = 2 * st + hk;
-      const h8 arh = F[4 * st + 0], arl = F[4 * st + 1], aih = F[4 * st + 2], ail = F[4 * st + 3];
-      const h8 narh = neg(arh), narl = neg(arl);  // Im = a_i b_r + (-a_r) b_i
+      h8 arh[2], arl[2], aih[2], ail[2], brh[2], brl[2], bih[2], bil[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        const int Ic = 64 * wc + 32 * i + (lane & 31);
+        arh[i] = As[buf][4 * g + 0][Ic];
+        arl[i] = As[buf][4 * g + 1][Ic];
+        aih[i] = As[buf][4 * g + 2][Ic];
+        ail[i] = As[buf][4 * g + 3][Ic];
+      }
 #pragma unroll
       for (int a = 0; a < 2; a++) {
-        if (decltype(sk)::value && skip[a]) continue;
         const int R = 64 * wr + 32 * a + (lane & 31);
-        const h8 brh = Bs[buf][4 * g + 0][R];
-        const h8 brl = Bs[buf][4 * g + 1][R];
-        const h8 bih = Bs[buf][4 * g + 2][R];
-        const h8 bil = Bs[buf][4 * g + 3][R];
-        f16v r = accR[a], m = accI[a];
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arh, r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, aih, m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, arh, r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, aih, m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arl, r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, ail, m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, aih, r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, narh, m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, aih, r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, narh, m, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, ail, r, 0, 0, 0);
-        m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, narl, m, 0, 0, 0);
-        accR[a] = r;
-        accI[a] = m;
+        brh[a] = Bs[buf][4 * g + 0][R];
+        brl[a] = Bs[buf][4 * g + 1][R];
+        bih[a] = Bs[buf][4 * g + 2][R];
+        bil[a] = Bs[buf][4 * g + 3][R];
       }
-    }
-  };
-
-  // Pipeline: chunk ch computes from LDS buffer ch&1 while chunk ch+1 (in
-  // registers since the previous step) is stored into the other buffer and
-  // chunk ch+2 loads.  Past the end the last chunk is loaded and stored
-  // again (harmless: that buffer is not read any more).
-  const long last = nch - 1;
-  int kf = (int)(ch0 / npc), cf = (int)(ch0 % npc);  // next chunk to fetch
-  long nf = 0;                                        // its index, relative to ch0
-  auto fetch_next = [&](Pre& P) {
-    fetch(P, kf, cf);
-    if (nf < last) {
-      nf++;
-      if (++cf == npc) { cf = 0; kf++; }
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        const h8 narh = neg(arh[i]), narl = neg(arl[i]);  // Im = a_i b_r + (-a_r) b_i
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          if (decltype(sk)::value && skip[a][i]) continue;
+          f16v r = accR[a][i], m = accI[a][i];
+          r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh[a], arh[i], r, 0, 0, 0);
+          m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh[a], aih[i], m, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl[a], arh[i], r, 0, 0, 0);
+          m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl[a], aih[i], m, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh[a], arl[i], r, 0, 0, 0);
+          m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh[a], ail[i], m, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih[a], aih[i], r, 0, 0, 0);
+          m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih[a], narh, m, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil[a], aih[i], r, 0, 0, 0);
+          m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil[a], narh, m, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih[a], ail[i], r, 0, 0, 0);
+          m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih[a], narl, m, 0, 0, 0);
+          accR[a][i] = r;
+          accI[a][i] = m;
+        }
+      }
     }
   };
   // the chunk loop, with the tile skips compiled in only where an item has
   // one (so that the products of the others stay one basic block)
   auto run = [&](auto sk) {
-    int ki = (int)(ch0 / npc), ci = (int)(ch0 % npc);  // next I-fragment chunk
-    long ni = 0;
-    auto fetch_i_next = [&](h8* F) {
-      fetch_i(F, ki, ci);
-      if (ni < last) {
-        ni++;
-        if (++ci == npc) { ci = 0; ki++; }
-      }
-    };
     zero();
-    Pre P0, P1;
-    h8 F0[8], F1[8];
-    fetch_i_next(F0);
-    fetch_next(P0);
-    stage(P0, 0);
-    fetch_next(P1);
-    __syncthreads();
+    __syncthreads();  // chunk 0 staged
     int g = 0;
     for (long ch = 0; ch < nch; ch += 2) {
-      fetch_next(P0);    // column side of chunk ch+2
-      fetch_i_next(F1);  // I side of chunk ch+1
-      body(0, P1, F0, sk);  // computes ch, stores ch+1
+      products(0, sk);
       if (++g == G || ch == last) {
         flush(sk);
         zero();
@@ -401,9 +433,7 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
       }
       __syncthreads();
       if (ch == last) break;
-      fetch_next(P1);    // chunk ch+3
-      fetch_i_next(F0);  // chunk ch+2
-      body(1, P0, F1, sk);  // computes ch+1, stores ch+2
+      products(1, sk);
       if (++g == G || ch + 1 == last) {
         flush(sk);
         zero();
@@ -412,7 +442,7 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
       __syncthreads();
     }
   };
-  if (skip[0] | skip[1])
+  if (skip[0][0] | skip[0][1] | skip[1][0] | skip[1][1])
     run(std::true_type{});
   else
     run(std::false_type{});
@@ -430,6 +460,7 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
                                                        const double* __restrict__ ws, int G, int nfull, int sp,
                                                        double2* __restrict__ eps, double2* __restrict__ part) {
   using namespace cm;
+  __shared__ h8 As[2][16][TB];  // [buf][g*4 + plane][I]
   __shared__ h8 Bs[2][16][TB];  // [buf][g*4 + plane][merged column], weighted
   const int b = blockIdx.x;
   const long nch = (long)nkc * npc;
@@ -450,7 +481,7 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed(const h8* __restrict__ S,
   }
   int It, Nc;
   item_of(item, pb, nom, It, Nc);
-  chi0_cols(S, M, strideK, np, pb, pbp, npc, c0, c1, wt, nom, cs, ws, G, It, Nc, eps, slot, Bs);
+  chi0_cols(S, M, strideK, np, pb, pbp, npc, c0, c1, wt, nom, cs, ws, G, It, Nc, eps, slot, As, Bs);
 }
 
 // eps[I, J, w] += sum over s of the part slots (t*sp + s) of split item
@@ -734,6 +765,7 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed_batch(const Chi0Q* __rest
                                                              const int4* __restrict__ items,
                                                              const int* __restrict__ xoff, int nom, int G) {
   using namespace cm;
+  __shared__ h8 As[2][16][TB];
   __shared__ h8 Bs[2][16][TB];
   // block b runs on XCD b % 8 (workgroups are dealt to the XCDs round
   // robin, and each XCD starts its workgroups in order) and takes item
@@ -744,7 +776,7 @@ __global__ void __launch_bounds__(512) pygw_chi0_mixed_batch(const Chi0Q* __rest
   const int4 it = items[o + i];  // (q, It, Nc)
   const Chi0Q d = qs[it.x];
   chi0_cols(d.S, d.M, 0, d.np, d.pb, d.pbp, d.npc, 0, d.npc, d.wt, nom, d.cs, d.ws, G, it.y, it.z, d.eps,
-            nullptr, Bs);
+            nullptr, As, Bs);
 }
 
 // Host-side plan of a batch (opaque to callers): per-q sizes and pointers,
