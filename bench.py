@@ -746,6 +746,82 @@ def run_gw0(args):
     print(json.dumps(rec))
 
 
+def run_gamma(args):
+    """The q = 0 step of the C5 chain with PyGW's head/wings terms: the wing
+    accumulation beside chi0 (pygw.py:3192-3193), the inversion with the
+    head/wings update (pygw.py:3202-3263) and the head/wing terms of MWM
+    (pygw.py:3359-3364), timed beside the q != 0 stages they accompany
+    (split-precision M; the q = 0 extras in FP64)."""
+    import torch
+
+    from pygw_amd import polarization as P
+    wl = ChainWorkload()
+    b, base = wl.b, wl.base
+    kq, M, Ms, enk_q = wl.q_inputs((0, 0, 0), "mixed")
+    nk, npair = M.shape[0], M.shape[1]
+    rng = np.random.default_rng(BASE_SEED_GW0 + 1)
+    # synthetic momentum matrix elements / transition energies, and head
+    pm = torch.as_tensor((rng.normal(size=(nk, npair)) + 1j * rng.normal(size=(nk, npair))) * 0.05, device="cuda")
+    head0 = torch.as_tensor(2.0 + 3.0 / (1.0 + wl.om ** 2) + 0j, device="cuda")
+    nom = base.n_om
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(8)]
+    s = torch.cuda.current_stream()
+
+    def step():
+        ev[0].record(s)
+        kcw = P.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, wl.om, wl.tetc, layout="native",
+                                  check=False)
+        ev[1].record(s)
+        eps = P.chi0_accumulate(M, kcw, precision="mixed")
+        ev[2].record(s)
+        w1 = P.chi0_wings(M, kcw, pm)
+        ev[3].record(s)
+        e2 = eps.clone()
+        ev[4].record(s)
+        P.eps_inverse(e2, check=False)
+        ev[5].record(s)
+        eps, head, w1, w2, emac = P.eps_inverse_head(eps, head0, w1, check=False)
+        ev[6].record(s)
+        mwm = P.calc_mwm(Ms, eps, wl.wkq, precision="mixed")
+        ev[7].record(s)
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(s)
+        P.calc_mwm_head(Ms, mwm, head, w1, w2, 1.0, 1.0, 0, wl.nbgw, wl.nb2)
+        t1.record(s)
+        torch.cuda.synchronize()
+        return np.array([ev[i].elapsed_time(ev[i + 1]) / 1e3 for i in (1, 2, 4, 5, 6)] +
+                        [t0.elapsed_time(t1) / 1e3]), emac
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    st = np.zeros(6)
+    for _ in range(args.steps):
+        t, emac = step()
+        st += t
+    st /= args.steps
+    chi0, wings, inv, inv_head, mwm, mwm_head = st
+    extra = wings + (inv_head - inv) + mwm_head
+    pb, K = base.pb, nk * npair
+    f_w = 8.0 * pb * K * nom
+    print(f"[bench] gamma extras {extra * 1e3:.2f} ms (chi0 {chi0:.3f} s)", file=sys.stderr, flush=True)
+    rec = {"metric": "q = 0 head/wings extras, seconds per q = 0 step (wings + head/wings update + MWM head terms)",
+           "value": extra, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": extra * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64 (wings, head update, MWM head); fp16x3-split chi0/MWM beside them", "data": "synthetic",
+           "config": {"workload": f"gamma: the c5 chain at q = 0 (8x8x8 k, {b.n_o}+{b.n_e} bands, PB={pb}, {nom} "
+                                  f"Matsubara points, MWM rows of {wl.nbgw}x{wl.nb2} at {len(wl.klist)} k); "
+                                  "synthetic momentum matrix elements and head"},
+           "stages_s": {"chi0_mixed": chi0, "chi0_wings": wings, "eps_inverse": inv,
+                        "eps_inverse_head": inv_head, "mwm_mixed": mwm, "mwm_head": mwm_head},
+           "roofline": {"bound": "mfma", "kernel": "pygw_gm_wings", "achieved": f_w / wings / 1e12,
+                        "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": f_w / wings / PEAK_FP64,
+                        "traffic": None, "flops_per_launch": f_w,
+                        "note": "8 FLOP per (I, pair, frequency); time of the whole wing call"},
+           "emac_w0": [complex(emac[0, 0]).real, complex(emac[0, 1]).real]}
+    print(json.dumps(rec))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -758,7 +834,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.steps is None:
-        args.steps = {"c5": 2, "c2": 8, "c4": 3, "gw0": 2}.get(args.config, 1)
+        args.steps = {"c5": 2, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2}.get(args.config, 1)
     if args.copy_calibration:
         # one copy kernel that reads and writes exactly 1 GiB: the ratio of its
         # FETCH_SIZE / WRITE_SIZE counters to 1 GiB scales the kernels' counters
@@ -776,6 +852,9 @@ def main():
         return
     if args.config == "gw0":
         run_gw0(args)
+        return
+    if args.config == "gamma":
+        run_gamma(args)
         return
     mixed = not args.no_mixed
     wl = Workload(args.config, args.nq, mixed, args.fp64_kfrac)
