@@ -1,0 +1,79 @@
+"""C-ABI argument checks and workspace queries: every entry point rejects
+null arrays and non-positive sizes with PYGW_E_ARG (-10) before touching a
+device, and the workspace queries give the documented sizes.  Runs on the
+host (no device calls)."""
+import ctypes as C
+import os
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "pygw_amd", "libpygw_hip.so")
+pytestmark = pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+E_ARG = -10
+
+
+def _fn(name):
+    from pygw_amd import _lib
+    return getattr(_lib.lib(), name)
+
+
+def _zeros(fn):
+    """Arguments of the signature all null / zero."""
+    out = []
+    for t in fn.argtypes:
+        if t in (C.c_double,):
+            out.append(0.0)
+        elif t is C.c_void_p or (isinstance(t, type) and issubclass(t, C._Pointer)):
+            out.append(None)
+        else:
+            out.append(0)
+    return out
+
+
+@pytest.mark.parametrize("name", [
+    "pygw_tetra_weights", "pygw_tetra_weights_svd", "pygw_chi0_accumulate", "pygw_chi0_autotune",
+    "pygw_eps_inverse", "pygw_mwm", "pygw_selfc_spline", "pygw_selfc_convolve", "pygw_pade_qp",
+    "pygw_pade_qp_gw0", "pygw_selfc_qsum", "pygw_gw0_enk", "pygw_tetra_idos", "pygw_chi0_wings",
+    "pygw_eps_inverse_head", "pygw_mwm_head", "pygw_chi0_batch_run", "pygw_chi0_batch_autotune",
+    "pygw_chi0_batch_free", "pygw_chi0_workspace", "pygw_eps_inverse_workspace", "pygw_mwm_workspace",
+    "pygw_tetra_weights_workspace", "pygw_chi0_wings_workspace", "pygw_eps_inverse_head_workspace",
+])
+def test_null_and_zero_arguments_rejected(name):
+    fn = _fn(name)
+    assert fn(*_zeros(fn)) == E_ARG
+
+
+def test_chi0_workspace_sizes():
+    q = _fn("pygw_chi0_workspace")
+    s = [C.c_size_t(0) for _ in range(4)]
+    assert q(900, 2400, 64, 1, 1, C.byref(s[0])) == 0
+    assert q(900, 2400, 64, 2, 1, C.byref(s[1])) == 0
+    assert q(900, 2400, 64, 512, 1, C.byref(s[2])) == 0
+    per_k = s[1].value - s[0].value
+    # planes (16 h8 of 16 B per 32-pair chunk and padded column) + FP32 weights
+    assert per_k == 75 * 16 * 1024 * 16 + 64 * 75 * 32 * 4
+    # the slab is capped at 1 GiB of k-points; the part slots are 64 MiB
+    assert s[2].value - s[0].value == ((1 << 30) // per_k - 1) * per_k
+    fixed = s[0].value - per_k
+    assert fixed >= 256 * 128 * 128 * 16
+    assert q(900, 2400, 64, 1, 0, C.byref(s[3])) == 0 and s[3].value == 0   # FP64 needs none
+    assert q(900, 2400, 64, 1, 2, C.byref(s[3])) == E_ARG
+
+
+def test_other_workspace_queries():
+    b = C.c_size_t(0)
+    assert _fn("pygw_eps_inverse_workspace")(900, 64, C.byref(b)) == 0 and b.value > 0
+    assert _fn("pygw_mwm_workspace")(64, 1120, 900, 64, 0, C.byref(b)) == 0 and b.value == 0
+    assert _fn("pygw_mwm_workspace")(64, 1120, 900, 64, 1, C.byref(b)) == 0 and b.value > 0
+    assert _fn("pygw_tetra_weights_workspace")(512, 2400, 64, C.byref(b)) == 0 and b.value > 0
+    assert _fn("pygw_chi0_wings_workspace")(900, 64, 512, C.byref(b)) == 0 and b.value > 0
+    a = C.c_size_t(0)
+    assert _fn("pygw_eps_inverse_head_workspace")(900, 64, C.byref(a)) == 0
+    _fn("pygw_eps_inverse_workspace")(900, 64, C.byref(b))
+    assert a.value > b.value
+    pb = (C.c_int * 3)(302, 700, 1099)
+    npair = (C.c_int * 3)(60000, 77000, 90000)
+    assert _fn("pygw_chi0_batch_workspace")(3, pb, npair, 48, C.byref(b)) == 0 and b.value > 0
+    bad = (C.c_int * 3)(302, 0, 1099)
+    assert _fn("pygw_chi0_batch_workspace")(3, bad, npair, 48, C.byref(b)) == E_ARG
