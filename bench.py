@@ -482,7 +482,7 @@ def run_batch(args):
     """C4: n_q (default 32) q-points of a metal (3 bands crossing E_F) on
     the 8x8x8 k-mesh, PB_q ~ U{300..1100}, per-(q, k) windows of 40-260
     pairs, 48 frequencies.  One step = the whole batch: the weights of every
-    q gathered to its window pairs, then one persistent split-precision chi0
+    q gathered to its window pairs, then one split-precision chi0
     launch over all q and eps^-1 of every q (both captured in HIP graphs,
     fold interval autotuned).  The FP64 path runs the same q one after
     another (chi0_accumulate on the complex64-rounded M, then eps^-1).
@@ -610,7 +610,7 @@ def run_batch(args):
         "roofline": {"bound": "mfma", "kernel": kern, "achieved": f_tot / tm[1] / 1e12, "peak": PEAK_FP16 / 1e12,
                      "unit": "TFLOP/s", "frac": f_tot / tm[1] / PEAK_FP16, "traffic": load_traffic("c4", kern),
                      "flops_per_launch": f_tot, "executed_flops_per_launch": 3 * f_tot,
-                     "timed": "HIP events around the captured batch (prep + persistent contraction + mirror)"},
+                     "timed": "HIP events around the captured batch (prep + contraction + mirror)"},
         "fp64": {"s_per_q": s_64, "stages_s_per_q": dict(zip(("polarization_weights", "chi0", "eps_inverse"),
                                                               map(float, t64 / nq))),
                  "chi0_tflops": f_tot / t64[1] / 1e12, "chi0_frac_fp64_peak": f_tot / t64[1] / PEAK_FP64},

@@ -182,7 +182,7 @@ extern "C" int pygw_chi0_set_schedule(int precision, int pb, int n_pairs, int n_
   return PYGW_OK;
 }
 
-// --- batched split-precision contraction over many q (one persistent launch)
+// --- batched split-precision contraction over many q (one launch)
 namespace {
 bool batch_args_ok(int n_q, const int* pb, const int* n_pairs, int n_om) {
   if (n_q <= 0 || !pb || !n_pairs || n_om <= 0 || n_om > 65535) return false;

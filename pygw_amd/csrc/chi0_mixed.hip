@@ -373,51 +373,66 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
   };
   auto products = [&](int buf, auto sk) {
     const int hk = lane >> 5;
+    // fragments of K steps 0 and 1; a step-1 fragment is read from LDS as
+    // soon as the step-0 fragment it replaces has fed its last MFMA, so the
+    // LDS latency of step 1 hides behind the rest of step 0
+    h8 A[2][2][4], B[2][2][4];  // [st][i or a][plane]
+    auto ldA = [&](int st, int i) {
+      const int g = 2 * st + hk, Ic = 64 * wc + 32 * i + (lane & 31);
 #pragma unroll
-    for (int st = 0; st < 2; st++) {
-      const int g = 2 * st + hk;
-      h8 arh[2], arl[2], aih[2], ail[2], brh[2], brl[2], bih[2], bil[2];
+      for (int q = 0; q < 4; q++) A[st][i][q] = As[buf][4 * g + q][Ic];
+    };
+    auto ldB = [&](int st, int a) {
+      const int g = 2 * st + hk, R = 64 * wr + 32 * a + (lane & 31);
 #pragma unroll
-      for (int i = 0; i < 2; i++) {
-        const int Ic = 64 * wc + 32 * i + (lane & 31);
-        arh[i] = As[buf][4 * g + 0][Ic];
-        arl[i] = As[buf][4 * g + 1][Ic];
-        aih[i] = As[buf][4 * g + 2][Ic];
-        ail[i] = As[buf][4 * g + 3][Ic];
-      }
-#pragma unroll
-      for (int a = 0; a < 2; a++) {
-        const int R = 64 * wr + 32 * a + (lane & 31);
-        brh[a] = Bs[buf][4 * g + 0][R];
-        brl[a] = Bs[buf][4 * g + 1][R];
-        bih[a] = Bs[buf][4 * g + 2][R];
-        bil[a] = Bs[buf][4 * g + 3][R];
-      }
-#pragma unroll
-      for (int i = 0; i < 2; i++) {
-        const h8 narh = neg(arh[i]), narl = neg(arl[i]);  // Im = a_i b_r + (-a_r) b_i
-#pragma unroll
-        for (int a = 0; a < 2; a++) {
-          if (decltype(sk)::value && skip[a][i]) continue;
-          f16v r = accR[a][i], m = accI[a][i];
-          r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh[a], arh[i], r, 0, 0, 0);
-          m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh[a], aih[i], m, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl[a], arh[i], r, 0, 0, 0);
-          m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl[a], aih[i], m, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh[a], arl[i], r, 0, 0, 0);
-          m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh[a], ail[i], m, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih[a], aih[i], r, 0, 0, 0);
-          m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih[a], narh, m, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil[a], aih[i], r, 0, 0, 0);
-          m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil[a], narh, m, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih[a], ail[i], r, 0, 0, 0);
-          m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih[a], narl, m, 0, 0, 0);
-          accR[a][i] = r;
-          accI[a][i] = m;
-        }
-      }
-    }
+      for (int q = 0; q < 4; q++) B[st][a][q] = Bs[buf][4 * g + q][R];
+    };
+    auto tile = [&](int st, int i, int a) {
+      if (decltype(sk)::value && skip[a][i]) return;
+      const h8 arh = A[st][i][0], arl = A[st][i][1], aih = A[st][i][2], ail = A[st][i][3];
+      const h8 brh = B[st][a][0], brl = B[st][a][1], bih = B[st][a][2], bil = B[st][a][3];
+      const h8 narh = neg(arh), narl = neg(arl);  // Im = a_i b_r + (-a_r) b_i
+      f16v r = accR[a][i], m = accI[a][i];
+      r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arh, r, 0, 0, 0);
+      m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, aih, m, 0, 0, 0);
+      r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, arh, r, 0, 0, 0);
+      m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brl, aih, m, 0, 0, 0);
+      r = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, arl, r, 0, 0, 0);
+      m = __builtin_amdgcn_mfma_f32_32x32x16_f16(brh, ail, m, 0, 0, 0);
+      r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, aih, r, 0, 0, 0);
+      m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, narh, m, 0, 0, 0);
+      r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, aih, r, 0, 0, 0);
+      m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bil, narh, m, 0, 0, 0);
+      r = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, ail, r, 0, 0, 0);
+      m = __builtin_amdgcn_mfma_f32_32x32x16_f16(bih, narl, m, 0, 0, 0);
+      accR[a][i] = r;
+      accI[a][i] = m;
+    };
+    // (the scheduling barriers keep the reads where they are written)
+    ldA(0, 0);
+    ldB(0, 0);
+    ldA(0, 1);
+    ldB(0, 1);
+    tile(0, 0, 0);
+    tile(0, 0, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    ldA(1, 0);   // A(0, 0) done
+    __builtin_amdgcn_sched_barrier(0);
+    tile(0, 1, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    ldB(1, 0);   // B(0, 0) done
+    __builtin_amdgcn_sched_barrier(0);
+    tile(0, 1, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    ldA(1, 1);   // A(0, 1), B(0, 1) done
+    ldB(1, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    tile(1, 0, 0);
+    tile(1, 0, 1);
+    tile(1, 1, 0);
+    tile(1, 1, 1);
   };
+
   // the chunk loop, with the tile skips compiled in only where an item has
   // one (so that the products of the others stay one basic block)
   auto run = [&](auto sk) {
@@ -742,14 +757,14 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }
 // ---------------------------------------------------------------------------
-// Batched contraction over many q-points in one persistent launch.  Each q
+// Batched contraction over many q-points in one launch.  Each q
 // has its own PB and pair count (all pairs of all k concatenated: only the
 // (k, p) list matters to the sum), its own operands, scales and eps.  Work
 // item = (q, I tile, merged-column chunk).  The items, in order of
 // decreasing cost (the q's chunk count) and then I tile, are cut into
 // segments of 32 (one per CU of an XCD) dealt to the 8 XCDs round robin;
-// each XCD starts its items in list order, so blocks that run together on
-// an XCD share an I slab in its L2.
+// one block per item, block b taking item b >> 3 of XCD b & 7's list, so
+// blocks that run together on an XCD share an I slab in its L2.
 // ---------------------------------------------------------------------------
 struct Chi0Q {
   const h8* S;        // fragment planes [npc][16][pbp]

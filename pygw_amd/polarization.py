@@ -423,7 +423,7 @@ class QPlan:
 
 class Chi0Batch:
     """Split-precision chi0 for a batch of q-points with their own PB and
-    pair counts, in one persistent launch (the q loop of G0W0.Compute,
+    pair counts, in one launch (the q loop of G0W0.Compute,
     pygw.py:3654-3718, around the contraction of calc_eps,
     pygw.py:3161-3200).
 
