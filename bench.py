@@ -14,12 +14,14 @@ contractions; value = split-precision seconds per q when its chi0 is within
 1e-6 (relative Frobenius, every frequency) of FP64 on the same
 complex64-rounded M, else the FP64 seconds per q.
 
-    python bench.py [--steps K] [--warmup W] [--config c5|c2|c3|c4] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--config c5|c2|c3|c4|gw0|gamma|coulomb] [--no-cpu]
 
 --config c2 / c3: chi0 + eps^-1 only (FP64 and split precision) at
 8x8x8 k, 16+48 bands, PB = 600 / 10x10x10 k, 32+200 bands, PB = 1200.
 --config c4: a metal batch of 32 q (PB_q 300..1100, ragged per-k pair
 windows) through the one-launch batched chi0, then eps^-1 per q.
+--config gw0 / gamma: the GW0 iteration on stored mwm / the q = 0 extras.
+--config coulomb: the v^(1/2) basis (Coul_setev) and minm = mmat conj(barcvm).
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs one GPU.
 """
 from __future__ import annotations
@@ -822,6 +824,85 @@ def run_gamma(args):
     print(json.dumps(rec))
 
 
+def run_coulomb(args):
+    """The v^(1/2) basis of one q and the matrix elements in it at C5 size:
+    Coul_setev on a 1000 x 1000 Coulomb matrix (900 eigenvalues kept at
+    barcevtol = 1e-7, so PB = 900; pygw.py:2440, 2505-2545), then minm =
+    mmat conj(barcvm) for all 512 k x 2400 pairs (pygw.py:2666), written as
+    complex64 (the split-precision chi0's operand) and as complex128."""
+    import torch
+
+    from pygw_amd import coulomb, synth
+    mb, nk, npair, evtol = 1000, 512, 2400, 1e-7
+    vmat, lam = synth.coulomb_matrix(mb, 100)
+    vd = torch.as_tensor(vmat, device="cuda")
+    K = nk * npair
+    g = torch.Generator(device="cuda")
+    g.manual_seed(20261211)
+    mm = torch.randn((K, mb), dtype=torch.complex128, device="cuda", generator=g)
+    out64 = torch.empty((K, 900), dtype=torch.complex64, device="cuda")
+    s = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+
+    def step():
+        ev[0].record(s)
+        cb = coulomb.coul_setev(vd, evtol)
+        ev[1].record(s)
+        coulomb.calc_minm_vhalf(mm, cb.barcvm, complex64=True, out=out64[:, :cb.matsize])
+        ev[2].record(s)
+        o128 = coulomb.calc_minm_vhalf(mm, cb.barcvm)
+        ev[3].record(s)
+        torch.cuda.synchronize()
+        del o128
+        return np.array([ev[i].elapsed_time(ev[i + 1]) / 1e3 for i in range(3)]), cb
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    st = np.zeros(3)
+    for _ in range(args.steps):
+        t, cb = step()
+        st += t
+    st /= args.steps
+    t_ev, t_m32, t_m128 = st
+    assert cb.matsize == 900
+    ev_err = float(np.abs(cb.ev.cpu().numpy() - lam).max() / lam.max())
+    f_m = 8.0 * K * mb * cb.matsize
+    value = t_ev + t_m32
+    print(f"[bench] coulomb setev {t_ev:.3f} s ({cb.sweeps} sweeps), minm c64 {t_m32:.3f} s "
+          f"({f_m / t_m32 / 1e12:.1f} TF/s), c128 {t_m128:.3f} s", file=sys.stderr, flush=True)
+    rec = {"metric": "Coul_setev + minm = mmat conj(barcvm) seconds per q (all k)", "value": value, "unit": "s/q",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": value * 1e3,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64 (c64 output)",
+           "data": "synthetic",
+           "config": {"workload": f"coulomb: mbsize {mb} Hermitian Coulomb matrix (900 eigenvalues 4pi/g^2, 100 "
+                                  f"below barcevtol {evtol}), PB = 900; minm for {nk} k x {npair} pairs"},
+           "stages_s": {"coul_setev": t_ev, "minm_c64": t_m32, "minm_c128": t_m128},
+           "jacobi_sweeps": cb.sweeps, "max_eigenvalue_error_rel": ev_err,
+           "roofline": {"bound": "mfma", "kernel": "pygw_minm_vhalf", "achieved": f_m / t_m32 / 1e12,
+                        "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": f_m / t_m32 / PEAK_FP64,
+                        "traffic": None, "flops_per_launch": f_m,
+                        "note": "8 FLOP per (row, basis function, kept vector); complex64 output"}}
+    if not args.no_cpu:
+        rec["cpu_baseline"] = coulomb_cpu_baseline(vmat, mm[:2 * npair].cpu().numpy(), evtol, nk // 2, t_ev)
+    print(json.dumps(rec))
+
+
+def coulomb_cpu_baseline(vmat, mm_sample, evtol, scale, t_gpu_ev):
+    """PyGW's own steps on the host: scipy eigh (LAPACK, lower) + the
+    Coul_setev selection, then numpy matmul(mmat, conj(barcvm)) on 2 of 512
+    k, extrapolated."""
+    from oracle import coulomb as O
+    nthr = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    _, _, _, bv, _ = O.coul_setev(vmat, evtol)
+    t1 = time.perf_counter()
+    O.minm(mm_sample, bv)
+    t2 = time.perf_counter()
+    return {"value": (t1 - t0) + (t2 - t1) * scale, "unit": "s/q", "cores": nthr, "kind": "port",
+            "sample": f"scipy eigh of the {vmat.shape[0]}-square matrix in full; numpy matmul on 2 of 512 k (x{scale})",
+            "stages_s": {"coul_setev": t1 - t0, "minm_extrapolated": (t2 - t1) * scale}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -834,7 +915,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.steps is None:
-        args.steps = {"c5": 2, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2}.get(args.config, 1)
+        args.steps = {"c5": 2, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2, "coulomb": 2}.get(args.config, 1)
     if args.copy_calibration:
         # one copy kernel that reads and writes exactly 1 GiB: the ratio of its
         # FETCH_SIZE / WRITE_SIZE counters to 1 GiB scales the kernels' counters
@@ -855,6 +936,9 @@ def main():
         return
     if args.config == "gamma":
         run_gamma(args)
+        return
+    if args.config == "coulomb":
+        run_coulomb(args)
         return
     mixed = not args.no_mixed
     wl = Workload(args.config, args.nq, mixed, args.fp64_kfrac)

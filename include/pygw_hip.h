@@ -10,7 +10,8 @@
  *   - the return value is 0 on success, < 0 on error:
  *       -10 invalid argument, -11 HIP runtime error,
  *       -12 geometry/classification error (where PyGW's Fortran would `stop`),
- *       -13 matrix not Hermitian positive definite;
+ *       -13 matrix not Hermitian positive definite,
+ *       -14 eigensolver not converged;
  *   - where an `info` pointer is given (host memory), the call synchronises
  *     `stream` and fills it; pass NULL to stay asynchronous;
  *   - `workspace` may be NULL (the library then allocates on `stream`);
@@ -284,6 +285,50 @@ int pygw_eps_inverse_head(void* eps, int pb, int n_om, void* head, void* epsw1, 
 int pygw_mwm_head(const void* M_selfe, int n_k, int nb1, int nb2, int pb, int ncg_c, int complex64, const void* head,
                   const void* epsw1, const void* epsw2, int n_om, double coefs1, double coefs2, void* mwm,
                   void* stream);
+
+/* ------------------------------------------------------------------------
+ * The v^(1/2) basis of the Coulomb matrix and the matrix elements in it.
+ *
+ * pygw_coul_setev replaces linalg.eigh(Vmat) (pygw.py:2440), the q = 0
+ *   choice of immax (pygw.py:2447-2454) and Coul_setev with
+ *   Real_sqrt_Vcoul = False (pygw.py:18, 2505-2545).
+ *   vmat: complex128 C order [mbsize][mbsize], Hermitian; only its lower
+ *   triangle is read (like eigh).  Eigenvalues and eigenvectors come from a
+ *   parallel cyclic Jacobi method on the device.
+ *   - ev [mbsize]: ascending (the eigh order).
+ *   - vecs: optional, NULL or Fortran [mbsize, mbsize] eigenvectors in that
+ *     order (PyGW's self.Vmat).
+ *   - wi0 (q = 0 only, else NULL): complex128 [mbsize] overlaps of the basis
+ *     with the constant function.  *immax is the eigenvector (eigh order)
+ *     maximising |sum_i wi0[i] conj(V[i][j])|; it is dropped
+ *     (iop_coul_x = 0) or kept whatever its eigenvalue (iop_coul_x != 0).
+ *   - The other eigenvectors are kept when ev > evtol.
+ *   - *matsize = the kept count (PB).
+ *   - barcev [mbsize] (first matsize used): kept eigenvalues, decreasing.
+ *   - barcvm: Fortran [mbsize, matsize], leading dimension mbsize, with
+ *     mbsize columns allocated; column m = v_j sqrt|ev_j|.
+ *   Eigenvector phases (and bases of degenerate eigenspaces) differ from
+ *   LAPACK's; the products downstream do not depend on them.
+ *   The call synchronises the stream once per sweep.
+ *   - *n_sweeps (optional): the sweeps run.
+ *   - Returns PYGW_E_NOCONV after 80 sweeps without convergence.
+ *   workspace: NULL (allocated) or pygw_coul_setev_workspace bytes.
+ *   mbsize <= 16384.
+ * pygw_minm_vhalf replaces minm = la.matmul(mmat, conj(barcvm))
+ *   (pygw.py:2666) and minc = la.matmul(mmat_c, conj(barcvm[:loctmatsize,
+ *   :])) (pygw.py:2684):
+ *     minm[r][m] = sum_{i < n_in} mmat[r*ld_mmat + i] conj(barcvm[m*ld_barcvm + i]).
+ *   mmat: complex128, rows of ld_mmat >= n_in.
+ *   minm: row-major [n_rows][matsize], complex128 or (complex64_out = 1)
+ *   the FP64 result rounded to complex64 (the operand of the
+ *   split-precision chi0).  FP64 MFMA.
+ * ------------------------------------------------------------------------ */
+int pygw_coul_setev_workspace(int mbsize, size_t* bytes);
+int pygw_coul_setev(const void* vmat, int mbsize, double evtol, const void* wi0, int iop_coul_x, double* ev,
+                    double* barcev, void* barcvm, void* vecs, int* matsize, int* immax, int* n_sweeps,
+                    void* workspace, size_t ws_bytes, void* stream);
+int pygw_minm_vhalf(const void* mmat, long n_rows, int n_in, long ld_mmat, const void* barcvm, int ld_barcvm,
+                    int matsize, int complex64_out, void* minm, void* stream);
 
 #ifdef __cplusplus
 }

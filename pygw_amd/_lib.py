@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libpygw_hip.so")
 _lib = None
 
 ERRORS = {-10: "invalid argument", -11: "HIP runtime error", -12: "geometry/classification error",
-          -13: "matrix not Hermitian positive definite"}
+          -13: "matrix not Hermitian positive definite", -14: "eigensolver did not converge"}
 
 
 class PygwError(RuntimeError):
@@ -77,6 +77,12 @@ _SIGS = {
                                         C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
     "pygw_mwm_head": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                 C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
+    "pygw_coul_setev_workspace": (C.c_int, [C.c_int, C.POINTER(C.c_size_t)]),
+    "pygw_coul_setev": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "pygw_minm_vhalf": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_long, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                  C.c_void_p, C.c_void_p]),
 }
 
 

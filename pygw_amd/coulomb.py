@@ -1,0 +1,102 @@
+"""The v^(1/2) basis of the Coulomb matrix and its application to the
+product-basis matrix elements, on the gfx950 backend.
+
+* ``coul_setev`` replaces ``linalg.eigh(Vmat)`` at the end of
+  ``Coulomb`` (pygw.py:2440), the q = 0 choice of ``immax``
+  (pygw.py:2447-2454) and ``Coul_setev`` (pygw.py:2505-2545, the
+  ``Real_sqrt_Vcoul = False`` branch, pygw.py:18): eigenvalues above
+  ``evtol`` kept in decreasing order, ``barcvm[:, m] = v_j sqrt|ev_j|``.
+* ``calc_minm_vhalf`` replaces ``minm = la.matmul(mmat, conj(barcvm))``
+  (pygw.py:2666) and, with ``n_in = loctmatsize``, ``minc = la.matmul(
+  mmat_c, conj(barcvm[:loctmatsize, :]))`` (pygw.py:2684).
+
+Eigenvectors are defined up to a phase per column (and a unitary mixing
+within a degenerate eigenvalue): barcvm differs from LAPACK's by such
+factors, which every physical quantity downstream (eps^-1 in the rotated
+basis, MWM, Sigma_c) does not see.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+from .polarization import _dev, _require_device, _stream_ptr
+
+
+@dataclass
+class CoulombBasis:
+    ev: torch.Tensor        # float64 [mbsize], ascending (the eigh order)
+    barcev: torch.Tensor    # float64 [matsize], decreasing
+    barcvm: torch.Tensor    # complex128 [mbsize, matsize] (Fortran-strided view)
+    matsize: int
+    immax: int              # q = 0: eigenvector with the largest overlap with 1 (eigh order); else -1
+    vmat: torch.Tensor | None  # complex128 [mbsize, mbsize] eigenvectors in eigh order, if asked for
+    sweeps: int
+
+
+def coul_setev(vmat, evtol=1e-8, wi0=None, iop_coul_x=0, *, eigenvectors=False, device="cuda"):
+    """vmat: complex128 [mbsize, mbsize] Hermitian Coulomb matrix (its lower
+    triangle is used, like eigh).  wi0: complex128 [mbsize] overlaps <chi|1>
+    of the basis with the constant function (q = 0 only, pygw.py:2448-2450),
+    or None for q != 0."""
+    device = _require_device(device)
+    v = _dev(vmat, torch.complex128, device).contiguous()
+    mb = v.shape[0]
+    if v.dim() != 2 or v.shape[1] != mb:
+        raise ValueError("vmat must be square")
+    w = None if wi0 is None else _dev(wi0, torch.complex128, device).contiguous()
+    if w is not None and w.numel() != mb:
+        raise ValueError("wi0 must have mbsize entries")
+    ev = torch.empty(mb, dtype=torch.float64, device=device)
+    barcev = torch.empty(mb, dtype=torch.float64, device=device)
+    cols = torch.empty((mb, mb), dtype=torch.complex128, device=device)   # column m at cols[m]
+    vecs = torch.empty((mb, mb), dtype=torch.complex128, device=device) if eigenvectors else None
+    ms, im, sw = C.c_int(0), C.c_int(-1), C.c_int(0)
+    _lib.call("pygw_coul_setev", C.c_void_p(v.data_ptr()), mb, float(evtol),
+              None if w is None else C.c_void_p(w.data_ptr()), int(iop_coul_x), C.c_void_p(ev.data_ptr()),
+              C.c_void_p(barcev.data_ptr()), C.c_void_p(cols.data_ptr()),
+              None if vecs is None else C.c_void_p(vecs.data_ptr()), C.byref(ms), C.byref(im), C.byref(sw),
+              None, 0, _stream_ptr(device))
+    m = ms.value
+    return CoulombBasis(ev=ev, barcev=barcev[:m], barcvm=cols[:m].T, matsize=m,
+                        immax=im.value if w is not None else -1,
+                        vmat=None if vecs is None else vecs.T, sweeps=sw.value)
+
+
+def calc_minm_vhalf(mmat, barcvm, n_in=None, *, complex64=False, out=None):
+    """minm[..., m] = sum_{i < n_in} mmat[..., i] conj(barcvm[i, m]).
+
+    mmat: complex128 [..., ld] device tensor, rows contiguous (the last axis
+    may be longer than n_in, e.g. mmat_c against the first loctmatsize rows
+    of barcvm); barcvm: complex128 [mbsize, matsize] with unit stride along
+    mbsize (the view ``CoulombBasis.barcvm``).  Returns complex128 (or
+    complex64 when ``complex64``) [..., matsize]."""
+    if not (mmat.is_cuda and barcvm.is_cuda):
+        raise RuntimeError("calc_minm_vhalf needs device tensors")
+    if mmat.dtype != torch.complex128 or barcvm.dtype != torch.complex128:
+        raise TypeError("mmat and barcvm must be complex128")
+    mb, ms = barcvm.shape
+    if barcvm.stride(0) != 1:
+        raise ValueError("barcvm must have unit stride along mbsize (Fortran order)")
+    ldb = barcvm.stride(1) if ms > 1 else mb
+    ld = mmat.shape[-1]
+    n_in = ld if n_in is None else int(n_in)
+    if n_in > mb or n_in > ld:
+        raise ValueError("n_in exceeds the inner dimension")
+    if mmat.stride(-1) != 1:
+        raise ValueError("mmat rows must be contiguous")
+    rows = mmat.reshape(-1, ld)
+    if rows.data_ptr() != mmat.data_ptr() or (rows.shape[0] > 1 and rows.stride(0) != ld):
+        raise ValueError("mmat rows must be equally spaced")
+    nrows = rows.shape[0]
+    dt = torch.complex64 if complex64 else torch.complex128
+    if out is None:
+        out = torch.empty(mmat.shape[:-1] + (ms,), dtype=dt, device=mmat.device)
+    elif out.dtype != dt or not out.is_contiguous() or out.numel() != nrows * ms:
+        raise ValueError("out must be a contiguous tensor of the result's size and type")
+    _lib.call("pygw_minm_vhalf", C.c_void_p(mmat.data_ptr()), nrows, n_in, ld, C.c_void_p(barcvm.data_ptr()),
+              ldb, ms, int(complex64), C.c_void_p(out.data_ptr()), _stream_ptr(mmat.device))
+    return out

@@ -8,6 +8,7 @@
 #define PYGW_E_HIP -11
 #define PYGW_E_GEOM -12
 #define PYGW_E_NOTPD -13
+#define PYGW_E_NOCONV -14
 
 #define HIP_TRY(x)                                  \
   do {                                              \

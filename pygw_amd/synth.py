@@ -385,3 +385,19 @@ def gw0_system(div=(2, 2, 2), n_occ=4, n_emp=6, nbgw=4, n_core=1, n_om=16, iop_m
     wtet = np.full(atet.shape[0], 1.0 / atet.shape[0])
     return GW0System(tuple(div), e, core, jrk, mwm, sigx, vxc, fm.omega, fm.womeg, fm.omega_precise,
                      fm.womeg_precise, atet, wtet, 2 * n_occ, ibgw)
+
+
+def coulomb_matrix(mbsize=1000, n_null=100, seed=20261211):
+    """Synthetic Hermitian Coulomb matrix [mbsize, mbsize] (complex128, C
+    order) of known spectrum: mbsize - n_null eigenvalues 4 pi / g^2 with
+    g ~ U[0.25, 9] Bohr^-1 (the plane-wave part's range), and n_null
+    near-null eigenvalues 10^U[-12, -8] (the linear dependences that
+    barcevtol removes), in the eigenbasis of a random unitary.  Returns
+    (vmat, eigenvalues)."""
+    rng = np.random.default_rng(seed)
+    g = rng.uniform(0.25, 9.0, mbsize - n_null)
+    lam = np.concatenate([4 * np.pi / g ** 2, 10 ** rng.uniform(-12, -8, n_null)])
+    Q, R = np.linalg.qr(rng.normal(size=(mbsize, mbsize)) + 1j * rng.normal(size=(mbsize, mbsize)))
+    Q = Q * (np.diag(R) / np.abs(np.diag(R)))
+    A = (Q * lam) @ Q.conj().T
+    return (A + A.conj().T) / 2, np.sort(lam)

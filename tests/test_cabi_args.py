@@ -38,6 +38,7 @@ def _zeros(fn):
     "pygw_eps_inverse_head", "pygw_mwm_head", "pygw_chi0_batch_run", "pygw_chi0_batch_autotune",
     "pygw_chi0_batch_free", "pygw_chi0_workspace", "pygw_eps_inverse_workspace", "pygw_mwm_workspace",
     "pygw_tetra_weights_workspace", "pygw_chi0_wings_workspace", "pygw_eps_inverse_head_workspace",
+    "pygw_coul_setev", "pygw_coul_setev_workspace", "pygw_minm_vhalf",
 ])
 def test_null_and_zero_arguments_rejected(name):
     fn = _fn(name)
@@ -77,3 +78,17 @@ def test_other_workspace_queries():
     assert _fn("pygw_chi0_batch_workspace")(3, pb, npair, 48, C.byref(b)) == 0 and b.value > 0
     bad = (C.c_int * 3)(302, 0, 1099)
     assert _fn("pygw_chi0_batch_workspace")(3, bad, npair, 48, C.byref(b)) == E_ARG
+
+
+def test_coulomb_workspace_and_bounds():
+    q = _fn("pygw_coul_setev_workspace")
+    s = C.c_size_t(0)
+    # two ping-pong copies of the (even-padded) matrix and the eigenvectors, complex128
+    assert q(999, C.byref(s)) == 0 and s.value >= 3 * 1000 * 1000 * 16
+    assert q(16385, C.byref(s)) == E_ARG
+    mv = _fn("pygw_minm_vhalf")
+    p = C.c_void_p(16)     # never dereferenced: the checks come first
+    # n_in above a leading dimension, and a bad output kind
+    assert mv(p, 10, 8, 7, p, 8, 4, 0, p, None) == E_ARG
+    assert mv(p, 10, 8, 8, p, 7, 4, 0, p, None) == E_ARG
+    assert mv(p, 10, 8, 8, p, 8, 4, 2, p, None) == E_ARG

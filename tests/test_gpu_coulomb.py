@@ -1,0 +1,121 @@
+"""GPU parity of the Coulomb v^(1/2) basis (pygw_coul_setev: Jacobi
+eigensolver + Coul_setev selection) and of minm = mmat conj(barcvm)
+(pygw_minm_vhalf) against the scipy/numpy oracle (oracle/coulomb.py).
+
+Eigenvectors are compared where they are defined: eigenvalues, kept count
+and order, barcvm barcvm^H (phase-free), |<gpu column, oracle column>| for
+separated eigenvalues, and the residual and orthonormality of the GPU
+eigenvectors themselves."""
+import numpy as np
+import pytest
+
+from oracle import coulomb as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _spectrum(n, kind, rng):
+    if kind == "flat":
+        return rng.uniform(0.5, 2.0, n)
+    if kind == "graded":
+        return np.concatenate([10 ** rng.uniform(-3, 3, n - n // 5), 10 ** rng.uniform(-13, -9, n // 5)])
+    if kind == "coulomb":   # 4 pi / |q + G|^2 - like values and a tail of near-null vectors
+        g = rng.uniform(0.25, 9.0, n - n // 10)
+        return np.concatenate([4 * np.pi / g ** 2, 10 ** rng.uniform(-12, -8, n // 10)])
+    raise ValueError(kind)
+
+
+def _check_basis(A, got, evtol, lam, wi0=None, iop=0):
+    ev, V, barcev, barcvm, immax = O.coul_setev(A, evtol, wi0, iop)
+    nrm = np.abs(lam).max()
+    ev_g = got.ev.cpu().numpy()
+    assert np.abs(ev_g - ev).max() < 1e-12 * nrm
+    assert got.matsize == len(barcev) and got.immax == immax
+    bg = got.barcev.cpu().numpy()
+    assert np.abs(bg - barcev).max() < 1e-12 * nrm
+    B = got.barcvm.cpu().numpy()
+    assert B.shape == barcvm.shape
+    assert np.abs(B @ B.conj().T - barcvm @ barcvm.conj().T).max() < 1e-12 * nrm
+    gap = np.minimum(np.abs(np.diff(np.r_[np.inf, barcev])), np.abs(np.diff(np.r_[barcev, -np.inf])))
+    sep = gap > 1e-6 * nrm
+    ov = np.abs(np.einsum("im,im->m", B.conj(), barcvm))
+    assert np.all(np.abs(ov - np.abs(barcev))[sep] < 1e-9 * nrm)
+    if got.vmat is not None:
+        Vg = got.vmat.cpu().numpy()
+        assert np.abs(A @ Vg - Vg * ev_g).max() < 1e-12 * nrm
+        assert np.abs(Vg.conj().T @ Vg - np.eye(len(ev))).max() < 1e-12
+    return B
+
+
+@pytest.mark.parametrize("n,kind", [(1, "flat"), (2, "flat"), (7, "flat"), (64, "graded"), (301, "graded"),
+                                    (1000, "coulomb")])
+def test_coul_setev_vs_oracle(n, kind):
+    from pygw_amd import coulomb
+    rng = np.random.default_rng(n)
+    lam = _spectrum(n, kind, rng)
+    A, _ = O.hermitian_with_spectrum(lam, rng)
+    print(f"case coul_setev n={n} {kind}", flush=True)
+    for evtol in (-1e-10, 1e-7):
+        got = coulomb.coul_setev(A, evtol, eigenvectors=n <= 301)
+        print(f"  evtol={evtol} matsize={got.matsize} sweeps={got.sweeps}", flush=True)
+        _check_basis(A, got, evtol, lam)
+
+
+def test_coul_setev_lower_triangle_and_q0():
+    """Only the lower triangle is read (the upper one here is garbage); at
+    q = 0 the vector with the largest overlap with wi0 is dropped
+    (iop_coul_x = 0) or kept below evtol (iop_coul_x = 1)."""
+    from pygw_amd import coulomb
+    rng = np.random.default_rng(11)
+    n = 90
+    lam = _spectrum(n, "graded", rng)
+    A, Q = O.hermitian_with_spectrum(lam, rng)
+    Ag = np.tril(A) + np.triu(rng.normal(size=(n, n)), 1)
+    j0 = int(np.argsort(lam)[40])
+    wi0 = 3.0 * Q[:, j0] + 1e-3 * (rng.normal(size=n) + 1j * rng.normal(size=n))
+    print("case coul_setev q=0", flush=True)
+    for iop, evtol in ((0, 1e-7), (1, 1.5 * lam[j0])):
+        got = coulomb.coul_setev(Ag, evtol, wi0, iop)
+        assert got.immax == 40
+        _check_basis(A, got, evtol, lam, wi0, iop)
+
+
+@pytest.mark.parametrize("shape,mb,ms,n_in", [((3, 37), 101, 90, None), ((1007,), 150, 130, 77),
+                                              ((2, 64), 64, 64, None), ((5,), 16, 1, 3)])
+def test_minm_vhalf_vs_oracle(shape, mb, ms, n_in):
+    import torch
+    from pygw_amd import coulomb
+    rng = np.random.default_rng(mb + ms)
+    ld = mb if n_in is None else n_in
+    mm = rng.normal(size=shape + (ld,)) + 1j * rng.normal(size=shape + (ld,))
+    bv = np.asfortranarray(rng.normal(size=(mb, ms)) + 1j * rng.normal(size=(mb, ms)))
+    want = O.minm(mm, bv, n_in)
+    print(f"case minm shape={shape} mb={mb} ms={ms} n_in={n_in}", flush=True)
+    mt = torch.as_tensor(mm, device="cuda")
+    bt = torch.as_tensor(bv.T.copy(), device="cuda").T    # unit stride along mbsize
+    got = coulomb.calc_minm_vhalf(mt, bt, n_in).cpu().numpy()
+    assert got.shape == want.shape
+    assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
+    g32 = coulomb.calc_minm_vhalf(mt, bt, n_in, complex64=True).cpu().numpy()
+    assert g32.dtype == np.complex64
+    assert np.abs(g32 - want).max() <= 2.0 ** -23 * np.abs(want).max()
+
+
+def test_vhalf_chain_is_phase_free():
+    """M from the GPU basis and M from the oracle's basis give the same
+    M M^H (the rows' Gram matrix, what every contraction over the product
+    basis sees up to a unitary change of basis)."""
+    import torch
+    from pygw_amd import coulomb
+    rng = np.random.default_rng(5)
+    n = 240
+    lam = _spectrum(n, "coulomb", rng)
+    A, _ = O.hermitian_with_spectrum(lam, rng)
+    got = coulomb.coul_setev(A, 1e-7)
+    _, _, _, bref, _ = O.coul_setev(A, 1e-7)
+    mm = rng.normal(size=(200, n)) + 1j * rng.normal(size=(200, n))
+    print("case vhalf chain", flush=True)
+    Mg = coulomb.calc_minm_vhalf(torch.as_tensor(mm, device="cuda"), got.barcvm).cpu().numpy()
+    Mr = O.minm(mm, bref)
+    G, R = Mg @ Mg.conj().T, Mr @ Mr.conj().T
+    assert np.abs(G - R).max() < 1e-11 * np.abs(R).max()
