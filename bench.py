@@ -880,7 +880,7 @@ def run_coulomb(args):
            "jacobi_sweeps": cb.sweeps, "max_eigenvalue_error_rel": ev_err,
            "roofline": {"bound": "mfma", "kernel": "pygw_minm_vhalf", "achieved": f_m / t_m32 / 1e12,
                         "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": f_m / t_m32 / PEAK_FP64,
-                        "traffic": None, "flops_per_launch": f_m,
+                        "traffic": load_traffic("coulomb", "pygw_minm_vhalf<float2>"), "flops_per_launch": f_m,
                         "note": "8 FLOP per (row, basis function, kept vector); complex64 output"}}
     if not args.no_cpu:
         rec["cpu_baseline"] = coulomb_cpu_baseline(vmat, mm[:2 * npair].cpu().numpy(), evtol, nk // 2, t_ev)

@@ -294,7 +294,7 @@ int pygw_mwm_head(const void* M_selfe, int n_k, int nb1, int nb2, int pb, int nc
  *   Real_sqrt_Vcoul = False (pygw.py:18, 2505-2545).
  *   vmat: complex128 C order [mbsize][mbsize], Hermitian; only its lower
  *   triangle is read (like eigh).  Eigenvalues and eigenvectors come from a
- *   parallel cyclic Jacobi method on the device.
+ *   block cyclic Jacobi method on the device.
  *   - ev [mbsize]: ascending (the eigh order).
  *   - vecs: optional, NULL or Fortran [mbsize, mbsize] eigenvectors in that
  *     order (PyGW's self.Vmat).
@@ -311,7 +311,7 @@ int pygw_mwm_head(const void* M_selfe, int n_k, int nb1, int nb2, int pb, int nc
  *   LAPACK's; the products downstream do not depend on them.
  *   The call synchronises the stream once per sweep.
  *   - *n_sweeps (optional): the sweeps run.
- *   - Returns PYGW_E_NOCONV after 80 sweeps without convergence.
+ *   - Returns PYGW_E_NOCONV after 60 sweeps without convergence.
  *   workspace: NULL (allocated) or pygw_coul_setev_workspace bytes.
  *   mbsize <= 16384.
  * pygw_minm_vhalf replaces minm = la.matmul(mmat, conj(barcvm))

@@ -10,13 +10,18 @@
 //   calc_minm (pygw.py:2666, 2684): minm = mmat conj(barcvm), the last step
 //   of the matrix elements M the polarization contracts.
 //
-// The eigensolver is a parallel cyclic Jacobi method for complex Hermitian
-// matrices: one launch per round of the round-robin ordering (n/2 disjoint
-// index pairs), each thread owning one 2 x 2 block (u, v) of the rotated
-// matrix A' = J_u^H A J_v (out of place, ping-pong buffers) and of the
-// eigenvectors V' = V J_v (in place).  A rotation is applied when its
-// off-diagonal element is above tol sqrt(|a_pp a_qq|) and above a floor
-// of 1e-16 ||A||_F; the sweeps stop when no pair is above the threshold.
+// The eigensolver is a block cyclic Jacobi method for complex Hermitian
+// matrices.  The (padded) index range is cut into blocks of 16; a round of
+// the round-robin ordering pairs the blocks, and each 64 x 64 pivot
+// submatrix gets one inner cyclic Jacobi sweep in LDS, accumulating its
+// unitary U (one workgroup per pair).  One launch then applies all of a
+// round's U's: A' = U_u^H A U_v tile by tile (two FP64 MFMA products per
+// 32 x 32 tile) and V' = V U_v, both in place; tiles whose U's are all the
+// identity are skipped.
+// A scalar rotation is applied when its off-diagonal element is above tol
+// sqrt(|a_pp a_qq|) and above a floor of 4e-15 ||A||_F (the rounding level
+// the tile products leave); the sweeps stop when no pair is above that.  The
+// matrix is first scaled by a power of two to ||A||_F in [0.5, 1).
 // The GEMM minm = mmat conj(barcvm) runs on v_mfma_f64_16x16x4f64 with four
 // real products per complex product.
 #include "common.h"
@@ -24,10 +29,14 @@
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 namespace cl {
-constexpr int TP = 16;          // pairs per tile side in a Jacobi round
-constexpr double kTol = 1e-14;  // relative rotation threshold
-constexpr double kFloor = 1e-16;  // absolute threshold, times ||A||_F
-constexpr int kMaxSweeps = 80;
+constexpr int BS = 16;            // indices per block
+constexpr int PB2 = 2 * BS;       // pivot submatrix side
+constexpr int NS = BS * BS;       // threads of the solve (one per 2 x 2 block)
+constexpr int TW = PB2 / 2;       // update tile side per wave (2 x 2 waves)
+constexpr int NF = TW / 16;       // 16 x 16 MFMA tiles per wave side
+constexpr double kTol = 1e-14;    // relative rotation threshold
+constexpr double kFloor = 4e-15;  // absolute threshold, times ||A||_F
+constexpr int kMaxSweeps = 60;
 
 // pair k of round r of the round-robin ordering of n (even) indices
 __device__ __forceinline__ void pair_of(int k, int r, int n, int& p, int& q) {
@@ -53,15 +62,36 @@ struct Rot {
   int p, q;
 };
 
-// J = [[cs, sn], [-sn conj(e), cs conj(e)]] makes J^H [[a, c], [conj c, b]] J diagonal
-__device__ __forceinline__ Rot make_rot(const double2* __restrict__ A, int n, int p, int q, double thr) {
+// 1/sqrt(x) and 1/x for normal x > 0: the hardware approximations and two
+// Newton steps each (quadratic convergence to full double precision)
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+
+// whether the off-diagonal element c of (a, b) is above the thresholds
+// (compared as squares: the matrix is scaled to ||A||_F ~ 1)
+__device__ __forceinline__ bool above(double a, double b, double ac2, double thr) {
+  return ac2 > thr * thr && ac2 > kTol * kTol * fabs(a * b);
+}
+
+// J = [[cs, sn], [-sn conj(e), cs conj(e)]] makes J^H [[a, c], [conj c, b]] J
+// diagonal; ac = 0 marks the identity (c below the thresholds)
+__device__ __forceinline__ Rot make_rot(double a, double b, double2 c, int p, int q, double thr) {
   Rot R;
   R.p = p;
   R.q = q;
-  const double a = A[(size_t)p * n + p].x, b = A[(size_t)q * n + q].x;
-  const double2 c = A[(size_t)p * n + q];
-  const double ac = hypot(c.x, c.y);
-  if (ac <= thr || ac <= kTol * sqrt(fabs(a * b))) {
+  const double ac2 = c.x * c.x + c.y * c.y;
+  if (!above(a, b, ac2, thr)) {
     R.cs = 1.0;
     R.sn = 0.0;
     R.er = 1.0;
@@ -70,14 +100,16 @@ __device__ __forceinline__ Rot make_rot(const double2* __restrict__ A, int n, in
     R.ac = 0.0;
     return R;
   }
-  const double th = (b - a) / (2.0 * ac);
-  const double t = th == 0.0 ? 1.0 : copysign(1.0, th) / (fabs(th) + sqrt(th * th + 1.0));
-  R.cs = 1.0 / sqrt(1.0 + t * t);
+  const double ri = rsqrt_nr(ac2);  // 1/|c|
+  const double th = 0.5 * (b - a) * ri, at = fabs(th);
+  double t = th == 0.0 ? 1.0 : (at > 1e150 ? 0.5 / at : rcp_nr(at + sqrt(fma(at, at, 1.0))));
+  t = copysign(t, th == 0.0 ? 1.0 : th);
+  R.cs = rsqrt_nr(fma(t, t, 1.0));
   R.sn = t * R.cs;
-  R.er = c.x / ac;
-  R.ei = c.y / ac;
+  R.er = c.x * ri;
+  R.ei = c.y * ri;
   R.t = t;
-  R.ac = ac;
+  R.ac = ac2 * ri;
   return R;
 }
 
@@ -120,72 +152,228 @@ __global__ void __launch_bounds__(256) pygw_jac_prep(const double2* __restrict__
   if (threadIdx.x == 0) rn[i] = red[0];
 }
 
-// thr[0] = floor * ||W||_F (row sums added in order)
+// s = 2^-e with ||W||_F = f 2^e, f in [0.5, 1) (row sums added in order):
+// thr[0] = floor * ||W||_F s (the threshold of the scaled matrix),
+// thr[1] = s, thr[2] = 1/s (0 matrix: s = 1)
 __global__ void pygw_jac_norm(const double* __restrict__ rn, int n, double* __restrict__ thr) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double s = 0.0;
   for (int i = 0; i < n; i++) s += rn[i];
-  thr[0] = cl::kFloor * sqrt(s);
+  const double f = sqrt(s);
+  int e = 0;
+  if (f > 0.0) frexp(f, &e);
+  thr[0] = cl::kFloor * ldexp(f, -e);
+  thr[1] = ldexp(1.0, -e);
+  thr[2] = ldexp(1.0, e);
 }
 
-// one round r: B = J^H A J for the n/2 pairs of the round, V = V J
-__global__ void __launch_bounds__(256) pygw_jac_round(const double2* __restrict__ A, double2* __restrict__ B,
-                                                     double2* __restrict__ V, int n, int r,
+// W *= thr[1] (exact: a power of two)
+__global__ void __launch_bounds__(256) pygw_jac_scale(double2* __restrict__ W, long nn,
                                                      const double* __restrict__ thr) {
+  const double s = thr[1];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nn; i += (long)gridDim.x * blockDim.x) {
+    const double2 x = W[i];
+    W[i] = double2{x.x * s, x.y * s};
+  }
+}
+
+// one inner cyclic sweep on the 2BS x 2BS pivot submatrix of pair blockIdx.x
+// of round r (blocks I < J: local index l < BS is I*BS + l, else J*BS + l -
+// BS); U (the product of the sweep's rotations) to Ubuf[blockIdx.x],
+// live[blockIdx.x] = whether any rotation was applied (else U = 1).  One
+// thread per 2 x 2 block of the submatrix; blocks whose row and column
+// rotations are both the identity are left as they are.
+__global__ void __launch_bounds__(cl::NS) pygw_bjac_solve(const double2* __restrict__ A, int n, int r,
+                                                       const double* __restrict__ thr, double2* __restrict__ Ubuf,
+                                                       int* __restrict__ live) {
   using namespace cl;
-  __shared__ Rot ru[TP], rv[TP];
-  const int np = n / 2;
-  const int u0 = blockIdx.y * TP, v0 = blockIdx.x * TP;
+  __shared__ double2 S[PB2][PB2 + 1];
+  __shared__ double2 U[PB2][PB2 + 1];
+  __shared__ Rot rot[BS];
+  __shared__ int any;
+  const int tid = threadIdx.x;
+  int I, J;
+  pair_of(blockIdx.x, r, n / BS, I, J);
+  auto G = [&](int l) { return l < BS ? I * BS + l : J * BS + (l - BS); };
+  for (int e = tid; e < PB2 * PB2; e += NS) {
+    const int g = e / PB2, d = e % PB2;
+    S[g][d] = A[(size_t)G(g) * n + G(d)];
+    U[g][d] = double2{g == d ? 1.0 : 0.0, 0.0};
+  }
+  if (tid == 0) any = 0;
   const double th = thr[0];
-  if (threadIdx.x < 2 * TP) {
-    const int k = (threadIdx.x < TP ? u0 : v0) + (threadIdx.x % TP);
-    if (k < np) {
+  __syncthreads();
+  for (int ir = 0; ir < PB2 - 1; ir++) {
+    if (tid < BS) {
       int p, q;
-      pair_of(k, r, n, p, q);
-      const Rot R = make_rot(A, n, p, q, th);
-      if (threadIdx.x < TP)
-        ru[threadIdx.x] = R;
-      else
-        rv[threadIdx.x - TP] = R;
+      pair_of(tid, ir, PB2, p, q);
+      rot[tid] = make_rot(S[p][p].x, S[q][q].x, S[p][q], p, q, th);
+      if (rot[tid].ac != 0.0) any = 1;  // ac = 0 marks the identity
     }
+    __syncthreads();
+    {
+      const int u = tid / BS, v = tid % BS;
+      const Rot Ru = rot[u], Rv = rot[v];
+      if (Ru.ac == 0.0 && Rv.ac == 0.0) goto next;
+      const double2 eb{Rv.er, -Rv.ei};
+      const double2 j10 = cscale(eb, -Rv.sn), j11 = cscale(eb, Rv.cs);
+      {
+        const double2 x0 = U[Ru.p][Rv.p], x1 = U[Ru.p][Rv.q], y0 = U[Ru.q][Rv.p], y1 = U[Ru.q][Rv.q];
+        U[Ru.p][Rv.p] = cadd(cscale(x0, Rv.cs), cmul(x1, j10));
+        U[Ru.p][Rv.q] = cadd(cscale(x0, Rv.sn), cmul(x1, j11));
+        U[Ru.q][Rv.p] = cadd(cscale(y0, Rv.cs), cmul(y1, j10));
+        U[Ru.q][Rv.q] = cadd(cscale(y0, Rv.sn), cmul(y1, j11));
+      }
+      if (u == v) {
+        const double a = S[Ru.p][Ru.p].x, b = S[Ru.q][Ru.q].x;
+        S[Ru.p][Ru.p] = double2{a - Ru.t * Ru.ac, 0.0};
+        S[Ru.q][Ru.q] = double2{b + Ru.t * Ru.ac, 0.0};
+        S[Ru.p][Ru.q] = double2{0.0, 0.0};
+        S[Ru.q][Ru.p] = double2{0.0, 0.0};
+        goto next;
+      }
+      const double2 x00 = S[Ru.p][Rv.p], x01 = S[Ru.p][Rv.q], x10 = S[Ru.q][Rv.p], x11 = S[Ru.q][Rv.q];
+      const double2 y00 = cadd(cscale(x00, Rv.cs), cmul(x01, j10)), y01 = cadd(cscale(x00, Rv.sn), cmul(x01, j11));
+      const double2 y10 = cadd(cscale(x10, Rv.cs), cmul(x11, j10)), y11 = cadd(cscale(x10, Rv.sn), cmul(x11, j11));
+      const double2 eu{Ru.er, Ru.ei};
+      const double2 a1 = cscale(eu, -Ru.sn), b1 = cscale(eu, Ru.cs);
+      S[Ru.p][Rv.p] = cadd(cscale(y00, Ru.cs), cmul(a1, y10));
+      S[Ru.p][Rv.q] = cadd(cscale(y01, Ru.cs), cmul(a1, y11));
+      S[Ru.q][Rv.p] = cadd(cscale(y00, Ru.sn), cmul(b1, y10));
+      S[Ru.q][Rv.q] = cadd(cscale(y01, Ru.sn), cmul(b1, y11));
+    }
+  next:
+    __syncthreads();
+  }
+  double2* Uo = Ubuf + (size_t)blockIdx.x * PB2 * PB2;
+  for (int e = tid; e < PB2 * PB2; e += NS) Uo[e] = U[e / PB2][e % PB2];
+  if (tid == 0) live[blockIdx.x] = any;
+}
+
+// apply the round's U's, in place (every element belongs to one tile).
+// Workgroups t < nt^2 (nt = n/2BS pairs): the A tile of pairs (t / nt, t % nt),
+// A = U_u^H A U_v; the rest: the V tile of rows 2BS (t / nt) .. + 2BS - 1 and the
+// columns of pair t % nt, V = V U_v.  Tiles whose U's are all 1 are skipped.
+__global__ void __launch_bounds__(256) pygw_bjac_update(double2* __restrict__ A, double2* __restrict__ V, int n,
+                                                       int r, const double2* __restrict__ Ubuf,
+                                                       const int* __restrict__ live) {
+  using namespace cl;
+  __shared__ double2 Xs[PB2][PB2 + 1];
+  __shared__ double2 Us[PB2][PB2 + 1];
+  const int nt = n / PB2, nb = n / BS;
+  int t = blockIdx.x;
+  const bool isV = t >= nt * nt;
+  if (isV) t -= nt * nt;
+  const int tu = t / nt, tv = t % nt;
+  if (!live[tv] && (isV || !live[tu])) return;
+  int Iu, Ju, Iv, Jv;
+  pair_of(tu, r, nb, Iu, Ju);
+  pair_of(tv, r, nb, Iv, Jv);
+  auto Gu = [&](int l) { return isV ? tu * PB2 + l : (l < BS ? Iu * BS + l : Ju * BS + (l - BS)); };
+  auto Gv = [&](int l) { return l < BS ? Iv * BS + l : Jv * BS + (l - BS); };
+  const double2* X = isV ? V : A;
+  const double2* Uv = Ubuf + (size_t)tv * PB2 * PB2;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1, li = lane & 15, lk = lane >> 4;
+  for (int e = tid; e < PB2 * PB2; e += 256) {
+    const int g = e / PB2, d = e % PB2;
+    Xs[g][d] = X[(size_t)Gu(g) * n + Gv(d)];
+    Us[g][d] = Uv[e];
   }
   __syncthreads();
-  const int tu = threadIdx.x / TP, tv = threadIdx.x % TP;
-  const int u = u0 + tu, v = v0 + tv;
-  if (u >= np || v >= np) return;
-  const Rot U = ru[tu], W = rv[tv];
-  const size_t pp = (size_t)U.p * n + W.p, pq = (size_t)U.p * n + W.q;
-  const size_t qp = (size_t)U.q * n + W.p, qq = (size_t)U.q * n + W.q;
-  // V J_v on rows p_u, q_u
-  {
-    const double2 eb{W.er, -W.ei};                       // conj(e_v)
-    const double2 j10 = cscale(eb, -W.sn), j11 = cscale(eb, W.cs);
-    const double2 x0 = V[pp], x1 = V[pq], y0 = V[qp], y1 = V[qq];
-    V[pp] = cadd(cscale(x0, W.cs), cmul(x1, j10));
-    V[pq] = cadd(cscale(x0, W.sn), cmul(x1, j11));
-    V[qp] = cadd(cscale(y0, W.cs), cmul(y1, j10));
-    V[qq] = cadd(cscale(y0, W.sn), cmul(y1, j11));
+  d4 tR[NF][NF], tI[NF][NF];
+#pragma unroll
+  for (int a = 0; a < NF; a++)
+#pragma unroll
+    for (int b = 0; b < NF; b++) {
+      tR[a][b] = d4{0, 0, 0, 0};
+      tI[a][b] = d4{0, 0, 0, 0};
+    }
+  // T = X U_v: Re = xr ur - xi ui, Im = xr ui + xi ur
+  for (int k0 = 0; k0 < PB2; k0 += 4) {
+    const int kk = k0 + lk;
+    double xr[NF], xi[NF], ur[NF], ui[NF];
+#pragma unroll
+    for (int s = 0; s < NF; s++) {
+      const double2 x = Xs[TW * wr + 16 * s + li][kk];
+      const double2 u = Us[kk][TW * wc + 16 * s + li];
+      xr[s] = x.x;
+      xi[s] = x.y;
+      ur[s] = u.x;
+      ui[s] = u.y;
+    }
+#pragma unroll
+    for (int a = 0; a < NF; a++)
+#pragma unroll
+      for (int b = 0; b < NF; b++) {
+        tR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[a], ur[b], tR[a][b], 0, 0, 0);
+        tR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-xi[a], ui[b], tR[a][b], 0, 0, 0);
+        tI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[a], ui[b], tI[a][b], 0, 0, 0);
+        tI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[a], ur[b], tI[a][b], 0, 0, 0);
+      }
   }
-  if (u == v) {
-    B[pp] = double2{A[pp].x - U.t * U.ac, 0.0};
-    B[qq] = double2{A[qq].x + U.t * U.ac, 0.0};
-    B[pq] = double2{0.0, 0.0};
-    B[qp] = double2{0.0, 0.0};
+  if (isV) {
+#pragma unroll
+    for (int a = 0; a < NF; a++)
+#pragma unroll
+      for (int b = 0; b < NF; b++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int row = TW * wr + 16 * a + 4 * v + lk, col = TW * wc + 16 * b + li;
+          V[(size_t)Gu(row) * n + Gv(col)] = double2{tR[a][b][v], tI[a][b][v]};
+        }
     return;
   }
-  const double2 x00 = A[pp], x01 = A[pq], x10 = A[qp], x11 = A[qq];
-  // Y = X J_v
-  const double2 eb{W.er, -W.ei};
-  const double2 j10 = cscale(eb, -W.sn), j11 = cscale(eb, W.cs);
-  const double2 y00 = cadd(cscale(x00, W.cs), cmul(x01, j10)), y01 = cadd(cscale(x00, W.sn), cmul(x01, j11));
-  const double2 y10 = cadd(cscale(x10, W.cs), cmul(x11, j10)), y11 = cadd(cscale(x10, W.sn), cmul(x11, j11));
-  // J_u^H Y: row 0 = cs Y0 - sn e Y1, row 1 = sn Y0 + cs e Y1
-  const double2 e{U.er, U.ei};
-  const double2 a1 = cscale(e, -U.sn), b1 = cscale(e, U.cs);
-  B[pp] = cadd(cscale(y00, U.cs), cmul(a1, y10));
-  B[pq] = cadd(cscale(y01, U.cs), cmul(a1, y11));
-  B[qp] = cadd(cscale(y00, U.sn), cmul(b1, y10));
-  B[qq] = cadd(cscale(y01, U.sn), cmul(b1, y11));
+  __syncthreads();  // every wave is done with Xs and Us
+  const double2* Uu = Ubuf + (size_t)tu * PB2 * PB2;
+#pragma unroll
+  for (int a = 0; a < NF; a++)
+#pragma unroll
+    for (int b = 0; b < NF; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        Xs[TW * wr + 16 * a + 4 * v + lk][TW * wc + 16 * b + li] = double2{tR[a][b][v], tI[a][b][v]};
+  for (int e = tid; e < PB2 * PB2; e += 256) Us[e / PB2][e % PB2] = Uu[e];
+  __syncthreads();
+  // Y = U_u^H T: Re = ur tr + ui ti, Im = ur ti - ui tr (u = U_u[gamma][alpha])
+#pragma unroll
+  for (int a = 0; a < NF; a++)
+#pragma unroll
+    for (int b = 0; b < NF; b++) {
+      tR[a][b] = d4{0, 0, 0, 0};
+      tI[a][b] = d4{0, 0, 0, 0};
+    }
+  for (int k0 = 0; k0 < PB2; k0 += 4) {
+    const int kk = k0 + lk;
+    double ur[NF], ui[NF], xr[NF], xi[NF];
+#pragma unroll
+    for (int s = 0; s < NF; s++) {
+      const double2 u = Us[kk][TW * wr + 16 * s + li];
+      const double2 x = Xs[kk][TW * wc + 16 * s + li];
+      ur[s] = u.x;
+      ui[s] = u.y;
+      xr[s] = x.x;
+      xi[s] = x.y;
+    }
+#pragma unroll
+    for (int a = 0; a < NF; a++)
+#pragma unroll
+      for (int b = 0; b < NF; b++) {
+        tR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[a], xr[b], tR[a][b], 0, 0, 0);
+        tR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[a], xi[b], tR[a][b], 0, 0, 0);
+        tI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[a], xi[b], tI[a][b], 0, 0, 0);
+        tI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ui[a], xr[b], tI[a][b], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int a = 0; a < NF; a++)
+#pragma unroll
+    for (int b = 0; b < NF; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = TW * wr + 16 * a + 4 * v + lk, col = TW * wc + 16 * b + li;
+        A[(size_t)Gu(row) * n + Gv(col)] = double2{tR[a][b][v], tI[a][b][v]};
+      }
 }
 
 // count of pairs p < q whose rotation would be applied
@@ -196,8 +384,7 @@ __global__ void __launch_bounds__(256) pygw_jac_check(const double2* __restrict_
   unsigned c = 0;
   for (int q = p + 1 + threadIdx.x; q < n; q += blockDim.x) {
     const double2 x = A[(size_t)p * n + q];
-    const double ac = hypot(x.x, x.y);
-    c += ac > th && ac > cl::kTol * sqrt(fabs(a * A[(size_t)q * n + q].x));
+    c += cl::above(a, A[(size_t)q * n + q].x, x.x * x.x + x.y * x.y, th);
   }
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, c);
@@ -205,7 +392,8 @@ __global__ void __launch_bounds__(256) pygw_jac_check(const double2* __restrict_
 
 // ascending order of the n0 eigenvalues (ties by index): ev[pos], perm[pos]
 __global__ void __launch_bounds__(256) pygw_jac_rank(const double2* __restrict__ A, int n0, int n,
-                                                    double* __restrict__ ev, int* __restrict__ perm) {
+                                                    const double* __restrict__ thr, double* __restrict__ ev,
+                                                    int* __restrict__ perm) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n0) return;
   const double x = A[(size_t)j * n + j].x;
@@ -214,7 +402,7 @@ __global__ void __launch_bounds__(256) pygw_jac_rank(const double2* __restrict__
     const double y = A[(size_t)i * n + i].x;
     pos += y < x || (y == x && i < j);
   }
-  ev[pos] = x;
+  ev[pos] = x * thr[2];
   perm[pos] = j;
 }
 
@@ -327,12 +515,13 @@ __global__ void __launch_bounds__(256) pygw_jac_vectors(const double2* __restric
 }
 
 namespace cl {
-inline int npad(int n0) { return n0 + (n0 & 1); }
-// workspace: W[2][n][n], V[n][n] complex; rn[n], thr, ev[n], ovl[n] double;
-// perm[n], colsrc[n], info[2] int, counter
+// the index range padded to whole pivot submatrices (zero rows and columns)
+inline int npad(int n0) { return (n0 + PB2 - 1) / PB2 * PB2; }
+// workspace: W[n][n], V[n][n], U[n/2BS][2BS][2BS] complex; rn[n], thr,
+// ev[n], ovl[n] double; perm[n], colsrc[n], live[n/2BS], info[2] int, counter
 inline size_t ws_bytes(int n0) {
   const size_t n = (size_t)npad(n0);
-  return 3 * n * n * 16 + (3 * n + 8) * 8 + (2 * n + 8) * 4 + 512;
+  return 2 * n * n * 16 + n * PB2 * 16 + (3 * n + 8) * 8 + (3 * n + 8) * 4 + 512;
 }
 }  // namespace cl
 
@@ -347,10 +536,10 @@ int coul_setev_launch(const void* vmat, int n0, double evtol, const void* wi0, i
   char* p = (char*)ws;
   double2* W0 = (double2*)p;
   p += (size_t)n * n * 16;
-  double2* W1 = (double2*)p;
-  p += (size_t)n * n * 16;
   double2* V = (double2*)p;
   p += (size_t)n * n * 16;
+  double2* Ub = (double2*)p;
+  p += (size_t)n * PB2 * 16;
   double* rn = (double*)p;
   p += (size_t)n * 8;
   double* thr = (double*)p;
@@ -363,16 +552,18 @@ int coul_setev_launch(const void* vmat, int n0, double evtol, const void* wi0, i
   p += (size_t)n * 4;
   int* colsrc = (int*)p;
   p += (size_t)n * 4;
+  int* live = (int*)p;
+  p += (size_t)n * 4;
   int* info = (int*)p;
   p += 8 * 4;
   unsigned* cnt = (unsigned*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
 
   hipLaunchKernelGGL(pygw_jac_prep, dim3(n), dim3(256), 0, stream, (const double2*)vmat, n0, n, W0, V, rn);
   hipLaunchKernelGGL(pygw_jac_norm, dim3(1), dim3(64), 0, stream, rn, n, thr);
+  hipLaunchKernelGGL(pygw_jac_scale, dim3(ceil_div((long)n * n, 1024)), dim3(256), 0, stream, W0, (long)n * n, thr);
   HIP_TRY(hipGetLastError());
   double2* src = W0;
-  double2* dst = W1;
-  const int tiles = ceil_div(n / 2, TP);
+  const int nt = n / PB2, nb = n / BS;
   int sw = 0;
   bool done = false;
   unsigned h = 0;
@@ -387,17 +578,15 @@ int coul_setev_launch(const void* vmat, int n0, double evtol, const void* wi0, i
       break;
     }
     if (sw == kMaxSweeps) break;
-    for (int r = 0; r < n - 1; r++) {
-      hipLaunchKernelGGL(pygw_jac_round, dim3(tiles, tiles), dim3(256), 0, stream, src, dst, V, n, r, thr);
-      double2* t = src;
-      src = dst;
-      dst = t;
+    for (int r = 0; r < nb - 1; r++) {
+      hipLaunchKernelGGL(pygw_bjac_solve, dim3(nt), dim3(NS), 0, stream, src, n, r, thr, Ub, live);
+      hipLaunchKernelGGL(pygw_bjac_update, dim3(2 * nt * nt), dim3(256), 0, stream, src, V, n, r, Ub, live);
     }
     HIP_TRY(hipGetLastError());
   }
   if (sweeps) *sweeps = sw;
   if (!done) return PYGW_E_NOCONV;
-  hipLaunchKernelGGL(pygw_jac_rank, dim3(ceil_div(n0, 256)), dim3(256), 0, stream, src, n0, n, ev, perm);
+  hipLaunchKernelGGL(pygw_jac_rank, dim3(ceil_div(n0, 256)), dim3(256), 0, stream, src, n0, n, thr, ev, perm);
   if (wi0)
     hipLaunchKernelGGL(pygw_jac_overlap, dim3(n0), dim3(256), 0, stream, V, n0, n, perm, (const double2*)wi0, ovl);
   hipLaunchKernelGGL(pygw_jac_select, dim3(1), dim3(1024), 0, stream, ev, perm, n0, evtol, wi0 ? ovl : nullptr,
@@ -420,20 +609,29 @@ int coul_setev_launch(const void* vmat, int n0, double evtol, const void* wi0, i
 // with leading dimension ldm, barcvm Fortran with leading dimension ldb,
 // minm row-major [nrows][matsize], complex128 or rounded to complex64.
 // One block: 64 rows x 64 columns, four waves of 32 x 32 (2 x 2 MFMA tiles).
+// Items (row strip, column tile) in row-strip order are dealt to the 8 XCDs
+// in contiguous runs (block b takes item (b & 7) per + (b >> 3)), so the
+// column tiles of a strip run together on one XCD and read the strip from
+// its L2.
 // ---------------------------------------------------------------------------
 namespace vh {
 constexpr int TR = 64, TC = 64, KC = 16;
 }
 
+// (at least 4 waves per SIMD: the complex64 variant otherwise takes 92 VGPRs
+// and runs 3)
 template <typename OT>
-__global__ void __launch_bounds__(256) pygw_minm_vhalf(const double2* __restrict__ A, long nrows, int nin, long ldm,
+__global__ void __launch_bounds__(256, 4) pygw_minm_vhalf(const double2* __restrict__ A, long nrows, int nin, long ldm,
                                                       const double2* __restrict__ Bm, long ldb, int matsize,
-                                                      OT* __restrict__ C) {
+                                                      long per, OT* __restrict__ C) {
   using namespace vh;
   __shared__ double2 As[TR][KC + 1];
   __shared__ double2 Bs[KC][TC + 1];
-  const long r0 = (long)blockIdx.x * TR;
-  const int c0 = blockIdx.y * TC;
+  const int nct = (matsize + TC - 1) / TC;
+  const long item = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (item >= ((nrows + TR - 1) / TR) * nct) return;
+  const long r0 = item / nct * TR;
+  const int c0 = (int)(item % nct) * TC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int li = lane & 15, lk = lane >> 4;
@@ -506,14 +704,15 @@ __global__ void __launch_bounds__(256) pygw_minm_vhalf(const double2* __restrict
 int minm_vhalf_launch(const void* mmat, long nrows, int nin, long ldm, const void* barcvm, long ldb, int matsize,
                       void* minm, int out_c64, hipStream_t stream) {
   using namespace vh;
-  const long gx = (nrows + TR - 1) / TR;
-  if (gx > 0x7fffffffL) return PYGW_E_ARG;
-  dim3 grid((unsigned)gx, ceil_div(matsize, TC));
+  const long items = (nrows + TR - 1) / TR * ((matsize + TC - 1) / TC);
+  const long per = (items + 7) / 8;
+  if (8 * per > 0x7fffffffL) return PYGW_E_ARG;
+  dim3 grid((unsigned)(8 * per));
   if (out_c64)
     hipLaunchKernelGGL(pygw_minm_vhalf<float2>, grid, dim3(256), 0, stream, (const double2*)mmat, nrows, nin, ldm,
-                       (const double2*)barcvm, ldb, matsize, (float2*)minm);
+                       (const double2*)barcvm, ldb, matsize, per, (float2*)minm);
   else
     hipLaunchKernelGGL(pygw_minm_vhalf<double2>, grid, dim3(256), 0, stream, (const double2*)mmat, nrows, nin, ldm,
-                       (const double2*)barcvm, ldb, matsize, (double2*)minm);
+                       (const double2*)barcvm, ldb, matsize, per, (double2*)minm);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }

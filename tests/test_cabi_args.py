@@ -83,8 +83,8 @@ def test_other_workspace_queries():
 def test_coulomb_workspace_and_bounds():
     q = _fn("pygw_coul_setev_workspace")
     s = C.c_size_t(0)
-    # two ping-pong copies of the (even-padded) matrix and the eigenvectors, complex128
-    assert q(999, C.byref(s)) == 0 and s.value >= 3 * 1000 * 1000 * 16
+    # the matrix padded to whole 32 x 32 pivot blocks and the eigenvectors (complex128), the U's
+    assert q(999, C.byref(s)) == 0 and s.value >= 2 * 1024 * 1024 * 16 + 1024 * 32 * 16
     assert q(16385, C.byref(s)) == E_ARG
     mv = _fn("pygw_minm_vhalf")
     p = C.c_void_p(16)     # never dereferenced: the checks come first
