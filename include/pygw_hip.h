@@ -323,6 +323,25 @@ int pygw_mwm_head(const void* M_selfe, int n_k, int nb1, int nb2, int pb, int nc
  *   the FP64 result rounded to complex64 (the operand of the
  *   split-precision chi0).  FP64 MFMA.
  * ------------------------------------------------------------------------ */
+/* pygw_minm_is replaces fvxcn.calc_minm_is (pygw.py:2661,
+ *   for_vxcnn.f90:1030-1134), the interstitial part of mmat:
+ *     mmat[ie1][ie2][col0 + G] = 1/sqrt(vol) sum_{i < nvi, j < nvj}
+ *       Aeigq[mstart+ie2][j] mpwipw[G][K(i, j)] Aeigk[nstart+ie1][i],
+ *   K(i, j) = indgq_inverse[i_g0[G_i - G_j + iumklap]] with G_i =
+ *   gindex[indgk_i[i]], G_j = gindex[indgk_j[j]]; no term where the
+ *   difference leaves the i_g0 box [2n1+1][2n2+1][2n3+1] (C order) or an
+ *   index is -1 or out of range.  All index arrays int32, 0-based, on the
+ *   device; iumklap: 3 ints in host memory.  Aeigk [nbmax_i][ngi] and Aeigq
+ *   [nbmax_j][ngj] complex128 row-major, mpwipw [ngq][ngq_barc] complex128
+ *   (ngq_barc <= 4096).  mmat complex128, rows (ie1*(mend-mstart) + ie2) of
+ *   ld_mmat >= col0 + ngq (col0 = loctmatsize writes into the concatenated
+ *   mmat).  FP64 MFMA.  workspace: NULL or pygw_minm_is_workspace bytes. */
+int pygw_minm_is_workspace(int ngq, int nvi, int nvj, int ndim, size_t* bytes);
+int pygw_minm_is(int nstart, int nend, int mstart, int mend, const void* Aeigk, int nbmax_i, int ngi,
+                 const void* Aeigq, int nbmax_j, int ngj, const int* iumklap, const void* mpwipw, int ngq,
+                 int ngq_barc, int nvi, int nvj, const int* indgk_i, const int* indgk_j, const int* indgq_inverse,
+                 int inv_size, const int* gindex, int ng, const int* i_g0, int n1, int n2, int n3, double vol,
+                 void* mmat, long ld_mmat, int col0, void* workspace, size_t ws_bytes, void* stream);
 int pygw_coul_setev_workspace(int mbsize, size_t* bytes);
 int pygw_coul_setev(const void* vmat, int mbsize, double evtol, const void* wi0, int iop_coul_x, double* ev,
                     double* barcev, void* barcvm, void* vecs, int* matsize, int* immax, int* n_sweeps,

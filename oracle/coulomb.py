@@ -60,3 +60,34 @@ def hermitian_with_spectrum(lam, rng):
     Q = Q * (np.diag(R) / np.abs(np.diag(R)))
     A = (Q * lam) @ Q.conj().T
     return (A + A.conj().T) / 2, Q
+
+
+def minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi, nvj, indgk_i, indgk_j,
+            indgq_inverse, gindex, i_g0, Vol):
+    """calc_minm_IS (for_vxcnn.f90:1030-1134), called at pygw.py:2661: the
+    interstitial part of the matrix elements,
+      mmat[ie1, ie2, iGp] = 1/sqrt(Vol) sum_{ivi, ivj} Aeigq[mstart+ie2, ivj]
+                            mpwipw[iGp, K(ivi, ivj)] Aeigk[nstart+ie1, ivi],
+    K(ivi, ivj) the product-basis plane wave G_ki - G_kj + umklap (through
+    i_g0 and indgq_inverse; no term where there is none).  Index arrays are
+    0-based (-1 = none), i_g0 is [2n1+1, 2n2+1, 2n3+1]."""
+    n = (np.array(i_g0.shape) - 1) // 2
+    gi = gindex[np.asarray(indgk_i)[:nvi]]
+    gj = gindex[np.asarray(indgk_j)[:nvj]]
+    iGv = gi[:, None, :] - gj[None, :, :] + np.asarray(iumklap)[None, None, :]       # [nvi, nvj, 3]
+    inside = np.all(np.abs(iGv) <= n, axis=-1)
+    c = np.where(inside[..., None], iGv + n, 0)
+    idg = np.where(inside, i_g0[c[..., 0], c[..., 1], c[..., 2]], -1)
+    ok = (idg >= 0) & (idg < len(indgq_inverse))
+    K = np.where(ok, np.asarray(indgq_inverse)[np.where(ok, idg, 0)], -1)             # [nvi, nvj]
+    Ak = Aeigk[nstart:nend, :nvi]
+    Aq = Aeigq[mstart:mend, :nvj]
+    ngq = mpwipw.shape[0]
+    out = np.zeros((nend - nstart, mend - mstart, ngq), dtype=complex)
+    valid = (K >= 0).T
+    for iGp in range(ngq):
+        tmat = np.where(valid, mpwipw[iGp][np.where(K >= 0, K, 0).T], 0.0)          # [nvj, nvi]
+        tmat2 = tmat @ Ak.T                                                          # [nvj, ndim]
+        mnn = Aq @ tmat2                                                             # [mdim, ndim]
+        out[:, :, iGp] = mnn.T / np.sqrt(Vol)
+    return out

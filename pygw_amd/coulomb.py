@@ -6,6 +6,8 @@ product-basis matrix elements, on the gfx950 backend.
   (pygw.py:2447-2454) and ``Coul_setev`` (pygw.py:2505-2545, the
   ``Real_sqrt_Vcoul = False`` branch, pygw.py:18): eigenvalues above
   ``evtol`` kept in decreasing order, ``barcvm[:, m] = v_j sqrt|ev_j|``.
+* ``calc_minm_is`` replaces ``fvxcn.calc_minm_is`` (pygw.py:2661,
+  for_vxcnn.f90:1030-1134), the interstitial part of mmat.
 * ``calc_minm_vhalf`` replaces ``minm = la.matmul(mmat, conj(barcvm))``
   (pygw.py:2666) and, with ``n_in = loctmatsize``, ``minc = la.matmul(
   mmat_c, conj(barcvm[:loctmatsize, :]))`` (pygw.py:2684).
@@ -20,6 +22,7 @@ from __future__ import annotations
 import ctypes as C
 from dataclasses import dataclass
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -99,4 +102,40 @@ def calc_minm_vhalf(mmat, barcvm, n_in=None, *, complex64=False, out=None):
         raise ValueError("out must be a contiguous tensor of the result's size and type")
     _lib.call("pygw_minm_vhalf", C.c_void_p(mmat.data_ptr()), nrows, n_in, ld, C.c_void_p(barcvm.data_ptr()),
               ldb, ms, int(complex64), C.c_void_p(out.data_ptr()), _stream_ptr(mmat.device))
+    return out
+
+
+def calc_minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi, nvj, indgk_i, indgk_j,
+                 indgq_inverse, gindex, i_g0, Vol, *, out=None, col0=0, device="cuda"):
+    """fvxcn.calc_minm_is (pygw.py:2661, for_vxcnn.f90:1030-1134): the
+    interstitial part of the matrix elements, mmat_it[ie1, ie2, G] =
+    1/sqrt(Vol) sum_{i, j} Aeigq[mstart+ie2, j] mpwipw[G, K(G_i - G_j +
+    umklap)] Aeigk[nstart+ie1, i], with PyGW's 0-based index arrays (i_g0
+    [2n1+1, 2n2+1, 2n3+1], -1 = none).  Returns complex128 [nend-nstart,
+    mend-mstart, ngq], or writes it into out[..., col0:col0 + ngq] of a
+    contiguous complex128 [nend-nstart, mend-mstart, ld] (the columns of
+    mmat = concatenate((mmat_mt, mmat_it), -1) after loctmatsize)."""
+    device = _require_device(device)
+    i32 = lambda x: _dev(np.asarray(x), torch.int32, device).contiguous()
+    ak = _dev(Aeigk, torch.complex128, device).contiguous()
+    aq = _dev(Aeigq, torch.complex128, device).contiguous()
+    mp = _dev(mpwipw, torch.complex128, device).contiguous()
+    ig0 = np.asarray(i_g0)
+    n1, n2, n3 = [(s - 1) // 2 for s in ig0.shape]
+    gi, gj, inv, gx, g0 = i32(indgk_i), i32(indgk_j), i32(indgq_inverse), i32(gindex), i32(ig0)
+    ndim, mdim, ngq = nend - nstart, mend - mstart, mp.shape[0]
+    if out is None:
+        out = torch.empty((ndim, mdim, ngq), dtype=torch.complex128, device=device)
+        ld = ngq
+    else:
+        if out.dtype != torch.complex128 or not out.is_contiguous() or out.shape[:2] != (ndim, mdim):
+            raise ValueError("out must be a contiguous complex128 [nend-nstart, mend-mstart, ld] tensor")
+        ld = out.shape[2]
+    umk = (C.c_int * 3)(*[int(u) for u in np.asarray(iumklap)])
+    _lib.call("pygw_minm_is", int(nstart), int(nend), int(mstart), int(mend), C.c_void_p(ak.data_ptr()),
+              ak.shape[0], ak.shape[1], C.c_void_p(aq.data_ptr()), aq.shape[0], aq.shape[1], umk,
+              C.c_void_p(mp.data_ptr()), ngq, mp.shape[1], int(nvi), int(nvj), C.c_void_p(gi.data_ptr()),
+              C.c_void_p(gj.data_ptr()), C.c_void_p(inv.data_ptr()), inv.numel(), C.c_void_p(gx.data_ptr()),
+              gx.shape[0], C.c_void_p(g0.data_ptr()), n1, n2, n3, float(Vol), C.c_void_p(out.data_ptr()), ld,
+              int(col0), None, 0, _stream_ptr(device))
     return out

@@ -9,8 +9,16 @@ int coul_setev_launch(const void* vmat, int n0, double evtol, const void* wi0, i
 int minm_vhalf_launch(const void* mmat, long nrows, int nin, long ldm, const void* barcvm, long ldb, int matsize,
                       void* minm, int out_c64, hipStream_t stream);
 
+size_t minm_is_ws_bytes(int ngq, int nvi, int nvj, int ndim);
+int minm_is_launch(int nstart, int nend, int mstart, int mend, const void* Aeigk, int ldk, const void* Aeigq,
+                   int ldq, const int* umk, const void* mpwipw, int ngq, int ngq_barc, int nvi, int nvj,
+                   const int* indgk_i, const int* indgk_j, const int* inv, int inv_size, const int* gindex,
+                   int ng, const int* i_g0, int n1, int n2, int n3, double vol, void* mmat, long ldo, int col0, void* ws,
+                   size_t ws_size, hipStream_t stream);
+
 namespace {
 constexpr int kMaxMb = 16384;
+constexpr int kMaxBarc = 4096;
 }
 
 extern "C" int pygw_coul_setev_workspace(int mbsize, size_t* bytes) {
@@ -47,4 +55,43 @@ extern "C" int pygw_minm_vhalf(const void* mmat, long n_rows, int n_in, long ld_
     return PYGW_E_ARG;
   return minm_vhalf_launch(mmat, n_rows, n_in, ld_mmat, barcvm, ld_barcvm, matsize, minm, complex64_out,
                            (hipStream_t)stream);
+}
+
+extern "C" int pygw_minm_is_workspace(int ngq, int nvi, int nvj, int ndim, size_t* bytes) {
+  if (ngq <= 0 || nvi <= 0 || nvj <= 0 || ndim <= 0 || !bytes) return PYGW_E_ARG;
+  *bytes = minm_is_ws_bytes(ngq, nvi, nvj, ndim);
+  return PYGW_OK;
+}
+
+extern "C" int pygw_minm_is(int nstart, int nend, int mstart, int mend, const void* Aeigk, int nbmax_i, int ngi,
+                            const void* Aeigq, int nbmax_j, int ngj, const int* iumklap, const void* mpwipw, int ngq,
+                            int ngq_barc, int nvi, int nvj, const int* indgk_i, const int* indgk_j,
+                            const int* indgq_inverse, int inv_size, const int* gindex, int ng, const int* i_g0, int n1,
+                            int n2, int n3, double vol, void* mmat, long ld_mmat, int col0, void* workspace,
+                            size_t ws_bytes, void* stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  if (!Aeigk || !Aeigq || !iumklap || !mpwipw || !indgk_i || !indgk_j || !indgq_inverse || !gindex || !i_g0 ||
+      !mmat)
+    return PYGW_E_ARG;
+  if (nstart < 0 || nend <= nstart || nend > nbmax_i || mstart < 0 || mend <= mstart || mend > nbmax_j ||
+      nvi <= 0 || nvi > ngi || nvj <= 0 || nvj > ngj || ngq <= 0 || ngq > 65535 || ngq_barc <= 0 ||
+      ngq_barc > kMaxBarc || inv_size <= 0 || ng <= 0 || n1 < 0 || n2 < 0 || n3 < 0 || !(vol > 0.0) || col0 < 0 ||
+      ld_mmat < (long)col0 + ngq)
+    return PYGW_E_ARG;
+  const int ndim = nend - nstart;
+  void* ws = workspace;
+  bool own = false;
+  if (!ws) {
+    ws_bytes = minm_is_ws_bytes(ngq, nvi, nvj, ndim);
+    HIP_TRY(hipMallocAsync(&ws, ws_bytes, stream));
+    own = true;
+  }
+  int rc = minm_is_launch(nstart, nend, mstart, mend, Aeigk, ngi, Aeigq, ngj, iumklap, mpwipw, ngq, ngq_barc, nvi, nvj,
+                          indgk_i, indgk_j, indgq_inverse, inv_size, gindex, ng, i_g0, n1, n2, n3, vol, mmat, ld_mmat,
+                          col0, ws, ws_bytes, stream);
+  if (own) {
+    hipError_t e = hipFreeAsync(ws, stream);
+    if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
+  }
+  return rc;
 }

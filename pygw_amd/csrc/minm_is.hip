@@ -1,0 +1,215 @@
+// Interstitial part of the product-basis matrix elements on gfx950
+// (calc_minm_IS, for_vxcnn.f90:1030-1134, called at pygw.py:2661):
+//
+//   mmat[ie1][ie2][G] = 1/sqrt(Vol) sum_{i < nvi, j < nvj}
+//       Aeigq[mstart+ie2][j] mpwipw[G][K(i, j)] Aeigk[nstart+ie1][i]
+//
+// with K(i, j) the product-basis plane wave equal to G_i - G_j + umklap
+// (G_i of k, G_j of k - q; no term where there is none).
+//
+//   pygw_is_map    K(i, j) from the integer G vectors (i_g0, indgq_inverse)
+//   pygw_is_gemm1  T[G][j][ie1] = sum_i mpwipw[G][K(i, j)] Aeigk[nstart+ie1][i]
+//                  (the mpwipw row of G staged in LDS and gathered through K)
+//   pygw_is_gemm2  mmat[ie1][ie2][col0 + G] = sum_j Aeigq[mstart+ie2][j] T[G][j][ie1] / sqrt(Vol)
+//
+// Both products run on v_mfma_f64_16x16x4f64 (64 x 64 tiles of four waves,
+// four real products per complex product).
+#include "common.h"
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+namespace is {
+constexpr int TM = 64, TN = 64, KC = 16;
+constexpr int kMaxBarc = 4096;  // mpwipw row staged in LDS (64 KB)
+
+// C[m][n] += sum_k A(m, k) B(k, n) over one 64 x 64 tile: la(m, k) and
+// lb(k, n) give the operands (zero outside), the four waves hold 32 x 32
+// quarters as 2 x 2 MFMA tiles; acc rows 32 wr + 16 a + 4 v + lk, columns
+// 32 wc + 16 b + li
+template <class LA, class LB>
+__device__ __forceinline__ void tile(int K, LA la, LB lb, double2 (&As)[TM][KC + 1], double2 (&Bs)[KC][TN + 1],
+                                     d4 (&aR)[2][2], d4 (&aI)[2][2]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1, li = lane & 15, lk = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += KC) {
+    for (int e = tid; e < TM * KC; e += 256) {
+      const int m = e % TM, c = e / TM;  // consecutive threads: consecutive m
+      As[m][c] = la(m, k0 + c);
+    }
+    for (int e = tid; e < KC * TN; e += 256) {
+      const int c = e % KC, n = e / KC;  // consecutive threads: consecutive k
+      Bs[c][n] = lb(k0 + c, n);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ks++) {
+      const int kk = ks * 4 + lk;
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const double2 x = As[32 * wr + 16 * t + li][kk];
+        const double2 y = Bs[kk][32 * wc + 16 * t + li];
+        ar[t] = x.x;
+        ai[t] = x.y;
+        br[t] = y.x;
+        bi[t] = y.y;
+      }
+      // a b: Re = ar br - ai bi, Im = ar bi + ai br
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          aR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], aR[a][b], 0, 0, 0);
+          aR[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[b], aR[a][b], 0, 0, 0);
+          aI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], aI[a][b], 0, 0, 0);
+          aI[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], aI[a][b], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void zero(d4 (&aR)[2][2], d4 (&aI)[2][2]) {
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      aR[a][b] = d4{0, 0, 0, 0};
+      aI[a][b] = d4{0, 0, 0, 0};
+    }
+}
+}  // namespace is
+
+// kmap[i][j] = K(i, j) (0-based index into mpwipw's columns) or -1
+// (for_vxcnn.f90:1083-1091: G = gindex[indgk_i[i]] - gindex[indgk_j[j]] +
+// umklap; inside the i_g0 box, i_g0 gives its index in gindex, and
+// indgq_inverse its index in the product basis)
+// (indices outside gindex, i_g0 or the product basis count as none)
+__global__ void __launch_bounds__(256) pygw_is_map(const int* __restrict__ gindex, int ng,
+                                                  const int* __restrict__ indgk_i, const int* __restrict__ indgk_j,
+                                                  int nvi, int nvj, const int* __restrict__ i_g0, int n1, int n2,
+                                                  int n3, const int* __restrict__ inv, int inv_size, int ngq_barc,
+                                                  int u0, int u1, int u2, int* __restrict__ kmap) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)nvi * nvj) return;
+  const int i = (int)(e / nvj), j = (int)(e % nvj);
+  const int a = indgk_i[i], b = indgk_j[j];
+  if (a < 0 || a >= ng || b < 0 || b >= ng) {
+    kmap[e] = -1;
+    return;
+  }
+  const int* gi = gindex + 3 * (size_t)a;
+  const int* gj = gindex + 3 * (size_t)b;
+  const int x = gi[0] - gj[0] + u0, y = gi[1] - gj[1] + u1, z = gi[2] - gj[2] + u2;
+  int K = -1;
+  if (abs(x) <= n1 && abs(y) <= n2 && abs(z) <= n3) {
+    const int idg = i_g0[((size_t)(x + n1) * (2 * n2 + 1) + (y + n2)) * (2 * n3 + 1) + (z + n3)];
+    if (idg >= 0 && idg < inv_size) K = inv[idg];
+    if (K >= ngq_barc) K = -1;
+  }
+  kmap[e] = K;
+}
+
+// T[G][j][ie1] = sum_i mpwipw[G][kmap[i][j]] Ak[nstart+ie1][i]; block: one
+// G (blockIdx.y), 64 j x 64 ie1
+__global__ void __launch_bounds__(256) pygw_is_gemm1(const double2* __restrict__ mpwipw, int ngq_barc,
+                                                    const int* __restrict__ kmap, int nvi, int nvj,
+                                                    const double2* __restrict__ Ak, int ldk, int nstart, int ndim,
+                                                    double2* __restrict__ T) {
+  using namespace is;
+  __shared__ double2 row[kMaxBarc];
+  __shared__ double2 As[TM][KC + 1];
+  __shared__ double2 Bs[KC][TN + 1];
+  const int G = blockIdx.y;
+  const int ntn = (ndim + TN - 1) / TN;
+  const int j0 = (blockIdx.x / ntn) * TM, e0 = (blockIdx.x % ntn) * TN;
+  for (int c = threadIdx.x; c < ngq_barc; c += 256) row[c] = mpwipw[(size_t)G * ngq_barc + c];
+  __syncthreads();
+  d4 aR[2][2], aI[2][2];
+  zero(aR, aI);
+  auto la = [&](int m, int k) -> double2 {  // A(j, i)
+    const int j = j0 + m;
+    if (j >= nvj || k >= nvi) return double2{0.0, 0.0};
+    const int K = kmap[(size_t)k * nvj + j];
+    return K >= 0 ? row[K] : double2{0.0, 0.0};
+  };
+  auto lb = [&](int k, int n) -> double2 {  // B(i, ie1)
+    const int ie1 = e0 + n;
+    return (k < nvi && ie1 < ndim) ? Ak[(size_t)(nstart + ie1) * ldk + k] : double2{0.0, 0.0};
+  };
+  tile(nvi, la, lb, As, Bs, aR, aI);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 1, wc = wave & 1, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int j = j0 + 32 * wr + 16 * a + 4 * v + lk, ie1 = e0 + 32 * wc + 16 * b + li;
+        if (j < nvj && ie1 < ndim) T[((size_t)G * nvj + j) * ndim + ie1] = double2{aR[a][b][v], aI[a][b][v]};
+      }
+}
+
+// mmat[(ie1 mdim + ie2) ldo + col0 + G] = s sum_j Aq[mstart+ie2][j] T[G][j][ie1];
+// block: one G (blockIdx.y), 64 ie2 x 64 ie1
+__global__ void __launch_bounds__(256) pygw_is_gemm2(const double2* __restrict__ Aq, int ldq, int mstart, int mdim,
+                                                    const double2* __restrict__ T, int nvj, int ndim, double s,
+                                                    double2* __restrict__ out, long ldo, int col0) {
+  using namespace is;
+  __shared__ double2 As[TM][KC + 1];
+  __shared__ double2 Bs[KC][TN + 1];
+  const int G = blockIdx.y;
+  const int ntn = (ndim + TN - 1) / TN;
+  const int m0 = (blockIdx.x / ntn) * TM, e0 = (blockIdx.x % ntn) * TN;
+  d4 aR[2][2], aI[2][2];
+  zero(aR, aI);
+  const double2* TG = T + (size_t)G * nvj * ndim;
+  auto la = [&](int m, int k) -> double2 {  // A(ie2, j)
+    const int ie2 = m0 + m;
+    return (ie2 < mdim && k < nvj) ? Aq[(size_t)(mstart + ie2) * ldq + k] : double2{0.0, 0.0};
+  };
+  auto lb = [&](int k, int n) -> double2 {  // B(j, ie1)
+    const int ie1 = e0 + n;
+    return (k < nvj && ie1 < ndim) ? TG[(size_t)k * ndim + ie1] : double2{0.0, 0.0};
+  };
+  tile(nvj, la, lb, As, Bs, aR, aI);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 1, wc = wave & 1, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int ie2 = m0 + 32 * wr + 16 * a + 4 * v + lk, ie1 = e0 + 32 * wc + 16 * b + li;
+        if (ie2 < mdim && ie1 < ndim)
+          out[((size_t)ie1 * mdim + ie2) * ldo + col0 + G] = double2{s * aR[a][b][v], s * aI[a][b][v]};
+      }
+}
+
+size_t minm_is_ws_bytes(int ngq, int nvi, int nvj, int ndim) {
+  return (((size_t)nvi * nvj * 4 + 255) & ~(size_t)255) + (size_t)ngq * nvj * ndim * 16;
+}
+
+int minm_is_launch(int nstart, int nend, int mstart, int mend, const void* Aeigk, int ldk, const void* Aeigq,
+                   int ldq, const int* umk, const void* mpwipw, int ngq, int ngq_barc, int nvi, int nvj,
+                   const int* indgk_i, const int* indgk_j, const int* inv, int inv_size, const int* gindex,
+                   int ng, const int* i_g0, int n1, int n2, int n3, double vol, void* mmat, long ldo, int col0, void* ws,
+                   size_t ws_size, hipStream_t stream) {
+  using namespace is;
+  const int ndim = nend - nstart, mdim = mend - mstart;
+  if (ws_size < minm_is_ws_bytes(ngq, nvi, nvj, ndim) || ngq_barc > kMaxBarc) return PYGW_E_ARG;
+  int* kmap = (int*)ws;
+  double2* T = (double2*)((char*)ws + (((size_t)nvi * nvj * 4 + 255) & ~(size_t)255));
+  hipLaunchKernelGGL(pygw_is_map, dim3(ceil_div((long)nvi * nvj, 256)), dim3(256), 0, stream, gindex, ng,
+                     indgk_i, indgk_j, nvi, nvj, i_g0, n1, n2, n3, inv, inv_size, ngq_barc, umk[0], umk[1], umk[2],
+                     kmap);
+  const int ntn = ceil_div(ndim, TN);
+  hipLaunchKernelGGL(pygw_is_gemm1, dim3(ceil_div(nvj, TM) * ntn, ngq), dim3(256), 0, stream,
+                     (const double2*)mpwipw, ngq_barc, kmap, nvi, nvj, (const double2*)Aeigk, ldk, nstart, ndim, T);
+  hipLaunchKernelGGL(pygw_is_gemm2, dim3(ceil_div(mdim, TM) * ntn, ngq), dim3(256), 0, stream,
+                     (const double2*)Aeigq, ldq, mstart, mdim, T, nvj, ndim, 1.0 / sqrt(vol), (double2*)mmat, ldo,
+                     col0);
+  return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+}

@@ -401,3 +401,50 @@ def coulomb_matrix(mbsize=1000, n_null=100, seed=20261211):
     Q = Q * (np.diag(R) / np.abs(np.diag(R)))
     A = (Q * lam) @ Q.conj().T
     return (A + A.conj().T) / 2, np.sort(lam)
+
+
+@dataclass
+class InterstitialSystem:
+    """Inputs of calc_minm_IS (for_vxcnn.f90:1030) with PyGW's conventions:
+    gindex [ng, 3] integer G vectors (shortest first), i_g0 the inverse map
+    [2m+1, 2m+1, 2m+1] (-1 = none), indgk_i / indgk_j the plane waves of k and
+    k - q (indices into gindex, ngi >= nvi entries), indgq the product-basis
+    plane waves and indgq_inverse its inverse, mpwipw [ngq, ngq_barc],
+    Aeigk / Aeigq [nbmax, ngi / ngj] coefficients; all indices 0-based."""
+    gindex: np.ndarray
+    i_g0: np.ndarray
+    indgk_i: np.ndarray
+    indgk_j: np.ndarray
+    indgq: np.ndarray
+    indgq_inverse: np.ndarray
+    mpwipw: np.ndarray
+    Aeigk: np.ndarray
+    Aeigq: np.ndarray
+    iumklap: np.ndarray
+    nvi: int
+    nvj: int
+    vol: float
+
+    def args(self, nstart, nend, mstart, mend):
+        return (nstart, nend, mstart, mend, self.Aeigk, self.Aeigq, self.iumklap, self.mpwipw, self.nvi, self.nvj,
+                self.indgk_i, self.indgk_j, self.indgq_inverse, self.gindex, self.i_g0, self.vol)
+
+
+def interstitial_system(seed=7, gmax=3, nvi=37, nvj=41, ngq=23, ngq_barc=29, nb=9, vol=270.0,
+                        iumklap=(1, 0, -1)) -> InterstitialSystem:
+    rng = np.random.default_rng(seed)
+    r = np.arange(-gmax, gmax + 1)
+    g = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    g = g[np.lexsort((g[:, 2], g[:, 1], g[:, 0], (g ** 2).sum(1)))]      # shortest first
+    i_g0 = -np.ones((2 * gmax + 1,) * 3, dtype=np.int64)
+    i_g0[g[:, 0] + gmax, g[:, 1] + gmax, g[:, 2] + gmax] = np.arange(len(g))
+    short = np.arange(min(len(g), 3 * max(nvi, nvj)))
+    indgk_i = np.concatenate([rng.permutation(short)[:nvi], rng.integers(0, len(g), 3)])
+    indgk_j = np.concatenate([rng.permutation(short)[:nvj], rng.integers(0, len(g), 2)])
+    indgq = rng.permutation(np.arange(2 * ngq_barc))[:ngq_barc]
+    inv = -np.ones(indgq.max() + 1, dtype=np.int64)
+    inv[indgq] = np.arange(ngq_barc)
+    cplx = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    return InterstitialSystem(gindex=g.astype(np.int64), i_g0=i_g0, indgk_i=indgk_i, indgk_j=indgk_j, indgq=indgq,
+                              indgq_inverse=inv, mpwipw=cplx(ngq, ngq_barc), Aeigk=cplx(nb, len(indgk_i)),
+                              Aeigq=cplx(nb, len(indgk_j)), iumklap=np.array(iumklap), nvi=nvi, nvj=nvj, vol=vol)

@@ -119,3 +119,24 @@ def test_vhalf_chain_is_phase_free():
     Mr = O.minm(mm, bref)
     G, R = Mg @ Mg.conj().T, Mr @ Mr.conj().T
     assert np.abs(G - R).max() < 1e-11 * np.abs(R).max()
+
+
+@pytest.mark.parametrize("kw,rng_", [(dict(seed=7, gmax=3, nvi=37, nvj=41, ngq=23, ngq_barc=29, nb=9), (1, 6, 2, 9)),
+                                     (dict(seed=8, gmax=5, nvi=300, nvj=321, ngq=117, ngq_barc=150, nb=70),
+                                      (3, 23, 10, 70))])
+def test_minm_is_vs_oracle(kw, rng_):
+    """The interstitial matrix elements against the restatement of
+    calc_minm_IS, alone and written after loctmatsize MT columns of the
+    concatenated mmat (the columns before them left alone)."""
+    import torch
+    from pygw_amd import coulomb, synth
+    s = synth.interstitial_system(**kw)
+    want = O.minm_is(*s.args(*rng_))
+    print(f"case minm_is {kw} bands {rng_}", flush=True)
+    got = coulomb.calc_minm_is(*s.args(*rng_)).cpu().numpy()
+    assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
+    loct = 13
+    full = torch.full(want.shape[:2] + (loct + want.shape[2],), 7.0 + 0j, dtype=torch.complex128, device="cuda")
+    coulomb.calc_minm_is(*s.args(*rng_), out=full, col0=loct)
+    f = full.cpu().numpy()
+    assert np.all(f[..., :loct] == 7.0) and np.abs(f[..., loct:] - want).max() < 1e-13 * np.abs(want).max()

@@ -66,3 +66,36 @@ def test_minm_is_v_half():
     pad = np.concatenate([mc, np.zeros((7, 14))], axis=1)
     longer = np.concatenate([mc, rng.normal(size=(7, 4))], axis=1)    # rows of ld 14, n_in = 10
     assert np.abs(O.minm(longer, barcvm, 10) - O.minm(pad, barcvm)).max() < 1e-13
+
+
+def _is_system(seed=7):
+    from pygw_amd import synth
+    return synth.interstitial_system(seed=seed, gmax=3, nvi=37, nvj=41, ngq=23, ngq_barc=29, nb=9)
+
+
+def test_minm_is_is_a_plane_wave_convolution():
+    """The interstitial matrix elements against an independent statement of
+    what they are: the sum over every pair of plane waves (G_i of k, G_j of
+    k - q) whose G_i - G_j + umklap is a basis plane wave K, found by a
+    dictionary of G vectors rather than through i_g0 / indgq_inverse, of
+    mpwipw[G, K] Aeigk[ie1, i] Aeigq[ie2, j] / sqrt(Vol) (einsum)."""
+    s = _is_system()
+    got = O.minm_is(*s.args(1, 6, 2, 9))
+    where = {tuple(g): b for b, g in enumerate(s.gindex[s.indgq])}
+    nvi, nvj = s.nvi, s.nvj
+    Kmap = np.full((nvi, nvj), -1)
+    for i in range(nvi):
+        for j in range(nvj):
+            g = tuple(s.gindex[s.indgk_i[i]] - s.gindex[s.indgk_j[j]] + s.iumklap)
+            Kmap[i, j] = where.get(g, -1)
+    assert (Kmap >= 0).sum() > 100 and (Kmap < 0).sum() > 100      # both cases occur
+    T = np.where((Kmap >= 0)[None], s.mpwipw[:, np.where(Kmap >= 0, Kmap, 0)], 0)   # [ngq, nvi, nvj]
+    want = np.einsum("gij,ai,bj->abg", T, s.Aeigk[1:6, :nvi], s.Aeigq[2:9, :nvj]) / np.sqrt(s.vol)
+    assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
+    # with mpwipw = 1 on a matching plane wave, the entry is the correlation of the coefficient sets
+    s.mpwipw = np.eye(s.mpwipw.shape[0], s.mpwipw.shape[1], dtype=complex)
+    got = O.minm_is(*s.args(0, 1, 0, 1))
+    g0 = 0
+    pairs = [(i, j) for i in range(nvi) for j in range(nvj) if Kmap[i, j] == g0]
+    ref = sum(s.Aeigk[0, i] * s.Aeigq[0, j] for i, j in pairs) / np.sqrt(s.vol)
+    assert abs(got[0, 0, g0] - ref) < 1e-13
