@@ -14,7 +14,7 @@ contractions; value = split-precision seconds per q when its chi0 is within
 1e-6 (relative Frobenius, every frequency) of FP64 on the same
 complex64-rounded M, else the FP64 seconds per q.
 
-    python bench.py [--steps K] [--warmup W] [--config c5|c2|c3|c4|gw0|gamma|coulomb] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--config c5|c2|c3|c4|gw0|gamma|coulomb|minm_is] [--no-cpu]
 
 --config c2 / c3: chi0 + eps^-1 only (FP64 and split precision) at
 8x8x8 k, 16+48 bands, PB = 600 / 10x10x10 k, 32+200 bands, PB = 1200.
@@ -22,6 +22,7 @@ complex64-rounded M, else the FP64 seconds per q.
 windows) through the one-launch batched chi0, then eps^-1 per q.
 --config gw0 / gamma: the GW0 iteration on stored mwm / the q = 0 extras.
 --config coulomb: the v^(1/2) basis (Coul_setev) and minm = mmat conj(barcvm).
+--config minm_is: the interstitial part of mmat (calc_minm_IS) over all k of a q.
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs one GPU.
 """
 from __future__ import annotations
@@ -903,6 +904,66 @@ def coulomb_cpu_baseline(vmat, mm_sample, evtol, scale, t_gpu_ev):
             "stages_s": {"coul_setev": t1 - t0, "minm_extrapolated": (t2 - t1) * scale}}
 
 
+def run_minm_is(args):
+    """The interstitial matrix elements of one q (calc_minm_IS per k,
+    pygw.py:2661): 512 k, LAPW bases of 600 plane waves at k and k - q,
+    400 product-basis plane waves (mpwipw against 700), the 20 occupied x
+    120 empty band window of the polarization.  The same synthetic inputs
+    serve every k (the work per k is the same)."""
+    import torch
+
+    from pygw_amd import coulomb, synth
+    nk, rng_ = 512, (0, 20, 20, 140)
+    s = synth.interstitial_system(seed=20261212, gmax=8, nvi=600, nvj=600, ngq=400, ngq_barc=700, nb=140)
+    d = lambda x, dt: torch.as_tensor(np.asarray(x), dtype=dt, device="cuda")
+    dev = synth.InterstitialSystem(gindex=d(s.gindex, torch.int32), i_g0=d(s.i_g0, torch.int32),
+                                   indgk_i=d(s.indgk_i, torch.int32), indgk_j=d(s.indgk_j, torch.int32),
+                                   indgq=s.indgq, indgq_inverse=d(s.indgq_inverse, torch.int32),
+                                   mpwipw=d(s.mpwipw, torch.complex128), Aeigk=d(s.Aeigk, torch.complex128),
+                                   Aeigq=d(s.Aeigq, torch.complex128), iumklap=s.iumklap, nvi=s.nvi, nvj=s.nvj,
+                                   vol=s.vol)
+    ndim, mdim, ngq = rng_[1] - rng_[0], rng_[3] - rng_[2], s.mpwipw.shape[0]
+    out = torch.empty((ndim, mdim, ngq), dtype=torch.complex128, device="cuda")
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def step():
+        e0.record(st)
+        for _ in range(nk):
+            coulomb.calc_minm_is(*dev.args(*rng_), out=out)
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    t = sum(step() for _ in range(args.steps)) / args.steps
+    f_k = 8.0 * ngq * ndim * s.nvj * (s.nvi + mdim)
+    print(f"[bench] minm_is {t:.3f} s/q ({f_k * nk / t / 1e12:.1f} TF/s)", file=sys.stderr, flush=True)
+    rec = {"metric": "interstitial matrix elements (calc_minm_IS) seconds per q (all k)", "value": t, "unit": "s/q",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"minm_is: {nk} k, nvi = nvj = {s.nvi} plane waves, ngq = {ngq} (mpwipw against "
+                                  f"{s.mpwipw.shape[1]}), bands {ndim} x {mdim}"},
+           "roofline": {"bound": "mfma", "kernel": "pygw_is_gemm1 + pygw_is_gemm2", "achieved": f_k * nk / t / 1e12,
+                        "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": f_k * nk / t / PEAK_FP64,
+                        "traffic": (load_traffic("minm_is", "pygw_is_gemm1<4>") or 0.0) +
+                                   (load_traffic("minm_is", "pygw_is_gemm2<4>") or 0.0) or None, "flops_per_k": f_k,
+                        "note": "8 FLOP per complex multiply-add of the two products; time of whole calls"}}
+    if not args.no_cpu:
+        from oracle import coulomb as O
+        t0 = time.perf_counter()
+        want = O.minm_is(*s.args(*rng_))
+        t1 = time.perf_counter()
+        err = float(np.abs(out.cpu().numpy() - want).max() / np.abs(want).max())
+        rec["max_rel_error_vs_oracle"] = err
+        rec["cpu_baseline"] = {"value": (t1 - t0) * nk, "unit": "s/q", "cores": int(os.environ.get(
+            "OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+            "sample": f"the numpy restatement (two matmuls per product-basis plane wave, as the Fortran's zgemm pair) "
+                      f"on 1 of {nk} k, x{nk}"}
+    print(json.dumps(rec))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -915,7 +976,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.steps is None:
-        args.steps = {"c5": 2, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2, "coulomb": 2}.get(args.config, 1)
+        args.steps = {"c5": 2, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2, "coulomb": 2, "minm_is": 2}.get(args.config, 1)
     if args.copy_calibration:
         # one copy kernel that reads and writes exactly 1 GiB: the ratio of its
         # FETCH_SIZE / WRITE_SIZE counters to 1 GiB scales the kernels' counters
@@ -939,6 +1000,9 @@ def main():
         return
     if args.config == "coulomb":
         run_coulomb(args)
+        return
+    if args.config == "minm_is":
+        run_minm_is(args)
         return
     mixed = not args.no_mixed
     wl = Workload(args.config, args.nq, mixed, args.fp64_kfrac)

@@ -116,13 +116,12 @@ def calc_minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi,
     contiguous complex128 [nend-nstart, mend-mstart, ld] (the columns of
     mmat = concatenate((mmat_mt, mmat_it), -1) after loctmatsize)."""
     device = _require_device(device)
-    i32 = lambda x: _dev(np.asarray(x), torch.int32, device).contiguous()
+    i32 = lambda x: _dev(x, torch.int32, device).contiguous()
     ak = _dev(Aeigk, torch.complex128, device).contiguous()
     aq = _dev(Aeigq, torch.complex128, device).contiguous()
     mp = _dev(mpwipw, torch.complex128, device).contiguous()
-    ig0 = np.asarray(i_g0)
-    n1, n2, n3 = [(s - 1) // 2 for s in ig0.shape]
-    gi, gj, inv, gx, g0 = i32(indgk_i), i32(indgk_j), i32(indgq_inverse), i32(gindex), i32(ig0)
+    n1, n2, n3 = [(s - 1) // 2 for s in i_g0.shape]
+    gi, gj, inv, gx, g0 = i32(indgk_i), i32(indgk_j), i32(indgq_inverse), i32(gindex), i32(i_g0)
     ndim, mdim, ngq = nend - nstart, mend - mstart, mp.shape[0]
     if out is None:
         out = torch.empty((ndim, mdim, ngq), dtype=torch.complex128, device=device)

@@ -12,25 +12,33 @@
 //                  (the mpwipw row of G staged in LDS and gathered through K)
 //   pygw_is_gemm2  mmat[ie1][ie2][col0 + G] = sum_j Aeigq[mstart+ie2][j] T[G][j][ie1] / sqrt(Vol)
 //
-// Both products run on v_mfma_f64_16x16x4f64 (64 x 64 tiles of four waves,
-// four real products per complex product).
+// Both products run on v_mfma_f64_16x16x4f64 (tiles of four waves of 32 x 32,
+// 64 x 64 or 128 x 32 for a band window of at most 32; four real products
+// per complex product).
 #include "common.h"
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 namespace is {
-constexpr int TM = 64, TN = 64, KC = 16;
+constexpr int KC = 16;
 constexpr int kMaxBarc = 4096;  // mpwipw row staged in LDS (64 KB)
 
-// C[m][n] += sum_k A(m, k) B(k, n) over one 64 x 64 tile: la(m, k) and
-// lb(k, n) give the operands (zero outside), the four waves hold 32 x 32
-// quarters as 2 x 2 MFMA tiles; acc rows 32 wr + 16 a + 4 v + lk, columns
-// 32 wc + 16 b + li
-template <class LA, class LB>
-__device__ __forceinline__ void tile(int K, LA la, LB lb, double2 (&As)[TM][KC + 1], double2 (&Bs)[KC][TN + 1],
-                                     d4 (&aR)[2][2], d4 (&aI)[2][2]) {
+// Tile of WR x (4 / WR) waves, each 32 x 32 (2 x 2 MFMA tiles): 64 x 64 (WR
+// = 2) or 128 x 32 (WR = 4, for a narrow band window)
+template <int WR>
+struct L {
+  static constexpr int WC = 4 / WR, TM = 32 * WR, TN = 32 * WC;
+};
+
+// C[m][n] += sum_k A(m, k) B(k, n) over one tile: la(m, k) and lb(k, n)
+// give the operands (zero outside); acc rows 32 wr + 16 a + 4 v + lk,
+// columns 32 wc + 16 b + li
+template <int WR, class LA, class LB>
+__device__ __forceinline__ void tile(int K, LA la, LB lb, double2 (&As)[L<WR>::TM][KC + 1],
+                                     double2 (&Bs)[KC][L<WR>::TN + 1], d4 (&aR)[2][2], d4 (&aI)[2][2]) {
+  constexpr int TM = L<WR>::TM, TN = L<WR>::TN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1, li = lane & 15, lk = lane >> 4;
+  const int wr = wave / L<WR>::WC, wc = wave % L<WR>::WC, li = lane & 15, lk = lane >> 4;
   for (int k0 = 0; k0 < K; k0 += KC) {
     for (int e = tid; e < TM * KC; e += 256) {
       const int m = e % TM, c = e / TM;  // consecutive threads: consecutive m
@@ -111,13 +119,15 @@ __global__ void __launch_bounds__(256) pygw_is_map(const int* __restrict__ ginde
 }
 
 // T[G][j][ie1] = sum_i mpwipw[G][kmap[i][j]] Ak[nstart+ie1][i]; block: one
-// G (blockIdx.y), 64 j x 64 ie1
+// G (blockIdx.y), TM j x TN ie1
+template <int WR>
 __global__ void __launch_bounds__(256) pygw_is_gemm1(const double2* __restrict__ mpwipw, int ngq_barc,
                                                     const int* __restrict__ kmap, int nvi, int nvj,
                                                     const double2* __restrict__ Ak, int ldk, int nstart, int ndim,
                                                     double2* __restrict__ T) {
   using namespace is;
-  __shared__ double2 row[kMaxBarc];
+  constexpr int TM = L<WR>::TM, TN = L<WR>::TN;
+  extern __shared__ double2 row[];  // ngq_barc entries (dynamic: small rows leave room for more blocks)
   __shared__ double2 As[TM][KC + 1];
   __shared__ double2 Bs[KC][TN + 1];
   const int G = blockIdx.y;
@@ -137,9 +147,9 @@ __global__ void __launch_bounds__(256) pygw_is_gemm1(const double2* __restrict__
     const int ie1 = e0 + n;
     return (k < nvi && ie1 < ndim) ? Ak[(size_t)(nstart + ie1) * ldk + k] : double2{0.0, 0.0};
   };
-  tile(nvi, la, lb, As, Bs, aR, aI);
+  tile<WR>(nvi, la, lb, As, Bs, aR, aI);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wr = wave >> 1, wc = wave & 1, li = lane & 15, lk = lane >> 4;
+  const int wr = wave / L<WR>::WC, wc = wave % L<WR>::WC, li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -152,11 +162,13 @@ __global__ void __launch_bounds__(256) pygw_is_gemm1(const double2* __restrict__
 }
 
 // mmat[(ie1 mdim + ie2) ldo + col0 + G] = s sum_j Aq[mstart+ie2][j] T[G][j][ie1];
-// block: one G (blockIdx.y), 64 ie2 x 64 ie1
+// block: one G (blockIdx.y), TM ie2 x TN ie1
+template <int WR>
 __global__ void __launch_bounds__(256) pygw_is_gemm2(const double2* __restrict__ Aq, int ldq, int mstart, int mdim,
                                                     const double2* __restrict__ T, int nvj, int ndim, double s,
                                                     double2* __restrict__ out, long ldo, int col0) {
   using namespace is;
+  constexpr int TM = L<WR>::TM, TN = L<WR>::TN;
   __shared__ double2 As[TM][KC + 1];
   __shared__ double2 Bs[KC][TN + 1];
   const int G = blockIdx.y;
@@ -173,9 +185,9 @@ __global__ void __launch_bounds__(256) pygw_is_gemm2(const double2* __restrict__
     const int ie1 = e0 + n;
     return (k < nvj && ie1 < ndim) ? TG[(size_t)k * ndim + ie1] : double2{0.0, 0.0};
   };
-  tile(nvj, la, lb, As, Bs, aR, aI);
+  tile<WR>(nvj, la, lb, As, Bs, aR, aI);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wr = wave >> 1, wc = wave & 1, li = lane & 15, lk = lane >> 4;
+  const int wr = wave / L<WR>::WC, wc = wave % L<WR>::WC, li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -205,11 +217,21 @@ int minm_is_launch(int nstart, int nend, int mstart, int mend, const void* Aeigk
   hipLaunchKernelGGL(pygw_is_map, dim3(ceil_div((long)nvi * nvj, 256)), dim3(256), 0, stream, gindex, ng,
                      indgk_i, indgk_j, nvi, nvj, i_g0, n1, n2, n3, inv, inv_size, ngq_barc, umk[0], umk[1], umk[2],
                      kmap);
-  const int ntn = ceil_div(ndim, TN);
-  hipLaunchKernelGGL(pygw_is_gemm1, dim3(ceil_div(nvj, TM) * ntn, ngq), dim3(256), 0, stream,
-                     (const double2*)mpwipw, ngq_barc, kmap, nvi, nvj, (const double2*)Aeigk, ldk, nstart, ndim, T);
-  hipLaunchKernelGGL(pygw_is_gemm2, dim3(ceil_div(mdim, TM) * ntn, ngq), dim3(256), 0, stream,
-                     (const double2*)Aeigq, ldq, mstart, mdim, T, nvj, ndim, 1.0 / sqrt(vol), (double2*)mmat, ldo,
-                     col0);
+  auto run = [&](auto wr) {
+    constexpr int WR = decltype(wr)::value, TM = L<WR>::TM, TN = L<WR>::TN;
+    const int ntn = ceil_div(ndim, TN);
+    hipLaunchKernelGGL(pygw_is_gemm1<WR>, dim3(ceil_div(nvj, TM) * ntn, ngq), dim3(256), ngq_barc * 16, stream,
+                       (const double2*)mpwipw, ngq_barc, kmap, nvi, nvj, (const double2*)Aeigk, ldk, nstart, ndim,
+                       T);
+    hipLaunchKernelGGL(pygw_is_gemm2<WR>, dim3(ceil_div(mdim, TM) * ntn, ngq), dim3(256), 0, stream,
+                       (const double2*)Aeigq, ldq, mstart, mdim, T, nvj, ndim, 1.0 / sqrt(vol), (double2*)mmat, ldo,
+                       col0);
+  };
+  // a band window of at most 32 (the occupied bands of the polarization)
+  // runs on 128 x 32 tiles, wider ones on 64 x 64
+  if (ndim <= 32)
+    run(std::integral_constant<int, 4>());
+  else
+    run(std::integral_constant<int, 2>());
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }

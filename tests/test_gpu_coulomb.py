@@ -123,7 +123,8 @@ def test_vhalf_chain_is_phase_free():
 
 @pytest.mark.parametrize("kw,rng_", [(dict(seed=7, gmax=3, nvi=37, nvj=41, ngq=23, ngq_barc=29, nb=9), (1, 6, 2, 9)),
                                      (dict(seed=8, gmax=5, nvi=300, nvj=321, ngq=117, ngq_barc=150, nb=70),
-                                      (3, 23, 10, 70))])
+                                      (3, 23, 10, 70)),
+                                     (dict(seed=9, gmax=4, nvi=90, nvj=80, ngq=40, ngq_barc=60, nb=60), (0, 45, 5, 50))])
 def test_minm_is_vs_oracle(kw, rng_):
     """The interstitial matrix elements against the restatement of
     calc_minm_IS, alone and written after loctmatsize MT columns of the
