@@ -100,6 +100,8 @@ def calc_minm_vhalf(mmat, barcvm, n_in=None, *, complex64=False, out=None):
         out = torch.empty(mmat.shape[:-1] + (ms,), dtype=dt, device=mmat.device)
     elif out.dtype != dt or not out.is_contiguous() or out.numel() != nrows * ms:
         raise ValueError("out must be a contiguous tensor of the result's size and type")
+    if ms == 0 or nrows == 0:      # nothing kept (every eigenvalue at or below evtol): empty, as numpy gives
+        return out
     _lib.call("pygw_minm_vhalf", C.c_void_p(mmat.data_ptr()), nrows, n_in, ld, C.c_void_p(barcvm.data_ptr()),
               ldb, ms, int(complex64), C.c_void_p(out.data_ptr()), _stream_ptr(mmat.device))
     return out

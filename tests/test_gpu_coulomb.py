@@ -141,3 +141,24 @@ def test_minm_is_vs_oracle(kw, rng_):
     coulomb.calc_minm_is(*s.args(*rng_), out=full, col0=loct)
     f = full.cpu().numpy()
     assert np.all(f[..., :loct] == 7.0) and np.abs(f[..., loct:] - want).max() < 1e-13 * np.abs(want).max()
+
+
+def test_coul_setev_nothing_kept_and_degenerate():
+    """Every eigenvalue at or below evtol: PB = 0 and an empty minm, as
+    numpy gives; a degenerate spectrum (a multiple eigenvalue) keeps its
+    whole eigenspace: barcvm barcvm^H equals the oracle's."""
+    import torch
+    from pygw_amd import coulomb
+    rng = np.random.default_rng(21)
+    lam = np.concatenate([np.full(5, 1e-9), [-1e-12, 0.0]])
+    A, _ = O.hermitian_with_spectrum(lam, rng)
+    print("case coul_setev nothing kept", flush=True)
+    got = coulomb.coul_setev(A, 1e-7)
+    assert got.matsize == 0 and got.barcvm.shape == (7, 0)
+    m = coulomb.calc_minm_vhalf(torch.zeros((3, 7), dtype=torch.complex128, device="cuda"), got.barcvm)
+    assert m.shape == (3, 0)
+    lam = np.concatenate([np.full(6, 2.0), np.full(4, 0.5), [3.0, 1e-3]])
+    A, _ = O.hermitian_with_spectrum(lam, rng)
+    print("case coul_setev degenerate", flush=True)
+    got = coulomb.coul_setev(A, 1e-7, eigenvectors=True)
+    _check_basis(A, got, 1e-7, lam)
