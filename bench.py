@@ -865,6 +865,21 @@ def run_coulomb(args):
         st += t
     st /= args.steps
     t_ev, t_m32, t_m128 = st
+    # the orthogonalised interstitial basis and the plane-wave block of Vmat (once per q)
+    pws = synth.plane_wave_system(ngq=400, ngq_barc=700, gmax=10)
+    pd = {k: torch.as_tensor(np.asarray(getattr(pws, k)), device="cuda") for k in ("indgq", "gindex", "i_g0", "ipwint")}
+    gq = torch.as_tensor(pws.gqlen, device="cuda")
+    vblk = torch.zeros((1000, 1000), dtype=torch.complex128, device="cuda")
+    tp = []
+    for _ in range(1 + args.steps):
+        ev[0].record(s)
+        mpw = coulomb.pw_basis(pd["indgq"], 400, 700, pd["ipwint"], pd["gindex"], pd["i_g0"])
+        ev[1].record(s)
+        coulomb.coul_pw_block(mpw, gq, False, vmat=vblk, row0=600)
+        ev[2].record(s)
+        torch.cuda.synchronize()
+        tp.append([ev[0].elapsed_time(ev[1]) / 1e3, ev[1].elapsed_time(ev[2]) / 1e3])
+    t_pw, t_blk = np.mean(tp[1:], axis=0)
     assert cb.matsize == 900
     ev_err = float(np.abs(cb.ev.cpu().numpy() - lam).max() / lam.max())
     f_m = 8.0 * K * mb * cb.matsize
@@ -877,7 +892,8 @@ def run_coulomb(args):
            "data": "synthetic",
            "config": {"workload": f"coulomb: mbsize {mb} Hermitian Coulomb matrix (900 eigenvalues 4pi/g^2, 100 "
                                   f"below barcevtol {evtol}), PB = 900; minm for {nk} k x {npair} pairs"},
-           "stages_s": {"coul_setev": t_ev, "minm_c64": t_m32, "minm_c128": t_m128},
+           "stages_s": {"coul_setev": t_ev, "minm_c64": t_m32, "minm_c128": t_m128,
+                        "pw_basis_400x700": t_pw, "coul_pw_block_400x700": t_blk},
            "jacobi_sweeps": cb.sweeps, "max_eigenvalue_error_rel": ev_err,
            "roofline": {"bound": "mfma", "kernel": "pygw_minm_vhalf", "achieved": f_m / t_m32 / 1e12,
                         "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": f_m / t_m32 / PEAK_FP64,

@@ -323,6 +323,28 @@ int pygw_mwm_head(const void* M_selfe, int n_k, int nb1, int nb2, int pb, int nc
  *   the FP64 result rounded to complex64 (the operand of the
  *   split-precision chi0).  FP64 MFMA.
  * ------------------------------------------------------------------------ */
+/* pygw_pw_basis replaces OrthogonalizedBasisInterstitials (pygw.py:
+ *   2126-2214, with PyGW's defaults Real_sqrt_olap = True and FORT = True,
+ *   i.e. cmp_pw_olap2, for_Coulomb.f90:306-325):
+ *     mpwipw = olap^(-1/2) tmat, olap[j][i] = ipwint[G_i - G_j] (i, j < ngq),
+ *     tmat[j][i] = ipwint[G_i - G_j] (j < ngq, i < ngq_barc),
+ *   G_i = gindex[indgq[i]], the lookup through i_g0 ([2n1+1][2n2+1][2n3+1],
+ *   C order, -1 = none); olap^(-1/2) from the device Jacobi eigensolver.
+ *   indgq [ngq_barc], gindex [ng][3], i_g0 int32 0-based; ipwint complex128
+ *   [ng]; mpwipw complex128 [ngq][ngq_barc] out.  A difference outside the
+ *   i_g0 box gives 0 (cmp_pw_olap2 reuses the previous pair's index there).
+ *   Returns PYGW_E_NOTPD if olap has a zero eigenvalue.  Synchronises.
+ * pygw_coul_pw_block replaces the plane-wave block of Coulomb (pygw.py:
+ *   2419-2430): vmat[(row0 + i) ld + row0 + k] = sum_j mpwipw[i][j] 4 pi /
+ *   gqlen[j]^2 conj(mpwipw[k][j]), gqlen float64 [ngq_barc] in device
+ *   memory (|q + G| of the basis), gqlen[0] taken as 1e100 when q_is_zero;
+ *   row0 = loctmatsize. */
+int pygw_pw_basis_workspace(int ngq, int ngq_barc, size_t* bytes);
+int pygw_pw_basis(const int* indgq, int ngq, int ngq_barc, const void* ipwint, int ng, const int* gindex,
+                  const int* i_g0, int n1, int n2, int n3, void* mpwipw, void* workspace, size_t ws_bytes,
+                  void* stream);
+int pygw_coul_pw_block(const void* mpwipw, int ngq, int ngq_barc, const double* gqlen, int q_is_zero, void* vmat,
+                       long ld_vmat, int row0, void* stream);
 /* pygw_minm_is replaces fvxcn.calc_minm_is (pygw.py:2661,
  *   for_vxcnn.f90:1030-1134), the interstitial part of mmat:
  *     mmat[ie1][ie2][col0 + G] = 1/sqrt(vol) sum_{i < nvi, j < nvj}

@@ -91,3 +91,39 @@ def minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi, nvj,
         mnn = Aq @ tmat2                                                             # [mdim, ndim]
         out[:, :, iGp] = mnn.T / np.sqrt(Vol)
     return out
+
+
+def pw_olap(indgq, nrow, ncol, ipwint, gindex, i_g0):
+    """cmp_pw_olap2 (for_Coulomb.f90:306-325): out[j, i] = ipwint[G_i - G_j]
+    for j < nrow, i < ncol, G_i = gindex[indgq[i]]; 0 where the difference
+    leaves the i_g0 box (the Fortran reuses the previous pair's index there)."""
+    n = (np.array(i_g0.shape) - 1) // 2
+    g = gindex[np.asarray(indgq)]
+    d = g[None, :ncol, :] - g[:nrow, None, :]                      # [j, i, 3] = G_i - G_j
+    inside = np.all(np.abs(d) <= n, axis=-1)
+    c = np.where(inside[..., None], d + n, 0)
+    idg = np.where(inside, i_g0[c[..., 0], c[..., 1], c[..., 2]], -1)
+    return np.where(idg >= 0, np.asarray(ipwint)[np.where(idg >= 0, idg, 0)], 0.0)
+
+
+def pw_basis(indgq, ngq, ngq_barc, ipwint, gindex, i_g0):
+    """OrthogonalizedBasisInterstitials (pygw.py:2126-2214) with PyGW's
+    defaults (Real_sqrt_olap = True, FORT = True): mpwipw = olap^(-1/2) tmat."""
+    olap = pw_olap(indgq, ngq, ngq, ipwint, gindex, i_g0)
+    if np.sum(np.abs(olap.imag)) < 1e-7:
+        olap = np.array(olap.real, dtype=float)
+    eps, sgi = scipy.linalg.eigh(olap)
+    sgiH = np.matmul(sgi, np.dot(np.diag(1 / np.sqrt(np.abs(eps))), np.conj(sgi.T)))
+    tmat = pw_olap(indgq, ngq, ngq_barc, ipwint, gindex, i_g0)
+    return np.dot(sgiH, tmat)
+
+
+def coul_pw_block(mpwipw, gqlen, q_is_zero):
+    """The plane-wave block of Coulomb (pygw.py:2419-2430):
+    mpwipw diag(4 pi / |q+G|^2) mpwipw^H, |q+G_0| = 1e100 at q = 0."""
+    gq = np.array(gqlen, dtype=float)
+    if q_is_zero:
+        gq[0] = 1e100
+    Vq = 4 * np.pi / gq ** 2
+    Vmm = Vq[None, :] * np.conj(mpwipw)
+    return np.dot(mpwipw, Vmm.T)

@@ -140,3 +140,39 @@ def calc_minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi,
               gx.shape[0], C.c_void_p(g0.data_ptr()), n1, n2, n3, float(Vol), C.c_void_p(out.data_ptr()), ld,
               int(col0), None, 0, _stream_ptr(device))
     return out
+
+
+def pw_basis(indgq, ngq, ngq_barc, ipwint, gindex, i_g0, *, device="cuda"):
+    """OrthogonalizedBasisInterstitials (pygw.py:2126-2214, Real_sqrt_olap =
+    True): mpwipw = olap^(-1/2) tmat with olap[j, i] = ipwint[G_i - G_j]
+    (i, j < ngq) and tmat the same for i < ngq_barc.  Returns complex128
+    [ngq, ngq_barc]."""
+    device = _require_device(device)
+    i32 = lambda x: _dev(x, torch.int32, device).contiguous()
+    iq, gx, g0 = i32(indgq), i32(gindex), i32(i_g0)
+    ip = _dev(ipwint, torch.complex128, device).contiguous()
+    n1, n2, n3 = [(s - 1) // 2 for s in i_g0.shape]
+    out = torch.empty((ngq, ngq_barc), dtype=torch.complex128, device=device)
+    _lib.call("pygw_pw_basis", C.c_void_p(iq.data_ptr()), int(ngq), int(ngq_barc), C.c_void_p(ip.data_ptr()),
+              ip.numel(), C.c_void_p(gx.data_ptr()), C.c_void_p(g0.data_ptr()), n1, n2, n3,
+              C.c_void_p(out.data_ptr()), None, 0, _stream_ptr(device))
+    return out
+
+
+def coul_pw_block(mpwipw, gqlen, q_is_zero, *, vmat=None, row0=0):
+    """Vmat[row0:, row0:] = mpwipw diag(4 pi / gqlen^2) mpwipw^H (pygw.py:
+    2419-2430; gqlen[0] = 1e100 when q_is_zero).  Into vmat (contiguous
+    complex128 [mbsize, mbsize], row0 = loctmatsize) or a new [ngq, ngq]."""
+    if mpwipw.dtype != torch.complex128 or not mpwipw.is_contiguous():
+        raise TypeError("mpwipw must be a contiguous complex128 tensor")
+    ngq, nb = mpwipw.shape
+    g = _dev(gqlen, torch.float64, mpwipw.device).contiguous()
+    if g.numel() != nb:
+        raise ValueError("gqlen must have ngq_barc entries")
+    if vmat is None:
+        vmat = torch.zeros((ngq, ngq), dtype=torch.complex128, device=mpwipw.device)
+    elif vmat.dtype != torch.complex128 or not vmat.is_contiguous() or vmat.shape[0] < row0 + ngq:
+        raise ValueError("vmat must be a contiguous complex128 [mbsize, mbsize] with row0 + ngq <= mbsize")
+    _lib.call("pygw_coul_pw_block", C.c_void_p(mpwipw.data_ptr()), ngq, nb, C.c_void_p(g.data_ptr()),
+              int(bool(q_is_zero)), C.c_void_p(vmat.data_ptr()), vmat.shape[1], int(row0), _stream_ptr(mpwipw.device))
+    return vmat

@@ -16,6 +16,13 @@ int minm_is_launch(int nstart, int nend, int mstart, int mend, const void* Aeigk
                    int ng, const int* i_g0, int n1, int n2, int n3, double vol, void* mmat, long ldo, int col0, void* ws,
                    size_t ws_size, hipStream_t stream);
 
+size_t pw_basis_ws_bytes(int ngq, int ngq_barc);
+int pw_basis_launch(const int* indgq, int ngq, int ngq_barc, const void* ipwint, int ng, const int* gindex,
+                    const int* i_g0, int n1, int n2, int n3, void* mpwipw, void* ws, size_t ws_size,
+                    hipStream_t stream);
+int coul_pw_block_launch(const void* mpwipw, int ngq, int ngq_barc, const double* gqlen, int q_is_zero, void* vmat,
+                         long ld, int row0, hipStream_t stream);
+
 namespace {
 constexpr int kMaxMb = 16384;
 constexpr int kMaxBarc = 4096;
@@ -94,4 +101,39 @@ extern "C" int pygw_minm_is(int nstart, int nend, int mstart, int mend, const vo
     if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
   }
   return rc;
+}
+
+extern "C" int pygw_pw_basis_workspace(int ngq, int ngq_barc, size_t* bytes) {
+  if (ngq <= 0 || ngq > kMaxMb || ngq_barc < ngq || !bytes) return PYGW_E_ARG;
+  *bytes = pw_basis_ws_bytes(ngq, ngq_barc);
+  return PYGW_OK;
+}
+
+extern "C" int pygw_pw_basis(const int* indgq, int ngq, int ngq_barc, const void* ipwint, int ng, const int* gindex,
+                             const int* i_g0, int n1, int n2, int n3, void* mpwipw, void* workspace, size_t ws_bytes,
+                             void* stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  if (!indgq || !ipwint || !gindex || !i_g0 || !mpwipw || ngq <= 0 || ngq > kMaxMb || ngq_barc < ngq || ng <= 0 ||
+      n1 < 0 || n2 < 0 || n3 < 0)
+    return PYGW_E_ARG;
+  void* ws = workspace;
+  bool own = false;
+  if (!ws) {
+    ws_bytes = pw_basis_ws_bytes(ngq, ngq_barc);
+    HIP_TRY(hipMallocAsync(&ws, ws_bytes, stream));
+    own = true;
+  }
+  int rc = pw_basis_launch(indgq, ngq, ngq_barc, ipwint, ng, gindex, i_g0, n1, n2, n3, mpwipw, ws, ws_bytes, stream);
+  if (own) {
+    hipError_t e = hipFreeAsync(ws, stream);
+    if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
+  }
+  return rc;
+}
+
+extern "C" int pygw_coul_pw_block(const void* mpwipw, int ngq, int ngq_barc, const double* gqlen, int q_is_zero,
+                                  void* vmat, long ld_vmat, int row0, void* stream) {
+  if (!mpwipw || !gqlen || !vmat || ngq <= 0 || ngq_barc <= 0 || row0 < 0 || ld_vmat < (long)row0 + ngq)
+    return PYGW_E_ARG;
+  return coul_pw_block_launch(mpwipw, ngq, ngq_barc, gqlen, q_is_zero, vmat, ld_vmat, row0, (hipStream_t)stream);
 }

@@ -448,3 +448,42 @@ def interstitial_system(seed=7, gmax=3, nvi=37, nvj=41, ngq=23, ngq_barc=29, nb=
     return InterstitialSystem(gindex=g.astype(np.int64), i_g0=i_g0, indgk_i=indgk_i, indgk_j=indgk_j, indgq=indgq,
                               indgq_inverse=inv, mpwipw=cplx(ngq, ngq_barc), Aeigk=cplx(nb, len(indgk_i)),
                               Aeigq=cplx(nb, len(indgk_j)), iumklap=np.array(iumklap), nvi=nvi, nvj=nvj, vol=vol)
+
+
+@dataclass
+class PlaneWaveSystem:
+    """Inputs of OrthogonalizedBasisInterstitials and of the plane-wave block
+    of Coulomb for a cubic cell of side a with two muffin-tin spheres:
+    gindex, i_g0 as in InterstitialSystem; indgq the product-basis plane
+    waves (the ngq_barc shortest, their differences inside the i_g0 box);
+    ipwint[G] = (1/V) integral over the interstitial of e^{iG.r}; gqlen =
+    |q + G| of the basis."""
+    gindex: np.ndarray
+    i_g0: np.ndarray
+    indgq: np.ndarray
+    ipwint: np.ndarray
+    gqlen: np.ndarray
+    ngq: int
+    ngq_barc: int
+
+
+def plane_wave_system(ngq=60, ngq_barc=90, gmax=8, a=10.0, rmt=2.0, q=(0.1, 0.0, 0.0), seed=5) -> PlaneWaveSystem:
+    rng = np.random.default_rng(seed)
+    r = np.arange(-gmax, gmax + 1)
+    g = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    g = g[np.lexsort((g[:, 2], g[:, 1], g[:, 0], (g ** 2).sum(1)))]
+    i_g0 = -np.ones((2 * gmax + 1,) * 3, dtype=np.int64)
+    i_g0[g[:, 0] + gmax, g[:, 1] + gmax, g[:, 2] + gmax] = np.arange(len(g))
+    half = np.flatnonzero(np.abs(g).max(1) <= gmax // 2)
+    indgq = np.concatenate([half[:ngq], rng.permutation(half[ngq:])[:ngq_barc - ngq]])
+    vol = a ** 3
+    gc = 2 * np.pi / a * g
+    x = np.linalg.norm(gc, axis=1) * rmt
+    with np.errstate(invalid="ignore", divide="ignore"):
+        j1x = np.where(x > 1e-8, (np.sin(x) / x ** 2 - np.cos(x) / x) / x, 1.0 / 3.0)
+    tau = np.array([[0.0, 0.0, 0.0], [0.25, 0.25, 0.25]])
+    ph = np.exp(2j * np.pi * g @ tau.T).sum(1)
+    ipwint = (np.all(g == 0, axis=1).astype(float) - ph * 4 * np.pi * rmt ** 3 / vol * j1x).astype(complex)
+    gqlen = np.linalg.norm(gc[indgq] + 2 * np.pi / a * np.asarray(q), axis=1)
+    return PlaneWaveSystem(gindex=g.astype(np.int64), i_g0=i_g0, indgq=indgq, ipwint=ipwint, gqlen=gqlen, ngq=ngq,
+                           ngq_barc=ngq_barc)

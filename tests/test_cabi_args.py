@@ -39,7 +39,7 @@ def _zeros(fn):
     "pygw_chi0_batch_free", "pygw_chi0_workspace", "pygw_eps_inverse_workspace", "pygw_mwm_workspace",
     "pygw_tetra_weights_workspace", "pygw_chi0_wings_workspace", "pygw_eps_inverse_head_workspace",
     "pygw_coul_setev", "pygw_coul_setev_workspace", "pygw_minm_vhalf", "pygw_minm_is",
-    "pygw_minm_is_workspace",
+    "pygw_minm_is_workspace", "pygw_pw_basis", "pygw_pw_basis_workspace", "pygw_coul_pw_block",
 ])
 def test_null_and_zero_arguments_rejected(name):
     fn = _fn(name)

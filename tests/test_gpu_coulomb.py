@@ -162,3 +162,28 @@ def test_coul_setev_nothing_kept_and_degenerate():
     print("case coul_setev degenerate", flush=True)
     got = coulomb.coul_setev(A, 1e-7, eigenvectors=True)
     _check_basis(A, got, 1e-7, lam)
+
+
+@pytest.mark.parametrize("ngq,ngq_barc,q0", [(60, 90, False), (150, 210, True), (7, 7, False)])
+def test_pw_basis_and_coulomb_block_vs_oracle(ngq, ngq_barc, q0):
+    """mpwipw (olap^(-1/2) tmat, phase-free) and the plane-wave block of the
+    Coulomb matrix against the oracle, the block written at row0 =
+    loctmatsize of a larger Vmat (the rest left alone); then that Coulomb
+    matrix through Coul_setev."""
+    import torch
+    from pygw_amd import coulomb, synth
+    s = synth.plane_wave_system(ngq=ngq, ngq_barc=ngq_barc, q=(0, 0, 0) if q0 else (0.1, 0.0, 0.0))
+    print(f"case pw_basis ngq={ngq} ngq_barc={ngq_barc} q0={q0}", flush=True)
+    want = O.pw_basis(s.indgq, ngq, ngq_barc, s.ipwint, s.gindex, s.i_g0)
+    got = coulomb.pw_basis(s.indgq, ngq, ngq_barc, s.ipwint, s.gindex, s.i_g0)
+    assert np.abs(got.cpu().numpy() - want).max() < 1e-12 * np.abs(want).max()
+    vw = O.coul_pw_block(want, s.gqlen, q0)
+    loct = 11
+    full = torch.full((loct + ngq, loct + ngq), 3.0 + 0j, dtype=torch.complex128, device="cuda")
+    coulomb.coul_pw_block(got, s.gqlen, q0, vmat=full, row0=loct)
+    f = full.cpu().numpy()
+    assert np.all(f[:loct] == 3.0) and np.all(f[:, :loct] == 3.0)
+    assert np.abs(f[loct:, loct:] - vw).max() < 1e-12 * np.abs(vw).max()
+    cb = coulomb.coul_setev(f[loct:, loct:], 1e-7)
+    ev = np.linalg.eigvalsh(vw)
+    assert np.abs(cb.ev.cpu().numpy() - ev).max() < 1e-12 * ev.max()

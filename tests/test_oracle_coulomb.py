@@ -99,3 +99,42 @@ def test_minm_is_is_a_plane_wave_convolution():
     pairs = [(i, j) for i in range(nvi) for j in range(nvj) if Kmap[i, j] == g0]
     ref = sum(s.Aeigk[0, i] * s.Aeigq[0, j] for i, j in pairs) / np.sqrt(s.vol)
     assert abs(got[0, 0, g0] - ref) < 1e-13
+
+
+def test_pw_basis_is_the_square_root_of_the_overlap():
+    """The basis' first ngq columns are tmat's first ngq columns (olap)
+    times olap^(-1/2): the Hermitian square root of olap, which squares back
+    to it; the remaining columns are olap^(-1/2) tmat (solved, not inverted)."""
+    from pygw_amd import synth
+    s = synth.plane_wave_system()
+    olap = O.pw_olap(s.indgq, s.ngq, s.ngq, s.ipwint, s.gindex, s.i_g0)
+    tmat = O.pw_olap(s.indgq, s.ngq, s.ngq_barc, s.ipwint, s.gindex, s.i_g0)
+    assert np.array_equal(tmat[:, :s.ngq], olap)
+    mp = O.pw_basis(s.indgq, s.ngq, s.ngq_barc, s.ipwint, s.gindex, s.i_g0)
+    r = mp[:, :s.ngq]
+    assert np.abs(r - r.conj().T).max() < 1e-13
+    assert np.abs(r @ r - olap).max() < 1e-13
+    assert np.min(np.linalg.eigvalsh((r + r.conj().T) / 2)) > 0
+    assert np.abs(r @ mp - tmat).max() < 1e-12       # olap^(1/2) mpwipw = tmat
+
+
+def test_coulomb_pw_block_closed_forms():
+    """With the interstitial the whole cell (ipwint = delta_G0) the basis is
+    the plane waves themselves and the block is diag(4 pi / |q+G|^2); in
+    general it is Hermitian positive definite, and at q = 0 the G = 0 term
+    drops out."""
+    from pygw_amd import synth
+    s = synth.plane_wave_system()
+    delta = np.zeros_like(s.ipwint)
+    delta[0] = 1.0
+    mp = O.pw_basis(s.indgq, s.ngq, s.ngq_barc, delta, s.gindex, s.i_g0)
+    assert np.abs(mp - np.eye(s.ngq, s.ngq_barc)).max() < 1e-14
+    V = O.coul_pw_block(mp, s.gqlen, False)
+    assert np.abs(V - np.diag(4 * np.pi / s.gqlen[:s.ngq] ** 2)).max() < 1e-14 * V.max().real
+    mp = O.pw_basis(s.indgq, s.ngq, s.ngq_barc, s.ipwint, s.gindex, s.i_g0)
+    V = O.coul_pw_block(mp, s.gqlen, False)
+    assert np.abs(V - V.conj().T).max() < 1e-13 * np.abs(V).max()
+    assert np.linalg.eigvalsh(V).min() > 0
+    V0 = O.coul_pw_block(mp, s.gqlen, True)
+    vq = 4 * np.pi / s.gqlen ** 2
+    assert np.abs(V - V0 - vq[0] * np.outer(mp[:, 0], mp[:, 0].conj())).max() < 1e-12 * np.abs(V).max()
