@@ -961,10 +961,11 @@ def run_minm_is(args):
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"minm_is: {nk} k, nvi = nvj = {s.nvi} plane waves, ngq = {ngq} (mpwipw against "
                                   f"{s.mpwipw.shape[1]}), bands {ndim} x {mdim}"},
-           "roofline": {"bound": "mfma", "kernel": "pygw_is_gemm1 + pygw_is_gemm2", "achieved": f_k * nk / t / 1e12,
+           "roofline": {"bound": "mfma", "kernel": "pygw_is_gemm1p + pygw_is_gemm2", "achieved": f_k * nk / t / 1e12,
                         "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": f_k * nk / t / PEAK_FP64,
-                        "traffic": (load_traffic("minm_is", "pygw_is_gemm1<4>") or 0.0) +
-                                   (load_traffic("minm_is", "pygw_is_gemm2<4>") or 0.0) or None, "flops_per_k": f_k,
+                        "traffic": sum(load_traffic("minm_is", k) or 0.0 for k in (
+                            "pygw_is_gemm1p", "pygw_is_gemm2<4>", "pygw_is_scatter", "pygw_is_zero",
+                            "pygw_is_map")) or None, "flops_per_k": f_k,
                         "note": "8 FLOP per complex multiply-add of the two products; time of whole calls"}}
     if not args.no_cpu:
         from oracle import coulomb as O

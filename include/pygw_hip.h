@@ -357,8 +357,12 @@ int pygw_coul_pw_block(const void* mpwipw, int ngq, int ngq_barc, const double* 
  *   [nbmax_j][ngj] complex128 row-major, mpwipw [ngq][ngq_barc] complex128
  *   (ngq_barc <= 4096).  mmat complex128, rows (ie1*(mend-mstart) + ie2) of
  *   ld_mmat >= col0 + ngq (col0 = loctmatsize writes into the concatenated
- *   mmat).  FP64 MFMA.  workspace: NULL or pygw_minm_is_workspace bytes. */
-int pygw_minm_is_workspace(int ngq, int nvi, int nvj, int ndim, size_t* bytes);
+ *   mmat).  FP64 MFMA.  workspace: NULL or pygw_minm_is_workspace bytes
+ *   (room for P [ngq_barc][nvj][ndim]: the first product is then a dense
+ *   GEMM over the product basis); a smaller workspace of at least the
+ *   kmap and T parts (nvi*nvj int32 rounded to 256 B, then ngq*nvj*ndim
+ *   complex128) runs the gathered form instead. */
+int pygw_minm_is_workspace(int ngq, int ngq_barc, int nvi, int nvj, int ndim, size_t* bytes);
 int pygw_minm_is(int nstart, int nend, int mstart, int mend, const void* Aeigk, int nbmax_i, int ngi,
                  const void* Aeigq, int nbmax_j, int ngj, const int* iumklap, const void* mpwipw, int ngq,
                  int ngq_barc, int nvi, int nvj, const int* indgk_i, const int* indgk_j, const int* indgq_inverse,

@@ -108,7 +108,7 @@ def calc_minm_vhalf(mmat, barcvm, n_in=None, *, complex64=False, out=None):
 
 
 def calc_minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi, nvj, indgk_i, indgk_j,
-                 indgq_inverse, gindex, i_g0, Vol, *, out=None, col0=0, device="cuda"):
+                 indgq_inverse, gindex, i_g0, Vol, *, out=None, col0=0, dense=True, device="cuda"):
     """fvxcn.calc_minm_is (pygw.py:2661, for_vxcnn.f90:1030-1134): the
     interstitial part of the matrix elements, mmat_it[ie1, ie2, G] =
     1/sqrt(Vol) sum_{i, j} Aeigq[mstart+ie2, j] mpwipw[G, K(G_i - G_j +
@@ -116,7 +116,9 @@ def calc_minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi,
     [2n1+1, 2n2+1, 2n3+1], -1 = none).  Returns complex128 [nend-nstart,
     mend-mstart, ngq], or writes it into out[..., col0:col0 + ngq] of a
     contiguous complex128 [nend-nstart, mend-mstart, ld] (the columns of
-    mmat = concatenate((mmat_mt, mmat_it), -1) after loctmatsize)."""
+    mmat = concatenate((mmat_mt, mmat_it), -1) after loctmatsize).
+    dense=False runs the gathered form of the first product (no
+    [ngq_barc, nvj, ndim] scatter buffer) instead of the dense GEMM."""
     device = _require_device(device)
     i32 = lambda x: _dev(x, torch.int32, device).contiguous()
     ak = _dev(Aeigk, torch.complex128, device).contiguous()
@@ -133,12 +135,17 @@ def calc_minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi,
             raise ValueError("out must be a contiguous complex128 [nend-nstart, mend-mstart, ld] tensor")
         ld = out.shape[2]
     umk = (C.c_int * 3)(*[int(u) for u in np.asarray(iumklap)])
+    ws, wsb = None, 0
+    if not dense:       # the kmap and T parts only
+        wsb = (int(nvi) * int(nvj) * 4 + 255) // 256 * 256 + ngq * int(nvj) * ndim * 16
+        wt = torch.empty(wsb, dtype=torch.uint8, device=device)
+        ws = C.c_void_p(wt.data_ptr())
     _lib.call("pygw_minm_is", int(nstart), int(nend), int(mstart), int(mend), C.c_void_p(ak.data_ptr()),
               ak.shape[0], ak.shape[1], C.c_void_p(aq.data_ptr()), aq.shape[0], aq.shape[1], umk,
               C.c_void_p(mp.data_ptr()), ngq, mp.shape[1], int(nvi), int(nvj), C.c_void_p(gi.data_ptr()),
               C.c_void_p(gj.data_ptr()), C.c_void_p(inv.data_ptr()), inv.numel(), C.c_void_p(gx.data_ptr()),
               gx.shape[0], C.c_void_p(g0.data_ptr()), n1, n2, n3, float(Vol), C.c_void_p(out.data_ptr()), ld,
-              int(col0), None, 0, _stream_ptr(device))
+              int(col0), ws, wsb, _stream_ptr(device))
     return out
 
 

@@ -10,6 +10,7 @@ int minm_vhalf_launch(const void* mmat, long nrows, int nin, long ldm, const voi
                       void* minm, int out_c64, hipStream_t stream);
 
 size_t minm_is_ws_bytes(int ngq, int nvi, int nvj, int ndim);
+size_t minm_is_ws_bytes_dense(int ngq, int ngq_barc, int nvi, int nvj, int ndim);
 int minm_is_launch(int nstart, int nend, int mstart, int mend, const void* Aeigk, int ldk, const void* Aeigq,
                    int ldq, const int* umk, const void* mpwipw, int ngq, int ngq_barc, int nvi, int nvj,
                    const int* indgk_i, const int* indgk_j, const int* inv, int inv_size, const int* gindex,
@@ -64,9 +65,9 @@ extern "C" int pygw_minm_vhalf(const void* mmat, long n_rows, int n_in, long ld_
                            (hipStream_t)stream);
 }
 
-extern "C" int pygw_minm_is_workspace(int ngq, int nvi, int nvj, int ndim, size_t* bytes) {
-  if (ngq <= 0 || nvi <= 0 || nvj <= 0 || ndim <= 0 || !bytes) return PYGW_E_ARG;
-  *bytes = minm_is_ws_bytes(ngq, nvi, nvj, ndim);
+extern "C" int pygw_minm_is_workspace(int ngq, int ngq_barc, int nvi, int nvj, int ndim, size_t* bytes) {
+  if (ngq <= 0 || ngq_barc <= 0 || nvi <= 0 || nvj <= 0 || ndim <= 0 || !bytes) return PYGW_E_ARG;
+  *bytes = minm_is_ws_bytes_dense(ngq, ngq_barc, nvi, nvj, ndim);
   return PYGW_OK;
 }
 
@@ -89,7 +90,7 @@ extern "C" int pygw_minm_is(int nstart, int nend, int mstart, int mend, const vo
   void* ws = workspace;
   bool own = false;
   if (!ws) {
-    ws_bytes = minm_is_ws_bytes(ngq, nvi, nvj, ndim);
+    ws_bytes = minm_is_ws_bytes_dense(ngq, ngq_barc, nvi, nvj, ndim);
     HIP_TRY(hipMallocAsync(&ws, ws_bytes, stream));
     own = true;
   }

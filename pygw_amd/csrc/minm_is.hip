@@ -10,6 +10,11 @@
 //   pygw_is_map    K(i, j) from the integer G vectors (i_g0, indgq_inverse)
 //   pygw_is_gemm1  T[G][j][ie1] = sum_i mpwipw[G][K(i, j)] Aeigk[nstart+ie1][i]
 //                  (the mpwipw row of G staged in LDS and gathered through K)
+//   or, with workspace for P (the default): pygw_is_scatter puts
+//                  P[K][j][ie1] = Aeigk[nstart+ie1][i] at K = K(i, j) (one i
+//                  per (K, j): a k-point's plane waves are distinct), and
+//                  pygw_is_gemm1p forms the same T as the dense GEMM
+//                  T[G][(j, ie1)] = sum_K mpwipw[G][K] P[K][(j, ie1)]
 //   pygw_is_gemm2  mmat[ie1][ie2][col0 + G] = sum_j Aeigq[mstart+ie2][j] T[G][j][ie1] / sqrt(Vol)
 //
 // Both products run on v_mfma_f64_16x16x4f64 (tiles of four waves of 32 x 32,
@@ -134,8 +139,77 @@ __global__ void __launch_bounds__(256) pygw_is_gemm2(const double2* __restrict__
       }
 }
 
+// P[K][j][ie1] += Aeigk[nstart+ie1][i] for every (i, j) with K = K(i, j) >= 0
+// (P zeroed first; one contribution per slot unless indgk_i repeats a
+// plane wave, which the atomic sum then adds like the reference's sum)
+__global__ void __launch_bounds__(256) pygw_is_scatter(const int* __restrict__ kmap, int nvi, int nvj,
+                                                      const double2* __restrict__ Ak, int ldk, int nstart, int ndim,
+                                                      double2* __restrict__ P) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)nvi * nvj * ndim) return;
+  const int ie1 = (int)(e % ndim);
+  const long ij = e / ndim;
+  const int i = (int)(ij / nvj), j = (int)(ij % nvj);
+  const int K = kmap[(size_t)i * nvj + j];
+  if (K < 0) return;
+  const double2 a = Ak[(size_t)(nstart + ie1) * ldk + i];
+  double* d = (double*)(P + ((size_t)K * nvj + j) * ndim + ie1);
+  atomicAdd(d, a.x);
+  atomicAdd(d + 1, a.y);
+}
+
+__global__ void __launch_bounds__(256) pygw_is_zero(double2* __restrict__ P, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    P[i] = double2{0.0, 0.0};
+}
+
+// T[G][n] = sum_K mpwipw[G][K] P[K][n] (n = j ndim + ie1 < N): a dense GEMM.
+// Items (column tile, row tile), row tiles innermost, dealt to the XCDs in
+// contiguous runs so the row tiles of a column slice of P share an L2.
+__global__ void __launch_bounds__(256) pygw_is_gemm1p(const double2* __restrict__ mpwipw, int ngq, int ngq_barc,
+                                                     const double2* __restrict__ P, long N, long per,
+                                                     double2* __restrict__ T) {
+  using namespace is;
+  __shared__ double2 As[64][KC + 1];
+  __shared__ double2 Bs[KC][64 + 1];
+  const int ntm = (ngq + 63) / 64;
+  const long item = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (item >= ((N + 63) / 64) * ntm) return;
+  const int m0 = (int)(item % ntm) * 64;
+  const long n0 = item / ntm * 64;
+  d4 aR[2][2], aI[2][2];
+  zero(aR, aI);
+  auto la = [&](int m, int k) -> double2 {
+    const int r = m0 + m;
+    return (r < ngq && k < ngq_barc) ? mpwipw[(size_t)r * ngq_barc + k] : double2{0.0, 0.0};
+  };
+  auto lb = [&](int k, int c) -> double2 {
+    const long col = n0 + c;
+    return (k < ngq_barc && col < N) ? P[(size_t)k * N + col] : double2{0.0, 0.0};
+  };
+  tile<2>(ngq_barc, la, lb, As, Bs, aR, aI);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 1, wc = wave & 1, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = m0 + 32 * wr + 16 * a + 4 * v + lk;
+        const long col = n0 + 32 * wc + 16 * b + li;
+        if (r < ngq && col < N) T[(size_t)r * N + col] = double2{aR[a][b][v], aI[a][b][v]};
+      }
+}
+
 size_t minm_is_ws_bytes(int ngq, int nvi, int nvj, int ndim) {
+  (void)ngq;
   return (((size_t)nvi * nvj * 4 + 255) & ~(size_t)255) + (size_t)ngq * nvj * ndim * 16;
+}
+
+// the dense form also holds P [ngq_barc][nvj][ndim]
+size_t minm_is_ws_bytes_dense(int ngq, int ngq_barc, int nvi, int nvj, int ndim) {
+  return minm_is_ws_bytes(ngq, nvi, nvj, ndim) + (size_t)ngq_barc * nvj * ndim * 16;
 }
 
 int minm_is_launch(int nstart, int nend, int mstart, int mend, const void* Aeigk, int ldk, const void* Aeigq,
@@ -151,12 +225,27 @@ int minm_is_launch(int nstart, int nend, int mstart, int mend, const void* Aeigk
   hipLaunchKernelGGL(pygw_is_map, dim3(ceil_div((long)nvi * nvj, 256)), dim3(256), 0, stream, gindex, ng,
                      indgk_i, indgk_j, nvi, nvj, i_g0, n1, n2, n3, inv, inv_size, ngq_barc, umk[0], umk[1], umk[2],
                      kmap);
+  // with room for P, the first product is the dense GEMM over the product
+  // basis; otherwise the gather form
+  const bool dense = ws_size >= minm_is_ws_bytes_dense(ngq, ngq_barc, nvi, nvj, ndim);
   auto run = [&](auto wr) {
     constexpr int WR = decltype(wr)::value, TM = L<WR>::TM, TN = L<WR>::TN;
     const int ntn = ceil_div(ndim, TN);
-    hipLaunchKernelGGL(pygw_is_gemm1<WR>, dim3(ceil_div(nvj, TM) * ntn, ngq), dim3(256), ngq_barc * 16, stream,
-                       (const double2*)mpwipw, ngq_barc, kmap, nvi, nvj, (const double2*)Aeigk, ldk, nstart, ndim,
-                       T);
+    if (dense) {
+      double2* P = T + (size_t)ngq * nvj * ndim;
+      const long nP = (long)ngq_barc * nvj * ndim, N = (long)nvj * ndim;
+      hipLaunchKernelGGL(pygw_is_zero, dim3((unsigned)std::min<long>(ceil_div(nP, 256), 1L << 20)), dim3(256), 0,
+                         stream, P, nP);
+      hipLaunchKernelGGL(pygw_is_scatter, dim3(ceil_div((long)nvi * nvj * ndim, 256)), dim3(256), 0, stream, kmap,
+                         nvi, nvj, (const double2*)Aeigk, ldk, nstart, ndim, P);
+      const long items = (N + 63) / 64 * ceil_div(ngq, 64), per = (items + 7) / 8;
+      hipLaunchKernelGGL(pygw_is_gemm1p, dim3((unsigned)(8 * per)), dim3(256), 0, stream, (const double2*)mpwipw,
+                         ngq, ngq_barc, P, N, per, T);
+    } else {
+      hipLaunchKernelGGL(pygw_is_gemm1<WR>, dim3(ceil_div(nvj, TM) * ntn, ngq), dim3(256), ngq_barc * 16, stream,
+                         (const double2*)mpwipw, ngq_barc, kmap, nvi, nvj, (const double2*)Aeigk, ldk, nstart, ndim,
+                         T);
+    }
     hipLaunchKernelGGL(pygw_is_gemm2<WR>, dim3(ceil_div(mdim, TM) * ntn, ngq), dim3(256), 0, stream,
                        (const double2*)Aeigq, ldq, mstart, mdim, T, nvj, ndim, 1.0 / sqrt(vol), (double2*)mmat, ldo,
                        col0);

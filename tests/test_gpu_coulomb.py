@@ -136,6 +136,8 @@ def test_minm_is_vs_oracle(kw, rng_):
     print(f"case minm_is {kw} bands {rng_}", flush=True)
     got = coulomb.calc_minm_is(*s.args(*rng_)).cpu().numpy()
     assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
+    gat = coulomb.calc_minm_is(*s.args(*rng_), dense=False).cpu().numpy()     # the gathered form
+    assert np.abs(gat - want).max() < 1e-13 * np.abs(want).max()
     loct = 13
     full = torch.full(want.shape[:2] + (loct + want.shape[2],), 7.0 + 0j, dtype=torch.complex128, device="cuda")
     coulomb.calc_minm_is(*s.args(*rng_), out=full, col0=loct)
