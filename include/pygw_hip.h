@@ -345,6 +345,29 @@ int pygw_pw_basis(const int* indgq, int ngq, int ngq_barc, const void* ipwint, i
                   void* stream);
 int pygw_coul_pw_block(const void* mpwipw, int ngq, int ngq_barc, const double* gqlen, int q_is_zero, void* vmat,
                        long ld_vmat, int row0, void* stream);
+/* pygw_mixed_coulomb replaces fCl.mixed_coulomb(vq, iat+1, True, jlam, ...)
+ *   (pygw.py:2396, for_Coulomb.f90:327-408) for the equivalent atoms of atom
+ *   iat (0-based): rows row0 + im of out (ld_out >= ngq_barc), im over
+ *   (equivalent atom, product function irm, M) in the reference's order:
+ *     Y_LM(R (q + G_ig)) e^{-2 pi i pos.G_ig} (4 pi/sqrt(vol)) (-i)^L
+ *     jlam[irm][ipw] (4 pi/gqlen0[ipw]^2 if q_vq),
+ *   ipw the group of ig (G_unique: 0-based starts of the equal-|q+G| groups
+ *   of the basis), the G = 0 column 0 at q = 0.  Device: jlam float64
+ *   [nmix][ngs], gindex int32 [ng][3], indgq int32, gqlen0 float64 [ngs],
+ *   G_unique int32 [ngs], out complex128.  Host: vq [3] (lattice units),
+ *   big_l [nmix] (L <= 8, nmix <= 64), k2cartes [3][3], rotloc [nat][3][3],
+ *   trotij [ndf][3][3], mult [nat] (<= 24 per atom), pos [ndf][3]
+ *   (fractional); R = rotloc[iat] trotij[idf]^T k2cartes.
+ * pygw_coul_mixed_block replaces mat2 = dot(mpwipw, Vmatit.T) and its two
+ *   placements (pygw.py:2409-2416): vmat[(loct + i) ld + im] = mat2[i][im],
+ *   vmat[im ld + loct + i] = conj(mat2[i][im]); vmatit complex128
+ *   [loct][ngq_barc], vmat [loct + ngq][ld]. */
+int pygw_mixed_coulomb(const double* vq, int iat, int q_vq, const double* jlam, int ngs, const int* gindex,
+                       const int* indgq, const double* gqlen0, const int* G_unique, int ngq_barc, const int* big_l,
+                       int nmix, const double* k2cartes, const double* rotloc, const double* trotij, const int* mult,
+                       const double* pos, int nat, double vol, void* out, long ld_out, int row0, void* stream);
+int pygw_coul_mixed_block(const void* mpwipw, int ngq, int ngq_barc, const void* vmatit, int loct, void* vmat,
+                          long ld_vmat, void* stream);
 /* pygw_minm_is replaces fvxcn.calc_minm_is (pygw.py:2661,
  *   for_vxcnn.f90:1030-1134), the interstitial part of mmat:
  *     mmat[ie1][ie2][col0 + G] = 1/sqrt(vol) sum_{i < nvi, j < nvj}

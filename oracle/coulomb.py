@@ -127,3 +127,53 @@ def coul_pw_block(mpwipw, gqlen, q_is_zero):
     Vq = 4 * np.pi / gq ** 2
     Vmm = Vq[None, :] * np.conj(mpwipw)
     return np.dot(mpwipw, Vmm.T)
+
+
+def ylm(v, lmax):
+    """Complex spherical harmonics Y_lm(v) (Condon-Shortley phase), index
+    l*l + l + m, from scipy; v = 0 (or on the z axis) takes phi = 0, as
+    ylm (sphbes.f90:132-271) does."""
+    import scipy.special as sp
+    v = np.asarray(v, dtype=float)
+    r, rxy = np.linalg.norm(v), np.hypot(v[0], v[1])
+    th = np.arctan2(rxy, v[2]) if r > 0 else 0.0
+    ph = np.arctan2(v[1], v[0]) if rxy > 0 else 0.0
+    return np.array([sp.sph_harm_y(l, m, th, ph) for l in range(lmax + 1) for m in range(-l, l + 1)])
+
+
+def mixed_coulomb(vq, iat, q_vq, jlam, gindex, indgq, gqlen0, G_unique, ngq_barc, big_l, k2cartes, rotloc,
+                  trotij, mult, pos, vol):
+    """Mixed_Coulomb (for_Coulomb.f90:327-408), iat 0-based: for the
+    product functions (irm, L, M) of the mult[iat] equivalent atoms,
+      vtemp[im, ig] = Y_LM(R (q + G_ig)) e^{-2 pi i pos . G_ig} (4 pi / sqrt(Vol))
+                      (-i)^L jlam[irm, ipw] (4 pi / gqlen0[ipw]^2 if q_vq)
+    with ipw the unique-length group of ig (G_unique its starts), R =
+    rotloc[iat] trotij[idf]^T k2cartes; at q = 0 the G = 0 column stays 0."""
+    big_l = np.asarray(big_l)
+    lmaxb = int(big_l.max())
+    per = int(np.sum(2 * big_l + 1))
+    vtemp = np.zeros((per * mult[iat], ngq_barc), dtype=complex)
+    ipw0 = 1 if np.sum(np.abs(vq)) < 1e-7 else 0
+    ngs = len(gqlen0)
+    grp = np.zeros(ngq_barc, dtype=int) - 1
+    for ipw in range(ngs):
+        i1 = G_unique[ipw + 1] if ipw < ngs - 1 else ngq_barc
+        grp[max(G_unique[ipw], ipw0):i1] = ipw
+    G = np.asarray(gindex)[np.asarray(indgq)[:ngq_barc]]
+    idf0 = int(np.sum(mult[:iat]))
+    prefac = 4 * np.pi / np.sqrt(vol)
+    im = 0
+    for ieq in range(mult[iat]):
+        idf = idf0 + ieq
+        rot = rotloc[iat] @ (trotij[idf].T @ k2cartes)
+        sph = np.array([ylm(rot @ (np.asarray(vq) + g), lmaxb) for g in G])          # [ngq_barc, (L+1)^2]
+        expqr = np.exp(-2j * np.pi * (G @ pos[:, idf]))
+        for irm, L in enumerate(big_l):
+            for M in range(-L, L + 1):
+                cv = prefac * (-1j) ** L * np.where(grp >= 0, jlam[irm, np.maximum(grp, 0)], 0.0)
+                if q_vq:
+                    with np.errstate(divide="ignore"):      # |q+G| = 0 only in the dropped G = 0 column
+                        cv = cv * np.where(grp >= 0, 4 * np.pi / gqlen0[np.maximum(grp, 0)] ** 2, 0.0)
+                vtemp[im] = sph[:, L * L + L + M] * expqr * cv
+                im += 1
+    return vtemp

@@ -3,7 +3,8 @@ from the reference sources by `make -C oracle ref` into oracle/_ref/.
 
 Fortran calling convention: every argument by reference, arrays
 column-major, f2py's hidden dimensions passed explicitly at the end
-(for_tetrahedra.f90:390, fnc.f90:139-176, for_pade.f90:3-51).
+(for_tetrahedra.f90:390, fnc.f90:139-176, for_pade.f90:3-51,
+for_Coulomb.f90:306,327, sphbes.f90:132).
 
 The Fortran `stop` statements terminate the calling process, so callers
 only pass inputs for which the reference is defined (the C restatement
@@ -163,3 +164,50 @@ def dostet(energy, ebnd, atet, wtet):
     e, a, w = _tet_args(ebnd, atet, wtet)
     return f(_d(energy), e.ctypes.data_as(C.c_void_p), a.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p),
              _i(e.shape[0]), _i(e.shape[1]), _i(a.shape[0]))
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def ylm(v, lmax):
+    """sphbes.f90:132-271: complex Y_lm(v), index l*l + l + m."""
+    f = _load("libref_coul.so").ylm_
+    y = np.zeros((lmax + 1) ** 2, dtype=np.complex128)
+    vv = np.asarray(v, dtype=np.float64).copy()
+    f(_p(y), _p(vv), _i(lmax))
+    return y
+
+
+def cmp_pw_olap2(indgq, ipwint, gindex, i_g0, ngq, ngq_barc):
+    """for_Coulomb.f90:306-325 (0-based indgq, i_g0 [2n1+1, 2n2+1, 2n3+1])."""
+    f = _load("libref_coul.so").cmp_pw_olap2_
+    out = np.zeros((ngq, ngq_barc), dtype=np.complex128, order="F")
+    iq = np.asarray(indgq, dtype=np.int32)
+    ip = np.asarray(ipwint, dtype=np.complex128)
+    gx = np.asfortranarray(np.asarray(gindex, dtype=np.int32))
+    g0 = np.asfortranarray(np.asarray(i_g0, dtype=np.int32))
+    n1, n2, n3 = [(s - 1) // 2 for s in g0.shape]
+    f(_p(out), _p(iq), _p(ip), _p(gx), _p(g0), _i(ngq), _i(ngq_barc), _i(gx.shape[0]), _i(len(iq)), _i(n1), _i(n2),
+      _i(n3))
+    return np.ascontiguousarray(out)
+
+
+def mixed_coulomb(vq, iat, q_vq, jlam, gindex, indgq, gqlen, gqlen0, G_unique, ngq_barc, sloctmatsize, big_l,
+                  k2cartes, rotloc, trotij, mult, pos, vol):
+    """for_Coulomb.f90:327-408 with PyGW's argument meaning (pygw.py:2396):
+    iat 1-based, indgq / G_unique 0-based, jlam [nmix, ngs], rotloc [nat, 3,
+    3], trotij [ndf, 3, 3], pos [3, ndf]; returns vtemp [sloctmatsize,
+    ngq_barc]."""
+    f = _load("libref_coul.so").mixed_coulomb_
+    out = np.zeros((sloctmatsize, ngq_barc), dtype=np.complex128, order="F")
+    F = lambda a, dt: np.asfortranarray(np.asarray(a, dtype=dt))
+    jl, gx, iq, gl, g0l, gu = (F(jlam, np.float64), F(gindex, np.int32), F(indgq, np.int32), F(gqlen, np.float64),
+                               F(gqlen0, np.float64), F(G_unique, np.int32))
+    bl, k2, rl, tr, mu, ps = (F(big_l, np.int32), F(k2cartes, np.float64), F(rotloc, np.float64),
+                              F(trotij, np.float64), F(mult, np.int32), F(pos, np.float64))
+    v = F(vq, np.float64)
+    f(_p(out), _p(v), _i(iat), _i(1 if q_vq else 0), _p(jl), _p(gx), _p(iq), _p(gl), _p(g0l), _p(gu), _i(ngq_barc),
+      _i(sloctmatsize), _p(bl), _p(k2), _p(rl), _p(tr), _p(mu), _p(ps), _d(vol), _i(jl.shape[0]), _i(rl.shape[0]),
+      _i(gx.shape[0]), _i(jl.shape[1]), _i(len(gu)), _i(ps.shape[1]), _i(len(iq)))
+    return np.ascontiguousarray(out)

@@ -183,3 +183,46 @@ def coul_pw_block(mpwipw, gqlen, q_is_zero, *, vmat=None, row0=0):
     _lib.call("pygw_coul_pw_block", C.c_void_p(mpwipw.data_ptr()), ngq, nb, C.c_void_p(g.data_ptr()),
               int(bool(q_is_zero)), C.c_void_p(vmat.data_ptr()), vmat.shape[1], int(row0), _stream_ptr(mpwipw.device))
     return vmat
+
+
+def mixed_coulomb(vq, iat, q_vq, jlam, gindex, indgq, gqlen0, G_unique, ngq_barc, big_l, k2cartes, rotloc, trotij,
+                  mult, pos, vol, *, out=None, row0=0, device="cuda"):
+    """fCl.mixed_coulomb (pygw.py:2396, for_Coulomb.f90:327-408) for atom iat
+    (0-based): the rows of Vmatit of its equivalent atoms' product
+    functions, complex128 [sum_ieq sum_irm (2L+1), ngq_barc], or written
+    into out[row0:, :ngq_barc] (Vmatit[istr:iend, :]).  pos is PyGW's
+    [3, ndf]."""
+    device = _require_device(device)
+    f64 = lambda x: _dev(x, torch.float64, device).contiguous()
+    i32 = lambda x: _dev(x, torch.int32, device).contiguous()
+    jl, g0, gu, gx, iq = f64(jlam), f64(gqlen0), i32(G_unique), i32(gindex), i32(indgq)
+    hd = lambda x, dt: np.ascontiguousarray(np.asarray(x), dtype=dt)
+    vqh, blh, k2h, rlh, trh, muh = (hd(vq, np.float64), hd(big_l, np.int32), hd(k2cartes, np.float64),
+                                    hd(rotloc, np.float64), hd(trotij, np.float64), hd(mult, np.int32))
+    psh = hd(np.asarray(pos).T, np.float64)
+    rows = int(muh[iat]) * int(np.sum(2 * blh + 1))
+    if out is None:
+        out = torch.empty((rows, ngq_barc), dtype=torch.complex128, device=device)
+    elif out.dtype != torch.complex128 or not out.is_contiguous() or out.shape[0] < row0 + rows:
+        raise ValueError("out must be a contiguous complex128 Vmatit with room for the rows")
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    _lib.call("pygw_mixed_coulomb", p(vqh), int(iat), int(bool(q_vq)), C.c_void_p(jl.data_ptr()), jl.shape[1],
+              C.c_void_p(gx.data_ptr()), C.c_void_p(iq.data_ptr()), C.c_void_p(g0.data_ptr()),
+              C.c_void_p(gu.data_ptr()), int(ngq_barc), p(blh), len(blh), p(k2h), p(rlh), p(trh), p(muh), p(psh),
+              len(muh), float(vol), C.c_void_p(out.data_ptr()), out.shape[1], int(row0), _stream_ptr(device))
+    return out
+
+
+def coul_mixed_block(mpwipw, vmatit, vmat):
+    """Vmat[loct:, :loct] = mpwipw Vmatit^T and Vmat[:loct, loct:] its
+    conjugate transpose (pygw.py:2409-2416); loct = Vmatit's rows."""
+    ngq, nb = mpwipw.shape
+    loct = vmatit.shape[0]
+    for x in (mpwipw, vmatit, vmat):
+        if x.dtype != torch.complex128 or not x.is_contiguous():
+            raise TypeError("complex128 contiguous tensors expected")
+    if vmatit.shape[1] != nb or vmat.shape[0] < loct + ngq:
+        raise ValueError("shapes do not match")
+    _lib.call("pygw_coul_mixed_block", C.c_void_p(mpwipw.data_ptr()), ngq, nb, C.c_void_p(vmatit.data_ptr()), loct,
+              C.c_void_p(vmat.data_ptr()), vmat.shape[1], _stream_ptr(mpwipw.device))
+    return vmat

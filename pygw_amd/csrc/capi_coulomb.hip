@@ -24,6 +24,15 @@ int pw_basis_launch(const int* indgq, int ngq, int ngq_barc, const void* ipwint,
 int coul_pw_block_launch(const void* mpwipw, int ngq, int ngq_barc, const double* gqlen, int q_is_zero, void* vmat,
                          long ld, int row0, hipStream_t stream);
 
+int mixed_coulomb_launch(const double* vq, int iat, int q_vq, const double* jlam, int ngs, const int* gindex,
+                         const int* indgq, const double* gqlen0, const int* G_unique, int ngq_barc, const int* big_l,
+                         int nmix, const double* k2cartes, const double* rotloc, const double* trotij, const int* mult,
+                         const double* pos, int nat, double vol, void* out, long ld, int row0, void* sph_ws,
+                         hipStream_t stream);
+size_t mixed_coulomb_ws_bytes(int max_mult, int ngq_barc);
+int coul_mixed_block_launch(const void* mpwipw, int ngq, int ngq_barc, const void* vmatit, int loct, void* vmat,
+                            long ld, hipStream_t stream);
+
 namespace {
 constexpr int kMaxMb = 16384;
 constexpr int kMaxBarc = 4096;
@@ -137,4 +146,30 @@ extern "C" int pygw_coul_pw_block(const void* mpwipw, int ngq, int ngq_barc, con
   if (!mpwipw || !gqlen || !vmat || ngq <= 0 || ngq_barc <= 0 || row0 < 0 || ld_vmat < (long)row0 + ngq)
     return PYGW_E_ARG;
   return coul_pw_block_launch(mpwipw, ngq, ngq_barc, gqlen, q_is_zero, vmat, ld_vmat, row0, (hipStream_t)stream);
+}
+
+extern "C" int pygw_mixed_coulomb(const double* vq, int iat, int q_vq, const double* jlam, int ngs, const int* gindex,
+                                  const int* indgq, const double* gqlen0, const int* G_unique, int ngq_barc,
+                                  const int* big_l, int nmix, const double* k2cartes, const double* rotloc,
+                                  const double* trotij, const int* mult, const double* pos, int nat, double vol,
+                                  void* out, long ld_out, int row0, void* stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  if (!vq || !jlam || !gindex || !indgq || !gqlen0 || !G_unique || !big_l || !k2cartes || !rotloc || !trotij ||
+      !mult || !pos || !out || ngs <= 0 || ngq_barc <= 0 || nmix <= 0 || nat <= 0 || iat < 0 || iat >= nat ||
+      !(vol > 0.0) || row0 < 0 || ld_out < ngq_barc)
+    return PYGW_E_ARG;
+  void* ws = nullptr;
+  HIP_TRY(hipMallocAsync(&ws, mixed_coulomb_ws_bytes(mult[iat] > 0 ? mult[iat] : 1, ngq_barc), stream));
+  int rc = mixed_coulomb_launch(vq, iat, q_vq, jlam, ngs, gindex, indgq, gqlen0, G_unique, ngq_barc, big_l, nmix,
+                                k2cartes, rotloc, trotij, mult, pos, nat, vol, out, ld_out, row0, ws, stream);
+  hipError_t e = hipFreeAsync(ws, stream);
+  if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
+  return rc;
+}
+
+extern "C" int pygw_coul_mixed_block(const void* mpwipw, int ngq, int ngq_barc, const void* vmatit, int loct,
+                                     void* vmat, long ld_vmat, void* stream) {
+  if (!mpwipw || !vmatit || !vmat || ngq <= 0 || ngq_barc <= 0 || loct <= 0 || ld_vmat < (long)loct + ngq)
+    return PYGW_E_ARG;
+  return coul_mixed_block_launch(mpwipw, ngq, ngq_barc, vmatit, loct, vmat, ld_vmat, (hipStream_t)stream);
 }

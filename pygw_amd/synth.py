@@ -487,3 +487,58 @@ def plane_wave_system(ngq=60, ngq_barc=90, gmax=8, a=10.0, rmt=2.0, q=(0.1, 0.0,
     gqlen = np.linalg.norm(gc[indgq] + 2 * np.pi / a * np.asarray(q), axis=1)
     return PlaneWaveSystem(gindex=g.astype(np.int64), i_g0=i_g0, indgq=indgq, ipwint=ipwint, gqlen=gqlen, ngq=ngq,
                            ngq_barc=ngq_barc)
+
+
+def _rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+@dataclass
+class MixedSystem:
+    """Inputs of Mixed_Coulomb (for_Coulomb.f90:327) for a cubic cell: two
+    inequivalent atoms (mult 1 and 2), product functions of L <= 3 on each,
+    the product-basis plane waves sorted by |q + G| with the starts of the
+    equal-length groups (G_unique) and those lengths (gqlen0), and
+    synthetic Bessel integrals jlam[atom] [nmix, ngs]."""
+    vq: np.ndarray
+    gindex: np.ndarray
+    indgq: np.ndarray
+    gqlen: np.ndarray
+    gqlen0: np.ndarray
+    G_unique: np.ndarray
+    jlam: list
+    big_l: list
+    k2cartes: np.ndarray
+    rotloc: np.ndarray
+    trotij: np.ndarray
+    mult: np.ndarray
+    pos: np.ndarray
+    vol: float
+    ngq_barc: int
+
+    def args(self, iat):
+        return (self.vq, iat, True, self.jlam[iat], self.gindex, self.indgq, self.gqlen0, self.G_unique,
+                self.ngq_barc, self.big_l[iat], self.k2cartes, self.rotloc, self.trotij, self.mult, self.pos, self.vol)
+
+
+def mixed_system(ngq_barc=120, q=(0.25, 0.0, 0.125), a=10.0, gmax=6, seed=3) -> MixedSystem:
+    rng = np.random.default_rng(seed)
+    r = np.arange(-gmax, gmax + 1)
+    g = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    vq = np.asarray(q, dtype=float)
+    k2c = 2 * np.pi / a * np.eye(3)
+    ln = np.linalg.norm((vq[None, :] + g) @ k2c.T, axis=1)
+    order = np.lexsort((g[:, 2], g[:, 1], g[:, 0], np.round(ln, 10)))
+    indgq = order[:ngq_barc]
+    gql = ln[indgq]
+    starts = np.flatnonzero(np.r_[True, np.abs(np.diff(gql)) > 1e-10])
+    big_l = [np.array([0, 0, 1, 1, 2, 3]), np.array([0, 1, 2, 2, 3])]
+    mult = np.array([1, 2])
+    jl = [rng.normal(size=(len(b), len(starts))) / (1 + gql[starts][None, :]) for b in big_l]
+    return MixedSystem(vq=vq, gindex=g.astype(np.int64), indgq=indgq, gqlen=gql, gqlen0=gql[starts],
+                       G_unique=starts, jlam=jl, big_l=big_l, k2cartes=k2c,
+                       rotloc=np.stack([_rotation(rng) for _ in range(2)]),
+                       trotij=np.stack([_rotation(rng) for _ in range(3)]), mult=mult,
+                       pos=rng.uniform(0, 1, (3, 3)), vol=a ** 3, ngq_barc=ngq_barc)

@@ -189,3 +189,34 @@ def test_pw_basis_and_coulomb_block_vs_oracle(ngq, ngq_barc, q0):
     cb = coulomb.coul_setev(f[loct:, loct:], 1e-7)
     ev = np.linalg.eigvalsh(vw)
     assert np.abs(cb.ev.cpu().numpy() - ev).max() < 1e-12 * ev.max()
+
+
+@pytest.mark.parametrize("q", [(0.25, 0.0, 0.125), (0.0, 0.0, 0.0)])
+def test_mixed_coulomb_and_block_vs_oracle(q):
+    """The MT / plane-wave rows of Vmatit of both atoms (written at their row
+    offsets) against the restatement of Mixed_Coulomb (itself pinned to
+    PyGW's compiled routine), then the two mixed blocks of Vmat."""
+    import torch
+    from pygw_amd import coulomb, synth
+    s = synth.mixed_system(q=q)
+    print(f"case mixed_coulomb q={q}", flush=True)
+    want = [O.mixed_coulomb(*s.args(iat)) for iat in (0, 1)]
+    loct = sum(w.shape[0] for w in want)
+    W = torch.zeros((loct, s.ngq_barc), dtype=torch.complex128, device="cuda")
+    row = 0
+    for iat in (0, 1):
+        a = s.args(iat)
+        coulomb.mixed_coulomb(*a, out=W, row0=row)
+        row += want[iat].shape[0]
+    Wn, Wr = W.cpu().numpy(), np.concatenate(want)
+    assert np.abs(Wn - Wr).max() < 1e-12 * np.abs(Wr).max()
+    rng = np.random.default_rng(2)
+    ngq = 70
+    mp = rng.normal(size=(ngq, s.ngq_barc)) + 1j * rng.normal(size=(ngq, s.ngq_barc))
+    V = torch.full((loct + ngq, loct + ngq), 5.0 + 0j, dtype=torch.complex128, device="cuda")
+    coulomb.coul_mixed_block(torch.as_tensor(mp, device="cuda"), W, V)
+    mat2 = mp @ Wr.T
+    Vn = V.cpu().numpy()
+    tol = 1e-12 * np.abs(mat2).max()
+    assert np.abs(Vn[loct:, :loct] - mat2).max() < tol and np.abs(Vn[:loct, loct:] - mat2.conj().T).max() < tol
+    assert np.all(Vn[:loct, :loct] == 5.0) and np.all(Vn[loct:, loct:] == 5.0)

@@ -3,7 +3,10 @@
 Coul_setev, and v^(1/2) as a square root of v."""
 import numpy as np
 
+import pytest
+
 from oracle import coulomb as O
+from oracle import ref
 
 
 def test_eigh_2x2_closed_form():
@@ -138,3 +141,25 @@ def test_coulomb_pw_block_closed_forms():
     V0 = O.coul_pw_block(mp, s.gqlen, True)
     vq = 4 * np.pi / s.gqlen ** 2
     assert np.abs(V - V0 - vq[0] * np.outer(mp[:, 0], mp[:, 0].conj())).max() < 1e-12 * np.abs(V).max()
+
+
+@pytest.mark.skipif(not ref.available("libref_coul.so"), reason="oracle/_ref not built")
+def test_ylm_and_mixed_coulomb_match_reference_fortran():
+    """Y_lm against PyGW's compiled ylm (sphbes.f90) on generic, axis and
+    zero vectors; Mixed_Coulomb against the compiled routine for both atoms,
+    at q != 0 and at q = 0 (the G = 0 column dropped)."""
+    from pygw_amd import synth
+    rng = np.random.default_rng(4)
+    for v in list(rng.normal(size=(6, 3))) + [np.zeros(3), np.array([0.0, 0.0, -2.0]), np.array([1.0, 0.0, 0.0])]:
+        assert np.abs(O.ylm(v, 5) - ref.ylm(v, 5)).max() < 1e-13
+    for q in ((0.25, 0.0, 0.125), (0.0, 0.0, 0.0)):
+        s = synth.mixed_system(q=q)
+        for iat in (0, 1):
+            want = ref.mixed_coulomb(s.vq, iat + 1, True, s.jlam[iat], s.gindex, s.indgq, s.gqlen, s.gqlen0,
+                                     s.G_unique, s.ngq_barc, int(np.sum(2 * s.big_l[iat] + 1)) * s.mult[iat],
+                                     s.big_l[iat], s.k2cartes, s.rotloc, s.trotij, s.mult, s.pos, s.vol)
+            got = O.mixed_coulomb(*s.args(iat))
+            assert got.shape == want.shape
+            assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+            if q == (0.0, 0.0, 0.0):
+                assert np.all(got[:, 0] == 0)
