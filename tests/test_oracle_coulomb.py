@@ -163,3 +163,16 @@ def test_ylm_and_mixed_coulomb_match_reference_fortran():
             assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
             if q == (0.0, 0.0, 0.0):
                 assert np.all(got[:, 0] == 0)
+
+
+@pytest.mark.skipif(not ref.available("libref_coul.so"), reason="oracle/_ref not built")
+def test_pw_overlap_matches_reference_fortran():
+    """olap and tmat of the interstitial basis against PyGW's compiled
+    cmp_pw_olap2 (every difference inside the i_g0 box, where its stale
+    index is never read)."""
+    from pygw_amd import synth
+    s = synth.plane_wave_system()
+    for ncol in (s.ngq, s.ngq_barc):
+        want = ref.cmp_pw_olap2(s.indgq, s.ipwint, s.gindex, s.i_g0, s.ngq, ncol)
+        got = O.pw_olap(s.indgq, s.ngq, ncol, s.ipwint, s.gindex, s.i_g0)
+        assert np.array_equal(got, want)
