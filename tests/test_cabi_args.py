@@ -94,3 +94,18 @@ def test_coulomb_workspace_and_bounds():
     assert mv(p, 10, 8, 7, p, 8, 4, 0, p, None) == E_ARG
     assert mv(p, 10, 8, 8, p, 7, 4, 0, p, None) == E_ARG
     assert mv(p, 10, 8, 8, p, 8, 4, 2, p, None) == E_ARG
+
+
+def test_small_workspaces_rejected_before_any_device_call():
+    """A caller workspace below the size the work needs gives PYGW_E_ARG
+    (checked before anything is launched; the pointers are never touched)."""
+    p = C.c_void_p(16)
+    ms, im, sw = C.c_int(0), C.c_int(0), C.c_int(0)
+    f = _fn("pygw_coul_setev")
+    assert f(p, 100, 1e-8, None, 0, p, p, p, None, C.byref(ms), C.byref(im), C.byref(sw), p, 1024, None) == E_ARG
+    f = _fn("pygw_pw_basis")
+    assert f(p, 50, 80, p, 1000, p, p, 3, 3, 3, p, p, 1024, None) == E_ARG
+    f = _fn("pygw_minm_is")
+    umk = (C.c_int * 3)(0, 0, 0)
+    assert f(0, 4, 0, 8, p, 10, 50, p, 10, 50, umk, p, 30, 40, 50, 50, p, p, p, 100, p, 1000, p, 3, 3, 3, 100.0, p,
+             30, 0, p, 64, None) == E_ARG
