@@ -8,11 +8,16 @@ already resident in HBM; Pade continuation + QP energies after the q loop
 (mcommon.py:167-210).
 
 Default workload (no flags): the full chain at 8x8x8 k (3072 tetrahedra),
-20+120 bands, PB = 900, 64 Matsubara points; the same q-points are run with
-the FP64 path and with the split-precision (FP16 hi+lo, 3-product MFMA)
-contractions; value = split-precision seconds per q when its chi0 is within
-1e-6 (relative Frobenius, every frequency) of FP64 on the same
-complex64-rounded M, else the FP64 seconds per q.
+20+120 bands, PB = 900, 64 Matsubara points, Sigma_c summed over a q-list of
+8 distinct q-points (wkq = 1/8) through the package's q loop
+(pygw_amd.qloop.SelfEnergyQLoop).  Every q of the list runs once on the FP64
+path and once with the split-precision (FP16 hi+lo, 3-product MFMA)
+contractions on the same complex64-rounded M (errors of chi0, eps^-1 and
+Sigma_c per q and frequency; quasiparticle energies of both q sums beside
+the Pade's own sensitivity); then the timed split-precision steps cycle over
+the q-list.  value = split-precision seconds per q when every q passes
+(chi0 and eps^-1 within 1e-6, Sigma_c within 1e-5, relative Frobenius per
+frequency), else the FP64 seconds per q.
 
     python bench.py [--steps K] [--warmup W] [--config c5|c2|c3|c4|gw0|gamma|coulomb|minm_is] [--no-cpu]
 
@@ -149,23 +154,18 @@ def rel_errors(P, wl):
 
 
 def cpu_baseline(wl, kslab):
-    """PyGW's own Fortran weights (oracle/_ref, OpenMP) for one q, plus the
-    calc_eps numpy loop (per (k, w) ZGEMM, then inv per w) on a k-slab and a
-    few frequencies, extrapolated linearly in N_k and n_om."""
-    from oracle import chain, ref
+    """The CPU path of one q on this host: the C restatement of
+    bz_calcqdepw_par2 (OpenMP) in full, plus the calc_eps numpy loop (per
+    (k, w) ZGEMM, then inv per w) on a k-slab and a few frequencies,
+    extrapolated linearly in N_k and n_om."""
+    from oracle import chain
     from pygw_amd import synth
 
     cfg, kq, _, _ = wl.items[0]
     b, om, tetc = wl.bands, wl.om, wl.tetc
     nthr = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t = time.perf_counter()
-    if ref.available("libref_tetra_omp.so"):
-        kcw = ref.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc, omp=True)
-        kind = "reference"
-    else:
-        from oracle import tetra
-        kcw, _, _ = tetra.bz_calcqdepw(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc, nthreads=nthr)
-        kind = "port"
+    kcw = port_weights(b, kq, om, tetc, nthr)
     t_w = time.perf_counter() - t
     M = synth.matrix_elements(cfg, b, kq, ks=range(kslab))
     no, ne = b.n_o, b.n_e
@@ -176,18 +176,51 @@ def cpu_baseline(wl, kslab):
     t = time.perf_counter()
     chain.eps_inverse(eps[:, :, :nw])
     t_inv = (time.perf_counter() - t) * cfg.n_om / nw
-    return {"value": t_w + t_chi + t_inv, "unit": "s/q", "cores": nthr, "kind": kind,
-            "sample": (f"one q of {cfg.name}: weights in full (PyGW's Fortran bz_calcqdepw_par2, OpenMP, from "
-                       f"oracle/_ref); chi0 = calc_eps numpy ZGEMM loop on {kslab} of {cfg.nk} k, extrapolated "
-                       f"x{cfg.nk / kslab:.0f}; inverse on {nw} of {cfg.n_om} frequencies, extrapolated"
-                       if kind == "reference" else "C restatement of the weights + numpy calc_eps loop"),
+    return {"value": t_w + t_chi + t_inv, "unit": "s/q", "cores": nthr, "kind": "port", **host_info(),
+            "sample": (f"one q of {cfg.name}: weights in full (C restatement of bz_calcqdepw_par2, OpenMP, {nthr} "
+                       f"threads); chi0 = calc_eps numpy ZGEMM loop on {kslab} of {cfg.nk} k, extrapolated "
+                       f"x{cfg.nk / kslab:.0f}; inverse on {nw} of {cfg.n_om} frequencies, extrapolated"),
             "stages_s": {"weights": t_w, "chi0_extrapolated": t_chi, "inverse_extrapolated": t_inv}}
+
+
+# the 8 distinct q-points (integer coordinates on the 8x8x8 k-mesh, none at
+# Gamma) of the c5 q-list: Sigma_c is summed over them, each once, with
+# wkq = 1/8 (pygw.py:3312,3706)
+C5_QLIST = ((1, 0, 0), (2, 0, 0), (3, 0, 0), (4, 0, 0), (1, 1, 0), (2, 1, 0), (2, 2, 0), (1, 1, 1))
+
+
+def host_info():
+    """CPU model and the host BLAS numpy calls (for the CPU baseline)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        b = np.__config__.CONFIG["Build Dependencies"]["blas"]
+        blas = f"{b.get('name')} {b.get('version')}"
+    except (AttributeError, KeyError, TypeError):
+        blas = "unknown"
+    return {"cpu_model": model, "logical_cpus_visible": os.cpu_count(), "blas": blas}
+
+
+def port_weights(b, kq, om, tetc, nthr):
+    """The C restatement of bz_calcqdepw_par2 (oracle/tetra_oracle.c, OpenMP
+    over band pairs per tetrahedron as for_tetrahedra.f90:440)."""
+    from oracle import tetra
+    kcw, _, _ = tetra.bz_calcqdepw(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc, nthreads=nthr)
+    return kcw
 
 
 class ChainWorkload:
     """Full chain at C5: 8x8x8 k-mesh, 20+120 bands, PB=900, 64 Matsubara
     points on PyGW's tan mesh (precise mesh x20 for the spline); Sigma_c of
-    the 8 bands around the gap at the 64 k of the 4x4x4 sub-lattice."""
+    the 8 bands around the gap at the 64 k of the 4x4x4 sub-lattice, summed
+    over the 8 q-points of C5_QLIST."""
 
     def __init__(self):
         from pygw_amd import mesh, synth
@@ -204,71 +237,58 @@ class ChainWorkload:
         self.nbgw = 8
         self.ibgw = b.n_o - self.nbgw // 2                        # 4 highest occupied + 4 lowest empty
         self.nb2 = b.enk.shape[0]
-        self.wkq = 1.0 / base.nk
+        self.qlist = C5_QLIST
+        self.wkq = 1.0 / len(self.qlist)
 
-    def q_inputs(self, q, prec):
-        """Device inputs of one q: M of the polarization pairs and the
-        self-energy rows (complex128 for fp64, complex64 for mixed)."""
+    def loop(self, prec):
+        from pygw_amd.qloop import SelfEnergyQLoop
+        b = self.b
+        return SelfEnergyQLoop(b.enk, 0.0, b.ncg, b.nomx, b.numin, self.om, self.wo, self.tetc, self.base.pb,
+                               len(self.qlist), len(self.klist), self.nbgw, self.nb2, omega_p=self.omp,
+                               womeg_p=self.wop, precision=prec)
+
+    def q_inputs(self, q):
+        """Device inputs of one q, complex64 (the split-precision operand;
+        the FP64 path takes the same values widened): kqid column, M of the
+        polarization pairs, the self-energy rows and the k-q energies."""
         import torch
 
         from pygw_amd import synth
         cfg = synth.PolConfig(**{**self.base.__dict__, "q": q})
         kq = self.kqid[:, synth.q_index(cfg)]
         b, kl = self.b, self.klist
-        M = synth.device_matrix_elements(cfg, b, kq, complex128=(prec == "fp64"))
+        M = synth.device_matrix_elements(cfg, b, kq, complex128=False)
         e2 = np.asfortranarray(np.hstack([b.enk[:, kl], b.enk[:, kq[kl]]]))
         b2 = synth.Bands(e2, b.ncg, b.nomx, b.numin)
         kq2 = np.arange(len(kl), 2 * len(kl), dtype=np.int32)
         kq2 = np.concatenate([kq2, kq2])
-        Ms = synth.device_matrix_elements(cfg, b2, kq2, nkc=len(kl), complex128=(prec == "fp64"),
+        Ms = synth.device_matrix_elements(cfg, b2, kq2, nkc=len(kl), complex128=False,
                                           rows=(self.ibgw, self.nbgw, 0, self.nb2), seed=cfg.seed + 99)
         enk_q = torch.as_tensor(np.ascontiguousarray(b.enk[:, kq[kl]].T), device="cuda")
         return kq, M, Ms, enk_q
 
-    def step(self, P, kq, M, Ms, enk_q, prec, ev=None, keep=False):
-        """weights -> chi0 -> eps^-1 -> MWM -> spline + convolution (one q)."""
-        import torch
-        b, base = self.b, self.base
-        s = torch.cuda.current_stream()
-        rec = (lambda i: ev[i].record(s)) if ev is not None else (lambda i: None)
-        rec(0)
-        kcw = P.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, self.om, self.tetc, layout="native",
-                                  check=False)
-        rec(1)
-        eps = P.chi0_accumulate(M, kcw, precision=prec)
-        rec(2)
-        chi = eps.clone() if keep else None
-        P.eps_inverse(eps, check=False)
-        rec(3)
-        mwm = P.calc_mwm(Ms, eps, self.wkq, precision=prec)
-        rec(4)
-        sc = P.compute_selfc_inside(enk_q, mwm.view(len(self.klist), base.n_om, self.nbgw, self.nb2), self.om,
-                                    self.wo, self.omp, self.wop)
-        rec(5)
-        return sc, chi, eps
+    def ene(self):
+        """Kohn-Sham energies of the Sigma_c states, (k, band) order."""
+        b = self.b
+        return np.ascontiguousarray(b.enk[self.ibgw:self.ibgw + self.nbgw][:, self.klist].T).reshape(-1)
 
 
 def chain_cpu_baseline(wl, kslab=2, nw=4, nconv=1):
-    """PyGW's own Fortran weights (oracle/_ref, OpenMP) in full for one q,
-    plus the numpy restatements of calc_eps (per-(k, w) ZGEMM) on `kslab`
-    k-points, of the inversion on `nw` frequencies, of MWM on `kslab`
-    self-energy k-points and of the spline + convolution on `nconv` of them,
-    each extrapolated linearly to the full q."""
-    from oracle import chain, ref
+    """The CPU path of one q, timed on this host: the C restatement of
+    bz_calcqdepw_par2 (OpenMP) in full, plus the numpy restatements of
+    calc_eps (per-(k, w) ZGEMM through the host BLAS) on `kslab` k-points,
+    of the inversion on `nw` frequencies, of MWM on `kslab` self-energy
+    k-points and of the spline + convolution on `nconv` of them, each
+    extrapolated linearly to the full q."""
+    from oracle import chain
     from pygw_amd import synth
 
     base, b, om = wl.base, wl.b, wl.om
-    cfg = synth.PolConfig(**{**base.__dict__, "q": (1, 0, 0)})
+    cfg = synth.PolConfig(**{**base.__dict__, "q": wl.qlist[0]})
     kq = wl.kqid[:, synth.q_index(cfg)]
     nthr = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t = time.perf_counter()
-    if ref.available("libref_tetra_omp.so"):
-        kcw = ref.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, wl.tetc, omp=True)
-        kind = "reference"
-    else:
-        from oracle import tetra
-        kcw, _, _ = tetra.bz_calcqdepw(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, wl.tetc, nthreads=nthr)
-        kind = "port"
+    kcw = port_weights(b, kq, om, wl.tetc, nthr)
     t_w = time.perf_counter() - t
     M = synth.matrix_elements(cfg, b, kq, ks=range(kslab))
     no, ne = b.n_o, b.n_e
@@ -290,8 +310,9 @@ def chain_cpu_baseline(wl, kslab=2, nw=4, nconv=1):
     for i in range(min(nconv, kslab)):
         chain.selfc_spline(b.enk[:, i], mw[i].reshape(base.n_om, wl.nbgw, wl.nb2), om, wl.omp, wl.wop)
     t_conv = (time.perf_counter() - t) * len(wl.klist) / min(nconv, kslab)
-    return {"value": t_w + t_chi + t_inv + t_mwm + t_conv, "unit": "s/q", "cores": nthr, "kind": kind,
-            "sample": (f"one q of c5: weights in full ({'PyGW Fortran bz_calcqdepw_par2, OpenMP' if kind == 'reference' else 'C restatement'}); "
+    return {"value": t_w + t_chi + t_inv + t_mwm + t_conv, "unit": "s/q", "cores": nthr, "kind": "port",
+            **host_info(),
+            "sample": (f"one q of c5: weights in full (C restatement of bz_calcqdepw_par2, OpenMP, {nthr} threads); "
                        f"calc_eps numpy ZGEMM loop on {kslab} of {base.nk} k (x{base.nk // kslab}); inverse on "
                        f"{nw} of {base.n_om} frequencies; MWM on {kslab} and spline+convolution on "
                        f"{min(nconv, kslab)} of {len(wl.klist)} self-energy k-points; all extrapolated linearly"),
@@ -325,10 +346,47 @@ def chi0_launches(pb, npair, nom, nk):
     return -(-nk // ks)
 
 
+def _rel_per_w(a, b_, one=None):
+    """Relative Frobenius error per frequency of Fortran [PB, PB, n_om]
+    tensors (with one: of a + 1 against b + 1)."""
+    import torch
+    if one is not None:
+        a, b_ = a + one, b_ + one
+    d = torch.linalg.matrix_norm((a - b_).permute(2, 0, 1))
+    return (d / torch.linalg.matrix_norm(b_.permute(2, 0, 1))).cpu().numpy()
+
+
+def pade_conditioning(P, sig, om, ene, nsamp=4, noise=1e-7, seed=5):
+    """Per-state sensitivity of the Pade quasiparticle energy: the largest
+    shift of E_qp over `nsamp` draws of complex Gaussian noise of relative
+    size `noise` on Sigma_c(i w) (Hartree)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    base = P.pade_quasiparticles(torch.as_tensor(sig), om, ene).cpu().numpy()[:, 5]
+    out = np.zeros_like(base)
+    for _ in range(nsamp):
+        z = sig * (noise * (rng.normal(size=sig.shape) + 1j * rng.normal(size=sig.shape)) / np.sqrt(2))
+        qp = P.pade_quasiparticles(torch.as_tensor(sig + z), om, ene).cpu().numpy()[:, 5]
+        out = np.maximum(out, np.abs(qp - base))
+    return out
+
+
 def run_chain(args):
-    """The default workload: chi0 -> eps^-1 -> Sigma_c at C5, mixed and FP64
-    over the same q-points; value = mixed seconds per q when the mixed chi0
-    is within 1e-6 (relative Frobenius, per frequency) of FP64."""
+    """The default workload: weights -> chi0 -> eps^-1 -> MWM -> Sigma_c at
+    C5 through the package's q loop (pygw_amd.qloop.SelfEnergyQLoop), FP64
+    and split precision.
+
+    1. Precision pass: every q of the 8-point q-list once on both paths (the
+       same complex64 M, widened for FP64): per q and frequency the relative
+       Frobenius errors of chi0, eps^-1 and the q's Sigma_c term; Sigma_c
+       summed over the q-list (each q once, wkq = 1/8) on both paths; Pade
+       quasiparticle energies of both sums with each state's Pade
+       sensitivity.  FP64 seconds per q are the stage times of this pass.
+    2. Timed pass: `warmup` + `steps` split-precision q steps cycling over the
+       q-list, inputs resident in HBM.
+    value = split-precision seconds per q when every q's chi0 and eps^-1 are
+    within 1e-6 and its Sigma_c within 1e-5 of FP64 (relative Frobenius,
+    every frequency), else the FP64 seconds per q."""
     import torch
 
     from pygw_amd import mesh
@@ -336,94 +394,130 @@ def run_chain(args):
 
     wl = ChainWorkload()
     base, b = wl.base, wl.b
-    nq = args.steps
-    qs = [((i + 1) % base.div[0], 0, 0) for i in range(nq + args.warmup)]
+    nql = len(wl.qlist)
     stage_names = ["polarization_weights", "chi0", "eps_inverse", "mwm", "selfc_convolution"]
-    res = {}
-    for prec in ("fp64", "mixed"):
-        sig = torch.zeros((len(wl.klist), wl.nbgw, base.n_om), dtype=torch.complex128, device="cuda")
-        times, walls = [], []
-        for iq, q in enumerate(qs):
-            kq, M, Ms, enk_q = wl.q_inputs(q, prec)
-            torch.cuda.synchronize()
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
-            t0 = time.perf_counter()
-            sc, _, _ = wl.step(P, kq, M, Ms, enk_q, prec, ev)
-            torch.cuda.synchronize()
-            wall = time.perf_counter() - t0
-            if iq >= args.warmup:
-                sig += sc
-                times.append([ev[i].elapsed_time(ev[i + 1]) / 1e3 for i in range(5)])
-                walls.append(wall)
-            print(f"[bench] {prec} q={q} {wall:.3f} s{' (warm-up)' if iq < args.warmup else ''}", file=sys.stderr,
-                  flush=True)
-            del M, Ms, sc
-        ene = np.ascontiguousarray(b.enk[wl.ibgw:wl.ibgw + wl.nbgw][:, wl.klist].T).reshape(-1)
-        qp = P.pade_quasiparticles(sig.view(-1, base.n_om), wl.om, ene).cpu().numpy()
-        tm = np.array(times).mean(0)
-        res[prec] = {"s_per_q": float(tm.sum()), "wall_per_q": float(np.mean(walls)), "sig": sig.cpu().numpy(),
-                     "qp": qp, "stages_s": dict(zip(stage_names, map(float, tm)))}
-    # precision report: one q with the complex64-rounded M on both sides
-    kq, M32, Ms32, enk_q = wl.q_inputs(qs[-1], "mixed")
-    s64, c64, e64 = wl.step(P, kq, M32.to(torch.complex128), Ms32.to(torch.complex128), enk_q, "fp64", keep=True)
-    smx, cmx, emx = wl.step(P, kq, M32, Ms32, enk_q, "mixed", keep=True)
+    t0 = time.perf_counter()
+    inputs = [wl.q_inputs(q) for q in wl.qlist]         # complex64 M of all q, resident (~75 GB)
     torch.cuda.synchronize()
-
-    def rel(a, b_):
-        d = torch.linalg.matrix_norm((a - b_).permute(2, 0, 1))
-        return (d / torch.linalg.matrix_norm(b_.permute(2, 0, 1))).cpu().numpy()
+    print(f"[bench] inputs of {nql} q generated ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    L64, Lmx = wl.loop("fp64"), wl.loop("mixed")
     one = torch.eye(base.pb, dtype=torch.complex128, device="cuda")[:, :, None]
-    r_chi = rel(cmx, c64)
-    r_inv = rel(emx + one, e64 + one)
-    dsig = (smx - s64).abs().amax().item() * mesh.H2EV * 1e3
-    r_sig = ((smx - s64).abs().pow(2).sum(dim=(0, 1)).sqrt() / s64.abs().pow(2).sum(dim=(0, 1)).sqrt()).cpu().numpy()
-    del M32, Ms32, c64, e64, cmx, emx
-    ok = bool(r_chi.max() <= 1e-6)
-    # Pade sensitivity: QP energies from the FP64 Sigma_c perturbed by 1e-7 relative noise
-    rng = np.random.default_rng(5)
-    sg = res["fp64"]["sig"]
-    noise = sg * (1e-7 * (rng.normal(size=sg.shape) + 1j * rng.normal(size=sg.shape)) / np.sqrt(2))
-    qp_n = P.pade_quasiparticles(torch.as_tensor((sg + noise).reshape(-1, base.n_om), device="cuda"), wl.om,
-                                 np.ascontiguousarray(b.enk[wl.ibgw:wl.ibgw + wl.nbgw][:, wl.klist].T).reshape(-1)
-                                 ).cpu().numpy()
-    r64, rmx = res["fp64"], res["mixed"]
-    dqp = np.abs(rmx["qp"][:, 5] - r64["qp"][:, 5]) * mesh.H2EV * 1e3
-    dqp_n = np.abs(qp_n[:, 5] - r64["qp"][:, 5]) * mesh.H2EV * 1e3
-    rel_sig = np.linalg.norm(rmx["sig"] - r64["sig"], axis=(0, 1)) / np.linalg.norm(r64["sig"], axis=(0, 1))
-    nk, no, ne, pb, nom = base.nk, b.n_o, b.n_e, base.pb, base.n_om
+    ev = lambda: [torch.cuda.Event(enable_timing=True) for _ in range(6)]  # noqa: E731
+    t64, errs = [], {"chi0": [], "eps_inv": [], "sigma_c_q": []}
+    for iq, (q, (kq, M, Ms, eq)) in enumerate(zip(wl.qlist, inputs)):
+        L64.set_q(kq, M, Ms, eq)                        # the complex64 values, widened
+        Lmx.set_q(kq, enk_kq=eq)
+        torch.cuda.synchronize()
+        e64 = ev()
+        L64.run_q(e64, keep_chi0=True)
+        L64.accumulate()
+        Lmx.run_q(keep_chi0=True, M=M, M_selfe=Ms)
+        Lmx.accumulate()
+        torch.cuda.synchronize()
+        t64.append([e64[i].elapsed_time(e64[i + 1]) / 1e3 for i in range(5)])
+        errs["chi0"].append(_rel_per_w(Lmx.chi, L64.chi))
+        errs["eps_inv"].append(_rel_per_w(Lmx.eps, L64.eps, one))
+        d = (Lmx.sc - L64.sc).abs().pow(2).sum(dim=(0, 1)).sqrt()
+        errs["sigma_c_q"].append((d / L64.sc.abs().pow(2).sum(dim=(0, 1)).sqrt()).cpu().numpy())
+        print(f"[bench] precision pass q={q}: fp64 {sum(t64[-1]):.3f} s; chi0 {errs['chi0'][-1].max():.2e} "
+              f"eps^-1 {errs['eps_inv'][-1].max():.2e} Sigma_c {errs['sigma_c_q'][-1].max():.2e}",
+              file=sys.stderr, flush=True)
+    sig64, sigmx = L64.sigc.cpu().numpy(), Lmx.sigc.cpu().numpy()
+    L64.chi = Lmx.chi = None
+    del L64
+    torch.cuda.empty_cache()
+    emax = {k: float(np.max(v)) for k, v in errs.items()}
+    gates = {"chi0": 1e-6, "eps_inv": 1e-6, "sigma_c_q": 1e-5}
+    ok = all(emax[k] <= gates[k] for k in gates)
+    # quasiparticle energies of the two q sums and the Pade's own sensitivity
+    ene = wl.ene()
+    nom = base.n_om
+    qp64 = P.pade_quasiparticles(torch.as_tensor(sig64.reshape(-1, nom)), wl.om, ene).cpu().numpy()
+    qpmx = P.pade_quasiparticles(torch.as_tensor(sigmx.reshape(-1, nom)), wl.om, ene).cpu().numpy()
+    dqp = np.abs(qpmx[:, 5] - qp64[:, 5]) * mesh.H2EV * 1e3
+    sens = pade_conditioning(P, sig64.reshape(-1, nom), wl.om, ene) * mesh.H2EV * 1e3
+    rel_sig = np.linalg.norm(sigmx - sig64, axis=(0, 1)) / np.linalg.norm(sig64, axis=(0, 1))
+    dsig = float(np.abs(sigmx - sig64).max()) * mesh.H2EV * 1e3
+    well = sens <= np.median(sens)
+    # timed pass: split precision, q steps cycling over the q-list
+    Lmx.reset()
+    s = torch.cuda.current_stream()
+    times, walls = [], []
+    for it in range(args.warmup + args.steps):
+        kq, M, Ms, eq = inputs[it % nql]
+        Lmx.set_q(kq, enk_kq=eq)
+        torch.cuda.synchronize()
+        e = ev()
+        tw = time.perf_counter()
+        Lmx.run_q(e, M=M, M_selfe=Ms)
+        Lmx.accumulate()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - tw
+        if it >= args.warmup:
+            times.append([e[i].elapsed_time(e[i + 1]) / 1e3 for i in range(5)])
+            walls.append(wall)
+        print(f"[bench] mixed q={wl.qlist[it % nql]} {wall:.3f} s{' (warm-up)' if it < args.warmup else ''}",
+              file=sys.stderr, flush=True)
+    del s
+    tmx, t64m = np.array(times).mean(0), np.array(t64).mean(0)
+    st_mx, st_64 = dict(zip(stage_names, map(float, tmx))), dict(zip(stage_names, map(float, t64m)))
+    s_mx, s_64 = float(tmx.sum()), float(t64m.sum())
+    nk, no, ne, pb = base.nk, b.n_o, b.n_e, base.pb
     f_chi = 4.0 * pb * pb * nk * no * ne * nom
-    head = rmx if ok else r64
-    t_chi = head["stages_s"]["chi0"]
+    t_chi = st_mx["chi0"] if ok else st_64["chi0"]
     kern = "pygw_chi0_mixed" if ok else "pygw_chi0_f64"
     nl = chi0_launches(pb, no * ne, nom, nk) if ok else 1
     peak = PEAK_FP16 if ok else PEAK_FP64
+    # weights: HBM roofline on the algorithmic bytes (kcw written once, enk
+    # and the index arrays read once) and the FP64-VALU count per (small
+    # tetrahedron, frequency) of the reference's closed forms
+    w_bytes = 8.0 * no * ne * nk * nom + 8.0 * b.enk.shape[0] * nk + 16.0 * base.ntet + 4.0 * nk
+    w_flops = base.ntet * no * ne * nom * (200.0 + 8 * 20.0)
+    t_w = st_mx["polarization_weights"]
     rec = {
         "metric": "chi0+eps^-1+Sigma_c seconds per q-point, mixed vs FP64 GPU at <=1e-6 rel err; % roofline",
-        "value": head["s_per_q"], "unit": "s/q", "n_gpus": 1, "steps": nq, "warmup": args.warmup,
-        "ms_per_step": head["s_per_q"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "value": s_mx if ok else s_64, "unit": "s/q", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": (s_mx if ok else s_64) * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None,
         "dtype": "fp16x3-split/fp32-acc chi0+MWM, f64 weights/inverse/Sigma_c" if ok else "f64",
         "data": "synthetic",
         "config": {"workload": f"c5: 8x8x8 k-mesh (3072 tetrahedra), {no}+{ne} bands, PB={pb}, {nom} Matsubara "
                                f"points (tan mesh, precise mesh x20 = {wl.omp.size}), Sigma_c of {wl.nbgw} bands at "
-                               f"{len(wl.klist)} k; one q per step",
-                   "precision": "mixed" if ok else "fp64", "q_points": [list(q) for q in qs[args.warmup:]]},
-        "stages_s": head["stages_s"],
+                               f"{len(wl.klist)} k summed over a q-list of {nql} (wkq = 1/{nql}); one q per step",
+                   "precision": "mixed" if ok else "fp64", "q_list": [list(q) for q in wl.qlist],
+                   "timed_steps_cycle_q_list": True},
+        "stages_s": st_mx if ok else st_64,
         "roofline": {"bound": "mfma", "kernel": kern, "achieved": f_chi / t_chi / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": f_chi / t_chi / peak, "traffic": load_traffic("c5", kern),
                      "launches_per_q": nl, "flops_per_launch": f_chi / nl, "time_per_launch_s": t_chi / nl,
                      "executed_flops_per_launch": (3 * f_chi if ok else f_chi) / nl},
-        "fp64": {"s_per_q": r64["s_per_q"], "stages_s": r64["stages_s"],
-                 "chi0_tflops": f_chi / r64["stages_s"]["chi0"] / 1e12,
-                 "chi0_frac_fp64_peak": f_chi / r64["stages_s"]["chi0"] / PEAK_FP64},
-        "mixed_speedup_vs_fp64": r64["s_per_q"] / rmx["s_per_q"],
+        "weights_roofline": {"bound": "hbm", "kernels": "pygw_tw_* (per q)", "achieved": w_bytes / t_w / 1e9,
+                             "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": w_bytes / t_w / PEAK_HBM,
+                             "algorithmic_bytes": w_bytes, "traffic": load_traffic("c5", "pygw_tw_*"),
+                             "fp64_valu_tflops": w_flops / t_w / 1e12,
+                             "fp64_valu_frac": w_flops / t_w / PEAK_FP64,
+                             "fp64_valu_count": "200 FP64 ops + 8 transcendentals (20 ops each) per (tetrahedron, "
+                                                "pair, frequency)"},
+        "fp64": {"s_per_q": s_64, "stages_s": st_64, "chi0_tflops": f_chi / st_64["chi0"] / 1e12,
+                 "chi0_frac_fp64_peak": f_chi / st_64["chi0"] / PEAK_FP64},
+        "mixed_speedup_vs_fp64": s_64 / s_mx,
         "rel_frobenius_error_mixed_vs_fp64": {
-            "chi0_max": float(r_chi.max()), "chi0_median": float(np.median(r_chi)),
-            "eps_inv_max": float(r_inv.max()), "eps_inv_median": float(np.median(r_inv)),
-            "sigma_c_max": float(r_sig.max()), "sigma_c_abs_max_meV": dsig,
-            "sigma_c_summed_over_q_max": float(rel_sig.max())},
-        "qp_energy_abs_diff_meV": {"mixed_vs_fp64_max": float(dqp.max()), "mixed_vs_fp64_median": float(np.median(dqp)),
-                                   "fp64_with_1e-7_noise_max": float(dqp_n.max()),
-                                   "fp64_with_1e-7_noise_median": float(np.median(dqp_n))},
+            "per_q_max": {k: [float(np.max(x)) for x in v] for k, v in errs.items()},
+            "chi0_max": emax["chi0"], "chi0_median": float(np.median(np.concatenate(errs["chi0"]))),
+            "eps_inv_max": emax["eps_inv"], "eps_inv_median": float(np.median(np.concatenate(errs["eps_inv"]))),
+            "sigma_c_q_max": emax["sigma_c_q"], "sigma_c_summed_over_q_max": float(rel_sig.max()),
+            "sigma_c_summed_abs_max_meV": dsig, "gates": gates, "passed": ok},
+        "quasiparticles": {
+            "states": int(dqp.size), "pade_points": min(32, nom),
+            "mixed_vs_fp64_abs_diff_meV": {"median": float(np.median(dqp)), "max": float(dqp.max())},
+            "pade_sensitivity_meV_for_1e-7_noise": {"median": float(np.median(sens)), "max": float(sens.max())},
+            "better_conditioned_half": {"median_diff_meV": float(np.median(dqp[well])),
+                                        "max_diff_meV": float(dqp[well].max()),
+                                        "median_sensitivity_meV": float(np.median(sens[well]))},
+            "states_with_diff_below_sensitivity": int(np.sum(dqp <= sens)),
+            "per_band_median_diff_meV": [float(np.median(dqp.reshape(-1, wl.nbgw)[:, i])) for i in range(wl.nbgw)],
+            "per_band_median_sensitivity_meV": [float(np.median(sens.reshape(-1, wl.nbgw)[:, i]))
+                                                for i in range(wl.nbgw)]},
     }
     print("[bench] precision report done", file=sys.stderr, flush=True)
     if not args.no_cpu:
@@ -433,12 +527,12 @@ def run_chain(args):
 
 
 def batch_cpu_baseline(batch, Ks, om, tetc, kqid, kslab=16, nw=2):
-    """One q of the batch (PB closest to the batch mean): PyGW's Fortran
-    weights (oracle/_ref, OpenMP) in full, the numpy calc_eps restatement on
+    """One q of the batch (PB closest to the batch mean): the C restatement
+    of the weights (OpenMP) in full, the numpy calc_eps restatement on
     the windows of `kslab` k-points and the inversion on `nw` frequencies;
     extrapolated linearly in k / frequency, then to the batch mean by the
     flop ratio sum_q PB_q^2 K_q / (n_q PB_i^2 K_i) (chi0) and PB^3 (inverse)."""
-    from oracle import chain, ref
+    from oracle import chain
     from pygw_amd import synth
 
     b, cfg = batch.bands, batch.cfg
@@ -448,13 +542,8 @@ def batch_cpu_baseline(batch, Ks, om, tetc, kqid, kslab=16, nw=2):
     kq = kqid[:, batch.q_index[i]]
     nthr = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t = time.perf_counter()
-    if ref.available("libref_tetra_omp.so"):
-        kcw = ref.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc, omp=True)
-        kind = "reference"
-    else:
-        from oracle import tetra
-        kcw, _, _ = tetra.bz_calcqdepw(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc, nthreads=nthr)
-        kind = "port"
+    kcw = port_weights(b, kq, om, tetc, nthr)
+    kind = "port"
     t_w = time.perf_counter() - t
     win = batch.windows[i]
     ks = list(range(kslab))
@@ -472,9 +561,9 @@ def batch_cpu_baseline(batch, Ks, om, tetc, kqid, kslab=16, nw=2):
     f_ratio = float(np.mean(pbs ** 2 * np.asarray(Ks))) / (pbs[i] ** 2 * Ks[i])
     i_ratio = float(np.mean(pbs ** 3)) / pbs[i] ** 3
     val = t_w + t_chi * f_ratio + t_inv * i_ratio
-    return {"value": val, "unit": "s/q", "cores": nthr, "kind": kind,
+    return {"value": val, "unit": "s/q", "cores": nthr, "kind": kind, **host_info(),
             "sample": (f"q {i} of the batch (PB={int(pbs[i])}, K={Ks[i]} window pairs): weights in full "
-                       f"({'PyGW Fortran bz_calcqdepw_par2, OpenMP' if kind == 'reference' else 'C restatement'}); "
+                       f"(C restatement of bz_calcqdepw_par2, OpenMP, {nthr} threads); "
                        f"calc_eps numpy ZGEMM loop on the windows of {kslab} of {cfg.nk} k ({k_in} pairs, "
                        f"x{Ks[i] / k_in:.1f}); inverse on {nw} of {cfg.n_om} frequencies; scaled to the batch "
                        f"mean by PB^2 K (x{f_ratio:.3f}) and PB^3 (x{i_ratio:.3f})"),
@@ -630,17 +719,17 @@ def run_batch(args):
 
 
 def gw0_cpu_baseline(g, mwm_item, nb2):
-    """PyGW's own per-(q, k) Sigma_c of the GW0 loop (Compute_selfc_inside,
-    spline branch, mcommon.py:126-144): scipy CubicSpline per (ie1, ie2)
-    and the compiled fnc.few_fr_convolutions (oracle/_ref) per (ie1, iom),
-    on one (q, k) item, extrapolated to all items of an iteration."""
+    """PyGW's per-(q, k) Sigma_c of the GW0 loop (Compute_selfc_inside,
+    spline branch, mcommon.py:126-144) restated on the host: scipy
+    CubicSpline per (ie1, ie2) and the numpy convolution (fr_convolution2,
+    fnc.f90:86-119, vectorised over ie2) per (ie1, iom), on one (q, k) item,
+    extrapolated to all items of an iteration."""
     from scipy import interpolate
 
-    from oracle import ref
+    from oracle import chain
     enk = g.ebnd[g.jrk[0, 0], :nb2]
     nom, nb1, _ = mwm_item.shape
     fa, fp = g.omega ** 2 + 1.0, g.omega_p ** 2 + 1.0
-    kind = "reference" if ref.available("libref_fnc.so") else "port"
     t = time.perf_counter()
     for ie1 in range(nb1):
         mp = np.zeros((nb2, g.omega_p.size), dtype=complex)
@@ -650,18 +739,17 @@ def gw0_cpu_baseline(g, mwm_item, nb2):
             si = interpolate.CubicSpline(g.omega, fa * v.imag, bc_type=((2, 0.0), (1, 0.0)), extrapolate=True)
             mp[ie2] = (sr(g.omega_p) + 1j * si(g.omega_p)) / fp
         for iom in range(nom):
-            if kind == "reference":
-                ref.few_fr_convolutions(enk, mp, mwm_item[iom, ie1], g.omega_p, g.omega[iom], g.womeg_p)
-            else:
-                from oracle import chain
-                sum(chain.fr_convolution2(enk[j], mp[j], mwm_item[iom, ie1, j], g.omega_p, g.omega[iom], g.womeg_p)
-                    for j in range(nb2))
+            z = enk - 1j * g.omega[iom]                                   # [nb2]
+            sc0 = ((mp - mwm_item[iom, ie1][:, None]) * g.womeg_p[None, :]
+                   / (g.omega_p[None, :] ** 2 + (z ** 2)[:, None])).sum(1)
+            m0 = mwm_item[iom, ie1]
+            tail = np.where(np.abs(m0) > 1e-16, m0 / chain.PI_F * np.arctan(g.omega_p[-1] / z), 0.0)
+            (z * sc0 / chain.PI_F + tail).sum()
     t1 = time.perf_counter() - t
     nitems = g.jrk.size
-    return {"value": t1 * nitems, "unit": "s/iteration", "cores": 1, "kind": kind,
-            "sample": (f"one (q, k) item of the {nitems} of an iteration: scipy CubicSpline per (ie1, ie2) and "
-                       f"{'the compiled fnc.few_fr_convolutions' if kind == 'reference' else 'the numpy convolution'} "
-                       f"per (ie1, iom), x{nitems}"),
+    return {"value": t1 * nitems, "unit": "s/iteration", "cores": 1, "kind": "port", **host_info(),
+            "sample": (f"one (q, k) item of the {nitems} of an iteration: scipy CubicSpline per (ie1, ie2) and the "
+                       f"numpy convolution per (ie1, iom), x{nitems}"),
             "item_s": t1}
 
 
@@ -915,7 +1003,7 @@ def coulomb_cpu_baseline(vmat, mm_sample, evtol, scale, t_gpu_ev):
     t1 = time.perf_counter()
     O.minm(mm_sample, bv)
     t2 = time.perf_counter()
-    return {"value": (t1 - t0) + (t2 - t1) * scale, "unit": "s/q", "cores": nthr, "kind": "port",
+    return {"value": (t1 - t0) + (t2 - t1) * scale, "unit": "s/q", "cores": nthr, "kind": "port", **host_info(),
             "sample": f"scipy eigh of the {vmat.shape[0]}-square matrix in full; numpy matmul on 2 of 512 k (x{scale})",
             "stages_s": {"coul_setev": t1 - t0, "minm_extrapolated": (t2 - t1) * scale}}
 
@@ -975,7 +1063,7 @@ def run_minm_is(args):
         err = float(np.abs(out.cpu().numpy() - want).max() / np.abs(want).max())
         rec["max_rel_error_vs_oracle"] = err
         rec["cpu_baseline"] = {"value": (t1 - t0) * nk, "unit": "s/q", "cores": int(os.environ.get(
-            "OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+            "OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port", **host_info(),
             "sample": f"the numpy restatement (two matmuls per product-basis plane wave, as the Fortran's zgemm pair) "
                       f"on 1 of {nk} k, x{nk}"}
     print(json.dumps(rec))
@@ -993,7 +1081,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.steps is None:
-        args.steps = {"c5": 2, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2, "coulomb": 2, "minm_is": 2}.get(args.config, 1)
+        args.steps = {"c5": 8, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2, "coulomb": 2, "minm_is": 2}.get(args.config, 1)
     if args.copy_calibration:
         # one copy kernel that reads and writes exactly 1 GiB: the ratio of its
         # FETCH_SIZE / WRITE_SIZE counters to 1 GiB scales the kernels' counters

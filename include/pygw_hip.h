@@ -361,11 +361,17 @@ int pygw_coul_pw_block(const void* mpwipw, int ngq, int ngq_barc, const double* 
  * pygw_coul_mixed_block replaces mat2 = dot(mpwipw, Vmatit.T) and its two
  *   placements (pygw.py:2409-2416): vmat[(loct + i) ld + im] = mat2[i][im],
  *   vmat[im ld + loct + i] = conj(mat2[i][im]); vmatit complex128
- *   [loct][ngq_barc], vmat [loct + ngq][ld]. */
+ *   [loct][ngq_barc], vmat [loct + ngq][ld].
+ * pygw_mixed_coulomb_workspace: bytes of the Y_lm scratch for atoms of up to
+ *   max_mult equivalent atoms (max_mult * ngq_barc * 81 complex128); the
+ *   workspace of pygw_mixed_coulomb must hold it for mult[iat] (NULL: the
+ *   call allocates it on `stream`). */
+int pygw_mixed_coulomb_workspace(int max_mult, int ngq_barc, size_t* bytes);
 int pygw_mixed_coulomb(const double* vq, int iat, int q_vq, const double* jlam, int ngs, const int* gindex,
                        const int* indgq, const double* gqlen0, const int* G_unique, int ngq_barc, const int* big_l,
                        int nmix, const double* k2cartes, const double* rotloc, const double* trotij, const int* mult,
-                       const double* pos, int nat, double vol, void* out, long ld_out, int row0, void* stream);
+                       const double* pos, int nat, double vol, void* out, long ld_out, int row0, void* workspace,
+                       size_t ws_bytes, void* stream);
 int pygw_coul_mixed_block(const void* mpwipw, int ngq, int ngq_barc, const void* vmatit, int loct, void* vmat,
                           long ld_vmat, void* stream);
 /* pygw_minm_is replaces fvxcn.calc_minm_is (pygw.py:2661,
@@ -378,7 +384,7 @@ int pygw_coul_mixed_block(const void* mpwipw, int ngq, int ngq_barc, const void*
  *   index is -1 or out of range.  All index arrays int32, 0-based, on the
  *   device; iumklap: 3 ints in host memory.  Aeigk [nbmax_i][ngi] and Aeigq
  *   [nbmax_j][ngj] complex128 row-major, mpwipw [ngq][ngq_barc] complex128
- *   (ngq_barc <= 4096).  mmat complex128, rows (ie1*(mend-mstart) + ie2) of
+ *   (ngq_barc <= 4096 in the gathered form; any in the dense form).  mmat complex128, rows (ie1*(mend-mstart) + ie2) of
  *   ld_mmat >= col0 + ngq (col0 = loctmatsize writes into the concatenated
  *   mmat).  FP64 MFMA.  workspace: NULL or pygw_minm_is_workspace bytes
  *   (room for P [ngq_barc][nvj][ndim]: the first product is then a dense

@@ -153,16 +153,20 @@ def pw_basis(indgq, ngq, ngq_barc, ipwint, gindex, i_g0, *, device="cuda"):
     """OrthogonalizedBasisInterstitials (pygw.py:2126-2214, Real_sqrt_olap =
     True): mpwipw = olap^(-1/2) tmat with olap[j, i] = ipwint[G_i - G_j]
     (i, j < ngq) and tmat the same for i < ngq_barc.  Returns complex128
-    [ngq, ngq_barc]."""
+    [ngq, ngq_barc]; raises PygwError (PYGW_E_NOTPD) when olap is singular
+    (linearly dependent plane waves, e.g. a repeated G in indgq)."""
     device = _require_device(device)
     i32 = lambda x: _dev(x, torch.int32, device).contiguous()
     iq, gx, g0 = i32(indgq), i32(gindex), i32(i_g0)
     ip = _dev(ipwint, torch.complex128, device).contiguous()
     n1, n2, n3 = [(s - 1) // 2 for s in i_g0.shape]
     out = torch.empty((ngq, ngq_barc), dtype=torch.complex128, device=device)
+    sz = C.c_size_t(0)
+    _lib.call("pygw_pw_basis_workspace", int(ngq), int(ngq_barc), C.byref(sz))
+    ws = torch.empty(sz.value, dtype=torch.uint8, device=device)
     _lib.call("pygw_pw_basis", C.c_void_p(iq.data_ptr()), int(ngq), int(ngq_barc), C.c_void_p(ip.data_ptr()),
               ip.numel(), C.c_void_p(gx.data_ptr()), C.c_void_p(g0.data_ptr()), n1, n2, n3,
-              C.c_void_p(out.data_ptr()), None, 0, _stream_ptr(device))
+              C.c_void_p(out.data_ptr()), C.c_void_p(ws.data_ptr()), sz.value, _stream_ptr(device))
     return out
 
 
@@ -186,12 +190,13 @@ def coul_pw_block(mpwipw, gqlen, q_is_zero, *, vmat=None, row0=0):
 
 
 def mixed_coulomb(vq, iat, q_vq, jlam, gindex, indgq, gqlen0, G_unique, ngq_barc, big_l, k2cartes, rotloc, trotij,
-                  mult, pos, vol, *, out=None, row0=0, device="cuda"):
+                  mult, pos, vol, *, out=None, row0=0, workspace=None, device="cuda"):
     """fCl.mixed_coulomb (pygw.py:2396, for_Coulomb.f90:327-408) for atom iat
     (0-based): the rows of Vmatit of its equivalent atoms' product
     functions, complex128 [sum_ieq sum_irm (2L+1), ngq_barc], or written
     into out[row0:, :ngq_barc] (Vmatit[istr:iend, :]).  pos is PyGW's
-    [3, ndf]."""
+    [3, ndf].  workspace: optional uint8 device tensor of at least
+    pygw_mixed_coulomb_workspace bytes (else one is taken from torch)."""
     device = _require_device(device)
     f64 = lambda x: _dev(x, torch.float64, device).contiguous()
     i32 = lambda x: _dev(x, torch.int32, device).contiguous()
@@ -206,10 +211,15 @@ def mixed_coulomb(vq, iat, q_vq, jlam, gindex, indgq, gqlen0, G_unique, ngq_barc
     elif out.dtype != torch.complex128 or not out.is_contiguous() or out.shape[0] < row0 + rows:
         raise ValueError("out must be a contiguous complex128 Vmatit with room for the rows")
     p = lambda a: a.ctypes.data_as(C.c_void_p)
+    sz = C.c_size_t(0)
+    _lib.call("pygw_mixed_coulomb_workspace", max(int(muh[iat]), 1), int(ngq_barc), C.byref(sz))
+    if workspace is None or workspace.numel() < sz.value:
+        workspace = torch.empty(sz.value, dtype=torch.uint8, device=device)
     _lib.call("pygw_mixed_coulomb", p(vqh), int(iat), int(bool(q_vq)), C.c_void_p(jl.data_ptr()), jl.shape[1],
               C.c_void_p(gx.data_ptr()), C.c_void_p(iq.data_ptr()), C.c_void_p(g0.data_ptr()),
               C.c_void_p(gu.data_ptr()), int(ngq_barc), p(blh), len(blh), p(k2h), p(rlh), p(trh), p(muh), p(psh),
-              len(muh), float(vol), C.c_void_p(out.data_ptr()), out.shape[1], int(row0), _stream_ptr(device))
+              len(muh), float(vol), C.c_void_p(out.data_ptr()), out.shape[1], int(row0),
+              C.c_void_p(workspace.data_ptr()), workspace.numel(), _stream_ptr(device))
     return out
 
 

@@ -93,20 +93,20 @@ extern "C" int pygw_eps_inverse(void* eps, int pb, int n_om, void* workspace, si
   int* dinfo = (int*)((char*)ws + need - 256 - sizeof(int) * (size_t)n_om);
   hipLaunchKernelGGL(pygw_fill_i32, dim3(ceil_div(n_om, 256)), dim3(256), 0, stream, dinfo, n_om, 0);
   int rc = eps_inverse_launch(eps, pb, n_om, ws, dinfo, stream);
+  int* h = nullptr;
+  if (rc == PYGW_OK && info && !(h = (int*)malloc(sizeof(int) * (size_t)n_om))) rc = PYGW_E_HIP;
   if (rc == PYGW_OK && info) {
-    int* h = (int*)malloc(sizeof(int) * (size_t)n_om);
-    if (!h) return PYGW_E_ARG;
     hipError_t e1 = hipMemcpyAsync(h, dinfo, sizeof(int) * (size_t)n_om, hipMemcpyDeviceToHost, stream);
     hipError_t e2 = hipStreamSynchronize(stream);
     int bad = 0;
     for (int w = 0; w < n_om; w++) bad |= h[w];
-    free(h);
     if (e1 != hipSuccess || e2 != hipSuccess) rc = PYGW_E_HIP;
     else {
       info[0] = bad;
       if (bad) rc = PYGW_E_NOTPD;
     }
   }
+  free(h);
   if (own) {
     hipError_t e = hipFreeAsync(ws, stream);
     if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;

@@ -35,7 +35,8 @@ int coul_mixed_block_launch(const void* mpwipw, int ngq, int ngq_barc, const voi
 
 namespace {
 constexpr int kMaxMb = 16384;
-constexpr int kMaxBarc = 4096;
+constexpr int kMaxBarc = 4096;                  // gathered form of pygw_minm_is (LDS row)
+constexpr size_t kDenseCap = (size_t)4 << 30;    // largest P buffer pygw_minm_is allocates itself
 }
 
 extern "C" int pygw_coul_setev_workspace(int mbsize, size_t* bytes) {
@@ -92,15 +93,23 @@ extern "C" int pygw_minm_is(int nstart, int nend, int mstart, int mend, const vo
     return PYGW_E_ARG;
   if (nstart < 0 || nend <= nstart || nend > nbmax_i || mstart < 0 || mend <= mstart || mend > nbmax_j ||
       nvi <= 0 || nvi > ngi || nvj <= 0 || nvj > ngj || ngq <= 0 || ngq > 65535 || ngq_barc <= 0 ||
-      ngq_barc > kMaxBarc || inv_size <= 0 || ng <= 0 || n1 < 0 || n2 < 0 || n3 < 0 || !(vol > 0.0) || col0 < 0 ||
+      inv_size <= 0 || ng <= 0 || n1 < 0 || n2 < 0 || n3 < 0 || !(vol > 0.0) || col0 < 0 ||
       ld_mmat < (long)col0 + ngq)
     return PYGW_E_ARG;
   const int ndim = nend - nstart;
   void* ws = workspace;
   bool own = false;
   if (!ws) {
+    // the dense form's P buffer when it is at most kDenseCap and can be had,
+    // else the gathered form's (kmap + T only)
     ws_bytes = minm_is_ws_bytes_dense(ngq, ngq_barc, nvi, nvj, ndim);
-    HIP_TRY(hipMallocAsync(&ws, ws_bytes, stream));
+    if (ws_bytes > kDenseCap || hipMallocAsync(&ws, ws_bytes, stream) != hipSuccess) {
+      (void)hipGetLastError();
+      ws = nullptr;
+      if (ngq_barc > kMaxBarc) return PYGW_E_ARG;
+      ws_bytes = minm_is_ws_bytes(ngq, nvi, nvj, ndim);
+      HIP_TRY(hipMallocAsync(&ws, ws_bytes, stream));
+    }
     own = true;
   }
   int rc = minm_is_launch(nstart, nend, mstart, mend, Aeigk, ngi, Aeigq, ngj, iumklap, mpwipw, ngq, ngq_barc, nvi, nvj,
@@ -148,22 +157,37 @@ extern "C" int pygw_coul_pw_block(const void* mpwipw, int ngq, int ngq_barc, con
   return coul_pw_block_launch(mpwipw, ngq, ngq_barc, gqlen, q_is_zero, vmat, ld_vmat, row0, (hipStream_t)stream);
 }
 
+extern "C" int pygw_mixed_coulomb_workspace(int max_mult, int ngq_barc, size_t* bytes) {
+  if (max_mult <= 0 || ngq_barc <= 0 || !bytes) return PYGW_E_ARG;
+  *bytes = mixed_coulomb_ws_bytes(max_mult, ngq_barc);
+  return PYGW_OK;
+}
+
 extern "C" int pygw_mixed_coulomb(const double* vq, int iat, int q_vq, const double* jlam, int ngs, const int* gindex,
                                   const int* indgq, const double* gqlen0, const int* G_unique, int ngq_barc,
                                   const int* big_l, int nmix, const double* k2cartes, const double* rotloc,
                                   const double* trotij, const int* mult, const double* pos, int nat, double vol,
-                                  void* out, long ld_out, int row0, void* stream_) {
+                                  void* out, long ld_out, int row0, void* workspace, size_t ws_bytes, void* stream_) {
   hipStream_t stream = (hipStream_t)stream_;
   if (!vq || !jlam || !gindex || !indgq || !gqlen0 || !G_unique || !big_l || !k2cartes || !rotloc || !trotij ||
       !mult || !pos || !out || ngs <= 0 || ngq_barc <= 0 || nmix <= 0 || nat <= 0 || iat < 0 || iat >= nat ||
       !(vol > 0.0) || row0 < 0 || ld_out < ngq_barc)
     return PYGW_E_ARG;
-  void* ws = nullptr;
-  HIP_TRY(hipMallocAsync(&ws, mixed_coulomb_ws_bytes(mult[iat] > 0 ? mult[iat] : 1, ngq_barc), stream));
+  const size_t need = mixed_coulomb_ws_bytes(mult[iat] > 0 ? mult[iat] : 1, ngq_barc);
+  void* ws = workspace;
+  bool own = false;
+  if (!ws) {
+    HIP_TRY(hipMallocAsync(&ws, need, stream));
+    own = true;
+  } else if (ws_bytes < need) {
+    return PYGW_E_ARG;
+  }
   int rc = mixed_coulomb_launch(vq, iat, q_vq, jlam, ngs, gindex, indgq, gqlen0, G_unique, ngq_barc, big_l, nmix,
                                 k2cartes, rotloc, trotij, mult, pos, nat, vol, out, ld_out, row0, ws, stream);
-  hipError_t e = hipFreeAsync(ws, stream);
-  if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
+  if (own) {
+    hipError_t e = hipFreeAsync(ws, stream);
+    if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
+  }
   return rc;
 }
 

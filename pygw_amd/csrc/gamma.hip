@@ -339,21 +339,24 @@ extern "C" int pygw_eps_inverse_head(void* eps, int pb, int n_om, void* head, vo
                        (double2*)eps, pb, (const double2*)hbuf, (const double2*)bw1, (const double2*)w2b);
     rc = eps_minus_identity(eps, pb, n_om, stream);
   }
+  int* h = nullptr;
+  if (rc == PYGW_OK && info && !(h = (int*)malloc(sizeof(int) * (size_t)n_om))) rc = PYGW_E_HIP;
   if (rc == PYGW_OK && info) {
-    int* h = (int*)malloc(sizeof(int) * (size_t)n_om);
-    if (!h) return PYGW_E_ARG;
     hipError_t e1 = hipMemcpyAsync(h, dinfo, sizeof(int) * (size_t)n_om, hipMemcpyDeviceToHost, stream);
     hipError_t e2 = hipStreamSynchronize(stream);
     int bad = 0;
     for (int w = 0; w < n_om; w++) bad |= h[w];
-    free(h);
     if (e1 != hipSuccess || e2 != hipSuccess) rc = PYGW_E_HIP;
     else {
       info[0] = bad;
       if (bad) rc = PYGW_E_NOTPD;
     }
   }
-  if (own) HIP_TRY(hipFreeAsync(ws, stream));
+  free(h);
+  if (own) {
+    hipError_t e = hipFreeAsync(ws, stream);
+    if (e != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
+  }
   return rc;
 }
 

@@ -219,15 +219,15 @@ int minm_is_launch(int nstart, int nend, int mstart, int mend, const void* Aeigk
                    size_t ws_size, hipStream_t stream) {
   using namespace is;
   const int ndim = nend - nstart, mdim = mend - mstart;
-  if (ws_size < minm_is_ws_bytes(ngq, nvi, nvj, ndim) || ngq_barc > kMaxBarc) return PYGW_E_ARG;
+  // with room for P, the first product is the dense GEMM over the product
+  // basis; otherwise the gather form, whose LDS row limits ngq_barc
+  const bool dense = ws_size >= minm_is_ws_bytes_dense(ngq, ngq_barc, nvi, nvj, ndim);
+  if (ws_size < minm_is_ws_bytes(ngq, nvi, nvj, ndim) || (!dense && ngq_barc > kMaxBarc)) return PYGW_E_ARG;
   int* kmap = (int*)ws;
   double2* T = (double2*)((char*)ws + (((size_t)nvi * nvj * 4 + 255) & ~(size_t)255));
   hipLaunchKernelGGL(pygw_is_map, dim3(ceil_div((long)nvi * nvj, 256)), dim3(256), 0, stream, gindex, ng,
                      indgk_i, indgk_j, nvi, nvj, i_g0, n1, n2, n3, inv, inv_size, ngq_barc, umk[0], umk[1], umk[2],
                      kmap);
-  // with room for P, the first product is the dense GEMM over the product
-  // basis; otherwise the gather form
-  const bool dense = ws_size >= minm_is_ws_bytes_dense(ngq, ngq_barc, nvi, nvj, ndim);
   auto run = [&](auto wr) {
     constexpr int WR = decltype(wr)::value, TM = L<WR>::TM, TN = L<WR>::TN;
     const int ntn = ceil_div(ndim, TN);

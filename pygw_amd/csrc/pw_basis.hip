@@ -14,7 +14,10 @@
 //
 // The eigendecomposition of olap is the block Jacobi method of the Coulomb
 // basis; the products run on the shared FP64 MFMA tile.
+#include <algorithm>
 #include <cfloat>
+#include <cmath>
+#include <vector>
 
 #include "gemm_tile.h"
 
@@ -196,6 +199,17 @@ int pw_basis_launch(const int* indgq, int ngq, int ngq_barc, const void* ipwint,
                                    coul_setev_ws_bytes(ngq), stream);
   if (rc != PYGW_OK) return rc;
   if (ms != ngq) return PYGW_E_NOTPD;
+  // olap is a Gram matrix: a (numerically) zero eigenvalue means linearly
+  // dependent plane waves (a repeated G in indgq), and olap^(-1/2) does not exist
+  std::vector<double> evh(n);
+  HIP_TRY(hipMemcpyAsync(evh.data(), ev, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  double emax = 0.0, emin = DBL_MAX;
+  for (double e : evh) {
+    emax = std::max(emax, std::fabs(e));
+    emin = std::min(emin, e);
+  }
+  if (!(emin > (double)n * DBL_EPSILON * emax)) return PYGW_E_NOTPD;
   hipLaunchKernelGGL(pygw_pw_weights, dim3(ceil_div(ngq, 256)), dim3(256), 0, stream, be, ngq, w);
   const int nt = ceil_div(ngq, 64);
   hipLaunchKernelGGL(pygw_pw_isqrt, dim3(nt * nt), dim3(256), 0, stream, B, ngq, w, S);

@@ -40,7 +40,7 @@ def _zeros(fn):
     "pygw_tetra_weights_workspace", "pygw_chi0_wings_workspace", "pygw_eps_inverse_head_workspace",
     "pygw_coul_setev", "pygw_coul_setev_workspace", "pygw_minm_vhalf", "pygw_minm_is",
     "pygw_minm_is_workspace", "pygw_pw_basis", "pygw_pw_basis_workspace", "pygw_coul_pw_block",
-    "pygw_mixed_coulomb", "pygw_coul_mixed_block",
+    "pygw_mixed_coulomb", "pygw_mixed_coulomb_workspace", "pygw_coul_mixed_block",
 ])
 def test_null_and_zero_arguments_rejected(name):
     fn = _fn(name)

@@ -221,3 +221,18 @@ def test_mixed_coulomb_and_block_vs_oracle(q):
     tol = 1e-12 * np.abs(mat2).max()
     assert np.abs(Vn[loct:, :loct] - mat2).max() < tol and np.abs(Vn[:loct, loct:] - mat2.conj().T).max() < tol
     assert np.all(Vn[:loct, :loct] == 5.0) and np.all(Vn[loct:, loct:] == 5.0)
+
+
+def test_pw_basis_singular_overlap_rejected():
+    """A repeated plane wave in indgq makes olap singular: pygw_pw_basis
+    reports PYGW_E_NOTPD instead of writing inf/NaN into mpwipw."""
+    from pygw_amd import _lib, coulomb, synth
+    s = synth.plane_wave_system(ngq=30, ngq_barc=45)
+    indgq = s.indgq.copy()
+    indgq[1] = indgq[0]
+    print("case pw_basis repeated plane wave", flush=True)
+    with pytest.raises(_lib.PygwError) as e:
+        coulomb.pw_basis(indgq, 30, 45, s.ipwint, s.gindex, s.i_g0)
+    assert e.value.code == -13
+    good = coulomb.pw_basis(s.indgq, 30, 45, s.ipwint, s.gindex, s.i_g0)     # the call after the error is unaffected
+    assert bool(good.isfinite().all())

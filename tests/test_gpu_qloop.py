@@ -1,0 +1,178 @@
+"""The q loop end to end against the oracle: weights -> chi0 -> eps^-1 ->
+MWM -> spline + convolution per q, Sigma_c summed over a 3-point q-list
+(wkq = 1/3), then the Pade quasiparticle energies (pygw.py:3654-3718,
+3039-3413; mcommon.py:80-210).
+
+The oracle runs PyGW's formulation stage by stage in FP64 (oracle/chain.py,
+oracle/tetra_oracle.c).  Every stage after the weights is deterministic
+FP64 arithmetic, so it is fed the GPU's weights and must agree to rounding
+(chi0 1e-13, eps^-1 50 eps cond, MWM 1e-13, Sigma_c 1e-11); the weights
+themselves are checked against the oracle's within their measured one-ulp
+atan/log spread.  The quasiparticle energies go through a 16-point Thiele
+continued fraction, whose sensitivity to its input is measured per state
+(noise at the level of the Sigma_c difference) and bounds the allowed
+difference.  The split-precision loop is held to 1e-6 (chi0, eps^-1) and
+1e-5 (Sigma_c) against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import chain, tetra
+from pygw_amd import mesh, synth
+
+pytestmark = pytest.mark.gpu
+
+QLIST = ((1, 0, 0), (0, 1, 1), (1, 1, 0))
+KS = np.array([0, 4, 13, 20])          # the Sigma_c k-points
+NB1 = 2                                 # Sigma_c bands: highest occupied, lowest empty
+
+
+def _system():
+    cfg = synth.small_config(div=(3, 3, 3), n_occ=3, n_emp=6, n_om=16, pb=60, seed=41, vhalf="coulomb",
+                             wmin_ev=0.05, wmax_ev=80.0)
+    b = synth.band_energies(cfg)
+    fm = mesh.FrequencyMesh(cfg.n_om, cfg.wmin_ev / mesh.H2EV, cfg.wmax_ev / mesh.H2EV, iop_multiple=5)
+    tetc = mesh.tetra_corners(cfg.div)
+    kqid = mesh.k_minus_q(cfg.div)
+    nb2 = b.enk.shape[0]
+    rng = np.random.default_rng(5)
+    qs = []
+    for q in QLIST:
+        c = synth.PolConfig(**{**cfg.__dict__, "q": q})
+        kq = kqid[:, synth.q_index(c)]
+        M = synth.matrix_elements(c, b, kq)
+        Ms = 0.3 * (rng.normal(size=(len(KS), NB1 * nb2, cfg.pb))
+                    + 1j * rng.normal(size=(len(KS), NB1 * nb2, cfg.pb))) / np.sqrt(cfg.pb)
+        enq = np.ascontiguousarray(b.enk[:, kq[KS]].T)
+        qs.append((kq, M, Ms, enq))
+    return cfg, b, fm, tetc, qs
+
+
+def _loop(cfg, b, fm, tetc, precision):
+    from pygw_amd.qloop import SelfEnergyQLoop
+    return SelfEnergyQLoop(b.enk, 0.0, b.ncg, b.nomx, b.numin, fm.omega, fm.womeg, tetc, cfg.pb, len(QLIST), len(KS),
+                           NB1, b.enk.shape[0], omega_p=fm.omega_precise, womeg_p=fm.womeg_precise,
+                           precision=precision)
+
+
+def _oracle_q(b, fm, cfg, kcw_native, M, Ms, enq):
+    """One q of calc_eps + calc_selfc on given weights [N_k, n_om, N_p]."""
+    chi = chain.calc_eps_accumulate(((ik, M[ik]) for ik in range(cfg.nk)), lambda ik: kcw_native[ik].T)
+    W = chain.eps_inverse(chi)
+    sc = np.empty((len(KS), NB1, cfg.n_om), dtype=complex)
+    for j in range(len(KS)):
+        mw = chain.calc_mwm(Ms[j], W, 1.0 / len(QLIST)).reshape(cfg.n_om, NB1, -1)
+        sc[j] = chain.selfc_spline(enq[j], mw, fm.omega, fm.omega_precise, fm.womeg_precise)
+    return chi, W, sc
+
+
+def _ene(b):
+    ib = b.n_o - 1
+    return np.ascontiguousarray(b.enk[ib:ib + NB1][:, KS].T).reshape(-1)
+
+
+def _oracle_qp(sig, om, ene):
+    return np.array([chain.analytic_continuation(om, s, e)[0] for s, e in zip(sig, ene)])
+
+
+def test_qloop_fp64_end_to_end_vs_oracle():
+    import torch
+    cfg, b, fm, tetc, qs = _system()
+    L = _loop(cfg, b, fm, tetc, "fp64")
+    sig_o = 0
+    for q, (kq, M, Ms, enq) in zip(QLIST, qs):
+        print(f"case qloop fp64 q={q}", flush=True)
+        L.add_q(kq, M, Ms, enq, keep_chi0=True)
+        kn = L.kcw.cpu().numpy()
+        # the weights against the oracle's, within their one-ulp atan/log spread
+        ko, spread, _ = tetra.with_spread(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, fm.omega, tetc)
+        kn_f = np.stack([kn[k].T.reshape(b.n_o, b.n_e, cfg.n_om) for k in range(cfg.nk)], axis=2)
+        assert tetra.within_spread(kn_f, ko, spread) <= 1.0
+        chi, W, sc = _oracle_q(b, fm, cfg, kn, M, Ms, enq)
+        got_chi, got_w, got_sc = L.chi.cpu().numpy(), L.eps.cpu().numpy(), L.sc.cpu().numpy()
+        assert np.linalg.norm(got_chi - chi) / np.linalg.norm(chi) < 1e-13
+        one = np.eye(cfg.pb)
+        for w in range(cfg.n_om):
+            a = (chi[:, :, w] + chi[:, :, w].conj().T) / 2 + one
+            err = np.linalg.norm(got_w[:, :, w] - W[:, :, w]) / np.linalg.norm(W[:, :, w] + one)
+            assert err < 50 * np.finfo(float).eps * np.linalg.cond(a)
+        assert np.abs(got_sc - sc).max() < 1e-11 * np.abs(sc).max()
+        sig_o = sig_o + sc
+    sig = L.sigc.cpu().numpy()
+    rs = np.abs(sig - sig_o).max() / np.abs(sig_o).max()
+    print(f"  Sigma_c summed over {len(QLIST)} q: max rel {rs:.2e}", flush=True)
+    assert rs < 1e-11 and L.n_added == len(QLIST)
+    # quasiparticles: GPU Pade of the GPU sum against the oracle's Pade of the
+    # oracle's sum, within 10x what noise of the size of their difference moves
+    # the continued fraction by (per state)
+    ene = _ene(b)
+    s_o = sig_o.reshape(-1, cfg.n_om)
+    qp = L.quasiparticles(ene).cpu().numpy()
+    qp_o = _oracle_qp(s_o, fm.omega, ene)
+    rng = np.random.default_rng(9)
+    eta = max(rs, 1e-14)
+    sens = np.zeros(ene.size)
+    for _ in range(4):
+        z = s_o * eta * (rng.normal(size=s_o.shape) + 1j * rng.normal(size=s_o.shape))
+        sens = np.maximum(sens, np.abs(_oracle_qp(s_o + z, fm.omega, ene) - qp_o))
+    # E_qp = e + Z Re S(e): the real part of S(e) moves it (sigx = vxc = 0)
+    sig_e = qp[:, 0] + 1j * qp[:, 1]
+    d = np.abs(sig_e - qp_o)
+    print(f"  Pade S(e): max diff {d.max():.2e} Ha, max sensitivity {sens.max():.2e} Ha", flush=True)
+    assert np.all(d <= 10 * sens + 1e-10 * np.abs(qp_o).max())
+    torch.cuda.synchronize()
+
+
+def test_qloop_split_precision_vs_oracle():
+    import torch
+    cfg, b, fm, tetc, qs = _system()
+    L = _loop(cfg, b, fm, tetc, "mixed")
+    sig_o = 0
+    for q, (kq, M, Ms, enq) in zip(QLIST, qs):
+        print(f"case qloop mixed q={q}", flush=True)
+        m32, ms32 = M.astype(np.complex64), Ms.astype(np.complex64)
+        L.add_q(kq, torch.as_tensor(m32, device="cuda"), torch.as_tensor(ms32, device="cuda"), enq, keep_chi0=True)
+        kn = L.kcw.cpu().numpy()
+        chi, W, sc = _oracle_q(b, fm, cfg, kn, m32.astype(complex), ms32.astype(complex), enq)
+        got_chi = L.chi.cpu().numpy()
+        rc = max(np.linalg.norm(got_chi[:, :, w] - chi[:, :, w]) / np.linalg.norm(chi[:, :, w])
+                 for w in range(cfg.n_om))
+        one = np.eye(cfg.pb)[:, :, None]
+        gw = L.eps.cpu().numpy() + one
+        ri = max(np.linalg.norm(gw[:, :, w] - (W + one)[:, :, w]) / np.linalg.norm((W + one)[:, :, w])
+                 for w in range(cfg.n_om))
+        print(f"  chi0 {rc:.2e} eps^-1 {ri:.2e}", flush=True)
+        assert rc < 1e-6 and ri < 1e-6
+        sig_o = sig_o + sc
+    sig = L.sigc.cpu().numpy()
+    rs = max(np.linalg.norm(sig[:, :, w] - sig_o[:, :, w]) / np.linalg.norm(sig_o[:, :, w]) for w in range(cfg.n_om))
+    print(f"  Sigma_c summed: max rel per frequency {rs:.2e}", flush=True)
+    assert rs < 1e-5
+    torch.cuda.synchronize()
+
+
+def test_qloop_graph_replay_matches_eager():
+    """One q step captured in a HIP graph and replayed for every q of the
+    list (inputs copied into the loop's buffers between replays, eager work
+    in between): the Sigma_c sum equals the eager loop's bit for bit."""
+    import torch
+    cfg, b, fm, tetc, qs = _system()
+    eager = _loop(cfg, b, fm, tetc, "mixed")
+    for kq, M, Ms, enq in qs:
+        eager.add_q(kq, M.astype(np.complex64), Ms.astype(np.complex64), enq)
+    want = eager.sigc.clone()
+    L = _loop(cfg, b, fm, tetc, "mixed")
+    kq, M, Ms, enq = qs[0]
+    L.set_q(kq, M.astype(np.complex64), Ms.astype(np.complex64), enq)
+    print("case qloop graph", flush=True)
+    g = L.capture()
+    for rep in range(2):
+        L.reset()
+        for kq, M, Ms, enq in qs:
+            L.set_q(kq, M.astype(np.complex64), Ms.astype(np.complex64), enq)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            eager.run_q()              # eager work between the replays
+        torch.cuda.synchronize()
+        assert torch.equal(L.sigc, want), rep
