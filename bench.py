@@ -55,12 +55,15 @@ def parse():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c5")
-    ap.add_argument("--nq", type=int, default=None)
+    ap.add_argument("--nq", type=int, default=None,
+                    help="c2: q-points; c5: the first nq of its q-list (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-fp64", action="store_true",
                     help="c4: time the split-precision batch only (no FP64 path, no precision check; profiling runs)")
     ap.add_argument("--cpu-kslab", type=int, default=2)
+    ap.add_argument("--save-sigma", default=None,
+                    help="c5: write the FP64 and split-precision Sigma_c sums and the Sigma_c energies to this .npz")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain 1 GiB device copy (scale for HBM byte counters)")
     ap.add_argument("--fp64-kfrac", type=float, default=None,
@@ -222,7 +225,7 @@ class ChainWorkload:
     the 8 bands around the gap at the 64 k of the 4x4x4 sub-lattice, summed
     over the 8 q-points of C5_QLIST."""
 
-    def __init__(self):
+    def __init__(self, nq=None):
         from pygw_amd import mesh, synth
 
         self.base = base = synth.CONFIGS["c5"]
@@ -237,7 +240,7 @@ class ChainWorkload:
         self.nbgw = 8
         self.ibgw = b.n_o - self.nbgw // 2                        # 4 highest occupied + 4 lowest empty
         self.nb2 = b.enk.shape[0]
-        self.qlist = C5_QLIST
+        self.qlist = C5_QLIST[:nq] if nq else C5_QLIST
         self.wkq = 1.0 / len(self.qlist)
 
     def loop(self, prec):
@@ -392,7 +395,7 @@ def run_chain(args):
     from pygw_amd import mesh
     from pygw_amd import polarization as P
 
-    wl = ChainWorkload()
+    wl = ChainWorkload(args.nq)
     base, b = wl.base, wl.b
     nql = len(wl.qlist)
     stage_names = ["polarization_weights", "chi0", "eps_inverse", "mwm", "selfc_convolution"]
@@ -423,6 +426,8 @@ def run_chain(args):
               f"eps^-1 {errs['eps_inv'][-1].max():.2e} Sigma_c {errs['sigma_c_q'][-1].max():.2e}",
               file=sys.stderr, flush=True)
     sig64, sigmx = L64.sigc.cpu().numpy(), Lmx.sigc.cpu().numpy()
+    if args.save_sigma:
+        np.savez(args.save_sigma, sig64=sig64, sigmx=sigmx, omega=wl.om, ene=wl.ene())
     L64.chi = Lmx.chi = None
     del L64
     torch.cuda.empty_cache()
