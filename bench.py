@@ -250,22 +250,24 @@ class ChainWorkload:
                                len(self.qlist), len(self.klist), self.nbgw, self.nb2, omega_p=self.omp,
                                womeg_p=self.wop, precision=prec)
 
-    def q_inputs(self, q):
+    def q_inputs(self, q, M=None, Ms=None):
         """Device inputs of one q, complex64 (the split-precision operand;
         the FP64 path takes the same values widened): kqid column, M of the
-        polarization pairs, the self-energy rows and the k-q energies."""
+        polarization pairs, the self-energy rows and the k-q energies.  M /
+        Ms: complex64 buffers to write into (the loop's input buffers), as
+        PyGW's calc_minm produces M for the q just before calc_eps."""
         import torch
 
         from pygw_amd import synth
         cfg = synth.PolConfig(**{**self.base.__dict__, "q": q})
         kq = self.kqid[:, synth.q_index(cfg)]
         b, kl = self.b, self.klist
-        M = synth.device_matrix_elements(cfg, b, kq, complex128=False)
+        M = synth.device_matrix_elements(cfg, b, kq, complex128=False, out=M)
         e2 = np.asfortranarray(np.hstack([b.enk[:, kl], b.enk[:, kq[kl]]]))
         b2 = synth.Bands(e2, b.ncg, b.nomx, b.numin)
         kq2 = np.arange(len(kl), 2 * len(kl), dtype=np.int32)
         kq2 = np.concatenate([kq2, kq2])
-        Ms = synth.device_matrix_elements(cfg, b2, kq2, nkc=len(kl), complex128=False,
+        Ms = synth.device_matrix_elements(cfg, b2, kq2, nkc=len(kl), complex128=False, out=Ms,
                                           rows=(self.ibgw, self.nbgw, 0, self.nb2), seed=cfg.seed + 99)
         enk_q = torch.as_tensor(np.ascontiguousarray(b.enk[:, kq[kl]].T), device="cuda")
         return kq, M, Ms, enk_q
@@ -399,15 +401,15 @@ def run_chain(args):
     base, b = wl.base, wl.b
     nql = len(wl.qlist)
     stage_names = ["polarization_weights", "chi0", "eps_inverse", "mwm", "selfc_convolution"]
-    t0 = time.perf_counter()
-    inputs = [wl.q_inputs(q) for q in wl.qlist]         # complex64 M of all q, resident (~75 GB)
-    torch.cuda.synchronize()
-    print(f"[bench] inputs of {nql} q generated ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     L64, Lmx = wl.loop("fp64"), wl.loop("mixed")
+    # the mixed loop's input buffers take each q's M as it is produced
+    Mbuf = torch.empty((base.nk, b.n_o * b.n_e, base.pb), dtype=torch.complex64, device="cuda")
+    Msbuf = torch.empty((len(wl.klist), wl.nbgw * wl.nb2, base.pb), dtype=torch.complex64, device="cuda")
     one = torch.eye(base.pb, dtype=torch.complex128, device="cuda")[:, :, None]
     ev = lambda: [torch.cuda.Event(enable_timing=True) for _ in range(6)]  # noqa: E731
     t64, errs = [], {"chi0": [], "eps_inv": [], "sigma_c_q": []}
-    for iq, (q, (kq, M, Ms, eq)) in enumerate(zip(wl.qlist, inputs)):
+    for iq, q in enumerate(wl.qlist):
+        kq, M, Ms, eq = wl.q_inputs(q, Mbuf, Msbuf)
         L64.set_q(kq, M, Ms, eq)                        # the complex64 values, widened
         Lmx.set_q(kq, enk_kq=eq)
         torch.cuda.synchronize()
@@ -449,7 +451,7 @@ def run_chain(args):
     s = torch.cuda.current_stream()
     times, walls = [], []
     for it in range(args.warmup + args.steps):
-        kq, M, Ms, eq = inputs[it % nql]
+        kq, M, Ms, eq = wl.q_inputs(wl.qlist[it % nql], Mbuf, Msbuf)   # not timed: the q's M is produced
         Lmx.set_q(kq, enk_kq=eq)
         torch.cuda.synchronize()
         e = ev()
@@ -853,7 +855,7 @@ def run_gamma(args):
     from pygw_amd import polarization as P
     wl = ChainWorkload()
     b, base = wl.b, wl.base
-    kq, M, Ms, enk_q = wl.q_inputs((0, 0, 0), "mixed")
+    kq, M, Ms, enk_q = wl.q_inputs((0, 0, 0))
     nk, npair = M.shape[0], M.shape[1]
     rng = np.random.default_rng(BASE_SEED_GW0 + 1)
     # synthetic momentum matrix elements / transition energies, and head
