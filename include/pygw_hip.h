@@ -11,7 +11,8 @@
  *       -10 invalid argument, -11 HIP runtime error,
  *       -12 geometry/classification error (where PyGW's Fortran would `stop`),
  *       -13 matrix not Hermitian positive definite,
- *       -14 eigensolver not converged;
+ *       -14 eigensolver not converged,
+ *       -15 sizes beyond the fixed scratch array of pygw_eps_inverse_noalloc;
  *   - where an `info` pointer is given (host memory), the call synchronises
  *     `stream` and fills it; pass NULL to stay asynchronous;
  *   - `workspace` may be NULL (the library then allocates on `stream`);
@@ -144,6 +145,19 @@ int pygw_chi0_batch_free(void* plan);
  * ------------------------------------------------------------------------ */
 int pygw_eps_inverse_workspace(int pb, int n_om, size_t* bytes);
 int pygw_eps_inverse(void* eps, int pb, int n_om, void* workspace, size_t ws_bytes, int* info, void* stream);
+
+/* pygw_eps_inverse_noalloc: pygw_eps_inverse with no allocation at all.  No
+ * counterpart in PyGW: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * allocation).  The kernels of pygw_eps_inverse run on a fixed device array
+ * of PYGW_NOALLOC_SCRATCH_BYTES that the library holds on every device it
+ * runs on until the process ends; the call needs pygw_eps_inverse_workspace
+ * (pb, n_om) <= that size (about PB 900 with 68 frequencies, PB 1200 with 51)
+ * and returns -15 otherwise, before any launch.  Calls on one device share
+ * its array: calls that overlap on one device (two streams or threads) race.
+ * info as for pygw_eps_inverse (NULL keeps the call asynchronous). */
+#define PYGW_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int pygw_eps_inverse_noalloc(void* eps, int pb, int n_om, int* info, void* stream);
 
 /* ------------------------------------------------------------------------
  * MWM contraction (calc_selfc, pygw.py:3353-3358):

@@ -13,7 +13,8 @@ LIB_PATH = os.path.join(_HERE, "libpygw_hip.so")
 _lib = None
 
 ERRORS = {-10: "invalid argument", -11: "HIP runtime error", -12: "geometry/classification error",
-          -13: "matrix not Hermitian positive definite", -14: "eigensolver did not converge"}
+          -13: "matrix not Hermitian positive definite", -14: "eigensolver did not converge",
+          -15: "sizes exceed the fixed scratch array of the no-allocation entry point"}
 
 
 class PygwError(RuntimeError):
@@ -45,6 +46,7 @@ _SIGS = {
     "pygw_chi0_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_void_p, C.c_void_p]),
     "pygw_chi0_batch_free": (C.c_int, [C.c_void_p]),
     "pygw_eps_inverse_workspace": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
+    "pygw_eps_inverse_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "pygw_eps_inverse": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p,
                                    C.c_void_p]),
     "pygw_mwm_workspace": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)]),

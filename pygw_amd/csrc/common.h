@@ -9,6 +9,7 @@
 #define PYGW_E_GEOM -12
 #define PYGW_E_NOTPD -13
 #define PYGW_E_NOCONV -14
+#define PYGW_E_SCRATCH -15
 
 #define HIP_TRY(x)                                  \
   do {                                              \
