@@ -10,6 +10,8 @@ contractions and the self-energy tail of PyGW's polarization path.
   calc_mwm_head        q = 0 head/wing terms of MWM      (pygw.py:3310-3311,3359-3364)
   fr_convolution2 / all_fr_convolutions / few_fr_convolutions
                                                          (fnc.f90:86-176)
+  spline_precise       the (w^2+1) cubic spline onto the precise mesh
+                                                         (mcommon.py:126-140)
   selfc_spline         spline branch of Compute_selfc_inside
                                                          (mcommon.py:126-144)
   padecof / padeg      Thiele continued fraction         (for_pade.f90:3-51)
@@ -171,32 +173,40 @@ def all_fr_convolutions_vec(enk, mwm, omega, womeg):
     return res.sum(axis=2).T
 
 
+def few_fr_convolutions(enk, mwm_p, mwm_iom, omega_p, omega_iom, womeg_p):
+    """sum_ie2 fr_convolution2(enk[ie2], mwm_p[ie2, :], mwm_iom[ie2], ...)
+    (fnc.f90:161-176).  mwm_p: [nb2, n_om_precise]."""
+    s = 0.0
+    for ie2 in range(len(enk)):
+        s = s + fr_convolution2(enk[ie2], mwm_p[ie2], mwm_iom[ie2], omega_p, omega_iom, womeg_p)
+    return s
+
+
+def spline_precise(mwm_col, omega, omega_p):
+    """mwm of one (ie1, ie2) on the precise mesh: cubic spline of (w^2+1)*mwm
+    with bc ((2,0),(1,0)), extrapolated, divided by (w_p^2+1)
+    (mcommon.py:126-140; the imaginary part only when its mean |.| > 1e-10)."""
+    from scipy import interpolate
+
+    nom = omega.size
+    fa, fp = omega ** 2 + 1.0, omega_p ** 2 + 1.0
+    sr = interpolate.CubicSpline(omega, fa * mwm_col.real, bc_type=((2, 0.0), (1, 0.0)), extrapolate=True)
+    if np.sum(np.abs(mwm_col.imag)) / nom > 1e-10:
+        si = interpolate.CubicSpline(omega, fa * mwm_col.imag, bc_type=((2, 0.0), (1, 0.0)), extrapolate=True)
+        return (sr(omega_p) + 1j * si(omega_p)) / fp
+    return sr(omega_p) / fp + 0j
+
+
 def selfc_spline(enk, mwm, omega, omega_p, womeg_p):
     """Spline branch of Compute_selfc_inside (mcommon.py:126-144): cubic
     spline of (w^2+1)*mwm with bc ((2,0),(1,0)) onto the precise mesh, then
     few_fr_convolutions against the coarse-mesh value mwm(w_n)."""
-    from scipy import interpolate
-
     nom, nb1, nb2 = mwm.shape
     sc = np.zeros((nb1, nom), dtype=complex)
-    fa = omega ** 2 + 1.0
-    fp = omega_p ** 2 + 1.0
     for ie1 in range(nb1):
-        mp = np.zeros((nb2, omega_p.size), dtype=complex)
-        for ie2 in range(nb2):
-            sr = interpolate.CubicSpline(omega, fa * mwm[:, ie1, ie2].real, bc_type=((2, 0.0), (1, 0.0)),
-                                         extrapolate=True)
-            if np.sum(np.abs(mwm[:, ie1, ie2].imag)) / nom > 1e-10:
-                si = interpolate.CubicSpline(omega, fa * mwm[:, ie1, ie2].imag, bc_type=((2, 0.0), (1, 0.0)),
-                                             extrapolate=True)
-                mp[ie2] = (sr(omega_p) + 1j * si(omega_p)) / fp
-            else:
-                mp[ie2] = sr(omega_p) / fp
+        mp = np.array([spline_precise(mwm[:, ie1, ie2], omega, omega_p) for ie2 in range(nb2)])
         for iom in range(nom):
-            s = 0.0
-            for ie2 in range(nb2):
-                s = s + fr_convolution2(enk[ie2], mp[ie2], mwm[iom, ie1, ie2], omega_p, omega[iom], womeg_p)
-            sc[ie1, iom] = s
+            sc[ie1, iom] = few_fr_convolutions(enk, mp, mwm[iom, ie1, :], omega_p, omega[iom], womeg_p)
     return sc
 
 

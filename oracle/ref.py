@@ -211,3 +211,23 @@ def mixed_coulomb(vq, iat, q_vq, jlam, gindex, indgq, gqlen, gqlen0, G_unique, n
       _i(sloctmatsize), _p(bl), _p(k2), _p(rl), _p(tr), _p(mu), _p(ps), _d(vol), _i(jl.shape[0]), _i(rl.shape[0]),
       _i(gx.shape[0]), _i(jl.shape[1]), _i(len(gu)), _i(ps.shape[1]), _i(len(iq)))
     return np.ascontiguousarray(out)
+
+
+def k_m_q_id(div):
+    """kqid[ik, iq] of k_m_q_id (for_kpts.f90:21-45), 0-based, int32 [N_k, N_k]."""
+    lib = _load("libref_kpts.so")
+    d = np.ascontiguousarray(div, dtype=np.int32)
+    nkp = int(np.prod(d))
+    out = np.zeros((nkp, nkp), dtype=np.int32, order="F")
+    lib.k_m_q_id_(C.c_void_p(out.ctypes.data), C.c_void_p(d.ctypes.data), _i(nkp))
+    return out
+
+
+def k2indx(k, div):
+    """k2indx (for_kpts.f90:2-7) of one integer k-point."""
+    lib = _load("libref_kpts.so")
+    kk = np.ascontiguousarray(k, dtype=np.int32)
+    d = np.ascontiguousarray(div, dtype=np.int32)
+    out = C.c_int(0)
+    lib.k2indx_(C.byref(out), C.c_void_p(kk.ctypes.data), C.c_void_p(d.ctypes.data))
+    return out.value

@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "_build", "liboracle.so")
+# ORACLE_LIB selects another build of tetra_oracle.c (the sanitizer build of
+# `make -C oracle sanitize`, run on the host only)
+_LIB = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
 ORC_F_F7A = 1

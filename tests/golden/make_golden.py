@@ -64,7 +64,27 @@ def main():
     mwm = (rng.normal(size=(nom, nb1, nb2)) + 1j * rng.normal(size=(nom, nb1, nb2))) / (1 + om[:, None, None] ** 2)
     sc = ref.all_fr_convolutions(enk, mwm, om, wo)
     np.savez_compressed(os.path.join(HERE, "conv.npz"), enk=enk, mwm=mwm, omega=om, womeg=wo, sc=sc)
-    print("pade, conv written")
+    # the spline branch of Compute_selfc_inside (mcommon.py:126-144): mwm on
+    # a 12-point tan mesh splined (scipy, as PyGW) onto the x5 precise mesh,
+    # then PyGW's compiled few_fr_convolutions (fnc.f90:161-176) per (ie1, iom)
+    from oracle import chain
+    fm = mesh.FrequencyMesh(12, 0.02, 15.0, iop_multiple=5)
+    nom, nb1, nb2 = fm.omega.size, 2, 6
+    a = rng.uniform(0.2, 1.4, (nb1, nb2))
+    mwm = (1.0 / (fm.omega[:, None, None] ** 2 + a[None] ** 2)) * (1 + 0.2j * rng.normal(size=(1, nb1, nb2)))
+    enk = rng.uniform(-0.6, 0.6, nb2)
+    enk[0] = 0.0                           # E = 0: atan(Omega_max/(-i w)) on its branch cut
+    mwmp = np.zeros((nb1, nb2, fm.omega_precise.size), dtype=complex)
+    sc = np.zeros((nb1, nom), dtype=complex)
+    for ie1 in range(nb1):
+        for ie2 in range(nb2):
+            mwmp[ie1, ie2] = chain.spline_precise(mwm[:, ie1, ie2], fm.omega, fm.omega_precise)
+        for iom in range(nom):
+            sc[ie1, iom] = ref.few_fr_convolutions(enk, mwmp[ie1], mwm[iom, ie1, :], fm.omega_precise, fm.omega[iom],
+                                                   fm.womeg_precise)
+    np.savez_compressed(os.path.join(HERE, "conv_spline.npz"), enk=enk, mwm=mwm, omega=fm.omega, womeg=fm.womeg,
+                        omega_p=fm.omega_precise, womeg_p=fm.womeg_precise, mwm_p=mwmp, sc=sc)
+    print("pade, conv, conv_spline written")
 
 
 if __name__ == "__main__":

@@ -133,3 +133,19 @@ def test_mwm_mixed_slabs_and_ragged(shape):
         want = chain.calc_mwm(M[k].astype(complex), W, 0.25)
         for w in range(nom):
             assert _rel(full[k, w], want[w]) < 1e-6
+
+
+def test_spline_convolution_vs_reference_fixture():
+    """The spline branch against PyGW's compiled few_fr_convolutions
+    (tests/golden/conv_spline.npz).  At E = 0 (band 0) the atan argument
+    lies on its branch cut; the kernel divides as C does and takes the
+    -pi/2 side, the amdflang build of PyGW the +pi/2 side: there the two
+    differ by exactly mwm(w_n) (checked on the oracle in test_oracle_pins)."""
+    from pygw_amd import polarization as P
+    g = np.load(os.path.join(GOLD, "conv_spline.npz"))
+    print("case spline conv fixture", flush=True)
+    got = P.compute_selfc_inside(g["enk"][None, :], g["mwm"][None], g["omega"], g["womeg"], g["omega_p"],
+                                 g["womeg_p"]).cpu().numpy()[0]
+    cut = (g["omega_p"][-1] / g["omega"] > 1.0 + 1e-9)[None, :]
+    want = g["sc"] - np.where(cut, g["mwm"][:, :, 0].T, 0.0)
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-13)
