@@ -331,9 +331,26 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
         }
   };
   bool first = true;
+  // Per merged row of the item, in LDS once per item: the eps offset
+  // (pb w + J) pb (-1 past the end) and the unscale 1/(s_J t_w).  Written by
+  // MFMA waves 0-1 before the first barrier of the chunk loop and read in the
+  // folds after it.  A fold loads the eps values of 8 accumulator elements
+  // at once and then adds and stores them, so the wave waits on one batch of
+  // loads instead of on every element.
+  __shared__ long rowoff[TB];
+  __shared__ double rowf[TB];
+  if (tid < TB) {
+    const int m = Nc * TB + tid;
+    if (m < mcount) {
+      const int J = I0 + m / nom, w = m % nom;
+      rowoff[tid] = ((long)pb * w + J) * (long)pb;
+      rowf[tid] = (double)ics[J] / ws[w];
+    } else {
+      rowoff[tid] = -1;
+      rowf[tid] = 0.0;
+    }
+  }
   auto flush = [&](auto sk) {
-    // opaque copies keep the compiler from hoisting the element addresses
-    // (and their scales) out of the chunk loop into live registers
     int Ib = I0 + 64 * wc + (lane & 31), rb = 64 * wr + 4 * (lane >> 5);
     asm volatile("" : "+v"(Ib), "+v"(rb));
 #pragma unroll
@@ -345,26 +362,26 @@ __device__ __forceinline__ void chi0_cols(const h8* __restrict__ S, const float2
       for (int a = 0; a < 2; a++) {
         if (decltype(sk)::value && skip[a][i]) continue;
 #pragma unroll
-        for (int v = 0; v < 16; v++) {
-          const int row = rb + 32 * a + 8 * (v >> 2) + (v & 3);
-          const int m = Nc * TB + row;
-          if (m < mcount) {
-            const int J = I0 + m / nom, w = m % nom;
-            const double f = fi * (double)ics[J] / ws[w];  // exact: powers of two
-            const double re = (double)accR[a][i][v] * f, im = (double)accI[a][i][v] * f;
-            if (part) {
-              double2* e = part + (size_t)row * TB + (I - I0);
-              if (first) {
-                *e = double2{re, im};
-              } else {
-                const double2 o = *e;
-                *e = double2{o.x + re, o.y + im};
-              }
-            } else {
-              double2* e = eps + ((size_t)pb * w + J) * (size_t)pb + I;
-              const double2 o = *e;
-              *e = double2{o.x + re, o.y + im};
-            }
+        for (int h = 0; h < 2; h++) {
+          double2 o[8];
+          long off[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int v = 8 * h + u;
+            const int row = rb + 32 * a + 8 * (v >> 2) + (v & 3);
+            off[u] = rowoff[row] < 0 ? -1 : part ? (long)row * TB + (I - I0) : rowoff[row] + I;
+          }
+          double2* base = part ? part : eps;
+#pragma unroll
+          for (int u = 0; u < 8; u++)
+            o[u] = (off[u] >= 0 && !(part && first)) ? base[off[u]] : double2{0.0, 0.0};
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int v = 8 * h + u;
+            const int row = rb + 32 * a + 8 * (v >> 2) + (v & 3);
+            const double f = fi * rowf[row];
+            if (off[u] >= 0)
+              base[off[u]] = double2{o[u].x + (double)accR[a][i][v] * f, o[u].y + (double)accI[a][i][v] * f};
           }
         }
       }
