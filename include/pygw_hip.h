@@ -212,6 +212,19 @@ int pygw_pade_qp(const void* sigc, int n_state, int n_om, const double* omega, i
                  const double* sigx, const double* vxc, double* out, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Pade continuation through arbitrary complex points.
+ * pygw_padecof replaces fpade.padecof(gn, zn) (for_pade.f90:3-25) for
+ *   n_state functions on common nodes: gn complex128 [n_state][n], zn
+ *   complex128 [n] (n <= 64), coef complex128 [n_state][n] (Thiele's
+ *   reciprocal differences p(j, j)).
+ * pygw_padeg replaces fpade.padeg(x, zn, coef) (for_pade.f90:30-51): the
+ *   continued fraction of each function at x complex128 [m]; gx complex128
+ *   [n_state][m].  (mcommon.py:336-338 calls the pair with zn = i omega_n.)
+ * ------------------------------------------------------------------------ */
+int pygw_padecof(const void* zn, const void* gn, int n, int n_state, void* coef, void* stream);
+int pygw_padeg(const void* x, int m, const void* zn, const void* coef, int n, int n_state, void* gx, void* stream);
+
+/* ------------------------------------------------------------------------
  * GW0 on stored mwm (the calceqp iteration, postprocess.py:206-321: Sigma_c
  * of every (q, irk) recomputed with the quasiparticle bands, mixed, then the
  * quasiparticle energies and the Fermi level again).

@@ -65,6 +65,8 @@ _SIGS = {
                                   C.c_void_p, C.c_void_p, C.c_void_p]),
     "pygw_pade_qp": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pygw_padecof": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "pygw_padeg": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "pygw_synth_minm_pairs": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_void_p, C.c_ulonglong, C.c_void_p, C.c_long, C.c_int,
                                         C.c_void_p, C.c_void_p]),

@@ -11,6 +11,9 @@ int selfc_spline_launch(const void* mwm, int nrows, int nb1, int nom, int nb2, c
 int selfc_conv_launch(const double* enk, int nrows, int nb1, int nb2, const void* mwm, int nom, const double* omega,
                       const void* mwmp, int nomp, const double* omp, const double* womp, void* out,
                       hipStream_t stream);
+int padecof_launch(const void* zn, const void* gn, int n, int nstate, void* coef, hipStream_t stream);
+int padeg_launch(const void* x, int m, const void* zn, const void* coef, int n, int nstate, void* gx,
+                 hipStream_t stream);
 int pade_qp_launch(const void* sig, int nstate, int nom, const double* omega, int npts, const double* ene,
                    const double* ene0, const double* sigx, const double* vxc, int iop_es, int iop_gw0, double* out,
                    hipStream_t stream);
@@ -73,6 +76,17 @@ extern "C" int pygw_pade_qp(const void* sigc, int n_state, int n_om, const doubl
                             const double* ene, const double* sigx, const double* vxc, double* out, void* stream) {
   if (!sigc || !omega || !ene || !out || n_state <= 0 || n_om <= 0) return PYGW_E_ARG;
   return pade_qp_launch(sigc, n_state, n_om, omega, n_pade, ene, nullptr, sigx, vxc, 0, 0, out, (hipStream_t)stream);
+}
+
+extern "C" int pygw_padecof(const void* zn, const void* gn, int n, int n_state, void* coef, void* stream) {
+  if (!zn || !gn || !coef || n <= 0 || n > 64 || n_state <= 0) return PYGW_E_ARG;
+  return padecof_launch(zn, gn, n, n_state, coef, (hipStream_t)stream);
+}
+
+extern "C" int pygw_padeg(const void* x, int m, const void* zn, const void* coef, int n, int n_state, void* gx,
+                          void* stream) {
+  if (!x || !zn || !coef || !gx || m <= 0 || n <= 0 || n > 64 || n_state <= 0) return PYGW_E_ARG;
+  return padeg_launch(x, m, zn, coef, n, n_state, gx, (hipStream_t)stream);
 }
 
 extern "C" int pygw_pade_qp_gw0(const void* sigc, int n_state, int n_om, const double* omega, int n_pade,

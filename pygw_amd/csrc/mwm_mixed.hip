@@ -9,13 +9,17 @@
 // low halves stay in FP16's normal range.  A real product takes three MFMA
 // terms (hi hi + hi lo + lo hi);
 //   Re X = m_r W_r + m_i W_i,   Im X = m_r W_i + (-m_i) W_r.
-// The FP32 accumulators of one 128 x 128 tile of X = conj(M) W run over the
-// whole I sum (PB/32 chunks); the epilogue unscales the tile exactly, takes
-// it times M[r,J] in FP64 and adds it into per-row FP64 sums, so the J
-// reduction never leaves the chip.  Each block owns 128 rows and one
-// frequency and sweeps the J tiles; blocks of one frequency sit on one XCD
-// (blocks are dealt to the 8 XCDs round robin), so that frequency's W planes
-// are read from HBM once per XCD and served from its L2.
+// One block forms one 128 x 128 tile of X = conj(M) W_w (128 rows, 128 J)
+// over the whole I sum (PB/32 chunks) in FP32 accumulators; the epilogue
+// unscales the tile exactly, takes it times M[r,J] in FP64 and writes the
+// tile's per-row FP64 sum over its J into a partial-sum slot, and
+// pygw_mwm_sum adds a row's slots in J-tile order (deterministic, no
+// atomics).  Work items (w, row tile, J tile) are grouped so that the 32
+// blocks an XCD runs at once share one frequency and R row tiles x all J
+// tiles (R x T = 32 for T J tiles): each I chunk of a row tile's planes is
+// read from HBM once and used by the T blocks of its J tiles, each chunk of
+// the frequency's W planes once and used by the R blocks of its rows (blocks
+// are dealt to the 8 XCDs round robin and each XCD starts its own in order).
 #include "common.h"
 
 #pragma clang fp contract(off)
@@ -149,23 +153,29 @@ __global__ void __launch_bounds__(64) pygw_mwm_colscale(const double2* __restric
 
 // MFMA 16x16x32 f16 layouts: A[i][k]: i = l%16, k = 8*(l/16) + j;
 // B[k][n]: n = l%16, k = 8*(l/16) + j;  D[i][n]: n = l%16, i = 4*(l/16) + v.
-// out: [N_k][n_om][rows_per_k], row = k*rows_per_k + rr.
+// part: [nw][T][rowp] complex128, the row sums of each (frequency, J tile).
+// Block b runs on XCD b % 8 as its (b / 8)-th block: group g = (b / 8) /
+// (R T) of that XCD is supergroup 8 g + b % 8 = (frequency, row group of R
+// row tiles), and within it item (b / 8) % (R T) = (J tile, row tile).
 __global__ void __launch_bounds__(512) pygw_mwm_mixed(const h8* __restrict__ SA, const h8* __restrict__ SB,
                                                       const float2* __restrict__ M, int nrows, int pb, int rowp,
-                                                      int pbp, int npc, int w0, const float* __restrict__ rs,
-                                                      const float* __restrict__ cs, double wkq, int rpk, int nom,
-                                                      int nrt, int nwork, double2* __restrict__ out) {
+                                                      int pbp, int npc, const float* __restrict__ rs,
+                                                      const float* __restrict__ cs, int w0, int nw, int nrt, int R,
+                                                      double2* __restrict__ part) {
   using namespace mm;
   __shared__ h8 As[2][16][TB];  // [buf][g*4 + plane][row]
   __shared__ h8 Bs[2][16][TB];  // [buf][g*4 + plane][J]
   __shared__ double2 racc[4][TB];
-  const int per = gridDim.x >> 3;
-  const int work = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  if (work >= nwork) return;
-  const int wl = work / nrt, rt = work % nrt;
+  const int T = pbp / TB;
+  const int ngrp = (nrt + R - 1) / R;      // row groups per frequency
+  const int local = blockIdx.x >> 3, per = R * T;
+  const int sg = (local / per) * 8 + (blockIdx.x & 7);
+  const int within = local % per;
+  const int wl = sg / ngrp, rg = sg % ngrp;
+  const int jt = within / R, rt = rg * R + within % R;
+  if (wl >= nw || rt >= nrt) return;       // the whole block leaves before any barrier
   const int w = w0 + wl;
   const int r0 = rt * TB;
-  const int T = pbp / TB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 2, wc = wave & 3;
   const int li = lane & 15, lk = lane >> 4;
@@ -176,7 +186,7 @@ __global__ void __launch_bounds__(512) pygw_mwm_mixed(const h8* __restrict__ SA,
   const unsigned boffB = (unsigned)((tid >> 7) * pbp + (tid & 127)) * 16u;
   const unsigned aqB = 4u * (unsigned)rowp * 16u, bqB = 4u * (unsigned)pbp * 16u;
   h8 pa[4], pbv[4];
-  auto fetch = [&](int jt, int c) {
+  auto fetch = [&](int c) {
     const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(SA + (size_t)c * 16 * rowp + r0), (short)0, (int)(16u * (unsigned)rowp * 16u), RSRC_FLAGS);
     const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
@@ -198,56 +208,20 @@ __global__ void __launch_bounds__(512) pygw_mwm_mixed(const h8* __restrict__ SA,
   };
 
   f4 accR[4][2], accI[4][2];
-  double2 rsum[4][4];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int v = 0; v < 4; v++) rsum[a][v] = double2{0.0, 0.0};
-  auto zero = [&]() {
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-      for (int b = 0; b < 2; b++) {
-        accR[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-        accI[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-      }
-  };
-  // X tile (rows x J) times M[r, J], added into the per-row sums
-  auto epilogue = [&](int jt) {
-    int rb = r0 + 64 * wr + 4 * lk, jb = jt * TB + 32 * wc + li;
-    asm volatile("" : "+v"(rb), "+v"(jb));  // keep the addresses out of the main loop's registers
-#pragma unroll
     for (int b = 0; b < 2; b++) {
-      const int J = jb + 16 * b;
-      if (J >= pb) continue;
-      const double fj = 1.0 / (double)csw[J];
-#pragma unroll
-      for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int r = rb + 16 * a + v;
-          if (r < nrows) {
-            const double f = fj / (double)rs[r];  // exact: powers of two
-            const double xr = (double)accR[a][b][v] * f, xi = (double)accI[a][b][v] * f;
-            const float2 m = M[(size_t)r * pb + J];
-            rsum[a][v].x += xr * (double)m.x - xi * (double)m.y;
-            rsum[a][v].y += xr * (double)m.y + xi * (double)m.x;
-          }
-        }
+      accR[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+      accI[a][b] = f4{0.f, 0.f, 0.f, 0.f};
     }
-  };
-
-  zero();
-  fetch(0, 0);
+  fetch(0);
   stage(0);
   __syncthreads();
-  const long nch = (long)T * npc;
-  int jn = npc > 1 ? 0 : 1, cn = npc > 1 ? 1 : 0;  // chunk after the current one
-  int jt = 0, cc = 0;                              // current chunk
-  for (long ch = 0; ch < nch; ch++) {
-    const int buf = (int)(ch & 1);
-    const bool more = ch + 1 < nch;
-    if (more) fetch(jn, cn);
+  for (int c = 0; c < npc; c++) {
+    const int buf = c & 1;
+    const bool more = c + 1 < npc;
+    if (more) fetch(c + 1);
     h8 brh[2], brl[2], bih[2], bil[2];
 #pragma unroll
     for (int b = 0; b < 2; b++) {
@@ -259,11 +233,11 @@ __global__ void __launch_bounds__(512) pygw_mwm_mixed(const h8* __restrict__ SA,
     }
 #pragma unroll
     for (int a = 0; a < 4; a++) {
-      const int R = 64 * wr + 16 * a + li;
-      const h8 arh = As[buf][4 * lk + 0][R];
-      const h8 arl = As[buf][4 * lk + 1][R];
-      const h8 aih = As[buf][4 * lk + 2][R];
-      const h8 ail = As[buf][4 * lk + 3][R];
+      const int Rw = 64 * wr + 16 * a + li;
+      const h8 arh = As[buf][4 * lk + 0][Rw];
+      const h8 arl = As[buf][4 * lk + 1][Rw];
+      const h8 aih = As[buf][4 * lk + 2][Rw];
+      const h8 ail = As[buf][4 * lk + 3][Rw];
       const h8 naih = neg(aih), nail = neg(ail);
 #pragma unroll
       for (int b = 0; b < 2; b++) {
@@ -284,17 +258,36 @@ __global__ void __launch_bounds__(512) pygw_mwm_mixed(const h8* __restrict__ SA,
         accI[a][b] = m;
       }
     }
-    if (more) {
-      stage(buf ^ 1);
-      if (++cn == npc) { cn = 0; jn++; }
-    }
-    if (++cc == npc) {  // this J tile's I sum is complete
-      epilogue(jt);
-      zero();
-      cc = 0;
-      jt++;
-    }
+    if (more) stage(buf ^ 1);
     __syncthreads();
+  }
+  // the X tile (rows x J) times M[r, J], summed over the tile's J per row
+  double2 rsum[4][4];
+  {
+    const int rb = r0 + 64 * wr + 4 * lk, jb = jt * TB + 32 * wc + li;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) rsum[a][v] = double2{0.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int J = jb + 16 * b;
+      if (J >= pb) continue;
+      const double fj = 1.0 / (double)csw[J];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int r = rb + 16 * a + v;
+          if (r < nrows) {
+            const double f = fj / (double)rs[r];  // exact: powers of two
+            const double xr = (double)accR[a][b][v] * f, xi = (double)accI[a][b][v] * f;
+            const float2 m = M[(size_t)r * pb + J];
+            rsum[a][v].x += xr * (double)m.x - xi * (double)m.y;
+            rsum[a][v].y += xr * (double)m.y + xi * (double)m.x;
+          }
+        }
+    }
   }
   // reduce the row sums over the 16 lanes of a row group, then over the 4 J waves
 #pragma unroll
@@ -311,14 +304,27 @@ __global__ void __launch_bounds__(512) pygw_mwm_mixed(const h8* __restrict__ SA,
     }
   __syncthreads();
   for (int r = tid; r < TB; r += NT) {
-    const int row = r0 + r;
-    if (row < nrows) {
-      double sx = 0.0, sy = 0.0;
+    double sx = 0.0, sy = 0.0;
 #pragma unroll
-      for (int q = 0; q < 4; q++) { sx += racc[q][r].x; sy += racc[q][r].y; }
-      const int k = row / rpk, rr = row % rpk;
-      out[((size_t)k * nom + w) * rpk + rr] = double2{wkq * sx, wkq * sy};
+    for (int q = 0; q < 4; q++) { sx += racc[q][r].x; sy += racc[q][r].y; }
+    part[((size_t)wl * T + jt) * rowp + r0 + r] = double2{sx, sy};
+  }
+}
+
+// out[k][w0 + wl][rr] = wkq * sum over the J tiles (in order) of part[wl][jt][row]
+__global__ void pygw_mwm_sum(const double2* __restrict__ part, int nrows, int rowp, int T, int w0, int nw, double wkq,
+                             int rpk, int nom, double2* __restrict__ out) {
+  const long n = (long)nw * nrows;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const int wl = (int)(e / nrows), row = (int)(e % nrows);
+    const double2* p = part + (size_t)wl * T * rowp + row;
+    double sx = 0.0, sy = 0.0;
+    for (int jt = 0; jt < T; jt++) {
+      sx += p[(size_t)jt * rowp].x;
+      sy += p[(size_t)jt * rowp].y;
     }
+    const int k = row / rpk, rr = row % rpk;
+    out[((size_t)k * nom + w0 + wl) * rpk + rr] = double2{wkq * sx, wkq * sy};
   }
 }
 
@@ -334,16 +340,22 @@ size_t w_planes_bytes_per_w(int pb) {
   const size_t pbp = (size_t)(pb + mm::TB - 1) / mm::TB * mm::TB;
   return npc * 16 * pbp * sizeof(h8);
 }
+// per frequency: its W planes and the row sums of its J tiles
+size_t per_w_bytes(int nrows, int pb) {
+  const size_t T = (size_t)(pb + mm::TB - 1) / mm::TB;
+  const size_t rowp = (size_t)(nrows + mm::TB - 1) / mm::TB * mm::TB;
+  return w_planes_bytes_per_w(pb) + align256(T * rowp * sizeof(double2));
+}
 size_t head_bytes(int nrows, int pb, int nom) {
   return align256(sizeof(float) * (size_t)nrows) + align256(sizeof(float) * (size_t)pb * nom);
 }
-constexpr size_t kWBudget = (size_t)1 << 30;  // W planes per slab of frequencies, bytes
+constexpr size_t kWBudget = (size_t)2 << 30;  // W planes and row sums per slab of frequencies, bytes
 }  // namespace
 
-// Workspace: scales, the row planes and the W planes of a slab of
-// min(n_om, 1 GiB / per-frequency bytes) frequencies.
+// Workspace: scales, the row planes, and the W planes and row sums of a slab
+// of min(n_om, 2 GiB / per-frequency bytes) frequencies.
 size_t mwm_mixed_ws_bytes(int nrows, int pb, int nom) {
-  const size_t per_w = w_planes_bytes_per_w(pb);
+  const size_t per_w = per_w_bytes(nrows, pb);
   size_t ws = kWBudget / per_w;
   if (ws < 1) ws = 1;
   if (ws > (size_t)nom) ws = (size_t)nom;
@@ -354,7 +366,7 @@ int mwm_mixed_launch(const void* M, int nrows, int pb, const void* W, int nom, d
                      size_t ws_bytes, void* out, hipStream_t stream) {
   using namespace mm;
   const size_t head = head_bytes(nrows, pb, nom), rbytes = rows_planes_bytes(nrows, pb);
-  const size_t per_w = w_planes_bytes_per_w(pb);
+  const size_t per_w = per_w_bytes(nrows, pb), wpl = w_planes_bytes_per_w(pb);
   if (ws_bytes < head + rbytes + per_w) return PYGW_E_ARG;
   size_t wslab = (ws_bytes - head - rbytes) / per_w;
   if (wslab > (size_t)nom) wslab = (size_t)nom;
@@ -362,10 +374,16 @@ int mwm_mixed_launch(const void* M, int nrows, int pb, const void* W, int nom, d
   float* cs = (float*)((char*)ws + align256(sizeof(float) * (size_t)nrows));
   h8* SA = (h8*)((char*)ws + head);
   h8* SB = (h8*)((char*)ws + head + rbytes);
+  double2* part = (double2*)((char*)ws + head + rbytes + wslab * wpl);  // wpl: a multiple of 256 B
   const int npc = ceil_div(pb, KC);
   const int rowp = ceil_div(nrows, TB) * TB;
   const int pbp = ceil_div(pb, TB) * TB;
-  const int nrt = rowp / TB;
+  const int nrt = rowp / TB, T = pbp / TB;
+  // row tiles per XCD group: R x T blocks of one frequency run together
+  int R = 32 / T;
+  if (R < 1) R = 1;
+  if (R > nrt) R = nrt;
+  const int ngrp = ceil_div(nrt, R);
   hipLaunchKernelGGL(pygw_mwm_rowscale, dim3(nrows), dim3(256), 0, stream, (const float2*)M, nrows, pb, rs);
   hipLaunchKernelGGL(pygw_mwm_colscale, dim3(pb, nom), dim3(64), 0, stream, (const double2*)W, pb, cs);
   {
@@ -382,11 +400,17 @@ int mwm_mixed_launch(const void* M, int nrows, int pb, const void* W, int nom, d
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(pygw_mwm_split_w, dim3(blocks), dim3(256), 0, stream, (const double2*)W, pb, pbp, npc, w0, nw,
                        (const float*)cs, SB);
-    const int nwork = nw * nrt;
-    const int grid = ceil_div(nwork, 8) * 8;
-    hipLaunchKernelGGL(pygw_mwm_mixed, dim3(grid), dim3(NT), 0, stream, (const h8*)SA, (const h8*)SB,
-                       (const float2*)M, nrows, pb, rowp, pbp, npc, w0, (const float*)rs, (const float*)cs, wkq, rpk,
-                       nom, nrt, nwork, (double2*)out);
+    const long nsg = (long)nw * ngrp;             // supergroups (frequency, row group)
+    const long grid = ((nsg + 7) / 8) * 8 * R * T;
+    if (grid > 0x7fffffffL) return PYGW_E_ARG;
+    hipLaunchKernelGGL(pygw_mwm_mixed, dim3((unsigned)grid), dim3(NT), 0, stream, (const h8*)SA, (const h8*)SB,
+                       (const float2*)M, nrows, pb, rowp, pbp, npc, (const float*)rs, (const float*)cs, w0, nw, nrt, R,
+                       part);
+    const long ne = (long)nw * nrows;
+    int sblocks = (int)((ne + 255) / 256);
+    if (sblocks > 65536) sblocks = 65536;
+    hipLaunchKernelGGL(pygw_mwm_sum, dim3(sblocks), dim3(256), 0, stream, (const double2*)part, nrows, rowp, T, w0, nw,
+                       wkq, rpk, nom, (double2*)out);
   }
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }

@@ -262,6 +262,46 @@ __global__ void pygw_pade_qp(const double2* __restrict__ sig, int nstate, int no
   o[5] = e + delta;
 }
 
+// Thiele reciprocal differences (Padecof, for_pade.f90:3-25) of n_state
+// functions on common complex nodes zn[n]: coef[s][j] = p(j, j) with
+//   p(1, j) = g(j),  p(i, j) = (p(i-1, i-1) - p(i-1, j)) / (zn(j) - zn(i-1)) / p(i-1, j),
+// row i computed in place.  One thread per function; n <= 64.
+__global__ void pygw_padecof(const double2* __restrict__ zn, const double2* __restrict__ gn, int n, int nstate,
+                             double2* __restrict__ coef) {
+  using namespace sc;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nstate) return;
+  double2 g[64];
+  for (int j = 0; j < n; j++) g[j] = gn[(size_t)s * n + j];
+  for (int i = 1; i < n; i++) {
+    const double2 pd = g[i - 1];
+    for (int j = i; j < n; j++) g[j] = cdiv(cdiv(csub(pd, g[j]), csub(zn[j], zn[i - 1])), g[j]);
+  }
+  for (int j = 0; j < n; j++) coef[(size_t)s * n + j] = g[j];
+}
+
+// The continued fraction at x[m] (PadeG, for_pade.f90:30-51):
+//   a(0) = 0, a(1) = P(1), b(0) = b(1) = 1,
+//   a(i+1) = a(i) + (x - zn(i)) P(i+1) a(i-1)  (b likewise),  G(x) = a(n)/b(n).
+// One thread per (function, point).
+__global__ void pygw_padeg(const double2* __restrict__ x, int m, const double2* __restrict__ zn,
+                           const double2* __restrict__ coef, int n, int nstate, double2* __restrict__ gx) {
+  using namespace sc;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nstate * m) return;
+  const int s = (int)(t / m), j = (int)(t % m);
+  const double2* P = coef + (size_t)s * n;
+  const double2 xv = x[j];
+  double2 a0 = {0.0, 0.0}, a1 = P[0], b0 = {1.0, 0.0}, b1 = {1.0, 0.0};
+  for (int i = 1; i < n; i++) {
+    const double2 f = cmul(csub(xv, zn[i - 1]), P[i]);
+    const double2 a2 = cadd(a1, cmul(f, a0));
+    const double2 b2 = cadd(b1, cmul(f, b0));
+    a0 = a1; a1 = a2; b0 = b1; b1 = b2;
+  }
+  gx[t] = cdiv(a1, b1);
+}
+
 // out = beta * out + alpha * sum_q parts[q] (q in order): the sum over q of
 // the per-q Sigma_c contributions of Compute_selfc (postprocess.py:176-186)
 // and the self-energy mixing of the GW0 loop (postprocess.py:297).
@@ -306,6 +346,21 @@ int selfc_conv_launch(const double* enk, int nrows, int nb1, int nb2, const void
   dim3 grid(nrows, ceil_div(nom, sc::CG));
   hipLaunchKernelGGL(pygw_selfc_conv, grid, dim3(sc::CT), 0, stream, enk, nb1, nb2, (const double2*)mwm, nom, omega,
                      (const double2*)mwmp, nomp, omp, womp, (double2*)out);
+  return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+}
+
+int padecof_launch(const void* zn, const void* gn, int n, int nstate, void* coef, hipStream_t stream) {
+  if (n < 1 || n > 64 || nstate < 1) return PYGW_E_ARG;
+  hipLaunchKernelGGL(pygw_padecof, dim3(ceil_div(nstate, 64)), dim3(64), 0, stream, (const double2*)zn,
+                     (const double2*)gn, n, nstate, (double2*)coef);
+  return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+}
+
+int padeg_launch(const void* x, int m, const void* zn, const void* coef, int n, int nstate, void* gx,
+                 hipStream_t stream) {
+  if (n < 1 || m < 1 || nstate < 1) return PYGW_E_ARG;
+  hipLaunchKernelGGL(pygw_padeg, dim3(ceil_div((long)nstate * m, 256)), dim3(256), 0, stream, (const double2*)x, m,
+                     (const double2*)zn, (const double2*)coef, n, nstate, (double2*)gx);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }
 

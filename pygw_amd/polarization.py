@@ -133,7 +133,10 @@ def eps_inverse(eps, *, device="cuda", check=True):
     if pb != pb2 or eps.stride() != (1, pb, pb * pb) or eps.dtype != torch.complex128:
         raise ValueError("eps must be a Fortran-strided [PB, PB, n_om] complex128 tensor")
     info = (C.c_int * 1)()
-    _lib.call("pygw_eps_inverse", C.c_void_p(eps.data_ptr()), pb, nom, None, 0,
+    sz = C.c_size_t(0)
+    _lib.call("pygw_eps_inverse_workspace", pb, nom, C.byref(sz))
+    ws = torch.empty(sz.value, dtype=torch.uint8, device=device)   # from torch's caching allocator
+    _lib.call("pygw_eps_inverse", C.c_void_p(eps.data_ptr()), pb, nom, C.c_void_p(ws.data_ptr()), sz.value,
               C.cast(info, C.c_void_p) if check else None, _stream_ptr(device))
     return eps
 
@@ -307,6 +310,43 @@ def pade_quasiparticles(sigc, omega, ene, sigx=None, vxc=None, npade=32, *, devi
               C.c_void_p(en.data_ptr()), C.c_void_p(sx.data_ptr()) if sx is not None else None,
               C.c_void_p(vx.data_ptr()) if vx is not None else None, C.c_void_p(out.data_ptr()), _stream_ptr(device))
     return out
+
+
+def padecof(gn, zn, *, device="cuda"):
+    """Thiele reciprocal differences (fpade.padecof, for_pade.f90:3-25).
+    gn complex [n] or [n_state, n] on the nodes zn [n] (n <= 64); returns
+    complex128 coefficients of the same shape."""
+    device = _require_device(device)
+    g = _dev(gn, torch.complex128, device)
+    one = g.dim() == 1
+    g = g.reshape(1, -1).contiguous() if one else g.contiguous()
+    z = _dev(zn, torch.complex128, device).contiguous()
+    ns, n = g.shape
+    if z.numel() != n:
+        raise ValueError("gn and zn must have the same number of points")
+    coef = torch.empty_like(g)
+    _lib.call("pygw_padecof", C.c_void_p(z.data_ptr()), C.c_void_p(g.data_ptr()), n, ns, C.c_void_p(coef.data_ptr()),
+              _stream_ptr(device))
+    return coef[0] if one else coef
+
+
+def padeg(x, zn, coef, *, device="cuda"):
+    """The continued fraction of padecof's coefficients at the complex points
+    x [m] (fpade.padeg, for_pade.f90:30-51); coef [n] or [n_state, n].
+    Returns complex128 [m] or [n_state, m]."""
+    device = _require_device(device)
+    c = _dev(coef, torch.complex128, device)
+    one = c.dim() == 1
+    c = c.reshape(1, -1).contiguous() if one else c.contiguous()
+    z = _dev(zn, torch.complex128, device).contiguous()
+    xv = _dev(x, torch.complex128, device).reshape(-1).contiguous()
+    ns, n = c.shape
+    if z.numel() != n:
+        raise ValueError("coef and zn must have the same number of points")
+    out = torch.empty((ns, xv.numel()), dtype=torch.complex128, device=device)
+    _lib.call("pygw_padeg", C.c_void_p(xv.data_ptr()), xv.numel(), C.c_void_p(z.data_ptr()), C.c_void_p(c.data_ptr()),
+              n, ns, C.c_void_p(out.data_ptr()), _stream_ptr(device))
+    return out[0] if one else out
 
 
 def bz_calcqdepw_par3(enk, ef, kqid, ncg, nomx, numin, dUl, omega, tetc, *, layout="fortran",

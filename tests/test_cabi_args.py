@@ -33,7 +33,7 @@ def _zeros(fn):
 
 @pytest.mark.parametrize("name", [
     "pygw_tetra_weights", "pygw_tetra_weights_svd", "pygw_chi0_accumulate", "pygw_chi0_autotune",
-    "pygw_eps_inverse", "pygw_eps_inverse_noalloc", "pygw_mwm", "pygw_selfc_spline", "pygw_selfc_convolve", "pygw_pade_qp",
+    "pygw_eps_inverse", "pygw_eps_inverse_noalloc", "pygw_mwm", "pygw_padecof", "pygw_padeg", "pygw_selfc_spline", "pygw_selfc_convolve", "pygw_pade_qp",
     "pygw_pade_qp_gw0", "pygw_selfc_qsum", "pygw_gw0_enk", "pygw_tetra_idos", "pygw_chi0_wings",
     "pygw_eps_inverse_head", "pygw_mwm_head", "pygw_chi0_batch_run", "pygw_chi0_batch_autotune",
     "pygw_chi0_batch_free", "pygw_chi0_workspace", "pygw_eps_inverse_workspace", "pygw_mwm_workspace",
