@@ -62,6 +62,11 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true",
                     help="c4: time the split-precision batch only (no FP64 path, no precision check; profiling runs)")
     ap.add_argument("--cpu-kslab", type=int, default=2)
+    ap.add_argument("--omega-rank", default="auto",
+                    help="c5 split-precision chi0: 'auto' (the q's own frequency basis, rank from its singular "
+                         "values), an integer rank, or 'off' (frequency by frequency)")
+    ap.add_argument("--uncompressed-steps", type=int, default=3,
+                    help="c5: split-precision q steps timed frequency by frequency for comparison (0: none)")
     ap.add_argument("--save-sigma", default=None,
                     help="c5: write the FP64 and split-precision Sigma_c sums and the Sigma_c energies to this .npz")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -243,12 +248,12 @@ class ChainWorkload:
         self.qlist = C5_QLIST[:nq] if nq else C5_QLIST
         self.wkq = 1.0 / len(self.qlist)
 
-    def loop(self, prec):
+    def loop(self, prec, omega_rank=None):
         from pygw_amd.qloop import SelfEnergyQLoop
         b = self.b
         return SelfEnergyQLoop(b.enk, 0.0, b.ncg, b.nomx, b.numin, self.om, self.wo, self.tetc, self.base.pb,
                                len(self.qlist), len(self.klist), self.nbgw, self.nb2, omega_p=self.omp,
-                               womeg_p=self.wop, precision=prec)
+                               womeg_p=self.wop, precision=prec, omega_rank=omega_rank)
 
     def q_inputs(self, q, M=None, Ms=None):
         """Device inputs of one q, complex64 (the split-precision operand;
@@ -417,7 +422,14 @@ def run_chain(args):
        quasiparticle energies of both sums with each state's Pade
        sensitivity.  FP64 seconds per q are the stage times of this pass.
     2. Timed pass: `warmup` + `steps` split-precision q steps cycling over the
-       q-list, inputs resident in HBM.
+       q-list, inputs resident in HBM.  By default the split-precision chi0
+       is contracted in the q's own frequency basis (--omega-rank auto: the
+       leading eigenvectors of the weights' Gram matrix, as PyGW's SVD mode
+       contracts in its analytic basis) and expanded to the 64 frequencies
+       before the inversion; the FP64 path stays frequency by frequency, so
+       the precision pass measures the basis truncation and the split
+       together.  A few steps frequency by frequency are timed for
+       comparison.
     value = split-precision seconds per q when every q's chi0 and eps^-1 are
     within 1e-6 and its Sigma_c within 1e-5 of FP64 (relative Frobenius,
     every frequency), else the FP64 seconds per q."""
@@ -430,7 +442,10 @@ def run_chain(args):
     base, b = wl.base, wl.b
     nql = len(wl.qlist)
     stage_names = ["polarization_weights", "chi0", "eps_inverse", "mwm", "selfc_convolution"]
-    L64, Lmx = wl.loop("fp64"), wl.loop("mixed")
+    orank = None if args.omega_rank == "off" else (args.omega_rank if args.omega_rank == "auto"
+                                                   else int(args.omega_rank))
+    L64, Lmx = wl.loop("fp64"), wl.loop("mixed", orank)
+    ranks = {"precision_pass": [], "timed": []}
     # the mixed loop's input buffers take each q's M as it is produced
     Mbuf = torch.empty((base.nk, b.n_o * b.n_e, base.pb), dtype=torch.complex64, device="cuda")
     Msbuf = torch.empty((len(wl.klist), wl.nbgw * wl.nb2, base.pb), dtype=torch.complex64, device="cuda")
@@ -453,7 +468,9 @@ def run_chain(args):
         errs["eps_inv"].append(_rel_per_w(Lmx.eps, L64.eps, one))
         d = (Lmx.sc - L64.sc).abs().pow(2).sum(dim=(0, 1)).sqrt()
         errs["sigma_c_q"].append((d / L64.sc.abs().pow(2).sum(dim=(0, 1)).sqrt()).cpu().numpy())
-        print(f"[bench] precision pass q={q}: fp64 {sum(t64[-1]):.3f} s; chi0 {errs['chi0'][-1].max():.2e} "
+        ranks["precision_pass"].append(Lmx.rank_used)
+        print(f"[bench] precision pass q={q}: fp64 {sum(t64[-1]):.3f} s; basis rank {Lmx.rank_used}; "
+              f"chi0 {errs['chi0'][-1].max():.2e} "
               f"eps^-1 {errs['eps_inv'][-1].max():.2e} Sigma_c {errs['sigma_c_q'][-1].max():.2e}",
               file=sys.stderr, flush=True)
     sig64, sigmx = L64.sigc.cpu().numpy(), Lmx.sigc.cpu().numpy()
@@ -492,17 +509,40 @@ def run_chain(args):
         if it >= args.warmup:
             times.append([e[i].elapsed_time(e[i + 1]) / 1e3 for i in range(5)])
             walls.append(wall)
+            ranks["timed"].append(Lmx.rank_used)
         print(f"[bench] mixed q={wl.qlist[it % nql]} {wall:.3f} s{' (warm-up)' if it < args.warmup else ''}",
               file=sys.stderr, flush=True)
+    # for comparison: the same split-precision steps frequency by frequency
+    unc = None
+    if orank is not None and args.uncompressed_steps > 0:
+        Lmx.omega_rank = None
+        tu = []
+        for it in range(1 + args.uncompressed_steps):
+            kq, M, Ms, eq = wl.q_inputs(wl.qlist[it % nql], Mbuf, Msbuf)
+            Lmx.set_q(kq, enk_kq=eq)
+            torch.cuda.synchronize()
+            e = ev()
+            Lmx.run_q(e, M=M, M_selfe=Ms)
+            torch.cuda.synchronize()
+            if it >= 1:
+                tu.append([e[i].elapsed_time(e[i + 1]) / 1e3 for i in range(5)])
+            print(f"[bench] mixed, frequency by frequency, q={wl.qlist[it % nql]}{' (warm-up)' if it < 1 else ''}",
+                  file=sys.stderr, flush=True)
+        Lmx.omega_rank = orank
+        tu = np.array(tu).mean(0)
+        unc = {"s_per_q": float(tu.sum()), "stages_s": dict(zip(stage_names, map(float, tu))),
+               "steps": args.uncompressed_steps}
     del s
     tmx, t64m = np.array(times).mean(0), np.array(t64).mean(0)
     st_mx, st_64 = dict(zip(stage_names, map(float, tmx))), dict(zip(stage_names, map(float, t64m)))
     s_mx, s_64 = float(tmx.sum()), float(t64m.sum())
     nk, no, ne, pb = base.nk, b.n_o, b.n_e, base.pb
     f_chi = 4.0 * pb * pb * nk * no * ne * nom
+    rmean = float(np.mean(ranks["timed"])) if orank is not None else float(nom)
+    f_exec = f_chi * rmean / nom if ok else f_chi          # the contraction run: rank basis functions
     t_chi = st_mx["chi0"] if ok else st_64["chi0"]
     kern = "pygw_chi0_mixed" if ok else "pygw_chi0_f64"
-    nl = chi0_launches(pb, no * ne, nom, nk) if ok else 1
+    nl = chi0_launches(pb, no * ne, int(round(rmean)) if ok else nom, nk) if ok else 1
     peak = PEAK_FP16 if ok else PEAK_FP64
     # weights: HBM roofline on the algorithmic bytes (kcw written once, enk
     # and the index arrays read once) and the FP64-VALU count per (small
@@ -515,18 +555,24 @@ def run_chain(args):
         "value": s_mx if ok else s_64, "unit": "s/q", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": (s_mx if ok else s_64) * 1e3, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16x3-split/fp32-acc chi0+MWM, f64 weights/inverse/Sigma_c" if ok else "f64",
+        "dtype": ("fp16x3-split/fp32-acc chi0+MWM, f64 weights/inverse/Sigma_c"
+                  + (", chi0 in the q's frequency basis" if orank is not None else "")) if ok else "f64",
         "data": "synthetic",
         "config": {"workload": f"c5: 8x8x8 k-mesh (3072 tetrahedra), {no}+{ne} bands, PB={pb}, {nom} Matsubara "
                                f"points (tan mesh, precise mesh x20 = {wl.omp.size}), Sigma_c of {wl.nbgw} bands at "
                                f"{len(wl.klist)} k summed over a q-list of {nql} (wkq = 1/{nql}); one q per step",
                    "precision": "mixed" if ok else "fp64", "q_list": [list(q) for q in wl.qlist],
-                   "timed_steps_cycle_q_list": True},
+                   "timed_steps_cycle_q_list": True,
+                   "chi0_frequency_basis": {"mode": "off" if orank is None else str(orank),
+                                            "ranks_precision_pass": ranks["precision_pass"],
+                                            "ranks_timed": ranks["timed"], "n_om": nom}},
         "stages_s": st_mx if ok else st_64,
-        "roofline": {"bound": "mfma", "kernel": kern, "achieved": f_chi / t_chi / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": f_chi / t_chi / peak, "traffic": load_traffic("c5", kern),
-                     "launches_per_q": nl, "flops_per_launch": f_chi / nl, "time_per_launch_s": t_chi / nl,
-                     "executed_flops_per_launch": (3 * f_chi if ok else f_chi) / nl},
+        "roofline": {"bound": "mfma", "kernel": kern, "achieved": f_exec / t_chi / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": f_exec / t_chi / peak, "traffic": load_traffic("c5", kern),
+                     "launches_per_q": nl, "flops_per_launch": f_exec / nl, "time_per_launch_s": t_chi / nl,
+                     "executed_flops_per_launch": (3 * f_exec if ok else f_exec) / nl,
+                     "basis_functions": rmean if ok else nom,
+                     "frequency_by_frequency_equivalent_tflops": f_chi / t_chi / 1e12},
         "weights_roofline": {"bound": "hbm", "kernels": "pygw_tw_* (per q)", "achieved": w_bytes / t_w / 1e9,
                              "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": w_bytes / t_w / PEAK_HBM,
                              "algorithmic_bytes": w_bytes, "traffic": load_traffic("c5", "pygw_tw_*"),
@@ -537,6 +583,7 @@ def run_chain(args):
         "fp64": {"s_per_q": s_64, "stages_s": st_64, "chi0_tflops": f_chi / st_64["chi0"] / 1e12,
                  "chi0_frac_fp64_peak": f_chi / st_64["chi0"] / PEAK_FP64},
         "mixed_speedup_vs_fp64": s_64 / s_mx,
+        "mixed_frequency_by_frequency": unc,
         "rel_frobenius_error_mixed_vs_fp64": {
             "per_q_max": {k: [float(np.max(x)) for x in v] for k, v in errs.items()},
             "chi0_max": emax["chi0"], "chi0_median": float(np.median(np.concatenate(errs["chi0"]))),
@@ -736,6 +783,7 @@ def run_batch(args):
                                                               map(float, t64 / nq))),
                  "chi0_tflops": f_tot / t64[1] / 1e12, "chi0_frac_fp64_peak": f_tot / t64[1] / PEAK_FP64},
         "mixed_speedup_vs_fp64": s_64 / s_mx,
+        "mixed_frequency_by_frequency": unc,
         "rel_frobenius_error_mixed_vs_fp64": {"chi0_max": float(r_chi.max()), "chi0_median": float(np.median(r_chi)),
                                               "eps_inv_max": float(r_inv.max()),
                                               "eps_inv_median": float(np.median(r_inv))},

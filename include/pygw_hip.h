@@ -99,6 +99,38 @@ int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, cons
                          int n_k, double coef, int precision, int beta, void* eps, void* workspace,
                          size_t ws_bytes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Contraction in a compressed frequency basis.  PyGW's SVD mode (iopfreq
+ * 5: svd_functions, pygw.py:3478-3501; bz_calcqdepw_par3,
+ * for_tetrahedra.f90:466-542; calc_eps, pygw.py:3206-3226) computes the
+ * weights in n_l basis functions, contracts chi0 with n_l "frequencies" and
+ * expands eps = dot(eps, Ul) before the inversion.  The same three steps here,
+ * with either PyGW's analytic basis (dUl, Ul) or a basis taken from the q's
+ * own weights:
+ *   pygw_omega_basis: U = eigenvectors of G = sum_{k,p} kcw[k,:,p]
+ *     kcw[k,:,p]^T (FP64, cyclic Jacobi), basis[w*n_om + r] (decreasing
+ *     eigenvalues), sv[r] = sqrt(lambda_r) (the singular values of the
+ *     weights as an (N_k N_p) x n_om matrix); info[0] = Jacobi sweeps.
+ *     n_om <= PYGW_OMEGA_MAX.  workspace: pygw_omega_basis_workspace bytes
+ *     or NULL (the call allocates it, stream-ordered).
+ *   pygw_omega_project: kcw_r[k][r][p] = sum_w P(w, r) kcw[k][w][p] with
+ *     P(w, r) = proj[w*sw + r*sr] (U: sw = n_om, sr = 1; Fortran dUl
+ *     (n_om, n_l): sw = 1, sr = n_om).  kcw_r then goes to
+ *     pygw_chi0_accumulate with n_om = rank.
+ *   pygw_omega_expand: eps[w] = beta*eps[w] + sum_r E(w, r) X[r] for the
+ *     complex128 [rank][pb][pb] result X of that contraction, E(w, r) =
+ *     ex[w*sw + r*sr] (U as above; PyGW's Ul [n_l][n_om]: sw = 1,
+ *     sr = n_om); eps complex128 [n_om][pb][pb].
+ * ------------------------------------------------------------------------ */
+#define PYGW_OMEGA_MAX 96
+int pygw_omega_basis_workspace(int n_k, int n_om, int n_pairs, size_t* bytes);
+int pygw_omega_basis(const double* kcw, int n_k, int n_om, int n_pairs, double* basis, double* sv, int* info,
+                     void* workspace, size_t ws_bytes, void* stream);
+int pygw_omega_project(const double* kcw, int n_k, int n_om, int n_pairs, const double* proj, long sw, long sr,
+                       int rank, double* kcw_r, void* stream);
+int pygw_omega_expand(const void* x, const double* ex, long sw, long sr, int pb, int rank, int n_om, int beta,
+                      void* eps, void* stream);
+
 /* Autotuner (no counterpart in PyGW): times the contraction's schedules
  * s = 1, 2, 4 (FP64: s Matsubara frequencies per 64x64 tile block; split
  * precision: FP32 tile sums folded into FP64 every 64*s chunks) on the given

@@ -126,6 +126,98 @@ def chi0_accumulate(M, kcw_native, coef=-2.0, *, eps=None, precision="fp64", k0=
     return _fortran_view(buf, pb, nom)
 
 
+def omega_basis(kcw_native, *, device="cuda", workspace=None):
+    """Frequency basis of one q's weights kcw_native [N_k, n_om, n_pairs]:
+    U = eigenvectors of sum_{k,p} kcw[k,:,p] kcw[k,:,p]^T in decreasing order
+    of eigenvalue and sv = the singular values of the weights as an
+    (N_k N_p) x n_om matrix.  Returns (U [n_om, n_om] float64, column r the
+    r-th vector; sv [n_om]), both on the device."""
+    device = _require_device(device)
+    kc = _dev(kcw_native, torch.float64, device).contiguous()
+    nk, nom, npair = kc.shape
+    U = torch.empty(nom, nom, dtype=torch.float64, device=device)
+    sv = torch.empty(nom, dtype=torch.float64, device=device)
+    if workspace is None:
+        sz = C.c_size_t(0)
+        _lib.call("pygw_omega_basis_workspace", nk, nom, npair, C.byref(sz))
+        workspace = torch.empty(sz.value, dtype=torch.uint8, device=device)
+    _lib.call("pygw_omega_basis", C.c_void_p(kc.data_ptr()), nk, nom, npair, C.c_void_p(U.data_ptr()),
+              C.c_void_p(sv.data_ptr()), None, C.c_void_p(workspace.data_ptr()), workspace.numel(),
+              _stream_ptr(device))
+    return U, sv
+
+
+def omega_rank(sv, tol=1e-7, margin=4):
+    """Basis size for the compressed contraction: the singular values above
+    tol * sv[0] plus `margin` more (at most n_om).  On the C5 weights the
+    relative chi0 change is 1e-8 at rank 8 and 5e-10 at rank 10-12."""
+    s = sv.detach().cpu().numpy() if isinstance(sv, torch.Tensor) else np.asarray(sv)
+    if s.size == 0 or not s[0] > 0:
+        return 1
+    return int(min(s.size, np.count_nonzero(s > tol * s[0]) + margin))
+
+
+def omega_project(kcw_native, proj, rank, *, out=None, device="cuda"):
+    """kcw_r[k, r, p] = sum_w proj[w, r] kcw[k, w, p] for r < rank (proj:
+    the basis U of omega_basis, or PyGW's dUl [n_om, n_l])."""
+    device = _require_device(device)
+    kc = _dev(kcw_native, torch.float64, device).contiguous()
+    nk, nom, npair = kc.shape
+    P = _dev(proj, torch.float64, device)
+    if P.shape[0] != nom or P.shape[1] < rank:
+        raise ValueError("proj must be [n_om, >= rank]")
+    if out is None:
+        out = torch.empty(nk, rank, npair, dtype=torch.float64, device=device)
+    _lib.call("pygw_omega_project", C.c_void_p(kc.data_ptr()), nk, nom, npair, C.c_void_p(P.data_ptr()),
+              P.stride(0), P.stride(1), int(rank), C.c_void_p(out.data_ptr()), _stream_ptr(device))
+    return out
+
+
+def omega_expand(X, expand, rank, *, eps=None, device="cuda"):
+    """eps[:, :, w] (+)= sum_r expand[w, r] X[:, :, r]: X the Fortran-strided
+    [PB, PB, rank] result of chi0_accumulate on projected weights, expand
+    [n_om, >= rank] (the basis U, or PyGW's Ul transposed: eps = dot(eps, Ul),
+    pygw.py:3224-3226).  Returns eps Fortran-strided [PB, PB, n_om]; with eps
+    given, the expansion is added to it."""
+    device = _require_device(device)
+    pb = X.shape[0]
+    if X.stride() != (1, pb, pb * pb) or X.shape[2] < rank:
+        raise ValueError("X must be a Fortran-strided [PB, PB, >= rank] complex128 tensor")
+    E = _dev(expand, torch.float64, device)
+    nom = E.shape[0]
+    beta = 0
+    if eps is None:
+        buf = torch.empty(pb * pb * nom, dtype=torch.complex128, device=device)
+    else:
+        if eps.stride() != (1, pb, pb * pb) or eps.shape != (pb, pb, nom):
+            raise ValueError("eps must be a Fortran-strided [PB, PB, n_om] complex128 tensor")
+        buf = eps.as_strided((pb * pb * nom,), (1,))
+        beta = 1
+    _lib.call("pygw_omega_expand", C.c_void_p(X.data_ptr()), C.c_void_p(E.data_ptr()), E.stride(0), E.stride(1), pb,
+              int(rank), nom, beta, C.c_void_p(buf.data_ptr()), _stream_ptr(device))
+    return _fortran_view(buf, pb, nom)
+
+
+def chi0_compressed(M, kcw_native, coef=-2.0, *, rank=None, basis=None, precision="mixed", k0=0,
+                    device="cuda"):
+    """chi0 of calc_eps through a compressed frequency basis: project the
+    weights on the q's own basis (omega_basis; rank from omega_rank unless
+    given), contract with `rank` basis functions in place of the n_om
+    frequencies, expand back.  Returns (eps [PB, PB, n_om] Fortran-strided,
+    rank, sv)."""
+    device = _require_device(device)
+    kc = _dev(kcw_native, torch.float64, device).contiguous()
+    if basis is None:
+        U, sv = omega_basis(kc, device=device)
+    else:
+        U, sv = basis
+    if rank is None:
+        rank = omega_rank(sv)
+    kr = omega_project(kc, U, rank, device=device)
+    X = chi0_accumulate(M, kr, coef, precision=precision, k0=k0, device=device)
+    return omega_expand(X, U, rank, device=device), rank, sv
+
+
 def eps_inverse(eps, *, device="cuda", check=True):
     """In place: eps[:,:,w] <- ((eps+eps^H)/2 + 1)^-1 - 1 (pygw.py:3229-3271)."""
     device = _require_device(device)

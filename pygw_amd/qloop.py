@@ -12,9 +12,10 @@ and after the loop the Pade continuation and the quasiparticle equation
 (mcommon.py:167-210, 315-343).  SelfEnergyQLoop holds every device buffer
 and workspace of that chain for one band structure, allocated once, so that
 add_q() launches the HIP kernels of libpygw_hip.so on the current stream
-with no allocation, memset or host synchronisation: a q step can be
-captured in a HIP graph (capture()) and replayed with the next q's inputs
-copied into the loop's input buffers (set_q()).
+with no allocation, memset or host synchronisation (but for the basis size
+of omega_rank="auto", read on the host): a q step can be captured in a HIP
+graph (capture()) and replayed with the next q's inputs copied into the
+loop's input buffers (set_q()).
 
 The per-q inputs are the k-q index column (kqm.kqid[:, iq]), M of the
 polarization pairs of every k ([N_k][n_o*n_e][PB], pair p = i_o*n_e + i_e,
@@ -31,7 +32,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .polarization import COMPAT_REFERENCE, _fortran_view, _require_device, _stream_ptr
+from .polarization import COMPAT_REFERENCE, _fortran_view, _require_device, _stream_ptr, omega_rank
 
 _PREC = {"fp64": 0, "mixed": 1}
 
@@ -49,10 +50,17 @@ class SelfEnergyQLoop:
     (iopfreq = 4, mcommon.py:126-144), else the coarse all_fr_convolutions.
     precision "fp64" or "mixed" (split-precision chi0 and MWM on MFMA; the
     weights, inversion, convolution and Pade are FP64 either way).
+    omega_rank: None contracts chi0 frequency by frequency; an int r or
+    "auto" contracts it in the q's own frequency basis (the leading r
+    eigenvectors of the weights' Gram matrix, "auto": polarization.omega_rank
+    of its singular values, read on the host once per q) and expands it back
+    to the n_om frequencies before the inversion, as PyGW's SVD mode does with
+    its analytic basis (pygw.py:3206-3226).
     """
 
     def __init__(self, enk, ef, ncg, nomx, numin, omega, womeg, tetc, pb, n_q, n_ks, nb1, nb2, *,
-                 omega_p=None, womeg_p=None, precision="mixed", compat=COMPAT_REFERENCE, device="cuda"):
+                 omega_p=None, womeg_p=None, precision="mixed", compat=COMPAT_REFERENCE, omega_rank=None,
+                 device="cuda"):
         d = _require_device(device)
         self.device = d = d if d.index is not None else torch.device(d.type, torch.cuda.current_device())
         if precision not in _PREC:
@@ -107,6 +115,18 @@ class SelfEnergyQLoop:
         self.ws_ei = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
         _lib.call("pygw_mwm_workspace", self.nks, self.rows, self.pb, self.nom, p, C.byref(sz))
         self.ws_mw = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
+        # compressed frequency basis of the contraction
+        if omega_rank is not None and omega_rank != "auto" and not 1 <= int(omega_rank) <= self.nom:
+            raise ValueError("omega_rank must be None, 'auto' or 1..n_om")
+        self.omega_rank, self.rank_used = omega_rank, None
+        if omega_rank is not None:
+            rmax = self.nom if omega_rank == "auto" else int(omega_rank)
+            self.U = torch.empty((self.nom, self.nom), dtype=torch.float64, device=d)
+            self.sv = torch.empty(self.nom, dtype=torch.float64, device=d)
+            _lib.call("pygw_omega_basis_workspace", self.nk, self.nom, self.np, C.byref(sz))
+            self.ws_ob = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
+            self.kcw_r = torch.empty((self.nk, rmax, self.np), dtype=torch.float64, device=d)
+            self.x_buf = torch.empty(self.pb * self.pb * rmax, dtype=torch.complex128, device=d)
         self.n_added = 0
 
     # -- inputs ---------------------------------------------------------------
@@ -161,8 +181,20 @@ class SelfEnergyQLoop:
                   self.numin, vp(self.om), self.nom, self.nk, self.nbnd, self.ntet, 1, self.compat, vp(self.kcw),
                   vp(self.ws_tw), self.ws_tw.numel(), None, s)
         rec(1)
-        _lib.call("pygw_chi0_accumulate", vp(Mq), self.np * self.pb, self.np, self.pb, vp(self.kcw), self.nom, 0,
-                  self.nk, -2.0, p, 0, vp(self.eps_buf), vp(self.ws_c0), self.ws_c0.numel(), s)
+        if self.omega_rank is None:
+            _lib.call("pygw_chi0_accumulate", vp(Mq), self.np * self.pb, self.np, self.pb, vp(self.kcw), self.nom, 0,
+                      self.nk, -2.0, p, 0, vp(self.eps_buf), vp(self.ws_c0), self.ws_c0.numel(), s)
+        else:
+            _lib.call("pygw_omega_basis", vp(self.kcw), self.nk, self.nom, self.np, vp(self.U), vp(self.sv), None,
+                      vp(self.ws_ob), self.ws_ob.numel(), s)
+            r = omega_rank(self.sv) if self.omega_rank == "auto" else int(self.omega_rank)
+            self.rank_used = r
+            _lib.call("pygw_omega_project", vp(self.kcw), self.nk, self.nom, self.np, vp(self.U), self.nom, 1, r,
+                      vp(self.kcw_r), s)
+            _lib.call("pygw_chi0_accumulate", vp(Mq), self.np * self.pb, self.np, self.pb, vp(self.kcw_r), r, 0,
+                      self.nk, -2.0, p, 0, vp(self.x_buf), vp(self.ws_c0), self.ws_c0.numel(), s)
+            _lib.call("pygw_omega_expand", vp(self.x_buf), vp(self.U), self.nom, 1, self.pb, r, self.nom, 0,
+                      vp(self.eps_buf), s)
         if keep_chi0:
             if self.chi is None:
                 self.chi = _fortran_view(torch.empty_like(self.eps_buf), self.pb, self.nom)
@@ -213,6 +245,8 @@ class SelfEnergyQLoop:
         on the given M / M_selfe, by pointer) in a HIP graph; replaying it
         after set_q() of the next q's kqid/energies (and M contents) runs
         that q.  Returns the torch.cuda.CUDAGraph."""
+        if self.omega_rank == "auto":
+            raise ValueError("graph capture needs a fixed omega_rank (the basis size is read on the host)")
         g = torch.cuda.CUDAGraph()
         self.run_q(keep_chi0=keep_chi0, M=M, M_selfe=M_selfe)   # first call outside the graph (module load)
         torch.cuda.synchronize(self.device)
