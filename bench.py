@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--omega-rank", default="auto",
                     help="c5 split-precision chi0: 'auto' (the q's own frequency basis, rank from its singular "
                          "values), an integer rank, or 'off' (frequency by frequency)")
+    ap.add_argument("--w-rank", default="auto",
+                    help="c5 split-precision MWM: W in its own frequency basis ('auto', an integer rank) or 'off'")
     ap.add_argument("--uncompressed-steps", type=int, default=3,
                     help="c5: split-precision q steps timed frequency by frequency for comparison (0: none)")
     ap.add_argument("--save-sigma", default=None,
@@ -248,12 +250,12 @@ class ChainWorkload:
         self.qlist = C5_QLIST[:nq] if nq else C5_QLIST
         self.wkq = 1.0 / len(self.qlist)
 
-    def loop(self, prec, omega_rank=None):
+    def loop(self, prec, omega_rank=None, w_rank=None):
         from pygw_amd.qloop import SelfEnergyQLoop
         b = self.b
         return SelfEnergyQLoop(b.enk, 0.0, b.ncg, b.nomx, b.numin, self.om, self.wo, self.tetc, self.base.pb,
                                len(self.qlist), len(self.klist), self.nbgw, self.nb2, omega_p=self.omp,
-                               womeg_p=self.wop, precision=prec, omega_rank=omega_rank)
+                               womeg_p=self.wop, precision=prec, omega_rank=omega_rank, w_rank=w_rank)
 
     def q_inputs(self, q, M=None, Ms=None):
         """Device inputs of one q, complex64 (the split-precision operand;
@@ -444,8 +446,9 @@ def run_chain(args):
     stage_names = ["polarization_weights", "chi0", "eps_inverse", "mwm", "selfc_convolution"]
     orank = None if args.omega_rank == "off" else (args.omega_rank if args.omega_rank == "auto"
                                                    else int(args.omega_rank))
-    L64, Lmx = wl.loop("fp64"), wl.loop("mixed", orank)
-    ranks = {"precision_pass": [], "timed": []}
+    wrank = None if args.w_rank == "off" else (args.w_rank if args.w_rank == "auto" else int(args.w_rank))
+    L64, Lmx = wl.loop("fp64"), wl.loop("mixed", orank, wrank)
+    ranks = {"precision_pass": [], "timed": [], "w_precision_pass": [], "w_timed": []}
     # the mixed loop's input buffers take each q's M as it is produced
     Mbuf = torch.empty((base.nk, b.n_o * b.n_e, base.pb), dtype=torch.complex64, device="cuda")
     Msbuf = torch.empty((len(wl.klist), wl.nbgw * wl.nb2, base.pb), dtype=torch.complex64, device="cuda")
@@ -469,7 +472,9 @@ def run_chain(args):
         d = (Lmx.sc - L64.sc).abs().pow(2).sum(dim=(0, 1)).sqrt()
         errs["sigma_c_q"].append((d / L64.sc.abs().pow(2).sum(dim=(0, 1)).sqrt()).cpu().numpy())
         ranks["precision_pass"].append(Lmx.rank_used)
-        print(f"[bench] precision pass q={q}: fp64 {sum(t64[-1]):.3f} s; basis rank {Lmx.rank_used}; "
+        ranks["w_precision_pass"].append(Lmx.w_rank_used)
+        print(f"[bench] precision pass q={q}: fp64 {sum(t64[-1]):.3f} s; basis rank {Lmx.rank_used}, "
+              f"W {Lmx.w_rank_used}; "
               f"chi0 {errs['chi0'][-1].max():.2e} "
               f"eps^-1 {errs['eps_inv'][-1].max():.2e} Sigma_c {errs['sigma_c_q'][-1].max():.2e}",
               file=sys.stderr, flush=True)
@@ -510,12 +515,13 @@ def run_chain(args):
             times.append([e[i].elapsed_time(e[i + 1]) / 1e3 for i in range(5)])
             walls.append(wall)
             ranks["timed"].append(Lmx.rank_used)
+            ranks["w_timed"].append(Lmx.w_rank_used)
         print(f"[bench] mixed q={wl.qlist[it % nql]} {wall:.3f} s{' (warm-up)' if it < args.warmup else ''}",
               file=sys.stderr, flush=True)
     # for comparison: the same split-precision steps frequency by frequency
     unc = None
-    if orank is not None and args.uncompressed_steps > 0:
-        Lmx.omega_rank = None
+    if (orank is not None or wrank is not None) and args.uncompressed_steps > 0:
+        Lmx.omega_rank = Lmx.w_rank = None
         tu = []
         for it in range(1 + args.uncompressed_steps):
             kq, M, Ms, eq = wl.q_inputs(wl.qlist[it % nql], Mbuf, Msbuf)
@@ -528,7 +534,7 @@ def run_chain(args):
                 tu.append([e[i].elapsed_time(e[i + 1]) / 1e3 for i in range(5)])
             print(f"[bench] mixed, frequency by frequency, q={wl.qlist[it % nql]}{' (warm-up)' if it < 1 else ''}",
                   file=sys.stderr, flush=True)
-        Lmx.omega_rank = orank
+        Lmx.omega_rank, Lmx.w_rank = orank, wrank
         tu = np.array(tu).mean(0)
         unc = {"s_per_q": float(tu.sum()), "stages_s": dict(zip(stage_names, map(float, tu))),
                "steps": args.uncompressed_steps}
@@ -556,7 +562,8 @@ def run_chain(args):
         "ms_per_step": (s_mx if ok else s_64) * 1e3, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None,
         "dtype": ("fp16x3-split/fp32-acc chi0+MWM, f64 weights/inverse/Sigma_c"
-                  + (", chi0 in the q's frequency basis" if orank is not None else "")) if ok else "f64",
+                  + (", chi0 in the q's frequency basis" if orank is not None else "")
+                  + (", MWM in W's frequency basis" if wrank is not None else "")) if ok else "f64",
         "data": "synthetic",
         "config": {"workload": f"c5: 8x8x8 k-mesh (3072 tetrahedra), {no}+{ne} bands, PB={pb}, {nom} Matsubara "
                                f"points (tan mesh, precise mesh x20 = {wl.omp.size}), Sigma_c of {wl.nbgw} bands at "
@@ -565,7 +572,10 @@ def run_chain(args):
                    "timed_steps_cycle_q_list": True,
                    "chi0_frequency_basis": {"mode": "off" if orank is None else str(orank),
                                             "ranks_precision_pass": ranks["precision_pass"],
-                                            "ranks_timed": ranks["timed"], "n_om": nom}},
+                                            "ranks_timed": ranks["timed"], "n_om": nom},
+                   "mwm_frequency_basis": {"mode": "off" if wrank is None else str(wrank),
+                                           "ranks_precision_pass": ranks["w_precision_pass"],
+                                           "ranks_timed": ranks["w_timed"], "n_om": nom}},
         "stages_s": st_mx if ok else st_64,
         "roofline": {"bound": "mfma", "kernel": kern, "achieved": f_exec / t_chi / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": f_exec / t_chi / peak, "traffic": load_traffic("c5", kern),

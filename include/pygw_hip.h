@@ -121,6 +121,15 @@ int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, cons
  *     complex128 [rank][pb][pb] result X of that contraction, E(w, r) =
  *     ex[w*sw + r*sr] (U as above; PyGW's Ul [n_l][n_om]: sw = 1,
  *     sr = n_om); eps complex128 [n_om][pb][pb].
+ *   pygw_omega_expand_batched: the same for n_batch blocks of nel complex
+ *     values: out[b][w][e] = beta*out[b][w][e] + sum_r E(w, r) x[b][r][e]
+ *     (e.g. the MWM of W in a frequency basis, [n_k][rank][rows] ->
+ *     [n_k][n_om][rows]; PyGW's SVD mode keeps mwm in the basis and
+ *     convolves the basis functions instead, mcommon.py:116-123).
+ * The basis of W = eps^-1 - 1 (PyGW: eps = dot(eps, dUl), pygw.py:3273-3276)
+ * is pygw_omega_basis of the complex128 [n_om][pb][pb] eps read as real
+ * [1][n_om][2 pb^2] (the real part of the Gram matrix of W), and its
+ * projection pygw_omega_project of the same real view.
  * ------------------------------------------------------------------------ */
 #define PYGW_OMEGA_MAX 96
 int pygw_omega_basis_workspace(int n_k, int n_om, int n_pairs, size_t* bytes);
@@ -130,6 +139,8 @@ int pygw_omega_project(const double* kcw, int n_k, int n_om, int n_pairs, const 
                        int rank, double* kcw_r, void* stream);
 int pygw_omega_expand(const void* x, const double* ex, long sw, long sr, int pb, int rank, int n_om, int beta,
                       void* eps, void* stream);
+int pygw_omega_expand_batched(const void* x, const double* ex, long sw, long sr, int n_batch, long nel, int rank,
+                              int n_om, int beta, void* out, void* stream);
 
 /* Autotuner (no counterpart in PyGW): times the contraction's schedules
  * s = 1, 2, 4 (FP64: s Matsubara frequencies per 64x64 tile block; split

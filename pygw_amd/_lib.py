@@ -42,6 +42,8 @@ _SIGS = {
                                      C.c_void_p, C.c_void_p]),
     "pygw_omega_expand": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_void_p, C.c_void_p]),
+    "pygw_omega_expand_batched": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_int, C.c_long, C.c_int,
+                                            C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "pygw_chi0_autotune": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
                                      C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p]),

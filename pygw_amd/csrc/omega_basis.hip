@@ -270,14 +270,17 @@ __global__ void __launch_bounds__(256) pygw_wproject(const double* __restrict__ 
   }
 }
 
-// eps[w][e] = (beta ? eps[w][e] : 0) + sum_r E(w, r) X[r][e], e < nel,
-// E(w, r) = ex[w*sw + r*sr]; a thread per element e (the same e in every
-// pass over r, so the passes after the first add to what the first wrote)
+// eps[b][w][e] = (beta ? eps[b][w][e] : 0) + sum_r E(w, r) X[b][r][e],
+// e < nel, batch b = blockIdx.y, E(w, r) = ex[w*sw + r*sr]; a thread per
+// element e (the same e in every pass over r, so the passes after the first
+// add to what the first wrote)
 __global__ void __launch_bounds__(256) pygw_omega_expand_k(const double2* __restrict__ X, const double* __restrict__ ex,
                                                            long sw, long sr, long nel, int rank, int nom, int beta,
                                                            double2* __restrict__ eps) {
   using namespace ob;
   __shared__ double Us[kMaxOm * RC];
+  X += (size_t)blockIdx.y * rank * nel;
+  eps += (size_t)blockIdx.y * nom * nel;
   for (int r0 = 0; r0 < rank; r0 += RC) {
     const int nr = rank - r0 < RC ? rank - r0 : RC;
     __syncthreads();
@@ -360,16 +363,21 @@ extern "C" int pygw_omega_project(const double* kcw, int n_k, int n_om, int n_pa
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }
 
-extern "C" int pygw_omega_expand(const void* x, const double* ex, long sw, long sr, int pb, int rank, int n_om,
-                                 int beta, void* eps, void* stream_) {
+extern "C" int pygw_omega_expand_batched(const void* x, const double* ex, long sw, long sr, int n_batch, long nel,
+                                         int rank, int n_om, int beta, void* out, void* stream_) {
   hipStream_t stream = (hipStream_t)stream_;
-  if (!x || !ex || !eps || pb <= 0 || n_om <= 0 || n_om > ob::kMaxOm || rank < 1 || rank > ob::kMaxOm || sw < 0 ||
-      sr < 0 || (beta != 0 && beta != 1))
+  if (!x || !ex || !out || n_batch <= 0 || n_batch > 65535 || nel <= 0 || n_om <= 0 || n_om > ob::kMaxOm ||
+      rank < 1 || rank > ob::kMaxOm || sw < 0 || sr < 0 || (beta != 0 && beta != 1))
     return PYGW_E_ARG;
-  const long nel = (long)pb * pb;
   int blocks = ceil_div(nel, 256);
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(pygw_omega_expand_k, dim3(blocks), dim3(256), 0, stream, (const double2*)x, ex, sw, sr, nel,
-                     rank, n_om, beta, (double2*)eps);
+  hipLaunchKernelGGL(pygw_omega_expand_k, dim3(blocks, n_batch), dim3(256), 0, stream, (const double2*)x, ex, sw, sr,
+                     nel, rank, n_om, beta, (double2*)out);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+}
+
+extern "C" int pygw_omega_expand(const void* x, const double* ex, long sw, long sr, int pb, int rank, int n_om,
+                                 int beta, void* eps, void* stream) {
+  if (pb <= 0) return PYGW_E_ARG;
+  return pygw_omega_expand_batched(x, ex, sw, sr, 1, (long)pb * pb, rank, n_om, beta, eps, stream);
 }

@@ -12,8 +12,8 @@ and after the loop the Pade continuation and the quasiparticle equation
 (mcommon.py:167-210, 315-343).  SelfEnergyQLoop holds every device buffer
 and workspace of that chain for one band structure, allocated once, so that
 add_q() launches the HIP kernels of libpygw_hip.so on the current stream
-with no allocation, memset or host synchronisation (but for the basis size
-of omega_rank="auto", read on the host): a q step can be captured in a HIP
+with no allocation, memset or host synchronisation (but for the basis sizes
+of omega_rank / w_rank "auto", read on the host): a q step can be captured in a HIP
 graph (capture()) and replayed with the next q's inputs copied into the
 loop's input buffers (set_q()).
 
@@ -55,12 +55,15 @@ class SelfEnergyQLoop:
     eigenvectors of the weights' Gram matrix, "auto": polarization.omega_rank
     of its singular values, read on the host once per q) and expands it back
     to the n_om frequencies before the inversion, as PyGW's SVD mode does with
-    its analytic basis (pygw.py:3206-3226).
+    its analytic basis (pygw.py:3206-3226).  w_rank: the same for the MWM
+    contraction, in the frequency basis of W = eps^-1 - 1 itself (PyGW's SVD
+    mode projects eps on its basis before calc_selfc, pygw.py:3273-3276);
+    mwm is expanded back to the n_om frequencies for the convolution.
     """
 
     def __init__(self, enk, ef, ncg, nomx, numin, omega, womeg, tetc, pb, n_q, n_ks, nb1, nb2, *,
                  omega_p=None, womeg_p=None, precision="mixed", compat=COMPAT_REFERENCE, omega_rank=None,
-                 device="cuda"):
+                 w_rank=None, device="cuda"):
         d = _require_device(device)
         self.device = d = d if d.index is not None else torch.device(d.type, torch.cuda.current_device())
         if precision not in _PREC:
@@ -116,17 +119,26 @@ class SelfEnergyQLoop:
         _lib.call("pygw_mwm_workspace", self.nks, self.rows, self.pb, self.nom, p, C.byref(sz))
         self.ws_mw = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
         # compressed frequency basis of the contraction
-        if omega_rank is not None and omega_rank != "auto" and not 1 <= int(omega_rank) <= self.nom:
-            raise ValueError("omega_rank must be None, 'auto' or 1..n_om")
+        for r in (omega_rank, w_rank):
+            if r is not None and r != "auto" and not 1 <= int(r) <= self.nom:
+                raise ValueError("omega_rank and w_rank must be None, 'auto' or 1..n_om")
         self.omega_rank, self.rank_used = omega_rank, None
+        self.w_rank, self.w_rank_used = w_rank, None
+        if omega_rank is not None or w_rank is not None:
+            _lib.call("pygw_omega_basis_workspace", self.nk, self.nom, self.np, C.byref(sz))
+            self.ws_ob = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
         if omega_rank is not None:
             rmax = self.nom if omega_rank == "auto" else int(omega_rank)
             self.U = torch.empty((self.nom, self.nom), dtype=torch.float64, device=d)
             self.sv = torch.empty(self.nom, dtype=torch.float64, device=d)
-            _lib.call("pygw_omega_basis_workspace", self.nk, self.nom, self.np, C.byref(sz))
-            self.ws_ob = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
             self.kcw_r = torch.empty((self.nk, rmax, self.np), dtype=torch.float64, device=d)
             self.x_buf = torch.empty(self.pb * self.pb * rmax, dtype=torch.complex128, device=d)
+        if w_rank is not None:
+            rmax = self.nom if w_rank == "auto" else int(w_rank)
+            self.Uw = torch.empty((self.nom, self.nom), dtype=torch.float64, device=d)
+            self.svw = torch.empty(self.nom, dtype=torch.float64, device=d)
+            self.xw_buf = torch.empty(self.pb * self.pb * rmax, dtype=torch.complex128, device=d)
+            self.mwm_r = torch.empty((self.nks, rmax, self.rows), dtype=torch.complex128, device=d)
         self.n_added = 0
 
     # -- inputs ---------------------------------------------------------------
@@ -203,8 +215,21 @@ class SelfEnergyQLoop:
         _lib.call("pygw_eps_inverse", vp(self.eps_buf), self.pb, self.nom, vp(self.ws_ei), self.ws_ei.numel(), None,
                   s)
         rec(3)
-        _lib.call("pygw_mwm", vp(Msq), self.nks, self.rows, self.pb, vp(self.eps_buf), self.nom, self.wkq, p,
-                  vp(self.mwm), vp(self.ws_mw), self.ws_mw.numel(), s)
+        if self.w_rank is None:
+            _lib.call("pygw_mwm", vp(Msq), self.nks, self.rows, self.pb, vp(self.eps_buf), self.nom, self.wkq, p,
+                      vp(self.mwm), vp(self.ws_mw), self.ws_mw.numel(), s)
+        else:
+            n2 = 2 * self.pb * self.pb        # W read as real [1][n_om][2 PB^2]
+            _lib.call("pygw_omega_basis", vp(self.eps_buf), 1, self.nom, n2, vp(self.Uw), vp(self.svw), None,
+                      vp(self.ws_ob), self.ws_ob.numel(), s)
+            r = omega_rank(self.svw) if self.w_rank == "auto" else int(self.w_rank)
+            self.w_rank_used = r
+            _lib.call("pygw_omega_project", vp(self.eps_buf), 1, self.nom, n2, vp(self.Uw), self.nom, 1, r,
+                      vp(self.xw_buf), s)
+            _lib.call("pygw_mwm", vp(Msq), self.nks, self.rows, self.pb, vp(self.xw_buf), r, self.wkq, p,
+                      vp(self.mwm_r), vp(self.ws_mw), self.ws_mw.numel(), s)
+            _lib.call("pygw_omega_expand_batched", vp(self.mwm_r), vp(self.Uw), self.nom, 1, self.nks, self.rows, r,
+                      self.nom, 0, vp(self.mwm), s)
         rec(4)
         if self.spline:
             _lib.call("pygw_selfc_spline", vp(self.mwm), self.nks, self.nb1, self.nb2, vp(self.om), self.nom,
@@ -245,8 +270,8 @@ class SelfEnergyQLoop:
         on the given M / M_selfe, by pointer) in a HIP graph; replaying it
         after set_q() of the next q's kqid/energies (and M contents) runs
         that q.  Returns the torch.cuda.CUDAGraph."""
-        if self.omega_rank == "auto":
-            raise ValueError("graph capture needs a fixed omega_rank (the basis size is read on the host)")
+        if self.omega_rank == "auto" or self.w_rank == "auto":
+            raise ValueError("graph capture needs fixed omega_rank / w_rank (the basis size is read on the host)")
         g = torch.cuda.CUDAGraph()
         self.run_q(keep_chi0=keep_chi0, M=M, M_selfe=M_selfe)   # first call outside the graph (module load)
         torch.cuda.synchronize(self.device)

@@ -84,6 +84,19 @@ def test_project_and_expand_match_numpy():
     np.testing.assert_allclose(got.cpu().numpy(), want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
     got2 = P.omega_expand(Xd, ex, rank, eps=got)                # beta = 1 adds
     np.testing.assert_allclose(got2.cpu().numpy(), 2 * want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+    # batched: [n_b][rank][nel] -> [n_b][n_om][nel] (the MWM rows of every k)
+    import ctypes as C
+    from pygw_amd import _lib
+    nb, nel, r2 = 6, 333, 9
+    xb = rng.normal(size=(nb, r2, nel)) + 1j * rng.normal(size=(nb, r2, nel))
+    Ed = torch.as_tensor(ex, device="cuda")
+    xd = torch.as_tensor(xb, device="cuda")
+    out = torch.empty((nb, nom, nel), dtype=torch.complex128, device="cuda")
+    print("case omega expand batched", flush=True)
+    _lib.call("pygw_omega_expand_batched", C.c_void_p(xd.data_ptr()), C.c_void_p(Ed.data_ptr()), rank, 1,
+              nb, nel, r2, nom, 0, C.c_void_p(out.data_ptr()), None)
+    want = np.einsum("brn,wr->bwn", xb, ex[:, :r2])
+    np.testing.assert_allclose(out.cpu().numpy(), want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
     torch.cuda.synchronize()
 
 

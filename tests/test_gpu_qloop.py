@@ -48,11 +48,11 @@ def _system():
     return cfg, b, fm, tetc, qs
 
 
-def _loop(cfg, b, fm, tetc, precision, omega_rank=None):
+def _loop(cfg, b, fm, tetc, precision, omega_rank=None, w_rank=None):
     from pygw_amd.qloop import SelfEnergyQLoop
     return SelfEnergyQLoop(b.enk, 0.0, b.ncg, b.nomx, b.numin, fm.omega, fm.womeg, tetc, cfg.pb, len(QLIST), len(KS),
                            NB1, b.enk.shape[0], omega_p=fm.omega_precise, womeg_p=fm.womeg_precise,
-                           precision=precision, omega_rank=omega_rank)
+                           precision=precision, omega_rank=omega_rank, w_rank=w_rank)
 
 
 def _oracle_q(b, fm, cfg, kcw_native, M, Ms, enq):
@@ -123,13 +123,13 @@ def test_qloop_fp64_end_to_end_vs_oracle():
     torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("omega_rank", [None, "auto"])
-def test_qloop_split_precision_vs_oracle(omega_rank):
-    """The split-precision loop, chi0 frequency by frequency or in each q's
-    own frequency basis (omega_rank auto), against the FP64 oracle."""
+@pytest.mark.parametrize("omega_rank,w_rank", [(None, None), ("auto", None), ("auto", "auto")])
+def test_qloop_split_precision_vs_oracle(omega_rank, w_rank):
+    """The split-precision loop, chi0 and MWM frequency by frequency or in
+    each q's own frequency bases (rank auto), against the FP64 oracle."""
     import torch
     cfg, b, fm, tetc, qs = _system()
-    L = _loop(cfg, b, fm, tetc, "mixed", omega_rank)
+    L = _loop(cfg, b, fm, tetc, "mixed", omega_rank, w_rank)
     sig_o = 0
     for q, (kq, M, Ms, enq) in zip(QLIST, qs):
         print(f"case qloop mixed q={q}", flush=True)
@@ -144,7 +144,7 @@ def test_qloop_split_precision_vs_oracle(omega_rank):
         gw = L.eps.cpu().numpy() + one
         ri = max(np.linalg.norm(gw[:, :, w] - (W + one)[:, :, w]) / np.linalg.norm((W + one)[:, :, w])
                  for w in range(cfg.n_om))
-        print(f"  chi0 {rc:.2e} eps^-1 {ri:.2e} (basis rank {L.rank_used})", flush=True)
+        print(f"  chi0 {rc:.2e} eps^-1 {ri:.2e} (basis ranks chi0 {L.rank_used}, W {L.w_rank_used})", flush=True)
         assert rc < 1e-6 and ri < 1e-6
         sig_o = sig_o + sc
     sig = L.sigc.cpu().numpy()
@@ -155,18 +155,18 @@ def test_qloop_split_precision_vs_oracle(omega_rank):
 
 
 @pytest.mark.parametrize("omega_rank", [None, 12])
-def test_qloop_graph_replay_matches_eager(omega_rank):
+def test_qloop_graph_replay_matches_eager(omega_rank):  # (a fixed basis size for chi0 and for W)
     """One q step captured in a HIP graph and replayed for every q of the
     list (inputs copied into the loop's buffers between replays, eager work
     in between): the Sigma_c sum equals the eager loop's bit for bit, also
     with chi0 in a fixed-size frequency basis."""
     import torch
     cfg, b, fm, tetc, qs = _system()
-    eager = _loop(cfg, b, fm, tetc, "mixed", omega_rank)
+    eager = _loop(cfg, b, fm, tetc, "mixed", omega_rank, omega_rank)
     for kq, M, Ms, enq in qs:
         eager.add_q(kq, M.astype(np.complex64), Ms.astype(np.complex64), enq)
     want = eager.sigc.clone()
-    L = _loop(cfg, b, fm, tetc, "mixed", omega_rank)
+    L = _loop(cfg, b, fm, tetc, "mixed", omega_rank, omega_rank)
     kq, M, Ms, enq = qs[0]
     L.set_q(kq, M.astype(np.complex64), Ms.astype(np.complex64), enq)
     print("case qloop graph", flush=True)
