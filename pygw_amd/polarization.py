@@ -147,10 +147,12 @@ def omega_basis(kcw_native, *, device="cuda", workspace=None):
     return U, sv
 
 
-def omega_rank(sv, tol=1e-7, margin=4):
+def omega_rank(sv, tol=1e-7, margin=2):
     """Basis size for the compressed contraction: the singular values above
-    tol * sv[0] plus `margin` more (at most n_om).  On the C5 weights the
-    relative chi0 change is 1e-8 at rank 8 and 5e-10 at rank 10-12."""
+    tol * sv[0] plus `margin` more (at most n_om).  On the C5 weights nine
+    singular values lie above 1e-7 of the first, the rest on a plateau near
+    1.5e-8; the relative chi0 change is 1e-8 at rank 8 and 5e-10 at ranks
+    10-12 (FP64, every frequency)."""
     s = sv.detach().cpu().numpy() if isinstance(sv, torch.Tensor) else np.asarray(sv)
     if s.size == 0 or not s[0] > 0:
         return 1
