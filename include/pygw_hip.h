@@ -108,18 +108,23 @@ int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, cons
  * with either PyGW's analytic basis (dUl, Ul) or a basis taken from the q's
  * own weights:
  *   pygw_omega_basis: U = eigenvectors of G = sum_{k,p} kcw[k,:,p]
- *     kcw[k,:,p]^T (FP64, cyclic Jacobi), basis[w*n_om + r] (decreasing
- *     eigenvalues), sv[r] = sqrt(lambda_r) (the singular values of the
- *     weights as an (N_k N_p) x n_om matrix); info[0] = Jacobi sweeps.
- *     n_om <= PYGW_OMEGA_MAX.  workspace: pygw_omega_basis_workspace bytes
- *     or NULL (the call allocates it, stream-ordered).
+ *     kcw[k,:,p]^T (FP64, cyclic Jacobi) in decreasing order of eigenvalue,
+ *     sv[r] = sqrt(lambda_r) (the singular values of the weights as an
+ *     (N_k N_p) x n_om matrix); info[0] = Jacobi sweeps.  balance 0:
+ *     proj = expand = U; balance 1: G is first scaled to unit diagonal
+ *     (every frequency's column of weights to unit norm, D their norms),
+ *     U its eigenvectors, proj = D^-1 U, expand = D U (both [n_om][n_om],
+ *     element (w, r) at w*n_om + r; proj^T expand = 1), sv of the scaled
+ *     matrix.  n_om <= PYGW_OMEGA_MAX.  workspace:
+ *     pygw_omega_basis_workspace bytes or NULL (the call allocates it,
+ *     stream-ordered).
  *   pygw_omega_project: kcw_r[k][r][p] = sum_w P(w, r) kcw[k][w][p] with
- *     P(w, r) = proj[w*sw + r*sr] (U: sw = n_om, sr = 1; Fortran dUl
+ *     P(w, r) = proj[w*sw + r*sr] (proj above: sw = n_om, sr = 1; Fortran dUl
  *     (n_om, n_l): sw = 1, sr = n_om).  kcw_r then goes to
  *     pygw_chi0_accumulate with n_om = rank.
  *   pygw_omega_expand: eps[w] = beta*eps[w] + sum_r E(w, r) X[r] for the
  *     complex128 [rank][pb][pb] result X of that contraction, E(w, r) =
- *     ex[w*sw + r*sr] (U as above; PyGW's Ul [n_l][n_om]: sw = 1,
+ *     ex[w*sw + r*sr] (expand above; PyGW's Ul [n_l][n_om]: sw = 1,
  *     sr = n_om); eps complex128 [n_om][pb][pb].
  *   pygw_omega_expand_batched: the same for n_batch blocks of nel complex
  *     values: out[b][w][e] = beta*out[b][w][e] + sum_r E(w, r) x[b][r][e]
@@ -133,8 +138,8 @@ int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, cons
  * ------------------------------------------------------------------------ */
 #define PYGW_OMEGA_MAX 96
 int pygw_omega_basis_workspace(int n_k, int n_om, int n_pairs, size_t* bytes);
-int pygw_omega_basis(const double* kcw, int n_k, int n_om, int n_pairs, double* basis, double* sv, int* info,
-                     void* workspace, size_t ws_bytes, void* stream);
+int pygw_omega_basis(const double* kcw, int n_k, int n_om, int n_pairs, int balance, double* proj, double* expand,
+                     double* sv, int* info, void* workspace, size_t ws_bytes, void* stream);
 int pygw_omega_project(const double* kcw, int n_k, int n_om, int n_pairs, const double* proj, long sw, long sr,
                        int rank, double* kcw_r, void* stream);
 int pygw_omega_expand(const void* x, const double* ex, long sw, long sr, int pb, int rank, int n_om, int beta,

@@ -21,6 +21,14 @@
 // components only; U is orthonormal, so the projection neither amplifies the
 // weights nor mixes frequencies beyond what U_R spans.
 //
+// Balanced basis: the frequencies are first scaled to equal weight (each
+// column of the weight matrix to unit norm, D = diag of the column norms),
+// U the eigenvectors of that Gram matrix, and then the projection uses
+// D^-1 U and the expansion D U.  The truncation and the split-precision
+// rounding then cost every frequency the same relative error, also where
+// the weights are small (high frequencies of a wide log mesh, where an
+// unbalanced basis makes chi0 a cancelling sum of larger terms).
+//
 // Gram matrix: FP64, each block sums its column tiles into its own partial
 // matrix and pygw_wgram_sum adds the partials in block order (deterministic).
 // Eigen-decomposition: cyclic Jacobi in one block with G and the accumulated
@@ -114,23 +122,33 @@ __global__ void __launch_bounds__(256) pygw_wgram_sum(const double* __restrict__
 // their rotations are applied together: all row updates, then all column
 // updates of A and of the accumulated rotations V.  Sweeps stop when the
 // off-diagonal Frobenius norm is below 1e-15 of the diagonal's.
-// basis[w*n + r] = V[w][i] for the eigenvalue of rank r (decreasing),
+// With balance, the matrix is first scaled to unit diagonal, G_ij / (d_i d_j)
+// with d_i = sqrt(G_ii) (1 where G_ii = 0): for a Gram matrix that is the
+// Gram matrix of the columns scaled to unit norm.  For the eigenvalue of
+// rank r (decreasing) and its eigenvector V[:, i]: proj[w*n + r] =
+// V[w][i] / d_w, expand[w*n + r] = V[w][i] d_w (d = 1 without balance),
 // sv[r] = sqrt(max(lambda, 0)); info[0] = sweeps used.
-__global__ void __launch_bounds__(1024) pygw_symeig(const double* __restrict__ G, int n, int max_sweeps,
-                                                    double* __restrict__ basis, double* __restrict__ sv,
-                                                    int* __restrict__ info) {
+__global__ void __launch_bounds__(1024) pygw_symeig(const double* __restrict__ G, int n, int max_sweeps, int balance,
+                                                    double* __restrict__ proj, double* __restrict__ expand,
+                                                    double* __restrict__ sv, int* __restrict__ info) {
   using namespace ob;
   __shared__ double A[kMaxOm][kMaxOm + 1];
   __shared__ double V[kMaxOm][kMaxOm + 1];
   __shared__ double rc[kMaxOm / 2], rs[kMaxOm / 2];
   __shared__ int rp[kMaxOm / 2], rq[kMaxOm / 2];
   __shared__ double red[2][16];
+  __shared__ double dn[kMaxOm];
   __shared__ int done;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int m = n + (n & 1), h = m / 2;
+  for (int i = t; i < n; i += 1024) {
+    const double g = G[(size_t)i * n + i];
+    dn[i] = (balance && g > 0.0) ? sqrt(g) : 1.0;
+  }
+  __syncthreads();
   for (int e = t; e < n * n; e += 1024) {
     const int i = e / n, j = e % n;
-    A[i][j] = G[e];
+    A[i][j] = G[e] / (dn[i] * dn[j]);
     V[i][j] = i == j ? 1.0 : 0.0;
   }
   __syncthreads();
@@ -229,7 +247,10 @@ __global__ void __launch_bounds__(1024) pygw_symeig(const double* __restrict__ G
       const double lj = A[j][j];
       r += (lj > li || (lj == li && j < i)) ? 1 : 0;
     }
-    for (int w = 0; w < n; w++) basis[(size_t)w * n + r] = V[w][i];
+    for (int w = 0; w < n; w++) {
+      proj[(size_t)w * n + r] = V[w][i] / dn[w];
+      expand[(size_t)w * n + r] = V[w][i] * dn[w];
+    }
     sv[r] = sqrt(fmax(li, 0.0));
   }
   if (t == 0 && info) info[0] = sweep;
@@ -323,10 +344,13 @@ extern "C" int pygw_omega_basis_workspace(int n_k, int n_om, int n_pairs, size_t
   return PYGW_OK;
 }
 
-extern "C" int pygw_omega_basis(const double* kcw, int n_k, int n_om, int n_pairs, double* basis, double* sv,
-                                int* info, void* workspace, size_t ws_bytes, void* stream_) {
+extern "C" int pygw_omega_basis(const double* kcw, int n_k, int n_om, int n_pairs, int balance, double* proj,
+                                double* expand, double* sv, int* info, void* workspace, size_t ws_bytes,
+                                void* stream_) {
   hipStream_t stream = (hipStream_t)stream_;
-  if (!kcw || !basis || !sv || n_k <= 0 || n_pairs <= 0 || n_om <= 0 || n_om > ob::kMaxOm) return PYGW_E_ARG;
+  if (!kcw || !proj || !expand || !sv || n_k <= 0 || n_pairs <= 0 || n_om <= 0 || n_om > ob::kMaxOm ||
+      (balance != 0 && balance != 1))
+    return PYGW_E_ARG;
   const size_t need = basis_ws(n_om);
   void* ws = workspace;
   bool own = false;
@@ -343,7 +367,8 @@ extern "C" int pygw_omega_basis(const double* kcw, int n_k, int n_om, int n_pair
   hipLaunchKernelGGL(pygw_wgram, dim3(nparts), dim3(256), 0, stream, kcw, n_k, n_om, n_pairs, part);
   hipLaunchKernelGGL(pygw_wgram_sum, dim3(ceil_div((long)n_om * n_om, 256)), dim3(256), 0, stream,
                      (const double*)part, nparts, n_om * n_om, G);
-  hipLaunchKernelGGL(pygw_symeig, dim3(1), dim3(1024), 0, stream, (const double*)G, n_om, 40, basis, sv, info);
+  hipLaunchKernelGGL(pygw_symeig, dim3(1), dim3(1024), 0, stream, (const double*)G, n_om, 40, balance, proj, expand,
+                     sv, info);
   int rc = hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
   if (own && hipFreeAsync(ws, stream) != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
   return rc;

@@ -126,25 +126,29 @@ def chi0_accumulate(M, kcw_native, coef=-2.0, *, eps=None, precision="fp64", k0=
     return _fortran_view(buf, pb, nom)
 
 
-def omega_basis(kcw_native, *, device="cuda", workspace=None):
-    """Frequency basis of one q's weights kcw_native [N_k, n_om, n_pairs]:
-    U = eigenvectors of sum_{k,p} kcw[k,:,p] kcw[k,:,p]^T in decreasing order
-    of eigenvalue and sv = the singular values of the weights as an
-    (N_k N_p) x n_om matrix.  Returns (U [n_om, n_om] float64, column r the
-    r-th vector; sv [n_om]), both on the device."""
+def omega_basis(kcw_native, *, balance=True, device="cuda", workspace=None):
+    """Frequency basis of one q's weights kcw_native [N_k, n_om, n_pairs],
+    from the eigenvectors U of the Gram matrix of the weights as an
+    (N_k N_p) x n_om matrix, in decreasing order of eigenvalue.  balance:
+    every frequency's column first scaled to unit norm (D the norms), so that
+    every frequency keeps the same relative accuracy.  Returns (proj, expand,
+    sv): proj = D^-1 U and expand = D U ([n_om, n_om] float64, column r for
+    basis function r; proj = expand = U without balance) and the singular
+    values sv [n_om] of the (scaled) matrix, all on the device."""
     device = _require_device(device)
     kc = _dev(kcw_native, torch.float64, device).contiguous()
     nk, nom, npair = kc.shape
-    U = torch.empty(nom, nom, dtype=torch.float64, device=device)
+    Pm = torch.empty(nom, nom, dtype=torch.float64, device=device)
+    Em = torch.empty(nom, nom, dtype=torch.float64, device=device)
     sv = torch.empty(nom, dtype=torch.float64, device=device)
     if workspace is None:
         sz = C.c_size_t(0)
         _lib.call("pygw_omega_basis_workspace", nk, nom, npair, C.byref(sz))
         workspace = torch.empty(sz.value, dtype=torch.uint8, device=device)
-    _lib.call("pygw_omega_basis", C.c_void_p(kc.data_ptr()), nk, nom, npair, C.c_void_p(U.data_ptr()),
-              C.c_void_p(sv.data_ptr()), None, C.c_void_p(workspace.data_ptr()), workspace.numel(),
-              _stream_ptr(device))
-    return U, sv
+    _lib.call("pygw_omega_basis", C.c_void_p(kc.data_ptr()), nk, nom, npair, int(bool(balance)),
+              C.c_void_p(Pm.data_ptr()), C.c_void_p(Em.data_ptr()), C.c_void_p(sv.data_ptr()), None,
+              C.c_void_p(workspace.data_ptr()), workspace.numel(), _stream_ptr(device))
+    return Pm, Em, sv
 
 
 def omega_rank(sv, tol=1e-7, margin=2):
@@ -203,21 +207,18 @@ def omega_expand(X, expand, rank, *, eps=None, device="cuda"):
 def chi0_compressed(M, kcw_native, coef=-2.0, *, rank=None, basis=None, precision="mixed", k0=0,
                     device="cuda"):
     """chi0 of calc_eps through a compressed frequency basis: project the
-    weights on the q's own basis (omega_basis; rank from omega_rank unless
-    given), contract with `rank` basis functions in place of the n_om
-    frequencies, expand back.  Returns (eps [PB, PB, n_om] Fortran-strided,
-    rank, sv)."""
+    weights on the q's own (balanced) basis (omega_basis; rank from
+    omega_rank unless given), contract with `rank` basis functions in place
+    of the n_om frequencies, expand back.  basis: (proj, expand, sv) of
+    omega_basis.  Returns (eps [PB, PB, n_om] Fortran-strided, rank, sv)."""
     device = _require_device(device)
     kc = _dev(kcw_native, torch.float64, device).contiguous()
-    if basis is None:
-        U, sv = omega_basis(kc, device=device)
-    else:
-        U, sv = basis
+    Pm, Em, sv = omega_basis(kc, device=device) if basis is None else basis
     if rank is None:
         rank = omega_rank(sv)
-    kr = omega_project(kc, U, rank, device=device)
+    kr = omega_project(kc, Pm, rank, device=device)
     X = chi0_accumulate(M, kr, coef, precision=precision, k0=k0, device=device)
-    return omega_expand(X, U, rank, device=device), rank, sv
+    return omega_expand(X, Em, rank, device=device), rank, sv
 
 
 def eps_inverse(eps, *, device="cuda", check=True):

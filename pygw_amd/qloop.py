@@ -52,7 +52,8 @@ class SelfEnergyQLoop:
     weights, inversion, convolution and Pade are FP64 either way).
     omega_rank: None contracts chi0 frequency by frequency; an int r or
     "auto" contracts it in the q's own frequency basis (the leading r
-    eigenvectors of the weights' Gram matrix, "auto": polarization.omega_rank
+    eigenvectors of the weights' Gram matrix with every frequency scaled to
+    unit norm, "auto": polarization.omega_rank
     of its singular values, read on the host once per q) and expands it back
     to the n_om frequencies before the inversion, as PyGW's SVD mode does with
     its analytic basis (pygw.py:3206-3226).  w_rank: the same for the MWM
@@ -129,13 +130,15 @@ class SelfEnergyQLoop:
             self.ws_ob = torch.empty(max(sz.value, 8), dtype=torch.uint8, device=d)
         if omega_rank is not None:
             rmax = self.nom if omega_rank == "auto" else int(omega_rank)
-            self.U = torch.empty((self.nom, self.nom), dtype=torch.float64, device=d)
+            self.Up = torch.empty((self.nom, self.nom), dtype=torch.float64, device=d)   # projection
+            self.Ue = torch.empty((self.nom, self.nom), dtype=torch.float64, device=d)   # expansion
             self.sv = torch.empty(self.nom, dtype=torch.float64, device=d)
             self.kcw_r = torch.empty((self.nk, rmax, self.np), dtype=torch.float64, device=d)
             self.x_buf = torch.empty(self.pb * self.pb * rmax, dtype=torch.complex128, device=d)
         if w_rank is not None:
             rmax = self.nom if w_rank == "auto" else int(w_rank)
-            self.Uw = torch.empty((self.nom, self.nom), dtype=torch.float64, device=d)
+            self.Uwp = torch.empty((self.nom, self.nom), dtype=torch.float64, device=d)
+            self.Uwe = torch.empty((self.nom, self.nom), dtype=torch.float64, device=d)
             self.svw = torch.empty(self.nom, dtype=torch.float64, device=d)
             self.xw_buf = torch.empty(self.pb * self.pb * rmax, dtype=torch.complex128, device=d)
             self.mwm_r = torch.empty((self.nks, rmax, self.rows), dtype=torch.complex128, device=d)
@@ -197,15 +200,15 @@ class SelfEnergyQLoop:
             _lib.call("pygw_chi0_accumulate", vp(Mq), self.np * self.pb, self.np, self.pb, vp(self.kcw), self.nom, 0,
                       self.nk, -2.0, p, 0, vp(self.eps_buf), vp(self.ws_c0), self.ws_c0.numel(), s)
         else:
-            _lib.call("pygw_omega_basis", vp(self.kcw), self.nk, self.nom, self.np, vp(self.U), vp(self.sv), None,
-                      vp(self.ws_ob), self.ws_ob.numel(), s)
+            _lib.call("pygw_omega_basis", vp(self.kcw), self.nk, self.nom, self.np, 1, vp(self.Up), vp(self.Ue),
+                      vp(self.sv), None, vp(self.ws_ob), self.ws_ob.numel(), s)
             r = omega_rank(self.sv) if self.omega_rank == "auto" else int(self.omega_rank)
             self.rank_used = r
-            _lib.call("pygw_omega_project", vp(self.kcw), self.nk, self.nom, self.np, vp(self.U), self.nom, 1, r,
+            _lib.call("pygw_omega_project", vp(self.kcw), self.nk, self.nom, self.np, vp(self.Up), self.nom, 1, r,
                       vp(self.kcw_r), s)
             _lib.call("pygw_chi0_accumulate", vp(Mq), self.np * self.pb, self.np, self.pb, vp(self.kcw_r), r, 0,
                       self.nk, -2.0, p, 0, vp(self.x_buf), vp(self.ws_c0), self.ws_c0.numel(), s)
-            _lib.call("pygw_omega_expand", vp(self.x_buf), vp(self.U), self.nom, 1, self.pb, r, self.nom, 0,
+            _lib.call("pygw_omega_expand", vp(self.x_buf), vp(self.Ue), self.nom, 1, self.pb, r, self.nom, 0,
                       vp(self.eps_buf), s)
         if keep_chi0:
             if self.chi is None:
@@ -220,15 +223,15 @@ class SelfEnergyQLoop:
                       vp(self.mwm), vp(self.ws_mw), self.ws_mw.numel(), s)
         else:
             n2 = 2 * self.pb * self.pb        # W read as real [1][n_om][2 PB^2]
-            _lib.call("pygw_omega_basis", vp(self.eps_buf), 1, self.nom, n2, vp(self.Uw), vp(self.svw), None,
-                      vp(self.ws_ob), self.ws_ob.numel(), s)
+            _lib.call("pygw_omega_basis", vp(self.eps_buf), 1, self.nom, n2, 1, vp(self.Uwp), vp(self.Uwe),
+                      vp(self.svw), None, vp(self.ws_ob), self.ws_ob.numel(), s)
             r = omega_rank(self.svw) if self.w_rank == "auto" else int(self.w_rank)
             self.w_rank_used = r
-            _lib.call("pygw_omega_project", vp(self.eps_buf), 1, self.nom, n2, vp(self.Uw), self.nom, 1, r,
+            _lib.call("pygw_omega_project", vp(self.eps_buf), 1, self.nom, n2, vp(self.Uwp), self.nom, 1, r,
                       vp(self.xw_buf), s)
             _lib.call("pygw_mwm", vp(Msq), self.nks, self.rows, self.pb, vp(self.xw_buf), r, self.wkq, p,
                       vp(self.mwm_r), vp(self.ws_mw), self.ws_mw.numel(), s)
-            _lib.call("pygw_omega_expand_batched", vp(self.mwm_r), vp(self.Uw), self.nom, 1, self.nks, self.rows, r,
+            _lib.call("pygw_omega_expand_batched", vp(self.mwm_r), vp(self.Uwe), self.nom, 1, self.nks, self.rows, r,
                       self.nom, 0, vp(self.mwm), s)
         rec(4)
         if self.spline:

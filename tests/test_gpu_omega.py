@@ -42,8 +42,9 @@ def test_basis_of_known_spectrum(nom):
     Q, _ = np.linalg.qr(rng.normal(size=(nom, nom)))
     s = np.geomspace(1.0, 1e-7, nom) if nom > 1 else np.ones(1)
     flat = (A * s) @ Q.T
-    U, sv = P.omega_basis(_flat_to_native(flat, nk, npair))
-    U, sv = U.cpu().numpy(), sv.cpu().numpy()
+    U, Ue, sv = P.omega_basis(_flat_to_native(flat, nk, npair), balance=False)
+    U, Ue, sv = U.cpu().numpy(), Ue.cpu().numpy(), sv.cpu().numpy()
+    assert np.array_equal(U, Ue)
     # eigenvalues of the Gram matrix to n*eps of its norm
     assert np.all(np.abs(sv ** 2 - s ** 2) <= 1e-14 * nom)
     assert np.all(np.diff(sv) <= 0)
@@ -52,6 +53,26 @@ def test_basis_of_known_spectrum(nom):
     good = s >= 1e-3
     dots = np.abs(np.sum(U[:, good] * Q[:, good], axis=0))
     assert np.all(dots >= 1 - 1e-10), dots.min()
+    torch.cuda.synchronize()
+
+
+def test_balanced_basis():
+    """Columns scaled to unit norm first: sv are the singular values of the
+    column-normalised matrix (numpy SVD), proj^T expand = 1, and at full
+    rank projection then expansion returns the weights."""
+    import torch
+    from pygw_amd import polarization as P
+    rng = np.random.default_rng(11)
+    nk, nom, npair = 4, 40, 53
+    flat = rng.normal(size=(nk * npair, nom)) * np.geomspace(1.0, 1e-4, nom)   # columns of very different norm
+    flat[:, :5] = flat[:, 5:10] * 3.0 + 1e-9 * flat[:, :5]                       # a nearly dependent block
+    print("case omega balanced basis", flush=True)
+    Pm, Em, sv = P.omega_basis(_flat_to_native(flat, nk, npair))
+    Pm, Em, sv = Pm.cpu().numpy(), Em.cpu().numpy(), sv.cpu().numpy()
+    s_ref = np.linalg.svd(flat / np.linalg.norm(flat, axis=0), compute_uv=False)
+    assert np.all(np.abs(sv ** 2 - s_ref ** 2) <= 1e-14 * nom * s_ref[0] ** 2)
+    np.testing.assert_allclose(Pm.T @ Em, np.eye(nom), atol=1e-11)
+    np.testing.assert_allclose(flat @ Pm @ Em.T, flat, rtol=0, atol=1e-11 * np.abs(flat).max())
     torch.cuda.synchronize()
 
 
@@ -132,23 +153,29 @@ def test_svd_mode_chain_vs_oracle():
     torch.cuda.synchronize()
 
 
-def test_compressed_chi0_vs_uncompressed():
+@pytest.mark.parametrize("mesh_kind", ["tan", "log"])
+def test_compressed_chi0_vs_uncompressed(mesh_kind):
+    """tan: PyGW's mesh of the chain workload; log: 64 points from 0.005 to
+    200 eV, the mesh of the c2/c3 workloads, where the weights fall by
+    orders of magnitude across the mesh (the balanced basis keeps every
+    frequency's relative error)."""
     import torch
     from pygw_amd import polarization as P
     cfg, b, fm, tetc, kq, M = _system((4, 4, 4), 4, 12, 32, 72, 44)
-    kn = P.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, fm.omega, tetc, layout="native")
-    print("case compressed chi0", flush=True)
+    om = fm.omega if mesh_kind == "tan" else mesh.log_mesh(0.005, 200.0, cfg.n_om)
+    kn = P.bz_calcqdepw_par2(b.enk, 0.0, kq, b.ncg, b.nomx, b.numin, om, tetc, layout="native")
+    print(f"case compressed chi0 {mesh_kind}", flush=True)
     ref = P.chi0_accumulate(M, kn, precision="fp64").cpu().numpy()
-    U, sv = P.omega_basis(kn)
-    rank = P.omega_rank(sv)
+    basis = P.omega_basis(kn)
+    rank = P.omega_rank(basis[2])
     assert rank < cfg.n_om
     nrm = np.linalg.norm(ref.reshape(-1, cfg.n_om), axis=0)
 
     def rel(x):
         return (np.linalg.norm((x - ref).reshape(-1, cfg.n_om), axis=0) / nrm).max()
 
-    e64, r64, _ = P.chi0_compressed(M, kn, basis=(U, sv), rank=rank, precision="fp64")
-    emx, _, _ = P.chi0_compressed(M.astype(np.complex64), kn, basis=(U, sv), rank=rank, precision="mixed")
+    e64, r64, _ = P.chi0_compressed(M, kn, basis=basis, rank=rank, precision="fp64")
+    emx, _, _ = P.chi0_compressed(M.astype(np.complex64), kn, basis=basis, rank=rank, precision="mixed")
     full_mx = P.chi0_accumulate(M.astype(np.complex64), kn, precision="mixed").cpu().numpy()
     d64, dmx, dfull = rel(e64.cpu().numpy()), rel(emx.cpu().numpy()), rel(full_mx)
     print(f"  rank {rank}/{cfg.n_om}: FP64 compressed {d64:.2e}, split compressed {dmx:.2e}, "
