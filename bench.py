@@ -72,7 +72,7 @@ def parse():
     ap.add_argument("--save-sigma", default=None,
                     help="c5: write the FP64 and split-precision Sigma_c sums and the Sigma_c energies to this .npz")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run one plain 4 GiB device copy (scale for HBM byte counters)")
+                    help="first run one plain 1 GiB device copy (scale for HBM byte counters)")
     ap.add_argument("--fp64-kfrac", type=float, default=None,
                     help="time the FP64 contraction on this fraction of k and extrapolate (large configs)")
     return ap.parse_args()
@@ -705,7 +705,8 @@ def run_batch(args):
 
     for i in range(nq):
         weights(i)
-    B = P.Chi0Batch(Ms, kcws, inverse=True)
+    orank = None if args.omega_rank == "off" else args.omega_rank
+    B = P.Chi0Batch(Ms, kcws, inverse=True, omega_rank=orank)
     best, tune = B.autotune()
     g_chi = B.capture(best, inverse=False)
     g_inv = torch.cuda.CUDAGraph()
@@ -775,7 +776,8 @@ def run_batch(args):
     r_chi, r_inv = np.concatenate(r_chi), np.concatenate(r_inv)
     ok = bool(r_chi.max() <= 1e-6)
     f_q = [4.0 * p * p * K * nom for p, K in zip(pbs, Ks)]
-    f_tot = float(sum(f_q))
+    f_full = float(sum(f_q))                        # frequency by frequency (the FP64 path)
+    f_tot = f_full * (B.rank / nom)                 # the contraction run: B.rank basis functions per q
     s_mx, s_64 = float(tm.sum()) / nq, float(t64.sum()) / nq
     kern = "pygw_chi0_mixed_batch"
     rec = {
@@ -787,15 +789,19 @@ def run_batch(args):
         "config": {"workload": f"c4: metal ({b.n_o}+{b.n_e} bands, 3 crossing E_F), 8x8x8 k-mesh, {nq} q in one "
                                f"batch, PB_q {min(pbs)}..{max(pbs)} (U{{300..1100}}), per-k windows of 40-260 "
                                f"pairs (K_q mean {np.mean(Ks):.0f}), {nom} frequencies; one step = the batch",
-                   "precision": "mixed" if ok else "fp64", "fold_schedule": best, "autotune_s": tune},
+                   "precision": "mixed" if ok else "fp64", "fold_schedule": best, "autotune_s": tune,
+                   "chi0_frequency_basis": {"mode": "off" if orank is None else str(orank), "rank": B.rank,
+                                            "n_om": nom}},
         "stages_s_per_q": {"polarization_weights": tm[0] / nq, "chi0": tm[1] / nq, "eps_inverse": tm[2] / nq},
         "roofline": {"bound": "mfma", "kernel": kern, "achieved": f_tot / tm[1] / 1e12, "peak": PEAK_FP16 / 1e12,
                      "unit": "TFLOP/s", "frac": f_tot / tm[1] / PEAK_FP16, "traffic": load_traffic("c4", kern),
                      "flops_per_launch": f_tot, "executed_flops_per_launch": 3 * f_tot,
-                     "timed": "HIP events around the captured batch (prep + contraction + mirror)"},
+                     "frequency_by_frequency_equivalent_tflops": f_full / tm[1] / 1e12,
+                     "timed": "HIP events around the captured batch (bases, projections, contraction, mirrors, "
+                              "expansions)"},
         "fp64": {"s_per_q": s_64, "stages_s_per_q": dict(zip(("polarization_weights", "chi0", "eps_inverse"),
                                                               map(float, t64 / nq))),
-                 "chi0_tflops": f_tot / t64[1] / 1e12, "chi0_frac_fp64_peak": f_tot / t64[1] / PEAK_FP64},
+                 "chi0_tflops": f_full / t64[1] / 1e12, "chi0_frac_fp64_peak": f_full / t64[1] / PEAK_FP64},
         "mixed_speedup_vs_fp64": s_64 / s_mx,
         "rel_frobenius_error_mixed_vs_fp64": {"chi0_max": float(r_chi.max()), "chi0_median": float(np.median(r_chi)),
                                               "eps_inv_max": float(r_inv.max()),
@@ -1173,15 +1179,17 @@ def main():
     if args.steps is None:
         args.steps = {"c5": 8, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2, "coulomb": 2, "minm_is": 2}.get(args.config, 1)
     if args.copy_calibration:
-        # one copy kernel that reads and writes exactly 4 GiB (16x the 256 MB
-        # last-level cache, so the read comes from HBM): the ratio of its
-        # FETCH_SIZE / WRITE_SIZE counters to 4 GiB scales the kernels' counters
-        src = torch.ones(1 << 29, dtype=torch.float64, device="cuda")
+        # one copy kernel that reads and writes exactly 1 GiB (one launch: under
+        # 2 GiB), its source evicted from the 256 MB last-level cache by a 2 GiB
+        # fill: the ratio of its FETCH_SIZE / WRITE_SIZE counters to 1 GiB
+        # scales the kernels' counters
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        evict = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
-        del src, dst
+        del src, dst, evict
     if args.config == "c5":
         run_chain(args)
         return

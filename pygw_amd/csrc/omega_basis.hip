@@ -139,6 +139,12 @@ __global__ void __launch_bounds__(1024) pygw_symeig(const double* __restrict__ G
   __shared__ double red[2][16];
   __shared__ double dn[kMaxOm];
   __shared__ int done;
+  // block b: the b-th matrix of a batch
+  G += (size_t)blockIdx.x * n * n;
+  proj += (size_t)blockIdx.x * n * n;
+  expand += (size_t)blockIdx.x * n * n;
+  sv += (size_t)blockIdx.x * n;
+  if (info) info += blockIdx.x;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int m = n + (n & 1), h = m / 2;
   for (int i = t; i < n; i += 1024) {
@@ -344,14 +350,17 @@ extern "C" int pygw_omega_basis_workspace(int n_k, int n_om, int n_pairs, size_t
   return PYGW_OK;
 }
 
-extern "C" int pygw_omega_basis(const double* kcw, int n_k, int n_om, int n_pairs, int balance, double* proj,
-                                double* expand, double* sv, int* info, void* workspace, size_t ws_bytes,
-                                void* stream_) {
+extern "C" int pygw_omega_basis_many(int n_q, const double* const* kcw, const int* n_k, const int* n_pairs, int n_om,
+                                     int balance, double* proj, double* expand, double* sv, int* info,
+                                     void* workspace, size_t ws_bytes, void* stream_) {
   hipStream_t stream = (hipStream_t)stream_;
-  if (!kcw || !proj || !expand || !sv || n_k <= 0 || n_pairs <= 0 || n_om <= 0 || n_om > ob::kMaxOm ||
-      (balance != 0 && balance != 1))
+  if (n_q <= 0 || n_q > 65535 || !kcw || !n_k || !n_pairs || !proj || !expand || !sv || n_om <= 0 ||
+      n_om > ob::kMaxOm || (balance != 0 && balance != 1))
     return PYGW_E_ARG;
-  const size_t need = basis_ws(n_om);
+  for (int q = 0; q < n_q; q++)
+    if (!kcw[q] || n_k[q] <= 0 || n_pairs[q] <= 0) return PYGW_E_ARG;
+  const size_t nn = (size_t)n_om * n_om;
+  const size_t need = basis_ws(n_om) + (size_t)(n_q - 1) * nn * sizeof(double);
   void* ws = workspace;
   bool own = false;
   if (!ws) {
@@ -360,18 +369,36 @@ extern "C" int pygw_omega_basis(const double* kcw, int n_k, int n_om, int n_pair
   } else if (ws_bytes < need) {
     return PYGW_E_ARG;
   }
-  const long ntiles = (long)n_k * ((n_pairs + ob::TP - 1) / ob::TP);
-  const int nparts = ntiles < ob::kGramBlocks ? (int)ntiles : ob::kGramBlocks;
+  // partial sums (reused q after q, stream-ordered), then the n_q Gram matrices
   double* part = (double*)ws;
-  double* G = part + (size_t)nparts * n_om * n_om;
-  hipLaunchKernelGGL(pygw_wgram, dim3(nparts), dim3(256), 0, stream, kcw, n_k, n_om, n_pairs, part);
-  hipLaunchKernelGGL(pygw_wgram_sum, dim3(ceil_div((long)n_om * n_om, 256)), dim3(256), 0, stream,
-                     (const double*)part, nparts, n_om * n_om, G);
-  hipLaunchKernelGGL(pygw_symeig, dim3(1), dim3(1024), 0, stream, (const double*)G, n_om, 40, balance, proj, expand,
+  double* G = part + (size_t)ob::kGramBlocks * nn;
+  for (int q = 0; q < n_q; q++) {
+    const long ntiles = (long)n_k[q] * ((n_pairs[q] + ob::TP - 1) / ob::TP);
+    const int nparts = ntiles < ob::kGramBlocks ? (int)ntiles : ob::kGramBlocks;
+    hipLaunchKernelGGL(pygw_wgram, dim3(nparts), dim3(256), 0, stream, kcw[q], n_k[q], n_om, n_pairs[q], part);
+    hipLaunchKernelGGL(pygw_wgram_sum, dim3(ceil_div((long)nn, 256)), dim3(256), 0, stream, (const double*)part,
+                       nparts, (int)nn, G + (size_t)q * nn);
+  }
+  hipLaunchKernelGGL(pygw_symeig, dim3(n_q), dim3(1024), 0, stream, (const double*)G, n_om, 40, balance, proj, expand,
                      sv, info);
   int rc = hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
   if (own && hipFreeAsync(ws, stream) != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
   return rc;
+}
+
+extern "C" int pygw_omega_basis(const double* kcw, int n_k, int n_om, int n_pairs, int balance, double* proj,
+                                double* expand, double* sv, int* info, void* workspace, size_t ws_bytes,
+                                void* stream) {
+  if (!kcw) return PYGW_E_ARG;
+  const double* k1[1] = {kcw};
+  return pygw_omega_basis_many(1, k1, &n_k, &n_pairs, n_om, balance, proj, expand, sv, info, workspace, ws_bytes,
+                               stream);
+}
+
+extern "C" int pygw_omega_basis_many_workspace(int n_q, int n_om, size_t* bytes) {
+  if (n_q <= 0 || n_om <= 0 || n_om > ob::kMaxOm || !bytes) return PYGW_E_ARG;
+  *bytes = basis_ws(n_om) + (size_t)(n_q - 1) * n_om * n_om * sizeof(double);
+  return PYGW_OK;
 }
 
 extern "C" int pygw_omega_project(const double* kcw, int n_k, int n_om, int n_pairs, const double* proj, long sw,

@@ -153,7 +153,9 @@ def omega_basis(kcw_native, *, balance=True, device="cuda", workspace=None):
 
 def omega_rank(sv, tol=1e-7, margin=2):
     """Basis size for the compressed contraction: the singular values above
-    tol * sv[0] plus `margin` more (at most n_om).  On the C5 weights nine
+    tol * sv[0] plus `margin` more (at most n_om).  The callers with
+    omega_rank="auto" contract frequency by frequency instead when this is
+    more than 3/4 of n_om (COMPRESS_MAX_FRACTION: too little to gain).  On the C5 weights nine
     singular values lie above 1e-7 of the first, the rest on a plateau near
     1.5e-8; the relative chi0 change is 1e-8 at rank 8 and 5e-10 at ranks
     10-12 (FP64, every frequency)."""
@@ -161,6 +163,10 @@ def omega_rank(sv, tol=1e-7, margin=2):
     if s.size == 0 or not s[0] > 0:
         return 1
     return int(min(s.size, np.count_nonzero(s > tol * s[0]) + margin))
+
+
+omega_rank_of = omega_rank
+COMPRESS_MAX_FRACTION = 0.75
 
 
 def omega_project(kcw_native, proj, rank, *, out=None, device="cuda"):
@@ -568,9 +574,15 @@ class Chi0Batch:
     eps of every q (complex128, Fortran [PB_q, PB_q, n_om]) and the
     workspace are allocated once; run() only launches kernels and can be
     captured in a HIP graph (capture()).  With inverse=True run() continues
-    with eps^-1 of every q (pygw_eps_inverse, explicit workspaces)."""
+    with eps^-1 of every q (pygw_eps_inverse, explicit workspaces).
 
-    def __init__(self, Ms, kcws, coef=-2.0, inverse=False, device="cuda"):
+    omega_rank (None, "auto" or an int): contract every q in its own balanced
+    frequency basis (omega_basis of its weights, recomputed by every run())
+    with a common number of basis functions, chosen once from the weights
+    given here ("auto": the largest omega_rank over the q's), and expand
+    each q's result back to the n_om frequencies."""
+
+    def __init__(self, Ms, kcws, coef=-2.0, inverse=False, omega_rank=None, device="cuda"):
         self.device = d = _require_device(device)
         if len(Ms) != len(kcws) or not Ms:
             raise ValueError("need one kcw per M")
@@ -585,12 +597,35 @@ class Chi0Batch:
                 raise ValueError("kcw[q] must be [n_om, K_q]")
         self.eps_buf = [torch.empty(int(p) * int(p) * self.nom, dtype=torch.complex128, device=d) for p in self.pb]
         self.eps = [_fortran_view(b, int(p), self.nom) for b, p in zip(self.eps_buf, self.pb)]
+        ptr = lambda ts: (C.c_void_p * len(ts))(*[t.data_ptr() for t in ts])  # noqa: E731
+        self.omega_rank, self.rank = omega_rank, self.nom
+        kin, ein, nom_c = self.kcws, self.eps_buf, self.nom
+        if omega_rank is not None:
+            nq, nom = self.nq, self.nom
+            self._wp = ptr(self.kcws)
+            self._ones = np.ones(nq, dtype=np.int32)
+            sz = C.c_size_t(0)
+            _lib.call("pygw_omega_basis_many_workspace", nq, nom, C.byref(sz))
+            self.ws_ob = torch.empty(sz.value, dtype=torch.uint8, device=d)
+            self.Up = torch.empty((nq, nom, nom), dtype=torch.float64, device=d)
+            self.Ue = torch.empty((nq, nom, nom), dtype=torch.float64, device=d)
+            self.sv = torch.empty((nq, nom), dtype=torch.float64, device=d)
+            self._bases()
+            self.rank = (max(omega_rank_of(self.sv[q]) for q in range(nq)) if omega_rank == "auto"
+                         else int(omega_rank))
+            if omega_rank == "auto" and self.rank > COMPRESS_MAX_FRACTION * nom:
+                self.omega_rank, self.rank = None, nom        # too little to gain: frequency by frequency
+        if self.omega_rank is not None:
+            self.kr = [torch.empty((self.rank, int(k)), dtype=torch.float64, device=d) for k in self.np]
+            self.x_buf = [torch.empty(int(p) * int(p) * self.rank, dtype=torch.complex128, device=d) for p in self.pb]
+            self._project()
+            kin, ein, nom_c = self.kr, self.x_buf, self.rank
         sz = C.c_size_t(0)
         _lib.call("pygw_chi0_batch_workspace", self.nq, C.c_void_p(self.pb.ctypes.data),
-                  C.c_void_p(self.np.ctypes.data), self.nom, C.byref(sz))
+                  C.c_void_p(self.np.ctypes.data), nom_c, C.byref(sz))
         self.ws = torch.empty(sz.value, dtype=torch.uint8, device=d)
-        ptr = lambda ts: (C.c_void_p * len(ts))(*[t.data_ptr() for t in ts])  # noqa: E731
-        self._mp, self._kp, self._ep = ptr(self.Ms), ptr(self.kcws), ptr(self.eps_buf)
+        self._mp, self._kp, self._ep = ptr(self.Ms), ptr(kin), ptr(ein)
+        self._nom_c = nom_c
         self.inverse = bool(inverse)
         self.ws_inv = []
         self._side = [torch.cuda.Stream(device=d) for _ in range(4)] if self.inverse else []  # default priority
@@ -601,15 +636,39 @@ class Chi0Batch:
                 self.ws_inv.append(torch.empty(si.value, dtype=torch.uint8, device=d))
         self.plan = C.c_void_p(0)
         _lib.call("pygw_chi0_batch_plan", self.nq, C.cast(self._mp, C.c_void_p), C.c_void_p(self.pb.ctypes.data),
-                  C.c_void_p(self.np.ctypes.data), C.cast(self._kp, C.c_void_p), self.nom, float(coef),
+                  C.c_void_p(self.np.ctypes.data), C.cast(self._kp, C.c_void_p), self._nom_c, float(coef),
                   C.cast(self._ep, C.c_void_p), C.c_void_p(self.ws.data_ptr()), sz.value, _stream_ptr(d),
                   C.byref(self.plan))
 
+    def _bases(self):
+        """The balanced frequency basis of every q's weights (one launch of
+        n_q eigen-decompositions)."""
+        _lib.call("pygw_omega_basis_many", self.nq, C.cast(self._wp, C.c_void_p), C.c_void_p(self._ones.ctypes.data),
+                  C.c_void_p(self.np.ctypes.data), self.nom, 1, C.c_void_p(self.Up.data_ptr()),
+                  C.c_void_p(self.Ue.data_ptr()), C.c_void_p(self.sv.data_ptr()), None,
+                  C.c_void_p(self.ws_ob.data_ptr()), self.ws_ob.numel(), _stream_ptr(self.device))
+
+    def _project(self):
+        s = _stream_ptr(self.device)
+        for q in range(self.nq):
+            _lib.call("pygw_omega_project", C.c_void_p(self.kcws[q].data_ptr()), 1, self.nom, int(self.np[q]),
+                      C.c_void_p(self.Up[q].data_ptr()), self.nom, 1, self.rank, C.c_void_p(self.kr[q].data_ptr()), s)
+
     def run(self, schedule=0, inverse=None):
         """Launch the batch (schedule 0 = autotuned / default fold interval);
-        inverse (default: the constructor's) continues with eps^-1 per q."""
+        inverse (default: the constructor's) continues with eps^-1 per q.
+        With omega_rank: bases and projections of every q first, the
+        expansions after the contraction."""
         s = _stream_ptr(self.device)
+        vp = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+        if self.omega_rank is not None:
+            self._bases()
+            self._project()
         _lib.call("pygw_chi0_batch_run", self.plan, int(schedule), s)
+        if self.omega_rank is not None:
+            for q in range(self.nq):
+                _lib.call("pygw_omega_expand", vp(self.x_buf[q]), vp(self.Ue[q]), self.nom, 1, int(self.pb[q]),
+                          self.rank, self.nom, 0, vp(self.eps_buf[q]), s)
         if not (self.inverse if inverse is None else inverse):
             return self.eps
         return self.invert()

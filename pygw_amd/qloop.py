@@ -32,7 +32,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .polarization import COMPAT_REFERENCE, _fortran_view, _require_device, _stream_ptr, omega_rank
+from .polarization import COMPAT_REFERENCE, COMPRESS_MAX_FRACTION, _fortran_view, _require_device, _stream_ptr, \
+    omega_rank
 
 _PREC = {"fp64": 0, "mixed": 1}
 
@@ -196,14 +197,18 @@ class SelfEnergyQLoop:
                   self.numin, vp(self.om), self.nom, self.nk, self.nbnd, self.ntet, 1, self.compat, vp(self.kcw),
                   vp(self.ws_tw), self.ws_tw.numel(), None, s)
         rec(1)
-        if self.omega_rank is None:
-            _lib.call("pygw_chi0_accumulate", vp(Mq), self.np * self.pb, self.np, self.pb, vp(self.kcw), self.nom, 0,
-                      self.nk, -2.0, p, 0, vp(self.eps_buf), vp(self.ws_c0), self.ws_c0.numel(), s)
-        else:
+        r = self.nom
+        if self.omega_rank is not None:
             _lib.call("pygw_omega_basis", vp(self.kcw), self.nk, self.nom, self.np, 1, vp(self.Up), vp(self.Ue),
                       vp(self.sv), None, vp(self.ws_ob), self.ws_ob.numel(), s)
             r = omega_rank(self.sv) if self.omega_rank == "auto" else int(self.omega_rank)
-            self.rank_used = r
+            if self.omega_rank == "auto" and r > COMPRESS_MAX_FRACTION * self.nom:
+                r = self.nom                        # too little to gain: frequency by frequency
+        self.rank_used = r
+        if r == self.nom and self.omega_rank != self.nom:
+            _lib.call("pygw_chi0_accumulate", vp(Mq), self.np * self.pb, self.np, self.pb, vp(self.kcw), self.nom, 0,
+                      self.nk, -2.0, p, 0, vp(self.eps_buf), vp(self.ws_c0), self.ws_c0.numel(), s)
+        else:
             _lib.call("pygw_omega_project", vp(self.kcw), self.nk, self.nom, self.np, vp(self.Up), self.nom, 1, r,
                       vp(self.kcw_r), s)
             _lib.call("pygw_chi0_accumulate", vp(Mq), self.np * self.pb, self.np, self.pb, vp(self.kcw_r), r, 0,
@@ -218,15 +223,19 @@ class SelfEnergyQLoop:
         _lib.call("pygw_eps_inverse", vp(self.eps_buf), self.pb, self.nom, vp(self.ws_ei), self.ws_ei.numel(), None,
                   s)
         rec(3)
-        if self.w_rank is None:
-            _lib.call("pygw_mwm", vp(Msq), self.nks, self.rows, self.pb, vp(self.eps_buf), self.nom, self.wkq, p,
-                      vp(self.mwm), vp(self.ws_mw), self.ws_mw.numel(), s)
-        else:
+        r = self.nom
+        if self.w_rank is not None:
             n2 = 2 * self.pb * self.pb        # W read as real [1][n_om][2 PB^2]
             _lib.call("pygw_omega_basis", vp(self.eps_buf), 1, self.nom, n2, 1, vp(self.Uwp), vp(self.Uwe),
                       vp(self.svw), None, vp(self.ws_ob), self.ws_ob.numel(), s)
             r = omega_rank(self.svw) if self.w_rank == "auto" else int(self.w_rank)
-            self.w_rank_used = r
+            if self.w_rank == "auto" and r > COMPRESS_MAX_FRACTION * self.nom:
+                r = self.nom
+        self.w_rank_used = r
+        if r == self.nom and self.w_rank != self.nom:
+            _lib.call("pygw_mwm", vp(Msq), self.nks, self.rows, self.pb, vp(self.eps_buf), self.nom, self.wkq, p,
+                      vp(self.mwm), vp(self.ws_mw), self.ws_mw.numel(), s)
+        else:
             _lib.call("pygw_omega_project", vp(self.eps_buf), 1, self.nom, n2, vp(self.Uwp), self.nom, 1, r,
                       vp(self.xw_buf), s)
             _lib.call("pygw_mwm", vp(Msq), self.nks, self.rows, self.pb, vp(self.xw_buf), r, self.wkq, p,
