@@ -140,6 +140,16 @@ int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, cons
 int pygw_omega_basis_workspace(int n_k, int n_om, int n_pairs, size_t* bytes);
 int pygw_omega_basis(const double* kcw, int n_k, int n_om, int n_pairs, int balance, double* proj, double* expand,
                      double* sv, int* info, void* workspace, size_t ws_bytes, void* stream);
+/* The same for n_q weight arrays at once (one eigen-decomposition launch of
+ * n_q blocks): kcw, n_k, n_pairs are HOST arrays of n_q device pointers /
+ * sizes ([n_k[q]][n_om][n_pairs[q]] each); proj, expand [n_q][n_om][n_om],
+ * sv [n_q][n_om], info [n_q] (optional) on the device; workspace:
+ * pygw_omega_basis_many_workspace bytes or NULL.  Capturable in a HIP graph
+ * (the host arrays are read when the call is made). */
+int pygw_omega_basis_many_workspace(int n_q, int n_om, size_t* bytes);
+int pygw_omega_basis_many(int n_q, const double* const* kcw, const int* n_k, const int* n_pairs, int n_om,
+                          int balance, double* proj, double* expand, double* sv, int* info, void* workspace,
+                          size_t ws_bytes, void* stream);
 int pygw_omega_project(const double* kcw, int n_k, int n_om, int n_pairs, const double* proj, long sw, long sr,
                        int rank, double* kcw_r, void* stream);
 int pygw_omega_expand(const void* x, const double* ex, long sw, long sr, int pb, int rank, int n_om, int beta,
@@ -449,6 +459,69 @@ int pygw_mixed_coulomb(const double* vq, int iat, int q_vq, const double* jlam, 
                        size_t ws_bytes, void* stream);
 int pygw_coul_mixed_block(const void* mpwipw, int ngq, int ngq_barc, const void* vmatit, int loct, void* vmat,
                           long ld_vmat, void* stream);
+/* The muffin-tin block of the Coulomb matrix and the q = 0 term
+ * (mt_coulomb.hip).  All arrays on the device, Fortran layouts as PyGW
+ * passes them:
+ * pygw_ewald_summation replaces fCl.ewald_summation(cq, lam_max, vpos, br2,
+ *   rbas, alat, Vol, ortho, r_cf, g_cf, eta) (pygw.py:2350,
+ *   for_Coulomb.f90:1-164): pq [3] cartesian q, pos [3][ndf] (Fortran
+ *   pos(3, ndf), fractional), br2 (3,3) (columns: reciprocal vectors), rbas
+ *   (3,3) (rows: lattice vectors), alat [3]; ortho 0/1; sgm complex128
+ *   Fortran ((lam_max+1)^2, ndf(ndf+1)/2), column i + j(j+1)/2 for atoms
+ *   i <= j (0-based).  lam_max <= PYGW_EWALD_MAX_LAM; r_cf and g_cf below
+ *   64 shortest basis vectors (else NaN).  Allocates nothing.
+ * pygw_mt_coulomb replaces fCl.mt_coulomb(vq, iat+1, loctmatsize, big_l,
+ *   nmix, im_start, tilg, rrint, djmm, rtlij, sgm, mult, lam_max)
+ *   (pygw.py:2394, for_Coulomb.f90:166-283): iat 0-based (the rows of that
+ *   atom type's functions, every other row 0) or -1 (all rows: the sum over
+ *   iat that PyGW forms); big_l int32 (max_nmix, nat), nmix [nat], im_start
+ *   [nat+1] (0-based offsets; also im_start_host in host memory), mult
+ *   [nat], tilg [tsize] (calc_tildeg's table), rrint (nat, nijrm), djmm
+ *   complex128 (dimdj, ndf), rtlij (max_nmix, max_nmix, nat, nat), sgm from
+ *   pygw_ewald_summation with lam_max >= 2 max L; vq [3] (q = 0 leaves the
+ *   L = 0 x L = 0 entries 0).  vmat complex128 Fortran (loctmatsize,
+ *   loctmatsize), overwritten; an entry whose lookup falls outside tilg,
+ *   djmm or sgm is NaN.  Allocates nothing.
+ * pygw_radial_olap: out[f][k] = RINT13g(rel, ul_f, us_f, F, F, dh, npt,
+ *   rx[0]) (radials.f90:140-187) with F = j_L(k rx) rx, L = lf[f] <= lmax
+ *   (mode 0: jlam of Calc_Olap_PW_Product, pygw.py:2110-2124) or
+ *   F = sin(k rx) (mode 1, lmax 0, every lf[f] 0: Zero_Momentum_Coulomb's
+ *   sing, pygw.py:2069-2086); rx [npt] the logarithmic mesh
+ *   (strc.radial_mesh, rx[0] e^{j dh}), ul, us float64 [n_fun][npt] (r u and
+ *   r us, PyGW's ul_product / us_product), lf int32 [n_fun], k [n_k], out
+ *   [n_fun][n_k].  workspace: pygw_radial_olap_workspace bytes (the table
+ *   of F) or NULL.
+ * pygw_zero_momentum_coulomb replaces the assembly of Zero_Momentum_Coulomb
+ *   (pygw.py:2055-2104) for the L = 0 product functions: the plane waves
+ *   gindex int32 (n_g, 3) (Fortran) sorted by length, g_start [n_short+1]
+ *   the first index of each run of equal length (pw_get_short_length; the
+ *   first plane wave, G = 0, excluded), glen0 [n_short] the first length of
+ *   each run, pos (3, ndf) fractional, sing float64 [*][n_short]
+ *   [n_sing][n_short] (pygw_radial_olap mode 1 of the L = 0 functions),
+ *   and for each L = 0 function a its row f_sing[a] of sing, atom
+ *   f_atom[a] and Vmat index f_row[a]:
+ *     vmat[f_row[a] + mbsize f_row[b]] = sum_g phase(i, j, g) 16 pi^2/(Vol glen0_g^4)
+ *                                        sing[f_sing[a]][g] sing[f_sing[b]][g],
+ *   phase(i, j, g) = the sum of e^{-2 pi i G.(pos_i - pos_j)} over the run
+ *   (pw_get_short_and_phase, for_Coulomb.f90:515-582).  Only those entries
+ *   are written.  workspace: pygw_zero_momentum_workspace bytes or NULL. */
+#define PYGW_EWALD_MAX_LAM 16
+int pygw_ewald_summation(const double* pq, int lam_max, const double* pos, const double* br2, const double* rbas,
+                         const double* alat, double vol, int ortho, double r_cf, double g_cf, double eta, int ndf,
+                         void* sgm, void* stream);
+int pygw_mt_coulomb(const double* vq, int iat, int loctmatsize, const int* big_l, const int* nmix,
+                    const int* im_start, const int* im_start_host, const int* mult, int nat, int max_nmix,
+                    const double* tilg, int tsize, const double* rrint, const void* djmm, int dimdj,
+                    const double* rtlij, const void* sgm, int lam_max, void* vmat, void* stream);
+int pygw_radial_olap_workspace(int mode, int lmax, int npt, int n_k, size_t* bytes);
+int pygw_radial_olap(int mode, int rel, const double* ul, const double* us, const int* lf, int n_fun, int lmax,
+                     int npt, const double* rx, double dh, const double* k, int n_k, double* out, void* workspace,
+                     size_t ws_bytes, void* stream);
+int pygw_zero_momentum_workspace(int ndf, int n_short, size_t* bytes);
+int pygw_zero_momentum_coulomb(const int* gindex, int n_g, const int* g_start, const double* glen0, int n_short,
+                               const double* pos, int ndf, const double* sing, int n_sing, const int* f_sing,
+                               const int* f_atom, const int* f_row, int n_fun0, double vol, int mbsize, void* vmat,
+                               void* workspace, size_t ws_bytes, void* stream);
 /* pygw_minm_is replaces fvxcn.calc_minm_is (pygw.py:2661,
  *   for_vxcnn.f90:1030-1134), the interstitial part of mmat:
  *     mmat[ie1][ie2][col0 + G] = 1/sqrt(vol) sum_{i < nvi, j < nvj}

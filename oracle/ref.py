@@ -231,3 +231,74 @@ def k2indx(k, div):
     out = C.c_int(0)
     lib.k2indx_(C.byref(out), C.c_void_p(kk.ctypes.data), C.c_void_p(d.ctypes.data))
     return out.value
+
+
+def ewald_summation(pq, lam_max, pos, br2, rbas, alat, vol, ortho, r_cf, g_cf, eta):
+    """for_Coulomb.f90:1-164: sgm [(lam_max+1)^2, ndf(ndf+1)/2] (column
+    i + j(j+1)/2 for the 0-based atom pair i <= j)."""
+    f = _load("libref_coul.so").ewald_summation_
+    ps = np.asfortranarray(np.asarray(pos, dtype=np.float64))
+    ndf = ps.shape[1]
+    out = np.zeros(((lam_max + 1) ** 2, ndf * (ndf + 1) // 2), dtype=np.complex128, order="F")
+    F = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))  # noqa: E731
+    q, b2, rb, al = F(pq), F(br2), F(rbas), F(alat)
+    orth = C.c_int(1 if ortho else 0)
+    f(_p(out), _p(q), _i(lam_max), _p(ps), _p(b2), _p(rb), _p(al), _d(vol), C.byref(orth), _d(r_cf), _d(g_cf), _d(eta),
+      _i(ndf))
+    return np.ascontiguousarray(out)
+
+
+def mt_coulomb(vq, iat, loctmatsize, big_l, nmix, im_start, tilg, rrint, djmm, rtlij, sgm, mult, lam_max):
+    """for_Coulomb.f90:166-283 with PyGW's arguments (pygw.py:2394): iat
+    1-based, big_l [max_nmix, nat] Fortran, im_start [nat+1] 0-based offsets,
+    rrint [nat, nijrm], djmm [dimdj, ndf], rtlij [max_nmix, max_nmix, nat,
+    nat] Fortran, sgm from ewald_summation.  Returns Vmat [loct, loct]."""
+    f = _load("libref_coul.so").mt_coulomb_
+    F = lambda a, dt: np.asfortranarray(np.asarray(a, dtype=dt))  # noqa: E731
+    out = np.zeros((loctmatsize, loctmatsize), dtype=np.complex128, order="F")
+    bl, nm, ims, tg = F(big_l, np.int32), F(nmix, np.int32), F(im_start, np.int32), F(tilg, np.float64)
+    rr, dj, rt, sg = F(rrint, np.float64), F(djmm, np.complex128), F(rtlij, np.float64), F(sgm, np.complex128)
+    mu, v = F(mult, np.int32), F(vq, np.float64)
+    nat, ndf = bl.shape[1], dj.shape[1]
+    f(_p(out), _p(v), _i(iat), _i(loctmatsize), _p(bl), _p(nm), _p(ims), _p(tg), _p(rr), _p(dj), _p(rt), _p(sg),
+      _p(mu), _i(nat), _i(ndf), _i(bl.shape[0]), _i(dj.shape[0]), _i(rr.shape[1]), _i(tg.size), _i(lam_max))
+    return np.ascontiguousarray(out)
+
+
+def rint13g(rel, a, b, x, y, dh, nrpt, rnot):
+    """radials.f90:140-187."""
+    f = _load("libref_rad.so").rint13g_
+    f.restype = C.c_double
+    F = lambda z: np.ascontiguousarray(np.asarray(z, dtype=np.float64))  # noqa: E731
+    A, B, X, Y = F(a), F(b), F(x), F(y)
+    r = C.c_int(1 if rel else 0)
+    return f(C.byref(r), _p(A), _p(B), _p(X), _p(Y), _d(dh), _i(nrpt), _d(rnot))
+
+
+def spher_bessel(lmax, x):
+    """for_Coulomb.f90 spher_bessel (sphbes.f90): j_l(x), l <= lmax -> [lmax+1, nx]."""
+    f = _load("libref_coul.so").spher_bessel_
+    xx = np.ascontiguousarray(np.asarray(x, dtype=np.float64).ravel())
+    out = np.zeros((lmax + 1, xx.size), dtype=np.float64, order="F")
+    f(_p(out), _i(lmax), _p(xx), _i(xx.size))
+    return np.ascontiguousarray(out)
+
+
+def pw_get_short_and_phase(glen, gindex, pos, mult):
+    """for_Coulomb.f90:493-582: the unique |G| (G = 0 dropped) and
+    phase[idf, jdf, igl] = sum over G of that length of exp(-2 pi i G.(pos_i - pos_j))."""
+    lib = _load("libref_coul.so")
+    gl = np.ascontiguousarray(np.asarray(glen, dtype=np.float64))
+    nk = gl.size
+    ngs, gind4 = C.c_int(0), np.zeros(nk, dtype=np.int32)
+    lib.pw_get_short_length_(C.byref(ngs), _p(gind4), _p(gl), _i(nk))
+    ngs = ngs.value
+    ps = np.asfortranarray(np.asarray(pos, dtype=np.float64))
+    mu = np.ascontiguousarray(np.asarray(mult, dtype=np.int32))
+    gx = np.asfortranarray(np.asarray(gindex, dtype=np.int32))
+    ndf = ps.shape[1]
+    glen0 = np.zeros(ngs, dtype=np.float64)
+    phase = np.zeros((ndf, ndf, ngs), dtype=np.complex128, order="F")
+    lib.pw_get_short_and_phase_(_p(glen0), _p(phase), _i(ngs), _p(gind4), _p(gx), _p(gl), _p(ps), _p(mu), _i(nk),
+                                _i(mu.size), _i(ndf))
+    return glen0, np.ascontiguousarray(phase), gind4

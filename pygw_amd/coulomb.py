@@ -236,3 +236,209 @@ def coul_mixed_block(mpwipw, vmatit, vmat):
     _lib.call("pygw_coul_mixed_block", C.c_void_p(mpwipw.data_ptr()), ngq, nb, C.c_void_p(vmatit.data_ptr()), loct,
               C.c_void_p(vmat.data_ptr()), vmat.shape[1], _stream_ptr(mpwipw.device))
     return vmat
+
+
+# --- the muffin-tin block and the q = 0 term ---------------------------------
+
+def calc_tildeg(lmax):
+    """calc_tildeg (pygw.py:1938-1962): the packed table of
+    tilde g(l1, l2, m1, m2) = (-1)^l1 8 pi^(3/2) / sqrt((2 l1+1)(2 l2+1)(2(l1+l2)+1))
+                              sqrt(C(l1+m1+l2+m2, l1+m1) C(l1-m1+l2-m2, l1-m1))
+    for l2 <= l1 <= lmax, -l1 <= m1 <= l1, 0 <= m2 <= l2 (that loop order);
+    (lmax+1)(lmax+2)(lmax+3)(3 lmax+2)/12 entries, read by MT_Coulomb's
+    gettildeg.  Host setup, as in the reference."""
+    from math import comb
+    out = []
+    for l1 in range(lmax + 1):
+        p0 = (-1) ** l1 * 8.0 * np.pi * np.sqrt(np.pi)
+        for l2 in range(l1 + 1):
+            p1 = p0 / np.sqrt((2 * l1 + 1) * (2 * l2 + 1) * (2 * (l1 + l2) + 1))
+            for m1 in range(-l1, l1 + 1):
+                for m2 in range(l2 + 1):
+                    out.append(p1 * np.sqrt(float(comb(l1 + m1 + l2 + m2, l1 + m1) * comb(l1 - m1 + l2 - m2, l1 - m1))))
+    return np.array(out)
+
+
+def _fortran_flat(x, dtype, device):
+    return _dev(np.asarray(x).ravel(order="F"), dtype, device)
+
+
+def ewald_summation(pq, lam_max, pos, br2, rbas, alat, vol, ortho, r_cf, g_cf, eta, *, out=None, device="cuda"):
+    """fCl.ewald_summation (pygw.py:2350, for_Coulomb.f90:1-164): the
+    structure constants sgm, complex128 [(lam_max+1)^2, ndf(ndf+1)/2] (a
+    Fortran-strided view; column i + j(j+1)/2 holds the atom pair i <= j).
+    pq: cartesian q; pos: [3, ndf] fractional positions (strc.vpos); br2:
+    reciprocal vectors as columns; rbas: lattice vectors as rows."""
+    device = _require_device(device)
+    pos = np.asarray(pos, dtype=np.float64)
+    ndf = pos.shape[1]
+    nlm, npair = (lam_max + 1) ** 2, ndf * (ndf + 1) // 2
+    if out is None:
+        out = torch.empty((npair, nlm), dtype=torch.complex128, device=device)
+    elif out.dtype != torch.complex128 or not out.is_contiguous() or tuple(out.shape) != (npair, nlm):
+        raise ValueError("out must be a contiguous complex128 [npair, (lam_max+1)^2] tensor")
+    t = [_fortran_flat(x, torch.float64, device) for x in (pq, pos, br2, rbas, alat)]
+    _lib.call("pygw_ewald_summation", C.c_void_p(t[0].data_ptr()), int(lam_max), C.c_void_p(t[1].data_ptr()),
+              C.c_void_p(t[2].data_ptr()), C.c_void_p(t[3].data_ptr()), C.c_void_p(t[4].data_ptr()), float(vol),
+              int(bool(ortho)), float(r_cf), float(g_cf), float(eta), ndf, C.c_void_p(out.data_ptr()),
+              _stream_ptr(device))
+    return out.T
+
+
+def mt_coulomb(vq, iat, loctmatsize, big_l, nmix, im_start, tilg, rrint, djmm, rtlij, sgm, mult, lam_max, *,
+               device="cuda"):
+    """fCl.mt_coulomb (pygw.py:2394, for_Coulomb.f90:166-283) with iat
+    0-based: the rows of atom type iat's product functions of the
+    muffin-tin Coulomb matrix, complex128 [loctmatsize, loctmatsize] (a
+    Fortran-strided view, every other row 0); iat=None gives every row at
+    once, the sum over iat that the reference accumulates.  big_l
+    [max_nmix, nat] and rtlij [max_nmix, max_nmix, nat, nat] as PyGW builds
+    them (pygw.py:2361-2374), rrint [nat, nijrm], djmm [dimdj, ndf], sgm
+    from ewald_summation (lam_max >= 2 max L)."""
+    device = _require_device(device)
+    big_l = np.asarray(big_l)
+    max_nmix, nat = big_l.shape
+    ims = np.ascontiguousarray(np.asarray(im_start), dtype=np.int32)
+    if ims.shape != (nat + 1,):
+        raise ValueError("im_start must hold nat + 1 offsets")
+    sg = sgm if isinstance(sgm, torch.Tensor) else torch.as_tensor(np.asarray(sgm))
+    if tuple(sg.shape)[0] != (lam_max + 1) ** 2:
+        raise ValueError("sgm must have (lam_max+1)^2 rows")
+    sg = sg.to(device=device, dtype=torch.complex128).T.contiguous()   # Fortran (nlm, npair)
+    djmm = np.asarray(djmm)
+    nm_h = np.asarray(nmix)
+    if nm_h.shape != (nat,) or nm_h.min() < 1 or nm_h.max() > max_nmix:
+        raise ValueError("nmix must give 1..max_nmix functions per atom type")
+    if np.shape(rrint) != (nat, max_nmix * (max_nmix + 1) // 2) or np.shape(rtlij) != (max_nmix, max_nmix, nat, nat):
+        raise ValueError("rrint must be [nat, nijrm] and rtlij [max_nmix, max_nmix, nat, nat]")
+    if djmm.ndim != 2 or djmm.shape[1] != int(np.sum(mult)):
+        raise ValueError("djmm must be [dimdj, ndf]")
+    per = np.array([np.sum(2 * big_l[: nm_h[a], a] + 1) for a in range(nat)]) * np.asarray(mult)
+    if not np.array_equal(np.diff(ims), per) or ims[0] != 0 or ims[-1] != loctmatsize:
+        raise ValueError("im_start does not match big_l, nmix and mult")
+    i32 = lambda x: _fortran_flat(x, torch.int32, device)  # noqa: E731
+    f64 = lambda x: _fortran_flat(x, torch.float64, device)  # noqa: E731
+    bl, nm, imd, mu = i32(big_l), i32(nmix), i32(ims), i32(mult)
+    tg, rr, rt, vqd = f64(tilg), f64(rrint), f64(rtlij), f64(vq)
+    dj = _fortran_flat(djmm, torch.complex128, device)
+    out = torch.empty((loctmatsize, loctmatsize), dtype=torch.complex128, device=device)
+    _lib.call("pygw_mt_coulomb", C.c_void_p(vqd.data_ptr()), -1 if iat is None else int(iat), int(loctmatsize),
+              C.c_void_p(bl.data_ptr()), C.c_void_p(nm.data_ptr()), C.c_void_p(imd.data_ptr()),
+              ims.ctypes.data_as(C.c_void_p), C.c_void_p(mu.data_ptr()), nat, max_nmix, C.c_void_p(tg.data_ptr()),
+              tg.numel(), C.c_void_p(rr.data_ptr()), C.c_void_p(dj.data_ptr()), djmm.shape[0], C.c_void_p(rt.data_ptr()),
+              C.c_void_p(sg.data_ptr()), int(lam_max), C.c_void_p(out.data_ptr()), _stream_ptr(device))
+    return out.T
+
+
+def radial_mesh(r0, dh, npt):
+    """strc.radial_mesh (gwienfile.py:218-222): r0 e^{dh j} as numpy forms it."""
+    return r0 * np.exp(dh) ** np.arange(npt)
+
+
+def radial_olap(mode, rel, ul, us, big_l, rx, dh, k, *, lmax=None, device="cuda"):
+    """out[f, ik] = rd.rint13g(rel, ul[f], us[f], F, F, dh, npt, rx[0]) with
+    F = j_L(k rx) rx, L = big_l[f] (mode 0) or F = sin(k rx) (mode 1) on the
+    logarithmic mesh rx (radials.f90:140-187); float64 [n_fun, n_k]."""
+    device = _require_device(device)
+    u = _dev(ul, torch.float64, device).contiguous()
+    s = _dev(us, torch.float64, device).contiguous()
+    r = _dev(rx, torch.float64, device).contiguous()
+    npt = r.numel()
+    lf = np.ascontiguousarray(np.asarray(big_l), dtype=np.int32)
+    if u.shape != s.shape or u.dim() != 2 or u.shape[0] != lf.size or u.shape[1] != npt:
+        raise ValueError("ul and us must be [n_fun, npt] with one L per function")
+    if mode == 1:
+        lmax = 0
+    elif lmax is None:
+        lmax = int(lf.max())
+    kk = _dev(k, torch.float64, device).contiguous()
+    lfd = _dev(lf, torch.int32, device)
+    out = torch.empty((lf.size, kk.numel()), dtype=torch.float64, device=device)
+    sz = C.c_size_t(0)
+    _lib.call("pygw_radial_olap_workspace", int(mode), int(lmax), int(npt), kk.numel(), C.byref(sz))
+    ws = torch.empty(sz.value, dtype=torch.uint8, device=device)
+    _lib.call("pygw_radial_olap", int(mode), int(bool(rel)), C.c_void_p(u.data_ptr()), C.c_void_p(s.data_ptr()),
+              C.c_void_p(lfd.data_ptr()), lf.size, int(lmax), int(npt), C.c_void_p(r.data_ptr()), float(dh),
+              C.c_void_p(kk.data_ptr()), kk.numel(), C.c_void_p(out.data_ptr()), C.c_void_p(ws.data_ptr()),
+              ws.numel(), _stream_ptr(device))
+    return out
+
+
+def calc_olap_pw_product(gqlen, Lmax, rx, dh, rel, big_l, ul_product, us_product, *, device="cuda"):
+    """Calc_Olap_PW_Product (pygw.py:2110-2124) for one atom type on its mesh
+    rx: jlam[irm, ig] = <j_L(|q+G| r) r | u_irm> by RINT13g, float64
+    [nmix, n_g]."""
+    return radial_olap(0, rel, ul_product, us_product, big_l, rx, dh, gqlen, lmax=Lmax, device=device)
+
+
+def pw_get_short_length(glen):
+    """pw_get_short_length (for_Coulomb.f90:493-513): ngs and gind4[ipw]
+    (1-based group of ipw; runs of lengths within 1e-10 of the run's first,
+    the leading G = 0 in group 0).  Host setup."""
+    glen = np.asarray(glen, dtype=np.float64)
+    gind4 = np.zeros(glen.size, dtype=np.int64)
+    igl, prev = 0, 0.0
+    for i, g in enumerate(glen):
+        if abs(g - prev) > 1e-10:
+            igl += 1
+            prev = g
+        gind4[i] = igl
+    return igl, gind4
+
+
+def zero_momentum_coulomb(glen, gindex, pos, vol, mbsize, rel, meshes, big_l, ul_product, us_product, atm, iprod,
+                          *, out=None, device="cuda"):
+    """Zero_Momentum_Coulomb (pygw.py:2055-2104) after its plane-wave
+    generation: glen and gindex [n_g, 3] the plane waves sorted by length
+    (fkp.generate_pw), pos [3, ndf] (strc.vpos), meshes[iat] = (r0, dh, npt),
+    big_l / ul_product / us_product per atom type, atm[idf] the type of atom
+    idf, iprod[(idf, irm)] the Vmat index of the L = 0 function irm of atom
+    idf.  Returns complex128 [mbsize, mbsize] (a Fortran-strided view, or
+    written into the Fortran-strided `out`) with the L = 0 x L = 0 entries
+    set and every other entry 0."""
+    device = _require_device(device)
+    glen = np.asarray(glen, dtype=np.float64)
+    ngs, gind4 = pw_get_short_length(glen)
+    if ngs == 0:
+        raise ValueError("no plane wave of nonzero length")
+    # the runs of pw_get_short_and_phase: plane waves 1.. (the first, G = 0, skipped)
+    g_start = np.searchsorted(gind4[1:], np.arange(1, ngs + 2)) + 1
+    first = np.searchsorted(gind4, np.arange(1, ngs + 1))
+    glen0 = glen[first]
+    sing_rows, f_sing, f_atom, f_row = [], [], [], []
+    row_of = {}
+    for iat in range(len(big_l)):
+        bl = np.asarray(big_l[iat])
+        l0 = np.flatnonzero(bl == 0)
+        if l0.size == 0:
+            continue
+        r0, dh, npt = meshes[iat]
+        s = radial_olap(1, rel, np.asarray(ul_product[iat])[l0], np.asarray(us_product[iat])[l0], np.zeros(l0.size),
+                        radial_mesh(r0, dh, npt), dh, glen0, device=device)
+        for a, irm in enumerate(l0):
+            row_of[(iat, int(irm))] = len(sing_rows)
+            sing_rows.append(s[a])
+    for idf, iat in enumerate(atm):
+        for irm in np.flatnonzero(np.asarray(big_l[iat]) == 0):
+            f_sing.append(row_of[(iat, int(irm))])
+            f_atom.append(idf)
+            f_row.append(int(iprod[(idf, int(irm))]))
+    sing = torch.stack(sing_rows).contiguous()
+    gx = _fortran_flat(np.asarray(gindex), torch.int32, device)
+    i32 = lambda x: _dev(np.asarray(x), torch.int32, device).contiguous()  # noqa: E731
+    gs, fs, fa, fr = i32(g_start), i32(f_sing), i32(f_atom), i32(f_row)
+    g0 = _dev(glen0, torch.float64, device)
+    ps = _fortran_flat(pos, torch.float64, device)
+    ndf = np.asarray(pos).shape[1]
+    if out is None:
+        out = torch.zeros((mbsize, mbsize), dtype=torch.complex128, device=device).T
+    elif out.dtype != torch.complex128 or out.stride() != (1, mbsize) or tuple(out.shape) != (mbsize, mbsize):
+        raise ValueError("out must be a Fortran-strided complex128 [mbsize, mbsize] view")
+    sz = C.c_size_t(0)
+    _lib.call("pygw_zero_momentum_workspace", ndf, ngs, C.byref(sz))
+    ws = torch.empty(sz.value, dtype=torch.uint8, device=device)
+    _lib.call("pygw_zero_momentum_coulomb", C.c_void_p(gx.data_ptr()), len(glen), C.c_void_p(gs.data_ptr()),
+              C.c_void_p(g0.data_ptr()), ngs, C.c_void_p(ps.data_ptr()), ndf, C.c_void_p(sing.data_ptr()),
+              sing.shape[0], C.c_void_p(fs.data_ptr()), C.c_void_p(fa.data_ptr()), C.c_void_p(fr.data_ptr()), len(f_row), float(vol),
+              int(mbsize), C.c_void_p(out.data_ptr()), C.c_void_p(ws.data_ptr()), ws.numel(), _stream_ptr(device))
+    return out

@@ -542,3 +542,121 @@ def mixed_system(ngq_barc=120, q=(0.25, 0.0, 0.125), a=10.0, gmax=6, seed=3) -> 
                        rotloc=np.stack([_rotation(rng) for _ in range(2)]),
                        trotij=np.stack([_rotation(rng) for _ in range(3)]), mult=mult,
                        pos=rng.uniform(0, 1, (3, 3)), vol=a ** 3, ngq_barc=ngq_barc)
+
+
+@dataclass
+class MTSystem:
+    """Inputs of Ewald_Summation, MT_Coulomb, Calc_Olap_PW_Product and
+    Zero_Momentum_Coulomb (for_Coulomb.f90:1-283, pygw.py:2055-2124) for a
+    cell of two inequivalent atoms (mult 1 and 2): the lattice (rbas rows,
+    br2 columns), positions [3, ndf], product functions of L <= lmax_b per
+    atom type with their radial functions on each type's logarithmic mesh
+    (r0, dh, npt), the packed tables (tilg, djmm), rrint, rtlij, Ewald
+    parameters and the length-sorted plane waves of the q = 0 term."""
+    rbas: np.ndarray
+    br2: np.ndarray
+    alat: np.ndarray
+    vol: float
+    ortho: bool
+    pos: np.ndarray
+    mult: np.ndarray
+    atm: list
+    pq: np.ndarray
+    eta: float
+    r_cf: float
+    g_cf: float
+    lam_max: int
+    big_l: list
+    big_l_f: np.ndarray
+    nmix: np.ndarray
+    im_start: np.ndarray
+    loctmatsize: int
+    iprod: dict
+    tilg: np.ndarray
+    rrint: np.ndarray
+    djmm: np.ndarray
+    rtlij: np.ndarray
+    meshes: list
+    ul: list
+    us: list
+    rel: bool
+    glen: np.ndarray
+    gindex: np.ndarray
+
+
+def mt_system(q=(0.11, -0.07, 0.05), ortho=False, eta=1.6, rcut=7.0, gcut=14.0, lmax_b=3, gmax=2.5,
+              identity_rotations=False, seed=13) -> MTSystem:
+    """r_cf = rcut eta and g_cf = gcut / eta.  identity_rotations: djmm of the
+    identity (every local frame aligned); else random complex entries."""
+    from .coulomb import calc_tildeg
+    rng = np.random.default_rng(seed)
+    if ortho:
+        alat = np.array([5.2, 5.9, 7.1])
+        rbas = np.diag(alat)
+    else:
+        a, c = 5.4, 8.3
+        alat = np.array([a, a, c])
+        rbas = np.array([[a, 0.0, 0.0], [-a / 2, a * np.sqrt(3) / 2, 0.0], [0.0, 0.0, c]])
+    vol = abs(np.linalg.det(rbas))
+    br2 = 2 * np.pi * np.linalg.inv(rbas)
+    mult = np.array([1, 2])
+    atm = [0, 1, 1]
+    pos = np.array([[0.0, 0.0, 0.0], [1 / 3, 2 / 3, 0.27], [2 / 3, 1 / 3, 0.77]]).T
+    big_l = [np.array([0, 0, 1, 1, 2, 3])[: 4 + (lmax_b >= 3) * 2], np.array([0, 1, 2, 2, 3, 0])]
+    big_l = [b[b <= lmax_b] for b in big_l]
+    nat = len(mult)
+    nmix = np.array([len(b) for b in big_l])
+    max_nmix = int(nmix.max())
+    big_l_f = np.zeros((max_nmix, nat), dtype=np.int64)
+    im_start = np.zeros(nat + 1, dtype=np.int64)
+    iprod = {}
+    im, idf = 0, 0
+    for iat in range(nat):
+        im_start[iat] = im
+        big_l_f[: nmix[iat], iat] = big_l[iat]
+        for _ in range(mult[iat]):
+            for irm, L in enumerate(big_l[iat]):
+                if L == 0:
+                    iprod[(idf, irm)] = im
+                im += 2 * L + 1
+            idf += 1
+    im_start[nat] = im
+    lmaxt = 2 * (lmax_b + 1)
+    tilg = calc_tildeg(lmaxt)
+    dimdj = sum((2 * l + 1) * (l + 1) for l in range(lmax_b + 1))
+    ndf = int(mult.sum())
+    if identity_rotations:
+        djmm = np.zeros((dimdj, ndf), dtype=complex)
+        for l in range(lmax_b + 1):
+            for m in range(0, l + 1):
+                djmm[l * (l + 1) * (4 * l - 1) // 6 + (l + 1) * (m + l) + m, :] = 1.0
+    else:
+        djmm = rng.normal(size=(dimdj, ndf)) + 1j * rng.normal(size=(dimdj, ndf))
+    nij = max_nmix * (max_nmix + 1) // 2
+    rrint = rng.uniform(0.1, 1.0, (nat, nij))
+    rtl = rng.uniform(0.2, 1.5, (nat, max_nmix))
+    rtlij = np.zeros((max_nmix, max_nmix, nat, nat))
+    for iat in range(nat):
+        for jat in range(nat):
+            rtlij[: nmix[jat], : nmix[iat], jat, iat] = np.outer(rtl[jat, : nmix[jat]], rtl[iat, : nmix[iat]])
+    # logarithmic meshes (odd and even npt) and smooth radial functions r u, r us
+    meshes, ul, us = [], [], []
+    for iat, (r0, rmt, npt) in enumerate(((1e-4, 2.1, 781), (5e-5, 1.9, 640))):
+        dh = np.log(rmt / r0) / (npt - 1)
+        meshes.append((r0, dh, npt))
+        rx = r0 * np.exp(dh) ** np.arange(npt)
+        c = rng.uniform(0.5, 2.0, (len(big_l[iat]), 3))
+        ul.append(np.array([rx ** (L + 1) * (c[i, 0] + c[i, 1] * rx) * np.exp(-c[i, 2] * rx)
+                            for i, L in enumerate(big_l[iat])]))
+        us.append(0.3 * ul[-1] * np.cos(rx)[None, :])
+    r = np.arange(-12, 13)
+    g = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    ln = np.linalg.norm(g @ br2.T, axis=1)
+    keep = ln <= gmax * 2 * np.pi / 2.0
+    g, ln = g[keep], ln[keep]
+    order = np.lexsort((g[:, 2], g[:, 1], g[:, 0], np.round(ln, 10)))
+    return MTSystem(rbas=rbas, br2=br2, alat=alat, vol=vol, ortho=ortho, pos=pos, mult=mult, atm=atm,
+                    pq=np.asarray(q, dtype=float), eta=eta, r_cf=rcut * eta, g_cf=gcut / eta,
+                    lam_max=2 * lmax_b, big_l=big_l, big_l_f=big_l_f, nmix=nmix, im_start=im_start,
+                    loctmatsize=int(im), iprod=iprod, tilg=tilg, rrint=rrint, djmm=djmm, rtlij=rtlij,
+                    meshes=meshes, ul=ul, us=us, rel=True, glen=ln[order], gindex=g[order])

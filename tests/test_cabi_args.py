@@ -41,6 +41,10 @@ def _zeros(fn):
     "pygw_coul_setev", "pygw_coul_setev_workspace", "pygw_minm_vhalf", "pygw_minm_is",
     "pygw_minm_is_workspace", "pygw_pw_basis", "pygw_pw_basis_workspace", "pygw_coul_pw_block",
     "pygw_mixed_coulomb", "pygw_mixed_coulomb_workspace", "pygw_coul_mixed_block",
+    "pygw_omega_basis_workspace", "pygw_omega_basis", "pygw_omega_basis_many_workspace", "pygw_omega_basis_many",
+    "pygw_omega_project", "pygw_omega_expand", "pygw_omega_expand_batched",
+    "pygw_ewald_summation", "pygw_mt_coulomb", "pygw_radial_olap_workspace", "pygw_radial_olap",
+    "pygw_zero_momentum_workspace", "pygw_zero_momentum_coulomb",
 ])
 def test_null_and_zero_arguments_rejected(name):
     fn = _fn(name)
@@ -109,3 +113,28 @@ def test_small_workspaces_rejected_before_any_device_call():
     umk = (C.c_int * 3)(0, 0, 0)
     assert f(0, 4, 0, 8, p, 10, 50, p, 10, 50, umk, p, 30, 40, 50, 50, p, p, p, 100, p, 1000, p, 3, 3, 3, 100.0, p,
              30, 0, p, 64, None) == E_ARG
+
+
+def test_muffin_tin_coulomb_workspaces_and_bounds():
+    s = C.c_size_t(0)
+    q = _fn("pygw_radial_olap_workspace")
+    # the table of j_L(k r) r, L <= lmax (mode 0), or of sin(k r) (mode 1, lmax 0)
+    assert q(0, 3, 781, 100, C.byref(s)) == 0 and s.value == 4 * 781 * 100 * 8
+    assert q(1, 0, 640, 7, C.byref(s)) == 0 and s.value == 640 * 7 * 8
+    assert q(1, 1, 640, 7, C.byref(s)) == E_ARG
+    assert q(0, 2, 3, 7, C.byref(s)) == E_ARG      # RINT13g needs 4 points
+    q = _fn("pygw_zero_momentum_workspace")
+    assert q(3, 40, C.byref(s)) == 0 and s.value == 3 * 3 * 40 * 16
+    p = C.c_void_p(16)     # never dereferenced: the checks come first
+    ew = _fn("pygw_ewald_summation")
+    # lam_max above the table of l, a bad ortho flag, eta 0
+    assert ew(p, 17, p, p, p, p, 100.0, 0, 10.0, 10.0, 1.0, 3, p, None) == E_ARG
+    assert ew(p, 8, p, p, p, p, 100.0, 2, 10.0, 10.0, 1.0, 3, p, None) == E_ARG
+    assert ew(p, 8, p, p, p, p, 100.0, 0, 10.0, 10.0, 0.0, 3, p, None) == E_ARG
+    mt = _fn("pygw_mt_coulomb")
+    ims = (C.c_int * 3)(0, 20, 64)
+    # iat outside [-1, nat), rows past loctmatsize
+    assert mt(p, 2, 64, p, p, p, ims, p, 2, 6, p, 2145, p, p, 50, p, p, 6, p, None) == E_ARG
+    assert mt(p, 1, 60, p, p, p, ims, p, 2, 6, p, 2145, p, p, 50, p, p, 6, p, None) == E_ARG
+    ro = _fn("pygw_radial_olap")
+    assert ro(0, 1, p, p, p, 4, 3, 781, p, 0.01, p, 100, p, p, 1024, None) == E_ARG   # workspace too small
