@@ -19,7 +19,7 @@ the q-list.  value = split-precision seconds per q when every q passes
 (chi0 and eps^-1 within 1e-6, Sigma_c within 1e-5, relative Frobenius per
 frequency), else the FP64 seconds per q.
 
-    python bench.py [--steps K] [--warmup W] [--config c5|c2|c3|c4|gw0|gamma|coulomb|minm_is] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--config c5|c2|c3|c4|gw0|gamma|coulomb|minm_is|mt_coulomb] [--no-cpu]
 
 --config c2 / c3: chi0 + eps^-1 only (FP64 and split precision) at
 8x8x8 k, 16+48 bands, PB = 600 / 10x10x10 k, 32+200 bands, PB = 1200.
@@ -28,6 +28,7 @@ windows) through the one-launch batched chi0, then eps^-1 per q.
 --config gw0 / gamma: the GW0 iteration on stored mwm / the q = 0 extras.
 --config coulomb: the v^(1/2) basis (Coul_setev) and minm = mmat conj(barcvm).
 --config minm_is: the interstitial part of mmat (calc_minm_IS) over all k of a q.
+--config mt_coulomb: the muffin-tin Coulomb block of a q (Ewald, jlam, MT_Coulomb).
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs one GPU.
 """
 from __future__ import annotations
@@ -1104,6 +1105,111 @@ def coulomb_cpu_baseline(vmat, mm_sample, evtol, scale, t_gpu_ev):
             "stages_s": {"coul_setev": t1 - t0, "minm_extrapolated": (t2 - t1) * scale}}
 
 
+def _unique_lengths(v):
+    """Lengths of the vectors v [n, 3] in runs equal within 1e-10 (first of each run)."""
+    ln = np.sort(np.linalg.norm(v, axis=1))
+    return ln[np.r_[True, np.diff(ln) > 1e-10]]
+
+
+def run_mt_coulomb(args):
+    """The muffin-tin block of the Coulomb matrix of one q at PyGW's sizes
+    (pygw.py:2348-2394): Ewald structure constants to lam_max = 16 for 8
+    atoms (36 pairs), the radial overlaps jlam of every atom type against
+    j_L(|q+G| r) r for the unique |q+G| (Calc_Olap_PW_Product), and
+    MT_Coulomb over all rows (loctmatsize 896); the q = 0 term
+    (Zero_Momentum_Coulomb) timed beside it."""
+    import torch
+
+    from pygw_amd import coulomb, synth
+    S = synth.mt_bench_system()
+    G = S.gindex[S.glen <= 5.0] @ S.br2.T
+    gq = _unique_lengths(G + S.pq[None, :])
+    lmaxb = int(max(int(b.max()) for b in S.big_l))
+    meshes = [coulomb.radial_mesh(*m) for m in S.meshes]
+    rx_d = [torch.as_tensor(r, device="cuda") for r in meshes]
+    ul_d = [torch.as_tensor(u, device="cuda") for u in S.ul]
+    us_d = [torch.as_tensor(u, device="cuda") for u in S.us]
+    gq_d = torch.as_tensor(gq, device="cuda")
+    s = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    mt_args = (S.loctmatsize, S.big_l_f, S.nmix, S.im_start, S.tilg, S.rrint, S.djmm, S.rtlij)
+
+    def step():
+        ev[0].record(s)
+        sgm = coulomb.ewald_summation(S.pq, S.lam_max, S.pos, S.br2, S.rbas, S.alat, S.vol, S.ortho, S.r_cf, S.g_cf,
+                                      S.eta)
+        ev[1].record(s)
+        jl = [coulomb.calc_olap_pw_product(gq_d, lmaxb, rx_d[a], S.meshes[a][1], S.rel, S.big_l[a], ul_d[a],
+                                           us_d[a]) for a in range(len(S.big_l))]
+        ev[2].record(s)
+        V = coulomb.mt_coulomb(S.pq, None, *mt_args, sgm, S.mult, S.lam_max)
+        ev[3].record(s)
+        torch.cuda.synchronize()
+        return np.array([ev[i].elapsed_time(ev[i + 1]) / 1e3 for i in range(3)]), (sgm, jl, V)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    st = np.zeros(3)
+    for _ in range(args.steps):
+        t, out = step()
+        st += t
+    st /= args.steps
+    t_ew, t_jl, t_mt = st
+    zm_args = (S.glen, S.gindex, S.pos, S.vol, S.loctmatsize + 500, S.rel, S.meshes, S.big_l, S.ul, S.us, S.atm,
+               S.iprod)
+    coulomb.zero_momentum_coulomb(*zm_args)
+    torch.cuda.synchronize()
+    tz = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        coulomb.zero_momentum_coulomb(*zm_args)
+        torch.cuda.synchronize()
+        tz.append(time.perf_counter() - t0)
+    value = float(st.sum())
+    print(f"[bench] mt_coulomb ewald {t_ew * 1e3:.2f} ms, jlam {t_jl * 1e3:.2f} ms ({len(gq)} |q+G|), "
+          f"mt {t_mt * 1e3:.2f} ms, zero momentum {np.mean(tz) * 1e3:.2f} ms", file=sys.stderr, flush=True)
+    rec = {"metric": "muffin-tin Coulomb block (Ewald + jlam + MT_Coulomb) seconds per q", "value": value,
+           "unit": "s/q", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": value * 1e3,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"mt_coulomb: 8 atoms (4 types x mult 2), lam_max {S.lam_max}, loctmatsize "
+                                  f"{S.loctmatsize}, {len(gq)} unique |q+G|, 781-point meshes, Lmax {lmaxb}"},
+           "stages_s": {"ewald_summation": t_ew, "calc_olap_pw_product_all_atoms": t_jl, "mt_coulomb": t_mt,
+                        "zero_momentum_coulomb_q0_host_timed": float(np.mean(tz))}}
+    if not args.no_cpu:
+        rec["cpu_baseline"] = mt_coulomb_cpu_baseline(S, gq, meshes, lmaxb, value)
+    print(json.dumps(rec))
+
+
+def mt_coulomb_cpu_baseline(S, gq, meshes, lmaxb, t_gpu):
+    """PyGW's own sequence on the host with its compiled Fortran
+    (oracle/_ref): fCl.ewald_summation, fCl.spher_bessel + rd.rint13g per
+    (function, |q+G|) as Calc_Olap_PW_Product calls them, fCl.mt_coulomb per
+    atom type."""
+    from oracle import ref
+    if not (ref.available("libref_coul.so") and ref.available("libref_rad.so")):
+        return None
+    t0 = time.perf_counter()
+    sgm = ref.ewald_summation(S.pq, S.lam_max, S.pos, S.br2, S.rbas, S.alat, S.vol, S.ortho, S.r_cf, S.g_cf, S.eta)
+    t1 = time.perf_counter()
+    for a in range(len(S.big_l)):
+        rx = meshes[a]
+        r0, dh, npt = S.meshes[a]
+        jln = ref.spher_bessel(lmaxb, np.outer(gq, rx).ravel()).reshape(lmaxb + 1, len(gq), npt) * rx
+        for i, L in enumerate(S.big_l[a]):
+            for g in range(len(gq)):
+                ref.rint13g(S.rel, S.ul[a][i], S.us[a][i], jln[L, g], jln[L, g], dh, npt, r0)
+    t2 = time.perf_counter()
+    for a in range(len(S.big_l)):
+        ref.mt_coulomb(S.pq, a + 1, S.loctmatsize, S.big_l_f, S.nmix, S.im_start, S.tilg, S.rrint, S.djmm, S.rtlij,
+                       sgm, S.mult, S.lam_max)
+    t3 = time.perf_counter()
+    return {"value": t3 - t0, "unit": "s/q", "cores": 1, "kind": "reference", **host_info(),
+            "sample": "the whole q: the reference's compiled Ewald_Summation, spher_bessel, RINT13g and MT_Coulomb "
+                      "(oracle/_ref), called in PyGW's order from numpy",
+            "stages_s": {"ewald_summation": t1 - t0, "calc_olap_pw_product_all_atoms": t2 - t1,
+                         "mt_coulomb": t3 - t2}}
+
+
 def run_minm_is(args):
     """The interstitial matrix elements of one q (calc_minm_IS per k,
     pygw.py:2661): 512 k, LAPW bases of 600 plane waves at k and k - q,
@@ -1177,7 +1283,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.steps is None:
-        args.steps = {"c5": 8, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2, "coulomb": 2, "minm_is": 2}.get(args.config, 1)
+        args.steps = {"c5": 8, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2, "coulomb": 2, "minm_is": 2, "mt_coulomb": 3}.get(args.config, 1)
     if args.copy_calibration:
         # one copy kernel that reads and writes exactly 1 GiB (one launch: under
         # 2 GiB), its source evicted from the 256 MB last-level cache by a 2 GiB
@@ -1207,6 +1313,9 @@ def main():
         return
     if args.config == "minm_is":
         run_minm_is(args)
+        return
+    if args.config == "mt_coulomb":
+        run_mt_coulomb(args)
         return
     mixed = not args.no_mixed
     wl = Workload(args.config, args.nq, mixed, args.fp64_kfrac)

@@ -464,12 +464,16 @@ int pygw_coul_mixed_block(const void* mpwipw, int ngq, int ngq_barc, const void*
  * passes them:
  * pygw_ewald_summation replaces fCl.ewald_summation(cq, lam_max, vpos, br2,
  *   rbas, alat, Vol, ortho, r_cf, g_cf, eta) (pygw.py:2350,
- *   for_Coulomb.f90:1-164): pq [3] cartesian q, pos [3][ndf] (Fortran
- *   pos(3, ndf), fractional), br2 (3,3) (columns: reciprocal vectors), rbas
- *   (3,3) (rows: lattice vectors), alat [3]; ortho 0/1; sgm complex128
- *   Fortran ((lam_max+1)^2, ndf(ndf+1)/2), column i + j(j+1)/2 for atoms
- *   i <= j (0-based).  lam_max <= PYGW_EWALD_MAX_LAM; r_cf and g_cf below
- *   64 shortest basis vectors (else NaN).  Allocates nothing.
+ *   for_Coulomb.f90:1-164).  The lattice arguments are in HOST memory, as
+ *   PyGW passes them: pq [3] cartesian q, pos [3][ndf] (Fortran pos(3, ndf),
+ *   fractional), br2 (3,3) (columns: reciprocal vectors), rbas (3,3) (rows:
+ *   lattice vectors), alat [3]; ortho 0/1; r_cf and g_cf below 64 shortest
+ *   basis vectors, lam_max <= PYGW_EWALD_MAX_LAM, ndf <= 1000.  The host lists the lattice
+ *   vectors as genr_str does and uploads them into the workspace
+ *   (pygw_ewald_workspace bytes for the same lattice arguments, or NULL);
+ *   the call synchronises the stream once before it returns.  sgm (device)
+ *   complex128 Fortran ((lam_max+1)^2, ndf(ndf+1)/2), column i + j(j+1)/2
+ *   for atoms i <= j (0-based).
  * pygw_mt_coulomb replaces fCl.mt_coulomb(vq, iat+1, loctmatsize, big_l,
  *   nmix, im_start, tilg, rrint, djmm, rtlij, sgm, mult, lam_max)
  *   (pygw.py:2394, for_Coulomb.f90:166-283): iat 0-based (the rows of that
@@ -490,7 +494,7 @@ int pygw_coul_mixed_block(const void* mpwipw, int ngq, int ngq_barc, const void*
  *   (strc.radial_mesh, rx[0] e^{j dh}), ul, us float64 [n_fun][npt] (r u and
  *   r us, PyGW's ul_product / us_product), lf int32 [n_fun], k [n_k], out
  *   [n_fun][n_k].  workspace: pygw_radial_olap_workspace bytes (the table
- *   of F) or NULL.
+ *   of F and the weights) or NULL.
  * pygw_zero_momentum_coulomb replaces the assembly of Zero_Momentum_Coulomb
  *   (pygw.py:2055-2104) for the L = 0 product functions: the plane waves
  *   gindex int32 (n_g, 3) (Fortran) sorted by length, g_start [n_short+1]
@@ -506,9 +510,11 @@ int pygw_coul_mixed_block(const void* mpwipw, int ngq, int ngq_barc, const void*
  *   (pw_get_short_and_phase, for_Coulomb.f90:515-582).  Only those entries
  *   are written.  workspace: pygw_zero_momentum_workspace bytes or NULL. */
 #define PYGW_EWALD_MAX_LAM 16
+int pygw_ewald_workspace(const double* pq, int lam_max, const double* pos, const double* br2, const double* rbas,
+                         const double* alat, int ortho, double r_cf, double g_cf, int ndf, size_t* bytes);
 int pygw_ewald_summation(const double* pq, int lam_max, const double* pos, const double* br2, const double* rbas,
                          const double* alat, double vol, int ortho, double r_cf, double g_cf, double eta, int ndf,
-                         void* sgm, void* stream);
+                         void* sgm, void* workspace, size_t ws_bytes, void* stream);
 int pygw_mt_coulomb(const double* vq, int iat, int loctmatsize, const int* big_l, const int* nmix,
                     const int* im_start, const int* im_start_host, const int* mult, int nat, int max_nmix,
                     const double* tilg, int tsize, const double* rrint, const void* djmm, int dimdj,

@@ -268,7 +268,8 @@ def ewald_summation(pq, lam_max, pos, br2, rbas, alat, vol, ortho, r_cf, g_cf, e
     structure constants sgm, complex128 [(lam_max+1)^2, ndf(ndf+1)/2] (a
     Fortran-strided view; column i + j(j+1)/2 holds the atom pair i <= j).
     pq: cartesian q; pos: [3, ndf] fractional positions (strc.vpos); br2:
-    reciprocal vectors as columns; rbas: lattice vectors as rows."""
+    reciprocal vectors as columns; rbas: lattice vectors as rows (all host
+    arrays; the call synchronises the current stream)."""
     device = _require_device(device)
     pos = np.asarray(pos, dtype=np.float64)
     ndf = pos.shape[1]
@@ -277,11 +278,15 @@ def ewald_summation(pq, lam_max, pos, br2, rbas, alat, vol, ortho, r_cf, g_cf, e
         out = torch.empty((npair, nlm), dtype=torch.complex128, device=device)
     elif out.dtype != torch.complex128 or not out.is_contiguous() or tuple(out.shape) != (npair, nlm):
         raise ValueError("out must be a contiguous complex128 [npair, (lam_max+1)^2] tensor")
-    t = [_fortran_flat(x, torch.float64, device) for x in (pq, pos, br2, rbas, alat)]
-    _lib.call("pygw_ewald_summation", C.c_void_p(t[0].data_ptr()), int(lam_max), C.c_void_p(t[1].data_ptr()),
-              C.c_void_p(t[2].data_ptr()), C.c_void_p(t[3].data_ptr()), C.c_void_p(t[4].data_ptr()), float(vol),
-              int(bool(ortho)), float(r_cf), float(g_cf), float(eta), ndf, C.c_void_p(out.data_ptr()),
-              _stream_ptr(device))
+    h = [np.ascontiguousarray(np.asarray(x, dtype=np.float64).ravel(order="F")) for x in (pq, pos, br2, rbas, alat)]
+    p = [x.ctypes.data_as(C.c_void_p) for x in h]
+    sz = C.c_size_t(0)
+    _lib.call("pygw_ewald_workspace", p[0], int(lam_max), p[1], p[2], p[3], p[4], int(bool(ortho)), float(r_cf),
+              float(g_cf), ndf, C.byref(sz))
+    ws = torch.empty(sz.value, dtype=torch.uint8, device=device)
+    _lib.call("pygw_ewald_summation", p[0], int(lam_max), p[1], p[2], p[3], p[4], float(vol), int(bool(ortho)),
+              float(r_cf), float(g_cf), float(eta), ndf, C.c_void_p(out.data_ptr()), C.c_void_p(ws.data_ptr()),
+              ws.numel(), _stream_ptr(device))
     return out.T
 
 

@@ -27,7 +27,11 @@
 //
 // Sums run in a fixed order per thread and are reduced in a fixed order, so
 // every result is the same from run to run.
+#include <algorithm>
+#include <array>
 #include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "common.h"
 #include "../../include/pygw_hip.h"
@@ -41,137 +45,186 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kPiF = 3.14159265358979;
 }  // namespace mt
 
-// Y_lm(v) of one (l, m) (Condon-Shortley phase, Y_l,-m = (-1)^m conj(Y_lm)):
-// the normalised associated Legendre recurrences
+// The Ewald sums run over vector lists that the host builds as genr_str
+// does (for_Coulomb.f90:418-491): candidates (i1, i2, i3) in [-imax, imax]^3,
+// imax = int(cutoff / shortest basis vector) + 1, kept for 1e-6 < |v| <=
+// cutoff, ordered by length (ties by generation order).
+//   rvec[4 (roff[pair] + n)] = (R + r_ij, |R + r_ij|), the real-space list of
+//     each atom pair;
+//   gvec[8 n] = (q + G, |q + G|, G), the reciprocal list (the same for every
+//     pair).
+//
+// tab[n][lm] = c_l(v_n) Y_lm(v_n) for every vector of a list, one thread per
+// (vector, m >= 0) that runs l = m .. lam and writes (l, m) and (l, -m):
+//   kind 0 (real space, v = R + r_ij, r = |v|): c_l = Gamma_l(r) e^{-i q.v},
+//     Gamma_0 = sqrt(pi) erfc(r/eta)/r,
+//     Gamma_l = Gamma_(l-1) (l - 1/2)/r + r^(l-2) e^{-r^2/eta^2} / eta^(2l-1);
+//   kind 1 (reciprocal space, v = q + G, g = |v|):
+//     c_l = pref e^{-eta^2 g^2/4} g^-2 (g/2)^l (-i)^l.
+// Y_lm by the normalised associated Legendre recurrences
 //   P_mm = -sqrt((2m+1)/(2m)) sin(t) P_(m-1)(m-1),  P_(m+1)m = sqrt(2m+3) cos(t) P_mm,
 //   P_lm = a_lm (cos(t) P_(l-1)m - b_lm P_(l-2)m),
-// times e^{i m phi}; v = 0 and v on the z axis take phi = 0.
-__device__ double2 ylm_one(const double v[3], int l, int m) {
-  const int am = abs(m);
-  const double rxy = sqrt(v[0] * v[0] + v[1] * v[1]), r = sqrt(rxy * rxy + v[2] * v[2]);
-  const double ct = r > 0.0 ? v[2] / r : 1.0, st = r > 0.0 ? rxy / r : 0.0;
+// times e^{i m phi}, Y_l,-m = (-1)^m conj(Y_lm); v on the z axis takes phi = 0.
+__global__ void __launch_bounds__(256) pygw_ew_tab(int kind, const double* __restrict__ vec, int vstride, int nvec,
+                                                   int lam, const double* __restrict__ pq, double eta, double pref,
+                                                   double2* __restrict__ tab) {
+  using namespace mt;
+  const long e = blockIdx.x * 256L + threadIdx.x;
+  const int m = (int)(e % (lam + 1));
+  const long n = e / (lam + 1);
+  if (n >= nvec) return;
+  const int nlm = (lam + 1) * (lam + 1);
+  const double* v = vec + (size_t)vstride * n;
+  const double r = v[3];
+  const double rxy = sqrt(v[0] * v[0] + v[1] * v[1]);
+  const double ct = v[2] / r, st = rxy / r;
   const double cp = rxy > 0.0 ? v[0] / rxy : 1.0, sp = rxy > 0.0 ? v[1] / rxy : 0.0;
-  double pmm = 1.0 / sqrt(4.0 * mt::kPi), cm = 1.0, sm = 0.0;
-  for (int k = 1; k <= am; k++) {
+  // P_mm and e^{i m phi}
+  double pmm = 1.0 / sqrt(4.0 * kPi), cm = 1.0, sm = 0.0;
+  for (int k = 1; k <= m; k++) {
     pmm *= -sqrt((2.0 * k + 1.0) / (2.0 * k)) * st;
     const double c2 = cm * cp - sm * sp, s2 = sm * cp + cm * sp;
     cm = c2;
     sm = s2;
   }
-  double p = pmm;
-  if (l > am) {
-    double p2 = pmm, p1 = sqrt(2.0 * am + 3.0) * ct * pmm;
-    for (int ll = am + 2; ll <= l; ll++) {
-      const double a = sqrt((4.0 * ll * ll - 1.0) / ((double)ll * ll - (double)am * am));
-      const double bb = sqrt(((ll - 1.0) * (ll - 1.0) - (double)am * am) / (4.0 * (ll - 1.0) * (ll - 1.0) - 1.0));
-      const double pn = a * (ct * p1 - bb * p2);
-      p2 = p1;
-      p1 = pn;
+  // c_l at l = 0, and its step to l + 1
+  double2 ph;        // the l-independent complex factor
+  double cl, gam = 0.0, rpow = 0.0, epow = 0.0, gausr = 0.0;
+  if (kind == 0) {
+    double sq, cq;
+    sincos(-(v[0] * pq[0] + v[1] * pq[1] + v[2] * pq[2]), &sq, &cq);
+    ph = double2{cq, sq};
+    gausr = exp(-r * r / (eta * eta));
+    gam = sqrt(kPiF) * erfc(r / eta) / r;
+    rpow = 1.0 / r;
+    epow = eta;
+    for (int ll = 1; ll <= m; ll++) {
+      gam = gam * (ll - 0.5) / r + rpow * gausr / epow;
+      rpow *= r;
+      epow *= eta * eta;
     }
-    p = p1;
+    cl = gam;
+  } else {
+    ph = double2{1.0, 0.0};
+    cl = pref * exp(-(eta * r) * (eta * r) / 4.0) / (r * r);
+    for (int k = 0; k < m; k++) cl *= 0.5 * r;
   }
-  if (m >= 0) return double2{p * cm, p * sm};
-  const double sg = (am & 1) ? -1.0 : 1.0;
-  return double2{sg * p * cm, -sg * p * sm};
+  double2* out = tab + (size_t)n * nlm;
+  const double sg = (m & 1) ? -1.0 : 1.0;
+  double p2 = 0.0, p1 = pmm;
+  for (int l = m; l <= lam; l++) {
+    double p;
+    if (l == m) {
+      p = pmm;
+    } else {
+      if (l == m + 1) {
+        p = sqrt(2.0 * m + 3.0) * ct * pmm;
+      } else {
+        const double aa = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+        const double bb = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+        p = aa * (ct * p1 - bb * p2);
+      }
+      p2 = p1;
+      p1 = p;
+      if (kind == 0) {
+        gam = gam * (l - 0.5) / r + rpow * gausr / epow;
+        rpow *= r;
+        epow *= eta * eta;
+        cl = gam;
+      } else {
+        cl *= 0.5 * r;
+      }
+    }
+    // z = c_l ph Y_lm (Y_lm = p e^{i m phi}), and for -m: c_l ph (-1)^m conj(p e^{i m phi})
+    double2 f{cl * ph.x, cl * ph.y};
+    if (kind == 1) {  // times (-i)^l
+      switch (l & 3) {
+        case 1: f = double2{f.y, -f.x}; break;
+        case 2: f = double2{-f.x, -f.y}; break;
+        case 3: f = double2{-f.y, f.x}; break;
+        default: break;
+      }
+    }
+    const double yr = p * cm, yi = p * sm;
+    out[l * l + l + m] = double2{f.x * yr - f.y * yi, f.x * yi + f.y * yr};
+    if (m > 0) out[l * l + l - m] = double2{sg * (f.x * yr + f.y * yi), sg * (f.y * yr - f.x * yi)};
+  }
 }
 
-// sgm[lm + nlm*pair] for the atom pair `pair` (i <= j, pair = i + j(j+1)/2) and
-// lm = l*l + l + m: one thread per entry sums the real-space terms, then the
-// reciprocal-space terms, over the candidate lattice points (i1, i2, i3) in
-// [-imax, imax]^3 (i1 slowest; imax = int(cutoff / shortest basis vector) + 1
-// as in genr_str_size) in that fixed order, and divides by Gamma(l + 1/2).
-// The threads of a wave share the pair and so the sequence of points.
-__global__ void __launch_bounds__(64) pygw_ew_sgm(const double* __restrict__ pq, int lam, const double* __restrict__ pos,
-                                                  const double* __restrict__ br2, const double* __restrict__ rbas,
-                                                  const double* __restrict__ alat, double vol, int ortho, double r_cf,
-                                                  double g_cf, double eta, double2* __restrict__ sgm) {
+// eph[pair][n] = e^{i G_n.r_ij}: one thread per entry.
+__global__ void __launch_bounds__(256) pygw_ew_phase(const double* __restrict__ gvec, int ng,
+                                                     const double* __restrict__ raa, int npair,
+                                                     double2* __restrict__ eph) {
+  const long e = blockIdx.x * 256L + threadIdx.x;
+  if (e >= (long)ng * npair) return;
+  const int n = (int)(e % ng), pair = (int)(e / ng);
+  const double* v = gvec + 8 * (size_t)n;
+  double sn, cn;
+  sincos(v[4] * raa[3 * pair] + v[5] * raa[3 * pair + 1] + v[6] * raa[3 * pair + 2], &sn, &cn);
+  eph[e] = double2{cn, sn};
+}
+
+// part[(c * npair + pair) * nlm + lm] = sum over the reciprocal vectors n of
+// chunk c (in list order) of gtab[n][lm] eph[pair][n]: one thread per lm of a
+// block of kEwPairs pairs and a chunk (blockIdx.z); each table entry is read
+// once for the block's pairs, the phases are the same across the block.
+constexpr int kEwPairs = 8;
+__global__ void __launch_bounds__(64) pygw_ew_recip(const double2* __restrict__ gtab, const double2* __restrict__ eph,
+                                                    int ng, int nlm, int npair, int nchunk,
+                                                    double2* __restrict__ part) {
+  const int lm = blockIdx.x * 64 + threadIdx.x, c = blockIdx.z, p0 = blockIdx.y * kEwPairs;
+  const int np = min(kEwPairs, npair - p0);
+  const int n0 = (int)((long)ng * c / nchunk), n1 = (int)((long)ng * (c + 1) / nchunk);
+  if (lm >= nlm) return;
+  double2 s[kEwPairs];
+#pragma unroll
+  for (int q = 0; q < kEwPairs; q++) s[q] = double2{0.0, 0.0};
+  for (int n = n0; n < n1; n++) {
+    const double2 t = gtab[(size_t)n * nlm + lm];
+#pragma unroll
+    for (int q = 0; q < kEwPairs; q++) {
+      if (q < np) {
+        const double2 e = eph[(size_t)(p0 + q) * ng + n];
+        s[q].x += t.x * e.x - t.y * e.y;
+        s[q].y += t.x * e.y + t.y * e.x;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kEwPairs; q++)
+    if (q < np) part[((size_t)c * npair + p0 + q) * nlm + lm] = s[q];
+}
+
+// sgm[lm + nlm pair] = (sum over the pair's real-space vectors of rtab[n][lm]
+// + sum_c part[c]) / Gamma(l + 1/2), minus 1/(eta pi) for l = 0, i = j: one
+// thread per (lm, pair), every sum in list / chunk order.
+__global__ void __launch_bounds__(64) pygw_ew_final(const double2* __restrict__ rtab, const int* __restrict__ roff,
+                                                    const double2* __restrict__ part, int nchunk, int npair, int lam,
+                                                    double eta, double2* __restrict__ sgm) {
   using namespace mt;
   const int nlm = (lam + 1) * (lam + 1);
   const int pair = blockIdx.x, lm = blockIdx.y * 64 + threadIdx.x;
   if (lm >= nlm) return;
-  int j = 0;
-  while ((j + 1) * (j + 2) / 2 <= pair) j++;
-  const int i = pair - j * (j + 1) / 2;
   int l = 0;
   while ((l + 1) * (l + 1) <= lm) l++;
-  const int m = lm - l * l - l;
-  // r_ij in cartesian coordinates (rbas rows are the lattice vectors; arrays Fortran)
-  double d[3], raa[3];
-  for (int x = 0; x < 3; x++) d[x] = pos[x + 3 * i] - pos[x + 3 * j];
-  for (int x = 0; x < 3; x++)
-    raa[x] = ortho ? d[x] * alat[x] : d[0] * rbas[0 + 3 * x] + d[1] * rbas[1 + 3 * x] + d[2] * rbas[2 + 3 * x];
-  double lr = 1e300, lg = 1e300;
-  for (int k = 0; k < 3; k++) {
-    double vr = 0.0, vg = 0.0;
-    for (int x = 0; x < 3; x++) {
-      vr += rbas[k + 3 * x] * rbas[k + 3 * x];
-      vg += br2[x + 3 * k] * br2[x + 3 * k];
-    }
-    lr = fmin(lr, sqrt(vr));
-    lg = fmin(lg, sqrt(vg));
+  double2 s1{0.0, 0.0};
+  for (int n = roff[pair]; n < roff[pair + 1]; n++) {
+    const double2 t = rtab[(size_t)n * nlm + lm];
+    s1.x += t.x;
+    s1.y += t.y;
   }
-  // at most 129^3 candidate points per sum: a degenerate lattice gives NaN
-  if (!(lr > 0.0) || !(lg > 0.0) || !(r_cf / lr < kMaxCells) || !(g_cf / lg < kMaxCells)) {
-    sgm[lm + (size_t)nlm * pair] = double2{__builtin_nan(""), __builtin_nan("")};
-    return;
-  }
-  double2 s1{0.0, 0.0}, s2{0.0, 0.0};
-  {
-    const int imax = (int)(r_cf / lr) + 1;
-    for (int i1 = -imax; i1 <= imax; i1++)
-      for (int i2 = -imax; i2 <= imax; i2++)
-        for (int i3 = -imax; i3 <= imax; i3++) {
-          double rp[3];
-          for (int x = 0; x < 3; x++) rp[x] = rbas[0 + 3 * x] * i1 + rbas[1 + 3 * x] * i2 + rbas[2 + 3 * x] * i3 + raa[x];
-          const double rl = sqrt(rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2]);
-          if (!(rl <= r_cf && rl > 1e-6)) continue;
-          // Gamma_l(r) by Gamma_l = Gamma_(l-1) (l - 1/2)/r + r^(l-2) e^{-r^2/eta^2} / eta^(2l-1)
-          const double gausr = exp(-rl * rl / (eta * eta));
-          double gam = sqrt(kPiF) * erfc(rl / eta) / rl, rpow = 1.0 / rl, epow = eta;
-          for (int ll = 1; ll <= l; ll++) {
-            gam = gam * (ll - 0.5) / rl + rpow * gausr / epow;
-            rpow *= rl;
-            epow *= eta * eta;
-          }
-          double sq, cq;
-          sincos(-(rp[0] * pq[0] + rp[1] * pq[1] + rp[2] * pq[2]), &sq, &cq);
-          const double2 y = ylm_one(rp, l, m);
-          const double2 c{gam * cq, gam * sq};
-          s1.x += c.x * y.x - c.y * y.y;
-          s1.y += c.x * y.y + c.y * y.x;
-        }
-  }
-  {
-    const int imax = (int)(g_cf / lg) + 1;
-    const double pref = 4.0 * kPiF * sqrt(kPiF) / vol;
-    // (-i)^l
-    const double2 il = (l & 3) == 0 ? double2{1.0, 0.0} : (l & 3) == 1 ? double2{0.0, -1.0}
-                     : (l & 3) == 2 ? double2{-1.0, 0.0} : double2{0.0, 1.0};
-    for (int i1 = -imax; i1 <= imax; i1++)
-      for (int i2 = -imax; i2 <= imax; i2++)
-        for (int i3 = -imax; i3 <= imax; i3++) {
-          double g[3], gpq[3];
-          for (int x = 0; x < 3; x++) {
-            g[x] = br2[x + 3 * 0] * i1 + br2[x + 3 * 1] * i2 + br2[x + 3 * 2] * i3;  // br2 columns: reciprocal vectors
-            gpq[x] = g[x] + pq[x];
-          }
-          const double gl = sqrt(gpq[0] * gpq[0] + gpq[1] * gpq[1] + gpq[2] * gpq[2]);
-          if (!(gl <= g_cf && gl > 1e-6)) continue;
-          double sg, cg;
-          sincos(g[0] * raa[0] + g[1] * raa[1] + g[2] * raa[2], &sg, &cg);
-          // pref e^{-eta^2 gl^2/4} gl^-2 (gl/2)^l (-i)^l e^{i G.r_ij}
-          double a = pref * exp(-(eta * gl) * (eta * gl) / 4.0) / (gl * gl);
-          for (int ll = 1; ll <= l; ll++) a *= 0.5 * gl;
-          const double2 ph{a * (il.x * cg - il.y * sg), a * (il.x * sg + il.y * cg)};
-          const double2 y = ylm_one(gpq, l, m);
-          s2.x += ph.x * y.x - ph.y * y.y;
-          s2.y += ph.x * y.y + ph.y * y.x;
-        }
+  double2 s2{0.0, 0.0};
+  for (int c = 0; c < nchunk; c++) {
+    const double2 t = part[((size_t)c * npair + pair) * nlm + lm];
+    s2.x += t.x;
+    s2.y += t.y;
   }
   double gam = sqrt(kPiF);
   for (int k = 1; k <= l; k++) gam *= (k - 0.5);
   double2 v{(s1.x + s2.x) / gam, (s1.y + s2.y) / gam};
-  if (lm == 0 && i == j) v.x -= 1.0 / (eta * kPiF);
+  int j = 0;
+  while ((j + 1) * (j + 2) / 2 <= pair) j++;
+  if (lm == 0 && pair - j * (j + 1) / 2 == j) v.x -= 1.0 / (eta * kPiF);
   sgm[lm + (size_t)nlm * pair] = v;
 }
 
@@ -356,65 +409,133 @@ __device__ double sph_bessel(int L, double x) {
 }
 
 
-// tab[(L*npt + j)*nk + k] = j_L(k r_j) r_j (mode 0, L <= lmax) or sin(k r_j)
-// (mode 1, L = 0): one thread per entry, consecutive threads along k.
+// tab[(L*npt + j)*nk + k] = j_L(k r_j) r_j (mode 0, L < nl) or sin(k r_j)
+// (mode 1, nl = 1): one thread per (j, k), consecutive threads along k; the
+// orders below x by one upward recurrence from j_0, j_1, the others by
+// sph_bessel.
 __global__ void __launch_bounds__(256) pygw_rad_table(int mode, int nl, int npt, const double* __restrict__ rx,
                                                       const double* __restrict__ k, int nk, double* __restrict__ tab) {
   const long e = blockIdx.x * 256L + threadIdx.x;
-  if (e >= (long)nl * npt * nk) return;
-  const int ik = (int)(e % nk);
-  const long lj = e / nk;
-  const int j = (int)(lj % npt), L = (int)(lj / npt);
+  if (e >= (long)npt * nk) return;
+  const int ik = (int)(e % nk), j = (int)(e / nk);
   const double r = rx[j], x = k[ik] * r;
-  tab[e] = mode == 0 ? sph_bessel(L, x) * r : sin(x);
-}
-
-// out[f][k] = sum_j w_j (A_f(j) + c2 B_f(j)) tab[L_f][j][k]: one thread per
-// (f, k), the threads of a block share f (blockIdx.y).  L_f outside [0, nl)
-// gives NaN.
-//
-// w_j is RINT13g's rule (radials.f90:140-187) for npt >= 4 points: Simpson's
-// weights dh r_j (1, 4, 2, ..., 4, 1)/3 from the second point (1-based, odd
-// npt) or the third (even npt, with a trapezoid dh (f_1 + f_2)/2 over the
-// first interval), and r0 f(r0)/3 for [0, r0].  The r_j of the weights are
-// the ones RINT13g forms: r0 D^(J0-1) at the first Simpson point J0 and a
-// product by D per point after it (which drifts from r0 D^j by up to ~1e-13
-// at the end of an 800-point mesh), r0 D^(J0-1) / D at point J0 - 1.
-__global__ void __launch_bounds__(256) pygw_rad_olap(const double* __restrict__ ul, const double* __restrict__ us,
-                                                     const int* __restrict__ lf, int nl, int npt,
-                                                     const double* __restrict__ rx, double dh, double D, double c2,
-                                                     const double* __restrict__ tab, int nk,
-                                                     double* __restrict__ out) {
-  const int f = blockIdx.y, ik = blockIdx.x * 256 + threadIdx.x;
-  if (ik >= nk) return;
-  const int L = lf[f];
-  if (L < 0 || L >= nl) {
-    out[(size_t)f * nk + ik] = __builtin_nan("");
+  const size_t stride = (size_t)npt * nk;
+  if (mode == 1) {
+    tab[e] = sin(x);
     return;
   }
-  const double* t = tab + (size_t)L * npt * nk + ik;
-  const double* a = ul + (size_t)f * npt;
-  const double* b = us + (size_t)f * npt;
+  if (x < 0.01) {
+    for (int L = 0; L < nl; L++) tab[e + L * stride] = sph_bessel(L, x) * r;
+    return;
+  }
+  double sx, cx;
+  sincos(x, &sx, &cx);
+  double ja = sx / x, jb = (sx / x - cx) / x;  // j_L, j_(L+1) at L = 0
+  for (int L = 0; L < nl; L++) {
+    tab[e + L * stride] = ((double)L < x ? ja : sph_bessel(L, x)) * r;
+    if ((double)(L + 1) < x) {  // j_(L+2) while it stays below x
+      const double jn = (2.0 * L + 3.0) / x * jb - ja;
+      ja = jb;
+      jb = jn;
+    }
+  }
+}
+
+// w[j] = RINT13g's rule (radials.f90:140-187) for npt >= 4 points as the
+// weight of point j: Simpson's weights dh r_j (1, 4, 2, ..., 4, 1)/3 from the
+// second point (1-based, odd npt) or the third (even npt, with a trapezoid
+// dh (f_1 + f_2)/2 over the first interval), and r0 f(r0)/3 for [0, r0].  The
+// r_j are the ones RINT13g forms: r0 D^(J0-1) at the first Simpson point J0
+// and a product by D per point after it (which drifts from r0 D^j by up to
+// ~1e-13 at the end of an 800-point mesh), r0 D^(J0-1) / D at point J0 - 1.
+// One thread walks the mesh in order.
+__global__ void pygw_rad_weights(int npt, const double* __restrict__ rx, double dh, double D, double* __restrict__ w) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
   const double r0 = rx[0];
   const bool odd = (npt & 1) != 0;
   const int j0 = odd ? 1 : 2;                // 0-based first Simpson point
   double R = odd ? r0 * D : r0 * (D * D);     // its r
   const double R1 = R / D;                    // r of point j0 - 1
-  double s = 0.0;
-  for (int j = 0; j < npt; j++) {
-    double w;
-    if (j < j0) {
-      if (j == 0)
-        w = r0 / 3.0 + (odd ? dh * R1 / 3.0 : 0.5 * dh * r0);
-      else  // even npt, second point
-        w = dh * R1 / 3.0 + 0.5 * dh * R1;
-    } else {
-      w = (j == npt - 1 ? 1.0 : (((j - j0) & 1) == 0 ? 4.0 : 2.0)) * dh * R / 3.0;
-      R *= D;
-    }
-    s += w * (a[j] + c2 * b[j]) * t[(size_t)j * nk];
+  const double dh3 = dh / 3.0;
+  w[0] = r0 / 3.0 + (odd ? dh3 * R1 : 0.5 * dh * r0);
+  if (!odd) w[1] = dh3 * R1 + 0.5 * dh * R1;
+  // pairs of points (4, 2) from j0 on; the last point has weight 1
+  const double c4 = 4.0 * dh3, c2 = 2.0 * dh3;
+  int j = j0;
+  for (; j + 2 < npt; j += 2) {
+    w[j] = c4 * R;
+    R *= D;
+    w[j + 1] = c2 * R;
+    R *= D;
   }
-  out[(size_t)f * nk + ik] = s;
+  for (; j < npt; j++) {
+    w[j] = (j == npt - 1 ? dh3 : ((j - j0) & 1) == 0 ? c4 : c2) * R;
+    R *= D;
+  }
+}
+
+// out[f][k] = sum_j w_j (A_f(j) + c2 B_f(j)) tab[L_f][j][k]: a block of
+// kOlapWaves waves covers 64 k (the lanes) and kFun consecutive functions
+// (blockIdx.y; A, B and L the same across the block), wave v the points
+// j in [v npt / kOlapWaves, (v+1) npt / kOlapWaves); the waves' partial sums
+// are added in wave order through LDS.  A table row is read once for
+// consecutive functions of the same L.  L_f outside [0, nl) gives NaN.
+constexpr int kFun = 4, kOlapWaves = 8;
+__global__ void __launch_bounds__(64 * kOlapWaves) pygw_rad_olap(const double* __restrict__ ul,
+                                                                 const double* __restrict__ us,
+                                                                 const int* __restrict__ lf, int nf, int nl, int npt,
+                                                                 const double* __restrict__ w, double c2,
+                                                                 const double* __restrict__ tab, int nk,
+                                                                 double* __restrict__ out) {
+  __shared__ double red[kOlapWaves - 1][kFun][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int f0 = blockIdx.y * kFun, ik = blockIdx.x * 64 + lane;
+  const int nq = min(kFun, nf - f0);
+  int L[kFun];
+  bool bad = false;
+#pragma unroll
+  for (int q = 0; q < kFun; q++) {
+    L[q] = q < nq ? lf[f0 + q] : -1;
+    if (q < nq && (L[q] < 0 || L[q] >= nl)) bad = true;
+  }
+  double acc[kFun];
+#pragma unroll
+  for (int q = 0; q < kFun; q++) acc[q] = 0.0;
+  const size_t stride = (size_t)npt * nk;
+  const int j0 = (int)((long)npt * wv / kOlapWaves), j1 = (int)((long)npt * (wv + 1) / kOlapWaves);
+  if (!bad && ik < nk) {
+    for (int j = j0; j < j1; j++) {
+      const double wj = w[j];
+      const double* t = tab + (size_t)j * nk + ik;
+      double tv = 0.0;
+      int cur = -1;
+#pragma unroll
+      for (int q = 0; q < kFun; q++) {
+        if (q < nq) {
+          if (L[q] != cur) {
+            cur = L[q];
+            tv = t[cur * stride];
+          }
+          const size_t o = (size_t)(f0 + q) * npt + j;
+          acc[q] += wj * (ul[o] + c2 * us[o]) * tv;
+        }
+      }
+    }
+  }
+  if (wv > 0) {
+#pragma unroll
+    for (int q = 0; q < kFun; q++) red[wv - 1][q][lane] = acc[q];
+  }
+  __syncthreads();
+  if (wv == 0 && ik < nk) {
+    for (int v = 1; v < kOlapWaves; v++) {
+#pragma unroll
+      for (int q = 0; q < kFun; q++) acc[q] += red[v - 1][q][lane];
+    }
+#pragma unroll
+    for (int q = 0; q < kFun; q++)
+      if (q < nq) out[(size_t)(f0 + q) * nk + ik] = bad ? __builtin_nan("") : acc[q];
+  }
 }
 
 // phase[(i*ndf + j)*ngs + g] = sum over the G of group g (gind4, 1-based, G 0
@@ -444,19 +565,22 @@ __global__ void __launch_bounds__(256) pygw_zm_phase(const int* __restrict__ gin
 }
 
 // V[im + mbs*jm] for the L = 0 functions: sum_g phase[i][j][g] c/G^4 s_a(g) s_b(g).
+// One wave per entry (four a block): lane l sums g = l, l + 64, ..., then the
+// lanes are added by a fixed butterfly, so the result does not depend on
+// timing.
 __global__ void __launch_bounds__(256) pygw_zm_fill(const int* __restrict__ fidx, const int* __restrict__ fdf,
                                                     const int* __restrict__ frow, int nf0, const double* __restrict__ sing,
                                                     int nsing,
                                                     const double* __restrict__ glen0, int ngs,
                                                     const double2* __restrict__ phase, int ndf, double vol, int mbs,
                                                     double2* __restrict__ V) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (e >= nf0 * nf0) return;
   const int a = e % nf0, b = e / nf0;
   if (frow[a] < 0 || frow[a] >= mbs || frow[b] < 0 || frow[b] >= mbs) return;
   if (fidx[a] < 0 || fidx[a] >= nsing || fidx[b] < 0 || fidx[b] >= nsing || fdf[a] < 0 || fdf[a] >= ndf ||
       fdf[b] < 0 || fdf[b] >= ndf) {
-    V[frow[a] + (size_t)mbs * frow[b]] = double2{__builtin_nan(""), __builtin_nan("")};
+    if (lane == 0) V[frow[a] + (size_t)mbs * frow[b]] = double2{__builtin_nan(""), __builtin_nan("")};
     return;
   }
   const double c = 16.0 * mt::kPi * mt::kPi / vol;
@@ -464,13 +588,17 @@ __global__ void __launch_bounds__(256) pygw_zm_fill(const int* __restrict__ fidx
   const double* sb = sing + (size_t)fidx[b] * ngs;
   const double2* ph = phase + ((size_t)fdf[a] * ndf + fdf[b]) * ngs;
   double2 s{0.0, 0.0};
-  for (int g = 0; g < ngs; g++) {
+  for (int g = lane; g < ngs; g += 64) {
     const double gl2 = glen0[g] * glen0[g];
     const double w = c / (gl2 * gl2) * sa[g] * sb[g];
     s.x += ph[g].x * w;
     s.y += ph[g].y * w;
   }
-  V[frow[a] + (size_t)mbs * frow[b]] = s;
+  for (int o = 32; o > 0; o >>= 1) {
+    s.x += __shfl_xor(s.x, o);
+    s.y += __shfl_xor(s.y, o);
+  }
+  if (lane == 0) V[frow[a] + (size_t)mbs * frow[b]] = s;
 }
 
 __global__ void pygw_zero_v2(double2* __restrict__ x, long n) {
@@ -479,17 +607,170 @@ __global__ void pygw_zero_v2(double2* __restrict__ x, long n) {
 }
 
 // ---------------------------------------------------------------------------
+namespace {
+// genr_str (for_Coulomb.f90:418-491): basis[k] (k < 3) the basis vectors,
+// the vectors i1 b0 + i2 b1 + i3 b2 + shift (i1 slowest) with 1e-6 < |v| <=
+// rmax, as (x, y, z, |v|), ordered by length (ties in generation order).
+// False if rmax spans more than kMaxCells shortest basis vectors.
+bool lattice_list(double rmax, const double shift[3], const double basis[3][3], std::vector<double>& out) {
+  double lmin = 1e300;
+  for (int k = 0; k < 3; k++)
+    lmin = std::fmin(lmin, std::sqrt(basis[k][0] * basis[k][0] + basis[k][1] * basis[k][1] + basis[k][2] * basis[k][2]));
+  if (!(lmin > 0.0) || !(rmax / lmin < mt::kMaxCells)) return false;
+  const int imax = (int)(rmax / lmin) + 1;
+  std::vector<std::array<double, 4>> v;
+  for (int i1 = -imax; i1 <= imax; i1++)
+    for (int i2 = -imax; i2 <= imax; i2++)
+      for (int i3 = -imax; i3 <= imax; i3++) {
+        std::array<double, 4> r;
+        for (int x = 0; x < 3; x++) r[x] = basis[0][x] * i1 + basis[1][x] * i2 + basis[2][x] * i3 + shift[x];
+        r[3] = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        if (r[3] <= rmax && r[3] > 1e-6) v.push_back(r);
+      }
+  std::stable_sort(v.begin(), v.end(), [](const std::array<double, 4>& a, const std::array<double, 4>& b) {
+    return a[3] < b[3];
+  });
+  out.clear();
+  for (const auto& r : v) out.insert(out.end(), r.begin(), r.end());
+  return true;
+}
+
+struct EwaldPlan {
+  std::vector<double> rvec, gvec, raa;  // the lists (4 / 8 doubles a vector) and r_ij per pair
+  std::vector<int> roff;
+  int ng = 0, npair = 0, nlm = 0, nchunk = 1;
+  size_t off_gvec = 0, off_raa = 0, off_pq = 0, off_roff = 0, off_gtab = 0, off_rtab = 0, off_eph = 0, off_part = 0;
+  size_t bytes = 0;
+};
+
+// The vector lists on the host and the workspace layout; false on a bad argument.
+bool ewald_plan(const double* pq, int lam_max, const double* pos, const double* br2, const double* rbas,
+                const double* alat, int ortho, double r_cf, double g_cf, int ndf, EwaldPlan& P) {
+  if (!pq || !pos || !br2 || !rbas || !alat || lam_max < 0 || lam_max > mt::kMaxLam || ndf <= 0 || ndf > 1000 ||
+      !(r_cf > 0.0) || !(g_cf > 0.0) || (ortho != 0 && ortho != 1))
+    return false;
+  P.npair = ndf * (ndf + 1) / 2;
+  P.nlm = (lam_max + 1) * (lam_max + 1);
+  double rb[3][3], gb[3][3];  // rows of rbas (Fortran rbas(k, x)); columns of br2 (br2(x, k))
+  for (int k = 0; k < 3; k++)
+    for (int x = 0; x < 3; x++) {
+      rb[k][x] = rbas[k + 3 * x];
+      gb[k][x] = br2[x + 3 * k];
+    }
+  P.rvec.clear();
+  P.raa.assign(3 * (size_t)P.npair, 0.0);
+  P.roff.assign(P.npair + 1, 0);
+  std::vector<double> one;
+  for (int j = 0; j < ndf; j++)
+    for (int i = 0; i <= j; i++) {
+      const int pair = i + j * (j + 1) / 2;
+      double d[3], raa[3];
+      for (int x = 0; x < 3; x++) d[x] = pos[x + 3 * i] - pos[x + 3 * j];
+      for (int x = 0; x < 3; x++) raa[x] = ortho ? d[x] * alat[x] : d[0] * rb[0][x] + d[1] * rb[1][x] + d[2] * rb[2][x];
+      for (int x = 0; x < 3; x++) P.raa[3 * pair + x] = raa[x];
+      if (!lattice_list(r_cf, raa, rb, one)) return false;
+      P.roff[pair] = (int)(P.rvec.size() / 4);  // pairs are visited in column order: pair = running index
+      P.rvec.insert(P.rvec.end(), one.begin(), one.end());
+    }
+  P.roff[P.npair] = (int)(P.rvec.size() / 4);
+  if (!lattice_list(g_cf, pq, gb, one)) return false;
+  P.ng = (int)(one.size() / 4);
+  P.gvec.assign(8 * (size_t)P.ng, 0.0);
+  for (int n = 0; n < P.ng; n++) {
+    for (int x = 0; x < 4; x++) P.gvec[8 * (size_t)n + x] = one[4 * (size_t)n + x];
+    for (int x = 0; x < 3; x++) P.gvec[8 * (size_t)n + 4 + x] = one[4 * (size_t)n + x] - pq[x];
+  }
+  // chunks of the reciprocal sum: about 2048 blocks, at least 32 vectors a chunk
+  const int tiles = ceil_div(P.npair, kEwPairs) * ceil_div(P.nlm, 64);
+  P.nchunk = std::max(1, std::min({64, ceil_div(2048, tiles), P.ng / 32}));
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t o = al(P.rvec.size() * sizeof(double) + 8);
+  P.off_gvec = o;
+  o = al(o + P.gvec.size() * sizeof(double) + 8);
+  P.off_raa = o;
+  o = al(o + P.raa.size() * sizeof(double));
+  P.off_pq = o;
+  o = al(o + 3 * sizeof(double));
+  P.off_roff = o;
+  o = al(o + P.roff.size() * sizeof(int));
+  P.off_gtab = o;
+  o = al(o + (size_t)std::max(P.ng, 1) * P.nlm * sizeof(double2));
+  P.off_rtab = o;
+  o = al(o + (size_t)std::max((int)(P.rvec.size() / 4), 1) * P.nlm * sizeof(double2));
+  P.off_eph = o;
+  o = al(o + (size_t)std::max(P.ng, 1) * P.npair * sizeof(double2));
+  P.off_part = o;
+  o = al(o + (size_t)P.nchunk * P.npair * P.nlm * sizeof(double2));
+  P.bytes = o;
+  return true;
+}
+}  // namespace
+
+extern "C" int pygw_ewald_workspace(const double* pq, int lam_max, const double* pos, const double* br2,
+                                    const double* rbas, const double* alat, int ortho, double r_cf, double g_cf,
+                                    int ndf, size_t* bytes) {
+  EwaldPlan P;
+  if (!bytes || !ewald_plan(pq, lam_max, pos, br2, rbas, alat, ortho, r_cf, g_cf, ndf, P)) return PYGW_E_ARG;
+  *bytes = P.bytes;
+  return PYGW_OK;
+}
+
 extern "C" int pygw_ewald_summation(const double* pq, int lam_max, const double* pos, const double* br2,
                                     const double* rbas, const double* alat, double vol, int ortho, double r_cf,
-                                    double g_cf, double eta, int ndf, void* sgm, void* stream_) {
+                                    double g_cf, double eta, int ndf, void* sgm, void* workspace, size_t ws_bytes,
+                                    void* stream_) {
   hipStream_t stream = (hipStream_t)stream_;
-  if (!pq || !pos || !br2 || !rbas || !alat || !sgm || lam_max < 0 || lam_max > mt::kMaxLam || ndf <= 0 ||
-      ndf > 4096 || !(vol > 0.0) || !(r_cf > 0.0) || !(g_cf > 0.0) || !(eta > 0.0) || (ortho != 0 && ortho != 1))
+  EwaldPlan P;
+  if (!sgm || !(vol > 0.0) || !(eta > 0.0) ||
+      !ewald_plan(pq, lam_max, pos, br2, rbas, alat, ortho, r_cf, g_cf, ndf, P))
     return PYGW_E_ARG;
-  const int npair = ndf * (ndf + 1) / 2, nlm = (lam_max + 1) * (lam_max + 1);
-  hipLaunchKernelGGL(pygw_ew_sgm, dim3(npair, ceil_div(nlm, 64)), dim3(64), 0, stream, pq, lam_max, pos, br2, rbas, alat,
-                     vol, ortho, r_cf, g_cf, eta, (double2*)sgm);
-  return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+  char* ws = (char*)workspace;
+  bool own = false;
+  if (!ws) {
+    HIP_TRY(hipMallocAsync((void**)&ws, P.bytes, stream));
+    own = true;
+  } else if (ws_bytes < P.bytes) {
+    return PYGW_E_ARG;
+  }
+  // the lists travel in one host buffer; the stream is synchronised before it is freed
+  std::vector<char> host(P.off_gtab, 0);
+  std::memcpy(host.data(), P.rvec.data(), P.rvec.size() * sizeof(double));
+  std::memcpy(host.data() + P.off_gvec, P.gvec.data(), P.gvec.size() * sizeof(double));
+  std::memcpy(host.data() + P.off_raa, P.raa.data(), P.raa.size() * sizeof(double));
+  std::memcpy(host.data() + P.off_pq, pq, 3 * sizeof(double));
+  std::memcpy(host.data() + P.off_roff, P.roff.data(), P.roff.size() * sizeof(int));
+  int rc = PYGW_OK;
+  if (hipMemcpyAsync(ws, host.data(), host.size(), hipMemcpyHostToDevice, stream) != hipSuccess) rc = PYGW_E_HIP;
+  const double* rvec = (const double*)ws;
+  const double* gvec = (const double*)(ws + P.off_gvec);
+  const double* raa = (const double*)(ws + P.off_raa);
+  const double* pqd = (const double*)(ws + P.off_pq);
+  const int* roff = (const int*)(ws + P.off_roff);
+  double2* gtab = (double2*)(ws + P.off_gtab);
+  double2* rtab = (double2*)(ws + P.off_rtab);
+  double2* eph = (double2*)(ws + P.off_eph);
+  double2* part = (double2*)(ws + P.off_part);
+  const double pref = 4.0 * mt::kPiF * std::sqrt(mt::kPiF) / vol;
+  const int nr = (int)(P.rvec.size() / 4);
+  if (rc == PYGW_OK && P.ng > 0) {
+    hipLaunchKernelGGL(pygw_ew_tab, dim3(ceil_div((long)P.ng * (lam_max + 1), 256)), dim3(256), 0, stream, 1, gvec, 8,
+                       P.ng, lam_max, pqd, eta, pref, gtab);
+    hipLaunchKernelGGL(pygw_ew_phase, dim3(ceil_div((long)P.ng * P.npair, 256)), dim3(256), 0, stream, gvec, P.ng, raa,
+                       P.npair, eph);
+  }
+  if (rc == PYGW_OK && nr > 0)
+    hipLaunchKernelGGL(pygw_ew_tab, dim3(ceil_div((long)nr * (lam_max + 1), 256)), dim3(256), 0, stream, 0, rvec, 4, nr,
+                       lam_max, pqd, eta, pref, rtab);
+  if (rc == PYGW_OK)
+    hipLaunchKernelGGL(pygw_ew_recip, dim3(ceil_div(P.nlm, 64), ceil_div(P.npair, kEwPairs), P.nchunk), dim3(64), 0,
+                       stream, gtab, eph, P.ng, P.nlm, P.npair, P.nchunk, part);
+  if (rc == PYGW_OK)
+    hipLaunchKernelGGL(pygw_ew_final, dim3(P.npair, ceil_div(P.nlm, 64)), dim3(64), 0, stream, rtab, roff, part,
+                       P.nchunk, P.npair, lam_max, eta, (double2*)sgm);
+  if (rc == PYGW_OK && hipGetLastError() != hipSuccess) rc = PYGW_E_HIP;
+  if (own && hipFreeAsync(ws, stream) != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
+  if (hipStreamSynchronize(stream) != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
+  return rc;
 }
 
 extern "C" int pygw_mt_coulomb(const double* vq, int iat, int loctmatsize, const int* big_l, const int* nmix,
@@ -518,7 +799,8 @@ extern "C" int pygw_mt_coulomb(const double* vq, int iat, int loctmatsize, const
 extern "C" int pygw_radial_olap_workspace(int mode, int lmax, int npt, int n_k, size_t* bytes) {
   if ((mode != 0 && mode != 1) || lmax < 0 || (mode == 1 && lmax != 0) || npt < 4 || n_k <= 0 || !bytes)
     return PYGW_E_ARG;
-  *bytes = (size_t)(lmax + 1) * npt * n_k * sizeof(double);
+  // the table of F, then the npt weights
+  *bytes = ((size_t)(lmax + 1) * npt * n_k * sizeof(double) + 255) / 256 * 256 + (size_t)npt * sizeof(double);
   return PYGW_OK;
 }
 
@@ -540,14 +822,16 @@ extern "C" int pygw_radial_olap(int mode, int rel, const double* ul, const doubl
   }
   // c2 = 1/c^2 (scalar relativistic) or RINT13g's single-precision literal 1e-22
   const double c2 = rel ? 1.0 / (137.0359895 * 137.0359895) : (double)1e-22f;
-  const long nt = (long)(lmax + 1) * npt * n_k;
   // RINT13g's ratio e^dh from the host's libm, as the compiled reference
   // forms it (a last-bit difference in it grows j-fold along the mesh)
   const double D = std::exp(dh);
-  hipLaunchKernelGGL(pygw_rad_table, dim3(ceil_div(nt, 256)), dim3(256), 0, stream, mode, lmax + 1, npt, rx, k, n_k,
-                     (double*)ws);
-  hipLaunchKernelGGL(pygw_rad_olap, dim3(ceil_div(n_k, 256), n_fun), dim3(256), 0, stream, ul, us, lf, lmax + 1, npt,
-                     rx, dh, D, c2, (const double*)ws, n_k, out);
+  const size_t tab_bytes = ((size_t)(lmax + 1) * npt * n_k * sizeof(double) + 255) / 256 * 256;
+  hipLaunchKernelGGL(pygw_rad_table, dim3(ceil_div((long)npt * n_k, 256)), dim3(256), 0, stream, mode, lmax + 1, npt,
+                     rx, k, n_k, (double*)ws);
+  double* wts = (double*)((char*)ws + tab_bytes);
+  hipLaunchKernelGGL(pygw_rad_weights, dim3(1), dim3(64), 0, stream, npt, rx, dh, D, wts);
+  hipLaunchKernelGGL(pygw_rad_olap, dim3(ceil_div(n_k, 64), ceil_div(n_fun, kFun)), dim3(64 * kOlapWaves), 0, stream,
+                     ul, us, lf, n_fun, lmax + 1, npt, (const double*)wts, c2, (const double*)ws, n_k, out);
   int rc = hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
   if (own && hipFreeAsync(ws, stream) != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;
   return rc;
@@ -581,7 +865,7 @@ extern "C" int pygw_zero_momentum_coulomb(const int* gindex, int n_g, const int*
   const long np = (long)ndf * ndf * n_short;
   hipLaunchKernelGGL(pygw_zm_phase, dim3(ceil_div(np, 256)), dim3(256), 0, stream, gindex, n_g, g_start, n_short, pos,
                      ndf, (double2*)ws);
-  hipLaunchKernelGGL(pygw_zm_fill, dim3(ceil_div((long)n_fun0 * n_fun0, 256)), dim3(256), 0, stream, f_sing, f_atom,
+  hipLaunchKernelGGL(pygw_zm_fill, dim3(ceil_div((long)n_fun0 * n_fun0, 4)), dim3(256), 0, stream, f_sing, f_atom,
                      f_row, n_fun0, sing, n_sing, glen0, n_short, (const double2*)ws, ndf, vol, mbsize, (double2*)vmat);
   int rc = hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
   if (own && hipFreeAsync(ws, stream) != hipSuccess && rc == PYGW_OK) rc = PYGW_E_HIP;

@@ -584,27 +584,15 @@ class MTSystem:
     gindex: np.ndarray
 
 
-def mt_system(q=(0.11, -0.07, 0.05), ortho=False, eta=1.6, rcut=7.0, gcut=14.0, lmax_b=3, gmax=2.5,
-              identity_rotations=False, seed=13) -> MTSystem:
-    """r_cf = rcut eta and g_cf = gcut / eta.  identity_rotations: djmm of the
-    identity (every local frame aligned); else random complex entries."""
+def _mt_build(rbas, alat, ortho, pos, mult, big_l, mesh_spec, q, eta, rcut, gcut, lam_max, gmax,
+              identity_rotations, rng) -> MTSystem:
     from .coulomb import calc_tildeg
-    rng = np.random.default_rng(seed)
-    if ortho:
-        alat = np.array([5.2, 5.9, 7.1])
-        rbas = np.diag(alat)
-    else:
-        a, c = 5.4, 8.3
-        alat = np.array([a, a, c])
-        rbas = np.array([[a, 0.0, 0.0], [-a / 2, a * np.sqrt(3) / 2, 0.0], [0.0, 0.0, c]])
     vol = abs(np.linalg.det(rbas))
     br2 = 2 * np.pi * np.linalg.inv(rbas)
-    mult = np.array([1, 2])
-    atm = [0, 1, 1]
-    pos = np.array([[0.0, 0.0, 0.0], [1 / 3, 2 / 3, 0.27], [2 / 3, 1 / 3, 0.77]]).T
-    big_l = [np.array([0, 0, 1, 1, 2, 3])[: 4 + (lmax_b >= 3) * 2], np.array([0, 1, 2, 2, 3, 0])]
-    big_l = [b[b <= lmax_b] for b in big_l]
+    mult = np.asarray(mult)
+    atm = [iat for iat in range(len(mult)) for _ in range(mult[iat])]
     nat = len(mult)
+    lmax_b = int(max(int(np.max(b)) for b in big_l))
     nmix = np.array([len(b) for b in big_l])
     max_nmix = int(nmix.max())
     big_l_f = np.zeros((max_nmix, nat), dtype=np.int64)
@@ -621,8 +609,7 @@ def mt_system(q=(0.11, -0.07, 0.05), ortho=False, eta=1.6, rcut=7.0, gcut=14.0, 
                 im += 2 * L + 1
             idf += 1
     im_start[nat] = im
-    lmaxt = 2 * (lmax_b + 1)
-    tilg = calc_tildeg(lmaxt)
+    tilg = calc_tildeg(2 * (lmax_b + 1))
     dimdj = sum((2 * l + 1) * (l + 1) for l in range(lmax_b + 1))
     ndf = int(mult.sum())
     if identity_rotations:
@@ -639,9 +626,9 @@ def mt_system(q=(0.11, -0.07, 0.05), ortho=False, eta=1.6, rcut=7.0, gcut=14.0, 
     for iat in range(nat):
         for jat in range(nat):
             rtlij[: nmix[jat], : nmix[iat], jat, iat] = np.outer(rtl[jat, : nmix[jat]], rtl[iat, : nmix[iat]])
-    # logarithmic meshes (odd and even npt) and smooth radial functions r u, r us
+    # logarithmic meshes and smooth radial functions r u, r us
     meshes, ul, us = [], [], []
-    for iat, (r0, rmt, npt) in enumerate(((1e-4, 2.1, 781), (5e-5, 1.9, 640))):
+    for iat, (r0, rmt, npt) in enumerate(mesh_spec):
         dh = np.log(rmt / r0) / (npt - 1)
         meshes.append((r0, dh, npt))
         rx = r0 * np.exp(dh) ** np.arange(npt)
@@ -649,14 +636,51 @@ def mt_system(q=(0.11, -0.07, 0.05), ortho=False, eta=1.6, rcut=7.0, gcut=14.0, 
         ul.append(np.array([rx ** (L + 1) * (c[i, 0] + c[i, 1] * rx) * np.exp(-c[i, 2] * rx)
                             for i, L in enumerate(big_l[iat])]))
         us.append(0.3 * ul[-1] * np.cos(rx)[None, :])
-    r = np.arange(-12, 13)
+    n = int(gmax / min(np.linalg.norm(br2, axis=0))) + 1
+    r = np.arange(-n, n + 1)
     g = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
     ln = np.linalg.norm(g @ br2.T, axis=1)
-    keep = ln <= gmax * 2 * np.pi / 2.0
+    keep = ln <= gmax
     g, ln = g[keep], ln[keep]
     order = np.lexsort((g[:, 2], g[:, 1], g[:, 0], np.round(ln, 10)))
     return MTSystem(rbas=rbas, br2=br2, alat=alat, vol=vol, ortho=ortho, pos=pos, mult=mult, atm=atm,
                     pq=np.asarray(q, dtype=float), eta=eta, r_cf=rcut * eta, g_cf=gcut / eta,
-                    lam_max=2 * lmax_b, big_l=big_l, big_l_f=big_l_f, nmix=nmix, im_start=im_start,
+                    lam_max=lam_max, big_l=big_l, big_l_f=big_l_f, nmix=nmix, im_start=im_start,
                     loctmatsize=int(im), iprod=iprod, tilg=tilg, rrint=rrint, djmm=djmm, rtlij=rtlij,
                     meshes=meshes, ul=ul, us=us, rel=True, glen=ln[order], gindex=g[order])
+
+
+def mt_system(q=(0.11, -0.07, 0.05), ortho=False, eta=1.6, rcut=7.0, gcut=14.0, lmax_b=3, gmax=2.5 * np.pi,
+              identity_rotations=False, seed=13) -> MTSystem:
+    """r_cf = rcut eta and g_cf = gcut / eta.  identity_rotations: djmm of the
+    identity (every local frame aligned); else random complex entries."""
+    rng = np.random.default_rng(seed)
+    if ortho:
+        alat = np.array([5.2, 5.9, 7.1])
+        rbas = np.diag(alat)
+    else:
+        a, c = 5.4, 8.3
+        alat = np.array([a, a, c])
+        rbas = np.array([[a, 0.0, 0.0], [-a / 2, a * np.sqrt(3) / 2, 0.0], [0.0, 0.0, c]])
+    pos = np.array([[0.0, 0.0, 0.0], [1 / 3, 2 / 3, 0.27], [2 / 3, 1 / 3, 0.77]]).T
+    big_l = [np.array([0, 0, 1, 1, 2, 3]), np.array([0, 1, 2, 2, 3, 0])]
+    big_l = [b[b <= lmax_b] for b in big_l]
+    return _mt_build(rbas, alat, ortho, pos, [1, 2], big_l, ((1e-4, 2.1, 781), (5e-5, 1.9, 640)), q, eta, rcut, gcut,
+                     2 * lmax_b, gmax, identity_rotations, rng)
+
+
+def mt_bench_system(q=(0.07, 0.05, -0.04), seed=29) -> MTSystem:
+    """A PyGW-sized muffin-tin block: 8 atoms (4 types of mult 2) in a
+    monoclinic cell of about 700 bohr^3, lmbmax = 3 with 26-30 product
+    functions per atom (L = 0..3; 112 (L, M) functions per atom, loctmatsize
+    904), structure constants to lam_max = 4 (lmbmax + 1) = 16, 781-point
+    radial meshes, plane waves to |G| = 7 bohr^-1 for the q = 0 term."""
+    rng = np.random.default_rng(seed)
+    rbas = np.array([[7.9, 0.0, 0.0], [1.3, 8.4, 0.0], [0.0, 0.0, 10.6]])
+    alat = np.array([7.9, 8.5, 10.6])
+    pos = np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.5], [0.25, 0.1, 0.3], [0.75, 0.9, 0.7],
+                    [0.1, 0.6, 0.2], [0.9, 0.4, 0.8], [0.6, 0.2, 0.1], [0.4, 0.8, 0.9]]).T
+    per_l = [(6, 8, 8, 6), (7, 8, 8, 6), (6, 7, 8, 6), (6, 8, 7, 7)]
+    big_l = [np.repeat(np.arange(4), n) for n in per_l]
+    meshes = [(1e-4, 2.3, 781), (5e-5, 2.1, 781), (1e-4, 1.9, 781), (1e-4, 1.8, 781)]
+    return _mt_build(rbas, alat, False, pos, [2, 2, 2, 2], big_l, meshes, q, 2.2, 7.0, 14.0, 16, 7.0, False, rng)

@@ -43,7 +43,7 @@ def _zeros(fn):
     "pygw_mixed_coulomb", "pygw_mixed_coulomb_workspace", "pygw_coul_mixed_block",
     "pygw_omega_basis_workspace", "pygw_omega_basis", "pygw_omega_basis_many_workspace", "pygw_omega_basis_many",
     "pygw_omega_project", "pygw_omega_expand", "pygw_omega_expand_batched",
-    "pygw_ewald_summation", "pygw_mt_coulomb", "pygw_radial_olap_workspace", "pygw_radial_olap",
+    "pygw_ewald_workspace", "pygw_ewald_summation", "pygw_mt_coulomb", "pygw_radial_olap_workspace", "pygw_radial_olap",
     "pygw_zero_momentum_workspace", "pygw_zero_momentum_coulomb",
 ])
 def test_null_and_zero_arguments_rejected(name):
@@ -118,19 +118,29 @@ def test_small_workspaces_rejected_before_any_device_call():
 def test_muffin_tin_coulomb_workspaces_and_bounds():
     s = C.c_size_t(0)
     q = _fn("pygw_radial_olap_workspace")
-    # the table of j_L(k r) r, L <= lmax (mode 0), or of sin(k r) (mode 1, lmax 0)
-    assert q(0, 3, 781, 100, C.byref(s)) == 0 and s.value == 4 * 781 * 100 * 8
-    assert q(1, 0, 640, 7, C.byref(s)) == 0 and s.value == 640 * 7 * 8
+    # the table of j_L(k r) r, L <= lmax (mode 0), or of sin(k r) (mode 1, lmax 0), at 256 bytes, then the weights
+    up = lambda b: (b + 255) // 256 * 256  # noqa: E731
+    assert q(0, 3, 781, 100, C.byref(s)) == 0 and s.value == up(4 * 781 * 100 * 8) + 781 * 8
+    assert q(1, 0, 640, 7, C.byref(s)) == 0 and s.value == up(640 * 7 * 8) + 640 * 8
     assert q(1, 1, 640, 7, C.byref(s)) == E_ARG
     assert q(0, 2, 3, 7, C.byref(s)) == E_ARG      # RINT13g needs 4 points
     q = _fn("pygw_zero_momentum_workspace")
     assert q(3, 40, C.byref(s)) == 0 and s.value == 3 * 3 * 40 * 16
     p = C.c_void_p(16)     # never dereferenced: the checks come first
+    # the Ewald lattice arguments are host arrays: a cubic cell of side 10 and 3 atoms
+    dbl = lambda *v: (C.c_double * len(v))(*v)  # noqa: E731
+    pq, pos, alat = dbl(0.1, 0.0, 0.0), dbl(*([0.0] * 9)), dbl(10.0, 10.0, 10.0)
+    rbas = dbl(10.0, 0, 0, 0, 10.0, 0, 0, 0, 10.0)
+    br2 = dbl(*(2 * 3.141592653589793 / 10.0 * x for x in (1, 0, 0, 0, 1, 0, 0, 0, 1)))
     ew = _fn("pygw_ewald_summation")
-    # lam_max above the table of l, a bad ortho flag, eta 0
-    assert ew(p, 17, p, p, p, p, 100.0, 0, 10.0, 10.0, 1.0, 3, p, None) == E_ARG
-    assert ew(p, 8, p, p, p, p, 100.0, 2, 10.0, 10.0, 1.0, 3, p, None) == E_ARG
-    assert ew(p, 8, p, p, p, p, 100.0, 0, 10.0, 10.0, 0.0, 3, p, None) == E_ARG
+    ewq = _fn("pygw_ewald_workspace")
+    assert ewq(pq, 8, pos, br2, rbas, alat, 0, 10.0, 10.0, 3, C.byref(s)) == 0 and s.value > 0
+    # lam_max above the table of l, a bad ortho flag, eta 0, a cutoff of 64 cells, a small workspace
+    assert ew(pq, 17, pos, br2, rbas, alat, 1000.0, 0, 10.0, 10.0, 1.0, 3, p, None, 0, None) == E_ARG
+    assert ew(pq, 8, pos, br2, rbas, alat, 1000.0, 2, 10.0, 10.0, 1.0, 3, p, None, 0, None) == E_ARG
+    assert ew(pq, 8, pos, br2, rbas, alat, 1000.0, 0, 10.0, 10.0, 0.0, 3, p, None, 0, None) == E_ARG
+    assert ew(pq, 8, pos, br2, rbas, alat, 1000.0, 0, 640.0, 10.0, 1.0, 3, p, None, 0, None) == E_ARG
+    assert ew(pq, 8, pos, br2, rbas, alat, 1000.0, 0, 10.0, 10.0, 1.0, 3, p, p, s.value - 1, None) == E_ARG
     mt = _fn("pygw_mt_coulomb")
     ims = (C.c_int * 3)(0, 20, 64)
     # iat outside [-1, nat), rows past loctmatsize
