@@ -1166,6 +1166,19 @@ def run_mt_coulomb(args):
         torch.cuda.synchronize()
         tz.append(time.perf_counter() - t0)
     value = float(st.sum())
+    # the radial overlaps of one atom type (table, weights, overlap kernels), timed alone on the stream
+    a0 = int(np.argmax([len(b) for b in S.big_l]))
+    tj = []
+    for _ in range(max(1, args.steps)):
+        ev[0].record(s)
+        coulomb.calc_olap_pw_product(gq_d, lmaxb, rx_d[a0], S.meshes[a0][1], S.rel, S.big_l[a0], ul_d[a0], us_d[a0])
+        ev[1].record(s)
+        torch.cuda.synchronize()
+        tj.append(ev[0].elapsed_time(ev[1]) / 1e3)
+    t_olap = float(np.mean(tj))
+    npt, nf = S.meshes[a0][2], len(S.big_l[a0])
+    # algorithmic bytes: the Bessel table written and read once, the radial functions, k, the mesh, the result
+    b_olap = 2.0 * 8 * (lmaxb + 1) * npt * len(gq) + 2.0 * 8 * nf * npt + 8.0 * (len(gq) + npt) + 8.0 * nf * len(gq)
     print(f"[bench] mt_coulomb ewald {t_ew * 1e3:.2f} ms, jlam {t_jl * 1e3:.2f} ms ({len(gq)} |q+G|), "
           f"mt {t_mt * 1e3:.2f} ms, zero momentum {np.mean(tz) * 1e3:.2f} ms", file=sys.stderr, flush=True)
     rec = {"metric": "muffin-tin Coulomb block (Ewald + jlam + MT_Coulomb) seconds per q", "value": value,
@@ -1174,7 +1187,14 @@ def run_mt_coulomb(args):
            "config": {"workload": f"mt_coulomb: 8 atoms (4 types x mult 2), lam_max {S.lam_max}, loctmatsize "
                                   f"{S.loctmatsize}, {len(gq)} unique |q+G|, 781-point meshes, Lmax {lmaxb}"},
            "stages_s": {"ewald_summation": t_ew, "calc_olap_pw_product_all_atoms": t_jl, "mt_coulomb": t_mt,
-                        "zero_momentum_coulomb_q0_host_timed": float(np.mean(tz))}}
+                        "zero_momentum_coulomb_q0_host_timed": float(np.mean(tz))},
+           "roofline": {"bound": "hbm", "kernel": "pygw_radial_olap (pygw_rad_table with the weights + "
+                                                  "pygw_rad_olap, one atom type)",
+                        "achieved": b_olap / t_olap / 1e9, "peak": 8000.0, "unit": "GB/s",
+                        "frac": b_olap / t_olap / 8e12, "traffic": None, "bytes_per_call": b_olap,
+                        "call_s": t_olap,
+                        "note": "Bessel table written and read once + radial functions + result; latency-bound "
+                                "at this size (one atom type: 781 points x 1473 |q+G| x 4 L)"}}
     if not args.no_cpu:
         rec["cpu_baseline"] = mt_coulomb_cpu_baseline(S, gq, meshes, lmaxb, value)
     print(json.dumps(rec))

@@ -490,7 +490,7 @@ int pygw_coul_mixed_block(const void* mpwipw, int ngq, int ngq_barc, const void*
  *   rx[0]) (radials.f90:140-187) with F = j_L(k rx) rx, L = lf[f] <= lmax
  *   (mode 0: jlam of Calc_Olap_PW_Product, pygw.py:2110-2124) or
  *   F = sin(k rx) (mode 1, lmax 0, every lf[f] 0: Zero_Momentum_Coulomb's
- *   sing, pygw.py:2069-2086); rx [npt] the logarithmic mesh
+ *   sing, pygw.py:2069-2086); rx [npt <= 1024] the logarithmic mesh
  *   (strc.radial_mesh, rx[0] e^{j dh}), ul, us float64 [n_fun][npt] (r u and
  *   r us, PyGW's ul_product / us_product), lf int32 [n_fun], k [n_k], out
  *   [n_fun][n_k].  workspace: pygw_radial_olap_workspace bytes (the table

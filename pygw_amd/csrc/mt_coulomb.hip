@@ -389,11 +389,12 @@ __device__ double sph_bessel(int L, double x) {
     return jc;
   }
   const int N = L + 40 + (int)x;
+  const double ix = 1.0 / x;
   double jp = 0.0, jc = 1e-100, norm = 0.0, want = 0.0;
   for (int l = N; l >= 0; l--) {
     if (l == L) want = jc;
     norm += (2.0 * l + 1.0) * jc * jc;
-    const double jm = (2.0 * l + 1.0) / x * jc - jp;
+    const double jm = (2.0 * l + 1.0) * ix * jc - jp;
     jp = jc;
     jc = jm;
     if (fabs(jc) > 1e150) {  // rescale every kept value alike
@@ -409,13 +410,48 @@ __device__ double sph_bessel(int L, double x) {
 }
 
 
+// w[j] = RINT13g's rule (radials.f90:140-187) for npt >= 4 points as the
+// weight of point j: Simpson's weights dh r_j (1, 4, 2, ..., 4, 1)/3 from the
+// second point (1-based, odd npt) or the third (even npt, with a trapezoid
+// dh (f_1 + f_2)/2 over the first interval), and r0 f(r0)/3 for [0, r0].  The
+// r_j are the ones RINT13g forms: r0 D^(J0-1) at the first Simpson point J0
+// and a product by D per point after it (which drifts from r0 D^j by up to
+// ~1e-13 at the end of an 800-point mesh), r0 D^(J0-1) / D at point J0 - 1.
+// One thread walks the mesh in order (thread 0 of pygw_rad_table's first
+// block, alongside the table).
+__device__ void rad_weights(int npt, const double* __restrict__ rx, double dh, double D, double* __restrict__ w) {
+  const double r0 = rx[0];
+  const bool odd = (npt & 1) != 0;
+  const int j0 = odd ? 1 : 2;                // 0-based first Simpson point
+  double R = odd ? r0 * D : r0 * (D * D);     // its r
+  const double R1 = R / D;                    // r of point j0 - 1
+  const double dh3 = dh / 3.0;
+  w[0] = r0 / 3.0 + (odd ? dh3 * R1 : 0.5 * dh * r0);
+  if (!odd) w[1] = dh3 * R1 + 0.5 * dh * R1;
+  // pairs of points (4, 2) from j0 on; the last point has weight 1
+  const double c4 = 4.0 * dh3, c2 = 2.0 * dh3;
+  int j = j0;
+  for (; j + 2 < npt; j += 2) {
+    w[j] = c4 * R;
+    R *= D;
+    w[j + 1] = c2 * R;
+    R *= D;
+  }
+  for (; j < npt; j++) {
+    w[j] = (j == npt - 1 ? dh3 : ((j - j0) & 1) == 0 ? c4 : c2) * R;
+    R *= D;
+  }
+}
+
 // tab[(L*npt + j)*nk + k] = j_L(k r_j) r_j (mode 0, L < nl) or sin(k r_j)
 // (mode 1, nl = 1): one thread per (j, k), consecutive threads along k; the
 // orders below x by one upward recurrence from j_0, j_1, the others by
 // sph_bessel.
 __global__ void __launch_bounds__(256) pygw_rad_table(int mode, int nl, int npt, const double* __restrict__ rx,
-                                                      const double* __restrict__ k, int nk, double* __restrict__ tab) {
+                                                      const double* __restrict__ k, int nk, double* __restrict__ tab,
+                                                      double dh, double D, double* __restrict__ w) {
   const long e = blockIdx.x * 256L + threadIdx.x;
+  if (e == 0) rad_weights(npt, rx, dh, D, w);
   if (e >= (long)npt * nk) return;
   const int ik = (int)(e % nk), j = (int)(e / nk);
   const double r = rx[j], x = k[ik] * r;
@@ -441,52 +477,21 @@ __global__ void __launch_bounds__(256) pygw_rad_table(int mode, int nl, int npt,
   }
 }
 
-// w[j] = RINT13g's rule (radials.f90:140-187) for npt >= 4 points as the
-// weight of point j: Simpson's weights dh r_j (1, 4, 2, ..., 4, 1)/3 from the
-// second point (1-based, odd npt) or the third (even npt, with a trapezoid
-// dh (f_1 + f_2)/2 over the first interval), and r0 f(r0)/3 for [0, r0].  The
-// r_j are the ones RINT13g forms: r0 D^(J0-1) at the first Simpson point J0
-// and a product by D per point after it (which drifts from r0 D^j by up to
-// ~1e-13 at the end of an 800-point mesh), r0 D^(J0-1) / D at point J0 - 1.
-// One thread walks the mesh in order.
-__global__ void pygw_rad_weights(int npt, const double* __restrict__ rx, double dh, double D, double* __restrict__ w) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  const double r0 = rx[0];
-  const bool odd = (npt & 1) != 0;
-  const int j0 = odd ? 1 : 2;                // 0-based first Simpson point
-  double R = odd ? r0 * D : r0 * (D * D);     // its r
-  const double R1 = R / D;                    // r of point j0 - 1
-  const double dh3 = dh / 3.0;
-  w[0] = r0 / 3.0 + (odd ? dh3 * R1 : 0.5 * dh * r0);
-  if (!odd) w[1] = dh3 * R1 + 0.5 * dh * R1;
-  // pairs of points (4, 2) from j0 on; the last point has weight 1
-  const double c4 = 4.0 * dh3, c2 = 2.0 * dh3;
-  int j = j0;
-  for (; j + 2 < npt; j += 2) {
-    w[j] = c4 * R;
-    R *= D;
-    w[j + 1] = c2 * R;
-    R *= D;
-  }
-  for (; j < npt; j++) {
-    w[j] = (j == npt - 1 ? dh3 : ((j - j0) & 1) == 0 ? c4 : c2) * R;
-    R *= D;
-  }
-}
-
 // out[f][k] = sum_j w_j (A_f(j) + c2 B_f(j)) tab[L_f][j][k]: a block of
 // kOlapWaves waves covers 64 k (the lanes) and kFun consecutive functions
-// (blockIdx.y; A, B and L the same across the block), wave v the points
-// j in [v npt / kOlapWaves, (v+1) npt / kOlapWaves); the waves' partial sums
-// are added in wave order through LDS.  A table row is read once for
-// consecutive functions of the same L.  L_f outside [0, nl) gives NaN.
-constexpr int kFun = 4, kOlapWaves = 8;
+// (blockIdx.y), wave v the points j in [v npt / kOlapWaves, (v+1) npt /
+// kOlapWaves).  The block first stages W[q][j] = w_j (A + c2 B) of its
+// functions in LDS (npt <= kOlapMaxPts); the waves' partial sums are added in
+// wave order through LDS.  A table row is read once for consecutive
+// functions of the same L.  L_f outside [0, nl) gives NaN.
+constexpr int kFun = 4, kOlapWaves = 16, kOlapMaxPts = 1024;
 __global__ void __launch_bounds__(64 * kOlapWaves) pygw_rad_olap(const double* __restrict__ ul,
                                                                  const double* __restrict__ us,
                                                                  const int* __restrict__ lf, int nf, int nl, int npt,
                                                                  const double* __restrict__ w, double c2,
                                                                  const double* __restrict__ tab, int nk,
                                                                  double* __restrict__ out) {
+  __shared__ double W[kFun][kOlapMaxPts];
   __shared__ double red[kOlapWaves - 1][kFun][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int f0 = blockIdx.y * kFun, ik = blockIdx.x * 64 + lane;
@@ -498,6 +503,12 @@ __global__ void __launch_bounds__(64 * kOlapWaves) pygw_rad_olap(const double* _
     L[q] = q < nq ? lf[f0 + q] : -1;
     if (q < nq && (L[q] < 0 || L[q] >= nl)) bad = true;
   }
+  for (int e = threadIdx.x; e < nq * npt; e += 64 * kOlapWaves) {
+    const int q = e / npt, j = e % npt;
+    const size_t o = (size_t)(f0 + q) * npt + j;
+    W[q][j] = w[j] * (ul[o] + c2 * us[o]);
+  }
+  __syncthreads();
   double acc[kFun];
 #pragma unroll
   for (int q = 0; q < kFun; q++) acc[q] = 0.0;
@@ -505,7 +516,6 @@ __global__ void __launch_bounds__(64 * kOlapWaves) pygw_rad_olap(const double* _
   const int j0 = (int)((long)npt * wv / kOlapWaves), j1 = (int)((long)npt * (wv + 1) / kOlapWaves);
   if (!bad && ik < nk) {
     for (int j = j0; j < j1; j++) {
-      const double wj = w[j];
       const double* t = tab + (size_t)j * nk + ik;
       double tv = 0.0;
       int cur = -1;
@@ -516,8 +526,7 @@ __global__ void __launch_bounds__(64 * kOlapWaves) pygw_rad_olap(const double* _
             cur = L[q];
             tv = t[cur * stride];
           }
-          const size_t o = (size_t)(f0 + q) * npt + j;
-          acc[q] += wj * (ul[o] + c2 * us[o]) * tv;
+          acc[q] += W[q][j] * tv;
         }
       }
     }
@@ -810,7 +819,7 @@ extern "C" int pygw_radial_olap(int mode, int rel, const double* ul, const doubl
   hipStream_t stream = (hipStream_t)stream_;
   size_t need = 0;
   if (pygw_radial_olap_workspace(mode, lmax, npt, n_k, &need) != PYGW_OK || !ul || !us || !lf || !rx || !k || !out ||
-      n_fun <= 0 || n_fun > 65535 || !(dh > 0.0) || (rel != 0 && rel != 1))
+      n_fun <= 0 || n_fun > 65535 || npt > kOlapMaxPts || !(dh > 0.0) || (rel != 0 && rel != 1))
     return PYGW_E_ARG;
   void* ws = workspace;
   bool own = false;
@@ -826,10 +835,9 @@ extern "C" int pygw_radial_olap(int mode, int rel, const double* ul, const doubl
   // forms it (a last-bit difference in it grows j-fold along the mesh)
   const double D = std::exp(dh);
   const size_t tab_bytes = ((size_t)(lmax + 1) * npt * n_k * sizeof(double) + 255) / 256 * 256;
-  hipLaunchKernelGGL(pygw_rad_table, dim3(ceil_div((long)npt * n_k, 256)), dim3(256), 0, stream, mode, lmax + 1, npt,
-                     rx, k, n_k, (double*)ws);
   double* wts = (double*)((char*)ws + tab_bytes);
-  hipLaunchKernelGGL(pygw_rad_weights, dim3(1), dim3(64), 0, stream, npt, rx, dh, D, wts);
+  hipLaunchKernelGGL(pygw_rad_table, dim3(ceil_div((long)npt * n_k, 256)), dim3(256), 0, stream, mode, lmax + 1, npt,
+                     rx, k, n_k, (double*)ws, dh, D, wts);
   hipLaunchKernelGGL(pygw_rad_olap, dim3(ceil_div(n_k, 64), ceil_div(n_fun, kFun)), dim3(64 * kOlapWaves), 0, stream,
                      ul, us, lf, n_fun, lmax + 1, npt, (const double*)wts, c2, (const double*)ws, n_k, out);
   int rc = hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
