@@ -558,6 +558,55 @@ int pygw_coul_setev(const void* vmat, int mbsize, double evtol, const void* wi0,
 int pygw_minm_vhalf(const void* mmat, long n_rows, int n_in, long ld_mmat, const void* barcvm, int ld_barcvm,
                     int matsize, int complex64_out, void* minm, void* stream);
 
+/* ------------------------------------------------------------------------
+ * The muffin-tin and core parts of the matrix elements (minm_mt.hip).
+ * pygw_minm_mt_plan (host only) lays out the atom / product-function /
+ *   radial-channel structure that PyGW passes to fvxcn.calc_minm_mt as
+ *   mult [nat], nmix [nat], big_l (max_nmix, nat), nLO_at (lomax+1, nat),
+ *   ncore [nat] (host arrays, Fortran layouts) and in1.lmax: with plan =
+ *   NULL it returns the size in *plan_ints, else fills plan (int32).  The
+ *   same plan is passed in host memory (plan_host) and device memory
+ *   (plan_dev) to the calls below; it does not depend on k or q.
+ * pygw_minm_mt replaces fvxcn.calc_minm_mt(ql, nst, nend, mst, mend, alfa,
+ *   beta, gama, alfp, betp, gamp, s3r, vpos, mult, nmix, big_l, nLO_at,
+ *   ncore, cgcoef, lmax, loctmatsize) (pygw.py:2655,
+ *   for_vxcnn.f90:839-1028):
+ *     mmat[(ie1 n2 + ie2) ld + col0 + im] = < u_im Y_lb,mb | psi*_ie2 psi_ie1 >
+ *                                          e^{-2 pi i ql.pos_idf},
+ *   ie1 < n1 = nend - nstart, ie2 < n2 = mend - mstart, im < loctmatsize:
+ *   the rows of the concatenated mmat that pygw_minm_is fills after
+ *   loctmatsize (col0 = 0 here).  Device arrays, PyGW's Fortran layouts:
+ *   ql [3], pos (3, ndf), alfa/beta (nbmax, (lmax+1)^2, ndf) and gama
+ *   (nbmax_g, nlomax, (lmax+1)^2, ndf) complex128 at k, alfp/betp/gamp
+ *   likewise at k - q (mbmax, mbmax_g), s3r (n_mix, n_fnc, n_fnc, nat),
+ *   cgcoef [n_cg] (cmp_all_Gaunt's table).  nlomax >= the largest nLO_at;
+ *   gama/gamp may be NULL when it is 0.  FP64 (MFMA).  workspace:
+ *   pygw_minm_mt_workspace bytes or NULL.
+ * pygw_minc replaces fvxcn.calc_minc(kil, nst, nend, cst, cend, alfa, beta,
+ *   gama, s3r, corind, ...) (pygw.py:2677, for_vxcnn.f90:1136-1273): rows
+ *   (ie1 (cend - cstart) + ic) of ld >= col0 + loctmatsize; pygw_minc2
+ *   replaces fvxcn.calc_minc2(kjl, cst, cend, mst, mend, alfp, betp, gamp,
+ *   ...) (pygw.py:2680, for_vxcnn.f90:1275-1397): rows (ic (mend - mstart)
+ *   + ie2).  corind int32 (ncg_rows, 5) Fortran (atom type, atom, core
+ *   index, l, m; 0-based) on the device.  Allocate nothing.
+ * ------------------------------------------------------------------------ */
+int pygw_minm_mt_plan(int nat, const int* mult, const int* nmix, const int* big_l, int max_nmix, const int* nLO_at,
+                      int lomax, const int* ncore, int lmax, int* plan, int* plan_ints);
+int pygw_minm_mt_workspace(const int* plan_host, int n1, int n2, size_t* bytes);
+int pygw_minm_mt(const int* plan_host, const int* plan_dev, const double* ql, int nstart, int nend, int mstart,
+                 int mend, const void* alfa, const void* beta, const void* gama, int nbmax, int nbmax_g,
+                 const void* alfp, const void* betp, const void* gamp, int mbmax, int mbmax_g, int nlomax,
+                 const double* s3r, int n_mix, int n_fnc, const double* pos, const double* cgcoef, int n_cg,
+                 void* mmat, long ld_mmat, int col0, void* workspace, size_t ws_bytes, void* stream);
+int pygw_minc(const int* plan_host, const int* plan_dev, const double* kl, int nstart, int nend, int cstart, int cend,
+              const void* alfa, const void* beta, const void* gama, int nbmax, int nbmax_g, int nlomax,
+              const double* s3r, int n_mix, int n_fnc, const int* corind, int ncg_rows, const double* pos,
+              const double* cgcoef, int n_cg, void* mmat, long ld_mmat, int col0, void* stream);
+int pygw_minc2(const int* plan_host, const int* plan_dev, const double* kmql, int cstart, int cend, int mstart,
+               int mend, const void* alfp, const void* betp, const void* gamp, int mbmax, int mbmax_g, int nlomax,
+               const double* s3r, int n_mix, int n_fnc, const int* corind, int ncg_rows, const double* pos,
+               const double* cgcoef, int n_cg, void* mmat, long ld_mmat, int col0, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -302,3 +302,58 @@ def pw_get_short_and_phase(glen, gindex, pos, mult):
     lib.pw_get_short_and_phase_(_p(glen0), _p(phase), _i(ngs), _p(gind4), _p(gx), _p(gl), _p(ps), _p(mu), _i(nk),
                                 _i(mu.size), _i(ndf))
     return glen0, np.ascontiguousarray(phase), gind4
+
+
+def cmp_all_gaunt(maxj):
+    """gaunt.f90:1-34."""
+    f = _load("libref_gaunt.so").cmp_all_gaunt_
+    n = (maxj + 1) * (maxj + 2) * (maxj + 3) * (16 * maxj * maxj + 29 * maxj + 10) // 60
+    out = np.zeros(n, dtype=np.float64)
+    f(_p(out), _i(maxj))
+    return out
+
+
+def getcgcoef(l1, l2, l3, m1, m2, cgcoef):
+    """gaunt.f90:36-101."""
+    f = _load("libref_gaunt.so").getcgcoef_
+    f.restype = C.c_double
+    cg = np.ascontiguousarray(cgcoef, dtype=np.float64)
+    return f(_i(l1), _i(l2), _i(l3), _i(m1), _i(m2), _p(cg))
+
+
+def _mt_common(s3r, corind, pos, mult, nmix, big_l, nLO_at, ncore, cgcoef):
+    F = lambda a, dt: np.asfortranarray(np.asarray(a, dtype=dt))  # noqa: E731
+    return (F(s3r, np.float64), F(corind, np.int32), F(pos, np.float64), F(mult, np.int32), F(nmix, np.int32),
+            F(big_l, np.int32), F(nLO_at, np.int32), F(ncore, np.int32), F(cgcoef, np.float64))
+
+
+def calc_minc(kl, nstart, nend, cstart, cend, alfa, beta, gama, s3r, corind, pos, mult, nmix, big_l, nLO_at, ncore,
+              cgcoef, lmax, loctmatsize):
+    """for_vxcnn.f90:1136-1273 (no zgemm); arrays as PyGW passes them."""
+    f = _load("libref_vxcnn.so").calc_minc_
+    F = lambda a: np.asfortranarray(np.asarray(a, dtype=np.complex128))  # noqa: E731
+    al, be, ga = F(alfa), F(beta), F(gama)
+    s3, co, ps, mu, nm, bl, nlo, nco, cg = _mt_common(s3r, corind, pos, mult, nmix, big_l, nLO_at, ncore, cgcoef)
+    out = np.zeros((nend - nstart, cend - cstart, loctmatsize), dtype=np.complex128, order="F")
+    k = np.asfortranarray(np.asarray(kl, dtype=np.float64))
+    f(_p(out), _p(k), _i(nstart), _i(nend), _i(cstart), _i(cend), _p(al), _p(be), _p(ga), _p(s3), _p(co), _p(ps),
+      _p(mu), _p(nm), _p(bl), _p(nlo), _p(nco), _p(cg), _i(lmax), _i(loctmatsize), _i(al.shape[0]), _i(mu.size),
+      _i(al.shape[1]), _i(ga.shape[1]), _i(al.shape[2]), _i(s3.shape[0]), _i(s3.shape[1]), _i(bl.shape[0]),
+      _i(nlo.shape[0] - 1), _i(ga.shape[0]), _i(ga.shape[2]), _i(ga.shape[3]), _i(co.shape[0]))
+    return np.ascontiguousarray(out)
+
+
+def calc_minc2(kmql, cstart, cend, mstart, mend, alfp, betp, gamp, s3r, corind, pos, mult, nmix, big_l, nLO_at,
+               ncore, cgcoef, lmax, loctmatsize):
+    """for_vxcnn.f90:1275-1397 (no zgemm)."""
+    f = _load("libref_vxcnn.so").calc_minc2_
+    F = lambda a: np.asfortranarray(np.asarray(a, dtype=np.complex128))  # noqa: E731
+    al, be, ga = F(alfp), F(betp), F(gamp)
+    s3, co, ps, mu, nm, bl, nlo, nco, cg = _mt_common(s3r, corind, pos, mult, nmix, big_l, nLO_at, ncore, cgcoef)
+    out = np.zeros((cend - cstart, mend - mstart, loctmatsize), dtype=np.complex128, order="F")
+    k = np.asfortranarray(np.asarray(kmql, dtype=np.float64))
+    f(_p(out), _p(k), _i(cstart), _i(cend), _i(mstart), _i(mend), _p(al), _p(be), _p(ga), _p(s3), _p(co), _p(ps),
+      _p(mu), _p(nm), _p(bl), _p(nlo), _p(nco), _p(cg), _i(lmax), _i(loctmatsize), _i(al.shape[0]), _i(mu.size),
+      _i(al.shape[1]), _i(ga.shape[1]), _i(al.shape[2]), _i(s3.shape[0]), _i(s3.shape[1]), _i(bl.shape[0]),
+      _i(nlo.shape[0] - 1), _i(ga.shape[0]), _i(ga.shape[2]), _i(ga.shape[3]), _i(co.shape[0]))
+    return np.ascontiguousarray(out)

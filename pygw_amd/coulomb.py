@@ -447,3 +447,177 @@ def zero_momentum_coulomb(glen, gindex, pos, vol, mbsize, rel, meshes, big_l, ul
               sing.shape[0], C.c_void_p(fs.data_ptr()), C.c_void_p(fa.data_ptr()), C.c_void_p(fr.data_ptr()), len(f_row), float(vol),
               int(mbsize), C.c_void_p(out.data_ptr()), C.c_void_p(ws.data_ptr()), ws.numel(), _stream_ptr(device))
     return out
+
+
+# --- the muffin-tin and core parts of the matrix elements --------------------
+
+def _w3j(j1, j2, j3, m1, m2, m3):
+    """Wigner 3j symbol (Racah's sum; integer arithmetic until the last step)."""
+    from math import factorial as f, isqrt  # noqa: F401
+    if m1 + m2 + m3 or not abs(j1 - j2) <= j3 <= j1 + j2 or abs(m1) > j1 or abs(m2) > j2 or abs(m3) > j3:
+        return 0.0
+    num = f(j1 + j2 - j3) * f(j1 - j2 + j3) * f(-j1 + j2 + j3) * f(j1 + m1) * f(j1 - m1) * f(j2 + m2) \
+        * f(j2 - m2) * f(j3 + m3) * f(j3 - m3)
+    den = f(j1 + j2 + j3 + 1)
+    tmin, tmax = max(0, j2 - j3 - m1, j1 - j3 + m2), min(j1 + j2 - j3, j1 - m1, j2 + m2)
+    # sum_t (-1)^t / D_t over a common denominator L = lcm-free product bound: use exact fractions of ints
+    from fractions import Fraction
+    s = sum(Fraction((-1) ** t, f(t) * f(j3 - j2 + t + m1) * f(j3 - j1 + t - m2) * f(j1 + j2 - j3 - t)
+                     * f(j1 - t - m1) * f(j2 - t + m2)) for t in range(tmin, tmax + 1))
+    return (-1) ** ((j1 - j2 - m3) % 2) * float(s) * (num / den) ** 0.5
+
+
+def cmp_all_gaunt(maxj):
+    """gn.cmp_all_gaunt (pygw.py:545, gaunt.f90:1-34): for l2 <= l1 <= maxj,
+    l3 = l1-l2 .. l1+l2, m1 = -l1 .. l1, m2 = 0 .. l2 (that order) the Gaunt
+    coefficient <Y_l3,m1+m2 | Y_l2m2 Y_l1m1> (0 for odd l1 + l2 + l3 or
+    |m1 + m2| > l3), the table getcgcoef reads.  Host setup, once per run."""
+    out = []
+    for l1 in range(maxj + 1):
+        for l2 in range(l1 + 1):
+            for l3 in range(l1 - l2, l1 + l2 + 1):
+                even = (l1 + l2 + l3) % 2 == 0
+                c = (2 * l1 + 1) * (2 * l2 + 1) * (2 * l3 + 1) / (4 * np.pi)
+                w0 = _w3j(l3, l2, l1, 0, 0, 0) if even else 0.0
+                for m1 in range(-l1, l1 + 1):
+                    for m2 in range(l2 + 1):
+                        m3 = m1 + m2
+                        if not even or abs(m3) > l3:
+                            out.append(0.0)
+                        else:   # <Y_l3m3|Y_l2m2 Y_l1m1> = (-1)^m3 sqrt(c) 3j(l3 l2 l1; 000) 3j(l3 l2 l1; -m3 m2 m1)
+                            out.append((-1) ** (m3 % 2) * c ** 0.5 * w0 * _w3j(l3, l2, l1, -m3, m2, m1))
+    return np.array(out)
+
+
+class MinmMTPlan:
+    """The atom / product-function / radial-channel layout of the muffin-tin
+    matrix elements (pygw_minm_mt_plan): built once per run from PyGW's
+    strc.mult, nmix, big_l [max_nmix, nat], in1.nLO_at [lomax+1, nat],
+    pb.ncore and in1.lmax; kept in host memory and on the device."""
+
+    def __init__(self, mult, nmix, big_l, nLO_at, ncore, lmax, *, device="cuda"):
+        device = _require_device(device)
+        f = lambda a: np.ascontiguousarray(np.asarray(a).ravel(order="F"), dtype=np.int32)  # noqa: E731
+        self.mult, self.nmix, self.ncore = f(mult), f(nmix), f(ncore)
+        bl, nlo = np.asarray(big_l), np.asarray(nLO_at)
+        self.big_l, self.nLO_at = f(bl), f(nlo)
+        p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        n = C.c_int(0)
+        args = (len(self.mult), p(self.mult), p(self.nmix), p(self.big_l), bl.shape[0], p(self.nLO_at),
+                nlo.shape[0] - 1, p(self.ncore), int(lmax))
+        _lib.call("pygw_minm_mt_plan", *args, None, C.byref(n))
+        self.host = np.zeros(n.value, dtype=np.int32)
+        _lib.call("pygw_minm_mt_plan", *args, p(self.host), C.byref(n))
+        self.dev = torch.as_tensor(self.host, device=device)
+        self.lmax, self.device = int(lmax), device
+        self.loctmatsize = int(self.host[5])
+        self.nlomax = int(nlo.max()) if nlo.size else 0
+
+    def ptrs(self):
+        return self.host.ctypes.data_as(C.c_void_p), C.c_void_p(self.dev.data_ptr())
+
+
+def _bands(a, device):
+    """A band-coefficient array in PyGW's index order ([nbmax, (lmax+1)^2,
+    ndf] or with nLOmax) stored on the device in Fortran order, as the
+    kernels read it."""
+    if isinstance(a, torch.Tensor):
+        return a.to(device=device, dtype=torch.complex128).permute(*reversed(range(a.dim()))).contiguous()
+    return torch.as_tensor(np.asarray(a, dtype=np.complex128).ravel(order="F"), device=device)
+
+
+def calc_minm_mt(plan, ql, nstart, nend, mstart, mend, alfa, beta, gama, alfp, betp, gamp, s3r, vpos, cgcoef, *,
+                 out=None, col0=0, workspace=None):
+    """fvxcn.calc_minm_mt (pygw.py:2655, for_vxcnn.f90:839-1028):
+    mmat_mt[ie1, ie2, im] for bands nstart..nend-1 at k and mstart..mend-1 at
+    k - q, complex128 [nend-nstart, mend-mstart, loctmatsize], or written into
+    out[..., col0:col0 + loctmatsize] of a contiguous [n1, n2, ld] (the
+    concatenated mmat).  alfa/beta [nbmax, (lmax+1)^2, ndf], gama [nbmax,
+    nLOmax, (lmax+1)^2, ndf] and the primed arrays as PyGW builds them
+    (numpy, Fortran semantics), s3r [n_mix, n_fnc, n_fnc, nat], vpos [3, ndf],
+    cgcoef = cmp_all_gaunt(maxnt)."""
+    device = plan.device
+    n1, n2, loct = nend - nstart, mend - mstart, plan.loctmatsize
+    al, ap = alfa.shape, alfp.shape
+    d = [_bands(x, device) for x in (alfa, beta, alfp, betp)]
+    nlo = plan.nlomax
+    ga = gp = None
+    if nlo > 0:
+        ga, gp = _bands(gama, device), _bands(gamp, device)
+    s3 = np.asarray(s3r, dtype=np.float64)
+    s3d = _dev(s3.ravel(order="F"), torch.float64, device)
+    qd = _dev(np.asarray(ql, dtype=np.float64), torch.float64, device)
+    ps = _dev(np.asarray(vpos, dtype=np.float64).ravel(order="F"), torch.float64, device)
+    cg = _dev(np.asarray(cgcoef, dtype=np.float64), torch.float64, device)
+    if out is None:
+        out = torch.empty((n1, n2, loct), dtype=torch.complex128, device=device)
+        ld = loct
+    else:
+        if out.dtype != torch.complex128 or not out.is_contiguous() or tuple(out.shape[:2]) != (n1, n2):
+            raise ValueError("out must be a contiguous complex128 [nend-nstart, mend-mstart, ld] tensor")
+        ld = out.shape[2]
+    ph, pd = plan.ptrs()
+    sz = C.c_size_t(0)
+    _lib.call("pygw_minm_mt_workspace", ph, n1, n2, C.byref(sz))
+    if workspace is None or workspace.numel() < sz.value:
+        workspace = torch.empty(sz.value, dtype=torch.uint8, device=device)
+    vp = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+    nbg = gama.shape[0] if nlo > 0 else 0
+    mbg = gamp.shape[0] if nlo > 0 else 0
+    _lib.call("pygw_minm_mt", ph, pd, vp(qd), int(nstart), int(nend), int(mstart), int(mend), vp(d[0]), vp(d[1]),
+              vp(ga), al[0], nbg, vp(d[2]), vp(d[3]), vp(gp), ap[0], mbg, nlo, vp(s3d), s3.shape[0],
+              s3.shape[1], vp(ps), vp(cg), cg.numel(), vp(out), ld, int(col0), vp(workspace), workspace.numel(),
+              _stream_ptr(device))
+    return out
+
+
+def _minc_common(plan, name, kv, b0, b1, c0, c1, a, b, g, s3r, corind, vpos, cgcoef, out, col0, shape):
+    device = plan.device
+    ad, bd = _bands(a, device), _bands(b, device)
+    nlo = plan.nlomax
+    gd = _bands(g, device) if nlo > 0 else None
+    s3 = np.asarray(s3r, dtype=np.float64)
+    s3d = _dev(s3.ravel(order="F"), torch.float64, device)
+    co = np.asarray(corind)
+    ty, at = plan.host[plan.host[7]:plan.host[7] + 4 * int(plan.host[1])].reshape(-1, 4)[:, 0], None
+    sel = co[c0:c1] if name == "pygw_minc" else co[b0:b1]
+    if co.ndim != 2 or co.shape[1] != 5 or sel.size == 0 or (sel[:, 1] < 0).any() or \
+            (sel[:, 1] >= len(ty)).any() or (ty[sel[:, 1]] != sel[:, 0]).any() or (sel[:, 2] < 0).any() or \
+            (sel[:, 2] >= np.asarray(plan.ncore)[sel[:, 0]]).any() or (np.abs(sel[:, 4]) > sel[:, 3]).any():
+        raise ValueError("corind rows must be (atom type, atom of that type, core index < ncore, l, |m| <= l)")
+    cod = _dev(co.ravel(order="F"), torch.int32, device)
+    kd = _dev(np.asarray(kv, dtype=np.float64), torch.float64, device)
+    ps = _dev(np.asarray(vpos, dtype=np.float64).ravel(order="F"), torch.float64, device)
+    cg = _dev(np.asarray(cgcoef, dtype=np.float64), torch.float64, device)
+    if out is None:
+        out = torch.empty((*shape, plan.loctmatsize), dtype=torch.complex128, device=device)
+        ld = plan.loctmatsize
+    else:
+        if out.dtype != torch.complex128 or not out.is_contiguous() or tuple(out.shape[:2]) != shape:
+            raise ValueError("out must be a contiguous complex128 tensor of the result's first two extents")
+        ld = out.shape[2]
+    ph, pd = plan.ptrs()
+    vp = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+    nbg = g.shape[0] if nlo > 0 else 0
+    _lib.call(name, ph, pd, vp(kd), int(b0), int(b1), int(c0), int(c1), vp(ad), vp(bd), vp(gd), a.shape[0],
+              nbg, nlo, vp(s3d), s3.shape[0], s3.shape[1], vp(cod), co.shape[0], vp(ps), vp(cg), cg.numel(), vp(out),
+              ld, int(col0), _stream_ptr(device))
+    return out
+
+
+def calc_minc(plan, kl, nstart, nend, cstart, cend, alfa, beta, gama, s3r, corind, vpos, cgcoef, *, out=None,
+              col0=0):
+    """fvxcn.calc_minc (pygw.py:2677, for_vxcnn.f90:1136-1273): mmat_c[ie1, ic,
+    im] = <u_im | psi_ie1,k psi*_core>, complex128 [nend-nstart, cend-cstart,
+    loctmatsize]; corind [ncg, 5] (atom type, atom, core index, l, m)."""
+    return _minc_common(plan, "pygw_minc", kl, nstart, nend, cstart, cend, alfa, beta, gama, s3r, corind, vpos,
+                        cgcoef, out, col0, (nend - nstart, cend - cstart))
+
+
+def calc_minc2(plan, kmql, cstart, cend, mstart, mend, alfp, betp, gamp, s3r, corind, vpos, cgcoef, *, out=None,
+               col0=0):
+    """fvxcn.calc_minc2 (pygw.py:2680, for_vxcnn.f90:1275-1397): mmat_c[ic, ie2,
+    im] = <u_im | psi_core psi*_ie2,k-q>, complex128 [cend-cstart,
+    mend-mstart, loctmatsize]."""
+    return _minc_common(plan, "pygw_minc2", kmql, cstart, cend, mstart, mend, alfp, betp, gamp, s3r, corind, vpos,
+                        cgcoef, out, col0, (cend - cstart, mend - mstart))

@@ -684,3 +684,93 @@ def mt_bench_system(q=(0.07, 0.05, -0.04), seed=29) -> MTSystem:
     big_l = [np.repeat(np.arange(4), n) for n in per_l]
     meshes = [(1e-4, 2.3, 781), (5e-5, 2.1, 781), (1e-4, 1.9, 781), (1e-4, 1.8, 781)]
     return _mt_build(rbas, alat, False, pos, [2, 2, 2, 2], big_l, meshes, q, 2.2, 7.0, 14.0, 16, 7.0, False, rng)
+
+
+@dataclass
+class MinmMTSystem:
+    """Inputs of calc_minm_MT / calc_minc / calc_minc2 (for_vxcnn.f90:839-1397):
+    band coefficients alfa/beta/gama at k and alfp/betp/gamp at k - q in
+    PyGW's index order, the radial integrals s3r [n_mix, n_fnc, n_fnc, nat]
+    (symmetric in the two radial channels; the first ncore channels of an
+    atom type are its core states), the product basis (big_l [max_nmix,
+    nat]), local orbitals nLO_at [lomax+1, nat], the core list corind [ncg, 5]
+    and the Gaunt table."""
+    mult: np.ndarray
+    nmix: np.ndarray
+    big_l: np.ndarray
+    nLO_at: np.ndarray
+    ncore: np.ndarray
+    lmax: int
+    loctmatsize: int
+    alfa: np.ndarray
+    beta: np.ndarray
+    gama: np.ndarray
+    alfp: np.ndarray
+    betp: np.ndarray
+    gamp: np.ndarray
+    s3r: np.ndarray
+    corind: np.ndarray
+    core_l: list
+    vpos: np.ndarray
+    ql: np.ndarray
+    kl: np.ndarray
+    cgcoef: np.ndarray
+
+
+def minm_mt_system(lmax=3, lmbmax=2, nbmax=7, mbmax=9, los=True, big=False, seed=17) -> MinmMTSystem:
+    """Two atom types (mult 1 and 2).  los: local orbitals on l <= 2 of
+    both types (else none).  big: 4 types of mult 2, lmax 10, lmbmax 3 and
+    product bases of 26-30 functions (a PyGW-sized atom set)."""
+    from .coulomb import cmp_all_gaunt
+    rng = np.random.default_rng(seed)
+    if big:
+        mult = np.array([2, 2, 2, 2])
+        big_l_list = [np.repeat(np.arange(4), n) for n in ((6, 8, 8, 6), (7, 8, 8, 6), (6, 7, 8, 6), (6, 8, 7, 7))]
+        lomax = lmax
+        nlo = np.zeros((lmax + 1, 4), dtype=int)
+        nlo[:4] = np.array([[1, 1, 1, 0], [1, 0, 1, 0], [0, 1, 1, 0], [1, 1, 0, 0]]).T[: min(4, lmax + 1)]
+        core_l = [[0, 0, 1], [0, 1], [0], [0, 0, 1, 1]]
+    else:
+        mult = np.array([1, 2])
+        base = [np.array([0, 0, 1, 1, 2, 3]), np.array([0, 1, 1, 2, 0])]
+        big_l_list = [b[b <= lmbmax] for b in base]
+        # nLO_at over every l <= lmax (PyGW runs with lomax >= lmax), local orbitals on l <= 2
+        lomax = lmax
+        nlo = np.zeros((lmax + 1, 2), dtype=int)
+        if los:
+            nlo[:3] = np.array([[1, 0, 2], [0, 1, 1]]).T[: min(3, lmax + 1)]
+        core_l = [[0, 1], [0]]
+    nat = len(mult)
+    ndf = int(mult.sum())
+    nmix = np.array([len(b) for b in big_l_list])
+    max_nmix = int(nmix.max())
+    big_l = np.zeros((max_nmix, nat), dtype=np.int64)
+    for a in range(nat):
+        big_l[: nmix[a], a] = big_l_list[a]
+    ncore = np.array([len(c) for c in core_l])
+    nlo_tot = [int(nlo[: min(lomax, lmax) + 1, a].sum()) for a in range(nat)]
+    n_fnc = max(int(ncore[a]) + 2 * (lmax + 1) + nlo_tot[a] for a in range(nat))
+    nlomax = int(nlo.max()) if nlo.size else 0
+    nlm = (lmax + 1) ** 2
+    cplx = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)  # noqa: E731
+    alfa, beta = cplx(nbmax, nlm, ndf), 0.3 * cplx(nbmax, nlm, ndf)
+    alfp, betp = cplx(mbmax, nlm, ndf), 0.3 * cplx(mbmax, nlm, ndf)
+    gama = 0.5 * cplx(nbmax, max(nlomax, 1), nlm, ndf)
+    gamp = 0.5 * cplx(mbmax, max(nlomax, 1), nlm, ndf)
+    s3r = rng.normal(size=(max_nmix, n_fnc, n_fnc, nat))
+    s3r = 0.5 * (s3r + s3r.transpose(0, 2, 1, 3))
+    loct = int(sum(mult[a] * np.sum(2 * big_l_list[a] + 1) for a in range(nat)))
+    corind = []
+    idf = 0
+    for a in range(nat):
+        for _ in range(mult[a]):
+            for ic, lc in enumerate(core_l[a]):
+                for mc in range(-lc, lc + 1):
+                    corind.append([a, idf, ic, lc, mc])
+            idf += 1
+    maxnt = max(lmax + 1, 2 * (lmbmax + 1) if not big else 8)
+    return MinmMTSystem(mult=mult, nmix=nmix, big_l=big_l, nLO_at=nlo, ncore=ncore, lmax=lmax, loctmatsize=loct,
+                        alfa=alfa, beta=beta, gama=gama, alfp=alfp, betp=betp, gamp=gamp, s3r=s3r,
+                        corind=np.array(corind, dtype=np.int64), core_l=core_l,
+                        vpos=rng.uniform(0, 1, (3, ndf)), ql=np.array([0.25, -0.125, 0.5]),
+                        kl=np.array([0.375, 0.25, -0.125]), cgcoef=cmp_all_gaunt(maxnt))

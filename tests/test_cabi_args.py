@@ -45,6 +45,7 @@ def _zeros(fn):
     "pygw_omega_project", "pygw_omega_expand", "pygw_omega_expand_batched",
     "pygw_ewald_workspace", "pygw_ewald_summation", "pygw_mt_coulomb", "pygw_radial_olap_workspace", "pygw_radial_olap",
     "pygw_zero_momentum_workspace", "pygw_zero_momentum_coulomb",
+    "pygw_minm_mt_plan", "pygw_minm_mt_workspace", "pygw_minm_mt", "pygw_minc", "pygw_minc2",
 ])
 def test_null_and_zero_arguments_rejected(name):
     fn = _fn(name)
@@ -148,3 +149,37 @@ def test_muffin_tin_coulomb_workspaces_and_bounds():
     assert mt(p, 1, 60, p, p, p, ims, p, 2, 6, p, 2145, p, p, 50, p, p, 6, p, None) == E_ARG
     ro = _fn("pygw_radial_olap")
     assert ro(0, 1, p, p, p, 4, 3, 781, p, 0.01, p, 100, p, p, 1024, None) == E_ARG   # workspace too small
+
+
+def test_minm_mt_plan_layout():
+    """The plan of two atom types (mult 1, 2), lmax 3: loctmatsize, channel
+    counts (u and udot of 16 (l, m) plus the local orbitals), tiles of 64
+    product functions, and the size query; bad arguments rejected."""
+    f = _fn("pygw_minm_mt_plan")
+    ints = lambda *v: (C.c_int * len(v))(*v)  # noqa: E731
+    mult, nmix, ncore = ints(1, 2), ints(6, 5), ints(2, 1)
+    big_l = ints(0, 0, 1, 1, 2, 3, 0, 1, 1, 2, 0, 0)          # (max_nmix 6, nat 2) Fortran
+    nlo = ints(1, 0, 2, 0, 0, 1, 1, 0)                         # (lomax+1 4, nat 2) Fortran
+    n = C.c_int(0)
+    assert f(2, mult, nmix, big_l, 6, nlo, 3, ncore, 3, None, C.byref(n)) == 0 and n.value > 12
+    plan = (C.c_int * n.value)()
+    assert f(2, mult, nmix, big_l, 6, nlo, 3, ncore, 3, plan, C.byref(n)) == 0
+    h = list(plan)
+    nat, ndf, lmax, nlm, cmax, loct, ntile = h[:7]
+    assert (nat, ndf, lmax, nlm) == (2, 3, 3, 16)
+    assert loct == (1 + 1 + 3 + 3 + 5 + 7) + 2 * (1 + 3 + 3 + 5 + 1)
+    # channels: type 0: 32 + 1 (l=0) + 2*5 (l=2) = 43; type 1: 32 + 3 + 5 = 40
+    assert cmax == 43 and ntile == 3
+    q = _fn("pygw_minm_mt_workspace")
+    s = C.c_size_t(0)
+    assert q(plan, 5, 7, C.byref(s)) == 0 and s.value == (5 * 43 * loct * 16 + 255) // 256 * 256 + 3 * 43 * 7 * 16
+    small = C.c_int(n.value - 1)
+    assert f(2, mult, nmix, big_l, 6, nlo, 3, ncore, 3, plan, C.byref(small)) == E_ARG
+    assert f(2, ints(1, 0), nmix, big_l, 6, nlo, 3, ncore, 3, None, C.byref(n)) == E_ARG   # mult 0
+    p = C.c_void_p(16)
+    mt = _fn("pygw_minm_mt")
+    # a window past nbmax, and nlomax below the plan's largest nLO_at (2)
+    assert mt(plan, p, p, 0, 8, 0, 5, p, p, p, 7, 7, p, p, p, 9, 9, 2, p, 6, 13, p, p, 100, p, loct, 0, None, 0,
+              None) == E_ARG
+    assert mt(plan, p, p, 0, 5, 0, 5, p, p, p, 7, 7, p, p, p, 9, 9, 1, p, 6, 13, p, p, 100, p, loct, 0, None, 0,
+              None) == E_ARG
