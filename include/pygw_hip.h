@@ -607,6 +607,44 @@ int pygw_minc2(const int* plan_host, const int* plan_dev, const double* kmql, in
                const double* s3r, int n_mix, int n_fnc, const int* corind, int ncg_rows, const double* pos,
                const double* cgcoef, int n_cg, void* mmat, long ld_mmat, int col0, void* stream);
 
+/* ------------------------------------------------------------------------
+ * The LAPW augmentation coefficients and the band coefficients (lapw.hip).
+ * pygw_lapw_plan (host only) packs what PyGW passes to
+ *   lapwc.gap2_set_lapwcoef that does not depend on k (pygw.py:2571):
+ *   mult [nat], lapw (nt, nat), nlo and nLO_at (lomax+1, nat) int32, umt
+ *   (nat, nt, 5), abcelo (nat, lomax+1, nloat, 3), rmt [nat], pos (3, ndf),
+ *   rotloc (nat, 3, 3), trotij (ndf, 3, 3), k2cartes (3, 3) float64 (host
+ *   arrays, Fortran layouts) and Vol: with plan = NULL it returns the size in
+ *   *plan_doubles, else fills plan.  nt <= 16; the local orbitals of each
+ *   (l, atom type) must fit their lohns block (nLO_at <= nlo - 1 + lapw) and
+ *   abcelo (nLO_at + 1 <= nloat); an APW l needs p = umt(:, l, 1) != 0.  The
+ *   same plan is passed in host (plan_host) and device (plan_dev) memory.
+ * pygw_lapw_coef replaces lapwc.gap2_set_lapwcoef(kl, indgk, iop, ifrot, nvk,
+ *   gindex, abcelo, rmt, pos, mult, umt, rotloc, trotij, Vol, k2cartes,
+ *   nLO_at, nlo, lapw, nLOmax) (pygw.py:2571,2595,2857;
+ *   lapwcoeff.f90:3-222): alm, blm (ngk, nt^2, ndf) and clm (ngk, nLOmax,
+ *   nt^2, ndf) complex128 on the device, Fortran order, every entry
+ *   written.  kl [3] host; indgk [ngk] int32 (0-based rows of gindex, the
+ *   first nvk plane waves), gindex (ng, 3) int32 Fortran, on the device; a
+ *   row whose index is not a row of gindex is written as NaN.  clm may be
+ *   NULL when the plan has no local orbitals.  Allocates nothing.
+ * pygw_lapw_band replaces alfa = la.matmul(Aeig, reshape(alm, (ngk, ncol)))
+ *   and its beta/gama twins (pygw.py:2577-2583, 2601-2607, 2863-2869):
+ *   alfa/beta (nb, ncol), gama (nb, ncol nlomax) complex128 Fortran =
+ *   A (alm, blm, clm read as (ngk, ...) Fortran), A(n, g) = Aeig[n lda + g]
+ *   (conjugated with conj_a, as PyGW forms Aeigq, pygw.py:2587), ncol =
+ *   nt^2 ndf.  FP64 (MFMA).  Allocates nothing.
+ * ------------------------------------------------------------------------ */
+int pygw_lapw_plan(int nat, const int* mult, int nt, int lomax, int nloat, const int* lapw, const int* nlo,
+                   const int* nLO_at, const double* umt, const double* abcelo, const double* rmt, const double* pos,
+                   const double* rotloc, const double* trotij, double vol, const double* k2cartes, double* plan,
+                   int* plan_doubles);
+int pygw_lapw_coef(const double* plan_host, const double* plan_dev, const double* kl, int iop, int ifrot, int nvk,
+                   const int* indgk, int ngk, const int* gindex, int ng, void* alm, void* blm, void* clm,
+                   void* stream);
+int pygw_lapw_band(const void* Aeig, long lda, int conj_a, int nb, int ngk, const void* alm, const void* blm,
+                   const void* clm, long ncol, int nlomax, void* alfa, void* beta, void* gama, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

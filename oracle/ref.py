@@ -357,3 +357,25 @@ def calc_minc2(kmql, cstart, cend, mstart, mend, alfp, betp, gamp, s3r, corind, 
       _i(al.shape[1]), _i(ga.shape[1]), _i(al.shape[2]), _i(s3.shape[0]), _i(s3.shape[1]), _i(bl.shape[0]),
       _i(nlo.shape[0] - 1), _i(ga.shape[0]), _i(ga.shape[2]), _i(ga.shape[3]), _i(co.shape[0]))
     return np.ascontiguousarray(out)
+
+
+def gap2_set_lapwcoef(kl, indgk, iop, ifrot, nvk, gindex, abcelo, rmt, pos, mult, umt, rotloc, trotij, vol,
+                      k2cartes, nLO_at, nlo, lapw, nLOmax):
+    """lapwcoeff.f90:3-222 (with sphbes/ylm of sphbes.f90); arrays as PyGW
+    passes them, logical ifrot as a 4-byte 0/1."""
+    f = _load("libref_lapw.so").gap2_set_lapwcoef_
+    F = lambda a, dt: np.asfortranarray(np.asarray(a, dtype=dt))  # noqa: E731
+    ig, gi = F(indgk, np.int32), F(gindex, np.int32)
+    ab, rm, ps, mu, um = F(abcelo, np.float64), F(rmt, np.float64), F(pos, np.float64), F(mult, np.int32), \
+        F(umt, np.float64)
+    rl, tr, kc = F(rotloc, np.float64), F(trotij, np.float64), F(k2cartes, np.float64)
+    nl, no, lw = F(nLO_at, np.int32), F(nlo, np.int32), F(lapw, np.int32)
+    k = F(kl, np.float64)
+    ngk, nat, nt, ndf = ig.size, mu.size, um.shape[1], ps.shape[1]
+    alm = np.zeros((ngk, nt * nt, ndf), dtype=np.complex128, order="F")
+    blm = np.zeros_like(alm, order="F")
+    clm = np.zeros((ngk, max(nLOmax, 1), nt * nt, ndf), dtype=np.complex128, order="F")
+    f(_p(alm), _p(blm), _p(clm), _p(k), _p(ig), _i(iop), _i(1 if ifrot else 0), _i(nvk), _p(gi), _p(ab), _p(rm),
+      _p(ps), _p(mu), _p(um), _p(rl), _p(tr), _d(vol), _p(kc), _p(nl), _p(no), _p(lw), _i(nLOmax), _i(nat), _i(nt),
+      _i(ngk), _i(gi.shape[0]), _i(ndf), _i(nl.shape[0] - 1), _i(ab.shape[2]))
+    return np.ascontiguousarray(alm), np.ascontiguousarray(blm), np.ascontiguousarray(clm[:, :nLOmax])

@@ -774,3 +774,109 @@ def minm_mt_system(lmax=3, lmbmax=2, nbmax=7, mbmax=9, los=True, big=False, seed
                         corind=np.array(corind, dtype=np.int64), core_l=core_l,
                         vpos=rng.uniform(0, 1, (3, ndf)), ql=np.array([0.25, -0.125, 0.5]),
                         kl=np.array([0.375, 0.25, -0.125]), cgcoef=cmp_all_gaunt(maxnt))
+
+
+@dataclass
+class LapwSystem:
+    """Inputs of gap2_set_lapwcoef (lapwcoeff.f90:3-222) for one k-point:
+    the per-run arrays in f2py's shapes (abcelo [nat, lomax+1, nloat, 3],
+    umt [nat, nt, 5], rotloc [nat, 3, 3], trotij [ndf, 3, 3], pos [3, ndf],
+    nLO_at/nlo [lomax+1, nat], lapw [nt, nat]), the G list gindex [ng, 3],
+    the k-point's rows indgk (nvk plane waves by |k+G|, then the
+    local-orbital rows) and eigenvectors Aeig [nbmax, ngk]."""
+    abcelo: np.ndarray
+    rmt: np.ndarray
+    pos: np.ndarray
+    mult: np.ndarray
+    umt: np.ndarray
+    rotloc: np.ndarray
+    trotij: np.ndarray
+    vol: float
+    k2cartes: np.ndarray
+    nLO_at: np.ndarray
+    nlo: np.ndarray
+    lapw: np.ndarray
+    nlomax: int
+    gindex: np.ndarray
+    kl: np.ndarray
+    indgk: np.ndarray
+    nvk: int
+    Aeig: np.ndarray
+    alat: float
+
+    def plan_args(self):
+        return (self.abcelo, self.rmt, self.pos, self.mult, self.umt, self.rotloc, self.trotij, self.vol,
+                self.k2cartes, self.nLO_at, self.nlo, self.lapw)
+
+    def coef_args(self, iop=1, ifrot=True):
+        return (self.kl, self.indgk, iop, ifrot, self.nvk, self.gindex, self.abcelo, self.rmt, self.pos, self.mult,
+                self.umt, self.rotloc, self.trotij, self.vol, self.k2cartes, self.nLO_at, self.nlo, self.lapw,
+                self.nlomax)
+
+
+def lapw_system(lmax=3, lomax=3, nvk=60, nbmax=12, mult=(1, 2), apw=(False, True), los=True, cubic=False,
+                wronskian=False, kl=(0.25, -0.125, 0.375), gmax=4, alat=8.0, seed=23) -> LapwSystem:
+    """A cell of len(mult) atom types.  apw[a]: atom type a is APW+lo for
+    l <= 2 and LAPW above (all LAPW without local orbitals); local orbitals (los) on l = 0 and 2
+    of the first type and l = 1 of every type.  cubic: a simple cubic cell
+    (k2cartes = 2 pi/alat I, so 2 pi (G+k).pos is (k+G).r in Cartesian
+    units).  wronskian: umt with dp pe - dpe p = 1/R^2, the normalisation
+    under which the coefficients match a plane wave in value and slope on the
+    sphere."""
+    rng = np.random.default_rng(seed)
+    mult = np.asarray(mult, dtype=np.int64)
+    nat, ndf, nt = len(mult), int(np.sum(mult)), lmax + 1
+    if cubic:
+        rbas = alat * np.eye(3)
+    else:
+        rbas = alat * (np.eye(3) + 0.15 * rng.uniform(-1, 1, (3, 3)))
+    vol = abs(np.linalg.det(rbas))
+    k2cartes = 2 * np.pi * np.linalg.inv(rbas)      # columns: reciprocal vectors
+    lapw = np.ones((nt, nat), dtype=np.int64)
+    nlo = np.zeros((lomax + 1, nat), dtype=np.int64)
+    for a in range(nat):
+        if apw[a] and los:                          # APW l always carries its lo
+            lapw[: min(3, nt), a] = 0
+            if los:
+                nlo[: min(3, lomax + 1), a] = 1     # the APW+lo orbitals
+        if los:
+            if lomax >= 1 and lmax >= 1:
+                nlo[1, a] += 1
+            if a == 0:
+                nlo[0, a] += 1
+                if lomax >= 2 and lmax >= 2:
+                    nlo[2, a] += 1
+    nLO_at = np.zeros_like(nlo)
+    for a in range(nat):
+        for l in range(lomax + 1):
+            nLO_at[l, a] = nlo[l, a] - (1 - lapw[l, a]) if l < nt else nlo[l, a]
+    nLO_at = np.maximum(nLO_at, 0)
+    nloat = int(nlo.max()) + 1 if nlo.size else 1
+    rmt = rng.uniform(1.6, 2.2, nat)
+    umt = np.zeros((nat, nt, 5))
+    for a in range(nat):
+        for l in range(nt):
+            p, pe, dp = rng.uniform(0.5, 1.5) * rng.choice([-1, 1]), rng.uniform(-1, 1), rng.uniform(-1, 1)
+            dpe = (dp * pe - 1.0 / rmt[a] ** 2) / p if wronskian else rng.uniform(-1, 1)
+            umt[a, l] = (p, pe, dp, dpe, rng.uniform(0.1, 1))
+    abcelo = rng.normal(size=(nat, lomax + 1, nloat, 3))
+    rotloc = np.stack([_rotation(rng) for _ in range(nat)])
+    trotij = np.stack([_rotation(rng) for _ in range(ndf)])
+    pos = rng.uniform(0, 1, (3, ndf))
+    r = np.arange(-gmax, gmax + 1)
+    G = np.array([(i, j, k) for i in r for j in r for k in r], dtype=np.int64)
+    kl = np.asarray(kl, dtype=np.float64)
+    kpg = np.linalg.norm((G + kl) @ k2cartes.T, axis=1)
+    order = np.argsort(kpg, kind="stable")
+    gindex = G[order]
+    kpg = kpg[order]
+    nlo_tot = int(sum(nlo[l, a] * (2 * l + 1) * mult[a] for a in range(nat) for l in range(lomax + 1)))
+    # plane waves: the nvk shortest k+G; local-orbital rows: further G vectors
+    lo_rows = rng.choice(np.arange(nvk, min(len(G), nvk + 4 * nlo_tot + 8)), size=nlo_tot, replace=False) \
+        if nlo_tot else np.zeros(0, dtype=np.int64)
+    indgk = np.concatenate([np.arange(nvk), lo_rows]).astype(np.int64)
+    ngk = indgk.size
+    Aeig = (rng.normal(size=(nbmax, ngk)) + 1j * rng.normal(size=(nbmax, ngk))) / np.sqrt(ngk)
+    return LapwSystem(abcelo=abcelo, rmt=rmt, pos=pos, mult=mult, umt=umt, rotloc=rotloc, trotij=trotij, vol=vol,
+                      k2cartes=k2cartes, nLO_at=nLO_at, nlo=nlo, lapw=lapw, nlomax=int(nLO_at.max()),
+                      gindex=gindex, kl=kl, indgk=indgk, nvk=nvk, Aeig=Aeig, alat=alat)

@@ -46,6 +46,7 @@ def _zeros(fn):
     "pygw_ewald_workspace", "pygw_ewald_summation", "pygw_mt_coulomb", "pygw_radial_olap_workspace", "pygw_radial_olap",
     "pygw_zero_momentum_workspace", "pygw_zero_momentum_coulomb",
     "pygw_minm_mt_plan", "pygw_minm_mt_workspace", "pygw_minm_mt", "pygw_minc", "pygw_minc2",
+    "pygw_lapw_plan", "pygw_lapw_coef", "pygw_lapw_band",
 ])
 def test_null_and_zero_arguments_rejected(name):
     fn = _fn(name)
@@ -183,3 +184,39 @@ def test_minm_mt_plan_layout():
               None) == E_ARG
     assert mt(plan, p, p, 0, 5, 0, 5, p, p, p, 7, 7, p, p, p, 9, 9, 1, p, 6, 13, p, p, 100, p, loct, 0, None, 0,
               None) == E_ARG
+
+
+def test_lapw_plan_layout_and_checks():
+    """The plan of lapw_system's two atom types (mult 1, 2; nt 4, lomax 3,
+    nloat 3): header and size; inconsistent local orbitals, a zero APW p and
+    a short buffer rejected."""
+    import numpy as np
+    from pygw_amd import synth
+    s = synth.lapw_system()
+    f = _fn("pygw_lapw_plan")
+    I = lambda a: np.ascontiguousarray(np.asarray(a).ravel(order="F"), dtype=np.int32)  # noqa: E731
+    D = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))  # noqa: E731
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    nt, lomax, nloat = s.umt.shape[1], s.nLO_at.shape[0] - 1, s.abcelo.shape[2]
+
+    def call(nlo_at=s.nLO_at, umt=s.umt, plan=None, n=None):
+        arrs = [I(s.mult), I(s.lapw), I(s.nlo), I(nlo_at), D(umt), D(s.abcelo), D(s.rmt), D(s.pos), D(s.rotloc),
+                D(s.trotij), D(s.k2cartes)]
+        n = n or C.c_int(0)
+        rc = f(2, P(arrs[0]), nt, lomax, nloat, P(arrs[1]), P(arrs[2]), P(arrs[3]), P(arrs[4]), P(arrs[5]),
+               P(arrs[6]), P(arrs[7]), P(arrs[8]), P(arrs[9]), s.vol, P(arrs[10]), plan, C.byref(n))
+        return rc, n.value
+
+    rc, n = call()
+    atw = 3 + nt + 2 * (lomax + 1) + 4 * nt + 3 * (lomax + 1) * nloat + 9
+    assert rc == 0 and n == 16 + 9 + 2 * atw + 13 * 3
+    plan = np.zeros(n)
+    assert call(plan=P(plan), n=C.c_int(n))[0] == 0
+    assert list(plan[:6]) == [2, nt, lomax, nloat, 3, 1] and plan[6] == s.vol
+    assert call(plan=P(plan), n=C.c_int(n - 1))[0] == E_ARG
+    bad = s.nLO_at.copy()
+    bad[1, 1] += 1          # more local orbitals than the lohns block holds
+    assert call(nlo_at=bad)[0] == E_ARG
+    u = s.umt.copy()
+    u[1, 0, 0] = 0.0        # APW l = 0 of type 1 with p = 0
+    assert call(umt=u)[0] == E_ARG
