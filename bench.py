@@ -1291,6 +1291,126 @@ def run_minm_is(args):
     print(json.dumps(rec))
 
 
+def _minm_mt_inputs(nvk=600, nb=140, seed=20261213):
+    """A PyGW-sized muffin-tin system: 4 atom types of 2 atoms, lmax 10,
+    local orbitals, product bases of 26-30 functions per type (L <= 3), 600
+    plane waves per k-point, 140 bands; the LAPW inputs at k and k - q and
+    the matrix-element inputs on the same atoms."""
+    import dataclasses
+
+    from pygw_amd import synth
+    sk = synth.lapw_system(lmax=10, lomax=10, nvk=nvk, nbmax=nb, mult=(2, 2, 2, 2), apw=(True, False, True, False),
+                           gmax=7, alat=11.0, kl=(0.125, 0.25, -0.375), seed=seed)
+    sq = synth.lapw_system(lmax=10, lomax=10, nvk=nvk, nbmax=nb, mult=(2, 2, 2, 2), apw=(True, False, True, False),
+                           gmax=7, alat=11.0, kl=(0.0, 0.125, -0.25), seed=seed)
+    base = synth.minm_mt_system(big=True, lmax=10, nbmax=nb, mbmax=nb)
+    nfnc = max(int(base.ncore[a]) + 2 * 11 + int(sk.nLO_at[:, a].sum()) for a in range(4))
+    rng = np.random.default_rng(seed)
+    s3r = rng.normal(size=(base.s3r.shape[0], nfnc, nfnc, 4))
+    s3r = 0.5 * (s3r + s3r.transpose(0, 2, 1, 3))
+    s = dataclasses.replace(base, mult=sk.mult, nLO_at=sk.nLO_at, s3r=s3r, vpos=sk.pos)
+    return sk, sq, s
+
+
+def run_minm_mt(args):
+    """The muffin-tin and core matrix elements of one q as calc_minm builds
+    them (pygw.py:2559-2680, polarization mode): per k, gap2_set_lapwcoef at
+    k and at k - q, the band coefficients alfa/beta/gama and alfp/betp/gamp
+    of all 140 bands, calc_minm_MT over the 20 occupied x 120 empty window
+    and calc_minc2 over the core states; 512 k, the same synthetic inputs at
+    every k (the work per k is the same)."""
+    import torch
+
+    from pygw_amd import coulomb, lapw
+    nk, win = 512, (0, 20, 20, 140)
+    sk, sq, s = _minm_mt_inputs()
+    dev = lambda x, dt: torch.as_tensor(np.asarray(x), dtype=dt, device="cuda")  # noqa: E731
+    plan = lapw.LapwPlan(*sk.plan_args())
+    mplan = coulomb.MinmMTPlan(s.mult, s.nmix, s.big_l, s.nLO_at, s.ncore, s.lmax)
+    ig_k, ig_q = dev(sk.indgk, torch.int32), dev(sq.indgk, torch.int32)
+    gk = dev(np.asarray(sk.gindex).ravel(order="F"), torch.int32)
+    gq = dev(np.asarray(sq.gindex).ravel(order="F"), torch.int32)
+    Ak, Aq = dev(sk.Aeig, torch.complex128), dev(sq.Aeig, torch.complex128)
+    ck = lapw.gap2_set_lapwcoef(plan, sk.kl, ig_k, 1, True, sk.nvk, gk)
+    cq = lapw.gap2_set_lapwcoef(plan, sq.kl, ig_q, 2, True, sq.nvk, gq)
+    bk = lapw.band_coefficients(Ak, *ck)
+    bq = lapw.band_coefficients(Aq, *cq, conj=True)
+    n1, n2, ncg, loct = win[1] - win[0], win[3] - win[2], len(s.corind), s.loctmatsize
+    mm = torch.empty((n1, n2, loct), dtype=torch.complex128, device="cuda")
+    mc = torch.empty((ncg, n2, loct), dtype=torch.complex128, device="cuda")
+    import ctypes as C
+    from pygw_amd import _lib
+    sz = C.c_size_t(0)
+    _lib.call("pygw_minm_mt_workspace", mplan.ptrs()[0], n1, n2, C.byref(sz))
+    ws = torch.empty(sz.value, dtype=torch.uint8, device="cuda")
+    consts = coulomb.mt_constants(mplan, s.s3r, s.vpos, s.cgcoef, s.corind)
+    st = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    tt = np.zeros(4)
+
+    def step():
+        times = np.zeros(4)
+        for _ in range(nk):
+            ev[0].record(st)
+            lapw.gap2_set_lapwcoef(plan, sk.kl, ig_k, 1, True, sk.nvk, gk, out=ck)
+            lapw.gap2_set_lapwcoef(plan, sq.kl, ig_q, 2, True, sq.nvk, gq, out=cq)
+            ev[1].record(st)
+            lapw.band_coefficients(Ak, *ck, out=bk)
+            lapw.band_coefficients(Aq, *cq, conj=True, out=bq)
+            ev[2].record(st)
+            coulomb.calc_minm_mt(mplan, s.ql, *win, *bk, *bq, consts, None, None, out=mm, workspace=ws)
+            ev[3].record(st)
+            coulomb.calc_minc2(mplan, sq.kl, 0, ncg, win[2], win[3], *bq, consts, None, None, None, out=mc)
+            ev[4].record(st)
+            torch.cuda.synchronize()
+            times += [ev[i].elapsed_time(ev[i + 1]) / 1e3 for i in range(4)]
+        return times
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    for _ in range(args.steps):
+        tt += step()
+    tt /= args.steps
+    t = float(tt.sum())
+    ngk, ntnt, ndf, nlo = ck[0].shape[0], ck[0].shape[1], ck[0].shape[2], plan.nlomax
+    f_band = 2 * 8.0 * sk.Aeig.shape[0] * ngk * ntnt * ndf * (2 + nlo)      # both sides, per k
+    print(f"[bench] minm_mt {t:.3f} s/q (coef {tt[0]:.3f}, band {tt[1]:.3f} = {f_band * nk / tt[1] / 1e12:.1f} TF/s, "
+          f"minm_mt {tt[2]:.3f}, minc2 {tt[3]:.3f})", file=sys.stderr, flush=True)
+    rec = {"metric": "muffin-tin + core matrix elements (LAPW coefficients -> alfa/beta/gama -> calc_minm_MT, "
+                     "calc_minc2) seconds per q (all k)", "value": t, "unit": "s/q",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"minm_mt: {nk} k, 4 atom types x 2 atoms, lmax 10, {sk.nvk} plane waves + "
+                                  f"{ngk - sk.nvk} local-orbital rows, {sk.Aeig.shape[0]} bands, window {n1} x {n2}, "
+                                  f"loctmatsize {loct}, {ncg} core states"},
+           "stages_s": {"lapw_coefficients": tt[0], "band_coefficients": tt[1], "calc_minm_mt": tt[2],
+                        "calc_minc2": tt[3]},
+           "roofline": {"bound": "mfma", "kernel": "pygw_lapw_band_k", "achieved": f_band * nk / tt[1] / 1e12,
+                        "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": f_band * nk / tt[1] / PEAK_FP64,
+                        "traffic": None, "flops_per_k": f_band,
+                        "note": "the band-coefficient products (8 FLOP per complex multiply-add, k and k - q); "
+                                "time of whole calls"}}
+    if not args.no_cpu:
+        from oracle import lapw as OL
+        from oracle import minm_mt as OM
+        from oracle import ref
+        t0 = time.perf_counter()
+        wk = OL.band_coefficients(sk.Aeig, *ref.gap2_set_lapwcoef(*sk.coef_args(1, True)))
+        wq = OL.band_coefficients(np.conj(sq.Aeig), *ref.gap2_set_lapwcoef(*sq.coef_args(2, True)))
+        t1 = time.perf_counter()
+        want = OM.calc_minm_mt(s.ql, *win, *wk, *wq, s.s3r, s.vpos, s.mult, s.nmix, s.big_l, s.nLO_at, s.ncore,
+                               s.cgcoef, s.lmax, s.loctmatsize)
+        t2 = time.perf_counter()
+        rec["max_rel_error_vs_oracle"] = float(np.abs(mm.cpu().numpy() - want).max() / np.abs(want).max())
+        rec["cpu_baseline"] = {"value": (t2 - t0) * nk, "unit": "s/q", "cores": int(os.environ.get(
+            "OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port", **host_info(),
+            "stages_s": {"lapw_coefficients_and_band_products": (t1 - t0) * nk, "calc_minm_mt": (t2 - t1) * nk},
+            "sample": f"1 of {nk} k, x{nk}: PyGW's compiled gap2_set_lapwcoef (1 core) and numpy products for the "
+                      f"coefficients, the numpy restatement of calc_minm_MT (the compiled one needs BLAS zgemm, "
+                      f"absent here); calc_minc2 not timed"}
+    print(json.dumps(rec))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -1303,7 +1423,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.steps is None:
-        args.steps = {"c5": 8, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2, "coulomb": 2, "minm_is": 2, "mt_coulomb": 3}.get(args.config, 1)
+        args.steps = {"c5": 8, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2, "coulomb": 2, "minm_is": 2, "mt_coulomb": 3, "minm_mt": 2}.get(args.config, 1)
     if args.copy_calibration:
         # one copy kernel that reads and writes exactly 1 GiB (one launch: under
         # 2 GiB), its source evicted from the 256 MB last-level cache by a 2 GiB
@@ -1336,6 +1456,9 @@ def main():
         return
     if args.config == "mt_coulomb":
         run_mt_coulomb(args)
+        return
+    if args.config == "minm_mt":
+        run_minm_mt(args)
         return
     mixed = not args.no_mixed
     wl = Workload(args.config, args.nq, mixed, args.fp64_kfrac)

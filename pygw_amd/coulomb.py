@@ -526,6 +526,50 @@ def _bands(a, device):
     return torch.as_tensor(np.asarray(a, dtype=np.complex128).ravel(order="F"), device=device)
 
 
+@dataclass
+class MTConstants:
+    """The per-run inputs of calc_minm_MT / calc_minc / calc_minc2 on the
+    device (s3r, vpos, cgcoef and corind in the layouts the kernels read):
+    built once by ``mt_constants`` and passed in place of s3r, so a k-loop
+    does not upload them again on every call."""
+    s3r: torch.Tensor
+    n_mix: int
+    n_fnc: int
+    pos: torch.Tensor
+    cg: torch.Tensor
+    corind: torch.Tensor
+    ncg_rows: int
+    corind_host: np.ndarray
+
+
+def mt_constants(plan, s3r, vpos, cgcoef, corind=None):
+    """Upload s3r [n_mix, n_fnc, n_fnc, nat], vpos [3, ndf], cgcoef and corind
+    [ncg, 5] (checked against the plan as calc_minc checks them) once."""
+    device = plan.device
+    s3 = np.asarray(s3r, dtype=np.float64)
+    co = np.zeros((0, 5), dtype=np.int64) if corind is None else np.asarray(corind)
+    if co.size:
+        _check_corind(plan, co)
+    return MTConstants(s3r=_dev(s3.ravel(order="F"), torch.float64, device), n_mix=s3.shape[0], n_fnc=s3.shape[1],
+                       pos=_dev(np.asarray(vpos, dtype=np.float64).ravel(order="F"), torch.float64, device),
+                       cg=_dev(np.asarray(cgcoef, dtype=np.float64), torch.float64, device),
+                       corind=_dev(co.ravel(order="F"), torch.int32, device), ncg_rows=co.shape[0],
+                       corind_host=co)
+
+
+def _consts(plan, s3r, vpos, cgcoef, corind=None):
+    return s3r if isinstance(s3r, MTConstants) else mt_constants(plan, s3r, vpos, cgcoef, corind)
+
+
+def _check_corind(plan, co, rows=None):
+    ty = plan.host[plan.host[7]:plan.host[7] + 4 * int(plan.host[1])].reshape(-1, 4)[:, 0]
+    sel = co if rows is None else co[rows[0]:rows[1]]
+    if co.ndim != 2 or co.shape[1] != 5 or sel.size == 0 or (sel[:, 1] < 0).any() or \
+            (sel[:, 1] >= len(ty)).any() or (ty[sel[:, 1]] != sel[:, 0]).any() or (sel[:, 2] < 0).any() or \
+            (sel[:, 2] >= np.asarray(plan.ncore)[sel[:, 0]]).any() or (np.abs(sel[:, 4]) > sel[:, 3]).any():
+        raise ValueError("corind rows must be (atom type, atom of that type, core index < ncore, l, |m| <= l)")
+
+
 def calc_minm_mt(plan, ql, nstart, nend, mstart, mend, alfa, beta, gama, alfp, betp, gamp, s3r, vpos, cgcoef, *,
                  out=None, col0=0, workspace=None):
     """fvxcn.calc_minm_mt (pygw.py:2655, for_vxcnn.f90:839-1028):
@@ -535,8 +579,10 @@ def calc_minm_mt(plan, ql, nstart, nend, mstart, mend, alfa, beta, gama, alfp, b
     concatenated mmat).  alfa/beta [nbmax, (lmax+1)^2, ndf], gama [nbmax,
     nLOmax, (lmax+1)^2, ndf] and the primed arrays as PyGW builds them
     (numpy, Fortran semantics), s3r [n_mix, n_fnc, n_fnc, nat], vpos [3, ndf],
-    cgcoef = cmp_all_gaunt(maxnt)."""
+    cgcoef = cmp_all_gaunt(maxnt); or s3r = mt_constants(...) (vpos and
+    cgcoef then unused)."""
     device = plan.device
+    k = _consts(plan, s3r, vpos, cgcoef)
     n1, n2, loct = nend - nstart, mend - mstart, plan.loctmatsize
     al, ap = alfa.shape, alfp.shape
     d = [_bands(x, device) for x in (alfa, beta, alfp, betp)]
@@ -544,11 +590,7 @@ def calc_minm_mt(plan, ql, nstart, nend, mstart, mend, alfa, beta, gama, alfp, b
     ga = gp = None
     if nlo > 0:
         ga, gp = _bands(gama, device), _bands(gamp, device)
-    s3 = np.asarray(s3r, dtype=np.float64)
-    s3d = _dev(s3.ravel(order="F"), torch.float64, device)
     qd = _dev(np.asarray(ql, dtype=np.float64), torch.float64, device)
-    ps = _dev(np.asarray(vpos, dtype=np.float64).ravel(order="F"), torch.float64, device)
-    cg = _dev(np.asarray(cgcoef, dtype=np.float64), torch.float64, device)
     if out is None:
         out = torch.empty((n1, n2, loct), dtype=torch.complex128, device=device)
         ld = loct
@@ -565,8 +607,8 @@ def calc_minm_mt(plan, ql, nstart, nend, mstart, mend, alfa, beta, gama, alfp, b
     nbg = gama.shape[0] if nlo > 0 else 0
     mbg = gamp.shape[0] if nlo > 0 else 0
     _lib.call("pygw_minm_mt", ph, pd, vp(qd), int(nstart), int(nend), int(mstart), int(mend), vp(d[0]), vp(d[1]),
-              vp(ga), al[0], nbg, vp(d[2]), vp(d[3]), vp(gp), ap[0], mbg, nlo, vp(s3d), s3.shape[0],
-              s3.shape[1], vp(ps), vp(cg), cg.numel(), vp(out), ld, int(col0), vp(workspace), workspace.numel(),
+              vp(ga), al[0], nbg, vp(d[2]), vp(d[3]), vp(gp), ap[0], mbg, nlo, vp(k.s3r), k.n_mix, k.n_fnc,
+              vp(k.pos), vp(k.cg), k.cg.numel(), vp(out), ld, int(col0), vp(workspace), workspace.numel(),
               _stream_ptr(device))
     return out
 
@@ -576,19 +618,10 @@ def _minc_common(plan, name, kv, b0, b1, c0, c1, a, b, g, s3r, corind, vpos, cgc
     ad, bd = _bands(a, device), _bands(b, device)
     nlo = plan.nlomax
     gd = _bands(g, device) if nlo > 0 else None
-    s3 = np.asarray(s3r, dtype=np.float64)
-    s3d = _dev(s3.ravel(order="F"), torch.float64, device)
-    co = np.asarray(corind)
-    ty, at = plan.host[plan.host[7]:plan.host[7] + 4 * int(plan.host[1])].reshape(-1, 4)[:, 0], None
-    sel = co[c0:c1] if name == "pygw_minc" else co[b0:b1]
-    if co.ndim != 2 or co.shape[1] != 5 or sel.size == 0 or (sel[:, 1] < 0).any() or \
-            (sel[:, 1] >= len(ty)).any() or (ty[sel[:, 1]] != sel[:, 0]).any() or (sel[:, 2] < 0).any() or \
-            (sel[:, 2] >= np.asarray(plan.ncore)[sel[:, 0]]).any() or (np.abs(sel[:, 4]) > sel[:, 3]).any():
-        raise ValueError("corind rows must be (atom type, atom of that type, core index < ncore, l, |m| <= l)")
-    cod = _dev(co.ravel(order="F"), torch.int32, device)
+    k = _consts(plan, s3r, vpos, cgcoef, corind)
+    rows = (c0, c1) if name == "pygw_minc" else (b0, b1)
+    _check_corind(plan, k.corind_host, rows)
     kd = _dev(np.asarray(kv, dtype=np.float64), torch.float64, device)
-    ps = _dev(np.asarray(vpos, dtype=np.float64).ravel(order="F"), torch.float64, device)
-    cg = _dev(np.asarray(cgcoef, dtype=np.float64), torch.float64, device)
     if out is None:
         out = torch.empty((*shape, plan.loctmatsize), dtype=torch.complex128, device=device)
         ld = plan.loctmatsize
@@ -600,8 +633,8 @@ def _minc_common(plan, name, kv, b0, b1, c0, c1, a, b, g, s3r, corind, vpos, cgc
     vp = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
     nbg = g.shape[0] if nlo > 0 else 0
     _lib.call(name, ph, pd, vp(kd), int(b0), int(b1), int(c0), int(c1), vp(ad), vp(bd), vp(gd), a.shape[0],
-              nbg, nlo, vp(s3d), s3.shape[0], s3.shape[1], vp(cod), co.shape[0], vp(ps), vp(cg), cg.numel(), vp(out),
-              ld, int(col0), _stream_ptr(device))
+              nbg, nlo, vp(k.s3r), k.n_mix, k.n_fnc, vp(k.corind), k.ncg_rows, vp(k.pos), vp(k.cg), k.cg.numel(),
+              vp(out), ld, int(col0), _stream_ptr(device))
     return out
 
 
@@ -609,7 +642,8 @@ def calc_minc(plan, kl, nstart, nend, cstart, cend, alfa, beta, gama, s3r, corin
               col0=0):
     """fvxcn.calc_minc (pygw.py:2677, for_vxcnn.f90:1136-1273): mmat_c[ie1, ic,
     im] = <u_im | psi_ie1,k psi*_core>, complex128 [nend-nstart, cend-cstart,
-    loctmatsize]; corind [ncg, 5] (atom type, atom, core index, l, m)."""
+    loctmatsize]; corind [ncg, 5] (atom type, atom, core index, l, m); or
+    s3r = mt_constants(..., corind) (corind, vpos, cgcoef then unused)."""
     return _minc_common(plan, "pygw_minc", kl, nstart, nend, cstart, cend, alfa, beta, gama, s3r, corind, vpos,
                         cgcoef, out, col0, (nend - nstart, cend - cstart))
 

@@ -11,8 +11,8 @@
 //   at x = R |k+G|; a local-orbital row of (iat, l, lo) (lohns,
 //   lapwcoeff.f90:421-449: (2l+1) mult rows per local orbital, atoms and l in
 //   order) holds alo/blo/clo comc in the l block of every atom of type iat.
-//   One thread per (row, atom): the Bessel functions of all l, then Y_lm
-//   by the normalised associated-Legendre recurrence (m outer, l inner) with
+//   One thread per (row, atom, m >= 0): the Bessel functions of all l, then
+//   Y_l,+-m for l >= m by the normalised associated-Legendre recurrence in l,
 //   every alm/blm/clm entry written as it is produced; consecutive threads
 //   hold consecutive rows, so each store of a wave is one contiguous run of
 //   the Fortran-ordered arrays.
@@ -20,7 +20,9 @@
 //   alfa = Aeig alm, beta = Aeig blm, gama = Aeig clm (pygw.py:2577-2583):
 //   one complex GEMM launch on FP64 MFMA for the three (z = 0, 1, 2), the
 //   coefficient arrays read as [ngk, columns] column-major.
-#include "gemm_tile.h"
+#include "common.h"
+
+typedef double d4 __attribute__((ext_vector_type(4)));
 
 #include <algorithm>
 #include <cmath>
@@ -137,14 +139,14 @@ __device__ inline void lo_row(const double* plan, int r, int& iat, int& l, int& 
 }  // namespace lw
 
 // alm/blm [ngk][nt^2][ndf], clm [ngk][nLOmax][nt^2][ndf] (Fortran order)
-__global__ void __launch_bounds__(256) pygw_lapw_coef_k(const double* __restrict__ plan, double kl0, double kl1,
+__global__ void __launch_bounds__(64) pygw_lapw_coef_k(const double* __restrict__ plan, double kl0, double kl1,
                                                        double kl2, int iop, int ifrot, int nvk,
                                                        const int* __restrict__ indgk, int ngk,
                                                        const int* __restrict__ gindex, int ng,
                                                        double2* __restrict__ alm, double2* __restrict__ blm,
                                                        double2* __restrict__ clm) {
   using namespace lw;
-  const int ig = blockIdx.x * blockDim.x + threadIdx.x, idf = blockIdx.y;
+  const int ig = blockIdx.x * blockDim.x + threadIdx.x, idf = blockIdx.y, m = blockIdx.z;
   if (ig >= ngk) return;
   const int nt = (int)plan[H_NT], nlomax = (int)plan[H_NLOMAX];
   const int lmax = nt - 1, ntnt = nt * nt;
@@ -156,11 +158,15 @@ __global__ void __launch_bounds__(256) pygw_lapw_coef_k(const double* __restrict
   const int gi = indgk[ig];
   if (gi < 0 || gi >= ng) {  // not a row of gindex: the row reads as NaN
     const double nan = __builtin_nan("");
-    for (int ilm = 0; ilm < ntnt; ilm++) {
-      const size_t e = (size_t)ig + (size_t)ngk * ((size_t)idf * ntnt + ilm);
-      alm[e] = blm[e] = double2{nan, nan};
-      for (int s = 0; s < nlomax; s++) clm[(size_t)ig + (size_t)ngk * (s + (size_t)nlomax * ((size_t)idf * ntnt + ilm))] = double2{nan, nan};
-    }
+    for (int l = m; l <= lmax; l++)
+      for (int sg = -1; sg <= 1; sg += 2) {
+        if (m == 0 && sg < 0) continue;
+        const int ilm = l * l + l + sg * m;
+        const size_t e = (size_t)ig + (size_t)ngk * ((size_t)idf * ntnt + ilm);
+        alm[e] = blm[e] = double2{nan, nan};
+        for (int s = 0; s < nlomax; s++)
+          clm[(size_t)ig + (size_t)ngk * (s + (size_t)nlomax * ((size_t)idf * ntnt + ilm))] = double2{nan, nan};
+      }
     return;
   }
   const double G[3] = {(double)gindex[gi], (double)gindex[gi + ng], (double)gindex[gi + 2 * ng]};
@@ -252,84 +258,94 @@ __global__ void __launch_bounds__(256) pygw_lapw_coef_k(const double* __restrict
       clm[(size_t)ig + col * (s + (size_t)nlomax * (base + ilm))] = double2{c * zr, c * zi};
     }
   };
+  // P_mm and e^{i m phi} by the same products as a loop over m would form
   double pmm = 1.0 / sqrt(4.0 * 3.14159265358979323846);
   double cm = 1.0, sm = 0.0;
-  for (int m = 0; m <= lmax; m++) {
-    if (m > 0) {
-      pmm *= -sqrt((2.0 * m + 1.0) / (2.0 * m)) * st;
-      const double c2 = cm * cp - sm * sp, s2 = sm * cp + cm * sp;
-      cm = c2;
-      sm = s2;
+  for (int k = 1; k <= m; k++) {
+    pmm *= -sqrt((2.0 * k + 1.0) / (2.0 * k)) * st;
+    const double c2 = cm * cp - sm * sp, s2 = sm * cp + cm * sp;
+    cm = c2;
+    sm = s2;
+  }
+  double p2 = 0.0, p1 = pmm;
+  for (int l = m; l <= lmax; l++) {
+    double p;
+    if (l == m) {
+      p = pmm;
+    } else if (l == m + 1) {
+      p = sqrt(2.0 * m + 3.0) * ct * pmm;
+    } else {
+      const double a = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+      const double b = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+      p = a * (ct * p1 - b * p2);
     }
-    double p2 = 0.0, p1 = pmm;
-    for (int l = m; l <= lmax; l++) {
-      double p;
-      if (l == m) {
-        p = pmm;
-      } else if (l == m + 1) {
-        p = sqrt(2.0 * m + 3.0) * ct * pmm;
-      } else {
-        const double a = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
-        const double b = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
-        p = a * (ct * p1 - b * p2);
-      }
-      if (l > m) {
-        p2 = p1;
-        p1 = p;
-      }
-      const double yr = p * cm, yi = p * sm;
-      put(l, l * l + l + m, yr, yi);
-      if (m > 0) {
-        const double sg = (m & 1) ? -1.0 : 1.0;  // Y_l,-m = (-1)^m conj(Y_lm)
-        put(l, l * l + l - m, sg * yr, -sg * yi);
-      }
+    if (l > m) {
+      p2 = p1;
+      p1 = p;
+    }
+    const double yr = p * cm, yi = p * sm;
+    put(l, l * l + l + m, yr, yi);
+    if (m > 0) {
+      const double sg = (m & 1) ? -1.0 : 1.0;  // Y_l,-m = (-1)^m conj(Y_lm)
+      put(l, l * l + l - m, sg * yr, -sg * yi);
     }
   }
 }
 
 // out[z][n + nb c] = sum_g A(n, g) X_z[g + ngk c] (A(n, g) = Aeig[n lda + g],
-// conjugated with conj_a), columns c < ncol_z
+// conjugated with conj_a), columns c < ncol_z.  Tiles of 32 bands x 32
+// columns (the band window is ~140 wide: 64-row tiles would leave a third of
+// the last one empty and too few blocks to fill 256 CUs), 4 waves of 16 x 16
+// on v_mfma_f64_16x16x4f64.  Both operands are contiguous in g, so the LDS
+// staging walks g fastest for both.
 __global__ void __launch_bounds__(256) pygw_lapw_band_k(const double2* __restrict__ Aeig, long lda, int conj_a,
                                                        int nb, int ngk, const double2* __restrict__ X0,
                                                        const double2* __restrict__ X1, const double2* __restrict__ X2,
                                                        long ncol01, long ncol2, double2* __restrict__ O0,
                                                        double2* __restrict__ O1, double2* __restrict__ O2) {
-  using namespace is;
-  __shared__ double2 As[64][KC + 1];
-  __shared__ double2 Bs[KC][64 + 1];
+  constexpr int T = 32, KC = 32;
+  __shared__ double2 As[T][KC + 1];  // [band][g]
+  __shared__ double2 Bs[T][KC + 1];  // [column][g]
   const int z = blockIdx.z;
   const long ncol = z < 2 ? ncol01 : ncol2;
-  const long n0 = (long)blockIdx.x * 64;
-  const int m0 = blockIdx.y * 64;
+  const long n0 = (long)blockIdx.x * T;
+  const int m0 = blockIdx.y * T;
   if (n0 >= ncol) return;
   const double2* X = z == 0 ? X0 : (z == 1 ? X1 : X2);
   double2* O = z == 0 ? O0 : (z == 1 ? O1 : O2);
   const double cs = conj_a ? -1.0 : 1.0;
-  d4 aR[2][2], aI[2][2];
-  zero(aR, aI);
-  auto la = [&](int m, int k) -> double2 {
-    const int r = m0 + m;
-    if (r >= nb || k >= ngk) return double2{0.0, 0.0};
-    const double2 a = Aeig[(size_t)r * lda + k];
-    return double2{a.x, cs * a.y};
-  };
-  auto lb = [&](int k, int c) -> double2 {
-    const long cc = n0 + c;
-    return (k < ngk && cc < ncol) ? X[(size_t)k + (size_t)ngk * cc] : double2{0.0, 0.0};
-  };
-  tile<2>(ngk, la, lb, As, Bs, aR, aI);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1, li = lane & 15, lk = lane >> 4;
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = m0 + 32 * wr + 16 * a + 4 * v + lk;
-        const long c = n0 + 32 * wc + 16 * b + li;
-        if (r < nb && c < ncol) O[(size_t)r + (size_t)nb * c] = double2{aR[a][b][v], aI[a][b][v]};
+  d4 aR = d4{0, 0, 0, 0}, aI = d4{0, 0, 0, 0};
+  for (int k0 = 0; k0 < ngk; k0 += KC) {
+    for (int e = tid; e < T * KC; e += 256) {
+      const int c = e % KC, m = e / KC, g = k0 + c;
+      double2 a{0.0, 0.0}, b{0.0, 0.0};
+      if (g < ngk) {
+        if (m0 + m < nb) a = Aeig[(size_t)(m0 + m) * lda + g];
+        if (n0 + m < ncol) b = X[(size_t)g + (size_t)ngk * (n0 + m)];
       }
+      As[m][c] = double2{a.x, cs * a.y};
+      Bs[m][c] = b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ks++) {
+      const int kk = ks * 4 + lk;
+      const double2 x = As[16 * wr + li][kk], y = Bs[16 * wc + li][kk];
+      aR = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, aR, 0, 0, 0);
+      aR = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, aR, 0, 0, 0);
+      aI = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, aI, 0, 0, 0);
+      aI = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.x, aI, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int r = m0 + 16 * wr + 4 * v + lk;
+    const long c = n0 + 16 * wc + li;
+    if (r < nb && c < ncol) O[(size_t)r + (size_t)nb * c] = double2{aR[v], aI[v]};
+  }
 }
 
 extern "C" int pygw_lapw_plan(int nat, const int* mult, int nt, int lomax, int nloat, const int* lapw,
@@ -422,7 +438,8 @@ extern "C" int pygw_lapw_coef(const double* plan_host, const double* plan_dev, c
       nvk > ngk || ng <= 0 || (plan_host[H_NLOMAX] > 0 && !clm))
     return PYGW_E_ARG;
   const int ndf = (int)plan_host[H_NDF];
-  hipLaunchKernelGGL(pygw_lapw_coef_k, dim3(ceil_div(ngk, 256), ndf), dim3(256), 0, (hipStream_t)stream, plan_dev,
+  hipLaunchKernelGGL(pygw_lapw_coef_k, dim3(ceil_div(ngk, 64), ndf, (int)plan_host[H_NT]), dim3(64), 0,
+                     (hipStream_t)stream, plan_dev,
                      kl[0], kl[1], kl[2], iop, ifrot ? 1 : 0, nvk, indgk, ngk, gindex, ng, (double2*)alm,
                      (double2*)blm, (double2*)clm);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
@@ -436,7 +453,7 @@ extern "C" int pygw_lapw_band(const void* Aeig, long lda, int conj_a, int nb, in
     return PYGW_E_ARG;
   const long ncol2 = ncol * nlomax;
   const int zs = nlomax > 0 ? 3 : 2;
-  hipLaunchKernelGGL(pygw_lapw_band_k, dim3(ceil_div(std::max(ncol, ncol2), 64), ceil_div(nb, 64), zs), dim3(256), 0,
+  hipLaunchKernelGGL(pygw_lapw_band_k, dim3(ceil_div(std::max(ncol, ncol2), 32), ceil_div(nb, 32), zs), dim3(256), 0,
                      (hipStream_t)stream, (const double2*)Aeig, lda, conj_a, nb, ngk, (const double2*)alm,
                      (const double2*)blm, (const double2*)clm, ncol, ncol2, (double2*)alfa, (double2*)beta,
                      (double2*)gama);

@@ -97,12 +97,14 @@ def gap2_set_lapwcoef(plan, kl, indgk, iop, ifrot, nvk, gindex, *, out=None):
     return a.permute(2, 1, 0), b.permute(2, 1, 0), (c.permute(3, 2, 1, 0) if c is not None else None)
 
 
-def band_coefficients(Aeig, alm, blm, clm, *, conj=False):
+def band_coefficients(Aeig, alm, blm, clm, *, conj=False, out=None):
     """alfa = Aeig alm, beta = Aeig blm, gama = Aeig clm (pygw.py:2577-2583)
     with Aeig [nbmax, ngk] (conj: its conjugate, as PyGW forms Aeigq from
     ks.all_As at k - q, pygw.py:2587).  alm/blm/clm as gap2_set_lapwcoef
     returns them; alfa/beta [nbmax, nt^2, ndf], gama [nbmax, nLOmax, nt^2,
-    ndf] (None without local orbitals), views of Fortran-ordered storage."""
+    ndf] (None without local orbitals), views of Fortran-ordered storage;
+    out = (alfa, beta, gama) from an earlier call of the same sizes is
+    reused."""
     device = alm.device
     A = Aeig if isinstance(Aeig, torch.Tensor) else torch.as_tensor(np.asarray(Aeig, dtype=np.complex128))
     A = A.to(device=device, dtype=torch.complex128)
@@ -114,9 +116,15 @@ def band_coefficients(Aeig, alm, blm, clm, *, conj=False):
         raise ValueError("alm must come from gap2_set_lapwcoef with Aeig's ngk rows")
     ntnt, ndf = alm.shape[1], alm.shape[2]
     nlo = clm.shape[1] if clm is not None else 0
-    alfa = torch.empty((ndf, ntnt, nb), dtype=torch.complex128, device=device)
-    beta = torch.empty_like(alfa)
-    gama = torch.empty((ndf, ntnt, nlo, nb), dtype=torch.complex128, device=device) if nlo > 0 else None
+    if out is None:
+        alfa = torch.empty((ndf, ntnt, nb), dtype=torch.complex128, device=device)
+        beta = torch.empty_like(alfa)
+        gama = torch.empty((ndf, ntnt, nlo, nb), dtype=torch.complex128, device=device) if nlo > 0 else None
+    else:
+        alfa, beta, gama = (x.permute(*reversed(range(x.dim()))) if x is not None else None for x in out)
+        if alfa.shape != (ndf, ntnt, nb) or not alfa.is_contiguous() or not beta.is_contiguous() or \
+                (nlo > 0 and (gama.shape != (ndf, ntnt, nlo, nb) or not gama.is_contiguous())):
+            raise ValueError("out does not match these band coefficients")
     vp = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
     _lib.call("pygw_lapw_band", vp(A), A.stride(0), 1 if conj else 0, nb, ngk, vp(alm), vp(blm), vp(clm),
               ntnt * ndf, nlo, vp(alfa), vp(beta), vp(gama), _stream_ptr(device))

@@ -72,3 +72,20 @@ def test_minc_matches_oracle(crange):
     want = O.calc_minc2(s.kl, c0, c1, 2, 9, s.alfp, s.betp, s.gamp, *args)
     got = C.calc_minc2(p, s.kl, c0, c1, 2, 9, s.alfp, s.betp, s.gamp, s.s3r, s.corind, s.vpos, s.cgcoef)
     assert np.abs(got.cpu().numpy() - want).max() < 1e-13 * np.abs(want).max()
+
+
+def test_uploaded_constants_give_the_same_results():
+    """s3r/vpos/cgcoef/corind uploaded once (mt_constants) in place of the
+    host arrays: calc_minm_mt, calc_minc and calc_minc2 unchanged, bit for bit."""
+    s = synth.minm_mt_system()
+    p = _plan(s)
+    k = C.mt_constants(p, s.s3r, s.vpos, s.cgcoef, s.corind)
+    a = C.calc_minm_mt(p, s.ql, 1, 6, 2, 9, s.alfa, s.beta, s.gama, s.alfp, s.betp, s.gamp, s.s3r, s.vpos, s.cgcoef)
+    b = C.calc_minm_mt(p, s.ql, 1, 6, 2, 9, s.alfa, s.beta, s.gama, s.alfp, s.betp, s.gamp, k, None, None)
+    assert torch.equal(a, b)
+    a = C.calc_minc(p, s.kl, 1, 6, 1, 5, s.alfa, s.beta, s.gama, s.s3r, s.corind, s.vpos, s.cgcoef)
+    b = C.calc_minc(p, s.kl, 1, 6, 1, 5, s.alfa, s.beta, s.gama, k, None, None, None)
+    assert torch.equal(a, b)
+    a = C.calc_minc2(p, s.kl, 0, len(s.corind), 2, 9, s.alfp, s.betp, s.gamp, s.s3r, s.corind, s.vpos, s.cgcoef)
+    b = C.calc_minc2(p, s.kl, 0, len(s.corind), 2, 9, s.alfp, s.betp, s.gamp, k, None, None, None)
+    assert torch.equal(a, b)
