@@ -379,3 +379,72 @@ def gap2_set_lapwcoef(kl, indgk, iop, ifrot, nvk, gindex, abcelo, rmt, pos, mult
       _p(ps), _p(mu), _p(um), _p(rl), _p(tr), _d(vol), _p(kc), _p(nl), _p(no), _p(lw), _i(nLOmax), _i(nat), _i(nt),
       _i(ngk), _i(gi.shape[0]), _i(ndf), _i(nl.shape[0] - 1), _i(ab.shape[2]))
     return np.ascontiguousarray(alm), np.ascontiguousarray(blm), np.ascontiguousarray(clm[:, :nLOmax])
+
+
+def _q0_tables(nLO_at, mult, tables):
+    F = lambda a, dt: np.asfortranarray(np.asarray(a, dtype=dt))  # noqa: E731
+    return F(nLO_at, np.int32), F(mult, np.int32), [F(t, np.float64) for t in tables]
+
+
+def calcmmatvv_mt(n1, n2, alfa, beta, gama, nLO_at, mult, *tables):
+    """for_q_0.f90:10-234 (0-based n1, n2; the nine radial tables in PyGW's
+    order iul_ul, iul_udl, iudl_ul, iudl_udl, iulol_ul, iulol_udl,
+    iul_ulol, iudl_ulol, iulol_ulol)."""
+    f = _load("libref_q0.so").calcmmatvv_mt_
+    F = lambda a: np.asfortranarray(np.asarray(a, dtype=np.complex128))  # noqa: E731
+    al, be, ga = F(alfa), F(beta), F(gama)
+    nl, mu, t = _q0_tables(nLO_at, mult, tables)
+    out = np.zeros(3, dtype=np.complex128)
+    f(_p(out), _i(n1 + 1), _i(n2 + 1), _p(al), _p(be), _p(ga), _p(nl), _p(mu), *[_p(x) for x in t],
+      _i(mu.size), _i(t[0].shape[1]), _i(al.shape[0]), _i(al.shape[1]), _i(al.shape[2]), _i(ga.shape[0]),
+      _i(ga.shape[1]), _i(ga.shape[2]), _i(ga.shape[3]), _i(t[4].shape[1]), _i(t[4].shape[2] - 1),
+      _i(nl.shape[0] - 1))
+    return out
+
+
+def calcmmatvv_mt2(ca1, cb1, cg1, a2, b2, g2, nLO_at, mult, *tables):
+    """for_q_0.f90:236-458."""
+    f = _load("libref_q0.so").calcmmatvv_mt2_
+    F = lambda a: np.asfortranarray(np.asarray(a, dtype=np.complex128))  # noqa: E731
+    x = [F(a) for a in (ca1, cb1, cg1, a2, b2, g2)]
+    nl, mu, t = _q0_tables(nLO_at, mult, tables)
+    out = np.zeros(3, dtype=np.complex128)
+    f(_p(out), *[_p(a) for a in x], _p(nl), _p(mu), *[_p(a) for a in t], _i(mu.size), _i(t[0].shape[1]),
+      _i(x[0].shape[0]), _i(x[0].shape[1]), _i(x[2].shape[0]), _i(x[2].shape[1]), _i(x[2].shape[2]),
+      _i(t[4].shape[1]), _i(t[4].shape[2] - 1), _i(nl.shape[0] - 1))
+    return out
+
+
+def calcmmatvv_is(i, j, kl, Ak, indgk, nvk, ipwint, gindex, i_g0, k2cartes):
+    """for_q_0.f90:460-507 (0-based i, j)."""
+    f = _load("libref_q0.so").calcmmatvv_is_
+    A = np.asfortranarray(np.asarray(Ak, dtype=np.complex128))
+    ig = np.asfortranarray(np.asarray(indgk, dtype=np.int32))
+    pw = np.asfortranarray(np.asarray(ipwint, dtype=np.complex128))
+    gi = np.asfortranarray(np.asarray(gindex, dtype=np.int32))
+    i0 = np.asfortranarray(np.asarray(i_g0, dtype=np.int32))
+    kc = np.asfortranarray(np.asarray(k2cartes, dtype=np.float64))
+    k = np.asfortranarray(np.asarray(kl, dtype=np.float64))
+    out = np.zeros(3, dtype=np.complex128)
+    n = [(i0.shape[c] - 1) // 2 for c in range(3)]
+    f(_p(out), _i(i + 1), _i(j + 1), _p(k), _p(A), _p(ig), _i(nvk), _p(pw), _p(gi), _p(i0), _p(kc), _i(n[0]),
+      _i(n[1]), _i(n[2]), _i(ig.size), _i(gi.shape[0]), _i(A.shape[0]), _i(A.shape[1]))
+    return out
+
+
+def calcmmatcv(iat, iv, corind, alfa, beta, gama, iul_ucl, iudl_ucl, iucl_ul, iucl_udl, iulol_ucl, iucl_ulol,
+               nLO_at):
+    """for_q_0.f90:510-683 (0-based iat, iv); the core tables of atom type
+    iat alone: iul_ucl & co [2, ncg_a], iulol_ucl / iucl_ulol [2, nloat,
+    ncg_a]."""
+    f = _load("libref_q0.so").calcmmatcv_
+    F = lambda a, dt: np.asfortranarray(np.asarray(a, dtype=dt))  # noqa: E731
+    co = F(corind, np.int32)
+    al, be, ga = F(alfa, np.complex128), F(beta, np.complex128), F(gama, np.complex128)
+    t = [F(x, np.float64) for x in (iul_ucl, iudl_ucl, iucl_ul, iucl_udl, iulol_ucl, iucl_ulol)]
+    nl = F(nLO_at, np.int32)
+    out = np.zeros((co.shape[0], 3), dtype=np.complex128, order="F")
+    f(_p(out), _i(iat + 1), _i(iv + 1), _p(co), _p(al), _p(be), _p(ga), *[_p(x) for x in t], _p(nl),
+      _i(co.shape[0]), _i(nl.shape[1]), _i(t[4].shape[1]), _i(al.shape[0]), _i(al.shape[2]), _i(al.shape[1]),
+      _i(ga.shape[0]), _i(ga.shape[1]), _i(ga.shape[2]), _i(ga.shape[3]), _i(t[0].shape[1]), _i(nl.shape[0] - 1))
+    return np.ascontiguousarray(out)

@@ -880,3 +880,128 @@ def lapw_system(lmax=3, lomax=3, nvk=60, nbmax=12, mult=(1, 2), apw=(False, True
     return LapwSystem(abcelo=abcelo, rmt=rmt, pos=pos, mult=mult, umt=umt, rotloc=rotloc, trotij=trotij, vol=vol,
                       k2cartes=k2cartes, nLO_at=nLO_at, nlo=nlo, lapw=lapw, nlomax=int(nLO_at.max()),
                       gindex=gindex, kl=kl, indgk=indgk, nvk=nvk, Aeig=Aeig, alat=alat)
+
+
+@dataclass
+class HeadSystem:
+    """Inputs of calc_head (pygw.py:2808-3037) for nirk irreducible k-points
+    of a full mesh (k_ind = identity, weight 1): per irk the band
+    coefficients (alfa, beta, gama), eigenvectors Aeig [nbmax, ngk], kl and
+    indgk/nvk; the radial integrals of the momentum operator (vv_tables:
+    iul_ul, iul_udl, iudl_ul, iudl_udl [2, lmax, nat]; iulol_ul, iulol_udl,
+    iul_ulol, iudl_ulol [2, nlo, lmax+1, nat]; iulol_ulol [2, nlo, nlo,
+    lmax+1, nat]) and of the core states (cv_tables: iul_ucl, iudl_ucl,
+    iucl_ul, iucl_udl [2, ncore_total]; iulol_ucl, iucl_ulol [2, nloat,
+    ncore_total]); the interstitial step function ipwint over gindex with
+    its index map i_g0; energies Ebnd [nirk, nbnd] (valence first), core
+    energies per core state, weights kcw [n_o, n_e, N_k, n_om] (Fortran
+    semantics, core rows first)."""
+    mult: np.ndarray
+    nLO_at: np.ndarray
+    lmax: int
+    vv_tables: tuple
+    corind: np.ndarray
+    core_off: np.ndarray
+    cv_tables: tuple
+    eig_core: np.ndarray
+    gindex: np.ndarray
+    i_g0: np.ndarray
+    ipwint: np.ndarray
+    k2cartes: np.ndarray
+    vol: float
+    kl: np.ndarray
+    Aeig: list
+    indgk: list
+    nvk: list
+    abc: list
+    Ebnd: np.ndarray
+    nomx: int
+    numin: int
+    nbmaxpol: int
+    ncg_p: int
+    k_ind: np.ndarray
+    weight: np.ndarray
+    kwfer: np.ndarray
+    omega: np.ndarray
+    kcw: np.ndarray
+
+
+def head_system(nirk=3, lmax=3, nbmax=9, nomx=3, numin=4, nvk=30, gk=2, n_om=6, mult=(1, 2), los=True,
+                metal=False, seed=31) -> HeadSystem:
+    """A small cell: band coefficients, plane-wave coefficients and radial
+    integrals drawn at random (the momentum elements are bilinear in them),
+    the plane waves of each k inside |G|_inf <= gk and gindex covering their
+    differences; local orbitals on l <= 2 (nLO_at has lmax + 1 rows); cores
+    (l = 0, 1) on every atom type.  metal: nomx >= numin (bands in both
+    windows), Fermi-surface weights kwfer and Eg < 0."""
+    rng = np.random.default_rng(seed)
+    mult = np.asarray(mult, dtype=np.int64)
+    nat, ndf, nt = len(mult), int(mult.sum()), lmax + 1
+    nLO_at = np.zeros((lmax + 1, nat), dtype=np.int64)
+    if los:
+        nLO_at[:3, 0] = [1, 0, 2][:min(3, lmax + 1)]
+        nLO_at[:3, -1] = np.maximum(nLO_at[:3, -1], [0, 1, 1][:min(3, lmax + 1)])
+    nlo = max(int(nLO_at.max()), 1)
+    R = lambda *s: rng.normal(size=s)  # noqa: E731
+    vv = tuple(R(2, lmax, nat) for _ in range(4)) + tuple(R(2, nlo, lmax + 1, nat) for _ in range(4)) + \
+        (R(2, nlo, nlo, lmax + 1, nat),)
+    core_l = [[0, 1], [0]] + [[0]] * max(0, nat - 2)
+    corind, core_off, off = [], [], 0
+    idf = 0
+    for a in range(nat):
+        core_off.append(off)
+        for _ in range(mult[a]):
+            for ic, lc in enumerate(core_l[a]):
+                for mc in range(-lc, lc + 1):
+                    corind.append([a, idf, ic, lc, mc])
+            idf += 1
+        off += len(core_l[a])
+    nct = off
+    cv = tuple(R(2, nct) for _ in range(4)) + tuple(R(2, nlo, nct) for _ in range(2))
+    corind = np.array(corind, dtype=np.int64)
+    eig_core = np.array([-20.0 - 5.0 * c[2] - 3.0 * c[0] for c in corind])
+    alat = 9.0
+    rbas = alat * (np.eye(3) + 0.1 * rng.uniform(-1, 1, (3, 3)))
+    vol = abs(np.linalg.det(rbas))
+    k2cartes = 2 * np.pi * np.linalg.inv(rbas)
+    gm = 2 * gk
+    r = np.arange(-gm, gm + 1)
+    g = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    g = g[np.lexsort((g[:, 2], g[:, 1], g[:, 0], (g ** 2).sum(1)))]
+    i_g0 = -np.ones((2 * gm + 1,) * 3, dtype=np.int64)
+    i_g0[g[:, 0] + gm, g[:, 1] + gm, g[:, 2] + gm] = np.arange(len(g))
+    # the interstitial step function: ipwint(-G) = conj(ipwint(G)), ipwint(0) real
+    ipw = (R(len(g)) + 1j * R(len(g))) * np.exp(-0.3 * (g ** 2).sum(1))
+    neg = i_g0[-g[:, 0] + gm, -g[:, 1] + gm, -g[:, 2] + gm]
+    ipwint = 0.5 * (ipw + np.conj(ipw[neg]))
+    ipwint[0] = 0.7
+    inner = np.where(np.abs(g).max(axis=1) <= gk)[0]
+    kls, Aeig, indgk, nvks, abc = [], [], [], [], []
+    nlm = (lmax + 1) ** 2
+    nlomax = int(nLO_at.max())
+    for irk in range(nirk):
+        kls.append(rng.uniform(-0.5, 0.5, 3))
+        ig = np.concatenate([rng.permutation(inner)[:nvk], rng.choice(inner, 3)])
+        indgk.append(ig)
+        nvks.append(nvk)
+        Aeig.append((R(nbmax, len(ig)) + 1j * R(nbmax, len(ig))) / np.sqrt(nvk))
+        cplx = lambda *s: R(*s) + 1j * R(*s)  # noqa: E731
+        abc.append((cplx(nbmax, nlm, ndf), 0.3 * cplx(nbmax, nlm, ndf),
+                    0.5 * cplx(nbmax, max(nlomax, 1), nlm, ndf) if nlomax else np.zeros((1, 1, 1, 1), complex)))
+    nbmaxpol = nbmax
+    Ebnd = np.zeros((nirk, nbmax))
+    for irk in range(nirk):
+        e = np.sort(np.concatenate([rng.uniform(-0.8, -0.05, nomx + 1), rng.uniform(0.05, 1.5, nbmax - nomx - 1)]))
+        if metal:
+            e[numin:nomx + 1] = rng.uniform(-0.05, 0.05, nomx + 1 - numin)
+        Ebnd[irk] = np.sort(e)
+    ncg_p = len(corind)
+    n_o, n_e = ncg_p + nomx + 1, nbmaxpol - numin
+    omega = np.geomspace(0.01, 5.0, n_om)
+    kcw = -np.abs(R(n_o, n_e, nirk, n_om)) * 0.01
+    kwfer = np.abs(R(nirk, nbmax)) * (0.1 if metal else 0.0)
+    return HeadSystem(mult=mult, nLO_at=nLO_at, lmax=lmax, vv_tables=vv, corind=corind,
+                      core_off=np.array(core_off), cv_tables=cv, eig_core=eig_core, gindex=g, i_g0=i_g0,
+                      ipwint=ipwint, k2cartes=k2cartes, vol=vol, kl=np.array(kls), Aeig=Aeig, indgk=indgk, nvk=nvks,
+                      abc=abc, Ebnd=Ebnd, nomx=nomx, numin=numin, nbmaxpol=nbmaxpol, ncg_p=ncg_p,
+                      k_ind=np.arange(nirk), weight=np.ones(nirk), kwfer=kwfer, omega=omega, kcw=kcw)
