@@ -645,6 +645,70 @@ int pygw_lapw_coef(const double* plan_host, const double* plan_dev, const double
 int pygw_lapw_band(const void* Aeig, long lda, int conj_a, int nb, int ngk, const void* alm, const void* blm,
                    const void* clm, long ncol, int nlomax, void* alfa, void* beta, void* gama, void* stream);
 
+/* ------------------------------------------------------------------------
+ * The head of the dielectric matrix at q = 0 and its momentum matrix
+ * elements (head.hip).
+ * Common layout arguments (host memory, Fortran layouts): nat atom types
+ * (<= 16) with mult [nat]; nLO_at (nrows = lomax+1 <= 17, nat) int32 (a
+ * count for an l at or beyond nrows reads as 0, where PyGW's Fortran reads
+ * past the array and multiplies zero gama entries); lmax = the second extent
+ * of iul_ul (PyGW's in1.nt - 1): alfa/beta (nbmax, (lmax+1)^2, ndf) and gama
+ * (nbmax_g, nlomax, (lmax+1)^2, ndf) complex128 on the device, gama NULL
+ * when nlomax = 0.
+ * pygw_mmat_vv replaces calc_head's double loop over bands (pygw.py:
+ *   2870-2909): f_q_0.calcmmatvv_mt2 / calcmmatvv_mt (for_q_0.f90:10-458)
+ *   and f_q_0.calcmmatvv_is (for_q_0.f90:460-507) in both orders, averaged:
+ *   mmatvv[(ie2 - mst) (nend - nst) + ie1 - nst][c] complex128.  vv_tables:
+ *   a host array of 9 device pointers (iul_ul, iul_udl, iudl_ul, iudl_udl
+ *   (2, lmax, nat); iulol_ul, iulol_udl, iul_ulol, iudl_ulol (2, nlo, lox,
+ *   nat); iulol_ulol (2, nlo, nlo, lox, nat); float64 Fortran; the last five
+ *   may be NULL when nlo = 0).  Interstitial part (nvk > 0): Aeig [nbmax][lda]
+ *   complex128 row-major, indgk [>= nvk] int32 (0-based rows of gindex (ng, 3)
+ *   int32 Fortran), ipwint [ng] complex128, i_g0 (2 n_g0[0]+1, 2 n_g0[1]+1,
+ *   2 n_g0[2]+1) int32 Fortran on the device; kl [3], n_g0 [3], k2cartes (3, 3)
+ *   in host memory.  FP64 (the overlap product on MFMA); workspace:
+ *   pygw_mmat_vv_workspace bytes.
+ * pygw_mmat_cv replaces f_q_0.calcmmatcv over the atoms with cores (pygw.py:
+ *   2911-2932, for_q_0.f90:510-683): mmatcv[(ie2 - mst) ncg + icg][c].
+ *   cv_tables: 6 device pointers (iul_ucl, iudl_ucl, iucl_ul, iucl_udl (2,
+ *   nct); iulol_ucl, iucl_ulol (2, nloat, nct)), the core states of atom type
+ *   iat from core_off[iat] (host) on; corind (ncg, 5) int32 Fortran on the
+ *   device; a core state with l + 1 > lmax or outside the tables reads as
+ *   NaN.  exact = 0 reproduces the reference (local orbitals at l - 1 times
+ *   iudl_ucl in one term, for_q_0.f90:648), 1 takes iulol_ucl there.
+ * pygw_calc_head replaces calc_head's sums (pygw.py:2940-3025): head[nom]
+ *   complex128 = -sum_irk 4 pi fspin weight[irk]/vol sum_pairs |p|^2/3/dE^2
+ *   kcw(pair, k_ind[irk], iom) (valence pairs with |dE| < 1e-5 dropped), plus
+ *   wpl2/(omega + eta_head)^2 (projected on dUl (nom_w, nom) row-major when
+ *   given) for metallic with iop_drude = 1, wpl2 from kwfer [nirk][ldE].
+ *   mmatvv [nirk][nbmaxpol-numin][nomx+1][3], mmatcv [nirk][..][ncg][3],
+ *   Ebnd [nirk][ldE], eig_core [ncg], kcw (ncg_p+nomx+1, nbmaxpol-numin,
+ *   nk_kcw, nom) Fortran, k_ind [nirk] int32, weight [nirk], all on the
+ *   device; workspace: pygw_calc_head_workspace bytes.
+ * pygw_wing_pm replaces calc_eps's pm of the q = 0 wing (pygw.py:3122-3141):
+ *   pm[k][(io (nbmaxpol - numin) + ie2] for the k-points whose irreducible
+ *   index is kii[k], the input of pygw_chi0_wings.  Allocates nothing.
+ * ------------------------------------------------------------------------ */
+int pygw_mmat_vv_workspace(int nvk, int nst, int nend, int mst, int mend, size_t* bytes);
+int pygw_mmat_vv(int nat, const int* mult, const int* nLO_at, int nrows, int lmax, const void* alfa, const void* beta,
+                 const void* gama, int nbmax, int nbmax_g, int nlomax, const double* const* vv_tables, int nlo,
+                 int lox, const void* Aeig, long lda, const double* kl, const int* indgk, int nvk, const void* ipwint,
+                 const int* gindex, int ng, const int* i_g0, const int* n_g0, const double* k2cartes, int nst,
+                 int nend, int mst, int mend, void* mmatvv, void* workspace, size_t ws_bytes, void* stream);
+int pygw_mmat_cv(int nat, const int* mult, const int* nLO_at, int nrows, int lmax, const int* core_off,
+                 const void* alfa, const void* beta, const void* gama, int nbmax, int nbmax_g, int nlomax,
+                 const double* const* cv_tables, int nloat, int nct, const int* corind, int ncg, int mst, int mend,
+                 int exact, void* mmatcv, void* stream);
+int pygw_calc_head_workspace(int nirk, int n_o, int n_e, size_t* bytes);
+int pygw_calc_head(const void* mmatvv, const void* mmatcv, const double* Ebnd, int ldE, const double* eig_core,
+                   int ncg, const double* kcw, int nk_kcw, const int* k_ind, const double* weight, int nirk, int nomx,
+                   int numin, int nbmaxpol, int ncg_p, int nom, double vol, int metallic, const double* kwfer,
+                   int iop_drude, double eta_head, const double* omega, const double* dUl, int nom_w, void* head,
+                   void* workspace, size_t ws_bytes, void* stream);
+int pygw_wing_pm(const void* mmatvv, const void* mmatcv, const double* Ebnd, int ldE, const double* eig_core, int ncg,
+                 const int* kii, int nk, int nomx, int numin, int nbmaxpol, int ncg_p, double vol, void* pm,
+                 void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,8 @@ def _zeros(fn):
     "pygw_zero_momentum_workspace", "pygw_zero_momentum_coulomb",
     "pygw_minm_mt_plan", "pygw_minm_mt_workspace", "pygw_minm_mt", "pygw_minc", "pygw_minc2",
     "pygw_lapw_plan", "pygw_lapw_coef", "pygw_lapw_band",
+    "pygw_mmat_vv_workspace", "pygw_mmat_vv", "pygw_mmat_cv", "pygw_calc_head_workspace", "pygw_calc_head",
+    "pygw_wing_pm",
 ])
 def test_null_and_zero_arguments_rejected(name):
     fn = _fn(name)
