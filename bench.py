@@ -1411,6 +1411,130 @@ def run_minm_mt(args):
     print(json.dumps(rec))
 
 
+def run_head(args):
+    """calc_head at q = 0 (pygw.py:2808-3037) for the 512 k of an 8x8x8 mesh
+    (all irreducible): per k the LAPW coefficients (ifrot False), alfa/beta/
+    gama of 140 bands, the momentum elements of the 20 occupied x 120 empty
+    pairs (muffin-tin, lmax 10, 8 atoms, + interstitial over 600 plane waves,
+    both orders) and of 36 core states; then the head over 64 frequencies
+    from kcw [56, 120, 512, 64].  The same synthetic inputs at every k."""
+    import torch
+
+    from pygw_amd import head as H
+    from pygw_amd import lapw, synth
+    nk, nomx, numin, nbmaxpol, nom = 512, 19, 20, 140, 64
+    s = synth.lapw_system(lmax=10, lomax=10, nvk=600, nbmax=nbmaxpol, mult=(2, 2, 2, 2),
+                          apw=(True, False, True, False), gmax=10, alat=11.0, kl=(0.125, 0.25, -0.375), seed=20261214)
+    assert np.abs(s.gindex[s.indgk[:s.nvk]]).max() <= 5
+    base = synth.minm_mt_system(big=True, lmax=10)
+    rng = np.random.default_rng(20261214)
+    g = s.gindex
+    gm = 10
+    i_g0 = -np.ones((2 * gm + 1,) * 3, dtype=np.int64)
+    i_g0[g[:, 0] + gm, g[:, 1] + gm, g[:, 2] + gm] = np.arange(len(g))
+    ipw = (rng.normal(size=len(g)) + 1j * rng.normal(size=len(g))) * np.exp(-0.1 * (g ** 2).sum(1))
+    neg = i_g0[-g[:, 0] + gm, -g[:, 1] + gm, -g[:, 2] + gm]
+    ipwint = 0.5 * (ipw + np.conj(ipw[neg]))
+    nlo = max(int(s.nLO_at.max()), 1)
+    R = lambda *sh: rng.normal(size=sh)  # noqa: E731
+    vv_t = tuple(R(2, 10, 4) for _ in range(4)) + tuple(R(2, nlo, 11, 4) for _ in range(4)) + \
+        (R(2, nlo, nlo, 11, 4),)
+    ncore = np.asarray(base.ncore)
+    core_off = np.concatenate([[0], np.cumsum(ncore)[:-1]])
+    nct = int(ncore.sum())
+    cv_t = tuple(R(2, nct) for _ in range(4)) + tuple(R(2, nlo, nct) for _ in range(2))
+    corind = base.corind
+    ncg = len(corind)
+    tabs = H.HeadTables(s.mult, s.nLO_at, vv_t, core_off, cv_t, corind)
+    pw = H.PlaneWaves(ipwint, g, i_g0, s.k2cartes)
+    plan = lapw.LapwPlan(*s.plan_args())
+    dev = lambda x, dt: torch.as_tensor(np.asarray(x), dtype=dt, device="cuda")  # noqa: E731
+    ig, A = dev(s.indgk, torch.int32), dev(s.Aeig, torch.complex128)
+    Ebnd = np.sort(np.concatenate([rng.uniform(-0.8, -0.05, (nk, nomx + 1)),
+                                   rng.uniform(0.05, 2.0, (nk, nbmaxpol - nomx - 1))], axis=1), axis=1)
+    eig_core = -20.0 - rng.uniform(0, 5, ncg)
+    n_o, n_e = ncg + nomx + 1, nbmaxpol - numin
+    kcw = (-0.01 * torch.rand((nom, nk, n_e, n_o), dtype=torch.float64, device="cuda")).permute(3, 2, 1, 0)
+    Ed, ecd = dev(Ebnd, torch.float64), dev(eig_core, torch.float64)
+    kind, wt = dev(np.arange(nk), torch.int32), dev(np.ones(nk), torch.float64)
+    mvv = torch.empty((nk, n_e, nomx + 1, 3), dtype=torch.complex128, device="cuda")
+    mcv = torch.empty((nk, n_e, ncg, 3), dtype=torch.complex128, device="cuda")
+    coef = lapw.gap2_set_lapwcoef(plan, s.kl, ig, 1, False, s.nvk, pw.gindex)
+    abc = lapw.band_coefficients(A, *coef)
+    st = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+
+    def step():
+        t = np.zeros(3)
+        ev[0].record(st)
+        for k in range(nk):
+            lapw.gap2_set_lapwcoef(plan, s.kl, ig, 1, False, s.nvk, pw.gindex, out=coef)
+            lapw.band_coefficients(A, *coef, out=abc)
+        ev[1].record(st)
+        for k in range(nk):
+            H.momentum_elements(tabs, pw, abc, A, s.kl, ig, s.nvk, 0, nomx + 1, numin, nbmaxpol, mmatvv=mvv[k],
+                                mmatcv=mcv[k])
+        ev[2].record(st)
+        head = H.head_from_elements(mvv, mcv, Ed, ecd, kcw, kind, wt, nomx, numin, nbmaxpol, ncg, s.vol)
+        ev[3].record(st)
+        torch.cuda.synchronize()
+        t += [ev[i].elapsed_time(ev[i + 1]) / 1e3 for i in range(3)]
+        return t, head
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    tt = np.zeros(3)
+    for _ in range(args.steps):
+        dt, head = step()
+        tt += dt
+    tt /= args.steps
+    t = float(tt.sum())
+    print(f"[bench] head {t:.3f} s (coefficients {tt[0]:.3f}, momentum elements {tt[1]:.3f}, head sums {tt[2]:.4f})",
+          file=sys.stderr, flush=True)
+    f_is = 8.0 * s.nvk * s.nvk * nbmaxpol * nk          # the overlap product T = O conj(A)^T per k
+    rec = {"metric": "q = 0 head (calc_head: LAPW coefficients, momentum matrix elements, head sums) seconds",
+           "value": t, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"head: {nk} k (8x8x8, all irreducible), 4 atom types x 2 atoms, lmax 10, "
+                                  f"{s.nvk} plane waves, bands {nomx + 1} x {nbmaxpol - numin}, {ncg} core states, "
+                                  f"{nom} frequencies"},
+           "stages_s": {"lapw_and_band_coefficients": tt[0], "momentum_elements": tt[1], "head_sums": tt[2]},
+           "roofline": {"bound": "mfma", "kernel": "pygw_hd_is_t", "achieved": None, "peak": PEAK_FP64 / 1e12,
+                        "unit": "TFLOP/s", "frac": None, "traffic": None, "flops_per_k": f_is / nk,
+                        "note": "kernel rates from the rocprof summary (profiles/head_kernels.md)"}}
+    if not args.no_cpu:
+        from oracle import lapw as OL
+        from oracle import ref
+        # PyGW's per-pair calls (compiled for_q_0.f90) on a sample of pairs of k 0
+        coefs = OL.band_coefficients(s.Aeig, *ref.gap2_set_lapwcoef(*s.coef_args(iop=1, ifrot=False)))
+        al, be, ga = coefs
+        vv0 = mvv[0].cpu().numpy()
+        sample = [(ie1, ie2) for ie1 in (0, 7, 19) for ie2 in (20, 77, 139)]
+        t0 = time.perf_counter()
+        errs = []
+        for ie1, ie2 in sample:
+            p = ref.calcmmatvv_mt2(np.conj(al[ie1]), np.conj(be[ie1]), np.conj(ga[ie1]), al[ie2], be[ie2], ga[ie2],
+                                   s.nLO_at, s.mult, *vv_t) + \
+                ref.calcmmatvv_is(ie1, ie2, s.kl, s.Aeig, s.indgk, s.nvk, ipwint, g, i_g0, s.k2cartes)
+            ph = ref.calcmmatvv_mt(ie2, ie1, al, be, ga, s.nLO_at, s.mult, *vv_t) + \
+                ref.calcmmatvv_is(ie2, ie1, s.kl, s.Aeig, s.indgk, s.nvk, ipwint, g, i_g0, s.k2cartes)
+            want = (p + np.conj(ph)) / 2
+            errs.append(np.abs(vv0[ie2 - numin, ie1] - want).max() / np.abs(want).max())
+        t1 = time.perf_counter()
+        per_pair = (t1 - t0) / len(sample)
+        t2 = time.perf_counter()
+        ref.gap2_set_lapwcoef(*s.coef_args(iop=1, ifrot=False))
+        t3 = time.perf_counter()
+        rec["max_rel_error_vs_reference_sample"] = float(max(errs))
+        rec["cpu_baseline"] = {"value": per_pair * (nomx + 1) * (nbmaxpol - numin) * nk + (t3 - t2) * nk,
+                               "unit": "s", "cores": 1, "kind": "reference", **host_info(),
+                               "sample": f"PyGW's compiled calcmmatvv_mt2 + calcmmatvv_is and their reverse (the "
+                                         f"calls calc_head makes per pair) on {len(sample)} pairs of one k, x "
+                                         f"{(nomx + 1) * (nbmaxpol - numin)} pairs x {nk} k, plus gap2_set_lapwcoef x "
+                                         f"{nk}; the core elements and the head sums not timed"}
+    print(json.dumps(rec))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -1423,7 +1547,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a HIP device"
     if args.steps is None:
-        args.steps = {"c5": 8, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2, "coulomb": 2, "minm_is": 2, "mt_coulomb": 3, "minm_mt": 2}.get(args.config, 1)
+        args.steps = {"c5": 8, "c2": 8, "c4": 3, "gw0": 2, "gamma": 2, "coulomb": 2, "minm_is": 2, "mt_coulomb": 3, "minm_mt": 2, "head": 2}.get(args.config, 1)
     if args.copy_calibration:
         # one copy kernel that reads and writes exactly 1 GiB (one launch: under
         # 2 GiB), its source evicted from the 256 MB last-level cache by a 2 GiB
@@ -1459,6 +1583,9 @@ def main():
         return
     if args.config == "minm_mt":
         run_minm_mt(args)
+        return
+    if args.config == "head":
+        run_head(args)
         return
     mixed = not args.no_mixed
     wl = Workload(args.config, args.nq, mixed, args.fp64_kfrac)

@@ -666,8 +666,8 @@ int pygw_lapw_band(const void* Aeig, long lda, int conj_a, int nb, int ngk, cons
  *   complex128 row-major, indgk [>= nvk] int32 (0-based rows of gindex (ng, 3)
  *   int32 Fortran), ipwint [ng] complex128, i_g0 (2 n_g0[0]+1, 2 n_g0[1]+1,
  *   2 n_g0[2]+1) int32 Fortran on the device; kl [3], n_g0 [3], k2cartes (3, 3)
- *   in host memory.  FP64 (the overlap product on MFMA); workspace:
- *   pygw_mmat_vv_workspace bytes.
+ *   in host memory.  FP64 (the products on MFMA); workspace:
+ *   pygw_mmat_vv_workspace bytes (ndf = sum of mult).
  * pygw_mmat_cv replaces f_q_0.calcmmatcv over the atoms with cores (pygw.py:
  *   2911-2932, for_q_0.f90:510-683): mmatcv[(ie2 - mst) ncg + icg][c].
  *   cv_tables: 6 device pointers (iul_ucl, iudl_ucl, iucl_ul, iucl_udl (2,
@@ -684,12 +684,13 @@ int pygw_lapw_band(const void* Aeig, long lda, int conj_a, int nb, int ngk, cons
  *   mmatvv [nirk][nbmaxpol-numin][nomx+1][3], mmatcv [nirk][..][ncg][3],
  *   Ebnd [nirk][ldE], eig_core [ncg], kcw (ncg_p+nomx+1, nbmaxpol-numin,
  *   nk_kcw, nom) Fortran, k_ind [nirk] int32, weight [nirk], all on the
- *   device; workspace: pygw_calc_head_workspace bytes.
+ *   device; nom <= 4096; workspace: pygw_calc_head_workspace bytes.
  * pygw_wing_pm replaces calc_eps's pm of the q = 0 wing (pygw.py:3122-3141):
  *   pm[k][(io (nbmaxpol - numin) + ie2] for the k-points whose irreducible
  *   index is kii[k], the input of pygw_chi0_wings.  Allocates nothing.
  * ------------------------------------------------------------------------ */
-int pygw_mmat_vv_workspace(int nvk, int nst, int nend, int mst, int mend, size_t* bytes);
+int pygw_mmat_vv_workspace(int ndf, int lmax, int nlomax, int nvk, int nst, int nend, int mst, int mend,
+                           size_t* bytes);
 int pygw_mmat_vv(int nat, const int* mult, const int* nLO_at, int nrows, int lmax, const void* alfa, const void* beta,
                  const void* gama, int nbmax, int nbmax_g, int nlomax, const double* const* vv_tables, int nlo,
                  int lox, const void* Aeig, long lda, const double* kl, const int* indgk, int nvk, const void* ipwint,

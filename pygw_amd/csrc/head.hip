@@ -4,12 +4,15 @@
 //   mmatvv[ie2 - mst][ie1 - nst][c] = (p + conj(p'))/2, p = <ie1| -i nabla |ie2>,
 //   p' = <ie2| -i nabla |ie1>, each the muffin-tin part (calcmmatvv_MT,
 //   for_q_0.f90:10-234) plus the interstitial part (calcmmatvv_is,
-//   for_q_0.f90:460-507).  Muffin tin: one thread per band pair walking
-//   (atom, l, m) with the l <-> l+1 radial integrals (both orders in the same
-//   thread).  Interstitial: T[G2][b] = sum_G1 ipwint(G2 - G1) conj(A[b][G1])
-//   as one FP64-MFMA GEMM (the overlap gathered through i_g0 as it is
-//   loaded), then one wave per pair for sum_G2 A[j][G2] (k+G2)_c T[G2][i] in
-//   both orders.
+//   for_q_0.f90:460-507).  Muffin tin: the coupling of (l, m) to (l +- 1,
+//   m + {-1, 0, 1}) applied to every band's coefficients (one thread per
+//   (band, channel)), then sum over channels of conj(u_n1) (K u_n2) in both
+//   orders as FP64-MFMA products.  Interstitial: the overlap O[G2][G1] =
+//   ipwint(G2 - G1) gathered through i_g0, T[G2][b] = sum_G1 O conj(A[b][G1])
+//   as an FP64-MFMA product, then one wave per pair for sum_G2 A[j][G2]
+//   (k+G2)_c T[G2][i] in both orders.  The products split their sums over
+//   blocks (small band windows leave few output tiles) and add the chunks in
+//   a fixed order.
 //   mmatcv[ie2 - mst][icg][c] (calcmmatcv, for_q_0.f90:510-683): one thread
 //   per (band, core state).
 //   head[iom] = -sum_irk 4 pi fspin w_irk/Vol sum |p|^2/3/(E_c - E_v)^2 kcw
@@ -27,11 +30,14 @@ namespace hd {
 constexpr int kMaxNat = 16;
 constexpr int kMaxRows = 17;  // rows of nLO_at (lomax + 1)
 
+constexpr int kMaxNdf = 64;
+
 struct Layout {
-  int nat, lmax, ntnt, nbmax, nbmax_g, nlomax, nlo, lox, nrows;
+  int nat, ndf, lmax, ntnt, nbmax, nbmax_g, nlomax, nlo, lox, nrows;
   int mult[kMaxNat];
   int nlo_at[kMaxNat * kMaxRows];  // nLO_at[l + nrows iat]
   int core_off[kMaxNat];
+  int df_atom[kMaxNdf];            // atom type of each atom
 };
 
 struct VvTables {  // [2][lmax][nat] and [2][nlo][lox][nat], [2][nlo][nlo][lox][nat] (Fortran)
@@ -62,170 +68,233 @@ struct Bands {  // Fortran alfa/beta (nbmax, ntnt, ndf), gama (nbmax_g, nlomax, 
   }
 };
 
-// <n1| -i nabla |n2> in the spheres (the loops of for_q_0.f90:236-458)
-__device__ void mt_elem(const Layout& L, const VvTables& T, const Bands& C, int n1, int n2, double2 out[3]) {
-  double2 pxpy{0, 0}, pxmy{0, 0}, pz{0, 0};
-  const int lmax = L.lmax, nat = L.nat;
-  auto t2 = [&](const double* t, int il, int l, int iat) { return t[il + 2 * (l + lmax * iat)]; };
-  auto t4 = [&](const double* t, int il, int o, int l, int iat) {
-    return t[il + 2 * (o + L.nlo * (l + L.lox * iat))];
-  };
-  auto t5 = [&](int il, int o1, int o2, int l, int iat) {
-    return T.ulol_ulol[il + 2 * (o1 + L.nlo * (o2 + L.nlo * (l + L.lox * iat)))];
-  };
-  int idf = 0;
-  for (int iat = 0; iat < nat; iat++) {
-    for (int q = 0; q < L.mult[iat]; q++, idf++) {
-      for (int l = 0; l < lmax; l++) {
-        const double den = (double)((2 * l + 1) * (2 * l + 3));
-        const double r1[4] = {t2(T.ul_ul, 0, l, iat), t2(T.ul_udl, 0, l, iat), t2(T.udl_ul, 0, l, iat),
-                              t2(T.udl_udl, 0, l, iat)};
-        const double r2[4] = {t2(T.ul_ul, 1, l, iat), t2(T.ul_udl, 1, l, iat), t2(T.udl_ul, 1, l, iat),
-                              t2(T.udl_udl, 1, l, iat)};
-        const int nl0 = nlo_of(L, l, iat), nl1 = nlo_of(L, l + 1, iat);
-        for (int m = -l; m <= l; m++) {
-          const int lm = l * (l + 1) + m;
-          const int l1mm1 = (l + 1) * (l + 2) + m - 1, l1m = l1mm1 + 1, l1m1 = l1mm1 + 2;
-          const double flm1 = -sqrt((double)((l + m + 1) * (l + m + 2)) / den);
-          const double flm3 = sqrt((double)((l - m + 1) * (l - m + 2)) / den);
-          const double flm5 = sqrt((double)((l - m + 1) * (l + m + 1)) / den);
-          // <A x u + B x udot | op | A y u + B y udot>, bra conjugated
-          auto ab = [&](int tx, int ty, const double* r) {
-            const double2 xa = conjd(C.A(n1, tx, idf)), xb = conjd(C.B(n1, tx, idf));
-            const double2 ya = C.A(n2, ty, idf), yb = C.B(n2, ty, idf);
-            double2 s = rmul(cmul(xa, ya), r[0]);
-            s = cadd(s, rmul(cmul(xa, yb), r[1]));
-            s = cadd(s, rmul(cmul(xb, ya), r[2]));
-            return cadd(s, rmul(cmul(xb, yb), r[3]));
-          };
-          double2 xpy1 = ab(l1m1, lm, r1), xpy2 = ab(lm, l1mm1, r2);
-          double2 xmy1 = ab(l1mm1, lm, r1), xmy2 = ab(lm, l1m1, r2);
-          double2 z1 = ab(l1m, lm, r1), z2 = ab(lm, l1m, r2);
-          for (int jlo = 0; jlo < nl0; jlo++) {  // local orbitals of band n2 at l
-            const double2 g = C.G(n2, jlo, lm, idf);
-            const double i1 = t4(T.ul_ulol, 0, jlo, l, iat), i2 = t4(T.udl_ulol, 0, jlo, l, iat);
-            auto add = [&](int t) {
-              return cadd(rmul(cmul(conjd(C.A(n1, t, idf)), g), i1), rmul(cmul(conjd(C.B(n1, t, idf)), g), i2));
-            };
-            xpy1 = cadd(xpy1, add(l1m1));
-            xmy1 = cadd(xmy1, add(l1mm1));
-            z1 = cadd(z1, add(l1m));
-          }
-          for (int jlo = 0; jlo < nl1; jlo++) {  // local orbitals of band n2 at l + 1
-            const double2 a = conjd(C.A(n1, lm, idf)), b = conjd(C.B(n1, lm, idf));
-            const double i1 = t4(T.ul_ulol, 1, jlo, l, iat), i2 = t4(T.udl_ulol, 1, jlo, l, iat);
-            auto add = [&](int t) {
-              const double2 g = C.G(n2, jlo, t, idf);
-              return cadd(rmul(cmul(a, g), i1), rmul(cmul(b, g), i2));
-            };
-            xpy2 = cadd(xpy2, add(l1mm1));
-            xmy2 = cadd(xmy2, add(l1m1));
-            z2 = cadd(z2, add(l1m));
-          }
-          for (int ilo = 0; ilo < nl1; ilo++) {  // local orbitals of band n1 at l + 1
-            const double2 a = C.A(n2, lm, idf), b = C.B(n2, lm, idf);
-            const double i1 = t4(T.ulol_ul, 0, ilo, l, iat), i2 = t4(T.ulol_udl, 0, ilo, l, iat);
-            auto add = [&](int t) {
-              const double2 c = conjd(C.G(n1, ilo, t, idf));
-              return cadd(rmul(cmul(c, a), i1), rmul(cmul(c, b), i2));
-            };
-            xpy1 = cadd(xpy1, add(l1m1));
-            xmy1 = cadd(xmy1, add(l1mm1));
-            z1 = cadd(z1, add(l1m));
-          }
-          for (int ilo = 0; ilo < nl0; ilo++) {  // local orbitals of band n1 at l
-            const double2 c = conjd(C.G(n1, ilo, lm, idf));
-            const double i1 = t4(T.ulol_ul, 1, ilo, l, iat), i2 = t4(T.ulol_udl, 1, ilo, l, iat);
-            auto add = [&](int t) {
-              return cadd(rmul(cmul(c, C.A(n2, t, idf)), i1), rmul(cmul(c, C.B(n2, t, idf)), i2));
-            };
-            xpy2 = cadd(xpy2, add(l1mm1));
-            xmy2 = cadd(xmy2, add(l1m1));
-            z2 = cadd(z2, add(l1m));
-          }
-          for (int jlo = 0; jlo < nl0; jlo++)
-            for (int ilo = 0; ilo < nl1; ilo++) {
-              const double i1 = t5(0, ilo, jlo, l, iat), i2 = t5(1, jlo, ilo, l, iat);
-              const double2 g0 = C.G(n2, jlo, lm, idf), c0 = conjd(C.G(n1, jlo, lm, idf));
-              xpy1 = cadd(xpy1, rmul(cmul(conjd(C.G(n1, ilo, l1m1, idf)), g0), i1));
-              xmy1 = cadd(xmy1, rmul(cmul(conjd(C.G(n1, ilo, l1mm1, idf)), g0), i1));
-              z1 = cadd(z1, rmul(cmul(conjd(C.G(n1, ilo, l1m, idf)), g0), i1));
-              xpy2 = cadd(xpy2, rmul(cmul(c0, C.G(n2, ilo, l1mm1, idf)), i2));
-              xmy2 = cadd(xmy2, rmul(cmul(c0, C.G(n2, ilo, l1m1, idf)), i2));
-              z2 = cadd(z2, rmul(cmul(c0, C.G(n2, ilo, l1m, idf)), i2));
-            }
-          pxpy = cadd(pxpy, cadd(rmul(xpy1, flm1), rmul(xpy2, flm3)));
-          pxmy = cadd(pxmy, cadd(rmul(xmy1, flm3), rmul(xmy2, flm1)));
-          pz = cadd(pz, rmul(cadd(z1, z2), flm5));
-        }
-      }
-    }
-  }
-  // mt12 = (-(pxpy + pxmy) i/2, (-pxpy + pxmy)/2, -i pz)
-  const double2 s = cadd(pxpy, pxmy), d = cadd(pxmy, rmul(pxpy, -1.0));
-  out[0] = double2{0.5 * s.y, -0.5 * s.x};
-  out[1] = rmul(d, 0.5);
-  out[2] = double2{pz.y, -pz.x};
+}  // namespace hd
+
+// The spheres' part as a sparse operator and two products.  With u_n[ch]
+// the coefficients of band n over the channels ch = (atom, (l, m), kind:
+// u, udot or local orbital o < nLO_at(l)), p_c(n1, n2) = sum_ch conj(u_n1[ch])
+// V_c,n2[ch], where V_c,n2 = K_c u_n2 collects, for the bra channel ch =
+// (L, M, X), the ket channels (L -/+ 1, M + {-1, 0, 1}, Y) of the six sums
+// of for_q_0.f90:236-458 weighted by their Clebsch-Gordan factor and radial
+// integral (R1 = <X_l+1| d/dr - l/r |Y_l>, R2 = <X_l| d/dr + (l+2)/r |Y_l+1>).
+// Thread per (band, channel): Uc[n][ch] = conj(u_n[ch]) and V[c][n][ch].
+namespace hd {
+struct ChanMap {  // channel ch = t + ntnt (kind + nk idf), kind 0 u, 1 udot, 2 + o local orbital o
+  int ntnt, nk;
+};
+
+// radial integral of the momentum operator between bra kind kx (at lb) and
+// ket kind ky: il = 0 the R1 table at l (bra at l+1), il = 1 the R2 table
+__device__ inline double rad(const Layout& L, const VvTables& T, int il, int l, int iat, int kx, int ky) {
+  auto t2 = [&](const double* t) { return t[il + 2 * (l + L.lmax * iat)]; };
+  auto t4 = [&](const double* t, int o) { return t[il + 2 * (o + L.nlo * (l + L.lox * iat))]; };
+  if (kx < 2 && ky < 2) return kx == 0 ? (ky == 0 ? t2(T.ul_ul) : t2(T.ul_udl)) : (ky == 0 ? t2(T.udl_ul) : t2(T.udl_udl));
+  if (kx < 2) return t4(kx == 0 ? T.ul_ulol : T.udl_ulol, ky - 2);
+  if (ky < 2) return t4(ky == 0 ? T.ulol_ul : T.ulol_udl, kx - 2);
+  return T.ulol_ulol[il + 2 * ((kx - 2) + L.nlo * ((ky - 2) + L.nlo * (l + L.lox * iat)))];
+}
+
+__device__ inline double2 coef(const Bands& C, int n, int t, int kind, int idf) {
+  return kind == 0 ? C.A(n, t, idf) : (kind == 1 ? C.B(n, t, idf) : C.G(n, kind - 2, t, idf));
 }
 }  // namespace hd
 
-// one thread per pair (ie1, ie2): mmatvv = (p + conj(p'))/2 of the spheres
-__global__ void __launch_bounds__(256) pygw_hd_mt(hd::Layout L, hd::VvTables T, hd::Bands C, int nst, int n1,
-                                                 int mst, int n2, double2* __restrict__ mmatvv) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n1 * n2) return;
-  const int ie1 = nst + e % n1, ie2 = mst + e / n1;
-  double2 p[3], q[3];
-  hd::mt_elem(L, T, C, ie1, ie2, p);
-  if (ie1 == ie2) {
-    for (int c = 0; c < 3; c++) q[c] = p[c];
-  } else {
-    hd::mt_elem(L, T, C, ie2, ie1, q);
+__global__ void __launch_bounds__(256) pygw_hd_kapply(hd::Layout L, hd::VvTables T, hd::Bands C, int b0, int nbw,
+                                                     long nch, double2* __restrict__ Uc, double2* __restrict__ V) {
+  using namespace hd;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= nbw * nch) return;
+  const int nb = (int)(e / nch), n = b0 + nb;
+  const long ch = e % nch;
+  const int nk = 2 + L.nlomax;
+  const int t = (int)(ch % L.ntnt), kind = (int)((ch / L.ntnt) % nk), idf = (int)(ch / ((long)L.ntnt * nk));
+  const int iat = L.df_atom[idf];
+  const int Lb = (int)sqrtf((float)t + 0.5f), Mb = t - Lb * (Lb + 1);
+  const bool valid = kind < 2 || (kind - 2) < nlo_of(L, Lb, iat);
+  const double2 z{0.0, 0.0};
+  Uc[e] = valid ? conjd(coef(C, n, t, kind, idf)) : z;
+  double2 pxpy = z, pxmy = z, pz = z;
+  if (valid) {
+    // ket sum over kinds Y at (l, m) with radial table (il, lr)
+    auto ket = [&](int il, int lr, int l, int m) {
+      double2 s = z;
+      const int tk = l * (l + 1) + m, nl = nlo_of(L, l, iat);
+      for (int ky = 0; ky < 2 + nl; ky++) s = cadd(s, rmul(coef(C, n, tk, ky, idf), rad(L, T, il, lr, iat, kind, ky)));
+      return s;
+    };
+    if (Lb >= 1 && Lb - 1 < L.lmax) {  // bra at l + 1 = Lb: xpy1 (m = Mb - 1), xmy1 (m = Mb + 1), z1 (m = Mb)
+      const int l = Lb - 1;
+      const double den = (double)((2 * l + 1) * (2 * l + 3));
+      int m = Mb - 1;
+      if (m >= -l && m <= l) pxpy = cadd(pxpy, rmul(ket(0, l, l, m), -sqrt((double)((l + m + 1) * (l + m + 2)) / den)));
+      m = Mb + 1;
+      if (m >= -l && m <= l) pxmy = cadd(pxmy, rmul(ket(0, l, l, m), sqrt((double)((l - m + 1) * (l - m + 2)) / den)));
+      m = Mb;
+      if (m >= -l && m <= l) pz = cadd(pz, rmul(ket(0, l, l, m), sqrt((double)((l - m + 1) * (l + m + 1)) / den)));
+    }
+    if (Lb < L.lmax) {  // bra at l = Lb: xpy2 (ket m - 1), xmy2 (ket m + 1), z2 (ket m) at l + 1
+      const int l = Lb, m = Mb;
+      const double den = (double)((2 * l + 1) * (2 * l + 3));
+      pxpy = cadd(pxpy, rmul(ket(1, l, l + 1, m - 1), sqrt((double)((l - m + 1) * (l - m + 2)) / den)));
+      pxmy = cadd(pxmy, rmul(ket(1, l, l + 1, m + 1), -sqrt((double)((l + m + 1) * (l + m + 2)) / den)));
+      pz = cadd(pz, rmul(ket(1, l, l + 1, m), sqrt((double)((l - m + 1) * (l + m + 1)) / den)));
+    }
   }
-  double2* o = mmatvv + ((size_t)(ie2 - mst) * n1 + (ie1 - nst)) * 3;
-  for (int c = 0; c < 3; c++) o[c] = double2{0.5 * (p[c].x + q[c].x), 0.5 * (p[c].y - q[c].y)};
+  // (-(pxpy + pxmy) i/2, (pxmy - pxpy)/2, -i pz)
+  const double2 sm = cadd(pxpy, pxmy), df = cadd(pxmy, rmul(pxpy, -1.0));
+  const size_t plane = (size_t)nbw * nch;
+  V[e] = double2{0.5 * sm.y, -0.5 * sm.x};
+  V[plane + e] = rmul(df, 0.5);
+  V[2 * plane + e] = double2{pz.y, -pz.x};
 }
 
-// T[G2][b] = sum_G1 ipwint(G2 - G1) conj(A[b0 + b][G1]), G1, G2 < nvk
-__global__ void __launch_bounds__(256) pygw_hd_is_t(const double2* __restrict__ A, long lda, int b0, int nbw,
-                                                   const int* __restrict__ indgk, int nvk,
+// Complex products with both operands contiguous along the sum, split over
+// the sum: part[s][job][m][n] = sum_{k in chunk s} A_job[m][k] B_job[n][k]
+// (B conjugated with conjB).  Tiles of 32 x 32 on v_mfma_f64_16x16x4f64,
+// both operands staged along k; the chunks are summed in a fixed order
+// afterwards, so the result does not depend on the schedule.
+namespace hd {
+constexpr int kMaxJobs = 6;
+struct Job {
+  const double2 *A, *B;
+  long lda, ldb, off;  // off: the job's first element in a partial plane
+  int M, N, tiles_n, tile0;
+};
+struct Jobs {
+  Job j[kMaxJobs];
+  int njob, ntile, conjB;
+  long K, chunk, plane;  // plane = sum over jobs of M N
+};
+}  // namespace hd
+
+__global__ void __launch_bounds__(256) pygw_hd_gemm(hd::Jobs J, double2* __restrict__ part) {
+  using namespace hd;
+  constexpr int TT = 32, KC = 32;
+  __shared__ double2 As[TT][KC + 1];
+  __shared__ double2 Bs[TT][KC + 1];
+  const int tile = blockIdx.x, sp = blockIdx.y;
+  int jb = 0;
+  while (jb + 1 < J.njob && tile >= J.j[jb + 1].tile0) jb++;
+  const Job& Jb = J.j[jb];
+  const int tl = tile - Jb.tile0;
+  const int m0 = (tl / Jb.tiles_n) * TT, c0 = (tl % Jb.tiles_n) * TT;
+  const long kb = sp * J.chunk, ke = std::min(J.K, kb + J.chunk);
+  const double cs = J.conjB ? -1.0 : 1.0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1, li = lane & 15, lk = lane >> 4;
+  d4 aR = d4{0, 0, 0, 0}, aI = d4{0, 0, 0, 0};
+  for (long k0 = kb; k0 < ke; k0 += KC) {
+    for (int e = tid; e < TT * KC; e += 256) {
+      const int c = e % KC, m = e / KC;
+      const long k = k0 + c;
+      double2 a{0.0, 0.0}, b{0.0, 0.0};
+      if (k < ke) {
+        if (m0 + m < Jb.M) a = Jb.A[(size_t)(m0 + m) * Jb.lda + k];
+        if (c0 + m < Jb.N) b = Jb.B[(size_t)(c0 + m) * Jb.ldb + k];
+      }
+      As[m][c] = a;
+      Bs[m][c] = double2{b.x, cs * b.y};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ks++) {
+      const int kk = ks * 4 + lk;
+      const double2 x = As[16 * wr + li][kk], y = Bs[16 * wc + li][kk];
+      aR = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.x, aR, 0, 0, 0);
+      aR = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.y, y.y, aR, 0, 0, 0);
+      aI = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, y.y, aI, 0, 0, 0);
+      aI = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, y.x, aI, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  double2* o = part + (size_t)sp * J.plane + Jb.off;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int r = m0 + 16 * wr + 4 * v + lk, c = c0 + 16 * wc + li;
+    if (r < Jb.M && c < Jb.N) o[(size_t)r * Jb.N + c] = double2{aR[v], aI[v]};
+  }
+}
+
+// host: the jobs' tiles and the chunk of the sum that fills the chip
+static hd::Jobs make_jobs(int njob, const hd::Job* jobs, long K, int conjB) {
+  hd::Jobs J{};
+  J.njob = njob;
+  J.K = K;
+  J.conjB = conjB;
+  long off = 0;
+  int tile = 0;
+  for (int i = 0; i < njob; i++) {
+    J.j[i] = jobs[i];
+    J.j[i].off = off;
+    J.j[i].tiles_n = ceil_div(jobs[i].N, 32);
+    J.j[i].tile0 = tile;
+    tile += ceil_div(jobs[i].M, 32) * J.j[i].tiles_n;
+    off += (long)jobs[i].M * jobs[i].N;
+  }
+  J.ntile = tile;
+  J.plane = off;
+  // about 2 blocks per CU, chunks of whole KC steps, at most 32 of them
+  int split = std::max(1, std::min(32, ceil_div(512, std::max(tile, 1))));
+  split = std::min<long>(split, std::max(1L, K / 64));
+  J.chunk = ((K + split - 1) / split + 31) / 32 * 32;
+  return J;
+}
+
+static int n_split(const hd::Jobs& J) { return (int)((J.K + J.chunk - 1) / J.chunk); }
+
+// mmatvv[ie2][ie1][c] = (P1_c[ie1][ie2] + conj(P2_c[ie2][ie1]))/2, the
+// chunks of both products summed in order (jobs 0-2: P1_c [n1][n2], 3-5: P2_c
+// [n2][n1])
+__global__ void __launch_bounds__(256) pygw_hd_mtcombine(const double2* __restrict__ part, int nsplit, long plane,
+                                                        int n1, int n2, double2* __restrict__ mmatvv) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n1 * n2 * 3) return;
+  const int c = e % 3, ie1 = (e / 3) % n1, ie2 = e / (3 * n1);
+  const size_t ip = (size_t)c * n1 * n2 + (size_t)ie1 * n2 + ie2;
+  const size_t iq = (size_t)(3 + c) * n1 * n2 + (size_t)ie2 * n1 + ie1;
+  double2 p{0.0, 0.0}, q{0.0, 0.0};
+  for (int sp = 0; sp < nsplit; sp++) {
+    const double2 x = part[sp * plane + ip], y = part[sp * plane + iq];
+    p.x += x.x;
+    p.y += x.y;
+    q.x += y.x;
+    q.y += y.y;
+  }
+  mmatvv[e] = double2{0.5 * (p.x + q.x), 0.5 * (p.y - q.y)};
+}
+
+// O[G2][G1] = ipwint(G2 - G1) over the k-point's plane waves (0 where the
+// difference is outside i_g0 or i_g0 holds no row)
+__global__ void __launch_bounds__(256) pygw_hd_olap(const int* __restrict__ indgk, int nvk,
                                                    const int* __restrict__ gindex, int ng,
                                                    const double2* __restrict__ ipwint, const int* __restrict__ i_g0,
-                                                   int g1, int g2, int g3, double2* __restrict__ Tm) {
-  using namespace is;
-  __shared__ double2 As[64][KC + 1];
-  __shared__ double2 Bs[KC][64 + 1];
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-  d4 aR[2][2], aI[2][2];
-  zero(aR, aI);
-  auto la = [&](int m, int k) -> double2 {
-    const int r = m0 + m;
-    if (r >= nvk || k >= nvk) return double2{0.0, 0.0};
-    const int a = indgk[r], b = indgk[k];
-    const int d0 = gindex[a] - gindex[b] + g1, d1 = gindex[a + ng] - gindex[b + ng] + g2,
-              d2 = gindex[a + 2 * ng] - gindex[b + 2 * ng] + g3;
-    if (d0 < 0 || d0 > 2 * g1 || d1 < 0 || d1 > 2 * g2 || d2 < 0 || d2 > 2 * g3) return double2{0.0, 0.0};
+                                                   int g1, int g2, int g3, double2* __restrict__ O) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)nvk * nvk) return;
+  const int r = (int)(e / nvk), k = (int)(e % nvk);
+  const int a = indgk[r], b = indgk[k];
+  const int d0 = gindex[a] - gindex[b] + g1, d1 = gindex[a + ng] - gindex[b + ng] + g2,
+            d2 = gindex[a + 2 * ng] - gindex[b + 2 * ng] + g3;
+  double2 v{0.0, 0.0};
+  if (d0 >= 0 && d0 <= 2 * g1 && d1 >= 0 && d1 <= 2 * g2 && d2 >= 0 && d2 <= 2 * g3) {
     const int idg = i_g0[d0 + (2 * g1 + 1) * (d1 + (2 * g2 + 1) * d2)];
-    return (idg >= 0 && idg < ng) ? ipwint[idg] : double2{0.0, 0.0};
-  };
-  auto lb = [&](int k, int c) -> double2 {
-    const int b = n0 + c;
-    if (k >= nvk || b >= nbw) return double2{0.0, 0.0};
-    const double2 x = A[(size_t)(b0 + b) * lda + k];
-    return double2{x.x, -x.y};
-  };
-  tile<2>(nvk, la, lb, As, Bs, aR, aI);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wr = wave >> 1, wc = wave & 1, li = lane & 15, lk = lane >> 4;
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = m0 + 32 * wr + 16 * a + 4 * v + lk, c = n0 + 32 * wc + 16 * b + li;
-        if (r < nvk && c < nbw) Tm[(size_t)r * nbw + c] = double2{aR[a][b][v], aI[a][b][v]};
-      }
+    if (idg >= 0 && idg < ng) v = ipwint[idg];
+  }
+  O[e] = v;
+}
+
+// T[G2][b] = the chunks of O conj(A)^T summed in order
+__global__ void __launch_bounds__(256) pygw_hd_sum(const double2* __restrict__ part, int nsplit, long plane,
+                                                  double2* __restrict__ T) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= plane) return;
+  double2 s{0.0, 0.0};
+  for (int sp = 0; sp < nsplit; sp++) {
+    const double2 x = part[sp * plane + e];
+    s.x += x.x;
+    s.y += x.y;
+  }
+  T[e] = s;
 }
 
 struct KpG {
@@ -396,17 +465,18 @@ __global__ void __launch_bounds__(256) pygw_hd_wpl2(const double2* __restrict__ 
   if (threadIdx.x == 0) *wpl2 = red[0];
 }
 
-// head[iom] = -sum_irk c4 w_irk sum_pair term kcw[pair][k_ind[irk]][iom] (+ Drude)
-__global__ void __launch_bounds__(256) pygw_hd_head(const double* __restrict__ term, const double* __restrict__ kcw,
-                                                   long n_pair, int nk_kcw, const int* __restrict__ k_ind,
-                                                   const double* __restrict__ weight, int nirk, double c4,
-                                                   const double* __restrict__ wpl2, const double* __restrict__ omega,
-                                                   const double* __restrict__ dUl, int nom_w, int nom,
-                                                   double eta_head, double2* __restrict__ head) {
+// part[iom][ch] = sum over the irk of chunk ch of c4 w_irk sum_pair term kcw
+// (one block per (frequency, chunk of irk): kcw's (pair) slab of a k and
+// frequency is contiguous)
+__global__ void __launch_bounds__(256) pygw_hd_head_part(const double* __restrict__ term,
+                                                        const double* __restrict__ kcw, long n_pair, int nk_kcw,
+                                                        const int* __restrict__ k_ind,
+                                                        const double* __restrict__ weight, int nirk, int per,
+                                                        double c4, double* __restrict__ part) {
   __shared__ double red[256];
-  const int iom = blockIdx.x;
+  const int iom = blockIdx.x, ch = blockIdx.y;
   double acc = 0.0;
-  for (int irk = 0; irk < nirk; irk++) {
+  for (int irk = ch * per; irk < std::min(nirk, (ch + 1) * per); irk++) {
     const double* kc = kcw + ((size_t)iom * nk_kcw + k_ind[irk]) * n_pair;
     const double* t = term + (size_t)irk * n_pair;
     double s = 0.0;
@@ -419,21 +489,31 @@ __global__ void __launch_bounds__(256) pygw_hd_head(const double* __restrict__ t
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    double h = -red[0];
-    if (wpl2) {
-      if (dUl) {
-        for (int w = 0; w < nom_w; w++) {
-          const double x = omega[w] + eta_head;
-          h += *wpl2 / (x * x) * dUl[(size_t)w * nom + iom];
-        }
-      } else {
-        const double x = omega[iom] + eta_head;
-        h += *wpl2 / (x * x);
+  if (threadIdx.x == 0) part[(size_t)iom * gridDim.y + ch] = red[0];
+}
+
+// head[iom] = -sum_ch part[iom][ch] (in order) + the Drude term
+__global__ void __launch_bounds__(256) pygw_hd_head_fin(const double* __restrict__ part, int nch,
+                                                       const double* __restrict__ wpl2,
+                                                       const double* __restrict__ omega,
+                                                       const double* __restrict__ dUl, int nom_w, int nom,
+                                                       double eta_head, double2* __restrict__ head) {
+  const int iom = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iom >= nom) return;
+  double h = 0.0;
+  for (int c = 0; c < nch; c++) h -= part[(size_t)iom * nch + c];
+  if (wpl2) {
+    if (dUl) {
+      for (int w = 0; w < nom_w; w++) {
+        const double x = omega[w] + eta_head;
+        h += *wpl2 / (x * x) * dUl[(size_t)w * nom + iom];
       }
+    } else {
+      const double x = omega[iom] + eta_head;
+      h += *wpl2 / (x * x);
     }
-    head[iom] = double2{h, 0.0};
   }
+  head[iom] = double2{h, 0.0};
 }
 
 // pm[k][ie12] (calc_eps's q = 0 wing): sum_c p_c coefw/sqrt(3)/(e1 - e2)
@@ -483,8 +563,9 @@ static bool fill_layout(hd::Layout& L, int nat, const int* mult, const int* nLO_
   L.lox = lox;
   L.nrows = nrows;
   for (int a = 0; a < nat; a++) {
-    if (mult[a] <= 0) return false;
+    if (mult[a] <= 0 || L.ndf + mult[a] > kMaxNdf) return false;
     L.mult[a] = mult[a];
+    for (int q = 0; q < mult[a]; q++) L.df_atom[L.ndf++] = a;
     for (int l = 0; l < nrows; l++) {
       const int n = nLO_at[l + nrows * a];
       if (n < 0) return false;
@@ -499,10 +580,19 @@ static bool fill_layout(hd::Layout& L, int nat, const int* mult, const int* nLO_
   return true;
 }
 
-extern "C" int pygw_mmat_vv_workspace(int nvk, int nst, int nend, int mst, int mend, size_t* bytes) {
-  if (nvk < 0 || nend <= nst || mend <= mst || nst < 0 || mst < 0 || !bytes) return PYGW_E_ARG;
-  const int nbw = std::max(nend, mend) - std::min(nst, mst);
-  *bytes = (size_t)std::max(nvk, 1) * nbw * sizeof(double2);
+static size_t al256(size_t x) { return (x + 255) / 256 * 256; }
+
+// workspace: Uc [nbw][nch], V [3][nbw][nch], the split products (32 partial
+// planes of 6 n1 n2), O [nvk][nvk], the split overlap product (32 planes of
+// nvk nbw) and T [nvk][nbw]
+extern "C" int pygw_mmat_vv_workspace(int ndf, int lmax, int nlomax, int nvk, int nst, int nend, int mst, int mend,
+                                      size_t* bytes) {
+  if (ndf <= 0 || lmax <= 0 || nlomax < 0 || nvk < 0 || nend <= nst || mend <= mst || nst < 0 || mst < 0 || !bytes)
+    return PYGW_E_ARG;
+  const size_t nbw = std::max(nend, mend) - std::min(nst, mst), n1 = nend - nst, n2 = mend - mst;
+  const size_t nch = (size_t)ndf * (lmax + 1) * (lmax + 1) * (2 + nlomax), nv = std::max(nvk, 1);
+  *bytes = al256(nbw * nch * 16) + al256(3 * nbw * nch * 16) + al256(32 * 6 * n1 * n2 * 16) + al256(nv * nv * 16) +
+           al256(32 * nv * nbw * 16) + al256(nv * nbw * 16);
   return PYGW_OK;
 }
 
@@ -518,26 +608,53 @@ extern "C" int pygw_mmat_vv(int nat, const int* mult, const int* nLO_at, int nro
   size_t need = 0;
   if (!fill_layout(L, nat, mult, nLO_at, nrows, lmax, nbmax, nbmax_g, nlomax, nlo, lox) || !alfa || !beta ||
       (nlomax > 0 && !gama) || !vv_tables || !Aeig || !kl || !mmatvv || nst < 0 || mst < 0 || nend > nbmax ||
-      mend > nbmax || (nlomax > 0 && (nend > nbmax_g || mend > nbmax_g)) ||
-      pygw_mmat_vv_workspace(nvk, nst, nend, mst, mend, &need) != PYGW_OK)
+      mend > nbmax || (nlomax > 0 && (nend > nbmax_g || mend > nbmax_g)) || !workspace ||
+      pygw_mmat_vv_workspace(L.ndf, lmax, nlomax, nvk, nst, nend, mst, mend, &need) != PYGW_OK || ws_bytes < need)
     return PYGW_E_ARG;
   for (int i = 0; i < 9; i++)
     if (!vv_tables[i] && (i < 4 || nlo > 0)) return PYGW_E_ARG;
   if (nvk > 0 && (!indgk || !ipwint || !gindex || ng <= 0 || !i_g0 || !n_g0 || !k2cartes || lda < nvk ||
-                  !workspace || ws_bytes < need || n_g0[0] < 0 || n_g0[1] < 0 || n_g0[2] < 0))
+                  n_g0[0] < 0 || n_g0[1] < 0 || n_g0[2] < 0))
     return PYGW_E_ARG;
   const int n1 = nend - nst, n2 = mend - mst;
+  const int b0 = std::min(nst, mst), nbw = std::max(nend, mend) - b0;
+  const long nch = (long)L.ndf * L.ntnt * (2 + nlomax);
+  const size_t nv = std::max(nvk, 1);
   const VvTables T{vv_tables[0], vv_tables[1], vv_tables[2], vv_tables[3], vv_tables[4],
                    vv_tables[5], vv_tables[6], vv_tables[7], vv_tables[8]};
   const Bands C{(const double2*)alfa, (const double2*)beta, (const double2*)gama, nbmax, nbmax_g, L.ntnt, nlomax};
-  hipLaunchKernelGGL(pygw_hd_mt, dim3(ceil_div((long)n1 * n2, 256)), dim3(256), 0, stream, L, T, C, nst, n1, mst, n2,
-                     (double2*)mmatvv);
+  char* ws = (char*)workspace;
+  double2* Uc = (double2*)ws;
+  ws += al256((size_t)nbw * nch * 16);
+  double2* V = (double2*)ws;
+  ws += al256(3 * (size_t)nbw * nch * 16);
+  double2* P = (double2*)ws;
+  ws += al256(32 * 6 * (size_t)n1 * n2 * 16);
+  double2* O = (double2*)ws;
+  ws += al256(nv * nv * 16);
+  double2* Tp = (double2*)ws;
+  ws += al256(32 * nv * nbw * 16);
+  double2* Tm = (double2*)ws;
+  hipLaunchKernelGGL(pygw_hd_kapply, dim3(ceil_div((long)nbw * nch, 256)), dim3(256), 0, stream, L, T, C, b0, nbw,
+                     nch, Uc, V);
+  Job jobs[6];
+  for (int c = 0; c < 3; c++) {
+    jobs[c] = Job{Uc + (size_t)(nst - b0) * nch, V + ((size_t)c * nbw + (mst - b0)) * nch, nch, nch, 0, n1, n2, 0, 0};
+    jobs[3 + c] = Job{Uc + (size_t)(mst - b0) * nch, V + ((size_t)c * nbw + (nst - b0)) * nch, nch, nch, 0, n2, n1,
+                      0, 0};
+  }
+  const Jobs Jm = make_jobs(6, jobs, nch, 0);
+  hipLaunchKernelGGL(pygw_hd_gemm, dim3(Jm.ntile, n_split(Jm)), dim3(256), 0, stream, Jm, P);
+  hipLaunchKernelGGL(pygw_hd_mtcombine, dim3(ceil_div((long)n1 * n2 * 3, 256)), dim3(256), 0, stream,
+                     (const double2*)P, n_split(Jm), Jm.plane, n1, n2, (double2*)mmatvv);
   if (nvk > 0) {
-    const int b0 = std::min(nst, mst), nbw = std::max(nend, mend) - b0;
-    double2* Tm = (double2*)workspace;
-    hipLaunchKernelGGL(pygw_hd_is_t, dim3(ceil_div(nbw, 64), ceil_div(nvk, 64)), dim3(256), 0, stream,
-                       (const double2*)Aeig, lda, b0, nbw, indgk, nvk, gindex, ng, (const double2*)ipwint, i_g0,
-                       n_g0[0], n_g0[1], n_g0[2], Tm);
+    hipLaunchKernelGGL(pygw_hd_olap, dim3(ceil_div((long)nvk * nvk, 256)), dim3(256), 0, stream, indgk, nvk, gindex,
+                       ng, (const double2*)ipwint, i_g0, n_g0[0], n_g0[1], n_g0[2], O);
+    const Job jt{O, (const double2*)Aeig + (size_t)b0 * lda, (long)nvk, lda, 0, nvk, nbw, 0, 0};
+    const Jobs Jt = make_jobs(1, &jt, nvk, 1);
+    hipLaunchKernelGGL(pygw_hd_gemm, dim3(Jt.ntile, n_split(Jt)), dim3(256), 0, stream, Jt, Tp);
+    hipLaunchKernelGGL(pygw_hd_sum, dim3(ceil_div(Jt.plane, 256)), dim3(256), 0, stream, (const double2*)Tp,
+                       n_split(Jt), Jt.plane, Tm);
     KpG kp;
     for (int i = 0; i < 9; i++) kp.k2c[i] = k2cartes[i];
     for (int c = 0; c < 3; c++) kp.kcar[c] = k2cartes[c] * kl[0] + k2cartes[c + 3] * kl[1] + k2cartes[c + 6] * kl[2];
@@ -575,9 +692,14 @@ extern "C" int pygw_mmat_cv(int nat, const int* mult, const int* nLO_at, int nro
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }
 
+namespace hd {
+constexpr int kHeadChunks = 16;  // blocks per frequency in the head sums
+constexpr int kMaxNom = 4096;
+}  // namespace hd
+
 extern "C" int pygw_calc_head_workspace(int nirk, int n_o, int n_e, size_t* bytes) {
   if (nirk <= 0 || n_o <= 0 || n_e <= 0 || !bytes) return PYGW_E_ARG;
-  *bytes = 256 + (size_t)nirk * n_o * n_e * sizeof(double);
+  *bytes = 256 + (size_t)hd::kMaxNom * hd::kHeadChunks * sizeof(double) + (size_t)nirk * n_o * n_e * sizeof(double);
   return PYGW_OK;
 }
 
@@ -592,21 +714,26 @@ extern "C" int pygw_calc_head(const void* mmatvv, const void* mmatcv, const doub
   size_t need = 0;
   const bool drude = metallic && iop_drude == 1;
   if (!mmatvv || (ncg_p > 0 && (!mmatcv || !eig_core || ncg < ncg_p)) || !Ebnd || !kcw || !k_ind || !weight ||
-      nomx < 0 || numin < 0 || n_e <= 0 || ldE < nbmaxpol || nom <= 0 || nk_kcw <= 0 || !(vol > 0.0) || !head ||
+      nomx < 0 || numin < 0 || n_e <= 0 || ldE < nbmaxpol || nom <= 0 || nom > hd::kMaxNom || nk_kcw <= 0 ||
+      !(vol > 0.0) || !head ||
       ncg_p < 0 || pygw_calc_head_workspace(nirk, n_o, n_e, &need) != PYGW_OK || !workspace || ws_bytes < need ||
       (drude && (!kwfer || !omega || (dUl ? nom_w <= 0 : nom_w != nom))))
     return PYGW_E_ARG;
   const double c4 = 4.0 * 3.14159265358979323846 * 2.0 / vol;  // 4 pi fspin / Vol, fspin = 2
   double* wpl2 = (double*)workspace;
-  double* term = (double*)((char*)workspace + 256);
+  double* part = (double*)((char*)workspace + 256);
+  double* term = part + (size_t)hd::kMaxNom * hd::kHeadChunks;
   hipLaunchKernelGGL(pygw_hd_terms, dim3(ceil_div((long)nirk * n_o * n_e, 256)), dim3(256), 0, stream,
                      (const double2*)mmatvv, (const double2*)mmatcv, Ebnd, ldE, eig_core, nirk, nvbm, ncg, ncg_p, n_e,
                      numin, term);
   if (drude)
     hipLaunchKernelGGL(pygw_hd_wpl2, dim3(1), dim3(256), 0, stream, (const double2*)mmatvv, kwfer, ldE, weight, nirk,
                        nvbm, n_e, numin, c4, wpl2);
-  hipLaunchKernelGGL(pygw_hd_head, dim3(nom), dim3(256), 0, stream, term, kcw, (long)n_o * n_e, nk_kcw, k_ind, weight,
-                     nirk, c4, drude ? wpl2 : nullptr, omega, dUl, nom_w, nom, eta_head, (double2*)head);
+  const int nch = std::min(hd::kHeadChunks, nirk), per = ceil_div(nirk, nch);
+  hipLaunchKernelGGL(pygw_hd_head_part, dim3(nom, nch), dim3(256), 0, stream, term, kcw, (long)n_o * n_e, nk_kcw,
+                     k_ind, weight, nirk, per, c4, part);
+  hipLaunchKernelGGL(pygw_hd_head_fin, dim3(ceil_div(nom, 256)), dim3(256), 0, stream, (const double*)part, nch,
+                     drude ? (const double*)wpl2 : nullptr, omega, dUl, nom_w, nom, eta_head, (double2*)head);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }
 

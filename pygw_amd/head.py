@@ -126,7 +126,8 @@ def momentum_elements(tables, pw, abc, Aeig, kl, indgk, nvk, nst, nend, mst, men
     if mmatvv is None:
         mmatvv = torch.empty((n2, n1, 3), dtype=torch.complex128, device=device)
     sz = C.c_size_t(0)
-    _lib.call("pygw_mmat_vv_workspace", int(nvk), nst, nend, mst, mend, C.byref(sz))
+    _lib.call("pygw_mmat_vv_workspace", int(tables.mult.sum()), tables.lmax, nlomax, int(nvk), nst, nend, mst, mend,
+              C.byref(sz))
     if workspace is None or workspace.numel() < sz.value:
         workspace = torch.empty(max(sz.value, 1), dtype=torch.uint8, device=device)
     k = np.ascontiguousarray(np.asarray(kl, dtype=np.float64))
