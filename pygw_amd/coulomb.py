@@ -655,3 +655,40 @@ def calc_minc2(plan, kmql, cstart, cend, mstart, mend, alfp, betp, gamp, s3r, co
     mend-mstart, loctmatsize]."""
     return _minc_common(plan, "pygw_minc2", kmql, cstart, cend, mstart, mend, alfp, betp, gamp, s3r, corind, vpos,
                         cgcoef, out, col0, (cend - cstart, mend - mstart))
+
+
+def fwi0(vol, mult, big_l, rtl):
+    """<chi_im | 1> of the muffin-tin product functions (pygw.py:2216-2228):
+    sqrt(4 pi/Vol) rtl[iat, irm] at the L = 0 functions (M = 0), 0 elsewhere;
+    the product functions ordered atom, function, M as the matrix elements
+    order them.  big_l: per atom type the L of each function; rtl [nat,
+    max_nmix] = <r^L | u_irm>.  Host arithmetic, as in PyGW."""
+    fct = np.sqrt(4 * np.pi / vol)
+    out = []
+    for iat, m in enumerate(np.asarray(mult)):
+        for _ in range(int(m)):
+            for irm, L in enumerate(big_l[iat]):
+                block = np.zeros(2 * int(L) + 1)
+                if L == 0:
+                    block[0] = fct * rtl[iat][irm]
+                out.append(block)
+    return np.concatenate(out) if out else np.zeros(0)
+
+
+def singular_coefficients(gqlen, vol, nkdiv):
+    """singc1, singc2 of the q = 0 Coulomb singularity (pygw.py:2456-2474):
+    intf_n - sum over the q-mesh and G (without q = G = 0) of
+    e^{-alpha Q^2}/Q^n / N_k, n = 1, 2, alpha = (Vol/(6 pi^2))^(1/3),
+    intf1 = Vol/(4 pi^2 alpha), intf2 = Vol sqrt(pi/alpha)/(4 pi^2).
+    gqlen[iq] = |q + G| over the q-point's Coulomb plane waves, iq = 0 the
+    Gamma point with its G = 0 first.  Host arithmetic, as in PyGW."""
+    alpha = (vol / (6 * np.pi ** 2)) ** (1.0 / 3.0)
+    ankp = int(np.prod(nkdiv))
+    sf1 = sf2 = 0.0
+    for iq, q in enumerate(gqlen):
+        Q1 = np.asarray(q, dtype=np.float64)[1 if iq == 0 else 0:]
+        e = np.exp(-alpha * Q1 ** 2)
+        sf1 += np.sum(e / Q1) / ankp
+        sf2 += np.sum(e / Q1 ** 2) / ankp
+    intf = vol / (4 * np.pi ** 2)
+    return intf / alpha - sf1, intf * np.sqrt(np.pi / alpha) - sf2
