@@ -11,6 +11,15 @@
 // All frequencies advance together (grid.z = frequency), so each step is
 // three launches over n_om matrices.  Matrices are column-major, contiguous
 // per frequency (PyGW's eps[:, :, iom] of a Fortran [PB, PB, n_om] array).
+//
+// Only the 64 x 64 tiles on and above the diagonal are eliminated.  With s_i
+// = -1 for the indices already swept and +1 otherwise, every step keeps
+// A_ji = s_i s_j conj(A_ij) (a Hermitian A; the step negates the coupling of
+// the pivot block to the unswept indices), so the strictly lower tiles follow
+// from the upper ones: the panels of a step read them through that relation,
+// and once every index is swept the inverse is Hermitian and the lower
+// triangle is its mirror.  The update then costs about half of the full
+// rank-nb update.
 #include "common.h"
 
 namespace ei {
@@ -18,18 +27,35 @@ constexpr int NB = 64;   // pivot block: each elimination step reads and writes 
 constexpr int KH = 32;   // k half staged at a time by the update
 }
 
-// (A + A^H)/2 + I, in place; one thread per (i <= j).
+// (A + A^H)/2 + I, in place: one block per 32 x 32 tile pair (I <= J) of a
+// frequency, both tiles staged in LDS so that every read and write walks
+// consecutive rows of the column-major matrix.
 __global__ void __launch_bounds__(256) pygw_ei_hermitize(double2* __restrict__ A, int n) {
-  const size_t w = blockIdx.y;
+  __shared__ double2 X[32][33], Y[32][33];
+  const size_t w = blockIdx.z;
   double2* a = A + w * (size_t)n * n;
-  const int j = blockIdx.x * 16 + threadIdx.y;
-  for (int i = threadIdx.x; i <= j && j < n; i += 16) {
-    double2 x = a[i + (size_t)n * j];
-    double2 y = a[j + (size_t)n * i];
-    double2 h = {(x.x + y.x) / 2.0, (x.y - y.y) / 2.0};
-    if (i == j) { h.x = h.x + 1.0; h.y = 0.0; }
-    a[i + (size_t)n * j] = h;
-    a[j + (size_t)n * i] = double2{h.x, -h.y};
+  const int I = blockIdx.x, J = blockIdx.y;
+  if (I > J) return;
+  const int i0 = 32 * I, j0 = 32 * J, tr = threadIdx.x & 31, tc = threadIdx.x >> 5;
+  for (int c = tc; c < 32; c += 8) {
+    const int i = i0 + tr, j = j0 + c, ii = j0 + tr, jj = i0 + c;
+    X[c][tr] = (i < n && j < n) ? a[i + (size_t)n * j] : double2{0.0, 0.0};     // X[c][r] = A[i0 + r][j0 + c]
+    Y[c][tr] = (ii < n && jj < n) ? a[ii + (size_t)n * jj] : double2{0.0, 0.0};  // Y[c][r] = A[j0 + r][i0 + c]
+  }
+  __syncthreads();
+  for (int c = tc; c < 32; c += 8) {
+    // h = (A[i][j] + conj(A[j][i]))/2 at i = i0 + tr, j = j0 + c; A[j][i] = Y[tr][c]
+    const int i = i0 + tr, j = j0 + c;
+    const double2 x = X[c][tr], y = Y[tr][c];
+    double2 h{(x.x + y.x) / 2.0, (x.y - y.y) / 2.0};
+    if (i == j) h = double2{h.x + 1.0, 0.0};
+    if (i < n && j < n) a[i + (size_t)n * j] = h;
+    // the mirror entry A[j0 + tr][i0 + c] = conj(h at i = i0 + c, j = j0 + tr)
+    const int ii = j0 + tr, jj = i0 + c;
+    const double2 x2 = X[tr][c], y2 = Y[c][tr];
+    double2 h2{(x2.x + y2.x) / 2.0, (x2.y - y2.y) / 2.0};
+    if (jj == ii) h2 = double2{h2.x + 1.0, 0.0};
+    if (ii < n && jj < n) a[ii + (size_t)n * jj] = double2{h2.x, -h2.y};
   }
 }
 
@@ -57,9 +83,11 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
 
 // Step 1: invert the pivot block (size nb <= NB) of every frequency in LDS
 // by unblocked Gauss-Jordan; B[w] is written column-major nb x nb.
-// One block of 256 threads per frequency.  info[w] |= 1 if a pivot is not
-// positive (matrix not positive definite).
-__global__ void __launch_bounds__(256) pygw_ei_pivot(const double2* __restrict__ A, int n, int p0, int nb,
+// One block of 1024 threads per frequency (the frequencies alone fill a
+// quarter of the CUs; each elimination step is 4 entries per thread between
+// two barriers).  info[w] |= 1 if a pivot is not positive (matrix not
+// positive definite).
+__global__ void __launch_bounds__(1024) pygw_ei_pivot(const double2* __restrict__ A, int n, int p0, int nb,
                                                       double2* __restrict__ B, int* __restrict__ info) {
   using namespace ei;
   __shared__ double2 S[NB][NB + 1];
@@ -123,7 +151,16 @@ __global__ void __launch_bounds__(256) pygw_ei_panels(const double2* __restrict_
   for (int e = threadIdx.x; e < NB * 64; e += blockDim.x) {
     const int c = e % NB, jj = e / NB;  // consecutive threads: consecutive rows of one column
     const int j = j0 + jj;
-    Cs[c][jj] = (c < nb && j < n) ? a[(p0 + c) + (size_t)n * j] : double2{0.0, 0.0};
+    double2 v{0.0, 0.0};
+    if (c < nb && j < n) {
+      if (j >= p0) {
+        v = a[(p0 + c) + (size_t)n * j];
+      } else {  // A[P, j] of a swept j: -conj(A[j, P]) (stored above the diagonal)
+        const double2 t = a[j + (size_t)n * (p0 + c)];
+        v = double2{-t.x, t.y};
+      }
+    }
+    Cs[c][jj] = v;
   }
   __syncthreads();
   const int jj = threadIdx.x & 63, rg = threadIdx.x >> 6;
@@ -140,7 +177,11 @@ __global__ void __launch_bounds__(256) pygw_ei_panels(const double2* __restrict_
       u[r + (size_t)NB * j] = s;
     }
     double2* l = L + w * (size_t)n * NB;
-    for (int c = 16 * rg; c < 16 * rg + 16 && c < nb; c++) l[j + (size_t)n * c] = a[j + (size_t)n * (p0 + c)];
+    for (int c = 16 * rg; c < 16 * rg + 16 && c < nb; c++) {
+      // A[j, P] below the pivot block: conj(A[P, j]) (both unswept)
+      const double2 v = Cs[c][jj];
+      l[j + (size_t)n * c] = j < p0 + nb ? a[j + (size_t)n * (p0 + c)] : double2{v.x, -v.y};
+    }
   }
 }
 
@@ -171,6 +212,7 @@ __global__ void __launch_bounds__(256) pygw_ei_update_mfma(double2* __restrict__
   const double2* l = L + w * (size_t)n * NB;
   const double2* b = B + w * NB * NB;
   const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  if (i0 > j0) return;  // strictly lower tile: implied by the upper one
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = wave & 1, wj = wave >> 1;  // wave's 32 x 32: rows i0+32wi.., cols j0+32wj..
   const int li = lane & 15, lk = lane >> 4;
@@ -239,6 +281,28 @@ __global__ void __launch_bounds__(256) pygw_ei_update_mfma(double2* __restrict__
       }
 }
 
+// the strictly lower 64 x 64 tiles of the inverse, A[i, j] = conj(A[j, i]):
+// one block per 32 x 32 tile of the strictly lower tile rows, staged through
+// LDS so that reads and writes walk consecutive rows
+__global__ void __launch_bounds__(256) pygw_ei_mirror(double2* __restrict__ A, int n) {
+  __shared__ double2 X[32][33];
+  const size_t w = blockIdx.z;
+  double2* a = A + w * (size_t)n * n;
+  const int i0 = 32 * blockIdx.x, j0 = 32 * blockIdx.y;  // target tile rows i0.., columns j0..
+  if (i0 / 64 <= j0 / 64) return;                        // not strictly below the 64-tile diagonal
+  const int tr = threadIdx.x & 31, tc = threadIdx.x >> 5;
+  for (int c = tc; c < 32; c += 8) {
+    const int i = j0 + tr, j = i0 + c;  // source A[j0 + tr][i0 + c]
+    X[c][tr] = (i < n && j < n) ? a[i + (size_t)n * j] : double2{0.0, 0.0};
+  }
+  __syncthreads();
+  for (int c = tc; c < 32; c += 8) {
+    const int i = i0 + tr, j = j0 + c;  // A[i][j] = conj(A[j][i]) = conj(X[tr][c])
+    const double2 v = X[tr][c];
+    if (i < n && j < n) a[i + (size_t)n * j] = double2{v.x, -v.y};
+  }
+}
+
 // (eps + eps^H)/2 + 1, inverted in place (the -1 is left to the caller)
 int eps_inverse_gj(void* eps, int n, int nom, void* ws, int* dinfo, hipStream_t stream) {
   using namespace ei;
@@ -246,14 +310,15 @@ int eps_inverse_gj(void* eps, int n, int nom, void* ws, int* dinfo, hipStream_t 
   double2* B = (double2*)ws;
   double2* U = B + (size_t)nom * NB * NB;
   double2* L = U + (size_t)nom * NB * n;
-  hipLaunchKernelGGL(pygw_ei_hermitize, dim3(ceil_div(n, 16), nom), dim3(16, 16), 0, stream, A, n);
+  hipLaunchKernelGGL(pygw_ei_hermitize, dim3(ceil_div(n, 32), ceil_div(n, 32), nom), dim3(256), 0, stream, A, n);
   for (int p0 = 0; p0 < n; p0 += NB) {
     const int nb = (n - p0) < NB ? (n - p0) : NB;
-    hipLaunchKernelGGL(pygw_ei_pivot, dim3(nom), dim3(256), 0, stream, A, n, p0, nb, B, dinfo);
+    hipLaunchKernelGGL(pygw_ei_pivot, dim3(nom), dim3(1024), 0, stream, A, n, p0, nb, B, dinfo);
     hipLaunchKernelGGL(pygw_ei_panels, dim3(ceil_div(n, 64), nom), dim3(256), 0, stream, A, n, p0, nb, B, U, L);
     hipLaunchKernelGGL(pygw_ei_update_mfma, dim3(ceil_div(n, 64), ceil_div(n, 64), nom), dim3(256), 0, stream, A, n,
                        p0, nb, B, U, L);
   }
+  hipLaunchKernelGGL(pygw_ei_mirror, dim3(ceil_div(n, 32), ceil_div(n, 32), nom), dim3(256), 0, stream, A, n);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }
 
