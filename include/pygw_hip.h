@@ -687,7 +687,9 @@ int pygw_lapw_band(const void* Aeig, long lda, int conj_a, int nb, int ngk, cons
  *   device; nom <= 4096; workspace: pygw_calc_head_workspace bytes.
  * pygw_wing_pm replaces calc_eps's pm of the q = 0 wing (pygw.py:3122-3141):
  *   pm[k][(io (nbmaxpol - numin) + ie2] for the k-points whose irreducible
- *   index is kii[k], the input of pygw_chi0_wings.  Allocates nothing.
+ *   index is kii[k] (< nirk, the first extent of mmatvv; a k whose index is
+ *   not reads as NaN), the input of pygw_chi0_wings.  In pygw_calc_head a
+ *   k_ind outside kcw's k-points makes the head NaN.  Allocates nothing.
  * ------------------------------------------------------------------------ */
 int pygw_mmat_vv_workspace(int ndf, int lmax, int nlomax, int nvk, int nst, int nend, int mst, int mend,
                            size_t* bytes);
@@ -707,7 +709,7 @@ int pygw_calc_head(const void* mmatvv, const void* mmatcv, const double* Ebnd, i
                    int iop_drude, double eta_head, const double* omega, const double* dUl, int nom_w, void* head,
                    void* workspace, size_t ws_bytes, void* stream);
 int pygw_wing_pm(const void* mmatvv, const void* mmatcv, const double* Ebnd, int ldE, const double* eig_core, int ncg,
-                 const int* kii, int nk, int nomx, int numin, int nbmaxpol, int ncg_p, double vol, void* pm,
+                 int nirk, const int* kii, int nk, int nomx, int numin, int nbmaxpol, int ncg_p, double vol, void* pm,
                  void* stream);
 
 #ifdef __cplusplus

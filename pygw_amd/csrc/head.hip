@@ -477,7 +477,12 @@ __global__ void __launch_bounds__(256) pygw_hd_head_part(const double* __restric
   const int iom = blockIdx.x, ch = blockIdx.y;
   double acc = 0.0;
   for (int irk = ch * per; irk < std::min(nirk, (ch + 1) * per); irk++) {
-    const double* kc = kcw + ((size_t)iom * nk_kcw + k_ind[irk]) * n_pair;
+    const int ik = k_ind[irk];
+    if (ik < 0 || ik >= nk_kcw) {  // not a k of kcw: the head reads as NaN
+      acc = __builtin_nan("");
+      continue;
+    }
+    const double* kc = kcw + ((size_t)iom * nk_kcw + ik) * n_pair;
     const double* t = term + (size_t)irk * n_pair;
     double s = 0.0;
     for (long p = threadIdx.x; p < n_pair; p += blockDim.x) s += t[p] * kc[p];
@@ -520,13 +525,17 @@ __global__ void __launch_bounds__(256) pygw_hd_head_fin(const double* __restrict
 __global__ void __launch_bounds__(256) pygw_hd_pm(const double2* __restrict__ mmatvv, const double2* __restrict__ mmatcv,
                                                  const double* __restrict__ Ebnd, int ldE,
                                                  const double* __restrict__ eig_core, const int* __restrict__ kii,
-                                                 int nk, int nvbm, int ncg, int ncg_p, int n_e, int numin,
+                                                 int nirk, int nk, int nvbm, int ncg, int ncg_p, int n_e, int numin,
                                                  double cw, double2* __restrict__ pm) {
   const long n_o = ncg_p + nvbm;
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (e >= nk * n_o * n_e) return;
   const int ie2 = (int)(e % n_e), io = (int)((e / n_e) % n_o), k = (int)(e / (n_o * n_e));
   const int irk = kii[k];
+  if (irk < 0 || irk >= nirk) {  // not an irreducible k of mmatvv: the row reads as NaN
+    pm[e] = double2{__builtin_nan(""), __builtin_nan("")};
+    return;
+  }
   const double e2 = Ebnd[(size_t)irk * ldE + numin + ie2];
   const double2* m;
   double e1;
@@ -738,15 +747,15 @@ extern "C" int pygw_calc_head(const void* mmatvv, const void* mmatcv, const doub
 }
 
 extern "C" int pygw_wing_pm(const void* mmatvv, const void* mmatcv, const double* Ebnd, int ldE,
-                            const double* eig_core, int ncg, const int* kii, int nk, int nomx, int numin,
+                            const double* eig_core, int ncg, int nirk, const int* kii, int nk, int nomx, int numin,
                             int nbmaxpol, int ncg_p, double vol, void* pm, void* stream) {
   const int nvbm = nomx + 1, n_e = nbmaxpol - numin;
   if (!mmatvv || (ncg_p > 0 && (!mmatcv || !eig_core || ncg < ncg_p)) || !Ebnd || ldE < nbmaxpol || !kii ||
-      nk <= 0 || nomx < 0 || numin < 0 || n_e <= 0 || ncg_p < 0 || !(vol > 0.0) || !pm)
+      nirk <= 0 || nk <= 0 || nomx < 0 || numin < 0 || n_e <= 0 || ncg_p < 0 || !(vol > 0.0) || !pm)
     return PYGW_E_ARG;
   const double cw = 2.0 * sqrt(3.14159265358979323846 / vol) / sqrt(3.0);
   hipLaunchKernelGGL(pygw_hd_pm, dim3(ceil_div((long)nk * (ncg_p + nvbm) * n_e, 256)), dim3(256), 0,
-                     (hipStream_t)stream, (const double2*)mmatvv, (const double2*)mmatcv, Ebnd, ldE, eig_core, kii, nk,
-                     nvbm, ncg, ncg_p, n_e, numin, cw, (double2*)pm);
+                     (hipStream_t)stream, (const double2*)mmatvv, (const double2*)mmatcv, Ebnd, ldE, eig_core, kii,
+                     nirk, nk, nvbm, ncg, ncg_p, n_e, numin, cw, (double2*)pm);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }

@@ -233,6 +233,6 @@ def wing_pm(mmatvv, mmatcv, Ebnd, eig_core, kii_ind, nomx, numin, nbmaxpol, ncg_
     ki = d(kii_ind, torch.int32).contiguous()
     nk = ki.numel()
     pm = torch.empty((nk, (ncg_p + nomx + 1) * (nbmaxpol - numin)), dtype=torch.complex128, device=device)
-    _lib.call("pygw_wing_pm", _vp(vv), _vp(cv), _vp(E), E.shape[1], _vp(ec), ncg, _vp(ki), nk, nomx, numin, nbmaxpol,
-              ncg_p, float(vol), _vp(pm), _stream_ptr(device))
+    _lib.call("pygw_wing_pm", _vp(vv), _vp(cv), _vp(E), E.shape[1], _vp(ec), ncg, vv.shape[0], _vp(ki), nk, nomx, numin,
+              nbmaxpol, ncg_p, float(vol), _vp(pm), _stream_ptr(device))
     return pm

@@ -156,3 +156,20 @@ def test_calc_head_from_lapw_coefficients():
     want = O.head_from_elements(wvv, wcv, base.Ebnd, base.eig_core, base.corind, base.kcw, base.k_ind, base.weight,
                                 base.nomx, base.numin, base.nbmaxpol, base.ncg_p, base.vol)
     _close(head, want)
+
+
+def test_indices_outside_the_arrays_read_as_nan():
+    """A k whose irreducible index is not a row of mmatvv gives a NaN pm row
+    (the others unchanged); a k_ind outside kcw's k-points a NaN head."""
+    s = synth.head_system(nirk=2)
+    rng = np.random.default_rng(6)
+    n_e, nv, ncg = s.nbmaxpol - s.numin, s.nomx + 1, len(s.corind)
+    vv = torch.as_tensor(rng.normal(size=(2, n_e, nv, 3)) + 0j, device="cuda")
+    cv = torch.as_tensor(rng.normal(size=(2, n_e, ncg, 3)) + 0j, device="cuda")
+    pm = H.wing_pm(vv, cv, s.Ebnd, s.eig_core, np.array([1, 2, 0]), s.nomx, s.numin, s.nbmaxpol, s.ncg_p,
+                   s.vol).cpu().numpy()
+    assert np.isnan(pm[1]).all() and not np.isnan(pm[[0, 2]]).any()
+    args = list(_head_args(s))
+    args[3] = np.array([0, 5])                       # k 5 is not a k-point of kcw (2 of them)
+    head = H.head_from_elements(vv, cv, *args).cpu().numpy()
+    assert np.isnan(head).all()
