@@ -1005,3 +1005,105 @@ def head_system(nirk=3, lmax=3, nbmax=9, nomx=3, numin=4, nvk=30, gk=2, n_om=6, 
                       ipwint=ipwint, k2cartes=k2cartes, vol=vol, kl=np.array(kls), Aeig=Aeig, indgk=indgk, nvk=nvks,
                       abc=abc, Ebnd=Ebnd, nomx=nomx, numin=numin, nbmaxpol=nbmaxpol, ncg_p=ncg_p,
                       k_ind=np.arange(nirk), weight=np.ones(nirk), kwfer=kwfer, omega=omega, kcw=kcw)
+
+
+@dataclass
+class PipelineSystem:
+    """Everything calc_minm needs for the k-points of one q on a full k-mesh
+    (no symmetry: every k irreducible), and the band energies calc_eps and
+    calc_selfc need after it.  The atoms and the G list come from a
+    ``LapwSystem`` (lapw, at its kl); per k: kl (fractional), indgk (the nvk
+    shortest |k+G| in gindex, then the local-orbital rows) and Aeig [nbmax,
+    ngk].  The product basis: the muffin-tin functions of a
+    ``minm_mt_system`` on the LAPW atoms (nmix, big_l, s3r, the core list
+    corind with core_l per atom type) and ngq plane waves against ngq_barc
+    (indgq, its inverse, mpwipw [ngq, ngq_barc]); vmat a Hermitian positive
+    definite [loct + ngq]^2 Coulomb matrix.  enk [nbmax, N_k] band energies
+    (Hartree, E_F = 0) with windows nomx/numin, eig_core per corind row."""
+    lapw: LapwSystem
+    nmix: np.ndarray
+    big_l: np.ndarray
+    ncore: np.ndarray
+    lmax: int
+    loct: int
+    s3r: np.ndarray
+    corind: np.ndarray
+    cgcoef: np.ndarray
+    div: tuple
+    kf: np.ndarray
+    kqid: np.ndarray
+    iq: int
+    ql: np.ndarray
+    i_g0: np.ndarray
+    indgk: list
+    nvk: int
+    Aeig: list
+    indgq: np.ndarray
+    indgq_inverse: np.ndarray
+    mpwipw: np.ndarray
+    vmat: np.ndarray
+    enk: np.ndarray
+    nomx: int
+    numin: int
+    eig_core: np.ndarray
+
+
+def pipeline_system(div=(2, 2, 2), q=(1, 0, 0), n_occ=2, n_emp=5, nvk=40, ngq=20, ngq_barc=40, cores=True,
+                    seed=61) -> PipelineSystem:
+    """q in mesh units (q / div is the fractional q-point)."""
+    from .coulomb import cmp_all_gaunt
+    rng = np.random.default_rng(seed)
+    cfg = small_config(div=div, n_occ=n_occ, n_emp=n_emp, q=q, seed=seed)
+    b = band_energies(cfg)
+    nbmax = b.enk.shape[0]
+    ls = lapw_system(lmax=3, lomax=3, nvk=nvk, nbmax=nbmax, gmax=4, seed=seed + 1)
+    nat = len(ls.mult)
+    base = minm_mt_system(lmax=3, nbmax=nbmax, mbmax=nbmax, seed=seed + 2)
+    core_l = base.core_l if cores else [[] for _ in range(nat)]
+    ncore = np.array([len(c) for c in core_l])
+    nfnc = max(int(ncore[a]) + 2 * (ls.lapw.shape[0]) + int(ls.nLO_at[:, a].sum()) for a in range(nat))
+    s3r = rng.normal(size=(base.s3r.shape[0], nfnc, nfnc, nat))
+    s3r = 0.5 * (s3r + s3r.transpose(0, 2, 1, 3))
+    corind = []
+    idf = 0
+    for a in range(nat):
+        for _ in range(ls.mult[a]):
+            for ic, lc in enumerate(core_l[a]):
+                for mc in range(-lc, lc + 1):
+                    corind.append([a, idf, ic, lc, mc])
+            idf += 1
+    corind = np.array(corind, dtype=np.int64).reshape(-1, 5)
+    eig_core = np.array([-20.0 - 5.0 * c[2] - 3.0 * c[0] for c in corind])
+    # k-mesh, k - q, the G list's inverse map
+    kf = mesh.kpoint_coords(div)
+    kqid = mesh.k_minus_q(div)
+    iq = q_index(cfg)
+    ql = np.asarray(q, dtype=np.float64) / np.asarray(div)
+    gmax = int(np.abs(ls.gindex).max())
+    i_g0 = -np.ones((2 * gmax + 1,) * 3, dtype=np.int64)
+    g = ls.gindex
+    i_g0[g[:, 0] + gmax, g[:, 1] + gmax, g[:, 2] + gmax] = np.arange(len(g))
+    nlo_tot = ls.indgk.size - ls.nvk
+    indgk, Aeig = [], []
+    for k in kf:
+        kpg = np.linalg.norm((g + k) @ ls.k2cartes.T, axis=1)
+        order = np.argsort(kpg, kind="stable")
+        lo = rng.choice(order[nvk:nvk + 4 * nlo_tot + 8], size=nlo_tot, replace=False) if nlo_tot else order[:0]
+        ig = np.concatenate([order[:nvk], lo]).astype(np.int64)
+        indgk.append(ig)
+        Aeig.append((rng.normal(size=(nbmax, ig.size)) + 1j * rng.normal(size=(nbmax, ig.size))) / np.sqrt(ig.size))
+    # the interstitial product basis of q
+    indgq = np.argsort(np.linalg.norm((g + ql) @ ls.k2cartes.T, axis=1), kind="stable")[:ngq_barc]
+    inv = -np.ones(indgq.max() + 1, dtype=np.int64)
+    inv[indgq] = np.arange(ngq_barc)
+    mpwipw = (rng.normal(size=(ngq, ngq_barc)) + 1j * rng.normal(size=(ngq, ngq_barc))) / np.sqrt(ngq_barc)
+    mb = base.loctmatsize + ngq
+    qm, _ = np.linalg.qr(rng.normal(size=(mb, mb)) + 1j * rng.normal(size=(mb, mb)))
+    lam = np.geomspace(0.05, 20.0, mb) * (1 + 0.01 * rng.uniform(size=mb))
+    vmat = (qm * lam) @ qm.conj().T
+    vmat = 0.5 * (vmat + vmat.conj().T)
+    return PipelineSystem(lapw=ls, nmix=base.nmix, big_l=base.big_l, ncore=ncore, lmax=base.lmax,
+                          loct=base.loctmatsize, s3r=s3r, corind=corind, cgcoef=cmp_all_gaunt(6), div=tuple(div),
+                          kf=kf, kqid=kqid, iq=iq, ql=ql, i_g0=i_g0, indgk=indgk, nvk=nvk, Aeig=Aeig, indgq=indgq,
+                          indgq_inverse=inv, mpwipw=mpwipw, vmat=vmat, enk=b.enk, nomx=b.nomx, numin=b.numin,
+                          eig_core=eig_core)
