@@ -1250,13 +1250,15 @@ def run_minm_is(args):
                                    vol=s.vol)
     ndim, mdim, ngq = rng_[1] - rng_[0], rng_[3] - rng_[2], s.mpwipw.shape[0]
     out = torch.empty((ndim, mdim, ngq), dtype=torch.complex128, device="cuda")
+    ws = torch.empty(coulomb.minm_is_workspace(ngq, s.mpwipw.shape[1], s.nvi, s.nvj, ndim, mdim),
+                     dtype=torch.uint8, device="cuda")
     st = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
     def step():
         e0.record(st)
         for _ in range(nk):
-            coulomb.calc_minm_is(*dev.args(*rng_), out=out)
+            coulomb.calc_minm_is(*dev.args(*rng_), out=out, workspace=ws)
         e1.record(st)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / 1e3
@@ -1264,18 +1266,25 @@ def run_minm_is(args):
     for _ in range(max(1, args.warmup)):
         step()
     t = sum(step() for _ in range(args.steps)) / args.steps
-    f_k = 8.0 * ngq * ndim * s.nvj * (s.nvi + mdim)
-    print(f"[bench] minm_is {t:.3f} s/q ({f_k * nk / t / 1e12:.1f} TF/s)", file=sys.stderr, flush=True)
+    nb = s.mpwipw.shape[1]
+    f_ref = 8.0 * ngq * ndim * s.nvj * (s.nvi + mdim)          # the Fortran's two products per plane wave
+    rho = coulomb._rho_cheaper(ngq, nb, s.nvj, ndim, mdim)
+    f_k = 8.0 * ndim * nb * mdim * (s.nvj + ngq) if rho else 8.0 * ngq * ndim * s.nvj * (nb + mdim)
+    kernels = (("pygw_is_gemm_ab<2>", "pygw_is_gemm_ab<1>", "pygw_is_reduce", "pygw_is_imap", "pygw_is_imap_fill")
+               if rho else ("pygw_is_gemm1p", "pygw_is_gemm2<4>", "pygw_is_scatter", "pygw_is_zero")) + ("pygw_is_map",)
+    print(f"[bench] minm_is {t:.3f} s/q ({f_k * nk / t / 1e12:.1f} TF/s on the products run, "
+          f"{f_ref * nk / t / 1e12:.1f} TF/s on the Fortran's count)", file=sys.stderr, flush=True)
     rec = {"metric": "interstitial matrix elements (calc_minm_IS) seconds per q (all k)", "value": t, "unit": "s/q",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"minm_is: {nk} k, nvi = nvj = {s.nvi} plane waves, ngq = {ngq} (mpwipw against "
-                                  f"{s.mpwipw.shape[1]}), bands {ndim} x {mdim}"},
-           "roofline": {"bound": "mfma", "kernel": "pygw_is_gemm1p + pygw_is_gemm2", "achieved": f_k * nk / t / 1e12,
+                                  f"{nb}), bands {ndim} x {mdim}",
+                      "association": "pair density first" if rho else "T first (dense)"},
+           "roofline": {"bound": "mfma", "kernel": " + ".join(kernels[:2]), "achieved": f_k * nk / t / 1e12,
                         "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": f_k * nk / t / PEAK_FP64,
-                        "traffic": sum(load_traffic("minm_is", k) or 0.0 for k in (
-                            "pygw_is_gemm1p", "pygw_is_gemm2<4>", "pygw_is_scatter", "pygw_is_zero",
-                            "pygw_is_map")) or None, "flops_per_k": f_k,
+                        "traffic": sum(load_traffic("minm_is", k) or 0.0 for k in kernels) or None,
+                        "flops_per_k": f_k, "fortran_flops_per_k": f_ref,
+                        "fortran_equivalent_tflops": f_ref * nk / t / 1e12,
                         "note": "8 FLOP per complex multiply-add of the two products; time of whole calls"}}
     if not args.no_cpu:
         from oracle import coulomb as O

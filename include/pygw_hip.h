@@ -540,12 +540,18 @@ int pygw_zero_momentum_coulomb(const int* gindex, int n_g, const int* g_start, c
  *   [nbmax_j][ngj] complex128 row-major, mpwipw [ngq][ngq_barc] complex128
  *   (ngq_barc <= 4096 in the gathered form; any in the dense form).  mmat complex128, rows (ie1*(mend-mstart) + ie2) of
  *   ld_mmat >= col0 + ngq (col0 = loctmatsize writes into the concatenated
- *   mmat).  FP64 MFMA.  workspace: NULL or pygw_minm_is_workspace bytes
- *   (room for P [ngq_barc][nvj][ndim]: the first product is then a dense
- *   GEMM over the product basis); a smaller workspace of at least the
- *   kmap and T parts (nvi*nvj int32 rounded to 256 B, then ngq*nvj*ndim
- *   complex128) runs the gathered form instead. */
-int pygw_minm_is_workspace(int ngq, int ngq_barc, int nvi, int nvj, int ndim, size_t* bytes);
+ *   mmat).  FP64 MFMA.  Three associations of the same sum, chosen by
+ *   FLOP count and workspace: the pair density rho[(ie1, ie2)][K] first,
+ *   then mmat = rho mpwipw^T (8 ndim ngq_barc mdim (nvj + ngq) FLOP), or
+ *   T[G][j][ie1] first, then Aeigq T (8 ngq ndim nvj (ngq_barc + mdim)),
+ *   the latter as a dense GEMM over the product basis or gathered.
+ *   workspace: NULL or pygw_minm_is_workspace bytes (the preferred form
+ *   of these sizes); a workspace that holds the dense form's buffers runs
+ *   it; one of at least the kmap and T parts (nvi*nvj int32 rounded to
+ *   256 B, then ngq*nvj*ndim complex128) runs the gathered form.  The
+ *   pair-density form adds its slices over j and over K in a fixed order:
+ *   results do not depend on the launch. */
+int pygw_minm_is_workspace(int ngq, int ngq_barc, int nvi, int nvj, int ndim, int mdim, size_t* bytes);
 int pygw_minm_is(int nstart, int nend, int mstart, int mend, const void* Aeigk, int nbmax_i, int ngi,
                  const void* Aeigq, int nbmax_j, int ngj, const int* iumklap, const void* mpwipw, int ngq,
                  int ngq_barc, int nvi, int nvj, const int* indgk_i, const int* indgk_j, const int* indgq_inverse,

@@ -108,7 +108,8 @@ def calc_minm_vhalf(mmat, barcvm, n_in=None, *, complex64=False, out=None):
 
 
 def calc_minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi, nvj, indgk_i, indgk_j,
-                 indgq_inverse, gindex, i_g0, Vol, *, out=None, col0=0, dense=True, device="cuda"):
+                 indgq_inverse, gindex, i_g0, Vol, *, out=None, col0=0, form="auto", workspace=None,
+                 device="cuda"):
     """fvxcn.calc_minm_is (pygw.py:2661, for_vxcnn.f90:1030-1134): the
     interstitial part of the matrix elements, mmat_it[ie1, ie2, G] =
     1/sqrt(Vol) sum_{i, j} Aeigq[mstart+ie2, j] mpwipw[G, K(G_i - G_j +
@@ -117,8 +118,12 @@ def calc_minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi,
     mend-mstart, ngq], or writes it into out[..., col0:col0 + ngq] of a
     contiguous complex128 [nend-nstart, mend-mstart, ld] (the columns of
     mmat = concatenate((mmat_mt, mmat_it), -1) after loctmatsize).
-    dense=False runs the gathered form of the first product (no
-    [ngq_barc, nvj, ndim] scatter buffer) instead of the dense GEMM."""
+    form "auto" runs the association the library prefers for these sizes
+    (the pair density first when it costs fewer products, else T first as a
+    dense GEMM), with workspace (a uint8 device tensor of at least
+    ``minm_is_workspace`` bytes, reused across calls) or else one the call
+    allocates; "dense" and "gather" force T first, as a dense GEMM or
+    gathered, through the size of the workspace given to the call."""
     device = _require_device(device)
     i32 = lambda x: _dev(x, torch.int32, device).contiguous()
     ak = _dev(Aeigk, torch.complex128, device).contiguous()
@@ -136,10 +141,17 @@ def calc_minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi,
         ld = out.shape[2]
     umk = (C.c_int * 3)(*[int(u) for u in np.asarray(iumklap)])
     ws, wsb = None, 0
-    if not dense:       # the kmap and T parts only
-        wsb = (int(nvi) * int(nvj) * 4 + 255) // 256 * 256 + ngq * int(nvj) * ndim * 16
-        wt = torch.empty(wsb, dtype=torch.uint8, device=device)
-        ws = C.c_void_p(wt.data_ptr())
+    if form in ("dense", "gather"):
+        kt = (int(nvi) * int(nvj) * 4 + 255) // 256 * 256 + ngq * int(nvj) * ndim * 16   # kmap and T
+        wsb = kt + (mp.shape[1] * int(nvj) * ndim * 16 if form == "dense" else 0)
+        if form == "dense" and _rho_cheaper(ngq, mp.shape[1], int(nvj), ndim, mdim) and \
+                wsb >= minm_is_workspace(ngq, mp.shape[1], nvi, nvj, ndim, mdim):
+            raise ValueError("the dense form cannot be selected for these sizes")
+        workspace = torch.empty(wsb, dtype=torch.uint8, device=device)
+    elif form != "auto":
+        raise ValueError("form must be 'auto', 'dense' or 'gather'")
+    if workspace is not None:
+        ws, wsb = C.c_void_p(workspace.data_ptr()), workspace.numel()
     _lib.call("pygw_minm_is", int(nstart), int(nend), int(mstart), int(mend), C.c_void_p(ak.data_ptr()),
               ak.shape[0], ak.shape[1], C.c_void_p(aq.data_ptr()), aq.shape[0], aq.shape[1], umk,
               C.c_void_p(mp.data_ptr()), ngq, mp.shape[1], int(nvi), int(nvj), C.c_void_p(gi.data_ptr()),
@@ -147,6 +159,18 @@ def calc_minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi,
               gx.shape[0], C.c_void_p(g0.data_ptr()), n1, n2, n3, float(Vol), C.c_void_p(out.data_ptr()), ld,
               int(col0), ws, wsb, _stream_ptr(device))
     return out
+
+
+def _rho_cheaper(ngq, ngq_barc, nvj, ndim, mdim):
+    return ndim * ngq_barc * mdim * (nvj + ngq) < ngq * ndim * nvj * (ngq_barc + mdim)
+
+
+def minm_is_workspace(ngq, ngq_barc, nvi, nvj, ndim, mdim):
+    """Bytes of the workspace calc_minm_is prefers for these sizes."""
+    sz = C.c_size_t(0)
+    _lib.call("pygw_minm_is_workspace", int(ngq), int(ngq_barc), int(nvi), int(nvj), int(ndim), int(mdim),
+              C.byref(sz))
+    return sz.value
 
 
 def pw_basis(indgq, ngq, ngq_barc, ipwint, gindex, i_g0, *, device="cuda"):

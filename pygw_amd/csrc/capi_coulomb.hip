@@ -11,6 +11,8 @@ int minm_vhalf_launch(const void* mmat, long nrows, int nin, long ldm, const voi
 
 size_t minm_is_ws_bytes(int ngq, int nvi, int nvj, int ndim);
 size_t minm_is_ws_bytes_dense(int ngq, int ngq_barc, int nvi, int nvj, int ndim);
+size_t minm_is_ws_bytes_rho(int ngq, int ngq_barc, int nvi, int nvj, int ndim, int mdim);
+bool minm_is_rho_cheaper(int ngq, int ngq_barc, int nvj, int ndim, int mdim);
 int minm_is_launch(int nstart, int nend, int mstart, int mend, const void* Aeigk, int ldk, const void* Aeigq,
                    int ldq, const int* umk, const void* mpwipw, int ngq, int ngq_barc, int nvi, int nvj,
                    const int* indgk_i, const int* indgk_j, const int* inv, int inv_size, const int* gindex,
@@ -36,7 +38,7 @@ int coul_mixed_block_launch(const void* mpwipw, int ngq, int ngq_barc, const voi
 namespace {
 constexpr int kMaxMb = 16384;
 constexpr int kMaxBarc = 4096;                  // gathered form of pygw_minm_is (LDS row)
-constexpr size_t kDenseCap = (size_t)4 << 30;    // largest P buffer pygw_minm_is allocates itself
+constexpr size_t kDenseCap = (size_t)4 << 30;    // largest workspace pygw_minm_is allocates itself
 }
 
 extern "C" int pygw_coul_setev_workspace(int mbsize, size_t* bytes) {
@@ -75,9 +77,17 @@ extern "C" int pygw_minm_vhalf(const void* mmat, long n_rows, int n_in, long ld_
                            (hipStream_t)stream);
 }
 
-extern "C" int pygw_minm_is_workspace(int ngq, int ngq_barc, int nvi, int nvj, int ndim, size_t* bytes) {
-  if (ngq <= 0 || ngq_barc <= 0 || nvi <= 0 || nvj <= 0 || ndim <= 0 || !bytes) return PYGW_E_ARG;
-  *bytes = minm_is_ws_bytes_dense(ngq, ngq_barc, nvi, nvj, ndim);
+// the workspace of the form pygw_minm_is prefers for these sizes: the pair
+// density when it costs fewer products, else the dense first product
+static size_t minm_is_preferred_bytes(int ngq, int ngq_barc, int nvi, int nvj, int ndim, int mdim) {
+  return minm_is_rho_cheaper(ngq, ngq_barc, nvj, ndim, mdim)
+             ? minm_is_ws_bytes_rho(ngq, ngq_barc, nvi, nvj, ndim, mdim)
+             : minm_is_ws_bytes_dense(ngq, ngq_barc, nvi, nvj, ndim);
+}
+
+extern "C" int pygw_minm_is_workspace(int ngq, int ngq_barc, int nvi, int nvj, int ndim, int mdim, size_t* bytes) {
+  if (ngq <= 0 || ngq_barc <= 0 || nvi <= 0 || nvj <= 0 || ndim <= 0 || mdim <= 0 || !bytes) return PYGW_E_ARG;
+  *bytes = minm_is_preferred_bytes(ngq, ngq_barc, nvi, nvj, ndim, mdim);
   return PYGW_OK;
 }
 
@@ -100,9 +110,9 @@ extern "C" int pygw_minm_is(int nstart, int nend, int mstart, int mend, const vo
   void* ws = workspace;
   bool own = false;
   if (!ws) {
-    // the dense form's P buffer when it is at most kDenseCap and can be had,
-    // else the gathered form's (kmap + T only)
-    ws_bytes = minm_is_ws_bytes_dense(ngq, ngq_barc, nvi, nvj, ndim);
+    // the preferred form's buffers (pair density or dense P) when they are
+    // at most kDenseCap and can be had, else the gathered form's (kmap + T)
+    ws_bytes = minm_is_preferred_bytes(ngq, ngq_barc, nvi, nvj, ndim, mend - mstart);
     if (ws_bytes > kDenseCap || hipMallocAsync(&ws, ws_bytes, stream) != hipSuccess) {
       (void)hipGetLastError();
       ws = nullptr;

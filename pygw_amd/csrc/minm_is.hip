@@ -17,9 +17,26 @@
 //                  T[G][(j, ie1)] = sum_K mpwipw[G][K] P[K][(j, ie1)]
 //   pygw_is_gemm2  mmat[ie1][ie2][col0 + G] = sum_j Aeigq[mstart+ie2][j] T[G][j][ie1] / sqrt(Vol)
 //
+// or, when it costs fewer products and the workspace holds it, the pair
+// density first (the same sum, associated the other way):
+//   pygw_is_imap_fill, pygw_is_imap  imap[K][j] = the i with K(i, j) = K
+//                        (-1: none; -2: several, when k repeats a plane wave)
+//   pygw_is_gemm_ab      rho[ie2][(ie1, K)] = sum_j Aeigq[mstart+ie2][j] Q(j, (ie1, K)),
+//                        Q(j, (ie1, K)) = Aeigk[nstart+ie1][imap[K][j]] gathered
+//                        while the tile is loaded (the sum over those i for
+//                        -2), split over j in fixed slices that
+//                        pygw_is_reduce adds in order into rho[(ie1, ie2)][K]
+//   pygw_is_gemm_ab      mmat[(ie1, ie2)][col0 + G] = sum_K rho[(ie1, ie2)][K] mpwipw[G][K] / sqrt(Vol)
+//                        (split over K the same way)
+// 8 ndim ngq_barc mdim (nvj + ngq) FLOP against 8 ngq ndim nvj (ngq_barc +
+// mdim): at the polarization's 20 x 120 window, 600 plane waves and 400 of
+// 700 basis plane waves, 1.34e10 against 3.15e10 per k.
+//
 // Both products run on v_mfma_f64_16x16x4f64 (tiles of four waves of 32 x 32,
 // 64 x 64 or 128 x 32 for a band window of at most 32; four real products
 // per complex product).
+#include <algorithm>
+
 #include "gemm_tile.h"
 
 namespace is {
@@ -202,6 +219,177 @@ __global__ void __launch_bounds__(256) pygw_is_gemm1p(const double2* __restrict_
       }
 }
 
+// imap[K][j] = -1 (nvj ints per product-basis plane wave)
+__global__ void __launch_bounds__(256) pygw_is_imap_fill(int* __restrict__ imap, long n) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) imap[e] = -1;
+}
+
+// imap[K][j] = i for K = K(i, j) >= 0: a k-point's plane waves are distinct,
+// so one i per (K, j); a second one marks the slot -2 (the gather then sums
+// every such i, as the reference's sum over i does)
+__global__ void __launch_bounds__(256) pygw_is_imap(const int* __restrict__ kmap, int nvi, int nvj,
+                                                   int* __restrict__ imap) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)nvi * nvj) return;
+  const int K = kmap[e];
+  if (K < 0) return;
+  const int i = (int)(e / nvj), j = (int)(e % nvj);
+  int* slot = imap + (size_t)K * nvj + j;
+  if (atomicCAS(slot, -1, i) != -1) atomicExch(slot, -2);
+}
+
+namespace is {
+// B(k, n) of the first pair-density product: Q(j, (ie1, K)) gathered from
+// Aeigk through imap (n = ie1 nb + K)
+struct GatherB {
+  const int* imap;
+  const int* kmap;
+  const double2* Ak;
+  int ldk, nstart, nvi, nvj, nb;
+  __device__ __forceinline__ double2 operator()(int j, long n) const {
+    const int ie1 = (int)(n / nb), K = (int)(n - (long)ie1 * nb);
+    const int v = imap[(size_t)K * nvj + j];
+    const double2* row = Ak + (size_t)(nstart + ie1) * ldk;
+    if (v >= 0) return row[v];
+    double2 a{0.0, 0.0};
+    if (v == -2)
+      for (int i = 0; i < nvi; i++)
+        if (kmap[(size_t)i * nvj + j] == K) {
+          a.x += row[i].x;
+          a.y += row[i].y;
+        }
+    return a;
+  }
+};
+}  // namespace is
+
+// W[s][m][n] = sum_{k in slice s} A[m lda + k] B(k, n), with B(k, n) =
+// B[k ldb + n] (BM 0), B[n ldb + k] (BM 1) or gathered by g (BM 2); slice s
+// covers k in [s kc, min(K, (s + 1) kc)).  Items (column tile, slice, row
+// tile) with row tiles innermost (rows_inner: the row tiles of one column
+// slice of B run together), else (row tile, slice, column tile), dealt to
+// the XCDs in contiguous runs so that the tiles sharing an operand slice
+// share an L2; 64 x 64 tiles.
+template <int BM>
+__global__ void __launch_bounds__(256) pygw_is_gemm_ab(const double2* __restrict__ A, long lda,
+                                                      const double2* __restrict__ B, long ldb, is::GatherB g, int M,
+                                                      long N, int K, int kc, int splits, int rows_inner, long per,
+                                                      double2* __restrict__ W) {
+  using namespace is;
+  __shared__ double2 As[64][KC + 1];
+  __shared__ double2 Bs[KC][64 + 1];
+  const int ntm = (M + 63) / 64;
+  const long ntn = (N + 63) / 64;
+  const long item = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (item >= ntn * ntm * splits) return;
+  int m0, s;
+  long n0;
+  if (rows_inner) {
+    m0 = (int)(item % ntm) * 64;
+    const long rest = item / ntm;
+    s = (int)(rest % splits);
+    n0 = rest / splits * 64;
+  } else {
+    n0 = item % ntn * 64;
+    const long rest = item / ntn;
+    s = (int)(rest % splits);
+    m0 = (int)(rest / splits) * 64;
+  }
+  const int k0 = s * kc, kn = min(K - k0, kc);
+  d4 aR[2][2], aI[2][2];
+  zero(aR, aI);
+  auto la = [&](int m, int k) -> double2 {
+    const int r = m0 + m;
+    return (r < M && k < kn) ? A[(size_t)r * lda + k0 + k] : double2{0.0, 0.0};
+  };
+  auto lb = [&](int k, int c) -> double2 {
+    const long col = n0 + c;
+    if (k >= kn || col >= N) return double2{0.0, 0.0};
+    if (BM == 2) return g(k0 + k, col);
+    return BM == 1 ? B[(size_t)col * ldb + k0 + k] : B[(size_t)(k0 + k) * ldb + col];
+  };
+  tile<2>(kn, la, lb, As, Bs, aR, aI);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 1, wc = wave & 1, li = lane & 15, lk = lane >> 4;
+  double2* Ws = W + (size_t)s * M * N;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = m0 + 32 * wr + 16 * a + 4 * v + lk;
+        const long col = n0 + 32 * wc + 16 * b + li;
+        if (r < M && col < N) Ws[(size_t)r * N + col] = double2{aR[a][b][v], aI[a][b][v]};
+      }
+}
+
+// out = scale sum_s W[s][m][n], the slices added in order: nb = 0 puts (m, n)
+// at out[m ldo + col0 + n]; nb > 0 reads n as (n / nb, n % nb) and puts it
+// at out[((n / nb) M + m) ldo + col0 + n % nb]
+__global__ void __launch_bounds__(256) pygw_is_reduce(const double2* __restrict__ W, int splits, int M, long N,
+                                                     double scale, int nb, double2* __restrict__ out, long ldo,
+                                                     int col0) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)M * N) return;
+  const int m = (int)(e / N);
+  const long n = e % N;
+  double2 a = W[e];
+  for (int s = 1; s < splits; s++) {
+    const double2 b = W[(size_t)s * M * N + e];
+    a.x += b.x;
+    a.y += b.y;
+  }
+  const size_t o = nb > 0 ? ((size_t)(n / nb) * M + m) * ldo + col0 + n % nb : (size_t)m * ldo + col0 + n;
+  out[o] = double2{scale * a.x, scale * a.y};
+}
+
+namespace is {
+// slices of the inner dimension so that tiles x slices reaches ~1024 blocks,
+// no slice shorter than 128 (a multiple of the 16-wide chunk)
+struct Split {
+  int kc, splits;
+};
+inline Split split_k(long tiles, int K) {
+  long s = std::max<long>(1, std::min<long>((1024 + tiles - 1) / tiles, K / 128));
+  int kc = (int)(((K + s - 1) / s + KC - 1) / KC * KC);
+  return Split{kc, (K + kc - 1) / kc};
+}
+inline long tiles64(long M, long N) { return ((M + 63) / 64) * ((N + 63) / 64); }
+
+struct RhoLayout {
+  size_t kmap, imap, rho, W, total;
+  Split s1, s2;
+};
+inline RhoLayout rho_layout(int ngq, int ngq_barc, int nvi, int nvj, int ndim, int mdim) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  RhoLayout r;
+  const long n1 = (long)ndim * ngq_barc, m2 = (long)ndim * mdim;
+  r.s1 = split_k(tiles64(mdim, n1), nvj);
+  r.s2 = split_k(tiles64(m2, ngq), ngq_barc);
+  r.kmap = 0;
+  r.imap = al((size_t)nvi * nvj * 4);
+  r.rho = r.imap + al((size_t)ngq_barc * nvj * 4);
+  r.W = r.rho + al((size_t)m2 * ngq_barc * 16);
+  r.total = r.W + std::max((size_t)r.s1.splits * mdim * n1, (size_t)r.s2.splits * m2 * ngq) * 16;
+  return r;
+}
+// FLOP of the two associations (complex multiply-adds x 8)
+inline bool rho_cheaper(int ngq, int ngq_barc, int nvj, int ndim, int mdim) {
+  const double rho = (double)ndim * ngq_barc * mdim * ((double)nvj + ngq);
+  const double t = (double)ngq * ndim * nvj * ((double)ngq_barc + mdim);
+  return rho < t;
+}
+}  // namespace is
+
+size_t minm_is_ws_bytes_rho(int ngq, int ngq_barc, int nvi, int nvj, int ndim, int mdim) {
+  return is::rho_layout(ngq, ngq_barc, nvi, nvj, ndim, mdim).total;
+}
+
+bool minm_is_rho_cheaper(int ngq, int ngq_barc, int nvj, int ndim, int mdim) {
+  return is::rho_cheaper(ngq, ngq_barc, nvj, ndim, mdim);
+}
+
 size_t minm_is_ws_bytes(int ngq, int nvi, int nvj, int ndim) {
   (void)ngq;
   return (((size_t)nvi * nvj * 4 + 255) & ~(size_t)255) + (size_t)ngq * nvj * ndim * 16;
@@ -219,11 +407,45 @@ int minm_is_launch(int nstart, int nend, int mstart, int mend, const void* Aeigk
                    size_t ws_size, hipStream_t stream) {
   using namespace is;
   const int ndim = nend - nstart, mdim = mend - mstart;
-  // with room for P, the first product is the dense GEMM over the product
-  // basis; otherwise the gather form, whose LDS row limits ngq_barc
-  const bool dense = ws_size >= minm_is_ws_bytes_dense(ngq, ngq_barc, nvi, nvj, ndim);
-  if (ws_size < minm_is_ws_bytes(ngq, nvi, nvj, ndim) || (!dense && ngq_barc > kMaxBarc)) return PYGW_E_ARG;
+  // the pair-density association when it costs fewer products and the
+  // workspace holds it; else, with room for P, the first product as the
+  // dense GEMM over the product basis; else the gather form, whose LDS row
+  // limits ngq_barc
+  const bool rho = rho_cheaper(ngq, ngq_barc, nvj, ndim, mdim) &&
+                   ws_size >= minm_is_ws_bytes_rho(ngq, ngq_barc, nvi, nvj, ndim, mdim);
+  const bool dense = !rho && ws_size >= minm_is_ws_bytes_dense(ngq, ngq_barc, nvi, nvj, ndim);
+  if (!rho && (ws_size < minm_is_ws_bytes(ngq, nvi, nvj, ndim) || (!dense && ngq_barc > kMaxBarc)))
+    return PYGW_E_ARG;
   int* kmap = (int*)ws;
+  if (rho) {
+    const RhoLayout r = rho_layout(ngq, ngq_barc, nvi, nvj, ndim, mdim);
+    int* imap = (int*)((char*)ws + r.imap);
+    double2* R = (double2*)((char*)ws + r.rho);
+    double2* W = (double2*)((char*)ws + r.W);
+    hipLaunchKernelGGL(pygw_is_map, dim3(ceil_div((long)nvi * nvj, 256)), dim3(256), 0, stream, gindex, ng,
+                       indgk_i, indgk_j, nvi, nvj, i_g0, n1, n2, n3, inv, inv_size, ngq_barc, umk[0], umk[1], umk[2],
+                       kmap);
+    const long nI = (long)ngq_barc * nvj, N1 = (long)ndim * ngq_barc, M2 = (long)ndim * mdim;
+    hipLaunchKernelGGL(pygw_is_imap_fill, dim3((unsigned)std::min<long>(ceil_div(nI, 256), 1L << 16)), dim3(256), 0,
+                       stream, imap, nI);
+    hipLaunchKernelGGL(pygw_is_imap, dim3(ceil_div((long)nvi * nvj, 256)), dim3(256), 0, stream, kmap, nvi, nvj,
+                       imap);
+    const GatherB g{imap, kmap, (const double2*)Aeigk, ldk, nstart, nvi, nvj, ngq_barc};
+    long items = tiles64(mdim, N1) * r.s1.splits, per = (items + 7) / 8;
+    hipLaunchKernelGGL(pygw_is_gemm_ab<2>, dim3((unsigned)(8 * per)), dim3(256), 0, stream,
+                       (const double2*)Aeigq + (size_t)mstart * ldq, (long)ldq, (const double2*)nullptr, 0L, g, mdim,
+                       N1, nvj, r.s1.kc, r.s1.splits, 1, per, W);
+    hipLaunchKernelGGL(pygw_is_reduce, dim3(ceil_div((long)mdim * N1, 256)), dim3(256), 0, stream, W, r.s1.splits,
+                       mdim, N1, 1.0, ngq_barc, R, (long)ngq_barc, 0);
+    items = tiles64(M2, ngq) * r.s2.splits;
+    per = (items + 7) / 8;
+    hipLaunchKernelGGL(pygw_is_gemm_ab<1>, dim3((unsigned)(8 * per)), dim3(256), 0, stream, (const double2*)R,
+                       (long)ngq_barc, (const double2*)mpwipw, (long)ngq_barc, g, (int)M2, (long)ngq, ngq_barc,
+                       r.s2.kc, r.s2.splits, 0, per, W);
+    hipLaunchKernelGGL(pygw_is_reduce, dim3(ceil_div(M2 * ngq, 256)), dim3(256), 0, stream, W, r.s2.splits, (int)M2,
+                       (long)ngq, 1.0 / sqrt(vol), 0, (double2*)mmat, ldo, col0);
+    return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+  }
   double2* T = (double2*)((char*)ws + (((size_t)nvi * nvj * 4 + 255) & ~(size_t)255));
   hipLaunchKernelGGL(pygw_is_map, dim3(ceil_div((long)nvi * nvj, 256)), dim3(256), 0, stream, gindex, ng,
                      indgk_i, indgk_j, nvi, nvj, i_g0, n1, n2, n3, inv, inv_size, ngq_barc, umk[0], umk[1], umk[2],

@@ -57,6 +57,7 @@ class MatrixElements:
             raise ValueError("barcvm must have loctmatsize + ngq rows")
         self.barcvm, self.matsize = barcvm, barcvm.shape[1]
         self.device = d
+        self.ws_is = None       # calc_minm_is's workspace, kept across k
 
     def _coefficients(self, kl, indgk, nv, A, iop, conj):
         c = L.gap2_set_lapwcoef(self.lapw_plan, kl, indgk, iop, True, nv, self.gindex_lapw)
@@ -68,9 +69,12 @@ class MatrixElements:
         umk = np.rint(np.asarray(kil, dtype=np.float64) - np.asarray(kjl, dtype=np.float64) - self.ql).astype(np.int64)
         C.calc_minm_mt(self.mt_plan, self.ql, nst, nend, mst, mend, *bk, *bq, self.consts, None, None, out=out,
                        col0=0)
+        n = C.minm_is_workspace(self.ngq, self.mpwipw.shape[1], nvk, nvq, nend - nst, mend - mst)
+        if self.ws_is is None or self.ws_is.numel() < n:
+            self.ws_is = torch.empty(n, dtype=torch.uint8, device=self.device)
         C.calc_minm_is(nst, nend, mst, mend, Aeigk, Aeigq, umk, self.mpwipw, nvk, nvq, indgk_k, indgk_q,
                        self.indgq_inverse, self.gindex, self.i_g0, self.vol, out=out, col0=self.loct,
-                       device=self.device)
+                       workspace=self.ws_is, device=self.device)
 
     def _bands(self, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq):
         A_k = _dev(Aeigk, torch.complex128, self.device)

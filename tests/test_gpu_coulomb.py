@@ -125,25 +125,60 @@ def test_vhalf_chain_is_phase_free():
 @pytest.mark.parametrize("kw,rng_", [(dict(seed=7, gmax=3, nvi=37, nvj=41, ngq=23, ngq_barc=29, nb=9), (1, 6, 2, 9)),
                                      (dict(seed=8, gmax=5, nvi=300, nvj=321, ngq=117, ngq_barc=150, nb=70),
                                       (3, 23, 10, 70)),
-                                     (dict(seed=9, gmax=4, nvi=90, nvj=80, ngq=40, ngq_barc=60, nb=60), (0, 45, 5, 50))])
+                                     (dict(seed=9, gmax=4, nvi=90, nvj=80, ngq=40, ngq_barc=60, nb=60), (0, 45, 5, 50)),
+                                     (dict(seed=10, gmax=4, nvi=90, nvj=80, ngq=10, ngq_barc=60, nb=60),
+                                      (0, 20, 5, 55))])
 def test_minm_is_vs_oracle(kw, rng_):
     """The interstitial matrix elements against the restatement of
-    calc_minm_IS, alone and written after loctmatsize MT columns of the
-    concatenated mmat (the columns before them left alone)."""
+    calc_minm_IS in every association the library has (the pair density
+    first where it costs fewer products, the first three cases; T first,
+    dense and gathered), alone and written after loctmatsize MT columns of
+    the concatenated mmat (the columns before them left alone); the
+    preferred form twice gives the same bits."""
     import torch
     from pygw_amd import coulomb, synth
     s = synth.interstitial_system(**kw)
     want = O.minm_is(*s.args(*rng_))
     print(f"case minm_is {kw} bands {rng_}", flush=True)
+    nd, md, ngq, nb = rng_[1] - rng_[0], rng_[3] - rng_[2], kw["ngq"], kw["ngq_barc"]
+    rho = coulomb._rho_cheaper(ngq, nb, kw["nvj"], nd, md)
+    assert rho == (kw["ngq"] != 10)
     got = coulomb.calc_minm_is(*s.args(*rng_)).cpu().numpy()
     assert np.abs(got - want).max() < 1e-13 * np.abs(want).max()
-    gat = coulomb.calc_minm_is(*s.args(*rng_), dense=False).cpu().numpy()     # the gathered form
-    assert np.abs(gat - want).max() < 1e-13 * np.abs(want).max()
+    ws = torch.empty(coulomb.minm_is_workspace(ngq, nb, kw["nvi"], kw["nvj"], nd, md), dtype=torch.uint8,
+                     device="cuda")
+    again = coulomb.calc_minm_is(*s.args(*rng_), workspace=ws).cpu().numpy()
+    assert np.array_equal(again, got)
+    forms = ["gather"] + ([] if rho else ["dense"])
+    if rho:
+        try:
+            forms.append(coulomb.calc_minm_is(*s.args(*rng_), form="dense") is not None and "dense")
+        except ValueError:
+            pass
+    for f in forms:
+        g = coulomb.calc_minm_is(*s.args(*rng_), form=f).cpu().numpy()
+        assert np.abs(g - want).max() < 1e-13 * np.abs(want).max(), f
     loct = 13
     full = torch.full(want.shape[:2] + (loct + want.shape[2],), 7.0 + 0j, dtype=torch.complex128, device="cuda")
     coulomb.calc_minm_is(*s.args(*rng_), out=full, col0=loct)
     f = full.cpu().numpy()
     assert np.all(f[..., :loct] == 7.0) and np.abs(f[..., loct:] - want).max() < 1e-13 * np.abs(want).max()
+
+
+def test_minm_is_repeated_plane_wave():
+    """A plane wave listed twice among k's first nvi: every association sums
+    both terms, as the reference's sum over i does (the pair-density form
+    marks such slots and sums them in the gather)."""
+    from pygw_amd import coulomb, synth
+    s = synth.interstitial_system(seed=12, gmax=4, nvi=90, nvj=80, ngq=40, ngq_barc=60, nb=60)
+    s.indgk_i[7] = s.indgk_i[3]
+    s.indgk_i[50] = s.indgk_i[3]
+    rng_ = (0, 45, 5, 50)
+    want = O.minm_is(*s.args(*rng_))
+    print("case minm_is repeated plane wave", flush=True)
+    for f in ("auto", "gather"):
+        got = coulomb.calc_minm_is(*s.args(*rng_), form=f).cpu().numpy()
+        assert np.abs(got - want).max() < 1e-13 * np.abs(want).max(), f
 
 
 def test_coul_setev_nothing_kept_and_degenerate():
