@@ -1321,6 +1321,108 @@ def _minm_mt_inputs(nvk=600, nb=140, seed=20261213):
     return sk, sq, s
 
 
+def run_minm(args):
+    """The product-basis matrix elements of one q as calc_eps consumes them
+    (calc_minm in polarization mode per k, pygw.py:2547-2700, 3096-3101):
+    LAPW coefficients at k and k - q, band coefficients, calc_minm_MT,
+    calc_minm_IS, calc_minc2 and minm = mmat conj(barcvm), through
+    pygw_amd.minm.MatrixElements, for the 512 k of one q at the shape of the
+    C5 chain's operand: 20 occupied x 120 empty bands (plus 36 core rows),
+    mbsize 1000 = loctmatsize 896 (8 atoms, lmax 10) + 104 interstitial
+    plane waves (against 200), PB 900 kept by coul_setev; 600 plane waves
+    per k.  The same synthetic inputs at every k; the k-points run in
+    batches of 8 (MatrixElements.polarization_minm_many: one product with
+    conj(barcvm) per batch)."""
+    import torch
+
+    from pygw_amd import coulomb, lapw, synth
+    from pygw_amd.minm import MatrixElements
+    nk, nvbm, ncbm, nb = 512, 20, 20, 140
+    ngq, ngq_barc, evtol = 104, 200, 1e-7
+    sk, sq, s = _minm_mt_inputs(nb=nb)
+    g, kc, nvk = sk.gindex, sk.k2cartes, sk.nvk
+    nlo_rows = sk.indgk.size - nvk
+    kil = np.asarray(sk.kl, dtype=np.float64)
+    ql = np.array([0.125, 0.125, -0.125])
+    kjl = kil - ql
+
+    def rows(kl):      # the nvk shortest |k+G|, then the local-orbital rows
+        order = np.argsort(np.linalg.norm((g + kl) @ kc.T, axis=1), kind="stable")
+        return order[:nvk + nlo_rows].astype(np.int32)
+
+    gm = int(np.abs(g).max())
+    i_g0 = -np.ones((2 * gm + 1,) * 3, dtype=np.int64)
+    i_g0[g[:, 0] + gm, g[:, 1] + gm, g[:, 2] + gm] = np.arange(len(g))
+    indgq = np.argsort(np.linalg.norm((g + ql) @ kc.T, axis=1), kind="stable")[:ngq_barc]
+    inv = -np.ones(indgq.max() + 1, dtype=np.int64)
+    inv[indgq] = np.arange(ngq_barc)
+    rng = np.random.default_rng(20261214)
+    mpwipw = (rng.normal(size=(ngq, ngq_barc)) + 1j * rng.normal(size=(ngq, ngq_barc))) / np.sqrt(ngq_barc)
+    mb = s.loctmatsize + ngq
+    vmat, _ = synth.coulomb_matrix(mb, 100)
+    cb = coulomb.coul_setev(torch.as_tensor(vmat, device="cuda"), evtol)
+    lp = lapw.LapwPlan(*sk.plan_args())
+    mp = coulomb.MinmMTPlan(s.mult, s.nmix, s.big_l, s.nLO_at, s.ncore, s.lmax)
+    consts = coulomb.mt_constants(mp, s.s3r, s.vpos, s.cgcoef, s.corind)
+    me = MatrixElements(lp, mp, consts, g, i_g0, mpwipw, inv, sk.vol, cb.barcvm, ql)
+    dev = lambda x, dt: torch.as_tensor(np.asarray(x), dtype=dt, device="cuda")  # noqa: E731
+    ig_k, ig_q = dev(rows(kil), torch.int32), dev(rows(kjl), torch.int32)
+    Ak = dev(sk.Aeig, torch.complex128)
+    Aq = dev(np.conj(sq.Aeig[:, :nvk + nlo_rows]), torch.complex128)
+    ncg = len(s.corind)
+    out = torch.empty((ncg + nvbm, nb - ncbm, cb.matsize), dtype=torch.complex64, device="cuda")
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    kb = 8                                      # k-points per product with conj(barcvm)
+    outb = torch.empty((kb,) + tuple(out.shape), dtype=torch.complex64, device="cuda")
+    batch = [(kil, ig_k, nvk, Ak, kjl, ig_q, nvk, Aq)] * kb
+
+    def step(prec):
+        e0.record(st)
+        for _ in range(nk // kb):
+            me.polarization_minm_many(batch, nvbm, ncbm, nb, ncg, out=outb, complex64=True, precision=prec)
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+
+    # the split-precision product against the FP64 one on this k (relative
+    # Frobenius error of the complex64 minm handed to the chain)
+    ref = me.polarization_minm(kil, ig_k, nvk, Ak, kjl, ig_q, nvk, Aq, nvbm, ncbm, nb, ncg, precision="fp64")
+    mix = me.polarization_minm(kil, ig_k, nvk, Ak, kjl, ig_q, nvk, Aq, nvbm, ncbm, nb, ncg, complex64=True,
+                               precision="mixed")
+    err = float(torch.linalg.norm(mix.to(torch.complex128) - ref) / torch.linalg.norm(ref))
+    times = {}
+    for prec in ("mixed", "fp64"):
+        for _ in range(max(1, args.warmup)):
+            step(prec)
+        times[prec] = sum(step(prec) for _ in range(args.steps)) / args.steps
+    t = times["mixed"]
+    pairs = (ncg + nvbm) * (nb - ncbm)
+    f_vh = 8.0 * pairs * mb * cb.matsize
+    print(f"[bench] minm {t:.3f} s/q with the split-precision v^1/2 product ({times['fp64']:.3f} FP64; rel err "
+          f"{err:.1e}; PB {cb.matsize}, mmat conj(barcvm) {f_vh * nk / 1e12:.1f} TFLOP per q)",
+          file=sys.stderr, flush=True)
+    rec = {"metric": "product-basis matrix elements of one q (calc_minm per k: LAPW coefficients, calc_minm_MT, "
+                     "calc_minm_IS, calc_minc2, mmat conj(barcvm)) seconds per q (all k)", "value": t, "unit": "s/q",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64, the v^1/2 product fp16x3-split/fp32-acc (M stored complex64)",
+           "data": "synthetic",
+           "config": {"workload": f"minm: {nk} k, 4 atom types x 2 atoms, lmax 10, {nvk} plane waves + {nlo_rows} "
+                                  f"local-orbital rows, bands {nvbm} x {nb - ncbm} + {ncg} core rows, mbsize {mb} "
+                                  f"(loctmatsize {s.loctmatsize} + {ngq} plane waves against {ngq_barc}), PB "
+                                  f"{cb.matsize}"},
+           "fp64": {"s_per_q": times["fp64"]},
+           "rel_frobenius_error_split_vs_fp64": err,
+           "roofline": {"bound": "mfma", "kernel": "pygw_vh_mixed<true> (minm = mmat conj(barcvm), split precision)",
+                        "flops_per_k": f_vh, "achieved_if_all_time": f_vh * nk / t / 1e12, "peak": PEAK_FP16 / 1e12,
+                        "unit": "TFLOP/s", "frac": f_vh * nk / t / PEAK_FP16, "traffic": None,
+                        "note": "the v^1/2 product's algorithmic FLOP over the whole call time: a lower bound of "
+                                "its rate"}}
+    print(json.dumps(rec))
+
+
 def run_minm_mt(args):
     """The muffin-tin and core matrix elements of one q as calc_minm builds
     them (pygw.py:2559-2680, polarization mode): per k, gap2_set_lapwcoef at
@@ -1592,6 +1694,9 @@ def main():
         return
     if args.config == "minm_mt":
         run_minm_mt(args)
+        return
+    if args.config == "minm":
+        run_minm(args)
         return
     if args.config == "head":
         run_head(args)

@@ -563,6 +563,24 @@ int pygw_coul_setev(const void* vmat, int mbsize, double evtol, const void* wi0,
                     void* workspace, size_t ws_bytes, void* stream);
 int pygw_minm_vhalf(const void* mmat, long n_rows, int n_in, long ld_mmat, const void* barcvm, int ld_barcvm,
                     int matsize, int complex64_out, void* minm, void* stream);
+/* pygw_minm_vhalf_mixed: the same product as pygw_minm_vhalf (pygw.py:2666,
+ *   2684) in split precision, for the split-precision polarization (which
+ *   contracts minm in complex64): mmat's rows and barcvm's columns split
+ *   into FP16 hi/lo parts (scaled per row / per column by powers of two),
+ *   three MFMA terms per real product, FP32 accumulation over i, the result
+ *   unscaled exactly; relative error ~1e-7 of each row's size.
+ *   pygw_minm_vhalf_split_basis writes barcvm's split planes and column
+ *   scales (once per basis) into `basis` of pygw_minm_vhalf_split_basis_bytes
+ *   bytes; pygw_minm_vhalf_mixed then takes them for any n_in <= mbsize,
+ *   with a workspace of pygw_minm_vhalf_mixed_workspace bytes (the rows'
+ *   planes and scales).  n_rows <= 2^30.  minm as in pygw_minm_vhalf. */
+int pygw_minm_vhalf_split_basis_bytes(int mbsize, int matsize, size_t* bytes);
+int pygw_minm_vhalf_split_basis(const void* barcvm, int mbsize, int ld_barcvm, int matsize, void* basis,
+                                size_t bytes, void* stream);
+int pygw_minm_vhalf_mixed_workspace(long n_rows, int n_in, size_t* bytes);
+int pygw_minm_vhalf_mixed(const void* mmat, long n_rows, int n_in, long ld_mmat, const void* basis, int mbsize,
+                          int matsize, int complex64_out, void* minm, void* workspace, size_t ws_bytes,
+                          void* stream);
 
 /* ------------------------------------------------------------------------
  * The muffin-tin and core parts of the matrix elements (minm_mt.hip).

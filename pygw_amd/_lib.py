@@ -109,6 +109,12 @@ _SIGS = {
     "pygw_mixed_coulomb_workspace": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
     "pygw_coul_mixed_block": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_long,
                                         C.c_void_p]),
+    "pygw_minm_vhalf_split_basis_bytes": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
+    "pygw_minm_vhalf_split_basis": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_size_t,
+                                              C.c_void_p]),
+    "pygw_minm_vhalf_mixed_workspace": (C.c_int, [C.c_long, C.c_int, C.POINTER(C.c_size_t)]),
+    "pygw_minm_vhalf_mixed": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_long, C.c_void_p, C.c_int, C.c_int,
+                                        C.c_int, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "pygw_minm_is_workspace": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                          C.POINTER(C.c_size_t)]),
     "pygw_minm_is": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,

@@ -10,7 +10,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpygw_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["tetra_weights.hip", "capi_tetra.hip", "chi0.hip", "chi0_mixed.hip", "eps_inv.hip", "capi_chain.hip", "synth.hip", "mwm.hip", "mwm_mixed.hip", "selfc.hip", "capi_selfc.hip", "gamma.hip", "dos.hip", "coulomb.hip", "minm_is.hip", "pw_basis.hip", "mixed_coulomb.hip", "capi_coulomb.hip", "capi_noalloc.hip", "omega_basis.hip", "mt_coulomb.hip", "minm_mt.hip", "lapw.hip", "head.hip"]
+SOURCES = ["tetra_weights.hip", "capi_tetra.hip", "chi0.hip", "chi0_mixed.hip", "eps_inv.hip", "capi_chain.hip", "synth.hip", "mwm.hip", "mwm_mixed.hip", "selfc.hip", "capi_selfc.hip", "gamma.hip", "dos.hip", "coulomb.hip", "minm_is.hip", "pw_basis.hip", "mixed_coulomb.hip", "capi_coulomb.hip", "capi_noalloc.hip", "omega_basis.hip", "mt_coulomb.hip", "minm_mt.hip", "lapw.hip", "head.hip", "vhalf_mixed.hip"]
 
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]

@@ -107,6 +107,60 @@ def calc_minm_vhalf(mmat, barcvm, n_in=None, *, complex64=False, out=None):
     return out
 
 
+class SplitBasis:
+    """barcvm's FP16 hi/lo planes and column scales for the split-precision
+    product (pygw_minm_vhalf_split_basis), built once per basis."""
+
+    def __init__(self, barcvm):
+        if not barcvm.is_cuda or barcvm.dtype != torch.complex128 or barcvm.stride(0) != 1:
+            raise ValueError("barcvm must be a complex128 device tensor with unit stride along mbsize")
+        self.mbsize, self.matsize = barcvm.shape
+        ldb = barcvm.stride(1) if self.matsize > 1 else self.mbsize
+        sz = C.c_size_t(0)
+        _lib.call("pygw_minm_vhalf_split_basis_bytes", self.mbsize, self.matsize, C.byref(sz))
+        self.buf = torch.empty(sz.value, dtype=torch.uint8, device=barcvm.device)
+        _lib.call("pygw_minm_vhalf_split_basis", C.c_void_p(barcvm.data_ptr()), self.mbsize, ldb, self.matsize,
+                  C.c_void_p(self.buf.data_ptr()), sz.value, _stream_ptr(barcvm.device))
+        self.device = barcvm.device
+
+
+def calc_minm_vhalf_mixed(mmat, basis, n_in=None, *, complex64=True, out=None, workspace=None):
+    """minm = mmat conj(barcvm) (pygw.py:2666, 2684) in split precision
+    (pygw_minm_vhalf_mixed): basis = SplitBasis(barcvm); mmat as for
+    calc_minm_vhalf.  workspace: an optional uint8 device tensor of at least
+    ``minm_vhalf_mixed_workspace(n_rows, n_in)`` bytes."""
+    if not mmat.is_cuda or mmat.dtype != torch.complex128:
+        raise TypeError("mmat must be a complex128 device tensor")
+    ld = mmat.shape[-1]
+    n_in = ld if n_in is None else int(n_in)
+    if n_in > basis.mbsize or n_in > ld or mmat.stride(-1) != 1:
+        raise ValueError("n_in exceeds the inner dimension, or mmat rows are not contiguous")
+    rows = mmat.reshape(-1, ld)
+    if rows.data_ptr() != mmat.data_ptr() or (rows.shape[0] > 1 and rows.stride(0) != ld):
+        raise ValueError("mmat rows must be equally spaced")
+    nrows, ms = rows.shape[0], basis.matsize
+    dt = torch.complex64 if complex64 else torch.complex128
+    if out is None:
+        out = torch.empty(mmat.shape[:-1] + (ms,), dtype=dt, device=mmat.device)
+    elif out.dtype != dt or not out.is_contiguous() or out.numel() != nrows * ms:
+        raise ValueError("out must be a contiguous tensor of the result's size and type")
+    if nrows == 0:
+        return out
+    need = minm_vhalf_mixed_workspace(nrows, n_in)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=mmat.device)
+    _lib.call("pygw_minm_vhalf_mixed", C.c_void_p(mmat.data_ptr()), nrows, n_in, ld, C.c_void_p(basis.buf.data_ptr()),
+              basis.mbsize, ms, int(complex64), C.c_void_p(out.data_ptr()), C.c_void_p(workspace.data_ptr()),
+              workspace.numel(), _stream_ptr(mmat.device))
+    return out
+
+
+def minm_vhalf_mixed_workspace(n_rows, n_in):
+    sz = C.c_size_t(0)
+    _lib.call("pygw_minm_vhalf_mixed_workspace", int(n_rows), int(n_in), C.byref(sz))
+    return sz.value
+
+
 def calc_minm_is(nstart, nend, mstart, mend, Aeigk, Aeigq, iumklap, mpwipw, nvi, nvj, indgk_i, indgk_j,
                  indgq_inverse, gindex, i_g0, Vol, *, out=None, col0=0, form="auto", workspace=None,
                  device="cuda"):
@@ -594,6 +648,14 @@ def _check_corind(plan, co, rows=None):
         raise ValueError("corind rows must be (atom type, atom of that type, core index < ncore, l, |m| <= l)")
 
 
+def minm_mt_workspace(plan, n1, n2):
+    """Bytes of calc_minm_mt's workspace for an n1 x n2 band window."""
+    ph, _ = plan.ptrs()
+    sz = C.c_size_t(0)
+    _lib.call("pygw_minm_mt_workspace", ph, int(n1), int(n2), C.byref(sz))
+    return sz.value
+
+
 def calc_minm_mt(plan, ql, nstart, nend, mstart, mend, alfa, beta, gama, alfp, betp, gamp, s3r, vpos, cgcoef, *,
                  out=None, col0=0, workspace=None):
     """fvxcn.calc_minm_mt (pygw.py:2655, for_vxcnn.f90:839-1028):
@@ -614,7 +676,7 @@ def calc_minm_mt(plan, ql, nstart, nend, mstart, mend, alfa, beta, gama, alfp, b
     ga = gp = None
     if nlo > 0:
         ga, gp = _bands(gama, device), _bands(gamp, device)
-    qd = _dev(np.asarray(ql, dtype=np.float64), torch.float64, device)
+    qd = _dev(ql if isinstance(ql, torch.Tensor) else np.asarray(ql, dtype=np.float64), torch.float64, device)
     if out is None:
         out = torch.empty((n1, n2, loct), dtype=torch.complex128, device=device)
         ld = loct
@@ -645,7 +707,7 @@ def _minc_common(plan, name, kv, b0, b1, c0, c1, a, b, g, s3r, corind, vpos, cgc
     k = _consts(plan, s3r, vpos, cgcoef, corind)
     rows = (c0, c1) if name == "pygw_minc" else (b0, b1)
     _check_corind(plan, k.corind_host, rows)
-    kd = _dev(np.asarray(kv, dtype=np.float64), torch.float64, device)
+    kd = _dev(kv if isinstance(kv, torch.Tensor) else np.asarray(kv, dtype=np.float64), torch.float64, device)
     if out is None:
         out = torch.empty((*shape, plan.loctmatsize), dtype=torch.complex128, device=device)
         ld = plan.loctmatsize

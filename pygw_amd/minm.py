@@ -57,7 +57,23 @@ class MatrixElements:
             raise ValueError("barcvm must have loctmatsize + ngq rows")
         self.barcvm, self.matsize = barcvm, barcvm.shape[1]
         self.device = d
+        self.ql_dev = torch.as_tensor(self.ql, device=d)
         self.ws_is = None       # calc_minm_is's workspace, kept across k
+        self.ws_mt = None       # calc_minm_mt's
+        self._kdev = {}         # k-points on the device (the MT and core kernels read them there)
+        self.mmat_buf = None    # polarization_minm's mmat, reused across k
+        self.mmat_many = None   # polarization_minm_many's
+        self.split_basis = None  # barcvm's split planes, made on the first split-precision call
+        self.ws_vh = None
+
+    def _kd(self, kl):
+        """kl on the device, uploaded once per distinct k-point (a host-to-
+        device copy per call would wait for the stream)."""
+        key = tuple(float(x) for x in np.asarray(kl, dtype=np.float64))
+        t = self._kdev.get(key)
+        if t is None:
+            t = self._kdev[key] = torch.as_tensor(np.asarray(key), device=self.device)
+        return t
 
     def _coefficients(self, kl, indgk, nv, A, iop, conj):
         c = L.gap2_set_lapwcoef(self.lapw_plan, kl, indgk, iop, True, nv, self.gindex_lapw)
@@ -65,22 +81,31 @@ class MatrixElements:
 
     def _mmat(self, out, kil, kjl, indgk_k, nvk, Aeigk, indgk_q, nvq, Aeigq, nst, nend, mst, mend, bk, bq):
         """mmat_mt and mmat_it of bands nst..nend-1 at k and mst..mend-1 at
-        k - q into out [nend-nst, mend-mst, mbsize] (pygw.py:2655-2663)."""
+        k - q into out [nend-nst, mend-mst, mbsize] (pygw.py:2655-2663); bk /
+        bq hold the band coefficients of exactly those bands."""
         umk = np.rint(np.asarray(kil, dtype=np.float64) - np.asarray(kjl, dtype=np.float64) - self.ql).astype(np.int64)
-        C.calc_minm_mt(self.mt_plan, self.ql, nst, nend, mst, mend, *bk, *bq, self.consts, None, None, out=out,
-                       col0=0)
-        n = C.minm_is_workspace(self.ngq, self.mpwipw.shape[1], nvk, nvq, nend - nst, mend - mst)
+        n1, n2 = nend - nst, mend - mst
+        n = C.minm_mt_workspace(self.mt_plan, n1, n2)
+        if self.ws_mt is None or self.ws_mt.numel() < n:
+            self.ws_mt = torch.empty(n, dtype=torch.uint8, device=self.device)
+        C.calc_minm_mt(self.mt_plan, self.ql_dev, 0, n1, 0, n2, *bk, *bq, self.consts, None, None, out=out, col0=0,
+                       workspace=self.ws_mt)
+        n = C.minm_is_workspace(self.ngq, self.mpwipw.shape[1], nvk, nvq, n1, n2)
         if self.ws_is is None or self.ws_is.numel() < n:
             self.ws_is = torch.empty(n, dtype=torch.uint8, device=self.device)
         C.calc_minm_is(nst, nend, mst, mend, Aeigk, Aeigq, umk, self.mpwipw, nvk, nvq, indgk_k, indgk_q,
                        self.indgq_inverse, self.gindex, self.i_g0, self.vol, out=out, col0=self.loct,
                        workspace=self.ws_is, device=self.device)
 
-    def _bands(self, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq):
+    def _bands(self, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, rows_k, rows_q):
+        """The eigenvectors on the device and the band coefficients of the
+        bands rows_k = (first, end) at k and rows_q at k - q only (the
+        reference forms them for all bands, pygw.py:2577-2606; the matrix
+        elements read no others)."""
         A_k = _dev(Aeigk, torch.complex128, self.device)
         A_q = _dev(Aeigq, torch.complex128, self.device)
-        bk = self._coefficients(kil, indgk_k, nvk, A_k, 1, False)
-        bq = self._coefficients(kjl, indgk_q, nvq, A_q, 2, False)
+        bk = self._coefficients(kil, indgk_k, nvk, A_k[rows_k[0]:rows_k[1]], 1, False)
+        bq = self._coefficients(kjl, indgk_q, nvq, A_q[rows_q[0]:rows_q[1]], 2, False)
         return A_k, A_q, bk, bq
 
     def calc_minm(self, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, band_limits, mode):
@@ -92,39 +117,79 @@ class MatrixElements:
         if mode not in ("selfe", "polarization"):
             raise ValueError("mode must be 'selfe' or 'polarization'")
         nst, nend, mst, mend, cst, cend = (int(x) for x in band_limits)
-        A_k, A_q, bk, bq = self._bands(kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq)
+        A_k, A_q, bk, bq = self._bands(kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, (nst, nend), (mst, mend))
         mmat = torch.empty((nend - nst, mend - mst, self.mbsize), dtype=torch.complex128, device=self.device)
         self._mmat(mmat, kil, kjl, indgk_k, nvk, A_k, indgk_q, nvq, A_q, nst, nend, mst, mend, bk, bq)
         minm = C.calc_minm_vhalf(mmat, self.barcvm)
         minc = None
         if cend > 0:
             if mode == "selfe":
-                mc = C.calc_minc(self.mt_plan, kil, nst, nend, cst, cend, *bk, self.consts, None, None, None)
+                mc = C.calc_minc(self.mt_plan, self._kd(kil), 0, nend - nst, cst, cend, *bk, self.consts, None, None,
+                                 None)
             else:
-                mc = C.calc_minc2(self.mt_plan, kjl, cst, cend, mst, mend, *bq, self.consts, None, None, None)
+                mc = C.calc_minc2(self.mt_plan, self._kd(kjl), cst, cend, 0, mend - mst, *bq, self.consts, None, None,
+                                  None)
             minc = C.calc_minm_vhalf(mc, self.barcvm, self.loct)
         return minm, minc
 
+    def _vhalf(self, mmat, precision, complex64, out):
+        if precision == "fp64":
+            if out is not None:
+                return C.calc_minm_vhalf(mmat, self.barcvm, complex64=complex64, out=out)
+            return C.calc_minm_vhalf(mmat, self.barcvm, complex64=complex64)
+        if precision != "mixed":
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if self.split_basis is None:
+            self.split_basis = C.SplitBasis(self.barcvm)
+        n = C.minm_vhalf_mixed_workspace(mmat.numel() // mmat.shape[-1], mmat.shape[-1])
+        if self.ws_vh is None or self.ws_vh.numel() < n:
+            self.ws_vh = torch.empty(n, dtype=torch.uint8, device=self.device)
+        return C.calc_minm_vhalf_mixed(mmat, self.split_basis, complex64=complex64, out=out, workspace=self.ws_vh)
+
     def polarization_minm(self, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, nvbm, ncbm, nbmaxpol, ncg_p,
-                          *, out=None, complex64=False):
+                          *, out=None, complex64=False, precision="fp64"):
         """The minm that calc_eps contracts (pygw.py:3096-3101): band_limits
         (0, nvbm, ncbm, nbmaxpol, 0, ncg_p), the core rows (calc_minc2) first
         and the occupied bands after them, [ncg_p + nvbm, nbmaxpol - ncbm,
         matsize].  The core rows are written into the same mmat with zero
         interstitial columns, so one product with conj(barcvm) covers both
-        (the zero columns add nothing to minc = mmat_c conj(barcvm[:loct]))."""
+        (the zero columns add nothing to minc = mmat_c conj(barcvm[:loct])).
+        precision "mixed": the product with conj(barcvm) in split precision
+        (the operand of the split-precision polarization; ~1e-7 of each
+        row's size), everything before it FP64."""
+        shape = (ncg_p + nvbm, nbmaxpol - ncbm, self.mbsize)
+        if self.mmat_buf is None or tuple(self.mmat_buf.shape) != shape:
+            self.mmat_buf = torch.empty(shape, dtype=torch.complex128, device=self.device)
+        self._pol_mmat(self.mmat_buf, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, nvbm, ncbm, nbmaxpol, ncg_p)
+        return self._vhalf(self.mmat_buf, precision, complex64, out)
+
+    def polarization_minm_many(self, ks, nvbm, ncbm, nbmaxpol, ncg_p, *, out=None, complex64=False,
+                               precision="fp64"):
+        """polarization_minm for several k-points at once: ks a list of
+        (kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq); returns [len(ks),
+        ncg_p + nvbm, nbmaxpol - ncbm, matsize].  The mmat of every k is built
+        first and the product with conj(barcvm) runs once over all their rows
+        (a launch that fills the chip where one k's rows would not)."""
+        shape = (len(ks), ncg_p + nvbm, nbmaxpol - ncbm, self.mbsize)
+        if self.mmat_many is None or tuple(self.mmat_many.shape) != shape:
+            self.mmat_many = torch.empty(shape, dtype=torch.complex128, device=self.device)
+        for j, k in enumerate(ks):
+            self._pol_mmat(self.mmat_many[j], *k, nvbm, ncbm, nbmaxpol, ncg_p)
+        return self._vhalf(self.mmat_many, precision, complex64, out)
+
+    def _pol_mmat(self, mmat, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, nvbm, ncbm, nbmaxpol, ncg_p):
+        """calc_eps's mmat of one k into mmat [ncg_p + nvbm, nbmaxpol - ncbm,
+        mbsize]: the core rows (MT columns only, the rest zero), then the
+        occupied bands."""
         n2 = nbmaxpol - ncbm
-        A_k, A_q, bk, bq = self._bands(kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq)
-        mmat = torch.empty((ncg_p + nvbm, n2, self.mbsize), dtype=torch.complex128, device=self.device)
+        A_k, A_q, bk, bq = self._bands(kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, (0, nvbm),
+                                       (ncbm, nbmaxpol))
         self._mmat(mmat[ncg_p:], kil, kjl, indgk_k, nvk, A_k, indgk_q, nvq, A_q, 0, nvbm, ncbm, nbmaxpol, bk, bq)
         if ncg_p > 0:
             core = mmat[:ncg_p]
             core[..., self.loct:].zero_()
-            C.calc_minc2(self.mt_plan, kjl, 0, ncg_p, ncbm, nbmaxpol, *bq, self.consts, None, None, None, out=core,
+            C.calc_minc2(self.mt_plan, self._kd(kjl), 0, ncg_p, 0, n2, *bq, self.consts, None, None, None, out=core,
                          col0=0)
-        if out is not None:
-            return C.calc_minm_vhalf(mmat, self.barcvm, complex64=complex64, out=out)
-        return C.calc_minm_vhalf(mmat, self.barcvm, complex64=complex64)
 
     def selfe_minm(self, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, ibgw, nbgw, nbands, ncg_c):
         """The minm that calc_selfc contracts (pygw.py:3339-3345): band_limits

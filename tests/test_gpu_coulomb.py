@@ -165,6 +165,40 @@ def test_minm_is_vs_oracle(kw, rng_):
     assert np.all(f[..., :loct] == 7.0) and np.abs(f[..., loct:] - want).max() < 1e-13 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("nrows,n_in,dense_rows", [(300, None, False), (1111, 640, True), (5, None, True)])
+def test_minm_vhalf_mixed_vs_fp64(nrows, n_in, dense_rows):
+    """The split-precision product with conj(barcvm) against the FP64 one:
+    rows of very different sizes (and a zero row), all of mbsize or the
+    first n_in (minc's loctmatsize); every entry within 1e-6 of its row's
+    largest, the whole within 3e-7 (Frobenius: the two 22-bit operand
+    splits, ~1.2e-7 each, and the FP32 accumulation), complex64 and
+    complex128 results alike."""
+    import torch
+    from pygw_amd import coulomb, synth
+    vmat, _ = synth.coulomb_matrix(700, 60, seed=31)
+    cb = coulomb.coul_setev(torch.as_tensor(vmat, device="cuda"), 1e-7)
+    rng = np.random.default_rng(nrows)
+    mb = vmat.shape[0]
+    m = (rng.normal(size=(nrows, mb)) + 1j * rng.normal(size=(nrows, mb)))
+    m *= 10.0 ** rng.uniform(-6, 3, size=(nrows, 1))
+    if dense_rows:
+        m[0] = 0.0
+    print(f"case minm_vhalf_mixed rows {nrows} n_in {n_in}", flush=True)
+    mm = torch.as_tensor(m, device="cuda")
+    basis = coulomb.SplitBasis(cb.barcvm)
+    want = coulomb.calc_minm_vhalf(mm, cb.barcvm, n_in).cpu().numpy()
+    for c64 in (True, False):
+        got = coulomb.calc_minm_vhalf_mixed(mm, basis, n_in, complex64=c64).cpu().numpy()
+        assert got.dtype == (np.complex64 if c64 else np.complex128)
+        scale = np.abs(want).max(axis=1, keepdims=True)
+        err = np.abs(got - want) / np.where(scale > 0, scale, 1.0)
+        fro = np.linalg.norm(got - want) / np.linalg.norm(want)
+        print(f"  complex64 {c64}: max {err.max():.2e} of the row's largest, Frobenius {fro:.2e}", flush=True)
+        assert err.max() < 1e-6 and fro < 3e-7, (err.max(), fro)
+        if dense_rows:
+            assert np.all(got[0] == 0)
+
+
 def test_minm_is_repeated_plane_wave():
     """A plane wave listed twice among k's first nvi: every association sums
     both terms, as the reference's sum over i does (the pair-density form

@@ -176,3 +176,69 @@ def test_pipeline_through_the_q_loop_vs_oracle():
     print(f"  Sigma_c from the LAPW matrix elements: max rel {e:.2e}", flush=True)
     assert e < 1e-11
     torch.cuda.synchronize()
+
+
+def test_pipeline_split_precision_vs_oracle():
+    """The same chain with the product with conj(barcvm) and the q loop in
+    split precision (minm handed over in complex64) against the FP64 oracle
+    composition: chi0 and eps^-1 within 1e-6 per frequency, Sigma_c within
+    1e-5 (the split-precision loop's own bars)."""
+    import torch
+    from pygw_amd.qloop import SelfEnergyQLoop
+    s = synth.pipeline_system(seed=67)
+    print("case pipeline q loop, split precision", flush=True)
+    me, cb = _gpu_builder(s)
+    B = cb.barcvm.cpu().numpy()
+    _, _, _, Bo, _ = OC.coul_setev(s.vmat, 1e-10)
+    ncg, nb, nk = len(s.corind), s.enk.shape[0], len(s.kf)
+    enk = np.vstack([np.repeat(s.eig_core[:, None], nk, axis=1), s.enk])
+    fm = mesh.FrequencyMesh(8, 0.05 / mesh.H2EV, 80.0 / mesh.H2EV, iop_multiple=5)
+    tetc = mesh.tetra_corners(s.div)
+    kq = s.kqid[:, s.iq]
+    pb = cb.matsize
+    L = SelfEnergyQLoop(enk, 0.0, ncg, s.nomx, s.numin, fm.omega, fm.womeg, tetc, pb, 1, len(KS), NB1, ncg + nb,
+                        omega_p=fm.omega_precise, womeg_p=fm.womeg_precise, precision="mixed")
+    M = torch.stack([me.polarization_minm(*_k(s, ik), s.nomx + 1, s.numin, nb, ncg, complex64=True,
+                                          precision="mixed").reshape(-1, pb) for ik in range(nk)])
+    Ms = torch.stack([_gpu_selfe(s, me, ik).reshape(-1, pb) for ik in KS]).to(torch.complex64)
+    enq = np.ascontiguousarray(enk[:, kq[list(KS)]].T)
+    L.add_q(kq, M, Ms, enq, keep_chi0=True)
+    kn = L.kcw.cpu().numpy()
+    Mo = [_oracle_pol(s, ik, Bo).reshape(-1, pb) for ik in range(nk)]
+    chi = chain.calc_eps_accumulate(((ik, Mo[ik]) for ik in range(nk)), lambda ik: kn[ik].T)
+    W = chain.eps_inverse(chi)
+    got_chi, got_w = L.chi.cpu().numpy(), L.eps.cpu().numpy()
+    one = np.eye(pb)
+    rc = ri = 0.0
+    for w in range(fm.omega.size):
+        rc = max(rc, _rel(B @ got_chi[:, :, w] @ B.conj().T, Bo @ chi[:, :, w] @ Bo.conj().T))
+        # eps^-1 itself (W + 1) in the basis-free form
+        ri = max(ri, _rel(B @ (got_w[:, :, w] + one) @ B.conj().T, Bo @ (W[:, :, w] + one) @ Bo.conj().T))
+    sc = np.empty((len(KS), NB1, fm.omega.size), dtype=complex)
+    for j, ik in enumerate(KS):
+        ms = _oracle_selfe(s, ik, Bo).reshape(-1, pb)
+        mw = chain.calc_mwm(ms, W, 1.0).reshape(fm.omega.size, NB1, -1)
+        sc[j] = chain.selfc_spline(enq[j], mw, fm.omega, fm.omega_precise, fm.womeg_precise)
+    got_sc = L.sigc.cpu().numpy()
+    rs = max(np.linalg.norm(got_sc[:, :, w] - sc[:, :, w]) / np.linalg.norm(sc[:, :, w])
+             for w in range(fm.omega.size))
+    print(f"  chi0 {rc:.2e} eps^-1 {ri:.2e} Sigma_c {rs:.2e}", flush=True)
+    assert rc < 1e-6 and ri < 1e-6 and rs < 1e-5
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_polarization_minm_many_equals_one_by_one(precision):
+    """Several k-points through one product with conj(barcvm) give what
+    each gives alone."""
+    import torch
+    s = synth.pipeline_system()
+    me, _ = _gpu_builder(s)
+    print(f"case polarization_minm_many {precision}", flush=True)
+    nb, ncg = s.enk.shape[0], len(s.corind)
+    ks = [_k(s, ik) for ik in range(len(s.kf))]
+    many = me.polarization_minm_many(ks, s.nomx + 1, s.numin, nb, ncg, precision=precision).cpu().numpy()
+    for ik, k in enumerate(ks):
+        one = me.polarization_minm(*k, s.nomx + 1, s.numin, nb, ncg, precision=precision).cpu().numpy()
+        assert np.abs(many[ik] - one).max() <= 1e-14 * np.abs(one).max()
+    torch.cuda.synchronize()
