@@ -48,7 +48,8 @@ def _zeros(fn):
     "pygw_minm_mt_plan", "pygw_minm_mt_workspace", "pygw_minm_mt", "pygw_minc", "pygw_minc2",
     "pygw_lapw_plan", "pygw_lapw_coef", "pygw_lapw_band",
     "pygw_mmat_vv_workspace", "pygw_mmat_vv", "pygw_mmat_cv", "pygw_calc_head_workspace", "pygw_calc_head",
-    "pygw_wing_pm",
+    "pygw_wing_pm", "pygw_minm_vhalf_split_basis_bytes", "pygw_minm_vhalf_split_basis",
+    "pygw_minm_vhalf_mixed_workspace", "pygw_minm_vhalf_mixed",
 ])
 def test_null_and_zero_arguments_rejected(name):
     fn = _fn(name)
@@ -222,3 +223,28 @@ def test_lapw_plan_layout_and_checks():
     u = s.umt.copy()
     u[1, 0, 0] = 0.0        # APW l = 0 of type 1 with p = 0
     assert call(umt=u)[0] == E_ARG
+
+
+def test_minm_is_workspace_follows_the_cheaper_association():
+    """The interstitial workspace query gives the pair-density buffers (map,
+    inverse map, rho, the slice sums) where that association costs fewer
+    products, and the dense T-first buffers (map, T, P) otherwise."""
+    q = _fn("pygw_minm_is_workspace")
+    b = C.c_size_t(0)
+    al = lambda x: (x + 255) // 256 * 256  # noqa: E731
+    # 600 plane waves, 400 of 700 basis plane waves, 20 x 120 bands: the pair density
+    assert q(400, 700, 600, 600, 20, 120, C.byref(b)) == 0
+    s1 = 3 * 120 * 20 * 700          # three slices of j (600) over 2 x 219 tiles
+    s2 = 4 * 20 * 120 * 400          # four slices of K (700) over 38 x 7 tiles
+    assert b.value == al(600 * 600 * 4) + al(700 * 600 * 4) + al(20 * 120 * 700 * 16) + max(s1, s2) * 16
+    # 10 basis plane waves against 60, 20 x 50 bands: T first (dense)
+    assert q(10, 60, 90, 80, 20, 50, C.byref(b)) == 0
+    assert b.value == al(90 * 80 * 4) + 10 * 80 * 20 * 16 + 60 * 80 * 20 * 16
+
+
+def test_vhalf_split_sizes():
+    b = C.c_size_t(0)
+    assert _fn("pygw_minm_vhalf_split_basis_bytes")(1000, 900, C.byref(b)) == 0
+    assert b.value == 3840 + 32 * 16 * 1024 * 16        # column scales (900 floats, 256-B aligned), planes
+    assert _fn("pygw_minm_vhalf_mixed_workspace")(6720, 1000, C.byref(b)) == 0
+    assert b.value == 26880 + 32 * 16 * 6784 * 16
