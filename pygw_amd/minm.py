@@ -58,8 +58,8 @@ class MatrixElements:
         self.barcvm, self.matsize = barcvm, barcvm.shape[1]
         self.device = d
         self.ql_dev = torch.as_tensor(self.ql, device=d)
-        self.ws_is = None       # calc_minm_is's workspace, kept across k
-        self.ws_mt = None       # calc_minm_mt's
+        self.ws = {}            # per stream slot: calc_minm_mt's and calc_minm_is's workspaces, kept across k
+        self.streams = []       # side streams of polarization_minm_many
         self._kdev = {}         # k-points on the device (the MT and core kernels read them there)
         self.mmat_buf = None    # polarization_minm's mmat, reused across k
         self.mmat_many = None   # polarization_minm_many's
@@ -79,23 +79,26 @@ class MatrixElements:
         c = L.gap2_set_lapwcoef(self.lapw_plan, kl, indgk, iop, True, nv, self.gindex_lapw)
         return L.band_coefficients(A, *c, conj=conj)
 
-    def _mmat(self, out, kil, kjl, indgk_k, nvk, Aeigk, indgk_q, nvq, Aeigq, nst, nend, mst, mend, bk, bq):
+    def _workspace(self, slot, name, n):
+        w = self.ws.get((slot, name))
+        if w is None or w.numel() < n:
+            w = self.ws[(slot, name)] = torch.empty(n, dtype=torch.uint8, device=self.device)
+        return w
+
+    def _mmat(self, out, kil, kjl, indgk_k, nvk, Aeigk, indgk_q, nvq, Aeigq, nst, nend, mst, mend, bk, bq, slot=0):
         """mmat_mt and mmat_it of bands nst..nend-1 at k and mst..mend-1 at
         k - q into out [nend-nst, mend-mst, mbsize] (pygw.py:2655-2663); bk /
-        bq hold the band coefficients of exactly those bands."""
+        bq hold the band coefficients of exactly those bands; slot names the
+        workspaces (one set per stream)."""
         umk = np.rint(np.asarray(kil, dtype=np.float64) - np.asarray(kjl, dtype=np.float64) - self.ql).astype(np.int64)
         n1, n2 = nend - nst, mend - mst
-        n = C.minm_mt_workspace(self.mt_plan, n1, n2)
-        if self.ws_mt is None or self.ws_mt.numel() < n:
-            self.ws_mt = torch.empty(n, dtype=torch.uint8, device=self.device)
+        ws = self._workspace(slot, "mt", C.minm_mt_workspace(self.mt_plan, n1, n2))
         C.calc_minm_mt(self.mt_plan, self.ql_dev, 0, n1, 0, n2, *bk, *bq, self.consts, None, None, out=out, col0=0,
-                       workspace=self.ws_mt)
-        n = C.minm_is_workspace(self.ngq, self.mpwipw.shape[1], nvk, nvq, n1, n2)
-        if self.ws_is is None or self.ws_is.numel() < n:
-            self.ws_is = torch.empty(n, dtype=torch.uint8, device=self.device)
+                       workspace=ws)
+        ws = self._workspace(slot, "is", C.minm_is_workspace(self.ngq, self.mpwipw.shape[1], nvk, nvq, n1, n2))
         C.calc_minm_is(nst, nend, mst, mend, Aeigk, Aeigq, umk, self.mpwipw, nvk, nvq, indgk_k, indgk_q,
                        self.indgq_inverse, self.gindex, self.i_g0, self.vol, out=out, col0=self.loct,
-                       workspace=self.ws_is, device=self.device)
+                       workspace=ws, device=self.device)
 
     def _bands(self, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, rows_k, rows_q):
         """The eigenvectors on the device and the band coefficients of the
@@ -164,27 +167,44 @@ class MatrixElements:
         return self._vhalf(self.mmat_buf, precision, complex64, out)
 
     def polarization_minm_many(self, ks, nvbm, ncbm, nbmaxpol, ncg_p, *, out=None, complex64=False,
-                               precision="fp64"):
+                               precision="fp64", streams=4):
         """polarization_minm for several k-points at once: ks a list of
         (kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq); returns [len(ks),
         ncg_p + nvbm, nbmaxpol - ncbm, matsize].  The mmat of every k is built
-        first and the product with conj(barcvm) runs once over all their rows
-        (a launch that fills the chip where one k's rows would not)."""
+        first, the k-points dealt round robin to `streams` side streams (their
+        launches of 20-120 bands each leave most of the chip idle alone), and
+        the product with conj(barcvm) runs once over all their rows on the
+        caller's stream, after every side stream's work."""
         shape = (len(ks), ncg_p + nvbm, nbmaxpol - ncbm, self.mbsize)
         if self.mmat_many is None or tuple(self.mmat_many.shape) != shape:
             self.mmat_many = torch.empty(shape, dtype=torch.complex128, device=self.device)
+        ns = max(1, min(int(streams), len(ks)))
+        while len(self.streams) < ns:
+            self.streams.append(torch.cuda.Stream(device=self.device))      # default priority
+        cur = torch.cuda.current_stream(self.device)
+        start = torch.cuda.Event()
+        start.record(cur)
+        for i in range(ns):
+            self.streams[i].wait_event(start)
         for j, k in enumerate(ks):
-            self._pol_mmat(self.mmat_many[j], *k, nvbm, ncbm, nbmaxpol, ncg_p)
+            with torch.cuda.stream(self.streams[j % ns]):
+                self._pol_mmat(self.mmat_many[j], *k, nvbm, ncbm, nbmaxpol, ncg_p, slot=j % ns)
+        for i in range(ns):
+            done = torch.cuda.Event()
+            done.record(self.streams[i])
+            cur.wait_event(done)
         return self._vhalf(self.mmat_many, precision, complex64, out)
 
-    def _pol_mmat(self, mmat, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, nvbm, ncbm, nbmaxpol, ncg_p):
+    def _pol_mmat(self, mmat, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, nvbm, ncbm, nbmaxpol, ncg_p,
+                  slot=0):
         """calc_eps's mmat of one k into mmat [ncg_p + nvbm, nbmaxpol - ncbm,
         mbsize]: the core rows (MT columns only, the rest zero), then the
         occupied bands."""
         n2 = nbmaxpol - ncbm
         A_k, A_q, bk, bq = self._bands(kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, (0, nvbm),
                                        (ncbm, nbmaxpol))
-        self._mmat(mmat[ncg_p:], kil, kjl, indgk_k, nvk, A_k, indgk_q, nvq, A_q, 0, nvbm, ncbm, nbmaxpol, bk, bq)
+        self._mmat(mmat[ncg_p:], kil, kjl, indgk_k, nvk, A_k, indgk_q, nvq, A_q, 0, nvbm, ncbm, nbmaxpol, bk, bq,
+                   slot)
         if ncg_p > 0:
             core = mmat[:ncg_p]
             core[..., self.loct:].zero_()
