@@ -1420,6 +1420,39 @@ def run_minm(args):
                         "unit": "TFLOP/s", "frac": f_vh * nk / t / PEAK_FP16, "traffic": None,
                         "note": "the v^1/2 product's algorithmic FLOP over the whole call time: a lower bound of "
                                 "its rate"}}
+    if not args.no_cpu:
+        from oracle import coulomb as OCo
+        from oracle import lapw as OL
+        from oracle import minm_mt as OM
+        from oracle import ref
+        import dataclasses
+        Bh = cb.barcvm.cpu().numpy()
+        st_ = {}
+        t0 = time.perf_counter()
+        lk = dataclasses.replace(sk, indgk=rows(kil).astype(np.int64), nvk=nvk)
+        lq = dataclasses.replace(sk, kl=kjl, indgk=rows(kjl).astype(np.int64), nvk=nvk)
+        Akh, Aqh = sk.Aeig, np.conj(sq.Aeig[:, :nvk + nlo_rows])
+        wk = OL.band_coefficients(Akh[:nvbm], *ref.gap2_set_lapwcoef(*lk.coef_args(1, True)))
+        wq = OL.band_coefficients(Aqh[ncbm:nb], *ref.gap2_set_lapwcoef(*lq.coef_args(2, True)))
+        t1 = time.perf_counter()
+        rest = (s.s3r, s.vpos, s.mult, s.nmix, s.big_l, s.nLO_at, s.ncore, s.cgcoef, s.lmax, s.loctmatsize)
+        mt = OM.calc_minm_mt(ql, 0, nvbm, 0, nb - ncbm, *wk, *wq, *rest)
+        mc = OM.calc_minc2(kjl, 0, ncg, 0, nb - ncbm, *wq, s.s3r, s.corind, *rest[1:])
+        t2 = time.perf_counter()
+        it = OCo.minm_is(0, nvbm, ncbm, nb, Akh, Aqh, np.zeros(3, dtype=int), mpwipw, nvk, nvk, rows(kil),
+                         rows(kjl), inv, g, i_g0, sk.vol)
+        t3 = time.perf_counter()
+        mm_h = np.concatenate((np.concatenate((mc, np.zeros(mc.shape[:2] + (ngq,), complex)), -1),
+                               np.concatenate((mt, it), -1)), 0)
+        OCo.minm(mm_h, Bh)
+        t4 = time.perf_counter()
+        st_ = {"lapw_coefficients_and_band_products": (t1 - t0) * nk, "calc_minm_mt_and_minc2": (t2 - t1) * nk,
+               "calc_minm_is": (t3 - t2) * nk, "v_half_product": (t4 - t3) * nk}
+        rec["cpu_baseline"] = {"value": (t4 - t0) * nk, "unit": "s/q", "cores": int(os.environ.get(
+            "OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port", **host_info(), "stages_s": st_,
+            "sample": f"1 of {nk} k, x{nk}: PyGW's compiled gap2_set_lapwcoef (1 core) and numpy products, the "
+                      f"numpy restatements of calc_minm_MT / calc_minc2 / calc_minm_IS, numpy matmul with "
+                      f"conj(barcvm) (FP64)"}
     print(json.dumps(rec))
 
 
