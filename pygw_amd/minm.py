@@ -135,19 +135,20 @@ class MatrixElements:
             minc = C.calc_minm_vhalf(mc, self.barcvm, self.loct)
         return minm, minc
 
-    def _vhalf(self, mmat, precision, complex64, out):
+    def _vhalf(self, mmat, precision, complex64, out, n_in=None):
         if precision == "fp64":
             if out is not None:
-                return C.calc_minm_vhalf(mmat, self.barcvm, complex64=complex64, out=out)
-            return C.calc_minm_vhalf(mmat, self.barcvm, complex64=complex64)
+                return C.calc_minm_vhalf(mmat, self.barcvm, n_in, complex64=complex64, out=out)
+            return C.calc_minm_vhalf(mmat, self.barcvm, n_in, complex64=complex64)
         if precision != "mixed":
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if self.split_basis is None:
             self.split_basis = C.SplitBasis(self.barcvm)
-        n = C.minm_vhalf_mixed_workspace(mmat.numel() // mmat.shape[-1], mmat.shape[-1])
+        n = C.minm_vhalf_mixed_workspace(mmat.numel() // mmat.shape[-1], n_in or mmat.shape[-1])
         if self.ws_vh is None or self.ws_vh.numel() < n:
             self.ws_vh = torch.empty(n, dtype=torch.uint8, device=self.device)
-        return C.calc_minm_vhalf_mixed(mmat, self.split_basis, complex64=complex64, out=out, workspace=self.ws_vh)
+        return C.calc_minm_vhalf_mixed(mmat, self.split_basis, n_in, complex64=complex64, out=out,
+                                       workspace=self.ws_vh)
 
     def polarization_minm(self, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, nvbm, ncbm, nbmaxpol, ncg_p,
                           *, out=None, complex64=False, precision="fp64"):
@@ -211,10 +212,20 @@ class MatrixElements:
             C.calc_minc2(self.mt_plan, self._kd(kjl), 0, ncg_p, 0, n2, *bq, self.consts, None, None, None, out=core,
                          col0=0)
 
-    def selfe_minm(self, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, ibgw, nbgw, nbands, ncg_c):
+    def selfe_minm(self, kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, ibgw, nbgw, nbands, ncg_c, *,
+                   complex64=False, precision="fp64"):
         """The minm that calc_selfc contracts (pygw.py:3339-3345): band_limits
         (ibgw, nbgw, 0, nbands, 0, ncg_c) and the core columns (calc_minc)
-        before the valence ones, [nbgw - ibgw, ncg_c + nbands, matsize]."""
-        minm, minc = self.calc_minm(kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq,
-                                    (ibgw, nbgw, 0, nbands, 0, ncg_c), "selfe")
-        return minm if minc is None else torch.cat((minc, minm), dim=1)
+        before the valence ones, [nbgw - ibgw, ncg_c + nbands, matsize];
+        precision "mixed" as in polarization_minm (the operand of the
+        split-precision MWM)."""
+        nst, nend = int(ibgw), int(nbgw)
+        A_k, A_q, bk, bq = self._bands(kil, indgk_k, nvk, Aeigk, kjl, indgk_q, nvq, Aeigq, (nst, nend), (0, nbands))
+        mmat = torch.empty((nend - nst, nbands, self.mbsize), dtype=torch.complex128, device=self.device)
+        self._mmat(mmat, kil, kjl, indgk_k, nvk, A_k, indgk_q, nvq, A_q, nst, nend, 0, nbands, bk, bq)
+        minm = self._vhalf(mmat, precision, complex64, None)
+        if ncg_c <= 0:
+            return minm
+        mc = C.calc_minc(self.mt_plan, self._kd(kil), 0, nend - nst, 0, ncg_c, *bk, self.consts, None, None, None)
+        minc = self._vhalf(mc, precision, complex64, None, self.loct)
+        return torch.cat((minc, minm), dim=1)

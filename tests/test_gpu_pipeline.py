@@ -179,8 +179,9 @@ def test_pipeline_through_the_q_loop_vs_oracle():
 
 
 def test_pipeline_split_precision_vs_oracle():
-    """The same chain with the product with conj(barcvm) and the q loop in
-    split precision (minm handed over in complex64) against the FP64 oracle
+    """The same chain with the products with conj(barcvm) (calc_eps's and
+    calc_selfc's minm) and the q loop in split precision (minm handed over in
+    complex64) against the FP64 oracle
     composition: chi0 and eps^-1 within 1e-6 per frequency, Sigma_c within
     1e-5 (the split-precision loop's own bars)."""
     import torch
@@ -200,7 +201,8 @@ def test_pipeline_split_precision_vs_oracle():
                         omega_p=fm.omega_precise, womeg_p=fm.womeg_precise, precision="mixed")
     M = torch.stack([me.polarization_minm(*_k(s, ik), s.nomx + 1, s.numin, nb, ncg, complex64=True,
                                           precision="mixed").reshape(-1, pb) for ik in range(nk)])
-    Ms = torch.stack([_gpu_selfe(s, me, ik).reshape(-1, pb) for ik in KS]).to(torch.complex64)
+    Ms = torch.stack([me.selfe_minm(*_k(s, ik), s.nomx, s.nomx + NB1, nb, ncg, complex64=True,
+                                    precision="mixed").reshape(-1, pb) for ik in KS])
     enq = np.ascontiguousarray(enk[:, kq[list(KS)]].T)
     L.add_q(kq, M, Ms, enq, keep_chi0=True)
     kn = L.kcw.cpu().numpy()
