@@ -148,19 +148,34 @@ __global__ void __launch_bounds__(256) pygw_ei_panels(const double2* __restrict_
   const double2* b = B + w * NB * NB;
   const int j0 = blockIdx.x * 64;
   for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) Bs[e % nb][e / nb] = b[(e % nb) + NB * (e / nb)];
-  for (int e = threadIdx.x; e < NB * 64; e += blockDim.x) {
-    const int c = e % NB, jj = e / NB;  // consecutive threads: consecutive rows of one column
-    const int j = j0 + jj;
-    double2 v{0.0, 0.0};
-    if (c < nb && j < n) {
-      if (j >= p0) {
-        v = a[(p0 + c) + (size_t)n * j];
-      } else {  // A[P, j] of a swept j: -conj(A[j, P]) (stored above the diagonal)
-        const double2 t = a[j + (size_t)n * (p0 + c)];
+  if (j0 + 64 <= p0) {
+    // every j of the tile swept: A[P, j] = -conj(A[j, P]) (stored above the
+    // diagonal), read down the columns p0 + c (consecutive threads:
+    // consecutive rows j)
+    for (int e = threadIdx.x; e < NB * 64; e += blockDim.x) {
+      const int jj = e % 64, c = e / 64;
+      double2 v{0.0, 0.0};
+      if (c < nb) {
+        const double2 t = a[(j0 + jj) + (size_t)n * (p0 + c)];
         v = double2{-t.x, t.y};
       }
+      Cs[c][jj] = v;
     }
-    Cs[c][jj] = v;
+  } else {
+    for (int e = threadIdx.x; e < NB * 64; e += blockDim.x) {
+      const int c = e % NB, jj = e / NB;  // consecutive threads: consecutive rows of one column
+      const int j = j0 + jj;
+      double2 v{0.0, 0.0};
+      if (c < nb && j < n) {
+        if (j >= p0) {
+          v = a[(p0 + c) + (size_t)n * j];
+        } else {  // A[P, j] of a swept j: -conj(A[j, P]) (stored above the diagonal)
+          const double2 t = a[j + (size_t)n * (p0 + c)];
+          v = double2{-t.x, t.y};
+        }
+      }
+      Cs[c][jj] = v;
+    }
   }
   __syncthreads();
   const int jj = threadIdx.x & 63, rg = threadIdx.x >> 6;
@@ -202,17 +217,28 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) pygw_ei_update_mfma(double2* __restrict__ A, int n, int p0, int nb,
                                                            const double2* __restrict__ B,
                                                            const double2* __restrict__ U,
-                                                           const double2* __restrict__ L) {
+                                                           const double2* __restrict__ L, int nom, int ntu,
+                                                           long per) {
   using namespace ei;
   __shared__ double Lr[KH][64 + 1], Li[KH][64 + 1];  // [k - k0][i]
   __shared__ double Ur[KH][64 + 1], Ui[KH][64 + 1];  // [k - k0][j]
-  const size_t w = blockIdx.z;
+  // work items (frequency, upper tile), frequency-major, dealt to the XCDs in
+  // contiguous runs: the blocks of one frequency run on one XCD one after
+  // another, so its panels U and L (1.8 MB at n = 900) stay in that XCD's L2
+  // while its ~120 tiles read them
+  const long item = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (item >= (long)nom * ntu) return;
+  const size_t w = (size_t)(item / ntu);
+  const int t = (int)(item % ntu);
+  // upper tile t -> (ti <= tj), the tiles of one column tile together
+  int tj = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((tj + 1) * (tj + 2) / 2 <= t) tj++;
+  while (tj * (tj + 1) / 2 > t) tj--;
+  const int i0 = (t - tj * (tj + 1) / 2) * 64, j0 = tj * 64;
   double2* a = A + w * (size_t)n * n;
   const double2* u = U + w * (size_t)NB * n;
   const double2* l = L + w * (size_t)n * NB;
   const double2* b = B + w * NB * NB;
-  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
-  if (i0 > j0) return;  // strictly lower tile: implied by the upper one
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = wave & 1, wj = wave >> 1;  // wave's 32 x 32: rows i0+32wi.., cols j0+32wj..
   const int li = lane & 15, lk = lane >> 4;
@@ -221,6 +247,20 @@ __global__ void __launch_bounds__(256) pygw_ei_update_mfma(double2* __restrict__
   for (int x = 0; x < 2; x++)
 #pragma unroll
     for (int y = 0; y < 2; y++) { sr[x][y] = d4{0, 0, 0, 0}; si[x][y] = d4{0, 0, 0, 0}; }
+  // the tile's old values, loaded before the product so that their HBM
+  // latency overlaps it (only the entries outside the pivot rows and columns
+  // are read back)
+  double2 old[2][2][4];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = i0 + 32 * wi + 16 * y + li, j = j0 + 32 * wj + 16 * x + 4 * v + lk;
+        const bool keep = i < n && j < n && !(i >= p0 && i < p0 + nb) && !(j >= p0 && j < p0 + nb);
+        old[x][y][v] = keep ? a[i + (size_t)n * j] : double2{0.0, 0.0};
+      }
   for (int k0 = 0; k0 < nb; k0 += KH) {
     if (k0) __syncthreads();  // the previous half is consumed
     for (int e = tid; e < 64 * KH; e += 256) {
@@ -277,7 +317,7 @@ __global__ void __launch_bounds__(256) pygw_ei_update_mfma(double2* __restrict__
         const double xr = sr[x][y][v], xi = si[x][y][v];
         if (ri) *aij = cj ? b[(i - p0) + NB * (j - p0)] : u[(i - p0) + (size_t)NB * j];  // B or U_j
         else if (cj) *aij = double2{-xr, -xi};                     // -L_i B
-        else { const double2 o = *aij; *aij = double2{o.x - xr, o.y - xi}; }  // A - L U
+        else { const double2 o = old[x][y][v]; *aij = double2{o.x - xr, o.y - xi}; }  // A - L U
       }
 }
 
@@ -311,12 +351,14 @@ int eps_inverse_gj(void* eps, int n, int nom, void* ws, int* dinfo, hipStream_t 
   double2* U = B + (size_t)nom * NB * NB;
   double2* L = U + (size_t)nom * NB * n;
   hipLaunchKernelGGL(pygw_ei_hermitize, dim3(ceil_div(n, 32), ceil_div(n, 32), nom), dim3(256), 0, stream, A, n);
+  const int T = ceil_div(n, 64), ntu = T * (T + 1) / 2;   // 64 x 64 tiles on and above the diagonal
+  const long per = ((long)nom * ntu + 7) / 8;
   for (int p0 = 0; p0 < n; p0 += NB) {
     const int nb = (n - p0) < NB ? (n - p0) : NB;
     hipLaunchKernelGGL(pygw_ei_pivot, dim3(nom), dim3(1024), 0, stream, A, n, p0, nb, B, dinfo);
     hipLaunchKernelGGL(pygw_ei_panels, dim3(ceil_div(n, 64), nom), dim3(256), 0, stream, A, n, p0, nb, B, U, L);
-    hipLaunchKernelGGL(pygw_ei_update_mfma, dim3(ceil_div(n, 64), ceil_div(n, 64), nom), dim3(256), 0, stream, A, n,
-                       p0, nb, B, U, L);
+    hipLaunchKernelGGL(pygw_ei_update_mfma, dim3((unsigned)(8 * per)), dim3(256), 0, stream, A, n, p0, nb, B, U, L,
+                       nom, ntu, per);
   }
   hipLaunchKernelGGL(pygw_ei_mirror, dim3(ceil_div(n, 32), ceil_div(n, 32), nom), dim3(256), 0, stream, A, n);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
