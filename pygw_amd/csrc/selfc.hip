@@ -340,9 +340,134 @@ int selfc_spline_launch(const void* mwm, int nrows, int nb1, int nom, int nb2, c
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }
 
+// The spline branch's convolution (mwmp given) with the kernel of each
+// (e, i w_n, w_l) shared by all rows of a k-point: one block per (k, group
+// of SG frequencies), one wave per row ie1 (rows in groups of SR, the block
+// looping over the groups), the precise mesh in tiles of SL points.  Per
+// (ie2, tile) the block first writes
+//   f[g][l] = w_l (e - i w_n) conj(den) / (|den|^2 pi),  den = w_l^2 + (e - i w_n)^2,
+// into LDS (the factor of fr_convolution2 without the numerator, fnc.f90:
+// 100-115), then each row's lanes add (mwmp[l] - mwm[w_n]) f[g][l] over
+// their points.  The terms are those of the reference's sum, each scaled by
+// (e - i w_n)/pi before the l sum rather than after it.
+namespace sc {
+constexpr int SG = 16;   // Matsubara points per block
+constexpr int SR = 8;    // rows (waves) per pass
+constexpr int SL = 256;  // precise-mesh points per LDS tile
+}  // namespace sc
+
+__global__ void __launch_bounds__(512) pygw_selfc_conv_k(const double* __restrict__ enk, int nb1, int nb2,
+                                                        const double2* __restrict__ mwm, int nom,
+                                                        const double* __restrict__ omega,
+                                                        const double2* __restrict__ mwmp, int nomp,
+                                                        const double* __restrict__ omp,
+                                                        const double* __restrict__ womp,
+                                                        double2* __restrict__ out) {
+  using namespace sc;
+  __shared__ double2 f[SG][SL];
+  const int k = blockIdx.x, i0 = blockIdx.y * SG;
+  const int ng = nom - i0 < SG ? nom - i0 : SG;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int nt = blockDim.x, nw = nt >> 6;
+  const double wmax = omp[nomp - 1];
+  for (int r0 = 0; r0 < nb1; r0 += nw) {
+    const int ie1 = r0 + wv;
+    const bool live = ie1 < nb1;
+    const size_t row = (size_t)k * nb1 + (live ? ie1 : 0);
+    double2 acc[SG];
+#pragma unroll
+    for (int g = 0; g < SG; g++) acc[g] = double2{0.0, 0.0};
+    for (int ie2 = 0; ie2 < nb2; ie2++) {
+      const double e = enk[(size_t)k * nb2 + ie2];
+      double2 mn[SG];
+#pragma unroll
+      for (int g = 0; g < SG; g++)
+        mn[g] = (live && g < ng) ? mwm[(((size_t)k * nom + i0 + g) * nb1 + ie1) * nb2 + ie2] : double2{0.0, 0.0};
+      const double2* ml_row = mwmp + (row * nb2 + ie2) * nomp;
+      for (int l0 = 0; l0 < nomp; l0 += SL) {
+        // this lane's points of the tile, loaded first so that their latency
+        // overlaps the kernel factors below
+        double2 mlv[SL / 64];
+#pragma unroll
+        for (int q = 0; q < SL / 64; q++) {
+          const int l = l0 + lane + 64 * q;
+          mlv[q] = (live && l < nomp) ? ml_row[l] : double2{0.0, 0.0};
+        }
+        __syncthreads();  // the previous tile is consumed
+        for (int x = t; x < SG * SL; x += nt) {
+          const int g = x / SL, ll = x % SL, l = l0 + ll;
+          double2 v = double2{0.0, 0.0};
+          if (g < ng && l < nomp) {
+            const double wn = omega[i0 + g];
+            const double o2 = omp[l] * omp[l];
+            const double2 den = double2{o2 + (e * e - wn * wn), -2.0 * e * wn};
+            const double d2 = den.x * den.x + den.y * den.y;
+            double rr = __builtin_amdgcn_rcp(d2);
+            rr = __builtin_fma(rr, __builtin_fma(-d2, rr, 1.0), rr);
+            rr = __builtin_fma(rr, __builtin_fma(-d2, rr, 1.0), rr);
+            // conj(den) / |den|^2 times w_l (e - i w_n) / pi
+            const double s = womp[l] * rr / PI_F;
+            const double2 cd = double2{den.x * s, -den.y * s};
+            v = cmul(double2{e, -wn}, cd);
+          }
+          f[g][ll] = v;
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+          for (int q = 0; q < SL / 64; q++) {
+            const int ll = lane + 64 * q;
+            if (l0 + ll >= nomp) break;
+            const double2 ml = mlv[q];
+#pragma unroll
+            for (int g = 0; g < SG; g++) {
+              const double2 num = double2{ml.x - mn[g].x, ml.y - mn[g].y};
+              acc[g] = cadd(acc[g], cmul(num, f[g][ll]));
+            }
+          }
+        }
+      }
+    }
+    if (live) {
+      // the atan tail once per (ie2, iom), the ie2 dealt over the row's lanes
+      for (int ie2 = lane; ie2 < nb2; ie2 += 64) {
+        const double e = enk[(size_t)k * nb2 + ie2];
+#pragma unroll
+        for (int g = 0; g < SG; g++) {
+          if (g < ng) {
+            const double2 mn = mwm[(((size_t)k * nom + i0 + g) * nb1 + ie1) * nb2 + ie2];
+            if (hypot(mn.x, mn.y) > 1e-16) {
+              const double2 eo = double2{e, -omega[i0 + g]};
+              const double2 at = catan_(cdiv(double2{wmax, 0.0}, eo));
+              acc[g] = cadd(acc[g], cmul(double2{mn.x / PI_F, mn.y / PI_F}, at));
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < SG; g++) {
+        double x = acc[g].x, y = acc[g].y;
+        for (int off = 32; off > 0; off >>= 1) {
+          x += __shfl_xor(x, off, 64);
+          y += __shfl_xor(y, off, 64);
+        }
+        if (lane == 0 && g < ng) out[row * nom + i0 + g] = double2{x, y};
+      }
+    }
+  }
+}
+
 int selfc_conv_launch(const double* enk, int nrows, int nb1, int nb2, const void* mwm, int nom, const double* omega,
                       const void* mwmp, int nomp, const double* omp, const double* womp, void* out,
                       hipStream_t stream) {
+  if (mwmp) {
+    // the spline branch: one block per (k, SG frequencies), a wave per row
+    const int waves = nb1 < sc::SR ? nb1 : sc::SR;
+    hipLaunchKernelGGL(pygw_selfc_conv_k, dim3(nrows / nb1, ceil_div(nom, sc::SG)), dim3(64 * waves), 0, stream, enk,
+                       nb1, nb2, (const double2*)mwm, nom, omega, (const double2*)mwmp, nomp, omp, womp,
+                       (double2*)out);
+    return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
+  }
   dim3 grid(nrows, ceil_div(nom, sc::CG));
   hipLaunchKernelGGL(pygw_selfc_conv, grid, dim3(sc::CT), 0, stream, enk, nb1, nb2, (const double2*)mwm, nom, omega,
                      (const double2*)mwmp, nomp, omp, womp, (double2*)out);
