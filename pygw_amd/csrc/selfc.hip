@@ -4,6 +4,8 @@
 // the Pade continuation plus the quasiparticle equation
 // (AnalyticContinuation iop_ac=0, mcommon.py:315-343; Compute_quasiparticles,
 // mcommon.py:167-210; Padecof/PadeG, for_pade.f90:3-51).
+#include <algorithm>
+
 #include "common.h"
 
 namespace sc {
@@ -37,62 +39,85 @@ __device__ __forceinline__ double2 catan_(double2 z) {
 // y'(x_{n-1})=0 (scipy CubicSpline bc_type=((2,0),(1,0)), extrapolating with
 // the end polynomials), evaluated on the precise mesh and divided by
 // (xp^2+1).  The imaginary part is splined only where sum|Im|/n > 1e-10, as
-// at mcommon.py:134-138.  One thread per (row, ie2), row = (k, ie1): mwm is
-// PyGW's per-k mwm[iom, ie1, ie2] stacked over k, [N_k][nom][nb1][nb2];
-// out is [nrows][nb2][nomp].
-__global__ void __launch_bounds__(64) pygw_selfc_spline(const double2* __restrict__ mwm, int nrows, int nb1, int nom,
-                                                        int nb2,
-                                                        const double* __restrict__ x, const double* __restrict__ xp,
-                                                        int nomp, double2* __restrict__ out) {
-  using namespace sc;
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= (long)nrows * nb2) return;
-  const int row = (int)(t / nb2), ie2 = (int)(t % nb2);
-  const int kq = row / nb1, ie1 = row % nb1;
+// at mcommon.py:134-138.  Row = (k, ie1): mwm is PyGW's per-k mwm[iom, ie1,
+// ie2] stacked over k, [N_k][nom][nb1][nb2]; out is [nrows][nb2][nomp].
+// One block per (row, ie2) pair: the pair's coarse values into LDS, the
+// two Thomas sweeps for the slopes (real, imaginary part) by two threads,
+// then every thread evaluates precise points j = t, t + 128, ... so that the
+// block writes its nomp values contiguously (one thread per pair wrote 64
+// rows nomp*16 bytes apart at each store: 1.66 ms against 0.4 ms at C5).
+__global__ void __launch_bounds__(128) pygw_selfc_spline_b(const double2* __restrict__ mwm, int nrows, int nb1,
+                                                          int nom, int nb2, const double* __restrict__ x,
+                                                          const double* __restrict__ xp, int nomp,
+                                                          double2* __restrict__ out) {
   constexpr int NMAX = 128;
-  double sr[NMAX], si[NMAX], cp[NMAX];
-  const int n = nom;
-  double imsum = 0.0;
-  for (int i = 0; i < n; i++) imsum += fabs(mwm[(((size_t)kq * nom + i) * nb1 + ie1) * nb2 + ie2].y);
-  const bool doim = imsum / n > 1e-10;
-  // slopes s: tridiagonal system (Thomas); rhs for both parts
-  auto yv = [&](int i, int part) {
+  __shared__ double yr[NMAX], yi[NMAX], sr[NMAX], si[NMAX], xs[NMAX], cpr[NMAX], cpi[NMAX];
+  __shared__ double c2r[NMAX], c3r[NMAX], c2i[NMAX], c3i[NMAX];
+  __shared__ double red[2];
+  const long t0 = blockIdx.x;   // (row, ie2)
+  const int row = (int)(t0 / nb2), ie2 = (int)(t0 % nb2);
+  const int kq = row / nb1, ie1 = row % nb1;
+  const int t = threadIdx.x, n = nom;
+  double im = 0.0;
+  for (int i = t; i < n; i += blockDim.x) {
     const double2 v = mwm[(((size_t)kq * nom + i) * nb1 + ie1) * nb2 + ie2];
-    return (x[i] * x[i] + 1.0) * (part ? v.y : v.x);
-  };
-  for (int part = 0; part < (doim ? 2 : 1); part++) {
-    double* s = part ? si : sr;
-    // row 0: 2 s0 + s1 = 3 m0 ; rows 1..n-2 ; row n-1: s = 0
-    double b0 = 2.0, c0 = 1.0;
-    double d0 = 3.0 * (yv(1, part) - yv(0, part)) / (x[1] - x[0]);
+    const double f = x[i] * x[i] + 1.0;
+    yr[i] = f * v.x;
+    yi[i] = f * v.y;
+    xs[i] = x[i];
+    im += fabs(v.y);
+  }
+  // sum |Im| over the coarse points (the order of the one-thread form's sum
+  // does not matter for the > 1e-10 test up to rounding)
+  for (int off = 32; off > 0; off >>= 1) im += __shfl_xor(im, off, 64);
+  if ((t & 63) == 0) red[t >> 6] = im;
+  __syncthreads();
+  const bool doim = (red[0] + red[1]) / n > 1e-10;
+  if (t < 2 && (t == 0 || doim)) {
+    const double* y = t ? yi : yr;
+    double* sl = t ? si : sr;
+    double* cp = t ? cpi : cpr;     // in LDS: a private array indexed at run time would live in scratch
+    const double b0 = 2.0, c0 = 1.0;
+    const double d0 = 3.0 * (y[1] - y[0]) / (xs[1] - xs[0]);
     cp[0] = c0 / b0;
-    s[0] = d0 / b0;
+    sl[0] = d0 / b0;
     for (int i = 1; i < n - 1; i++) {
-      const double h0 = x[i] - x[i - 1], h1 = x[i + 1] - x[i];
-      const double m0 = (yv(i, part) - yv(i - 1, part)) / h0, m1 = (yv(i + 1, part) - yv(i, part)) / h1;
+      const double h0 = xs[i] - xs[i - 1], h1 = xs[i + 1] - xs[i];
+      const double m0 = (y[i] - y[i - 1]) / h0, m1 = (y[i + 1] - y[i]) / h1;
       const double a = h1, b = 2.0 * (h0 + h1), c = h0, d = 3.0 * (h1 * m0 + h0 * m1);
       const double den = b - a * cp[i - 1];
       cp[i] = c / den;
-      s[i] = (d - a * s[i - 1]) / den;
+      sl[i] = (d - a * sl[i - 1]) / den;
     }
-    s[n - 1] = 0.0;
-    for (int i = n - 2; i >= 0; i--) s[i] = s[i] - cp[i] * s[i + 1];
+    sl[n - 1] = 0.0;
+    for (int i = n - 2; i >= 0; i--) sl[i] = sl[i] - cp[i] * sl[i + 1];
   }
-  double2* o = out + ((size_t)row * nb2 + ie2) * nomp;
-  int seg = 0;
-  for (int j = 0; j < nomp; j++) {
+  __syncthreads();
+  // the cubic of every coarse interval, once per pair
+  for (int x2 = t; x2 < 2 * (n - 1); x2 += blockDim.x) {
+    const int part = x2 / (n - 1), seg = x2 % (n - 1);
+    if (part == 1 && !doim) continue;
+    const double* y = part ? yi : yr;
+    const double* sl = part ? si : sr;
+    const double h = xs[seg + 1] - xs[seg];
+    const double m = (y[seg + 1] - y[seg]) / h;
+    (part ? c2i : c2r)[seg] = (3.0 * m - 2.0 * sl[seg] - sl[seg + 1]) / h;
+    (part ? c3i : c3r)[seg] = (sl[seg] + sl[seg + 1] - 2.0 * m) / (h * h);
+  }
+  __syncthreads();
+  double2* o = out + (size_t)t0 * nomp;
+  for (int j = t; j < nomp; j += blockDim.x) {
     const double xv = xp[j];
-    while (seg < n - 2 && xv > x[seg + 1]) seg++;
-    const double h = x[seg + 1] - x[seg], tt = xv - x[seg];
-    double res[2] = {0.0, 0.0};
-    for (int part = 0; part < (doim ? 2 : 1); part++) {
-      const double* s = part ? si : sr;
-      const double y0 = yv(seg, part), y1 = yv(seg + 1, part);
-      const double m = (y1 - y0) / h;
-      const double c2 = (3.0 * m - 2.0 * s[seg] - s[seg + 1]) / h;
-      const double c3 = (s[seg] + s[seg + 1] - 2.0 * m) / (h * h);
-      res[part] = y0 + tt * (s[seg] + tt * (c2 + tt * c3));
+    int lo = 0, hi = n - 2;            // the interval the one-thread form's scan stops at
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (xv > xs[mid + 1]) lo = mid + 1; else hi = mid;
     }
+    const int seg = lo;
+    const double tt = xv - xs[seg];
+    double res[2] = {0.0, 0.0};
+    res[0] = yr[seg] + tt * (sr[seg] + tt * (c2r[seg] + tt * c3r[seg]));
+    if (doim) res[1] = yi[seg] + tt * (si[seg] + tt * (c2i[seg] + tt * c3i[seg]));
     const double f = xv * xv + 1.0;
     o[j] = double2{res[0] / f, res[1] / f};
   }
@@ -335,7 +360,8 @@ int selfc_spline_launch(const void* mwm, int nrows, int nb1, int nom, int nb2, c
                         int nomp, void* out, hipStream_t stream) {
   if (nom > 128 || nom < 3) return PYGW_E_ARG;
   const long n = (long)nrows * nb2;
-  hipLaunchKernelGGL(pygw_selfc_spline, dim3(ceil_div(n, 64)), dim3(64), 0, stream, (const double2*)mwm, nrows, nb1,
+  if (n > 0x7fffffffL) return PYGW_E_ARG;
+  hipLaunchKernelGGL(pygw_selfc_spline_b, dim3((unsigned)n), dim3(128), 0, stream, (const double2*)mwm, nrows, nb1,
                      nom, nb2, x, xp, nomp, (double2*)out);
   return hipGetLastError() == hipSuccess ? PYGW_OK : PYGW_E_HIP;
 }

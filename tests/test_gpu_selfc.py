@@ -65,6 +65,26 @@ def test_spline_convolution_vs_oracle():
         np.testing.assert_allclose(got[k], want, rtol=1e-11, atol=1e-13)
 
 
+def test_spline_convolution_real_mwm_and_many_rows():
+    """The spline + convolution where the imaginary part is not splined (mwm
+    real: PyGW's sum|Im|/n <= 1e-10 rule) and with more rows per k than a
+    block of the convolution has waves (nb1 = 11 > 8), against the oracle."""
+    from pygw_amd import polarization as P
+    rng = np.random.default_rng(9)
+    fm = mesh.FrequencyMesh(12, 0.02, 20.0, iop_multiple=5)
+    om, wo, omp, wop = fm.omega, fm.womeg, fm.omega_precise, fm.womeg_precise
+    nk, nb1, nb2 = 2, 11, 5
+    a = rng.uniform(0.3, 1.5, (nk, nb1, nb2))
+    mwm = (1.0 / (om[None, :, None, None] ** 2 + a[:, None] ** 2)).astype(complex)
+    mwm[1] *= 1 + 0.2j
+    enk = rng.uniform(-0.6, 0.6, (nk, nb2))
+    print("case spline conv real / 11 rows", flush=True)
+    got = P.compute_selfc_inside(enk, mwm, om, wo, omp, wop).cpu().numpy()
+    for k in range(nk):
+        want = chain.selfc_spline(enk[k], mwm[k], om, omp, wop)
+        np.testing.assert_allclose(got[k], want, rtol=1e-11, atol=1e-13)
+
+
 def test_pade_qp_vs_reference_fixture():
     from pygw_amd import polarization as P
     g = np.load(os.path.join(GOLD, "pade.npz"))
