@@ -151,14 +151,15 @@ def omega_basis(kcw_native, *, balance=True, device="cuda", workspace=None):
     return Pm, Em, sv
 
 
-def omega_rank(sv, tol=1e-7, margin=2):
+def omega_rank(sv, tol=1e-7, margin=1):
     """Basis size for the compressed contraction: the singular values above
     tol * sv[0] plus `margin` more (at most n_om).  The callers with
     omega_rank="auto" contract frequency by frequency instead when this is
     more than 3/4 of n_om (COMPRESS_MAX_FRACTION: too little to gain).  On the C5 weights nine
     singular values lie above 1e-7 of the first, the rest on a plateau near
     1.5e-8; the relative chi0 change is 1e-8 at rank 8 and 5e-10 at ranks
-    10-12 (FP64, every frequency)."""
+    10-12 (FP64, every frequency): one function past the threshold (rank
+    10) is as accurate as more, each more costs 1/rank of the contraction."""
     s = sv.detach().cpu().numpy() if isinstance(sv, torch.Tensor) else np.asarray(sv)
     if s.size == 0 or not s[0] > 0:
         return 1
