@@ -443,6 +443,9 @@ def run_chain(args):
     import torch
 
     from pygw_amd import mesh
+    import ctypes as C
+
+    from pygw_amd import _lib
     from pygw_amd import polarization as P
 
     wl = ChainWorkload(args.nq)
@@ -506,16 +509,24 @@ def run_chain(args):
     Lmx.reset()
     s = torch.cuda.current_stream()
     times, walls = [], []
+    kern_ms, kern_n = 0.0, 0         # the contraction kernel alone, HIP events around each launch
     for it in range(args.warmup + args.steps):
         kq, M, Ms, eq = wl.q_inputs(wl.qlist[it % nql], Mbuf, Msbuf)   # not timed: the q's M is produced
         Lmx.set_q(kq, enk_kq=eq)
         torch.cuda.synchronize()
+        if it >= args.warmup and ok:
+            _lib.call("pygw_chi0_launch_timer", 64)
         e = ev()
         tw = time.perf_counter()
         Lmx.run_q(e, M=M, M_selfe=Ms)
         Lmx.accumulate()
         torch.cuda.synchronize()
         wall = time.perf_counter() - tw
+        if it >= args.warmup and ok:
+            tms, tn = C.c_double(0.0), C.c_int(0)
+            _lib.call("pygw_chi0_launch_time", C.byref(tms), C.byref(tn))
+            _lib.call("pygw_chi0_launch_timer", 0)
+            kern_ms, kern_n = kern_ms + tms.value, kern_n + tn.value
         if it >= args.warmup:
             times.append([e[i].elapsed_time(e[i + 1]) / 1e3 for i in range(5)])
             walls.append(wall)
@@ -554,6 +565,7 @@ def run_chain(args):
     t_chi = st_mx["chi0"] if ok else st_64["chi0"]
     kern = "pygw_chi0_mixed" if ok else "pygw_chi0_f64"
     nl = chi0_launches(pb, no * ne, int(round(rmean)) if ok else nom, nk) if ok else 1
+    t_kern = kern_ms / 1e3 / kern_n if kern_n else t_chi / nl
     peak = PEAK_FP16 if ok else PEAK_FP64
     # weights: HBM roofline on the algorithmic bytes (kcw written once, enk
     # and the index arrays read once) and the FP64-VALU count per (small
@@ -582,9 +594,12 @@ def run_chain(args):
                                            "ranks_precision_pass": ranks["w_precision_pass"],
                                            "ranks_timed": ranks["w_timed"], "n_om": nom}},
         "stages_s": st_mx if ok else st_64,
-        "roofline": {"bound": "mfma", "kernel": kern, "achieved": f_exec / t_chi / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": f_exec / t_chi / peak, "traffic": load_traffic("c5", kern),
-                     "launches_per_q": nl, "flops_per_launch": f_exec / nl, "time_per_launch_s": t_chi / nl,
+        "roofline": {"bound": "mfma", "kernel": kern, "achieved": f_exec / nl / t_kern / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": f_exec / nl / t_kern / peak, "traffic": load_traffic("c5", kern),
+                     "launches_per_q": nl, "flops_per_launch": f_exec / nl, "time_per_launch_s": t_kern,
+                     "time_per_launch_source": ("HIP events around each launch of the kernel in the timed steps "
+                                                f"({kern_n} launches)" if kern_n else "chi0 stage time / launches"),
+                     "chi0_stage_time_per_launch_s": t_chi / nl,
                      "executed_flops_per_launch": (3 * f_exec if ok else f_exec) / nl,
                      "basis_functions": rmean if ok else nom,
                      "frequency_by_frequency_equivalent_tflops": f_chi / t_chi / 1e12},

@@ -95,6 +95,14 @@ int pygw_tetra_weights(const double* enk, double ef, const int* kqid, const int*
  *         part slots it runs the last round unsplit.
  * ------------------------------------------------------------------------ */
 int pygw_chi0_workspace(int pb, int n_pairs, int n_om, int n_k, int precision, size_t* bytes);
+/* pygw_chi0_launch_timer / pygw_chi0_launch_time: measurement only, no
+ *   counterpart in the reference.  Arms HIP events around the next
+ *   max_launches launches of the split-precision contraction kernel (0
+ *   disarms); pygw_chi0_launch_time synchronises them and returns the summed
+ *   kernel time (ms) and the launch count since the last read.  Not for
+ *   work under HIP graph capture. */
+int pygw_chi0_launch_timer(int max_launches);
+int pygw_chi0_launch_time(double* ms, int* launches);
 int pygw_chi0_accumulate(const void* M, long stride_k, int n_pairs, int pb, const double* kcw, int n_om, int k0,
                          int n_k, double coef, int precision, int beta, void* eps, void* workspace,
                          size_t ws_bytes, void* stream);

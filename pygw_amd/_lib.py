@@ -109,6 +109,8 @@ _SIGS = {
     "pygw_mixed_coulomb_workspace": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
     "pygw_coul_mixed_block": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_long,
                                         C.c_void_p]),
+    "pygw_chi0_launch_timer": (C.c_int, [C.c_int]),
+    "pygw_chi0_launch_time": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "pygw_minm_vhalf_split_basis_bytes": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
     "pygw_minm_vhalf_split_basis": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_size_t,
                                               C.c_void_p]),

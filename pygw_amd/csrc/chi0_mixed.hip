@@ -707,6 +707,54 @@ size_t chi0_mixed_ws_bytes(int pb, int np, int nom, int nk) {
 // eps accumulates (beta = 1) or is overwritten (beta = 0).  A workspace
 // without room for the part slots (at least head + one k-point) runs the
 // last round unsplit.
+// Optional timing of the contraction kernel alone (pygw_chi0_launch_timer):
+// events recorded on the launch stream around every pygw_chi0_mixed launch,
+// their elapsed times summed when read.  For measurement runs, not for code
+// under HIP graph capture.
+namespace cm {
+struct LaunchTimer {
+  bool on = false;
+  std::vector<hipEvent_t> ev;   // start, stop pairs
+  size_t used = 0;
+};
+static LaunchTimer g_timer;
+}  // namespace cm
+
+extern "C" int pygw_chi0_launch_timer(int max_launches) {
+  using namespace cm;
+  for (hipEvent_t e : g_timer.ev) (void)hipEventDestroy(e);
+  g_timer.ev.clear();
+  g_timer.used = 0;
+  g_timer.on = max_launches > 0;
+  for (int i = 0; i < 2 * max_launches; i++) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) {
+      g_timer.on = false;
+      return PYGW_E_HIP;
+    }
+    g_timer.ev.push_back(e);
+  }
+  return PYGW_OK;
+}
+
+// total kernel time (ms) and launch count since the timer was armed (or last
+// read); synchronises the recorded events
+extern "C" int pygw_chi0_launch_time(double* ms, int* launches) {
+  using namespace cm;
+  if (!ms || !launches) return PYGW_E_ARG;
+  double t = 0.0;
+  for (size_t i = 0; i + 1 < 2 * g_timer.used + 1 && i < g_timer.ev.size(); i += 2) {
+    if (hipEventSynchronize(g_timer.ev[i + 1]) != hipSuccess) return PYGW_E_HIP;
+    float x = 0.f;
+    if (hipEventElapsedTime(&x, g_timer.ev[i], g_timer.ev[i + 1]) != hipSuccess) return PYGW_E_HIP;
+    t += x;
+  }
+  *ms = t;
+  *launches = (int)g_timer.used;
+  g_timer.used = 0;
+  return PYGW_OK;
+}
+
 int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double* kcw, int nom, int k0, int nkc,
                       double coef, int G, int beta, void* ws, size_t ws_bytes, void* eps, hipStream_t stream) {
   using namespace cm;
@@ -763,8 +811,11 @@ int chi0_mixed_launch(const void* M, long strideK, int np, int pb, const double*
     if ((long)sp > (long)n * npc) sp = (int)((long)n * npc);
     if (sp < 2) sp = 1;
     const int grid = sp > 1 ? nfull + rest * sp : nitems;
+    const bool timed = g_timer.on && 2 * (g_timer.used + 1) <= g_timer.ev.size();
+    if (timed) (void)hipEventRecord(g_timer.ev[2 * g_timer.used], stream);
     hipLaunchKernelGGL(pygw_chi0_mixed, dim3(grid), dim3(NT), 0, stream, (const h8*)S, Ms, strideK, np, pb, pbp, npc,
                        n, (const float*)wt, nom, cs, wsc, G, nfull, sp, (double2*)eps, part);
+    if (timed) (void)hipEventRecord(g_timer.ev[2 * g_timer.used++ + 1], stream);
     if (sp > 1)
       hipLaunchKernelGGL(pygw_chi0_tailfix, dim3(TB * TB / 256, rest), dim3(256), 0, stream, (const double2*)part, sp,
                          nfull, pb, nom, (double2*)eps);
