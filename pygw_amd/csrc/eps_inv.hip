@@ -141,13 +141,23 @@ __global__ void __launch_bounds__(256) pygw_ei_panels(const double2* __restrict_
                                                       const double2* __restrict__ B, double2* __restrict__ U,
                                                       double2* __restrict__ L) {
   using namespace ei;
-  __shared__ double2 Bs[NB][NB + 1];
-  __shared__ double2 Cs[NB][64 + 1];  // A[p0 + c][j0 + jj]
+  typedef double d4v __attribute__((ext_vector_type(4)));
+  __shared__ double Br[NB][NB + 1], Bi[NB][NB + 1];  // [c][r] = B[r][c]
+  __shared__ double Cr[NB][64 + 1], Ci[NB][64 + 1];  // [c][jj] = A[p0 + c][j0 + jj]
   const size_t w = blockIdx.y;
   const double2* a = A + w * (size_t)n * n;
   const double2* b = B + w * NB * NB;
   const int j0 = blockIdx.x * 64;
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) Bs[e % nb][e / nb] = b[(e % nb) + NB * (e / nb)];
+  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
+    const int r = e % NB, c = e / NB;     // consecutive threads: consecutive rows of column c of B
+    const double2 v = (r < nb && c < nb) ? b[r + NB * c] : double2{0.0, 0.0};
+    Br[c][r] = v.x;
+    Bi[c][r] = v.y;
+  }
+  auto put = [&](int c, int jj, double2 v) {
+    Cr[c][jj] = v.x;
+    Ci[c][jj] = v.y;
+  };
   if (j0 + 64 <= p0) {
     // every j of the tile swept: A[P, j] = -conj(A[j, P]) (stored above the
     // diagonal), read down the columns p0 + c (consecutive threads:
@@ -159,7 +169,7 @@ __global__ void __launch_bounds__(256) pygw_ei_panels(const double2* __restrict_
         const double2 t = a[(j0 + jj) + (size_t)n * (p0 + c)];
         v = double2{-t.x, t.y};
       }
-      Cs[c][jj] = v;
+      put(c, jj, v);
     }
   } else {
     for (int e = threadIdx.x; e < NB * 64; e += blockDim.x) {
@@ -174,30 +184,62 @@ __global__ void __launch_bounds__(256) pygw_ei_panels(const double2* __restrict_
           v = double2{-t.x, t.y};
         }
       }
-      Cs[c][jj] = v;
+      put(c, jj, v);
     }
   }
   __syncthreads();
+  // U[r][j] = sum_c B[r][c] A[P_c][j] on v_mfma_f64_16x16x4f64, formed as
+  // D[j][r] (D row 4v + lane/16 indexes j, D column lane%16 indexes r), so
+  // each group of 16 lanes writes 16 consecutive rows r of column j of U;
+  // 4 waves of 32 (j) x 32 (r)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wj = wave & 1, wr = wave >> 1, li = lane & 15, lk = lane >> 4;
+  d4v sr[2][2], si[2][2];  // [j tile][r tile]
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) { sr[x][y] = d4v{0, 0, 0, 0}; si[x][y] = d4v{0, 0, 0, 0}; }
+#pragma unroll 4
+  for (int ks = 0; ks < NB / 4; ks++) {
+    const int c = 4 * ks + lk;
+    double cr[2], ci[2], nci[2], br[2], bi[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      cr[t] = Cr[c][32 * wj + 16 * t + li];
+      ci[t] = Ci[c][32 * wj + 16 * t + li];
+      nci[t] = -ci[t];
+      br[t] = Br[c][32 * wr + 16 * t + li];
+      bi[t] = Bi[c][32 * wr + 16 * t + li];
+    }
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 2; y++) {
+        sr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(cr[x], br[y], sr[x][y], 0, 0, 0);
+        sr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(nci[x], bi[y], sr[x][y], 0, 0, 0);
+        si[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(cr[x], bi[y], si[x][y], 0, 0, 0);
+        si[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ci[x], br[y], si[x][y], 0, 0, 0);
+      }
+  }
+  double2* u = U + w * (size_t)NB * n;
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int j = j0 + 32 * wj + 16 * x + 4 * v + lk;
+        const int r = 32 * wr + 16 * y + li;
+        if (j < n && r < nb) u[r + (size_t)NB * j] = double2{sr[x][y][v], si[x][y][v]};
+      }
+  // L: A[j, P] (the rows of the pivot columns), below the pivot block from
+  // the relation conj(A[P, j]) (both unswept)
+  double2* l = L + w * (size_t)n * NB;
   const int jj = threadIdx.x & 63, rg = threadIdx.x >> 6;
   const int j = j0 + jj;
-  if (j < n) {
-    double2* u = U + w * (size_t)NB * n;
-    for (int r = 16 * rg; r < 16 * rg + 16 && r < nb; r++) {
-      double2 s = {0.0, 0.0};
-      for (int c = 0; c < nb; c++) {
-        const double2 t = cmul(Bs[r][c], Cs[c][jj]);
-        s.x += t.x;
-        s.y += t.y;
-      }
-      u[r + (size_t)NB * j] = s;
-    }
-    double2* l = L + w * (size_t)n * NB;
-    for (int c = 16 * rg; c < 16 * rg + 16 && c < nb; c++) {
-      // A[j, P] below the pivot block: conj(A[P, j]) (both unswept)
-      const double2 v = Cs[c][jj];
-      l[j + (size_t)n * c] = j < p0 + nb ? a[j + (size_t)n * (p0 + c)] : double2{v.x, -v.y};
-    }
-  }
+  if (j < n)
+    for (int c = 16 * rg; c < 16 * rg + 16 && c < nb; c++)
+      l[j + (size_t)n * c] = j < p0 + nb ? a[j + (size_t)n * (p0 + c)] : double2{Cr[c][jj], -Ci[c][jj]};
 }
 
 // Step 3: rank-nb update of every 64 x 64 tile.
