@@ -98,31 +98,40 @@ __global__ void __launch_bounds__(1024) pygw_ei_pivot(const double2* __restrict_
     S[i][j] = a[(p0 + i) + (size_t)n * (p0 + j)];
   }
   __syncthreads();
+  // Each thread owns the entries e = tid, tid + 1024, ... (at most 4): per
+  // step it forms their new values from the pre-step row and column k
+  // (reads), then, after one barrier, stores them (writes): two barriers per
+  // step.
+  constexpr int PER = (NB * NB + 1023) / 1024;
   for (int k = 0; k < nb; k++) {
     const double2 piv = S[k][k];
     const double2 rp = cdiv(double2{1.0, 0.0}, piv);
     if (threadIdx.x == 0 && !(piv.x > 0.0)) atomicOr(&info[w], 1);
-    __syncthreads();
-    // update all (i, j) with i, j != k first, using the pre-step row/col k
-    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+    double2 nv[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + 1024 * q;
+      if (e >= nb * nb) break;
       const int i = e % nb, j = e / nb;
       if (i != k && j != k) {
         const double2 t = cmul(S[i][k], cmul(S[k][j], rp));
-        S[i][j].x -= t.x;
-        S[i][j].y -= t.y;
+        nv[q] = double2{S[i][j].x - t.x, S[i][j].y - t.y};
+      } else if (j == k && i != k) {
+        const double2 c = cmul(S[i][k], rp);
+        nv[q] = double2{-c.x, -c.y};
+      } else if (i == k && j != k) {
+        nv[q] = cmul(S[k][j], rp);
+      } else {
+        nv[q] = rp;
       }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < nb; e += blockDim.x) {
-      if (e != k) {
-        const double2 c = cmul(S[e][k], rp);
-        const double2 r = cmul(S[k][e], rp);
-        S[e][k] = double2{-c.x, -c.y};
-        S[k][e] = r;
-      }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + 1024 * q;
+      if (e >= nb * nb) break;
+      S[e % nb][e / nb] = nv[q];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) S[k][k] = rp;
     __syncthreads();
   }
   double2* b = B + w * NB * NB;
