@@ -24,7 +24,7 @@ __global__ void pygw_tw_project(const double* kcw, int np, int nom, int nk, cons
                                 int no, int ne, double* out);
 
 static const int kMaxInc = 24;
-static const size_t kW1tBudget = (size_t)2 << 30;  // preferred corner-weight buffer
+static const size_t kW1tBudget = (size_t)8 << 30;  // preferred corner-weight buffer (C5: 2 chunks of pairs)
 
 // head: incidence lists, counts, status words, list counter
 static size_t tw_head_bytes(int n_k) {
